@@ -1,0 +1,19 @@
+"""hyteg_amd -- MI355X-native matrix-free operator application and geometric
+multigrid for hierarchical hybrid tetrahedral grids (HyTeG's hot path).
+
+The numerical work runs in hand-written HIP kernels inside
+hyteg_amd/libhyteg_amd.so, reached through its C ABI (include/hyteg_amd.h).
+This module is a thin ctypes mirror of that ABI with HyTeG's names
+(PrimitiveStorage, P1Function, P1ConstantLaplaceOperator, ...).  There is no
+CPU fallback: if the library is missing, importing `lib()` raises.
+"""
+from ._lib import (ALL, DIRICHLET, F16, F32, F64, FREESLIP, GMG_FLAG, INNER, NEUMANN, ADD, REPLACE,
+                   HygError, lib, library_path)
+from .api import (Context, P1ConstantLaplaceOperator, P1Function, PrimitiveStorage, cg_solve,
+                  gmg_solve, prolongate, restrict)
+
+__all__ = [
+    "ALL", "DIRICHLET", "F16", "F32", "F64", "FREESLIP", "GMG_FLAG", "INNER", "NEUMANN", "ADD", "REPLACE",
+    "HygError", "lib", "library_path", "Context", "P1ConstantLaplaceOperator", "P1Function",
+    "PrimitiveStorage", "cg_solve", "gmg_solve", "prolongate", "restrict",
+]

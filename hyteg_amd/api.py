@@ -1,0 +1,197 @@
+"""Python mirror of the HyTeG-shaped API over the C ABI (ctypes)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import ALL, F64, GMG_FLAG, REPLACE, HygError, lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One HIP device, one stream (hg_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        st = lib().hg_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise HygError(f"hg_ctx_create failed with status {st}")
+        self.h = h
+
+    def check(self, status, what):
+        if status != 0:
+            msg = lib().hg_last_error(self.h).decode()
+            raise HygError(f"{what}: status {status}: {msg}")
+
+    def synchronize(self):
+        self.check(lib().hg_ctx_synchronize(self.h), "synchronize")
+
+    def memory(self):
+        out = C.c_int64()
+        self.check(lib().hg_ctx_memory(self.h, C.byref(out)), "memory")
+        return out.value
+
+    def event_timing(self, enable=True):
+        self.check(lib().hg_ctx_event_timing(self.h, int(enable)), "event_timing")
+
+    def last_ms(self):
+        out = C.c_double()
+        self.check(lib().hg_ctx_last_kernel_ms(self.h, C.byref(out)), "last_ms")
+        return out.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PrimitiveStorage:
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    @staticmethod
+    def cuboid(ctx, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(1, 1, 1)):
+        lo = np.asarray(lo, dtype=np.float64)
+        hi = np.asarray(hi, dtype=np.float64)
+        h = C.c_int64()
+        ctx.check(lib().hg_storage_cuboid(ctx.h, _ptr(lo), _ptr(hi), n[0], n[1], n[2], C.byref(h)), "meshCuboid")
+        return PrimitiveStorage(ctx, h.value)
+
+    @staticmethod
+    def from_cells(ctx, coords, cells):
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        cells = np.ascontiguousarray(cells, dtype=np.int64)
+        h = C.c_int64()
+        ctx.check(lib().hg_storage_create(ctx.h, len(coords), _ptr(coords), len(cells), _ptr(cells), C.byref(h)),
+                  "storage")
+        return PrimitiveStorage(ctx, h.value)
+
+    def counts(self):
+        out = np.zeros(4, dtype=np.int64)
+        self.ctx.check(lib().hg_storage_counts(self.ctx.h, self.h, _ptr(out)), "counts")
+        return tuple(int(v) for v in out)
+
+    def ndofs(self, level):
+        out = C.c_int64()
+        self.ctx.check(lib().hg_p1_ndofs(self.ctx.h, self.h, level, C.byref(out)), "ndofs")
+        return out.value
+
+    def coords(self, level):
+        out = np.zeros((self.ndofs(level), 3))
+        self.ctx.check(lib().hg_p1_dof_coords(self.ctx.h, self.h, level, _ptr(out)), "coords")
+        return out
+
+    def flags(self, level):
+        out = np.zeros(self.ndofs(level), dtype=np.int32)
+        self.ctx.check(lib().hg_p1_dof_flags(self.ctx.h, self.h, level, _ptr(out)), "flags")
+        return out
+
+
+class P1Function:
+    def __init__(self, name, storage, min_level, max_level, dtype=F64):
+        self.name = name
+        self.storage = storage
+        self.ctx = storage.ctx
+        self.min_level, self.max_level = min_level, max_level
+        h = C.c_int64()
+        self.ctx.check(lib().hg_p1_function_create(self.ctx.h, storage.h, min_level, max_level, dtype, C.byref(h)),
+                       f"P1Function {name}")
+        self.h = h.value
+
+    def destroy(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().hg_function_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def upload(self, level, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != (self.storage.ndofs(level),):
+            raise ValueError("wrong number of values")
+        self.ctx.check(lib().hg_function_upload(self.ctx.h, self.h, level, _ptr(v)), "upload")
+
+    def download(self, level):
+        out = np.zeros(self.storage.ndofs(level))
+        self.ctx.check(lib().hg_function_download(self.ctx.h, self.h, level, _ptr(out)), "download")
+        return out
+
+    def _lincomb(self, fn, scalars, funcs, level, flag):
+        s = np.ascontiguousarray(scalars, dtype=np.float64)
+        f = np.ascontiguousarray([g.h for g in funcs], dtype=np.int64)
+        self.ctx.check(fn(self.ctx.h, self.h, len(funcs), _ptr(s), _ptr(f), level, flag), "assign/add")
+
+    def assign(self, scalars, funcs, level, flag=ALL):
+        self._lincomb(lib().hg_function_assign, scalars, funcs, level, flag)
+
+    def add(self, scalars, funcs, level, flag=ALL):
+        self._lincomb(lib().hg_function_add, scalars, funcs, level, flag)
+
+    def interpolate(self, value, level, flag=ALL):
+        self.ctx.check(lib().hg_function_interpolate_const(self.ctx.h, self.h, float(value), level, flag),
+                       "interpolate")
+
+    def dot(self, other, level, flag=ALL):
+        out = C.c_double()
+        self.ctx.check(lib().hg_function_dot(self.ctx.h, self.h, other.h, level, flag, C.byref(out)), "dot")
+        return out.value
+
+    def copy_from(self, other, level):
+        self.ctx.check(lib().hg_function_copy_from(self.ctx.h, self.h, other.h, level), "copyFrom")
+
+    def communicate(self, level):
+        self.ctx.check(lib().hg_function_communicate(self.ctx.h, self.h, level), "communicate")
+
+
+class P1ConstantLaplaceOperator:
+    def __init__(self, storage, min_level, max_level):
+        self.storage = storage
+        self.ctx = storage.ctx
+        h = C.c_int64()
+        self.ctx.check(lib().hg_p1_laplace_create(self.ctx.h, storage.h, min_level, max_level, C.byref(h)),
+                       "P1ConstantLaplaceOperator")
+        self.h = h.value
+
+    def apply(self, src, dst, level, flag, update=REPLACE):
+        self.ctx.check(lib().hg_p1_apply(self.ctx.h, self.h, src.h, dst.h, level, flag, update), "apply")
+
+    def smooth_sor(self, x, b, relax, level, flag):
+        self.ctx.check(lib().hg_p1_smooth_sor(self.ctx.h, self.h, x.h, b.h, relax, level, flag), "smooth_sor")
+
+    def smooth_gs(self, x, b, level, flag):
+        self.smooth_sor(x, b, 1.0, level, flag)
+
+    def cell_stencil(self, level, cell):
+        out = np.zeros(15)
+        self.ctx.check(lib().hg_p1_cell_stencil(self.ctx.h, self.h, level, cell, _ptr(out)), "cell_stencil")
+        return out
+
+
+def restrict(f, source_level, flag):
+    f.ctx.check(lib().hg_p1_restrict(f.ctx.h, f.h, source_level, flag), "restrict")
+
+
+def prolongate(f, source_level, flag, update):
+    f.ctx.check(lib().hg_p1_prolongate(f.ctx.h, f.h, source_level, flag, update), "prolongate")
+
+
+def cg_solve(A, x, b, level, max_iter, rel_tol, abs_tol):
+    it = C.c_int()
+    A.ctx.check(lib().hg_p1_cg_solve(A.ctx.h, A.h, x.h, b.h, level, max_iter, rel_tol, abs_tol, C.byref(it)), "cg")
+    return it.value
+
+
+def gmg_solve(A, x, b, min_level, max_level, pre=3, post=3, cycles=10, tol=0.0, coarse_max_iter=10000,
+              coarse_tol=1e-16):
+    hist = np.zeros(cycles + 1)
+    done = C.c_int()
+    A.ctx.check(lib().hg_p1_gmg_solve(A.ctx.h, A.h, x.h, b.h, min_level, max_level, pre, post, cycles, tol,
+                                      coarse_max_iter, coarse_tol, _ptr(hist), C.byref(done)), "gmg")
+    return done.value, hist[: done.value + 1]

@@ -1,0 +1,907 @@
+// Context, storage, per-level lattice layout, halo exchange, functions and
+// BLAS-1 kernels of the hyteg_amd library, plus their C-ABI entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "abi_util.hpp"
+#include "hyteg_amd.h"
+#include "runtime.hpp"
+
+namespace hg {
+
+// ------------------------------------------------------------------------
+// device buffers
+// ------------------------------------------------------------------------
+template < class T >
+void DevBuf< T >::alloc( Context* c, size_t count )
+{
+   release();
+   ctx = c;
+   n   = count;
+   if ( count == 0 )
+      return;
+   HG_CHECK_HIP( hipMalloc( (void**) &p, count * sizeof( T ) ) );
+   HG_CHECK_HIP( hipMemsetAsync( p, 0, count * sizeof( T ), c->stream ) );
+   c->bytes += int64_t( count * sizeof( T ) );
+}
+template < class T >
+void DevBuf< T >::upload( Context* c, const std::vector< T >& h )
+{
+   alloc( c, h.size() );
+   if ( !h.empty() )
+   {
+      HG_CHECK_HIP( hipMemcpyAsync( p, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice, c->stream ) );
+      HG_CHECK_HIP( hipStreamSynchronize( c->stream ) );
+   }
+}
+template < class T >
+void DevBuf< T >::release()
+{
+   if ( p )
+   {
+      (void) hipFree( p );
+      if ( ctx )
+         ctx->bytes -= int64_t( n * sizeof( T ) );
+   }
+   p = nullptr;
+   n = 0;
+}
+template struct DevBuf< double >;
+template struct DevBuf< float >;
+template struct DevBuf< int64_t >;
+template struct DevBuf< int32_t >;
+template struct DevBuf< uint32_t >;
+template struct DevBuf< uint8_t >;
+template struct DevBuf< int8_t >;
+template struct DevBuf< CellMeta >;
+
+// ------------------------------------------------------------------------
+// storage / lattice set-up
+// ------------------------------------------------------------------------
+void Storage::init( Context* ctx )
+{
+   const Topology&           t = topo;
+   std::vector< CellMeta >   m( t.nc );
+   for ( int c = 0; c < t.nc; ++c )
+   {
+      auto localOf = [&]( int g ) {
+         for ( int k = 0; k < 4; ++k )
+            if ( t.cv[c][k] == g )
+               return int8_t( k );
+         throw Error( -4, "inconsistent topology" );
+      };
+      for ( int k = 0; k < 4; ++k )
+      {
+         m[c].face[k] = t.cf[c][k];
+         for ( int j = 0; j < 3; ++j )
+            m[c].fperm[k][j] = localOf( t.fv[t.cf[c][k]][j] );
+         m[c].vert[k] = t.cv[c][k];
+      }
+      for ( int k = 0; k < 6; ++k )
+      {
+         m[c].edge[k] = t.ce[c][k];
+         for ( int j = 0; j < 2; ++j )
+            m[c].eperm[k][j] = localOf( t.ev[t.ce[c][k]][j] );
+      }
+   }
+   meta.upload( ctx, m );
+
+   auto csr = [&]( int nprim, auto primOfLocal, int nlocal, DevBuf< int32_t >& ptr, DevBuf< int32_t >& ent ) {
+      std::vector< std::vector< int32_t > > lists( nprim );
+      for ( int c = 0; c < t.nc; ++c )
+         for ( int k = 0; k < nlocal; ++k )
+         {
+            lists[primOfLocal( c, k )].push_back( c );
+            lists[primOfLocal( c, k )].push_back( k );
+         }
+      std::vector< int32_t > p( nprim + 1, 0 ), e;
+      for ( int i = 0; i < nprim; ++i )
+      {
+         p[i + 1] = p[i] + int32_t( lists[i].size() / 2 );
+         e.insert( e.end(), lists[i].begin(), lists[i].end() );
+      }
+      ptr.upload( ctx, p );
+      ent.upload( ctx, e );
+   };
+   csr( t.nf, [&]( int c, int k ) { return t.cf[c][k]; }, 4, faceCellPtr, faceCells );
+   csr( t.ne, [&]( int c, int k ) { return t.ce[c][k]; }, 6, edgeCellPtr, edgeCells );
+   csr( t.nv, [&]( int c, int k ) { return t.cv[c][k]; }, 4, vertCellPtr, vertCells );
+}
+
+Lattice& Storage::lattice( Context* ctx, int L )
+{
+   if ( L < 0 || L > 12 )
+      throw Error( -1, "refinement level out of range" );
+   auto it = lattices.find( L );
+   if ( it != lattices.end() )
+      return *it->second;
+   auto lp     = std::make_unique< Lattice >();
+   Lattice& l  = *lp;
+   l.L         = L;
+   l.num       = LevelNumbering( topo, L );
+   l.mm        = l.num.mm;
+   l.w         = l.mm + 1;
+   l.sc        = tet( l.w );
+   l.nc        = topo.nc;
+   l.nb        = l.num.coff;
+   l.nstore    = l.nc * l.sc + l.nb;
+
+   l.hBndFlag.resize( l.nb );
+   for ( int64_t i = 0; i < l.nb; ++i )
+      l.hBndFlag[i] = uint8_t( l.num.flagOf( topo, i ) );
+   l.bndFlag.upload( ctx, l.hBndFlag );
+
+   // interior bit mask of one cell lattice, replicated over cells
+   const int64_t          ncellbits = l.nc * l.sc;
+   std::vector< uint32_t > mask( ( ncellbits + 31 ) / 32, 0u );
+   for ( int64_t c = 0; c < l.nc; ++c )
+      for ( int64_t z = 1; z < l.mm; ++z )
+         for ( int64_t y = 1; y + z < l.mm; ++y )
+            for ( int64_t x = 1; x + y + z < l.mm; ++x )
+            {
+               const int64_t pos = c * l.sc + tetIndex( l.w, x, y, z );
+               mask[pos >> 5] |= 1u << ( pos & 31 );
+            }
+   l.cellMask.upload( ctx, mask );
+
+   std::vector< int32_t > rows;
+   for ( int g = 0; g < 4; ++g )
+   {
+      l.rowGroup[g] = int64_t( rows.size() / 2 );
+      for ( int64_t z = 1; z <= l.mm - 3; ++z )
+         for ( int64_t y = 1; y + z <= l.mm - 2; ++y )
+         {
+            if ( ( y & 1 ) + 2 * ( z & 1 ) != g )
+               continue;
+            rows.push_back( int32_t( y ) );
+            rows.push_back( int32_t( z ) );
+         }
+   }
+   l.rowGroup[4] = int64_t( rows.size() / 2 );
+   l.nrows       = l.rowGroup[4];
+   l.rows.upload( ctx, rows );
+
+   lattices[L] = std::move( lp );
+   return l;
+}
+
+// ------------------------------------------------------------------------
+// halo geometry (device)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void decodeTri( int64_t n, int64_t i, int64_t& x, int64_t& y )
+{
+   const double b  = 2.0 * double( n ) + 1.0;
+   int64_t      yy = int64_t( floor( 0.5 * ( b - sqrt( b * b - 8.0 * double( i ) ) ) ) );
+   if ( yy < 0 )
+      yy = 0;
+   while ( yy > 0 && triIndex( n, 0, yy ) > i )
+      --yy;
+   while ( triIndex( n, 0, yy + 1 ) <= i )
+      ++yy;
+   y = yy;
+   x = i - triIndex( n, 0, yy );
+}
+
+// lattice position in cell c of the point with barycentric weights t (t[1..3] = x,y,z)
+__device__ __forceinline__ int64_t latticePos( int64_t c, int64_t sc, int64_t w, const int64_t t[4] )
+{
+   return c * sc + tetIndex( w, t[1], t[2], t[3] );
+}
+
+// kind 0: face-interior point (a,b) (0-based inside the interior triangle) of local face k
+__device__ __forceinline__ void faceToCell( const CellMeta& m, int k, int64_t mm, int64_t fx, int64_t fy, int64_t t[4] )
+{
+   t[0] = t[1] = t[2] = t[3] = 0;
+   t[m.fperm[k][0]] = mm - fx - fy;
+   t[m.fperm[k][1]] = fx;
+   t[m.fperm[k][2]] = fy;
+}
+__device__ __forceinline__ void edgeToCell( const CellMeta& m, int k, int64_t mm, int64_t s, int64_t t[4] )
+{
+   t[0] = t[1] = t[2] = t[3] = 0;
+   t[m.eperm[k][0]] = mm - s;
+   t[m.eperm[k][1]] = s;
+}
+
+template < class T >
+__global__ void kPull( const CellMeta* __restrict__ meta, int64_t nc, int64_t sc, int64_t w, int64_t mm, int64_t fin,
+                       int64_t ein, int64_t eoff, int64_t foff, T* __restrict__ v )
+{
+   const int64_t per = 4 * fin + 6 * ein + 4;
+   const int64_t gid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( gid >= per * nc )
+      return;
+   const int64_t   c = gid / per;
+   int64_t         j = gid - c * per;
+   const CellMeta& m = meta[c];
+   int64_t         t[4];
+   int64_t         src;
+   const int64_t   base = nc * sc;
+   if ( j < 4 * fin )
+   {
+      const int     k = int( j / fin );
+      const int64_t i = j - k * fin;
+      int64_t       a, b;
+      decodeTri( mm - 2, i, a, b );
+      faceToCell( m, k, mm, a + 1, b + 1, t );
+      src = base + foff + int64_t( m.face[k] ) * fin + i;
+   }
+   else if ( ( j -= 4 * fin ) < 6 * ein )
+   {
+      const int     k = int( j / ein );
+      const int64_t s = j - k * ein + 1;
+      edgeToCell( m, k, mm, s, t );
+      src = base + eoff + int64_t( m.edge[k] ) * ein + ( s - 1 );
+   }
+   else
+   {
+      const int k = int( j - 6 * ein );
+      t[0] = t[1] = t[2] = t[3] = 0;
+      t[k]                      = mm;
+      src                       = base + m.vert[k];
+   }
+   v[latticePos( c, sc, w, t )] = v[src];
+}
+
+template < class T >
+__global__ void kPushAdditive( const CellMeta* __restrict__ meta, const int32_t* __restrict__ fptr,
+                               const int32_t* __restrict__ fcells, const int32_t* __restrict__ eptr,
+                               const int32_t* __restrict__ ecells, const int32_t* __restrict__ vptr,
+                               const int32_t* __restrict__ vcells, const uint8_t* __restrict__ bndFlag, int64_t nb,
+                               int64_t nc, int64_t sc, int64_t w, int64_t mm, int64_t fin, int64_t ein, int64_t eoff,
+                               int64_t foff, int excludeFlag, int zeroOut, T* __restrict__ v )
+{
+   using A           = typename Acc< T >::type;
+   const int64_t b   = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nb )
+      return;
+   if ( bndFlag[b] & excludeFlag )
+      return;
+   const int64_t base = nc * sc;
+   A             s    = zeroOut ? A( 0 ) : toAcc( v[base + b] );
+   int64_t       t[4];
+   if ( b < eoff )
+   {
+      for ( int32_t e = vptr[b]; e < vptr[b + 1]; ++e )
+      {
+         const int64_t c = vcells[2 * e];
+         const int     k = vcells[2 * e + 1];
+         t[0] = t[1] = t[2] = t[3] = 0;
+         t[k]                      = mm;
+         s += toAcc( v[latticePos( c, sc, w, t )] );
+      }
+   }
+   else if ( b < foff )
+   {
+      const int64_t eid = ( b - eoff ) / ein;
+      const int64_t sidx = ( b - eoff ) - eid * ein + 1;
+      for ( int32_t e = eptr[eid]; e < eptr[eid + 1]; ++e )
+      {
+         const int64_t c = ecells[2 * e];
+         const int     k = ecells[2 * e + 1];
+         edgeToCell( meta[c], k, mm, sidx, t );
+         s += toAcc( v[latticePos( c, sc, w, t )] );
+      }
+   }
+   else
+   {
+      const int64_t f = ( b - foff ) / fin;
+      const int64_t i = ( b - foff ) - f * fin;
+      int64_t       a, bb;
+      decodeTri( mm - 2, i, a, bb );
+      for ( int32_t e = fptr[f]; e < fptr[f + 1]; ++e )
+      {
+         const int64_t c = fcells[2 * e];
+         const int     k = fcells[2 * e + 1];
+         faceToCell( meta[c], k, mm, a + 1, bb + 1, t );
+         s += toAcc( v[latticePos( c, sc, w, t )] );
+      }
+   }
+   v[base + b] = fromAcc< T >( s );
+}
+
+void pullHalos( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype )
+{
+   const int64_t n = lat.haloPerCell() * lat.nc;
+   if ( n == 0 )
+      return;
+   dispatchType( dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      kPull< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( st.meta.p, lat.nc, lat.sc, lat.w, lat.mm, lat.num.fin,
+                                                             lat.num.ein, lat.num.eoff, lat.num.foff, (T*) data );
+   } );
+   HG_CHECK_HIP( hipGetLastError() );
+}
+
+void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int excludeFlag, bool zeroOut )
+{
+   if ( lat.nb == 0 )
+      return;
+   dispatchType( dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      kPushAdditive< T ><<< gridFor( lat.nb ), kBlock, 0, ctx->stream >>>(
+          st.meta.p, st.faceCellPtr.p, st.faceCells.p, st.edgeCellPtr.p, st.edgeCells.p, st.vertCellPtr.p,
+          st.vertCells.p, lat.bndFlag.p, lat.nb, lat.nc, lat.sc, lat.w, lat.mm, lat.num.fin, lat.num.ein, lat.num.eoff,
+          lat.num.foff, excludeFlag, zeroOut ? 1 : 0, (T*) data );
+   } );
+   HG_CHECK_HIP( hipGetLastError() );
+}
+
+// ------------------------------------------------------------------------
+// BLAS-1 kernels over owned DoFs.  The cell part is processed flat (halo
+// copies included: they are refreshed before any read anyway); the owned
+// vertex/edge/face part is filtered by DoFType.
+// ------------------------------------------------------------------------
+template < class T >
+__global__ void kLinComb( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, int nsrc,
+                          double s0, double s1, double s2, const T* __restrict__ f0, const T* __restrict__ f1,
+                          const T* __restrict__ f2, int add, T* __restrict__ dst )
+{
+   using A           = typename Acc< T >::type;
+   const int64_t i   = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t n   = ( flag & kInner ? ncell : 0 ) + nb;
+   if ( i >= n )
+      return;
+   const int64_t p = ( flag & kInner ) ? i : ncell + i;
+   if ( p >= ncell && !( bndFlag[p - ncell] & flag ) )
+      return;
+   A v = A( 0 );
+   if ( nsrc > 0 )
+      v += A( s0 ) * toAcc( f0[p] );
+   if ( nsrc > 1 )
+      v += A( s1 ) * toAcc( f1[p] );
+   if ( nsrc > 2 )
+      v += A( s2 ) * toAcc( f2[p] );
+   if ( add )
+      v += toAcc( dst[p] );
+   dst[p] = fromAcc< T >( v );
+}
+
+template < class T >
+__global__ void kFill( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, double value,
+                       T* __restrict__ dst )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t n = ( flag & kInner ? ncell : 0 ) + nb;
+   if ( i >= n )
+      return;
+   const int64_t p = ( flag & kInner ) ? i : ncell + i;
+   if ( p >= ncell && !( bndFlag[p - ncell] & flag ) )
+      return;
+   dst[p] = fromAcc< T >( typename Acc< T >::type( value ) );
+}
+
+template < class TD, class TS >
+__global__ void kCast( int64_t n, const TS* __restrict__ src, TD* __restrict__ dst )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( i < n )
+   {
+      double v;
+      if constexpr ( std::is_same< TS, __half >::value )
+         v = double( __half2float( src[i] ) );
+      else
+         v = double( src[i] );
+      if constexpr ( std::is_same< TD, __half >::value )
+         dst[i] = __float2half( float( v ) );
+      else
+         dst[i] = TD( v );
+   }
+}
+
+template < class T >
+__global__ void kDotPartial( int64_t ncell, int64_t nb, const uint32_t* __restrict__ cellMask,
+                             const uint8_t* __restrict__ bndFlag, int flag, const T* __restrict__ a,
+                             const T* __restrict__ b, double* __restrict__ partials )
+{
+   __shared__ double red[kBlock];
+   double            s      = 0.0;
+   const int64_t     stride = int64_t( gridDim.x ) * blockDim.x;
+   if ( flag & kInner )
+      for ( int64_t p = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x; p < ncell; p += stride )
+         if ( ( cellMask[p >> 5] >> ( p & 31 ) ) & 1u )
+            s += double( toAcc( a[p] ) ) * double( toAcc( b[p] ) );
+   for ( int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x; i < nb; i += stride )
+      if ( bndFlag[i] & flag )
+         s += double( toAcc( a[ncell + i] ) ) * double( toAcc( b[ncell + i] ) );
+   red[threadIdx.x] = s;
+   __syncthreads();
+   for ( int o = kBlock / 2; o > 0; o >>= 1 )
+   {
+      if ( threadIdx.x < o )
+         red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+   }
+   if ( threadIdx.x == 0 )
+      partials[blockIdx.x] = red[0];
+}
+
+__global__ void kDotFinal( const double* __restrict__ partials, int n, double* __restrict__ out )
+{
+   __shared__ double red[kBlock];
+   double            s = 0.0;
+   for ( int i = threadIdx.x; i < n; i += blockDim.x )
+      s += partials[i];
+   red[threadIdx.x] = s;
+   __syncthreads();
+   for ( int o = kBlock / 2; o > 0; o >>= 1 )
+   {
+      if ( threadIdx.x < o )
+         red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+   }
+   if ( threadIdx.x == 0 )
+      out[0] = red[0];
+}
+
+double dotDevice( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag )
+{
+   dispatchType( dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      kDotPartial< T ><<< kDotBlocks, kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, lat.nb, lat.cellMask.p,
+                                                                lat.bndFlag.p, flag, (const T*) a, (const T*) b,
+                                                                ctx->partials.p );
+   } );
+   kDotFinal<<< 1, kBlock, 0, ctx->stream >>>( ctx->partials.p, kDotBlocks, ctx->partials.p + kDotBlocks );
+   HG_CHECK_HIP( hipGetLastError() );
+   HG_CHECK_HIP( hipMemcpyAsync( ctx->hostResult, ctx->partials.p + kDotBlocks, sizeof( double ), hipMemcpyDeviceToHost,
+                                 ctx->stream ) );
+   HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+   return *ctx->hostResult;
+}
+
+// ------------------------------------------------------------------------
+// context helpers
+// ------------------------------------------------------------------------
+Storage& Context::storage( int64_t h )
+{
+   auto it = storages.find( h );
+   if ( it == storages.end() )
+      throw Error( -3, "unknown storage handle" );
+   return *it->second;
+}
+Function& Context::function( int64_t h )
+{
+   auto it = functions.find( h );
+   if ( it == functions.end() )
+      throw Error( -3, "unknown function handle" );
+   return *it->second;
+}
+void Context::beginTiming()
+{
+   if ( timing )
+      HG_CHECK_HIP( hipEventRecord( ev0, stream ) );
+}
+void Context::endTiming()
+{
+   if ( timing )
+   {
+      HG_CHECK_HIP( hipEventRecord( ev1, stream ) );
+      HG_CHECK_HIP( hipEventSynchronize( ev1 ) );
+      float ms = 0.f;
+      HG_CHECK_HIP( hipEventElapsedTime( &ms, ev0, ev1 ) );
+      lastMs = ms;
+   }
+}
+
+static void checkLevel( const Function& f, int L )
+{
+   if ( L < f.minL || L > f.maxL )
+      throw Error( -1, "level " + std::to_string( L ) + " outside the function's level range" );
+}
+
+// host conversion helpers for canonical FP64 transfer
+template < class T >
+static T hostFrom( double v )
+{
+   if constexpr ( std::is_same< T, __half >::value )
+      return __float2half( float( v ) );
+   else
+      return T( v );
+}
+template < class T >
+static double hostTo( T v )
+{
+   if constexpr ( std::is_same< T, __half >::value )
+      return double( __half2float( v ) );
+   else
+      return double( v );
+}
+
+template < class F >
+static void forEachCanonical( const Lattice& l, F&& f ) // f(canonical id, storage pos)
+{
+   for ( int64_t b = 0; b < l.nb; ++b )
+      f( b, l.nc * l.sc + b );
+   int64_t id = l.num.coff;
+   for ( int64_t c = 0; c < l.nc; ++c )
+      for ( int64_t z = 1; z < l.mm; ++z )
+         for ( int64_t y = 1; y + z < l.mm; ++y )
+            for ( int64_t x = 1; x + y + z < l.mm; ++x )
+               f( id++, c * l.sc + tetIndex( l.w, x, y, z ) );
+}
+
+} // namespace hg
+
+using namespace hg;
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+int hg_ctx_create( int device, hg_ctx** out )
+{
+   if ( !out )
+      return HG_ERR_ARG;
+   *out = nullptr;
+   auto* ctx = new Context();
+   try
+   {
+      ctx->device = device;
+      HG_CHECK_HIP( hipSetDevice( device ) );
+      HG_CHECK_HIP( hipStreamCreateWithFlags( &ctx->stream, hipStreamNonBlocking ) );
+      HG_CHECK_HIP( hipHostMalloc( (void**) &ctx->hostResult, sizeof( double ) * 8, hipHostMallocDefault ) );
+      HG_CHECK_HIP( hipEventCreate( &ctx->ev0 ) );
+      HG_CHECK_HIP( hipEventCreate( &ctx->ev1 ) );
+      ctx->partials.alloc( ctx, kDotBlocks + 8 );
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+   } catch ( const std::exception& e )
+   {
+      delete ctx;
+      return HG_ERR_HIP;
+   }
+   *out = reinterpret_cast< hg_ctx* >( ctx );
+   return HG_OK;
+}
+
+int hg_ctx_destroy( hg_ctx* h )
+{
+   if ( !h )
+      return HG_ERR_ARG;
+   auto* ctx = reinterpret_cast< Context* >( h );
+   (void) hipSetDevice( ctx->device );
+   if ( ctx->stream )
+      (void) hipStreamSynchronize( ctx->stream );
+   ctx->operators.clear();
+   ctx->functions.clear();
+   ctx->storages.clear();
+   ctx->partials.release();
+   if ( ctx->hostResult )
+      (void) hipHostFree( ctx->hostResult );
+   if ( ctx->ev0 )
+      (void) hipEventDestroy( ctx->ev0 );
+   if ( ctx->ev1 )
+      (void) hipEventDestroy( ctx->ev1 );
+   if ( ctx->stream )
+      (void) hipStreamDestroy( ctx->stream );
+   delete ctx;
+   return HG_OK;
+}
+
+const char* hg_last_error( const hg_ctx* h )
+{
+   if ( !h )
+      return "null context";
+   return reinterpret_cast< const Context* >( h )->err.c_str();
+}
+
+int hg_ctx_synchronize( hg_ctx* h )
+{
+   return hg_guard( h, [&]( Context* ctx ) { HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) ); } );
+}
+
+int hg_ctx_memory( hg_ctx* h, int64_t* bytes )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !bytes )
+         throw Error( HG_ERR_ARG, "null output" );
+      *bytes = ctx->bytes;
+   } );
+}
+
+int hg_ctx_event_timing( hg_ctx* h, int enable )
+{
+   return hg_guard( h, [&]( Context* ctx ) { ctx->timing = enable != 0; } );
+}
+
+int hg_ctx_last_kernel_ms( hg_ctx* h, double* ms )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !ms )
+         throw Error( HG_ERR_ARG, "null output" );
+      *ms = ctx->lastMs;
+   } );
+}
+
+static int64_t addStorage( Context* ctx, Topology&& t )
+{
+   auto st  = std::make_unique< Storage >();
+   st->topo = std::move( t );
+   st->init( ctx );
+   const int64_t h    = ctx->nextHandle++;
+   ctx->storages[h] = std::move( st );
+   return h;
+}
+
+int hg_storage_cuboid( hg_ctx* h, const double* lo, const double* hi, int nx, int ny, int nz, hg_handle* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !lo || !hi || !out )
+         throw Error( HG_ERR_ARG, "null argument" );
+      *out = addStorage( ctx, Topology::cuboid( lo, hi, nx, ny, nz ) );
+   } );
+}
+
+int hg_storage_create( hg_ctx* h, int nv, const double* coords, int nc, const int64_t* cells, hg_handle* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !coords || !cells || !out || nv < 4 || nc < 1 )
+         throw Error( HG_ERR_ARG, "invalid mesh arrays" );
+      *out = addStorage( ctx, Topology::fromCells( nv, coords, nc, cells ) );
+   } );
+}
+
+int hg_storage_counts( hg_ctx* h, hg_handle s, int64_t* out4 )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out4 )
+         throw Error( HG_ERR_ARG, "null output" );
+      const Topology& t = ctx->storage( s ).topo;
+      out4[0]           = t.nv;
+      out4[1]           = t.ne;
+      out4[2]           = t.nf;
+      out4[3]           = t.nc;
+   } );
+}
+
+int hg_p1_ndofs( hg_ctx* h, hg_handle s, int level, int64_t* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out || level < 0 || level > 12 )
+         throw Error( HG_ERR_ARG, "invalid argument" );
+      *out = LevelNumbering( ctx->storage( s ).topo, level ).n;
+   } );
+}
+
+int hg_p1_dof_coords( hg_ctx* h, hg_handle s, int level, double* xyz )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !xyz || level < 0 || level > 12 )
+         throw Error( HG_ERR_ARG, "invalid argument" );
+      Storage&        st = ctx->storage( s );
+      const Topology& t  = st.topo;
+      LevelNumbering  nm( t, level );
+      const double    mm = double( nm.mm );
+      // every canonical DoF from the first cell that contains it
+      std::vector< char > done( nm.n, 0 );
+      for ( int c = 0; c < t.nc; ++c )
+      {
+         const double* v[4];
+         for ( int k = 0; k < 4; ++k )
+            v[k] = &t.X[3 * t.cv[c][k]];
+         for ( int64_t z = 0; z <= nm.mm; ++z )
+            for ( int64_t y = 0; y + z <= nm.mm; ++y )
+               for ( int64_t x = 0; x + y + z <= nm.mm; ++x )
+               {
+                  const int64_t id = nm.locate( t, c, x, y, z );
+                  if ( done[id] )
+                     continue;
+                  done[id] = 1;
+                  for ( int d = 0; d < 3; ++d )
+                     xyz[3 * id + d] = v[0][d] + ( x / mm ) * ( v[1][d] - v[0][d] ) + ( y / mm ) * ( v[2][d] - v[0][d] ) +
+                                       ( z / mm ) * ( v[3][d] - v[0][d] );
+               }
+      }
+   } );
+}
+
+int hg_p1_dof_flags( hg_ctx* h, hg_handle s, int level, int32_t* flags )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !flags || level < 0 || level > 12 )
+         throw Error( HG_ERR_ARG, "invalid argument" );
+      const Topology& t = ctx->storage( s ).topo;
+      LevelNumbering  nm( t, level );
+      for ( int64_t i = 0; i < nm.n; ++i )
+         flags[i] = nm.flagOf( t, i );
+   } );
+}
+
+int hg_p1_function_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, hg_handle* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out || minLevel < 0 || maxLevel < minLevel || maxLevel > 12 || dtype < 0 || dtype > 2 )
+         throw Error( HG_ERR_ARG, "invalid function parameters" );
+      Storage& st = ctx->storage( s );
+      auto     f  = std::make_unique< Function >();
+      f->st       = &st;
+      f->dtype    = dtype;
+      f->minL     = minLevel;
+      f->maxL     = maxLevel;
+      for ( int L = minLevel; L <= maxLevel; ++L )
+      {
+         Lattice& l = st.lattice( ctx, L );
+         f->data[L].alloc( ctx, size_t( l.nstore ) * f->elemSize() );
+      }
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+      *out                  = ctx->nextHandle++;
+      ctx->functions[*out] = std::move( f );
+   } );
+}
+
+int hg_function_destroy( hg_ctx* h, hg_handle fn )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+      if ( ctx->functions.erase( fn ) == 0 )
+         throw Error( HG_ERR_HANDLE, "unknown function handle" );
+   } );
+}
+
+int hg_function_storage( hg_ctx* h, hg_handle fn, hg_handle* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      Function& f = ctx->function( fn );
+      for ( auto& [k, v] : ctx->storages )
+         if ( v.get() == f.st )
+         {
+            *out = k;
+            return;
+         }
+      throw Error( HG_ERR_STATE, "function without storage" );
+   } );
+}
+
+int hg_function_upload( hg_ctx* h, hg_handle fn, int level, const double* canonical )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& f = ctx->function( fn );
+      checkLevel( f, level );
+      if ( !canonical )
+         throw Error( HG_ERR_ARG, "null data" );
+      Lattice& l = f.st->lattice( ctx, level );
+      dispatchType( f.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         std::vector< T > hbuf( l.nstore, hostFrom< T >( 0.0 ) );
+         forEachCanonical( l, [&]( int64_t id, int64_t pos ) { hbuf[pos] = hostFrom< T >( canonical[id] ); } );
+         HG_CHECK_HIP( hipMemcpyAsync( f.ptr( level ), hbuf.data(), l.nstore * sizeof( T ), hipMemcpyHostToDevice,
+                                       ctx->stream ) );
+         pullHalos( ctx, *f.st, l, f.ptr( level ), f.dtype );
+         HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+      } );
+   } );
+}
+
+int hg_function_download( hg_ctx* h, hg_handle fn, int level, double* canonical )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& f = ctx->function( fn );
+      checkLevel( f, level );
+      if ( !canonical )
+         throw Error( HG_ERR_ARG, "null data" );
+      Lattice& l = f.st->lattice( ctx, level );
+      dispatchType( f.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         std::vector< T > hbuf( l.nstore );
+         HG_CHECK_HIP( hipMemcpyAsync( hbuf.data(), f.ptr( level ), l.nstore * sizeof( T ), hipMemcpyDeviceToHost,
+                                       ctx->stream ) );
+         HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+         forEachCanonical( l, [&]( int64_t id, int64_t pos ) { canonical[id] = hostTo< T >( hbuf[pos] ); } );
+      } );
+   } );
+}
+
+static int linComb( hg_ctx* h, hg_handle dst, int nsrc, const double* scalars, const hg_handle* srcs, int level, int flag,
+                    int add )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( nsrc < 0 || nsrc > 3 || ( nsrc > 0 && ( !scalars || !srcs ) ) )
+         throw Error( HG_ERR_ARG, "assign/add take 0..3 source functions" );
+      Function& d = ctx->function( dst );
+      checkLevel( d, level );
+      void* src[3] = { nullptr, nullptr, nullptr };
+      for ( int i = 0; i < nsrc; ++i )
+      {
+         Function& s = ctx->function( srcs[i] );
+         checkLevel( s, level );
+         if ( s.st != d.st || s.dtype != d.dtype )
+            throw Error( HG_ERR_ARG, "assign/add need functions of one storage and one value type" );
+         src[i] = s.ptr( level );
+      }
+      Lattice&      l = d.st->lattice( ctx, level );
+      const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
+      if ( n == 0 )
+         return;
+      double s[3] = { 0, 0, 0 };
+      for ( int i = 0; i < nsrc; ++i )
+         s[i] = scalars[i];
+      dispatchType( d.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         kLinComb< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, nsrc, s[0],
+                                                                   s[1], s[2], (const T*) src[0], (const T*) src[1],
+                                                                   (const T*) src[2], add, (T*) d.ptr( level ) );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+int hg_function_assign( hg_ctx* h, hg_handle dst, int nsrc, const double* scalars, const hg_handle* srcs, int level, int flag )
+{
+   return linComb( h, dst, nsrc, scalars, srcs, level, flag, 0 );
+}
+
+int hg_function_add( hg_ctx* h, hg_handle dst, int nsrc, const double* scalars, const hg_handle* srcs, int level, int flag )
+{
+   return linComb( h, dst, nsrc, scalars, srcs, level, flag, 1 );
+}
+
+int hg_function_interpolate_const( hg_ctx* h, hg_handle fn, double value, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& f = ctx->function( fn );
+      checkLevel( f, level );
+      Lattice&      l = f.st->lattice( ctx, level );
+      const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
+      if ( n == 0 )
+         return;
+      dispatchType( f.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         kFill< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, value,
+                                                                (T*) f.ptr( level ) );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+int hg_function_dot( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, double* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      Function& fa = ctx->function( a );
+      Function& fb = ctx->function( b );
+      checkLevel( fa, level );
+      checkLevel( fb, level );
+      if ( fa.st != fb.st || fa.dtype != fb.dtype )
+         throw Error( HG_ERR_ARG, "dot needs functions of one storage and one value type" );
+      *out = dotDevice( ctx, fa.st->lattice( ctx, level ), fa.ptr( level ), fb.ptr( level ), fa.dtype, flag );
+   } );
+}
+
+int hg_function_copy_from( hg_ctx* h, hg_handle dst, hg_handle src, int level )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& d = ctx->function( dst );
+      Function& s = ctx->function( src );
+      checkLevel( d, level );
+      checkLevel( s, level );
+      if ( d.st != s.st )
+         throw Error( HG_ERR_ARG, "copyFrom needs functions of one storage" );
+      Lattice&      l = d.st->lattice( ctx, level );
+      const int64_t n = l.nstore;
+      dispatchType( d.dtype, [&]( auto td ) {
+         dispatchType( s.dtype, [&]( auto ts ) {
+            using TD = decltype( td );
+            using TS = decltype( ts );
+            kCast< TD, TS ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( n, (const TS*) s.ptr( level ),
+                                                                        (TD*) d.ptr( level ) );
+         } );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+int hg_function_communicate( hg_ctx* h, hg_handle fn, int level )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& f = ctx->function( fn );
+      checkLevel( f, level );
+      pullHalos( ctx, *f.st, f.st->lattice( ctx, level ), f.ptr( level ), f.dtype );
+   } );
+}
+
+} // extern "C"

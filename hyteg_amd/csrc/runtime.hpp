@@ -1,0 +1,255 @@
+// Internal runtime of the hyteg_amd library: context, device buffers,
+// per-level lattice layout and halo maps, functions.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lattice.hpp"
+#include "topology.hpp"
+
+namespace hg {
+
+struct Error : std::runtime_error
+{
+   int code;
+   Error( int c, const std::string& m )
+   : std::runtime_error( m )
+   , code( c )
+   {}
+};
+
+#define HG_CHECK_HIP( expr )                                                                                       \
+   do                                                                                                             \
+   {                                                                                                              \
+      hipError_t hg_e_ = ( expr );                                                                                \
+      if ( hg_e_ != hipSuccess )                                                                                  \
+         throw ::hg::Error( -2, std::string( #expr ) + ": " + hipGetErrorString( hg_e_ ) );                        \
+   } while ( 0 )
+
+struct Context;
+
+// Owning device allocation, accounted in its context.
+template < class T >
+struct DevBuf
+{
+   T*       p   = nullptr;
+   size_t   n   = 0;
+   Context* ctx = nullptr;
+
+   DevBuf() = default;
+   DevBuf( const DevBuf& ) = delete;
+   DevBuf& operator=( const DevBuf& ) = delete;
+   DevBuf( DevBuf&& o ) noexcept { swap( o ); }
+   DevBuf& operator=( DevBuf&& o ) noexcept
+   {
+      swap( o );
+      return *this;
+   }
+   ~DevBuf() { release(); }
+   void swap( DevBuf& o ) noexcept
+   {
+      std::swap( p, o.p );
+      std::swap( n, o.n );
+      std::swap( ctx, o.ctx );
+   }
+   void alloc( Context* c, size_t count );
+   void upload( Context* c, const std::vector< T >& h );
+   void release();
+};
+
+// Per-cell topology as seen from the cell: ids of its faces/edges/vertices and
+// which cell-local vertex each of their (sorted) vertices is.  Level-free.
+struct CellMeta
+{
+   int32_t face[4];
+   int8_t  fperm[4][3];
+   int32_t edge[6];
+   int8_t  eperm[6][2];
+   int32_t vert[4];
+};
+
+// One storage, one level: lattice layout and owners.  Storage of a function on
+// this level is
+//   [ nc macro-cell lattices of sc points each | nb owned vertex/edge/face DoFs ]
+// where the owned part is in canonical order (vertices, edge interiors, face
+// interiors) and the lattice points on a cell's boundary are halo copies of
+// owned DoFs.
+struct Lattice
+{
+   int            L = 0;
+   int64_t        mm = 1, w = 2, sc = 0, nc = 0, nb = 0, nstore = 0;
+   LevelNumbering num;
+
+   DevBuf< uint8_t >  bndFlag;  // DoFType of each owned vertex/edge/face DoF
+   DevBuf< uint32_t > cellMask; // bit set on cell-owned (interior) lattice points
+   std::vector< uint8_t > hBndFlag;
+
+   // interior rows (y,z) of a cell lattice, y,z >= 1, y+z <= mm-2, grouped by
+   // (y%2, z%2); group g = (y%2) + 2*(z%2) occupies [rowGroup[g], rowGroup[g+1])
+   DevBuf< int32_t > rows; // 2 ints per row
+   int64_t           rowGroup[5] = { 0, 0, 0, 0, 0 };
+   int64_t           nrows       = 0;
+
+   int64_t cellPos( int c, int64_t x, int64_t y, int64_t z ) const { return c * sc + tetIndex( w, x, y, z ); }
+   int64_t ownerPos( const Topology& t, int c, int64_t x, int64_t y, int64_t z ) const
+   {
+      const int64_t id = num.locate( t, c, x, y, z );
+      return id < num.coff ? nc * sc + id : cellPos( c, x, y, z );
+   }
+   // halo points per cell: 4 face interiors, 6 edge interiors, 4 vertices
+   int64_t haloPerCell() const { return 4 * num.fin + 6 * num.ein + 4; }
+};
+
+struct OperatorBase
+{
+   virtual ~OperatorBase() = default;
+};
+
+struct Storage
+{
+   Topology                                     topo;
+   // grid-transfer data keyed by fine level (built on first use)
+   std::map< int, std::unique_ptr< OperatorBase > > transfers;
+   std::map< int, std::unique_ptr< Lattice > >  lattices;
+   DevBuf< CellMeta >                           meta;
+   // cells incident to each face / edge / vertex: CSR of (cell, local index)
+   DevBuf< int32_t > faceCellPtr, faceCells, edgeCellPtr, edgeCells, vertCellPtr, vertCells;
+   Lattice&                                     lattice( Context* ctx, int L );
+   void                                         init( Context* ctx );
+};
+
+struct Function
+{
+   Storage*              st    = nullptr;
+   int                   dtype = 0; // 0 f64, 1 f32, 2 f16
+   int                   minL = 0, maxL = 0;
+   std::map< int, DevBuf< uint8_t > > data; // raw bytes, nstore elements
+   size_t                elemSize() const { return dtype == 0 ? 8 : ( dtype == 1 ? 4 : 2 ); }
+   void*                 ptr( int L )
+   {
+      auto it = data.find( L );
+      if ( it == data.end() )
+         throw Error( -1, "function has no data on level " + std::to_string( L ) );
+      return it->second.p;
+   }
+};
+
+struct Context
+{
+   int                                                device = 0;
+   hipStream_t                                        stream = nullptr;
+   std::string                                        err;
+   int64_t                                            bytes      = 0;
+   int64_t                                            nextHandle = 1;
+   std::map< int64_t, std::unique_ptr< Storage > >      storages;
+   std::map< int64_t, std::unique_ptr< Function > >     functions;
+   std::map< int64_t, std::unique_ptr< OperatorBase > > operators;
+
+   // FP64 reduction scratch
+   DevBuf< double > partials;
+   double*          hostResult = nullptr; // pinned
+
+   bool       timing = false;
+   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+   double     lastMs = 0.0;
+
+   Storage&  storage( int64_t h );
+   Function& function( int64_t h );
+   template < class Op >
+   Op& op( int64_t h )
+   {
+      auto it = operators.find( h );
+      if ( it == operators.end() )
+         throw Error( -3, "unknown operator handle" );
+      auto* p = dynamic_cast< Op* >( it->second.get() );
+      if ( !p )
+         throw Error( -3, "operator handle of wrong kind" );
+      return *p;
+   }
+   void beginTiming();
+   void endTiming();
+};
+
+// ---- device helpers used across translation units ----------------------
+constexpr int kBlock      = 256;
+constexpr int kDotBlocks  = 1024;
+
+inline unsigned gridFor( int64_t n, int block = kBlock )
+{
+   int64_t g = ( n + block - 1 ) / block;
+   if ( g < 1 )
+      g = 1;
+   if ( g > 2147483647LL )
+      throw Error( -1, "grid too large" );
+   return unsigned( g );
+}
+
+// Refresh every halo copy in the cell lattices from its owner (the
+// Vertex->Edge->Face->Cell communicate chain of VertexDoFFunction).
+void pullHalos( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype );
+// Sum the halo copies of every owned vertex/edge/face DoF into the owner
+// (communicateAdditively<Cell, Face/Edge/Vertex>).  Owners whose type matches
+// excludeFlag are left untouched; zeroOut replaces instead of adding.
+void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int excludeFlag, bool zeroOut );
+// FP64 dot over owned DoFs of matching type
+double dotDevice( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag );
+// storage type dispatch
+template < class F >
+void dispatchType( int dtype, F&& f )
+{
+   switch ( dtype )
+   {
+   case 0:
+      f( double() );
+      break;
+   case 1:
+      f( float() );
+      break;
+   case 2:
+      f( __half() );
+      break;
+   default:
+      throw Error( -1, "unknown storage type" );
+   }
+}
+// arithmetic type used for a storage type: FP64 for FP64, FP32 otherwise
+template < class T >
+struct Acc
+{
+   using type = float;
+};
+template <>
+struct Acc< double >
+{
+   using type = double;
+};
+template < class T >
+__host__ __device__ __forceinline__ typename Acc< T >::type toAcc( T v )
+{
+   return static_cast< typename Acc< T >::type >( v );
+}
+template <>
+__host__ __device__ __forceinline__ float toAcc< __half >( __half v )
+{
+   return __half2float( v );
+}
+template < class T >
+__host__ __device__ __forceinline__ T fromAcc( typename Acc< T >::type v )
+{
+   return static_cast< T >( v );
+}
+template <>
+__host__ __device__ __forceinline__ __half fromAcc< __half >( float v )
+{
+   return __float2half( v );
+}
+
+} // namespace hg

@@ -1,0 +1,155 @@
+/*
+ * hyteg_amd.h -- C ABI of the MI355X-native matrix-free operator / multigrid
+ * library.  Plain pointers, sizes and integer handles only; no device pointer
+ * and no C++ or torch type crosses this boundary.
+ *
+ * Every call returns 0 on success and a negative status otherwise; the message
+ * of the last failure is available from hg_last_error().  A context owns one
+ * HIP device, one HIP stream and every object created through it.  Contexts
+ * are not re-entrant: calls on one context must not overlap.
+ *
+ * Each entry point names the HyTeG interface it replaces (paths relative to
+ * the HyTeG source tree).
+ */
+#ifndef HYTEG_AMD_H
+#define HYTEG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hg_ctx hg_ctx;
+typedef int64_t       hg_handle;
+
+/* status codes */
+enum
+{
+   HG_OK            = 0,
+   HG_ERR_ARG       = -1,
+   HG_ERR_HIP       = -2,
+   HG_ERR_HANDLE    = -3,
+   HG_ERR_STATE     = -4,
+   HG_ERR_NOMEM     = -5,
+   HG_ERR_UNSUPPORT = -6
+};
+
+/* DoFType bits, types/types.hpp:36-44 */
+enum
+{
+   HG_INNER     = 1,
+   HG_DIRICHLET = 2,
+   HG_NEUMANN   = 4,
+   HG_FREESLIP  = 8,
+   HG_ALL       = 15
+};
+/* UpdateType, types/types.hpp:29-33 */
+enum
+{
+   HG_REPLACE = 0,
+   HG_ADD     = 1
+};
+/* storage value types of a function */
+enum
+{
+   HG_F64 = 0,
+   HG_F32 = 1,
+   HG_F16 = 2
+};
+
+/* ---- context (replaces walberla::Environment / MPIManager set-up) -------- */
+int         hg_ctx_create( int device, hg_ctx** out );
+int         hg_ctx_destroy( hg_ctx* ctx );
+const char* hg_last_error( const hg_ctx* ctx );
+int         hg_ctx_synchronize( hg_ctx* ctx );
+/* bytes of device memory held by the context */
+int hg_ctx_memory( hg_ctx* ctx, int64_t* bytes );
+
+/* ---- storage: MeshInfo + SetupPrimitiveStorage + PrimitiveStorage -------- */
+/* MeshInfo::meshCuboid(lowerLeft, upperRight, nx, ny, nz)  (mesh/MeshInfo.hpp:562,
+ * mesh/MeshGenCuboid.cpp:34-160), then SetupPrimitiveStorage(meshInfo, 1) with
+ * setMeshBoundaryFlagsOnBoundary(1, 0, true)
+ * (primitivestorage/SetupPrimitiveStorage.hpp:54-56, .cpp:977-988). */
+int hg_storage_cuboid( hg_ctx* ctx, const double* lower3, const double* upper3, int nx, int ny, int nz, hg_handle* out );
+/* MeshInfo built from vertex coordinates and tetrahedra (local vertex order as
+ * given; vertex ids 0..nv-1 in MeshInfo id order), e.g. a Gmsh file
+ * (mesh/GmshReaderForMSH22.hpp:60-170), same boundary flagging. */
+int hg_storage_create( hg_ctx* ctx, int nv, const double* coords, int nc, const int64_t* cells, hg_handle* out );
+/* nv, ne, nf, nc of the macro mesh (PrimitiveStorage::getNumberOf*) */
+int hg_storage_counts( hg_ctx* ctx, hg_handle storage, int64_t* out4 );
+/* number of P1 DoFs on a level in canonical order (numberOfGlobalDoFs,
+ * p1functionspace/VertexDoFFunction.cpp) */
+int hg_p1_ndofs( hg_ctx* ctx, hg_handle storage, int level, int64_t* out );
+/* coordinates (nd*3) and DoFType (nd) of every canonical P1 DoF */
+int hg_p1_dof_coords( hg_ctx* ctx, hg_handle storage, int level, double* xyz );
+int hg_p1_dof_flags( hg_ctx* ctx, hg_handle storage, int level, int32_t* flags );
+
+/* ---- functions: P1Function<T> (p1functionspace/VertexDoFFunction.hpp:58-512) */
+/* P1Function<T>(name, storage, minLevel, maxLevel), all values zero */
+int hg_p1_function_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, hg_handle* out );
+int hg_function_destroy( hg_ctx* ctx, hg_handle fn );
+/* storage handle a function lives on (Function::getStorage, functions/Function.hpp:35-80) */
+int hg_function_storage( hg_ctx* ctx, hg_handle fn, hg_handle* storage );
+/* host canonical FP64 array <-> device (interpolate from host data / getDoFs) */
+int hg_function_upload( hg_ctx* ctx, hg_handle fn, int level, const double* canonical );
+int hg_function_download( hg_ctx* ctx, hg_handle fn, int level, double* canonical );
+/* assign(scalars, functions, level, flag): dst = sum_i s_i f_i (VertexDoFFunction.cpp:1131) */
+int hg_function_assign( hg_ctx* ctx, hg_handle dst, int nsrc, const double* scalars, const hg_handle* srcs, int level, int flag );
+/* add(scalars, functions, level, flag): dst += sum_i s_i f_i (VertexDoFFunction.cpp:1224) */
+int hg_function_add( hg_ctx* ctx, hg_handle dst, int nsrc, const double* scalars, const hg_handle* srcs, int level, int flag );
+/* interpolate(constant, level, flag) (VertexDoFFunction.cpp:384) */
+int hg_function_interpolate_const( hg_ctx* ctx, hg_handle fn, double value, int level, int flag );
+/* dotGlobal(other, level, flag) -- FP64 accumulation (VertexDoFFunction.cpp:1710-1716) */
+int hg_function_dot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
+/* copyFrom<otherT>(other, level): cast copy between storage types (VertexDoFFunction.hpp:151) */
+int hg_function_copy_from( hg_ctx* ctx, hg_handle dst, hg_handle src, int level );
+/* communicate<Vertex,Edge>..<Face,Cell>: refresh every macro-cell halo copy from
+ * its owner (VertexDoFFunction.hpp:366-392, VertexDoFPackInfo.hpp:534-610) */
+int hg_function_communicate( hg_ctx* ctx, hg_handle fn, int level );
+
+/* ---- P1 constant-stencil operator (constant_stencil_operator/P1ConstantOperator.hpp:167-168) */
+/* P1ConstantLaplaceOperator(storage, minLevel, maxLevel) */
+int hg_p1_laplace_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, hg_handle* out );
+int hg_operator_destroy( hg_ctx* ctx, hg_handle op );
+/* Operator::apply(src, dst, level, flag, updateType) (operators/Operator.hpp:74-118,
+ * p1functionspace/P1Operator.hpp:192-320) */
+int hg_p1_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update );
+/* SORSmoothable::smooth_sor(dst, rhs, relax, level, flag) (solvers/Smoothables.hpp:34-120,
+ * P1Operator.hpp:340-418).  Multicolour ordering: macro-vertex/edge/face DoFs
+ * in graph colours, then the 8 parity colours of every macro-cell lattice.
+ * smooth_gs == relax 1. */
+int hg_p1_smooth_sor( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, double relax, int level, int flag );
+/* stencil weights of the cell interior (15, same order as
+ * apply_3D_macrocell_vertexdof_to_vertexdof_replace.cpp:36-50) */
+int hg_p1_cell_stencil( hg_ctx* ctx, hg_handle op, int level, int cell, double* out15 );
+
+/* ---- grid transfer ------------------------------------------------------ */
+/* P1toP1LinearRestriction::restrict(f, sourceLevel, flag)
+ * (gridtransferoperators/P1toP1LinearRestriction.cpp:169-345) */
+int hg_p1_restrict( hg_ctx* ctx, hg_handle fn, int sourceLevel, int flag );
+/* P1toP1LinearProlongation::prolongate / prolongateAndAdd(f, sourceLevel, flag)
+ * (gridtransferoperators/P1toP1LinearProlongation.hpp:34-58, .cpp:194-411) */
+int hg_p1_prolongate( hg_ctx* ctx, hg_handle fn, int sourceLevel, int flag, int update );
+
+/* ---- solvers (host control flow, device kernels) ------------------------ */
+/* CGSolver(storage, minLevel, maxLevel, maxIter, relTol, absTol).solve(A, x, b, level)
+ * (solvers/CGSolver.hpp:36-203).  *iterations receives the iteration count. */
+int hg_p1_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int maxIter, double relTol, double absTol,
+                    int* iterations );
+/* GeometricMultigridSolver(storage, GaussSeidelSmoother, CGSolver(minLevel), P1toP1LinearRestriction,
+ * P1toP1LinearProlongation, minLevel, maxLevel, pre, post).solve(A, x, b, maxLevel), called
+ * `cycles` times or until ||b - Ax||_Inner <= tol * ||b - Ax0||_Inner (tol > 0).
+ * res_hist (cycles+1 entries, may be NULL) receives the residual norms.
+ * (solvers/GeometricMultigridSolver.hpp:126-308, solvers/GaussSeidelSmoother.hpp:31-55) */
+int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre, int post,
+                     int cycles, double tol, int coarseMaxIter, double coarseTol, double* res_hist, int* cycles_done );
+
+/* ---- timing of the last entry point, device time in milliseconds --------- */
+int hg_ctx_event_timing( hg_ctx* ctx, int enable );
+int hg_ctx_last_kernel_ms( hg_ctx* ctx, double* ms );
+
+#ifdef __cplusplus
+}
+#endif
+#endif

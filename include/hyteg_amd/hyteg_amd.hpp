@@ -1,0 +1,536 @@
+// hyteg_amd.hpp -- C++ host API with HyTeG's Function / Operator / Solver
+// shape, on top of the C ABI in hyteg_amd.h.  Header-only; solver control
+// flow runs on the host, every numerical step is one or a few device kernels.
+//
+// Reference interfaces mirrored (paths relative to the HyTeG source tree):
+//   PrimitiveStorage                    primitivestorage/PrimitiveStorage.hpp:116-220
+//   P1Function<T>                       p1functionspace/P1Function.hpp:27, VertexDoFFunction.hpp:58-512
+//   Operator<Src,Dst>::apply            operators/Operator.hpp:34-118
+//   GSSmoothable / SORSmoothable        solvers/Smoothables.hpp:34-120
+//   P1ConstantLaplaceOperator           constant_stencil_operator/P1ConstantOperator.hpp:167-168
+//   P1toP1LinearRestriction/Prolongation gridtransferoperators/P1toP1Linear*.hpp
+//   Solver<Op>                          solvers/Solver.hpp:27-34
+//   GaussSeidelSmoother, SORSmoother    solvers/GaussSeidelSmoother.hpp:31-55, SORSmoother.hpp
+//   CGSolver                            solvers/CGSolver.hpp:36-203
+//   GeometricMultigridSolver            solvers/GeometricMultigridSolver.hpp:38-308
+#pragma once
+
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../hyteg_amd.h"
+
+namespace hyteg_amd {
+
+using uint_t = unsigned long;
+
+enum DoFType : int
+{
+   None              = 0,
+   Inner             = HG_INNER,
+   DirichletBoundary = HG_DIRICHLET,
+   NeumannBoundary   = HG_NEUMANN,
+   FreeslipBoundary  = HG_FREESLIP,
+   All               = HG_ALL
+};
+inline DoFType operator|( DoFType a, DoFType b )
+{
+   return DoFType( int( a ) | int( b ) );
+}
+enum UpdateType : int
+{
+   Replace = HG_REPLACE,
+   Add     = HG_ADD
+};
+enum class CycleType
+{
+   VCYCLE,
+   WCYCLE
+};
+
+inline void check( hg_ctx* ctx, int status, const char* what )
+{
+   if ( status != HG_OK )
+      throw std::runtime_error( std::string( what ) + ": " + hg_last_error( ctx ) );
+}
+
+// Owns an hg_ctx (one device, one stream).
+class Context
+{
+ public:
+   explicit Context( int device = 0 )
+   {
+      if ( hg_ctx_create( device, &ctx_ ) != HG_OK )
+         throw std::runtime_error( "hg_ctx_create failed" );
+   }
+   ~Context()
+   {
+      if ( ctx_ )
+         hg_ctx_destroy( ctx_ );
+   }
+   Context( const Context& )            = delete;
+   Context& operator=( const Context& ) = delete;
+   hg_ctx*  get() const { return ctx_; }
+   void     synchronize() const { check( ctx_, hg_ctx_synchronize( ctx_ ), "synchronize" ); }
+
+ private:
+   hg_ctx* ctx_ = nullptr;
+};
+
+// PrimitiveStorage: macro mesh with boundary flags (setMeshBoundaryFlagsOnBoundary(1,0,true)).
+class PrimitiveStorage
+{
+ public:
+   PrimitiveStorage( std::shared_ptr< Context > ctx, hg_handle h )
+   : owner_( std::move( ctx ) )
+   , ctx_( owner_->get() )
+   , h_( h )
+   {}
+   // view of a storage created elsewhere in the same context
+   static std::shared_ptr< PrimitiveStorage > borrow( hg_ctx* ctx, hg_handle h )
+   {
+      return std::shared_ptr< PrimitiveStorage >( new PrimitiveStorage( ctx, h ) );
+   }
+   static std::shared_ptr< PrimitiveStorage >
+       cuboid( std::shared_ptr< Context > ctx, const double lo[3], const double hi[3], int nx, int ny, int nz )
+   {
+      hg_handle h;
+      check( ctx->get(), hg_storage_cuboid( ctx->get(), lo, hi, nx, ny, nz, &h ), "meshCuboid" );
+      return std::make_shared< PrimitiveStorage >( ctx, h );
+   }
+   static std::shared_ptr< PrimitiveStorage >
+       fromCells( std::shared_ptr< Context > ctx, const std::vector< double >& coords, const std::vector< int64_t >& cells )
+   {
+      hg_handle h;
+      check( ctx->get(),
+             hg_storage_create( ctx->get(), int( coords.size() / 3 ), coords.data(), int( cells.size() / 4 ), cells.data(), &h ),
+             "storage" );
+      return std::make_shared< PrimitiveStorage >( ctx, h );
+   }
+   hg_ctx*   ctx() const { return ctx_; }
+   hg_handle handle() const { return h_; }
+
+ private:
+   PrimitiveStorage( hg_ctx* ctx, hg_handle h )
+   : ctx_( ctx )
+   , h_( h )
+   {}
+   std::shared_ptr< Context > owner_;
+   hg_ctx*                    ctx_ = nullptr;
+   hg_handle                  h_   = 0;
+};
+
+template < typename T >
+struct DTypeOf;
+template <>
+struct DTypeOf< double >
+{
+   static constexpr int value = HG_F64;
+};
+template <>
+struct DTypeOf< float >
+{
+   static constexpr int value = HG_F32;
+};
+
+// P1Function<ValueType>.  Wraps an existing handle without owning it when
+// constructed through `borrow`.
+template < typename ValueType >
+class P1Function
+{
+ public:
+   using valueType = ValueType;
+
+   P1Function( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
+   : name_( name )
+   , storage_( storage )
+   , ctx_( storage->ctx() )
+   , owned_( true )
+   {
+      check( ctx_, hg_p1_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), DTypeOf< ValueType >::value, &h_ ),
+             "P1Function" );
+   }
+   ~P1Function()
+   {
+      if ( owned_ && h_ )
+         hg_function_destroy( ctx_, h_ );
+   }
+   P1Function( const P1Function& )            = delete;
+   P1Function& operator=( const P1Function& ) = delete;
+
+   static std::unique_ptr< P1Function > borrow( hg_ctx* ctx, hg_handle h )
+   {
+      return std::unique_ptr< P1Function >( new P1Function( ctx, h ) );
+   }
+
+   hg_handle handle() const { return h_; }
+   hg_ctx*   ctx() const { return ctx_; }
+
+   void assign( const std::vector< ValueType >&                                       scalars,
+                const std::vector< std::reference_wrapper< const P1Function > >& functions,
+                uint_t                                                            level,
+                DoFType                                                           flag = All ) const
+   {
+      linComb( scalars, functions, level, flag, false );
+   }
+   void add( const std::vector< ValueType >&                                       scalars,
+             const std::vector< std::reference_wrapper< const P1Function > >& functions,
+             uint_t                                                            level,
+             DoFType                                                           flag = All ) const
+   {
+      linComb( scalars, functions, level, flag, true );
+   }
+   void interpolate( ValueType constant, uint_t level, DoFType flag = All ) const
+   {
+      check( ctx_, hg_function_interpolate_const( ctx_, h_, double( constant ), int( level ), flag ), "interpolate" );
+   }
+   void setToZero( uint_t level ) const { interpolate( ValueType( 0 ), level, All ); }
+   ValueType dotGlobal( const P1Function& other, uint_t level, DoFType flag = All ) const
+   {
+      double r = 0.0;
+      check( ctx_, hg_function_dot( ctx_, h_, other.h_, int( level ), flag, &r ), "dotGlobal" );
+      return ValueType( r );
+   }
+   template < typename OtherT >
+   void copyFrom( const P1Function< OtherT >& other, uint_t level ) const
+   {
+      check( ctx_, hg_function_copy_from( ctx_, h_, other.handle(), int( level ) ), "copyFrom" );
+   }
+   void communicate( uint_t level ) const { check( ctx_, hg_function_communicate( ctx_, h_, int( level ) ), "communicate" ); }
+
+ private:
+   P1Function( hg_ctx* ctx, hg_handle h )
+   : ctx_( ctx )
+   , h_( h )
+   , owned_( false )
+   {}
+   void linComb( const std::vector< ValueType >&                                       scalars,
+                 const std::vector< std::reference_wrapper< const P1Function > >& functions,
+                 uint_t                                                            level,
+                 DoFType                                                           flag,
+                 bool                                                              add ) const
+   {
+      if ( scalars.size() != functions.size() )
+         throw std::invalid_argument( "assign: sizes differ" );
+      std::vector< double >    s( scalars.begin(), scalars.end() );
+      std::vector< hg_handle > f;
+      for ( const auto& r : functions )
+         f.push_back( r.get().h_ );
+      // the device kernel combines up to three sources per pass
+      size_t i = 0;
+      bool   first = true;
+      do
+      {
+         const int n = int( std::min< size_t >( 3, f.size() - i ) );
+         const int st =
+             ( add || !first ) ? hg_function_add( ctx_, h_, n, s.data() + i, f.data() + i, int( level ), flag ) :
+                                 hg_function_assign( ctx_, h_, n, s.data() + i, f.data() + i, int( level ), flag );
+         check( ctx_, st, "assign" );
+         i += size_t( n );
+         first = false;
+      } while ( i < f.size() );
+   }
+
+   std::string                         name_;
+   std::shared_ptr< PrimitiveStorage > storage_;
+   hg_ctx*                             ctx_   = nullptr;
+   hg_handle                           h_     = 0;
+   bool                                owned_ = true;
+};
+
+// Operator<Src,Dst>
+template < class SrcT, class DstT >
+class Operator
+{
+ public:
+   using srcType = SrcT;
+   using dstType = DstT;
+   virtual ~Operator() = default;
+   virtual void apply( const SrcT& src, const DstT& dst, uint_t level, DoFType flag, UpdateType updateType = Replace ) const = 0;
+};
+
+template < class F >
+class GSSmoothable
+{
+ public:
+   virtual ~GSSmoothable()                                                                    = default;
+   virtual void smooth_gs( const F& dst, const F& rhs, uint_t level, DoFType flag ) const = 0;
+};
+template < class F >
+class SORSmoothable
+{
+ public:
+   virtual ~SORSmoothable() = default;
+   virtual void smooth_sor( const F& dst, const F& rhs, typename F::valueType relax, uint_t level, DoFType flag ) const = 0;
+};
+
+class P1ConstantLaplaceOperator : public Operator< P1Function< double >, P1Function< double > >,
+                                  public GSSmoothable< P1Function< double > >,
+                                  public SORSmoothable< P1Function< double > >
+{
+ public:
+   P1ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
+   : ctx_( storage->ctx() )
+   , owned_( true )
+   {
+      check( ctx_, hg_p1_laplace_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), &h_ ), "P1ConstantLaplaceOperator" );
+   }
+   static std::unique_ptr< P1ConstantLaplaceOperator > borrow( hg_ctx* ctx, hg_handle h )
+   {
+      return std::unique_ptr< P1ConstantLaplaceOperator >( new P1ConstantLaplaceOperator( ctx, h ) );
+   }
+   ~P1ConstantLaplaceOperator() override
+   {
+      if ( owned_ && h_ )
+         hg_operator_destroy( ctx_, h_ );
+   }
+   void apply( const P1Function< double >& src, const P1Function< double >& dst, uint_t level, DoFType flag,
+               UpdateType updateType = Replace ) const override
+   {
+      check( ctx_, hg_p1_apply( ctx_, h_, src.handle(), dst.handle(), int( level ), flag, updateType ), "apply" );
+   }
+   void smooth_gs( const P1Function< double >& dst, const P1Function< double >& rhs, uint_t level, DoFType flag ) const override
+   {
+      smooth_sor( dst, rhs, 1.0, level, flag );
+   }
+   void smooth_sor( const P1Function< double >& dst, const P1Function< double >& rhs, double relax, uint_t level,
+                    DoFType flag ) const override
+   {
+      check( ctx_, hg_p1_smooth_sor( ctx_, h_, dst.handle(), rhs.handle(), relax, int( level ), flag ), "smooth_sor" );
+   }
+   hg_handle handle() const { return h_; }
+
+ private:
+   P1ConstantLaplaceOperator( hg_ctx* ctx, hg_handle h )
+   : ctx_( ctx )
+   , h_( h )
+   , owned_( false )
+   {}
+   hg_ctx*   ctx_ = nullptr;
+   hg_handle h_   = 0;
+   bool      owned_;
+};
+
+template < class F >
+class RestrictionOperator
+{
+ public:
+   virtual ~RestrictionOperator()                                                    = default;
+   virtual void restrict( const F& function, uint_t sourceLevel, DoFType flag ) const = 0;
+};
+template < class F >
+class ProlongationOperator
+{
+ public:
+   virtual ~ProlongationOperator()                                                           = default;
+   virtual void prolongate( const F& function, uint_t sourceLevel, DoFType flag ) const       = 0;
+   virtual void prolongateAndAdd( const F& function, uint_t sourceLevel, DoFType flag ) const = 0;
+};
+
+template < typename ValueType = double >
+class P1toP1LinearRestriction : public RestrictionOperator< P1Function< ValueType > >
+{
+ public:
+   void restrict( const P1Function< ValueType >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      check( f.ctx(), hg_p1_restrict( f.ctx(), f.handle(), int( sourceLevel ), flag ), "restrict" );
+   }
+};
+template < typename ValueType = double >
+class P1toP1LinearProlongation : public ProlongationOperator< P1Function< ValueType > >
+{
+ public:
+   void prolongate( const P1Function< ValueType >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      check( f.ctx(), hg_p1_prolongate( f.ctx(), f.handle(), int( sourceLevel ), flag, HG_REPLACE ), "prolongate" );
+   }
+   void prolongateAndAdd( const P1Function< ValueType >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      check( f.ctx(), hg_p1_prolongate( f.ctx(), f.handle(), int( sourceLevel ), flag, HG_ADD ), "prolongateAndAdd" );
+   }
+};
+
+// Solver<Op>
+template < class OperatorType >
+class Solver
+{
+ public:
+   virtual ~Solver() = default;
+   virtual void solve( const OperatorType&                   A,
+                       const typename OperatorType::srcType& x,
+                       const typename OperatorType::dstType& b,
+                       uint_t                                level ) = 0;
+};
+
+template < class OperatorType >
+class GaussSeidelSmoother : public Solver< OperatorType >
+{
+ public:
+   void solve( const OperatorType& A, const typename OperatorType::srcType& x, const typename OperatorType::dstType& b,
+               uint_t level ) override
+   {
+      if ( const auto* gs = dynamic_cast< const GSSmoothable< typename OperatorType::srcType >* >( &A ) )
+         gs->smooth_gs( x, b, level, flag_ );
+      else
+         throw std::runtime_error( "The Gauss-Seidel Operator requires the GSSmoothable interface." );
+   }
+
+ private:
+   DoFType flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+template < class OperatorType >
+class SORSmoother : public Solver< OperatorType >
+{
+ public:
+   explicit SORSmoother( double relax )
+   : relax_( relax )
+   {}
+   void solve( const OperatorType& A, const typename OperatorType::srcType& x, const typename OperatorType::dstType& b,
+               uint_t level ) override
+   {
+      if ( const auto* s = dynamic_cast< const SORSmoothable< typename OperatorType::srcType >* >( &A ) )
+         s->smooth_sor( x, b, relax_, level, flag_ );
+      else
+         throw std::runtime_error( "The SOR Operator requires the SORSmoothable interface." );
+   }
+
+ private:
+   double  relax_;
+   DoFType flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+// CGSolver with the identity preconditioner (CGSolver.hpp:94-203, 340-358)
+template < class OperatorType >
+class CGSolver : public Solver< OperatorType >
+{
+ public:
+   using FunctionType = typename OperatorType::srcType;
+   using ValueType    = typename FunctionType::valueType;
+
+   CGSolver( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel,
+             uint_t maxIter = std::numeric_limits< uint_t >::max(), ValueType relativeTolerance = 1e-16,
+             ValueType absoluteTolerance = 1e-16 )
+   : maxIter_( maxIter )
+   , relTol_( relativeTolerance )
+   , absTol_( absoluteTolerance )
+   , p_( "p", storage, minLevel, maxLevel )
+   , ap_( "ap", storage, minLevel, maxLevel )
+   , r_( "r", storage, minLevel, maxLevel )
+   {}
+
+   void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      if ( maxIter_ == 0 )
+         return;
+      p_.setToZero( level );
+      ap_.setToZero( level );
+      r_.setToZero( level );
+      A.apply( x, p_, level, flag_, Replace );
+      r_.assign( { 1.0, -1.0 }, { b, p_ }, level, flag_ );
+      // identity preconditioner: z = r
+      p_.assign( { 1.0 }, { r_ }, level, flag_ );
+      ValueType prsold    = r_.dotGlobal( r_, level, flag_ );
+      ValueType res_start = std::sqrt( prsold );
+      iterations_         = 0;
+      if ( res_start < absTol_ )
+         return;
+      for ( uint_t i = 0; i < maxIter_; ++i )
+      {
+         A.apply( p_, ap_, level, flag_, Replace );
+         const ValueType pAp   = p_.dotGlobal( ap_, level, flag_ );
+         const ValueType alpha = prsold / pAp;
+         x.add( { alpha }, { p_ }, level, flag_ );
+         r_.add( { -alpha }, { ap_ }, level, flag_ );
+         const ValueType rsnew  = r_.dotGlobal( r_, level, flag_ );
+         const ValueType relRes = std::sqrt( rsnew ) / res_start;
+         iterations_            = i;
+         if ( relRes < relTol_ || std::sqrt( rsnew ) < absTol_ )
+            break;
+         const ValueType beta = rsnew / prsold;
+         p_.assign( { 1.0, beta }, { r_, p_ }, level, flag_ );
+         prsold = rsnew;
+      }
+   }
+   uint_t getIterations() const { return iterations_; }
+
+ private:
+   uint_t       maxIter_;
+   ValueType    relTol_, absTol_;
+   FunctionType p_, ap_, r_;
+   DoFType      flag_       = Inner | NeumannBoundary | FreeslipBoundary;
+   uint_t       iterations_ = 0;
+};
+
+// GeometricMultigridSolver (GeometricMultigridSolver.hpp:126-308)
+template < class OperatorType >
+class GeometricMultigridSolver : public Solver< OperatorType >
+{
+ public:
+   using FunctionType = typename OperatorType::srcType;
+
+   GeometricMultigridSolver( const std::shared_ptr< PrimitiveStorage >&                storage,
+                             std::shared_ptr< Solver< OperatorType > >                smoother,
+                             std::shared_ptr< Solver< OperatorType > >                coarseSolver,
+                             std::shared_ptr< RestrictionOperator< FunctionType > >  restrictionOperator,
+                             std::shared_ptr< ProlongationOperator< FunctionType > > prolongationOperator,
+                             uint_t minLevel, uint_t maxLevel, uint_t preSmoothSteps = 3, uint_t postSmoothSteps = 3,
+                             uint_t smoothIncrementOnCoarserGrids = 0, CycleType cycleType = CycleType::VCYCLE )
+   : minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , pre_( preSmoothSteps )
+   , post_( postSmoothSteps )
+   , inc_( smoothIncrementOnCoarserGrids )
+   , cycleType_( cycleType )
+   , smoother_( std::move( smoother ) )
+   , coarse_( std::move( coarseSolver ) )
+   , restriction_( std::move( restrictionOperator ) )
+   , prolongation_( std::move( prolongationOperator ) )
+   , tmp_( "gmg_tmp", storage, minLevel, maxLevel )
+   {}
+
+   void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      invokedLevel_ = level;
+      solveRecursively( A, x, b, level );
+   }
+
+ private:
+   void solveRecursively( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level )
+   {
+      if ( level == minLevel_ )
+      {
+         coarse_->solve( A, x, b, minLevel_ );
+         return;
+      }
+      const uint_t preSteps = pre_ + inc_ * ( invokedLevel_ - level );
+      for ( uint_t i = 0; i < preSteps; ++i )
+         smoother_->solve( A, x, b, level );
+      A.apply( x, tmp_, level, flag_ );
+      tmp_.assign( { 1.0, -1.0 }, { b, tmp_ }, level, flag_ );
+      restriction_->restrict( tmp_, level, flag_ );
+      b.assign( { 1.0 }, { tmp_ }, level - 1, flag_ );
+      x.interpolate( 0, level - 1 );
+      solveRecursively( A, x, b, level - 1 );
+      if ( cycleType_ == CycleType::WCYCLE )
+         solveRecursively( A, x, b, level - 1 );
+      prolongation_->prolongateAndAdd( x, level - 1, flag_ );
+      const uint_t postSteps = post_ + inc_ * ( invokedLevel_ - level );
+      for ( uint_t i = 0; i < postSteps; ++i )
+         smoother_->solve( A, x, b, level );
+   }
+
+   uint_t                                                  minLevel_, maxLevel_, pre_, post_, inc_, invokedLevel_ = 0;
+   CycleType                                               cycleType_;
+   std::shared_ptr< Solver< OperatorType > >               smoother_, coarse_;
+   std::shared_ptr< RestrictionOperator< FunctionType > >  restriction_;
+   std::shared_ptr< ProlongationOperator< FunctionType > > prolongation_;
+   FunctionType                                            tmp_;
+   DoFType                                                 flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+} // namespace hyteg_amd

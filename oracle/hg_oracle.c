@@ -1,0 +1,1156 @@
+/*
+ * hg_oracle.c -- CPU restatement of HyTeG's P1 matrix-free operator, SOR,
+ * transfer, CG and geometric-multigrid path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY: the correctness oracle and CPU baseline of the
+ * hyteg_amd build.  Never linked into or called by the product library.
+ *
+ * Reference files followed (paths relative to the HyTeG source tree):
+ *   mesh/MeshGenCuboid.cpp:34-160, mesh/MeshInfo.cpp:37-84       (mesh, ordering)
+ *   primitivestorage/SetupPrimitiveStorage.cpp:98-380, 977-1130  (primitives, boundary flags)
+ *   indexing/MacroCellIndexing.hpp:40-64, Levelinfo.hpp:31-130   (lattice layout)
+ *   volumedofspace/CellDoFIndexing.hpp:38-200                    (6 micro-cell types)
+ *   forms/form_fenics_generated/p1_tet_diffusion.h               (P1 stiffness)
+ *   p1functionspace/P1Elements.hpp:627-700                       (stencil assembly)
+ *   p1functionspace/P1Operator.hpp:192-418, 1044-1108, 1352-1670 (apply, SOR)
+ *   gridtransferoperators/P1toP1LinearRestriction.cpp:169-345 + generated
+ *       restrict_3D_macrocell_P1_pull_additive.cpp               (restriction)
+ *   gridtransferoperators/P1toP1LinearProlongation.cpp:194-411   (prolongation)
+ *   solvers/CGSolver.hpp:94-203,340-358, GeometricMultigridSolver.hpp:180-308
+ *
+ * Parity pins (see tests/): numpy brute-force global assembly, linear
+ * null-space and energy identities, R = P^T adjointness, linear reproduction,
+ * direct-solve agreement, P1GMG3DConvergenceTest's rate bound.
+ */
+#include "hg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* lattice helpers                                                            */
+/* ------------------------------------------------------------------------- */
+static inline int64_t tri( int64_t n ) { return n <= 0 ? 0 : n * ( n + 1 ) / 2; }
+static inline int64_t tet( int64_t n ) { return n <= 0 ? 0 : n * ( n + 1 ) * ( n + 2 ) / 6; }
+/* MacroCellIndexing.hpp:40-53, width n (points per edge) */
+static inline int64_t tri_index( int64_t n, int64_t x, int64_t y ) { return y * n - y * ( y - 1 ) / 2 + x; }
+static inline int64_t tet_index( int64_t n, int64_t x, int64_t y, int64_t z )
+{
+   return tet( n ) - tet( n - z ) + y * ( n - z ) - y * ( y - 1 ) / 2 + x;
+}
+
+/* 6 micro-cell types (CellDoFIndexing.hpp:38-60, 150-200) */
+static const int VOF[6][4][3] = {
+    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, /* WHITE_UP   */
+    { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } }, /* BLUE_UP    */
+    { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, /* GREEN_UP   */
+    { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } }, /* WHITE_DOWN */
+    { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } }, /* BLUE_DOWN  */
+    { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } }, /* GREEN_DOWN */
+};
+/* micro-cells per row by type (CellDoFIndexing.hpp:64-83): m - WSHIFT */
+static const int WSHIFT[6] = { 0, 1, 1, 2, 1, 1 };
+
+/* 15-point stencil offsets (apply_3D_macrocell_vertexdof_to_vertexdof_replace.cpp:36-50) */
+static const int OFF15[15][3] = { { 0, 0, 0 },  { -1, 0, 0 }, { -1, 0, 1 }, { -1, 1, -1 }, { -1, 1, 0 },
+                                  { 0, -1, 0 }, { 0, -1, 1 }, { 0, 0, -1 }, { 0, 0, 1 },   { 0, 1, -1 },
+                                  { 0, 1, 0 },  { 1, -1, 0 }, { 1, -1, 1 }, { 1, 0, -1 },  { 1, 0, 0 } };
+static int off_slot( int dx, int dy, int dz )
+{
+   for ( int k = 0; k < 15; ++k )
+      if ( OFF15[k][0] == dx && OFF15[k][1] == dy && OFF15[k][2] == dz )
+         return k;
+   return -1;
+}
+
+/* local edges / faces of a cell (SetupPrimitiveStorage.cpp:290-300) */
+static const int LEDGE[6][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 0, 3 }, { 1, 3 }, { 2, 3 } };
+static const int LFACE[4][3] = { { 0, 1, 2 }, { 0, 1, 3 }, { 0, 2, 3 }, { 1, 2, 3 } };
+
+/* ------------------------------------------------------------------------- */
+/* mesh                                                                       */
+/* ------------------------------------------------------------------------- */
+struct or_mesh
+{
+   int     nv, ne, nf, nc;
+   double* X;
+   int*    ev; /* ne*2, sorted */
+   int*    fv; /* nf*3, sorted */
+   int*    cv; /* nc*4, local order */
+   int*    ce; /* nc*6 */
+   int*    cf; /* nc*4 */
+   int *   vflag, *eflag, *fflag;
+   int *   vnc, *enc, *fnc;
+};
+
+static int cmp_i2( const void* a, const void* b )
+{
+   const int* x = a;
+   const int* y = b;
+   for ( int k = 0; k < 2; ++k )
+      if ( x[k] != y[k] )
+         return x[k] < y[k] ? -1 : 1;
+   return 0;
+}
+static int cmp_i3( const void* a, const void* b )
+{
+   const int* x = a;
+   const int* y = b;
+   for ( int k = 0; k < 3; ++k )
+      if ( x[k] != y[k] )
+         return x[k] < y[k] ? -1 : 1;
+   return 0;
+}
+static int cmp_i4( const void* a, const void* b )
+{
+   const int* x = a;
+   const int* y = b;
+   for ( int k = 0; k < 4; ++k )
+      if ( x[k] != y[k] )
+         return x[k] < y[k] ? -1 : 1;
+   return 0;
+}
+static void sort_small( int* a, int n )
+{
+   for ( int i = 1; i < n; ++i )
+      for ( int j = i; j > 0 && a[j - 1] > a[j]; --j )
+      {
+         int t    = a[j];
+         a[j]     = a[j - 1];
+         a[j - 1] = t;
+      }
+}
+static int find_edge( const or_mesh* m, int a, int b )
+{
+   int key[2] = { a < b ? a : b, a < b ? b : a };
+   int lo = 0, hi = m->ne - 1;
+   while ( lo <= hi )
+   {
+      int mid = ( lo + hi ) / 2;
+      int c   = cmp_i2( key, m->ev + 2 * mid );
+      if ( c == 0 )
+         return mid;
+      if ( c < 0 )
+         hi = mid - 1;
+      else
+         lo = mid + 1;
+   }
+   return -1;
+}
+static int find_face( const or_mesh* m, int a, int b, int c )
+{
+   int key[3] = { a, b, c };
+   sort_small( key, 3 );
+   int lo = 0, hi = m->nf - 1;
+   while ( lo <= hi )
+   {
+      int mid = ( lo + hi ) / 2;
+      int r   = cmp_i3( key, m->fv + 3 * mid );
+      if ( r == 0 )
+         return mid;
+      if ( r < 0 )
+         hi = mid - 1;
+      else
+         lo = mid + 1;
+   }
+   return -1;
+}
+
+or_mesh* or_mesh_create( int nv, const double* coords, int nc, const int64_t* cells )
+{
+   or_mesh* m = calloc( 1, sizeof( or_mesh ) );
+   m->nv      = nv;
+   m->nc      = nc;
+   m->X       = malloc( sizeof( double ) * 3 * nv );
+   memcpy( m->X, coords, sizeof( double ) * 3 * nv );
+   m->cv = malloc( sizeof( int ) * 4 * nc );
+   for ( int i = 0; i < 4 * nc; ++i )
+      m->cv[i] = (int) cells[i];
+   qsort( m->cv, nc, 4 * sizeof( int ), cmp_i4 );
+
+   /* edges and faces: all sorted tuples, unique */
+   int* et = malloc( sizeof( int ) * 12 * nc );
+   int* ft = malloc( sizeof( int ) * 12 * nc );
+   for ( int c = 0; c < nc; ++c )
+   {
+      for ( int k = 0; k < 6; ++k )
+      {
+         int a = m->cv[4 * c + LEDGE[k][0]], b = m->cv[4 * c + LEDGE[k][1]];
+         et[12 * c + 2 * k]     = a < b ? a : b;
+         et[12 * c + 2 * k + 1] = a < b ? b : a;
+      }
+      for ( int k = 0; k < 4; ++k )
+      {
+         int* f = ft + 12 * c + 3 * k;
+         for ( int j = 0; j < 3; ++j )
+            f[j] = m->cv[4 * c + LFACE[k][j]];
+         sort_small( f, 3 );
+      }
+   }
+   qsort( et, 6 * nc, 2 * sizeof( int ), cmp_i2 );
+   qsort( ft, 4 * nc, 3 * sizeof( int ), cmp_i3 );
+   m->ev = malloc( sizeof( int ) * 12 * nc );
+   m->fv = malloc( sizeof( int ) * 12 * nc );
+   int ne = 0, nf = 0;
+   for ( int i = 0; i < 6 * nc; ++i )
+      if ( ne == 0 || cmp_i2( et + 2 * i, m->ev + 2 * ( ne - 1 ) ) != 0 )
+      {
+         memcpy( m->ev + 2 * ne, et + 2 * i, 2 * sizeof( int ) );
+         ++ne;
+      }
+   for ( int i = 0; i < 4 * nc; ++i )
+      if ( nf == 0 || cmp_i3( ft + 3 * i, m->fv + 3 * ( nf - 1 ) ) != 0 )
+      {
+         memcpy( m->fv + 3 * nf, ft + 3 * i, 3 * sizeof( int ) );
+         ++nf;
+      }
+   free( et );
+   free( ft );
+   m->ne = ne;
+   m->nf = nf;
+
+   m->ce  = malloc( sizeof( int ) * 6 * nc );
+   m->cf  = malloc( sizeof( int ) * 4 * nc );
+   m->vnc = calloc( nv, sizeof( int ) );
+   m->enc = calloc( ne, sizeof( int ) );
+   m->fnc = calloc( nf, sizeof( int ) );
+   for ( int c = 0; c < nc; ++c )
+   {
+      const int* v = m->cv + 4 * c;
+      for ( int k = 0; k < 6; ++k )
+      {
+         m->ce[6 * c + k] = find_edge( m, v[LEDGE[k][0]], v[LEDGE[k][1]] );
+         m->enc[m->ce[6 * c + k]]++;
+      }
+      for ( int k = 0; k < 4; ++k )
+      {
+         m->cf[4 * c + k] = find_face( m, v[LFACE[k][0]], v[LFACE[k][1]], v[LFACE[k][2]] );
+         m->fnc[m->cf[4 * c + k]]++;
+      }
+      for ( int k = 0; k < 4; ++k )
+         m->vnc[v[k]]++;
+   }
+   /* boundary flags: face on boundary iff one neighbour cell, lower-dim
+    * primitives on boundary iff a neighbour face is (SetupPrimitiveStorage.cpp:1092-1160) */
+   m->vflag = malloc( sizeof( int ) * nv );
+   m->eflag = malloc( sizeof( int ) * ne );
+   m->fflag = malloc( sizeof( int ) * nf );
+   for ( int i = 0; i < nv; ++i )
+      m->vflag[i] = OR_INNER;
+   for ( int i = 0; i < ne; ++i )
+      m->eflag[i] = OR_INNER;
+   for ( int f = 0; f < nf; ++f )
+   {
+      m->fflag[f] = m->fnc[f] == 1 ? OR_DIRICHLET : OR_INNER;
+      if ( m->fnc[f] == 1 )
+      {
+         const int* fv = m->fv + 3 * f;
+         for ( int j = 0; j < 3; ++j )
+            m->vflag[fv[j]] = OR_DIRICHLET;
+         m->eflag[find_edge( m, fv[0], fv[1] )] = OR_DIRICHLET;
+         m->eflag[find_edge( m, fv[0], fv[2] )] = OR_DIRICHLET;
+         m->eflag[find_edge( m, fv[1], fv[2] )] = OR_DIRICHLET;
+      }
+   }
+   return m;
+}
+
+or_mesh* or_mesh_cuboid( double x0, double y0, double z0, double x1, double y1, double z1, int nx, int ny, int nz )
+{
+   /* MeshGenCuboid.cpp:59-138: vertices z-y-x ordered, six tets per hex */
+   int     nv = ( nx + 1 ) * ( ny + 1 ) * ( nz + 1 );
+   double* X  = malloc( sizeof( double ) * 3 * nv );
+   double  hx = ( x1 - x0 ) / nx, hy = ( y1 - y0 ) / ny, hz = ( z1 - z0 ) / nz;
+   int     id = 0;
+   for ( int iz = 0; iz <= nz; ++iz )
+      for ( int iy = 0; iy <= ny; ++iy )
+         for ( int ix = 0; ix <= nx; ++ix )
+         {
+            X[3 * id]     = x0 + ix * hx;
+            X[3 * id + 1] = y0 + iy * hy;
+            X[3 * id + 2] = z0 + iz * hz;
+            ++id;
+         }
+   static const int tNode[6][4] = { { 4, 5, 7, 3 }, { 0, 3, 1, 4 }, { 4, 1, 5, 3 },
+                                    { 5, 6, 7, 2 }, { 1, 3, 2, 5 }, { 7, 2, 3, 5 } };
+   static const int hoff[8][3]  = { { 0, 0, 0 }, { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 },
+                                   { 0, 0, 1 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 1, 1 } };
+   int              nc          = 6 * nx * ny * nz;
+   int64_t*         cells       = malloc( sizeof( int64_t ) * 4 * nc );
+   int              c           = 0;
+   for ( int iz = 0; iz < nz; ++iz )
+      for ( int iy = 0; iy < ny; ++iy )
+         for ( int ix = 0; ix < nx; ++ix )
+            for ( int t = 0; t < 6; ++t )
+            {
+               for ( int k = 0; k < 4; ++k )
+               {
+                  const int* o       = hoff[tNode[t][k]];
+                  cells[4 * c + k] = ( iz + o[2] ) * ( nx + 1 ) * ( ny + 1 ) + ( iy + o[1] ) * ( nx + 1 ) + ( ix + o[0] );
+               }
+               ++c;
+            }
+   or_mesh* m = or_mesh_create( nv, X, nc, cells );
+   free( X );
+   free( cells );
+   return m;
+}
+
+void or_mesh_free( or_mesh* m )
+{
+   if ( !m )
+      return;
+   free( m->X );
+   free( m->ev );
+   free( m->fv );
+   free( m->cv );
+   free( m->ce );
+   free( m->cf );
+   free( m->vflag );
+   free( m->eflag );
+   free( m->fflag );
+   free( m->vnc );
+   free( m->enc );
+   free( m->fnc );
+   free( m );
+}
+
+void or_mesh_counts( const or_mesh* m, int64_t* out4 )
+{
+   out4[0] = m->nv;
+   out4[1] = m->ne;
+   out4[2] = m->nf;
+   out4[3] = m->nc;
+}
+void or_mesh_cell_vertices( const or_mesh* m, int64_t* out )
+{
+   for ( int i = 0; i < 4 * m->nc; ++i )
+      out[i] = m->cv[i];
+}
+
+/* ------------------------------------------------------------------------- */
+/* canonical numbering                                                        */
+/* ------------------------------------------------------------------------- */
+typedef struct
+{
+   int64_t mm;                         /* 2^L micro-edges per macro-edge */
+   int64_t eoff, foff, coff, n;        /* block offsets and total */
+   int64_t ein, fin, cin;              /* interior points per edge/face/cell */
+} lvl_t;
+
+static lvl_t lvl_info( const or_mesh* m, int L )
+{
+   lvl_t l;
+   l.mm   = (int64_t) 1 << L;
+   l.ein  = l.mm - 1;
+   l.fin  = tri( l.mm - 2 );
+   l.cin  = tet( l.mm - 3 );
+   l.eoff = m->nv;
+   l.foff = l.eoff + m->ne * l.ein;
+   l.coff = l.foff + m->nf * l.fin;
+   l.n    = l.coff + m->nc * l.cin;
+   return l;
+}
+
+int64_t or_p1_ndofs( const or_mesh* m, int level ) { return lvl_info( m, level ).n; }
+int64_t or_p1_nbnd( const or_mesh* m, int level ) { return lvl_info( m, level ).coff; }
+
+static int local_of( const int* cv, int g )
+{
+   for ( int k = 0; k < 4; ++k )
+      if ( cv[k] == g )
+         return k;
+   return -1;
+}
+
+static int64_t locate_l( const or_mesh* m, const lvl_t* l, int c, int64_t x, int64_t y, int64_t z )
+{
+   const int64_t t[4] = { l->mm - x - y - z, x, y, z };
+   int           S[4], ns = 0;
+   for ( int k = 0; k < 4; ++k )
+      if ( t[k] != 0 )
+         S[ns++] = k;
+   const int* cv = m->cv + 4 * c;
+   if ( ns == 1 )
+      return cv[S[0]];
+   if ( ns == 2 )
+   {
+      int e = -1;
+      for ( int k = 0; k < 6; ++k )
+         if ( ( LEDGE[k][0] == S[0] && LEDGE[k][1] == S[1] ) )
+            e = m->ce[6 * c + k];
+      const int     b   = m->ev[2 * e + 1];
+      const int64_t pos = t[local_of( cv, b )];
+      return l->eoff + e * l->ein + pos - 1;
+   }
+   if ( ns == 3 )
+   {
+      int f = -1;
+      for ( int k = 0; k < 4; ++k )
+         if ( LFACE[k][0] == S[0] && LFACE[k][1] == S[1] && LFACE[k][2] == S[2] )
+            f = m->cf[4 * c + k];
+      const int*    fv = m->fv + 3 * f;
+      const int64_t fx = t[local_of( cv, fv[1] )];
+      const int64_t fy = t[local_of( cv, fv[2] )];
+      return l->foff + f * l->fin + tri_index( l->mm - 2, fx - 1, fy - 1 );
+   }
+   return l->coff + c * l->cin + tet_index( l->mm - 3, x - 1, y - 1, z - 1 );
+}
+
+int64_t or_p1_locate( const or_mesh* m, int level, int c, int x, int y, int z )
+{
+   lvl_t l = lvl_info( m, level );
+   return locate_l( m, &l, c, x, y, z );
+}
+
+/* owner kind of a canonical id: 0 vertex, 1 edge, 2 face, 3 cell; *prim = index */
+static int owner_of( const or_mesh* m, const lvl_t* l, int64_t id, int64_t* prim )
+{
+   if ( id < l->eoff )
+   {
+      *prim = id;
+      return 0;
+   }
+   if ( id < l->foff )
+   {
+      *prim = ( id - l->eoff ) / l->ein;
+      return 1;
+   }
+   if ( id < l->coff )
+   {
+      *prim = ( id - l->foff ) / l->fin;
+      return 2;
+   }
+   *prim = ( id - l->coff ) / l->cin;
+   return 3;
+}
+
+static int owner_flag( const or_mesh* m, const lvl_t* l, int64_t id )
+{
+   int64_t p;
+   switch ( owner_of( m, l, id, &p ) )
+   {
+   case 0:
+      return m->vflag[p];
+   case 1:
+      return m->eflag[p];
+   case 2:
+      return m->fflag[p];
+   default:
+      return OR_INNER;
+   }
+}
+
+/* 1 / number of cells adjacent to the primitive owning id */
+static double inv_ncells( const or_mesh* m, const lvl_t* l, int64_t id )
+{
+   int64_t p;
+   switch ( owner_of( m, l, id, &p ) )
+   {
+   case 0:
+      return 1.0 / m->vnc[p];
+   case 1:
+      return 1.0 / m->enc[p];
+   case 2:
+      return 1.0 / m->fnc[p];
+   default:
+      return 1.0;
+   }
+}
+
+/* vertex set of an owner primitive, returns its size */
+static int owner_vertices( const or_mesh* m, const lvl_t* l, int64_t id, int* vs )
+{
+   int64_t p;
+   int     k = owner_of( m, l, id, &p );
+   if ( k == 0 )
+   {
+      vs[0] = (int) p;
+      return 1;
+   }
+   if ( k == 1 )
+   {
+      vs[0] = m->ev[2 * p];
+      vs[1] = m->ev[2 * p + 1];
+      return 2;
+   }
+   if ( k == 2 )
+   {
+      for ( int j = 0; j < 3; ++j )
+         vs[j] = m->fv[3 * p + j];
+      return 3;
+   }
+   for ( int j = 0; j < 4; ++j )
+      vs[j] = m->cv[4 * p + j];
+   return 4;
+}
+
+void or_p1_coords( const or_mesh* m, int level, double* xyz )
+{
+   lvl_t         l  = lvl_info( m, level );
+   const double* X  = m->X;
+   double        mm = (double) l.mm;
+   for ( int v = 0; v < m->nv; ++v )
+      for ( int d = 0; d < 3; ++d )
+         xyz[3 * v + d] = X[3 * v + d];
+   for ( int e = 0; e < m->ne; ++e )
+   {
+      const double* a = X + 3 * m->ev[2 * e];
+      const double* b = X + 3 * m->ev[2 * e + 1];
+      for ( int64_t t = 1; t < l.mm; ++t )
+         for ( int d = 0; d < 3; ++d )
+            xyz[3 * ( l.eoff + e * l.ein + t - 1 ) + d] = a[d] + ( t / mm ) * ( b[d] - a[d] );
+   }
+   for ( int f = 0; f < m->nf; ++f )
+   {
+      const double* a = X + 3 * m->fv[3 * f];
+      const double* b = X + 3 * m->fv[3 * f + 1];
+      const double* c = X + 3 * m->fv[3 * f + 2];
+      for ( int64_t y = 1; y < l.mm; ++y )
+         for ( int64_t x = 1; x + y < l.mm; ++x )
+         {
+            int64_t id = l.foff + f * l.fin + tri_index( l.mm - 2, x - 1, y - 1 );
+            for ( int d = 0; d < 3; ++d )
+               xyz[3 * id + d] = a[d] + ( x / mm ) * ( b[d] - a[d] ) + ( y / mm ) * ( c[d] - a[d] );
+         }
+   }
+   for ( int c = 0; c < m->nc; ++c )
+   {
+      const double* v0 = X + 3 * m->cv[4 * c];
+      const double* v1 = X + 3 * m->cv[4 * c + 1];
+      const double* v2 = X + 3 * m->cv[4 * c + 2];
+      const double* v3 = X + 3 * m->cv[4 * c + 3];
+      for ( int64_t z = 1; z < l.mm; ++z )
+         for ( int64_t y = 1; y + z < l.mm; ++y )
+            for ( int64_t x = 1; x + y + z < l.mm; ++x )
+            {
+               int64_t id = l.coff + c * l.cin + tet_index( l.mm - 3, x - 1, y - 1, z - 1 );
+               for ( int d = 0; d < 3; ++d )
+                  xyz[3 * id + d] = v0[d] + ( x / mm ) * ( v1[d] - v0[d] ) + ( y / mm ) * ( v2[d] - v0[d] ) +
+                                    ( z / mm ) * ( v3[d] - v0[d] );
+            }
+   }
+}
+
+void or_p1_flags( const or_mesh* m, int level, int32_t* flags )
+{
+   lvl_t l = lvl_info( m, level );
+   for ( int64_t i = 0; i < l.n; ++i )
+      flags[i] = owner_flag( m, &l, i );
+}
+
+/* ------------------------------------------------------------------------- */
+/* P1 element stiffness (p1_tet_diffusion: K_ij = |T| grad(phi_i).grad(phi_j)) */
+/* ------------------------------------------------------------------------- */
+static void p1_stiffness( const double p[4][3], double K[4][4] )
+{
+   double J[3][3];
+   for ( int r = 0; r < 3; ++r )
+      for ( int k = 0; k < 3; ++k )
+         J[r][k] = p[k + 1][r] - p[0][r]; /* columns = edge vectors */
+   double det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
+                J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
+   double inv[3][3];
+   inv[0][0] = ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) / det;
+   inv[0][1] = ( J[0][2] * J[2][1] - J[0][1] * J[2][2] ) / det;
+   inv[0][2] = ( J[0][1] * J[1][2] - J[0][2] * J[1][1] ) / det;
+   inv[1][0] = ( J[1][2] * J[2][0] - J[1][0] * J[2][2] ) / det;
+   inv[1][1] = ( J[0][0] * J[2][2] - J[0][2] * J[2][0] ) / det;
+   inv[1][2] = ( J[0][2] * J[1][0] - J[0][0] * J[1][2] ) / det;
+   inv[2][0] = ( J[1][0] * J[2][1] - J[1][1] * J[2][0] ) / det;
+   inv[2][1] = ( J[0][1] * J[2][0] - J[0][0] * J[2][1] ) / det;
+   inv[2][2] = ( J[0][0] * J[1][1] - J[0][1] * J[1][0] ) / det;
+   /* gradient of barycentric k (k=1..3) is row k-1 of J^-1 */
+   double g[4][3];
+   for ( int d = 0; d < 3; ++d )
+   {
+      g[1][d] = inv[0][d];
+      g[2][d] = inv[1][d];
+      g[3][d] = inv[2][d];
+      g[0][d] = -( g[1][d] + g[2][d] + g[3][d] );
+   }
+   double vol = fabs( det ) / 6.0;
+   for ( int i = 0; i < 4; ++i )
+      for ( int j = 0; j < 4; ++j )
+         K[i][j] = vol * ( g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2] );
+}
+
+static void micro_coord( const or_mesh* m, int c, double mm, const int q[3], double out[3] )
+{
+   const double* v0 = m->X + 3 * m->cv[4 * c];
+   const double* v1 = m->X + 3 * m->cv[4 * c + 1];
+   const double* v2 = m->X + 3 * m->cv[4 * c + 2];
+   const double* v3 = m->X + 3 * m->cv[4 * c + 3];
+   for ( int d = 0; d < 3; ++d )
+      out[d] = v0[d] + ( q[0] / mm ) * ( v1[d] - v0[d] ) + ( q[1] / mm ) * ( v2[d] - v0[d] ) + ( q[2] / mm ) * ( v3[d] - v0[d] );
+}
+
+/* element matrices of the 6 micro-cell types of cell c at level L */
+static void cell_type_matrices( const or_mesh* m, int c, int64_t mm, double K[6][4][4] )
+{
+   for ( int t = 0; t < 6; ++t )
+   {
+      double p[4][3];
+      for ( int k = 0; k < 4; ++k )
+         micro_coord( m, c, (double) mm, VOF[t][k], p[k] );
+      p1_stiffness( p, K[t] );
+   }
+}
+
+/* ------------------------------------------------------------------------- */
+/* P1 operator                                                                */
+/* ------------------------------------------------------------------------- */
+typedef struct
+{
+   int64_t n, cap;
+   int64_t* col;
+   double*  val;
+} row_t;
+
+static void row_add( row_t* r, int64_t col, double v )
+{
+   for ( int64_t k = 0; k < r->n; ++k )
+      if ( r->col[k] == col )
+      {
+         r->val[k] += v;
+         return;
+      }
+   if ( r->n == r->cap )
+   {
+      r->cap = r->cap ? 2 * r->cap : 16;
+      r->col = realloc( r->col, sizeof( int64_t ) * r->cap );
+      r->val = realloc( r->val, sizeof( double ) * r->cap );
+   }
+   r->col[r->n] = col;
+   r->val[r->n] = v;
+   r->n++;
+}
+
+struct or_p1op
+{
+   const or_mesh* m;
+   int            L;
+   lvl_t          l;
+   int64_t        w, sc;    /* lattice width, lattice size */
+   int64_t*       cmap;     /* nc * sc canonical ids of every cell lattice point */
+   double*        st;       /* nc * 15 cell stencils */
+   int64_t*       brp;      /* bnd CSR rowptr (nbnd+1) */
+   int64_t*       bcol;
+   double*        bval;
+   unsigned char* bcls;     /* 1 if the column lies in the closure of the row's owner */
+   int32_t*       flags;    /* canonical flags */
+};
+
+static inline int64_t lat( int64_t w, int64_t x, int64_t y, int64_t z ) { return tet_index( w, x, y, z ); }
+
+or_p1op* or_p1op_create( const or_mesh* m, int level )
+{
+   or_p1op* op = calloc( 1, sizeof( or_p1op ) );
+   op->m       = m;
+   op->L       = level;
+   op->l       = lvl_info( m, level );
+   const int64_t mm = op->l.mm;
+   op->w            = mm + 1;
+   op->sc           = tet( op->w );
+   op->cmap         = malloc( sizeof( int64_t ) * m->nc * op->sc );
+   for ( int c = 0; c < m->nc; ++c )
+      for ( int64_t z = 0; z <= mm; ++z )
+         for ( int64_t y = 0; y + z <= mm; ++y )
+            for ( int64_t x = 0; x + y + z <= mm; ++x )
+               op->cmap[c * op->sc + lat( op->w, x, y, z )] = locate_l( m, &op->l, c, x, y, z );
+
+   /* cell stencils: sum element matrices of the 24 micro-cells around an inner
+    * point (P1Elements.hpp:627-700) */
+   op->st = calloc( 15 * m->nc, sizeof( double ) );
+   for ( int c = 0; c < m->nc; ++c )
+   {
+      double K[6][4][4];
+      cell_type_matrices( m, c, mm, K );
+      for ( int t = 0; t < 6; ++t )
+         for ( int i = 0; i < 4; ++i )
+            for ( int j = 0; j < 4; ++j )
+            {
+               int s = off_slot( VOF[t][j][0] - VOF[t][i][0], VOF[t][j][1] - VOF[t][i][1], VOF[t][j][2] - VOF[t][i][2] );
+               op->st[15 * c + s] += K[t][i][j];
+            }
+   }
+
+   /* rows of DoFs owned by vertices/edges/faces: all micro-cells of all
+    * neighbour cells touching the DoF (the macro-face/edge/vertex stencils) */
+   const int64_t nb   = op->l.coff;
+   row_t*        rows = calloc( nb > 0 ? nb : 1, sizeof( row_t ) );
+   for ( int c = 0; c < m->nc; ++c )
+   {
+      double K[6][4][4];
+      cell_type_matrices( m, c, mm, K );
+      for ( int64_t z = 0; z <= mm; ++z )
+         for ( int64_t y = 0; y + z <= mm; ++y )
+            for ( int64_t x = 0; x + y + z <= mm; ++x )
+            {
+               if ( x != 0 && y != 0 && z != 0 && x + y + z != mm )
+                  continue;
+               const int64_t p = op->cmap[c * op->sc + lat( op->w, x, y, z )];
+               for ( int t = 0; t < 6; ++t )
+               {
+                  const int64_t wt = mm - WSHIFT[t];
+                  for ( int i = 0; i < 4; ++i )
+                  {
+                     int64_t a[3] = { x - VOF[t][i][0], y - VOF[t][i][1], z - VOF[t][i][2] };
+                     if ( a[0] < 0 || a[1] < 0 || a[2] < 0 || a[0] + a[1] + a[2] > wt - 1 )
+                        continue;
+                     for ( int j = 0; j < 4; ++j )
+                     {
+                        int64_t q = op->cmap[c * op->sc + lat( op->w, a[0] + VOF[t][j][0], a[1] + VOF[t][j][1], a[2] + VOF[t][j][2] )];
+                        row_add( &rows[p], q, K[t][i][j] );
+                     }
+                  }
+               }
+            }
+   }
+   op->brp = malloc( sizeof( int64_t ) * ( nb + 1 ) );
+   op->brp[0] = 0;
+   for ( int64_t p = 0; p < nb; ++p )
+      op->brp[p + 1] = op->brp[p] + rows[p].n;
+   op->bcol = malloc( sizeof( int64_t ) * ( op->brp[nb] + 1 ) );
+   op->bval = malloc( sizeof( double ) * ( op->brp[nb] + 1 ) );
+   op->bcls = malloc( op->brp[nb] + 1 );
+   for ( int64_t p = 0; p < nb; ++p )
+   {
+      int vp[4], np = owner_vertices( m, &op->l, p, vp );
+      for ( int64_t k = 0; k < rows[p].n; ++k )
+      {
+         int64_t q               = rows[p].col[k];
+         op->bcol[op->brp[p] + k] = q;
+         op->bval[op->brp[p] + k] = rows[p].val[k];
+         int vq[4], nq = owner_vertices( m, &op->l, q, vq );
+         int sub = 1;
+         for ( int a = 0; a < nq && sub; ++a )
+         {
+            int found = 0;
+            for ( int b = 0; b < np; ++b )
+               found |= ( vq[a] == vp[b] );
+            sub = found;
+         }
+         op->bcls[op->brp[p] + k] = (unsigned char) sub;
+      }
+      free( rows[p].col );
+      free( rows[p].val );
+   }
+   free( rows );
+   op->flags = malloc( sizeof( int32_t ) * op->l.n );
+   or_p1_flags( m, level, op->flags );
+   return op;
+}
+
+void or_p1op_free( or_p1op* op )
+{
+   if ( !op )
+      return;
+   free( op->cmap );
+   free( op->st );
+   free( op->brp );
+   free( op->bcol );
+   free( op->bval );
+   free( op->bcls );
+   free( op->flags );
+   free( op );
+}
+
+void or_p1op_cell_stencil( const or_p1op* op, int c, double* out15 ) { memcpy( out15, op->st + 15 * c, 15 * sizeof( double ) ); }
+
+int64_t or_p1op_nnz( const or_p1op* op ) { return op->brp[op->l.coff] + 15 * op->m->nc * op->l.cin; }
+
+void or_p1op_csr( const or_p1op* op, int64_t* rowptr, int64_t* cols, double* vals )
+{
+   const int64_t nb = op->l.coff, mm = op->l.mm, w = op->w;
+   int64_t       k  = 0;
+   rowptr[0]        = 0;
+   for ( int64_t p = 0; p < nb; ++p )
+   {
+      for ( int64_t e = op->brp[p]; e < op->brp[p + 1]; ++e )
+      {
+         cols[k] = op->bcol[e];
+         vals[k] = op->bval[e];
+         ++k;
+      }
+      rowptr[p + 1] = k;
+   }
+   int64_t row = nb;
+   for ( int c = 0; c < op->m->nc; ++c )
+      for ( int64_t z = 1; z < mm; ++z )
+         for ( int64_t y = 1; y + z < mm; ++y )
+            for ( int64_t x = 1; x + y + z < mm; ++x )
+            {
+               for ( int s = 0; s < 15; ++s )
+               {
+                  cols[k] = op->cmap[c * op->sc + lat( w, x + OFF15[s][0], y + OFF15[s][1], z + OFF15[s][2] )];
+                  vals[k] = op->st[15 * c + s];
+                  ++k;
+               }
+               rowptr[++row] = k;
+            }
+}
+
+/* gather a full cell lattice from canonical storage (Face->Cell etc. halo copy) */
+static void gather_cell( const or_p1op* op, int c, const double* src, double* lattice )
+{
+   const int64_t* cm = op->cmap + c * op->sc;
+   for ( int64_t i = 0; i < op->sc; ++i )
+      lattice[i] = src[cm[i]];
+}
+
+void or_p1_apply( const or_p1op* op, const double* src, double* dst, int flag, int add )
+{
+   const int64_t nb = op->l.coff, mm = op->l.mm, w = op->w;
+   for ( int64_t p = 0; p < nb; ++p )
+   {
+      if ( !( op->flags[p] & flag ) )
+         continue;
+      double s = 0.0;
+      for ( int64_t e = op->brp[p]; e < op->brp[p + 1]; ++e )
+         s += op->bval[e] * src[op->bcol[e]];
+      dst[p] = add ? dst[p] + s : s;
+   }
+   if ( !( flag & OR_INNER ) || op->l.cin == 0 )
+      return;
+#pragma omp parallel
+   {
+      double* lt = malloc( sizeof( double ) * op->sc );
+#pragma omp for schedule( static )
+      for ( int c = 0; c < op->m->nc; ++c )
+      {
+         gather_cell( op, c, src, lt );
+         const double* st   = op->st + 15 * c;
+         double*       out  = dst + op->l.coff + c * op->l.cin;
+         int64_t       o    = 0;
+         int64_t       base[15];
+         for ( int64_t z = 1; z < mm; ++z )
+            for ( int64_t y = 1; y + z < mm; ++y )
+            {
+               for ( int s = 0; s < 15; ++s )
+                  base[s] = lat( w, 0, y + OFF15[s][1], z + OFF15[s][2] ) + OFF15[s][0];
+               for ( int64_t x = 1; x + y + z < mm; ++x )
+               {
+                  double v = 0.0;
+                  for ( int s = 0; s < 15; ++s )
+                     v += st[s] * lt[base[s] + x];
+                  out[o] = add ? out[o] + v : v;
+                  ++o;
+               }
+            }
+      }
+      free( lt );
+   }
+}
+
+void or_p1_sor( const or_p1op* op, double* x, const double* b, int flag, double relax )
+{
+   const int64_t nb = op->l.coff, mm = op->l.mm, w = op->w;
+   /* values held in ghost layers of vertex/edge/face memory are those of the
+    * bidirectional sync at sweep start (P1Operator.hpp:373-379); only the
+    * primitive's own closure is refreshed in between (V->E, E->F, :398-412). */
+   double* xold = malloc( sizeof( double ) * ( nb > 0 ? nb : 1 ) );
+   memcpy( xold, x, sizeof( double ) * nb );
+   for ( int64_t p = 0; p < nb; ++p )
+   {
+      if ( !( op->flags[p] & flag ) )
+         continue;
+      double s = b[p], diag = 0.0;
+      for ( int64_t e = op->brp[p]; e < op->brp[p + 1]; ++e )
+      {
+         const int64_t q = op->bcol[e];
+         if ( q == p )
+         {
+            diag = op->bval[e];
+            continue;
+         }
+         const double xq = ( q < nb && !op->bcls[e] ) ? xold[q] : x[q];
+         s -= op->bval[e] * xq;
+      }
+      x[p] = ( 1.0 - relax ) * x[p] + relax * s / diag;
+   }
+   free( xold );
+   if ( !( flag & OR_INNER ) || op->l.cin == 0 )
+      return;
+   /* cells are independent of each other (no shared inner DoFs) */
+#pragma omp parallel
+   {
+      double* lt = malloc( sizeof( double ) * op->sc );
+#pragma omp for schedule( static )
+      for ( int c = 0; c < op->m->nc; ++c )
+      {
+         gather_cell( op, c, x, lt );
+         const double* st  = op->st + 15 * c;
+         const double* bc  = b + op->l.coff + c * op->l.cin;
+         const double  inv = 1.0 / st[0];
+         int64_t       o   = 0;
+         int64_t       base[15];
+         for ( int64_t z = 1; z < mm; ++z )
+            for ( int64_t y = 1; y + z < mm; ++y )
+            {
+               for ( int s = 0; s < 15; ++s )
+                  base[s] = lat( w, 0, y + OFF15[s][1], z + OFF15[s][2] ) + OFF15[s][0];
+               for ( int64_t xx = 1; xx + y + z < mm; ++xx )
+               {
+                  double v = bc[o];
+                  for ( int s = 1; s < 15; ++s )
+                     v -= st[s] * lt[base[s] + xx];
+                  lt[base[0] + xx] = ( 1.0 - relax ) * lt[base[0] + xx] + relax * inv * v;
+                  ++o;
+               }
+            }
+         /* write back inner points */
+         double* xc = x + op->l.coff + c * op->l.cin;
+         o          = 0;
+         for ( int64_t z = 1; z < mm; ++z )
+            for ( int64_t y = 1; y + z < mm; ++y )
+               for ( int64_t xx = 1; xx + y + z < mm; ++xx )
+                  xc[o++] = lt[lat( w, xx, y, z )];
+      }
+      free( lt );
+   }
+}
+
+/* ------------------------------------------------------------------------- */
+/* transfer                                                                   */
+/* ------------------------------------------------------------------------- */
+static int64_t* build_cmap( const or_mesh* m, int L, int64_t* sc_out )
+{
+   lvl_t         l  = lvl_info( m, L );
+   const int64_t w  = l.mm + 1;
+   const int64_t sc = tet( w );
+   int64_t*      cm = malloc( sizeof( int64_t ) * m->nc * sc );
+   for ( int c = 0; c < m->nc; ++c )
+      for ( int64_t z = 0; z <= l.mm; ++z )
+         for ( int64_t y = 0; y + z <= l.mm; ++y )
+            for ( int64_t x = 0; x + y + z <= l.mm; ++x )
+               cm[c * sc + lat( w, x, y, z )] = locate_l( m, &l, c, x, y, z );
+   *sc_out = sc;
+   return cm;
+}
+
+void or_p1_restrict( const or_mesh* m, int fineLevel, const double* fine, double* coarse, int flag )
+{
+   const int     Lc = fineLevel - 1;
+   lvl_t         lf = lvl_info( m, fineLevel ), lc = lvl_info( m, Lc );
+   int64_t       scf, scc;
+   int64_t*      cmf = build_cmap( m, fineLevel, &scf );
+   int64_t*      cmc = build_cmap( m, Lc, &scc );
+   const int64_t wf = lf.mm + 1, wc = lc.mm + 1;
+   double*       acc = calloc( lc.n, sizeof( double ) );
+   double*       invn = malloc( sizeof( double ) * scf );
+   for ( int c = 0; c < m->nc; ++c )
+   {
+      for ( int64_t i = 0; i < scf; ++i )
+         invn[i] = inv_ncells( m, &lf, cmf[c * scf + i] );
+      for ( int64_t z = 0; z <= lc.mm; ++z )
+         for ( int64_t y = 0; y + z <= lc.mm; ++y )
+            for ( int64_t x = 0; x + y + z <= lc.mm; ++x )
+            {
+               const int64_t fx = 2 * x, fy = 2 * y, fz = 2 * z;
+               const int64_t ic = lat( wf, fx, fy, fz );
+               double        a  = invn[ic] * fine[cmf[c * scf + ic]];
+               for ( int s = 1; s < 15; ++s )
+               {
+                  const int64_t qx = fx + OFF15[s][0], qy = fy + OFF15[s][1], qz = fz + OFF15[s][2];
+                  if ( qx < 0 || qy < 0 || qz < 0 || qx + qy + qz > lf.mm )
+                     continue;
+                  const int64_t iq = lat( wf, qx, qy, qz );
+                  a += 0.5 * invn[iq] * fine[cmf[c * scf + iq]];
+               }
+               acc[cmc[c * scc + lat( wc, x, y, z )]] += a;
+            }
+   }
+   for ( int64_t p = 0; p < lc.n; ++p )
+      if ( p >= lc.coff || ( owner_flag( m, &lc, p ) & flag ) )
+         coarse[p] = acc[p];
+   free( acc );
+   free( invn );
+   free( cmf );
+   free( cmc );
+}
+
+void or_p1_prolongate( const or_mesh* m, int coarseLevel, const double* coarse, double* fine, int flag, int add )
+{
+   const int     Lf = coarseLevel + 1;
+   lvl_t         lf = lvl_info( m, Lf ), lc = lvl_info( m, coarseLevel );
+   int64_t       scf, scc;
+   int64_t*      cmf = build_cmap( m, Lf, &scf );
+   int64_t*      cmc = build_cmap( m, coarseLevel, &scc );
+   const int64_t wf = lf.mm + 1, wc = lc.mm + 1;
+   double*       acc = calloc( lf.n, sizeof( double ) );
+   for ( int c = 0; c < m->nc; ++c )
+      for ( int64_t z = 0; z <= lc.mm; ++z )
+         for ( int64_t y = 0; y + z <= lc.mm; ++y )
+            for ( int64_t x = 0; x + y + z <= lc.mm; ++x )
+            {
+               const double  v  = coarse[cmc[c * scc + lat( wc, x, y, z )]];
+               const int64_t fx = 2 * x, fy = 2 * y, fz = 2 * z;
+               int64_t       id = cmf[c * scf + lat( wf, fx, fy, fz )];
+               acc[id] += inv_ncells( m, &lf, id ) * v;
+               for ( int s = 1; s < 15; ++s )
+               {
+                  const int64_t qx = fx + OFF15[s][0], qy = fy + OFF15[s][1], qz = fz + OFF15[s][2];
+                  if ( qx < 0 || qy < 0 || qz < 0 || qx + qy + qz > lf.mm )
+                     continue;
+                  id = cmf[c * scf + lat( wf, qx, qy, qz )];
+                  acc[id] += 0.5 * inv_ncells( m, &lf, id ) * v;
+               }
+            }
+   for ( int64_t p = 0; p < lf.n; ++p )
+      if ( p >= lf.coff || ( owner_flag( m, &lf, p ) & flag ) )
+         fine[p] = add ? fine[p] + acc[p] : acc[p];
+   free( acc );
+   free( cmf );
+   free( cmc );
+}
+
+/* ------------------------------------------------------------------------- */
+/* BLAS-1, CG, GMG                                                            */
+/* ------------------------------------------------------------------------- */
+double or_dot( int64_t n, const double* a, const double* b, const int32_t* flags, int flag )
+{
+   double s = 0.0;
+   for ( int64_t i = 0; i < n; ++i )
+      if ( flags[i] & flag )
+         s += a[i] * b[i];
+   return s;
+}
+
+static void axpby_f( int64_t n, double alpha, const double* x, double beta, double* y, const int32_t* flags, int flag )
+{
+   for ( int64_t i = 0; i < n; ++i )
+      if ( flags[i] & flag )
+         y[i] = alpha * x[i] + beta * y[i];
+}
+
+int or_p1_cg( const or_p1op* op, double* x, const double* b, int flag, int maxIter, double relTol, double absTol )
+{
+   const int64_t  n  = op->l.n;
+   const int32_t* fl = op->flags;
+   double*        p  = calloc( n, sizeof( double ) );
+   double*        r  = calloc( n, sizeof( double ) );
+   double*        ap = calloc( n, sizeof( double ) );
+   int            it = maxIter;
+   /* init (CGSolver.hpp:340-358); identity preconditioner => z == r */
+   or_p1_apply( op, x, p, flag, 0 );
+   for ( int64_t i = 0; i < n; ++i )
+      if ( fl[i] & flag )
+         r[i] = b[i] - p[i];
+   for ( int64_t i = 0; i < n; ++i )
+      if ( fl[i] & flag )
+         p[i] = r[i];
+   double prsold    = or_dot( n, r, r, fl, flag );
+   double res_start = sqrt( or_dot( n, r, r, fl, flag ) );
+   if ( res_start < absTol || maxIter == 0 )
+   {
+      free( p );
+      free( r );
+      free( ap );
+      return 0;
+   }
+   for ( int i = 0; i < maxIter; ++i )
+   {
+      or_p1_apply( op, p, ap, flag, 0 );
+      double pAp   = or_dot( n, p, ap, fl, flag );
+      double alpha = prsold / pAp;
+      axpby_f( n, alpha, p, 1.0, x, fl, flag );
+      axpby_f( n, -alpha, ap, 1.0, r, fl, flag );
+      double rsnew  = or_dot( n, r, r, fl, flag );
+      double relRes = sqrt( rsnew ) / res_start;
+      if ( relRes < relTol || sqrt( rsnew ) < absTol )
+      {
+         it = i;
+         break;
+      }
+      double prsnew = rsnew;
+      double beta   = prsnew / prsold;
+      axpby_f( n, 1.0, r, beta, p, fl, flag );
+      prsold = prsnew;
+   }
+   free( p );
+   free( r );
+   free( ap );
+   return it;
+}
+
+typedef struct
+{
+   or_p1op* op;
+   double * x, *b, *tmp;
+   int64_t  n;
+} glevel_t;
+
+static void gmg_rec( const or_mesh* m, glevel_t* lv, int minL, int L, int pre, int post, int cmaxit, double ctol )
+{
+   const int flag = OR_INNER | OR_NEUMANN | OR_FREESLIP;
+   glevel_t* c    = &lv[L];
+   if ( L == minL )
+   {
+      or_p1_cg( c->op, c->x, c->b, flag, cmaxit, ctol, ctol );
+      return;
+   }
+   for ( int i = 0; i < pre; ++i )
+      or_p1_sor( c->op, c->x, c->b, flag, 1.0 );
+   or_p1_apply( c->op, c->x, c->tmp, flag, 0 );
+   axpby_f( c->n, 1.0, c->b, -1.0, c->tmp, c->op->flags, flag );
+   glevel_t* d = &lv[L - 1];
+   or_p1_restrict( m, L, c->tmp, d->tmp, flag );
+   axpby_f( d->n, 1.0, d->tmp, 0.0, d->b, d->op->flags, flag );
+   memset( d->x, 0, sizeof( double ) * d->n );
+   gmg_rec( m, lv, minL, L - 1, pre, post, cmaxit, ctol );
+   or_p1_prolongate( m, L - 1, d->x, c->x, flag, 1 );
+   for ( int i = 0; i < post; ++i )
+      or_p1_sor( c->op, c->x, c->b, flag, 1.0 );
+}
+
+int or_p1_gmg( const or_mesh* m, int minLevel, int maxLevel, double* x, const double* b, int pre, int post, int cycles,
+               double tol, double* res_hist, int coarseMaxIter, double coarseTol )
+{
+   glevel_t* lv = calloc( maxLevel + 1, sizeof( glevel_t ) );
+   for ( int L = minLevel; L <= maxLevel; ++L )
+   {
+      lv[L].op  = or_p1op_create( m, L );
+      lv[L].n   = lv[L].op->l.n;
+      lv[L].x   = L == maxLevel ? x : calloc( lv[L].n, sizeof( double ) );
+      lv[L].b   = calloc( lv[L].n, sizeof( double ) );
+      lv[L].tmp = calloc( lv[L].n, sizeof( double ) );
+   }
+   glevel_t* f = &lv[maxLevel];
+   memcpy( f->b, b, sizeof( double ) * f->n );
+   double* r = calloc( f->n, sizeof( double ) );
+   /* residual r = b - A x on Inner DoFs */
+#define RESID()                                                                   \
+   ( or_p1_apply( f->op, f->x, r, OR_INNER, 0 ),                                  \
+     axpby_f( f->n, 1.0, f->b, -1.0, r, f->op->flags, OR_INNER ),                 \
+     sqrt( or_dot( f->n, r, r, f->op->flags, OR_INNER ) ) )
+   double r0 = RESID();
+   if ( res_hist )
+      res_hist[0] = r0;
+   int k = 0;
+   for ( ; k < cycles; ++k )
+   {
+      gmg_rec( m, lv, minLevel, maxLevel, pre, post, coarseMaxIter, coarseTol );
+      double rk = RESID();
+      if ( res_hist )
+         res_hist[k + 1] = rk;
+      if ( tol > 0 && rk <= tol * r0 )
+      {
+         ++k;
+         break;
+      }
+   }
+#undef RESID
+   free( r );
+   for ( int L = minLevel; L <= maxLevel; ++L )
+   {
+      or_p1op_free( lv[L].op );
+      if ( L != maxLevel )
+         free( lv[L].x );
+      free( lv[L].b );
+      free( lv[L].tmp );
+   }
+   free( lv );
+   return k;
+}

@@ -1,0 +1,172 @@
+"""ctypes binding of the CPU oracle (oracle/libhg_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker / CPU baseline, never as
+the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+INNER, DIRICHLET, NEUMANN, FREESLIP, ALL = 1, 2, 4, 8, 15
+GMG_FLAG = INNER | NEUMANN | FREESLIP
+
+_d = C.c_double
+_i = C.c_int
+_i64 = C.c_int64
+_p = C.c_void_p
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libhg_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        sig = {
+            "or_mesh_create": (_p, [_i, _p, _i, _p]),
+            "or_mesh_cuboid": (_p, [_d, _d, _d, _d, _d, _d, _i, _i, _i]),
+            "or_mesh_free": (None, [_p]),
+            "or_mesh_counts": (None, [_p, _p]),
+            "or_mesh_cell_vertices": (None, [_p, _p]),
+            "or_p1_ndofs": (_i64, [_p, _i]),
+            "or_p1_nbnd": (_i64, [_p, _i]),
+            "or_p1_coords": (None, [_p, _i, _p]),
+            "or_p1_flags": (None, [_p, _i, _p]),
+            "or_p1_locate": (_i64, [_p, _i, _i, _i, _i, _i]),
+            "or_p1op_create": (_p, [_p, _i]),
+            "or_p1op_free": (None, [_p]),
+            "or_p1op_nnz": (_i64, [_p]),
+            "or_p1op_csr": (None, [_p, _p, _p, _p]),
+            "or_p1op_cell_stencil": (None, [_p, _i, _p]),
+            "or_p1_apply": (None, [_p, _p, _p, _i, _i]),
+            "or_p1_sor": (None, [_p, _p, _p, _i, _d]),
+            "or_p1_restrict": (None, [_p, _i, _p, _p, _i]),
+            "or_p1_prolongate": (None, [_p, _i, _p, _p, _i, _i]),
+            "or_dot": (_d, [_i64, _p, _p, _p, _i]),
+            "or_p1_cg": (_i, [_p, _p, _p, _i, _i, _d, _d]),
+            "or_p1_gmg": (_i, [_p, _i, _i, _p, _p, _i, _i, _i, _d, _p, _i, _d]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Mesh:
+    def __init__(self, handle):
+        self.h = handle
+
+    @staticmethod
+    def cuboid(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(1, 1, 1)):
+        return Mesh(lib().or_mesh_cuboid(*lo, *hi, *n))
+
+    @staticmethod
+    def from_arrays(coords, cells):
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        cells = np.ascontiguousarray(cells, dtype=np.int64)
+        return Mesh(lib().or_mesh_create(len(coords), _ptr(coords), len(cells), _ptr(cells)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.or_mesh_free(self.h)
+            self.h = None
+
+    def counts(self):
+        out = np.zeros(4, dtype=np.int64)
+        lib().or_mesh_counts(self.h, _ptr(out))
+        return tuple(int(v) for v in out)
+
+    def cell_vertices(self):
+        nc = self.counts()[3]
+        out = np.zeros((nc, 4), dtype=np.int64)
+        lib().or_mesh_cell_vertices(self.h, _ptr(out))
+        return out
+
+    def ndofs(self, level):
+        return int(lib().or_p1_ndofs(self.h, level))
+
+    def nbnd(self, level):
+        return int(lib().or_p1_nbnd(self.h, level))
+
+    def coords(self, level):
+        out = np.zeros((self.ndofs(level), 3))
+        lib().or_p1_coords(self.h, level, _ptr(out))
+        return out
+
+    def flags(self, level):
+        out = np.zeros(self.ndofs(level), dtype=np.int32)
+        lib().or_p1_flags(self.h, level, _ptr(out))
+        return out
+
+    def locate(self, level, c, x, y, z):
+        return int(lib().or_p1_locate(self.h, level, c, x, y, z))
+
+    def restrict(self, fine_level, fine, coarse, flag):
+        lib().or_p1_restrict(self.h, fine_level, _ptr(fine), _ptr(coarse), flag)
+
+    def prolongate(self, coarse_level, coarse, fine, flag, add):
+        lib().or_p1_prolongate(self.h, coarse_level, _ptr(coarse), _ptr(fine), flag, int(add))
+
+    def gmg(self, min_level, max_level, x, b, pre=3, post=3, cycles=10, tol=0.0, coarse_maxit=10000,
+            coarse_tol=1e-16):
+        hist = np.zeros(cycles + 1)
+        k = lib().or_p1_gmg(self.h, min_level, max_level, _ptr(x), _ptr(b), pre, post, cycles, tol,
+                            _ptr(hist), coarse_maxit, coarse_tol)
+        return k, hist[: k + 1]
+
+
+class P1Op:
+    def __init__(self, mesh, level):
+        self.mesh = mesh
+        self.level = level
+        self.h = lib().or_p1op_create(mesh.h, level)
+        self.n = mesh.ndofs(level)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.or_p1op_free(self.h)
+            self.h = None
+
+    def apply(self, src, dst, flag, add=False):
+        lib().or_p1_apply(self.h, _ptr(src), _ptr(dst), flag, int(add))
+
+    def sor(self, x, b, flag, relax=1.0):
+        lib().or_p1_sor(self.h, _ptr(x), _ptr(b), flag, relax)
+
+    def cg(self, x, b, flag, maxit, rel, abs_):
+        return lib().or_p1_cg(self.h, _ptr(x), _ptr(b), flag, maxit, rel, abs_)
+
+    def cell_stencil(self, c):
+        out = np.zeros(15)
+        lib().or_p1op_cell_stencil(self.h, c, _ptr(out))
+        return out
+
+    def csr(self):
+        import scipy.sparse as sp
+        nnz = int(lib().or_p1op_nnz(self.h))
+        rp = np.zeros(self.n + 1, dtype=np.int64)
+        cols = np.zeros(nnz, dtype=np.int64)
+        vals = np.zeros(nnz)
+        lib().or_p1op_csr(self.h, _ptr(rp), _ptr(cols), _ptr(vals))
+        return sp.csr_matrix((vals, cols, rp), shape=(self.n, self.n))
+
+
+def dot(a, b, flags, flag):
+    return float(lib().or_dot(len(a), _ptr(a), _ptr(b), _ptr(flags), flag))
