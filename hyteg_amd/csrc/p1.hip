@@ -5,16 +5,18 @@
 //  * cell interiors: one structured 15-point stencil per macro-cell, swept row
 //    by row ((y,z) rows, x contiguous) -- apply_3D_macrocell_vertexdof_to_vertexdof
 //    (P1generatedKernels/apply_3D_macrocell_vertexdof_to_vertexdof_replace.cpp:34-79);
-//  * DoFs owned by macro-vertices/edges/faces: sparse rows over owner storage
-//    positions, assembled from the micro-cells of all neighbour cells, i.e. the
-//    macro-face/edge/vertex stencils of P1Operator (P1ConstantOperator.cpp:239-358,
-//    P1Operator.hpp:211-245, 1044-1108).
+//  * DoFs owned by macro-vertices/edges/faces: the sum over incident cells of
+//    each cell's share of the stencil (one partial stencil per boundary class
+//    of the cell), read from the cell lattices' halo copies -- the macro-face/
+//    edge/vertex stencils of P1Operator (P1ConstantOperator.cpp:239-358,
+//    P1Operator.hpp:211-245, 1044-1108) evaluated cell by cell.
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <map>
 
 #include "abi_util.hpp"
+#include "halo.hpp"
 #include "hyteg_amd.h"
 #include "runtime.hpp"
 
@@ -28,8 +30,7 @@ __constant__ int cOff15[15][3] = { { 0, 0, 0 },  { -1, 0, 0 }, { -1, 0, 1 }, { -
 __constant__ int cMidDir[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 1, -1, 0 },
                                    { 0, 0, 1 }, { 1, 0, -1 }, { 0, 1, -1 }, { 1, -1, 1 } };
 
-static const int kMidDirHost[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 1, -1, 0 },
-                                       { 0, 0, 1 }, { 1, 0, -1 }, { 0, 1, -1 }, { 1, -1, 1 } };
+
 
 // ------------------------------------------------------------------------
 // host: P1 element stiffness of a tetrahedron, gradients of the barycentric
@@ -97,74 +98,56 @@ static int slot15( int dx, int dy, int dz )
    throw Error( -4, "offset outside the 15-point stencil" );
 }
 
-static bool onCellBoundary( int64_t mm, int64_t x, int64_t y, int64_t z )
-{
-   return x == 0 || y == 0 || z == 0 || x + y + z == mm;
-}
 
-// sparse rows keyed by owner storage position
-struct RowBuilder
-{
-   std::vector< std::vector< std::pair< int64_t, double > > > rows;
-   explicit RowBuilder( int64_t n )
-   : rows( n )
-   {}
-   void add( int64_t r, int64_t col, double v )
-   {
-      for ( auto& e : rows[r] )
-         if ( e.first == col )
-         {
-            e.second += v;
-            return;
-         }
-      rows[r].emplace_back( col, v );
-   }
-   void set( int64_t r, int64_t col, double v )
-   {
-      for ( auto& e : rows[r] )
-         if ( e.first == col )
-         {
-            e.second = v;
-            return;
-         }
-      rows[r].emplace_back( col, v );
-   }
-};
+// Representative lattice points of the 14 boundary classes of a cell (local
+// faces 0..3, edges 4..9, vertices 10..13) in a virtual lattice of 8 micro-edges
+// per macro-edge: which micro-cells of a cell contain a boundary point depends
+// on the point's class only.
+static constexpr int kVirt            = 8;
+static const int     kClassRep[14][3] = { { 2, 2, 0 }, { 2, 0, 2 }, { 0, 2, 2 }, { 2, 2, 4 }, { 4, 0, 0 },
+                                          { 0, 4, 0 }, { 4, 4, 0 }, { 0, 0, 4 }, { 4, 0, 4 }, { 0, 4, 4 },
+                                          { 0, 0, 0 }, { 8, 0, 0 }, { 0, 8, 0 }, { 0, 0, 8 } };
 
-struct DevRows
+static int hostPointClass( int64_t mm, int64_t x, int64_t y, int64_t z )
 {
-   int64_t           nrows = 0;
-   DevBuf< int64_t > ptr, col;
-   DevBuf< double >  val;
-   void              build( Context* ctx, RowBuilder& rb )
-   {
-      nrows = int64_t( rb.rows.size() );
-      std::vector< int64_t > p( nrows + 1, 0 ), c;
-      std::vector< double >  v;
-      for ( int64_t r = 0; r < nrows; ++r )
+   if ( x < 0 || y < 0 || z < 0 || x + y + z > mm )
+      return -2;
+   const int64_t t[4] = { mm - x - y - z, x, y, z };
+   int           zmask = 0, nz = 0;
+   for ( int k = 0; k < 4; ++k )
+      if ( t[k] == 0 )
       {
-         std::sort( rb.rows[r].begin(), rb.rows[r].end() );
-         for ( auto& e : rb.rows[r] )
-         {
-            c.push_back( e.first );
-            v.push_back( e.second );
-         }
-         p[r + 1] = int64_t( c.size() );
+         zmask |= 1 << k;
+         ++nz;
       }
-      ptr.upload( ctx, p );
-      col.upload( ctx, c );
-      val.upload( ctx, v );
+   if ( nz == 0 )
+      return -1;
+   const int keep = ( ~zmask ) & 15;
+   if ( nz == 1 )
+      for ( int k = 0; k < 4; ++k )
+         if ( zmask == ( 1 << ( 3 - k ) ) )
+            return k;
+   if ( nz == 2 )
+   {
+      const int pairs[6] = { 3, 5, 6, 9, 10, 12 };
+      for ( int k = 0; k < 6; ++k )
+         if ( keep == pairs[k] )
+            return 4 + k;
    }
-};
+   for ( int k = 0; k < 4; ++k )
+      if ( keep == ( 1 << k ) )
+         return 10 + k;
+   return -1;
+}
 
 struct P1OpLevel
 {
-   std::vector< double > hStencil; // nc*15
-   DevBuf< double >      stencil;
-   DevRows               bnd;      // rows of owned vertex/edge/face DoFs
-   DevBuf< double >      invDiag;
-   std::vector< int64_t > colorStart; // rows grouped by colour
-   DevBuf< int32_t >     colorRows;
+   std::vector< double >  hStencil; // nc*15, cell interiors
+   std::vector< double >  hCls;     // nc*14*15: the cell's share of the stencil at its boundary classes
+   DevBuf< double >       stencil, clsStencil;
+   DevBuf< double >       invDiag;    // per owned vertex/edge/face DoF
+   std::vector< int64_t > colorStart; // owned boundary DoFs grouped by colour
+   DevBuf< int32_t >      colorRows;
 };
 
 struct P1Operator : OperatorBase
@@ -180,7 +163,7 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
    Lattice&        lat = st.lattice( ctx, L );
    const int64_t   mm  = lat.mm;
    o.hStencil.assign( 15 * size_t( t.nc ), 0.0 );
-   RowBuilder rb( lat.nb );
+   o.hCls.assign( 14 * 15 * size_t( t.nc ), 0.0 );
    for ( int c = 0; c < t.nc; ++c )
    {
       double K[6][4][4];
@@ -194,196 +177,129 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
                const int* b = MicroCells::vof[ty][j];
                o.hStencil[15 * c + slot15( b[0] - a[0], b[1] - a[1], b[2] - a[2] )] += K[ty][i][j];
             }
-      // rows of the owned DoFs on this cell's boundary: this cell's share
+      // boundary classes: only the micro-cells inside this cell (the per-cell
+      // parts of the macro-face/edge/vertex stencils, P1ConstantOperator.cpp:680-730)
+      for ( int cls = 0; cls < 14; ++cls )
+      {
+         const int* p = kClassRep[cls];
+         for ( int ty = 0; ty < 6; ++ty )
+         {
+            const int wt = kVirt - MicroCells::rowShift[ty];
+            for ( int i = 0; i < 4; ++i )
+            {
+               const int* vi = MicroCells::vof[ty][i];
+               const int  ax = p[0] - vi[0], ay = p[1] - vi[1], az = p[2] - vi[2];
+               if ( ax < 0 || ay < 0 || az < 0 || ax + ay + az > wt - 1 )
+                  continue;
+               for ( int j = 0; j < 4; ++j )
+               {
+                  const int* vj = MicroCells::vof[ty][j];
+                  o.hCls[( 14 * size_t( c ) + cls ) * 15 + slot15( vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2] )] +=
+                      K[ty][i][j];
+               }
+            }
+         }
+      }
+   }
+   o.stencil.upload( ctx, o.hStencil );
+   o.clsStencil.upload( ctx, o.hCls );
+
+   // diagonal and coupling graph of the owned boundary DoFs
+   std::vector< double >                 diag( lat.nb, 0.0 );
+   std::vector< std::vector< int32_t > > adj( lat.nb );
+   for ( int c = 0; c < t.nc; ++c )
       for ( int64_t z = 0; z <= mm; ++z )
          for ( int64_t y = 0; y + z <= mm; ++y )
             for ( int64_t x = 0; x + y + z <= mm; ++x )
             {
-               if ( !onCellBoundary( mm, x, y, z ) )
+               const int cls = hostPointClass( mm, x, y, z );
+               if ( cls < 0 )
                   continue;
                const int64_t r = lat.num.locate( t, c, x, y, z );
-               for ( int ty = 0; ty < 6; ++ty )
+               const double* W = &o.hCls[( 14 * size_t( c ) + cls ) * 15];
+               diag[r] += W[0];
+               for ( int s = 1; s < 15; ++s )
                {
-                  const int64_t wt = mm - MicroCells::rowShift[ty];
-                  for ( int i = 0; i < 4; ++i )
+                  const int64_t qx = x + Stencil15::off[s][0], qy = y + Stencil15::off[s][1], qz = z + Stencil15::off[s][2];
+                  const int     qc = hostPointClass( mm, qx, qy, qz );
+                  if ( qc == -2 )
                   {
-                     const int*    vi = MicroCells::vof[ty][i];
-                     const int64_t ax = x - vi[0], ay = y - vi[1], az = z - vi[2];
-                     if ( ax < 0 || ay < 0 || az < 0 || ax + ay + az > wt - 1 )
-                        continue;
-                     for ( int j = 0; j < 4; ++j )
-                     {
-                        const int* vj = MicroCells::vof[ty][j];
-                        rb.add( r, lat.ownerPos( t, c, ax + vj[0], ay + vj[1], az + vj[2] ), K[ty][i][j] );
-                     }
+                     if ( W[s] != 0.0 )
+                        throw Error( -4, "boundary stencil reaches outside its cell" );
+                     continue;
                   }
+                  if ( W[s] == 0.0 || qc < 0 )
+                     continue;
+                  adj[r].push_back( int32_t( lat.num.locate( t, c, qx, qy, qz ) ) );
                }
             }
-   }
-   o.stencil.upload( ctx, o.hStencil );
-
-   // inverse diagonals and a greedy colouring of the owned-boundary rows
-   const int64_t           ncell = lat.nc * lat.sc;
-   std::vector< double >   inv( lat.nb, 0.0 );
-   std::vector< int >      color( lat.nb, -1 );
-   int                     ncolor = 0;
+   std::vector< double > inv( lat.nb );
+   for ( int64_t r = 0; r < lat.nb; ++r )
+      inv[r] = 1.0 / diag[r];
+   o.invDiag.upload( ctx, inv );
+   // greedy colouring in canonical order (vertices, edges, faces)
+   std::vector< int >  color( lat.nb, -1 );
+   int                 ncolor = 0;
+   std::vector< char > used;
    for ( int64_t r = 0; r < lat.nb; ++r )
    {
-      std::vector< char > used( 64, 0 );
-      for ( auto& e : rb.rows[r] )
-      {
-         if ( e.first == ncell + r )
-            inv[r] = 1.0 / e.second;
-         else if ( e.first >= ncell && color[e.first - ncell] >= 0 )
-         {
-            const int cc = color[e.first - ncell];
-            if ( cc >= int( used.size() ) )
-               used.resize( cc + 1, 0 );
-            used[cc] = 1;
-         }
-      }
+      used.assign( ncolor + 1, 0 );
+      for ( int32_t q : adj[r] )
+         if ( color[q] >= 0 )
+            used[color[q]] = 1;
       int cc = 0;
-      while ( cc < int( used.size() ) && used[cc] )
+      while ( used[cc] )
          ++cc;
       color[r] = cc;
       ncolor   = std::max( ncolor, cc + 1 );
    }
    o.colorStart.assign( ncolor + 1, 0 );
-   std::vector< int32_t > rowsByColor;
+   std::vector< std::vector< int32_t > > byColor( ncolor );
+   for ( int64_t r = 0; r < lat.nb; ++r )
+      byColor[color[r]].push_back( int32_t( r ) );
+   std::vector< int32_t > rows;
    for ( int cc = 0; cc < ncolor; ++cc )
    {
-      o.colorStart[cc] = int64_t( rowsByColor.size() );
-      for ( int64_t r = 0; r < lat.nb; ++r )
-         if ( color[r] == cc )
-            rowsByColor.push_back( int32_t( r ) );
+      o.colorStart[cc] = int64_t( rows.size() );
+      rows.insert( rows.end(), byColor[cc].begin(), byColor[cc].end() );
    }
-   o.colorStart[ncolor] = int64_t( rowsByColor.size() );
-   o.colorRows.upload( ctx, rowsByColor );
-   o.invDiag.upload( ctx, inv );
-   o.bnd.build( ctx, rb );
-}
-
-// ------------------------------------------------------------------------
-// transfer set-up (per storage, per fine level)
-// ------------------------------------------------------------------------
-struct P1Transfer : OperatorBase
-{
-   DevRows restrictRows; // coarse owned rows <- fine storage
-   DevRows prolongRows;  // fine owned rows   <- coarse storage
-};
-
-static P1Transfer& transferFor( Context* ctx, Storage& st, int fineL )
-{
-   auto it = st.transfers.find( fineL );
-   if ( it != st.transfers.end() )
-      return static_cast< P1Transfer& >( *it->second );
-   const Topology& t  = st.topo;
-   Lattice&        lf = st.lattice( ctx, fineL );
-   Lattice&        lc = st.lattice( ctx, fineL - 1 );
-   RowBuilder      rr( lc.nb ), pr( lf.nb );
-   for ( int c = 0; c < t.nc; ++c )
-   {
-      // R = P^T on coarse DoFs owned by vertices/edges/faces: centre weight 1,
-      // the 14 fine neighbours in the cell closure weight 1/2, every
-      // (coarse, fine) pair counted once over all cells -- what the additive
-      // communication of restrict_3D_macrocell_P1_pull_additive yields
-      for ( int64_t z = 0; z <= lc.mm; ++z )
-         for ( int64_t y = 0; y + z <= lc.mm; ++y )
-            for ( int64_t x = 0; x + y + z <= lc.mm; ++x )
-            {
-               if ( !onCellBoundary( lc.mm, x, y, z ) )
-                  continue;
-               const int64_t r = lc.num.locate( t, c, x, y, z );
-               rr.set( r, lf.ownerPos( t, c, 2 * x, 2 * y, 2 * z ), 1.0 );
-               for ( int s = 1; s < 15; ++s )
-               {
-                  const int64_t qx = 2 * x + Stencil15::off[s][0], qy = 2 * y + Stencil15::off[s][1],
-                                qz = 2 * z + Stencil15::off[s][2];
-                  if ( qx < 0 || qy < 0 || qz < 0 || qx + qy + qz > lf.mm )
-                     continue;
-                  rr.set( r, lf.ownerPos( t, c, qx, qy, qz ), 0.5 );
-               }
-            }
-      // linear interpolation onto fine DoFs owned by vertices/edges/faces
-      for ( int64_t z = 0; z <= lf.mm; ++z )
-         for ( int64_t y = 0; y + z <= lf.mm; ++y )
-            for ( int64_t x = 0; x + y + z <= lf.mm; ++x )
-            {
-               if ( !onCellBoundary( lf.mm, x, y, z ) )
-                  continue;
-               const int64_t r = lf.num.locate( t, c, x, y, z );
-               const int     p = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
-               if ( p == 0 )
-               {
-                  pr.set( r, lc.ownerPos( t, c, x / 2, y / 2, z / 2 ), 1.0 );
-                  continue;
-               }
-               const int* d = kMidDirHost[p];
-               pr.set( r, lc.ownerPos( t, c, ( x - d[0] ) / 2, ( y - d[1] ) / 2, ( z - d[2] ) / 2 ), 0.5 );
-               pr.set( r, lc.ownerPos( t, c, ( x + d[0] ) / 2, ( y + d[1] ) / 2, ( z + d[2] ) / 2 ), 0.5 );
-            }
-   }
-   auto tr = std::make_unique< P1Transfer >();
-   tr->restrictRows.build( ctx, rr );
-   tr->prolongRows.build( ctx, pr );
-   auto& ref            = *tr;
-   st.transfers[fineL] = std::move( tr );
-   return ref;
+   o.colorStart[ncolor] = int64_t( rows.size() );
+   o.colorRows.upload( ctx, rows );
 }
 
 // ------------------------------------------------------------------------
 // kernels
 // ------------------------------------------------------------------------
-// y[r] (=|+=) sum_k val[k] * x[col[k]] on rows whose owner type matches flag
-template < class T >
-__global__ void kRowsApply( int64_t nrows, const int64_t* __restrict__ ptr, const int64_t* __restrict__ col,
-                            const double* __restrict__ val, const uint8_t* __restrict__ rowFlag, int flag,
-                            const T* __restrict__ x, T* __restrict__ y, int add )
+// Block b of a 1D launch of 8*ceil(n/8) blocks -> logical block such that each
+// XCD (blocks are dispatched round-robin over the 8 XCDs) sweeps one contiguous
+// range of rows and keeps the neighbour rows it shares in its own L2.
+__device__ __forceinline__ int64_t xcdBlock( int64_t n )
 {
-   using A         = typename Acc< T >::type;
-   const int64_t r = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   if ( r >= nrows || !( rowFlag[r] & flag ) )
-      return;
-   A s = A( 0 );
-   for ( int64_t k = ptr[r]; k < ptr[r + 1]; ++k )
-      s += A( val[k] ) * toAcc( x[col[k]] );
-   if ( add )
-      s += toAcc( y[r] );
-   y[r] = fromAcc< T >( s );
+   const int64_t per = ( n + 7 ) / 8;
+   return int64_t( blockIdx.x % 8 ) * per + int64_t( blockIdx.x / 8 );
+}
+static unsigned xcdGrid( int64_t n )
+{
+   const int64_t g = 8 * ( ( n + 7 ) / 8 );
+   if ( g > 2147483647LL )
+      throw Error( -1, "grid too large" );
+   return unsigned( g );
 }
 
-// SOR update of the rows of one colour (owner positions ncell + r)
+// 15-point stencil on cell interiors; one block per (cell, interior row)
 template < class T >
-__global__ void kRowsSOR( int64_t n, const int32_t* __restrict__ rows, const int64_t* __restrict__ ptr,
-                          const int64_t* __restrict__ col, const double* __restrict__ val,
-                          const double* __restrict__ invDiag, const uint8_t* __restrict__ rowFlag, int flag,
-                          int64_t ncell, double relax, const T* __restrict__ b, T* __restrict__ x )
+__global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
+                            int64_t mm, const double* __restrict__ stencil, const T* __restrict__ src,
+                            T* __restrict__ dst, int add )
 {
-   using A         = typename Acc< T >::type;
-   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   if ( i >= n )
+   using A             = typename Acc< T >::type;
+   const int64_t lb    = xcdBlock( nrows * nc );
+   if ( lb >= nrows * nc )
       return;
-   const int64_t r = rows[i];
-   if ( !( rowFlag[r] & flag ) )
-      return;
-   const int64_t self = ncell + r;
-   A             s    = toAcc( b[self] );
-   for ( int64_t k = ptr[r]; k < ptr[r + 1]; ++k )
-      if ( col[k] != self )
-         s -= A( val[k] ) * toAcc( x[col[k]] );
-   const A xo = toAcc( x[self] );
-   x[self]    = fromAcc< T >( ( A( 1 ) - A( relax ) ) * xo + A( relax ) * A( invDiag[r] ) * s );
-}
-
-// 15-point stencil on cell interiors: one block per (interior row, cell)
-template < class T >
-__global__ void kCellApply( const int32_t* __restrict__ rows, int64_t sc, int64_t w, int64_t mm,
-                            const double* __restrict__ stencil, const T* __restrict__ src, T* __restrict__ dst, int add )
-{
-   using A            = typename Acc< T >::type;
-   const int64_t y    = rows[2 * blockIdx.x];
-   const int64_t z    = rows[2 * blockIdx.x + 1];
-   const int64_t c    = blockIdx.y;
+   const int64_t c    = lb / nrows;
+   const int64_t r    = lb - c * nrows;
+   const int64_t y    = rows[2 * r];
+   const int64_t z    = rows[2 * r + 1];
    const int64_t base = c * sc;
    A             st[15];
    int64_t       rb[15];
@@ -406,16 +322,20 @@ __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t sc, int64_
    }
 }
 
-// SOR on the cell-interior points of parity colour (cx, row group) in place
+// SOR on the cell-interior points of one parity colour, in place
 template < class T >
-__global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t sc, int64_t w, int64_t mm,
-                          const double* __restrict__ stencil, int cx, double relax, const T* __restrict__ b,
-                          T* __restrict__ x )
+__global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
+                          int64_t mm, const double* __restrict__ stencil, int cx, double relax,
+                          const T* __restrict__ b, T* __restrict__ x )
 {
-   using A            = typename Acc< T >::type;
-   const int64_t y    = rows[2 * blockIdx.x];
-   const int64_t z    = rows[2 * blockIdx.x + 1];
-   const int64_t c    = blockIdx.y;
+   using A          = typename Acc< T >::type;
+   const int64_t lb = xcdBlock( nrows * nc );
+   if ( lb >= nrows * nc )
+      return;
+   const int64_t c    = lb / nrows;
+   const int64_t r    = lb - c * nrows;
+   const int64_t y    = rows[2 * r];
+   const int64_t z    = rows[2 * r + 1];
    const int64_t base = c * sc;
    A             st[15];
    int64_t       rb[15];
@@ -438,15 +358,89 @@ __global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t sc, int64_t 
    }
 }
 
-// restriction to coarse cell-interior points (full weighting, R = P^T)
+// partial stencil sum of this DoF's share in cell c; skip = 1 leaves out the centre
 template < class T >
-__global__ void kRestrictCell( const int32_t* __restrict__ rows, int64_t scc, int64_t wc, int64_t mmc, int64_t scf,
-                               int64_t wf, const T* __restrict__ fine, T* __restrict__ coarse )
+__device__ __forceinline__ typename Acc< T >::type bndStencilSum( const BndGeom& g, const double* __restrict__ cls,
+                                                                  const T* __restrict__ v, int64_t c, int64_t x,
+                                                                  int64_t y, int64_t z, int k, int skip )
+{
+   using A           = typename Acc< T >::type;
+   const double* W   = cls + ( 14 * c + k ) * 15;
+   A             s   = A( 0 );
+   const int64_t base = c * g.sc;
+   for ( int q = skip; q < 15; ++q )
+   {
+      const double wq = W[q];
+      if ( wq == 0.0 )
+         continue;
+      const int64_t nx = x + cOff15[q][0], ny = y + cOff15[q][1], nz = z + cOff15[q][2];
+      if ( nx < 0 || ny < 0 || nz < 0 || nx + ny + nz > g.mm )
+         continue;
+      s += A( wq ) * toAcc( v[base + tetIndex( g.w, nx, ny, nz )] );
+   }
+   return s;
+}
+
+// apply on DoFs owned by macro-vertices/edges/faces: sum of every incident
+// cell's share, read from the cell lattices (halos refreshed before)
+template < class T >
+__global__ void kBndApply( BndGeom g, int64_t nb, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
+                           const double* __restrict__ cls, const T* __restrict__ src, T* __restrict__ dst, int add )
 {
    using A         = typename Acc< T >::type;
-   const int64_t y = rows[2 * blockIdx.x];
-   const int64_t z = rows[2 * blockIdx.x + 1];
-   const int64_t c = blockIdx.y;
+   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nb || !( bndFlag[b] & flag ) )
+      return;
+   A s = A( 0 );
+   forEachIncident( g, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int k ) {
+      s += bndStencilSum( g, cls, src, c, x, y, z, k, 0 );
+   } );
+   const int64_t o = nc * g.sc + b;
+   if ( add )
+      s += toAcc( dst[o] );
+   dst[o] = fromAcc< T >( s );
+}
+
+// SOR on the owned boundary DoFs of one colour; the new value is written to
+// the owner and to every halo copy, so later colours read it
+template < class T >
+__global__ void kBndSOR( BndGeom g, int64_t n, const int32_t* __restrict__ list, int64_t nc,
+                         const uint8_t* __restrict__ bndFlag, int flag, const double* __restrict__ cls,
+                         const double* __restrict__ invDiag, double relax, const T* __restrict__ rhs,
+                         T* __restrict__ x )
+{
+   using A         = typename Acc< T >::type;
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( i >= n )
+      return;
+   const int64_t b = list[i];
+   if ( !( bndFlag[b] & flag ) )
+      return;
+   const int64_t o = nc * g.sc + b;
+   A             s = toAcc( rhs[o] );
+   forEachIncident( g, b, [&]( int64_t c, int64_t xx, int64_t yy, int64_t zz, int k ) {
+      s -= bndStencilSum( g, cls, x, c, xx, yy, zz, k, 1 );
+   } );
+   const T nv = fromAcc< T >( ( A( 1 ) - A( relax ) ) * toAcc( x[o] ) + A( relax ) * A( invDiag[b] ) * s );
+   x[o]       = nv;
+   forEachIncident( g, b, [&]( int64_t c, int64_t xx, int64_t yy, int64_t zz, int ) {
+      x[c * g.sc + tetIndex( g.w, xx, yy, zz )] = nv;
+   } );
+}
+
+// restriction to coarse cell-interior points (full weighting, R = P^T)
+template < class T >
+__global__ void kRestrictCell( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t scc, int64_t wc,
+                               int64_t mmc, int64_t scf, int64_t wf, const T* __restrict__ fine, T* __restrict__ coarse )
+{
+   using A          = typename Acc< T >::type;
+   const int64_t lb = xcdBlock( nrows * nc );
+   if ( lb >= nrows * nc )
+      return;
+   const int64_t c = lb / nrows;
+   const int64_t r = lb - c * nrows;
+   const int64_t y = rows[2 * r];
+   const int64_t z = rows[2 * r + 1];
    int64_t       rb[15];
 #pragma unroll
    for ( int s = 0; s < 15; ++s )
@@ -458,20 +452,52 @@ __global__ void kRestrictCell( const int32_t* __restrict__ rows, int64_t scc, in
 #pragma unroll
       for ( int s = 1; s < 15; ++s )
          v += toAcc( fine[rb[s] + 2 * x] );
-      v = toAcc( fine[rb[0] + 2 * x] ) + A( 0.5 ) * v;
+      v               = toAcc( fine[rb[0] + 2 * x] ) + A( 0.5 ) * v;
       coarse[out + x] = fromAcc< T >( v );
    }
 }
 
+// restriction to coarse DoFs owned by vertices/edges/faces: every incident
+// cell adds its fine points, each scaled by 1/(cells sharing the fine point's
+// primitive), i.e. restrict_3D_macrocell_P1_pull_additive + additive communication
+template < class T >
+__global__ void kRestrictBnd( BndGeom gc, int64_t nbc, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
+                              const double* __restrict__ clsInvN, int64_t scf, int64_t wf, int64_t mmf,
+                              const T* __restrict__ fine, T* __restrict__ coarse )
+{
+   using A         = typename Acc< T >::type;
+   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nbc || !( bndFlag[b] & flag ) )
+      return;
+   A s = A( 0 );
+   forEachIncident( gc, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
+      for ( int q = 0; q < 15; ++q )
+      {
+         const int64_t fx = 2 * x + cOff15[q][0], fy = 2 * y + cOff15[q][1], fz = 2 * z + cOff15[q][2];
+         if ( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf )
+            continue;
+         const int cl  = pointClass( mmf, fx, fy, fz );
+         const A   wgt = ( q == 0 ? A( 1 ) : A( 0.5 ) ) * ( cl < 0 ? A( 1 ) : A( clsInvN[14 * c + cl] ) );
+         s += wgt * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
+      }
+   } );
+   coarse[nc * gc.sc + b] = fromAcc< T >( s );
+}
+
 // linear interpolation to fine cell-interior points
 template < class T >
-__global__ void kProlongCell( const int32_t* __restrict__ rows, int64_t scf, int64_t wf, int64_t mmf, int64_t scc,
-                              int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine, int add )
+__global__ void kProlongCell( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t scf, int64_t wf,
+                              int64_t mmf, int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine,
+                              int add )
 {
-   using A           = typename Acc< T >::type;
-   const int64_t y   = rows[2 * blockIdx.x];
-   const int64_t z   = rows[2 * blockIdx.x + 1];
-   const int64_t c   = blockIdx.y;
+   using A          = typename Acc< T >::type;
+   const int64_t lb = xcdBlock( nrows * nc );
+   if ( lb >= nrows * nc )
+      return;
+   const int64_t c   = lb / nrows;
+   const int64_t r   = lb - c * nrows;
+   const int64_t y   = rows[2 * r];
+   const int64_t z   = rows[2 * r + 1];
    const int64_t out = c * scf + tetRowStart( wf, y, z );
    for ( int64_t x = 1 + threadIdx.x; x + y + z < mmf; x += blockDim.x )
    {
@@ -491,23 +517,42 @@ __global__ void kProlongCell( const int32_t* __restrict__ rows, int64_t scf, int
    }
 }
 
+// linear interpolation to fine DoFs owned by vertices/edges/faces, read from
+// the coarse lattice of the first incident cell (halos refreshed before)
+template < class T >
+__global__ void kProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
+                             int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine, int add )
+{
+   using A         = typename Acc< T >::type;
+   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nbf || !( bndFlag[b] & flag ) )
+      return;
+   bool done = false;
+   A    v    = A( 0 );
+   forEachIncident( gf, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
+      if ( done )
+         return;
+      done        = true;
+      const int p = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
+      if ( p == 0 )
+         v = toAcc( coarse[c * scc + tetIndex( wc, x / 2, y / 2, z / 2 )] );
+      else
+      {
+         const int dx = cMidDir[p][0], dy = cMidDir[p][1], dz = cMidDir[p][2];
+         v = A( 0.5 ) * ( toAcc( coarse[c * scc + tetIndex( wc, ( x - dx ) / 2, ( y - dy ) / 2, ( z - dz ) / 2 )] ) +
+                          toAcc( coarse[c * scc + tetIndex( wc, ( x + dx ) / 2, ( y + dy ) / 2, ( z + dz ) / 2 )] ) );
+      }
+   } );
+   const int64_t o = nc * gf.sc + b;
+   if ( add )
+      v += toAcc( fine[o] );
+   fine[o] = fromAcc< T >( v );
+}
+
 // ------------------------------------------------------------------------
 // host drivers
 // ------------------------------------------------------------------------
 static constexpr int kRowBlock = 64;
-
-static void rowsApply( Context* ctx, const DevRows& R, const uint8_t* rowFlag, int flag, const void* x, void* y,
-                       int dtype, int add )
-{
-   if ( R.nrows == 0 )
-      return;
-   dispatchType( dtype, [&]( auto tag ) {
-      using T = decltype( tag );
-      kRowsApply< T ><<< gridFor( R.nrows ), kBlock, 0, ctx->stream >>>( R.nrows, R.ptr.p, R.col.p, R.val.p, rowFlag,
-                                                                        flag, (const T*) x, (T*) y, add );
-   } );
-   HG_CHECK_HIP( hipGetLastError() );
-}
 
 static P1OpLevel& levelOf( Context* ctx, P1Operator& op, int L )
 {
@@ -533,64 +578,66 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
    Lattice&   lat = op.st->lattice( ctx, L );
    void*      s   = src.ptr( L );
    void*      d   = dst.ptr( L );
-   ctx->beginTiming();
    pullHalos( ctx, *op.st, lat, s, src.dtype );
-   const int64_t ncell = lat.nc * lat.sc;
    dispatchType( src.dtype, [&]( auto tag ) {
       using T = decltype( tag );
-      rowsApply( ctx, o.bnd, lat.bndFlag.p, flag, s, (T*) d + ncell, src.dtype, update );
-      if ( ( flag & kInner ) && lat.nrows > 0 )
+      if ( lat.nb > 0 )
       {
-         dim3 grid( unsigned( lat.nrows ), unsigned( lat.nc ) );
-         kCellApply< T ><<< grid, kRowBlock, 0, ctx->stream >>>( lat.rows.p, lat.sc, lat.w, lat.mm, o.stencil.p,
-                                                               (const T*) s, (T*) d, update );
+         kBndApply< T ><<< gridFor( lat.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( *op.st, lat ), lat.nb, lat.nc,
+                                                                         lat.bndFlag.p, flag, o.clsStencil.p,
+                                                                         (const T*) s, (T*) d, update );
          HG_CHECK_HIP( hipGetLastError() );
       }
+      if ( ( flag & kInner ) && lat.nrows > 0 )
+      {
+         ctx->beginTiming();
+         kCellApply< T ><<< xcdGrid( lat.nrows * lat.nc ), kRowBlock, 0, ctx->stream >>>(
+             lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update );
+         HG_CHECK_HIP( hipGetLastError() );
+         ctx->endTiming();
+      }
    } );
-   ctx->endTiming();
 }
 
 void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax, int L, int flag )
 {
    if ( x.st != op.st || b.st != op.st || x.dtype != b.dtype || &x == &b )
       throw Error( -1, "smooth_sor needs distinct x/b on the operator's storage with one value type" );
-   P1OpLevel&    o     = levelOf( ctx, op, L );
-   Lattice&      lat   = op.st->lattice( ctx, L );
-   const int64_t ncell = lat.nc * lat.sc;
-   void*         xp    = x.ptr( L );
-   const void*   bp    = b.ptr( L );
+   P1OpLevel& o   = levelOf( ctx, op, L );
+   Lattice&   lat = op.st->lattice( ctx, L );
+   void*      xp  = x.ptr( L );
+   const void* bp = b.ptr( L );
    ctx->beginTiming();
+   pullHalos( ctx, *op.st, lat, xp, x.dtype );
    dispatchType( x.dtype, [&]( auto tag ) {
       using T = decltype( tag );
-      // owned vertex/edge/face DoFs, colour by colour (rows read owner values and
-      // cell interiors directly)
+      // owned vertex/edge/face DoFs colour by colour (each update also refreshes
+      // its halo copies), then the 8 parity colours of the cell interiors: every
+      // non-centre offset of the 15-point stencil has an odd component
+      const BndGeom g = BndGeom::of( *op.st, lat );
       for ( size_t cc = 0; cc + 1 < o.colorStart.size(); ++cc )
       {
          const int64_t n = o.colorStart[cc + 1] - o.colorStart[cc];
          if ( n == 0 )
             continue;
-         kRowsSOR< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>(
-             n, o.colorRows.p + o.colorStart[cc], o.bnd.ptr.p, o.bnd.col.p, o.bnd.val.p, o.invDiag.p, lat.bndFlag.p,
-             flag, ncell, relax, (const T*) bp, (T*) xp );
+         kBndSOR< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( g, n, o.colorRows.p + o.colorStart[cc], lat.nc,
+                                                                  lat.bndFlag.p, flag, o.clsStencil.p, o.invDiag.p,
+                                                                  relax, (const T*) bp, (T*) xp );
          HG_CHECK_HIP( hipGetLastError() );
       }
       if ( !( flag & kInner ) || lat.nrows == 0 )
          return;
-      pullHalos( ctx, *op.st, lat, xp, x.dtype );
-      // cell interiors: 8 parity colours (every non-centre stencil offset has an
-      // odd component, so points of one colour never couple)
       for ( int cz = 0; cz < 2; ++cz )
          for ( int cy = 0; cy < 2; ++cy )
             for ( int cx = 0; cx < 2; ++cx )
             {
-               const int     g = cy + 2 * cz;
-               const int64_t n = lat.rowGroup[g + 1] - lat.rowGroup[g];
+               const int     grp = cy + 2 * cz;
+               const int64_t n   = lat.rowGroup[grp + 1] - lat.rowGroup[grp];
                if ( n == 0 )
                   continue;
-               dim3 grid( unsigned( n ), unsigned( lat.nc ) );
-               kCellSOR< T ><<< grid, kRowBlock, 0, ctx->stream >>>( lat.rows.p + 2 * lat.rowGroup[g], lat.sc, lat.w,
-                                                                   lat.mm, o.stencil.p, cx, relax, (const T*) bp,
-                                                                   (T*) xp );
+               kCellSOR< T ><<< xcdGrid( n * lat.nc ), kRowBlock, 0, ctx->stream >>>(
+                   lat.rows.p + 2 * lat.rowGroup[grp], n, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, cx, relax,
+                   (const T*) bp, (T*) xp );
                HG_CHECK_HIP( hipGetLastError() );
             }
    } );
@@ -601,24 +648,28 @@ void p1Restrict( Context* ctx, Function& f, int fineL, int flag )
 {
    if ( fineL - 1 < f.minL || fineL > f.maxL )
       throw Error( -1, "restriction source level outside the function's level range" );
-   Storage&    st = *f.st;
-   Lattice&    lf = st.lattice( ctx, fineL );
-   Lattice&    lc = st.lattice( ctx, fineL - 1 );
-   P1Transfer& tr = transferFor( ctx, st, fineL );
-   void*       fp = f.ptr( fineL );
-   void*       cp = f.ptr( fineL - 1 );
+   Storage& st = *f.st;
+   Lattice& lf = st.lattice( ctx, fineL );
+   Lattice& lc = st.lattice( ctx, fineL - 1 );
+   void*    fp = f.ptr( fineL );
+   void*    cp = f.ptr( fineL - 1 );
    ctx->beginTiming();
    pullHalos( ctx, st, lf, fp, f.dtype );
    dispatchType( f.dtype, [&]( auto tag ) {
       using T = decltype( tag );
       if ( lc.nrows > 0 )
       {
-         dim3 grid( unsigned( lc.nrows ), unsigned( lc.nc ) );
-         kRestrictCell< T ><<< grid, kRowBlock, 0, ctx->stream >>>( lc.rows.p, lc.sc, lc.w, lc.mm, lf.sc, lf.w,
-                                                                  (const T*) fp, (T*) cp );
+         kRestrictCell< T ><<< xcdGrid( lc.nrows * lc.nc ), kRowBlock, 0, ctx->stream >>>(
+             lc.rowsNat.p, lc.nrows, lc.nc, lc.sc, lc.w, lc.mm, lf.sc, lf.w, (const T*) fp, (T*) cp );
          HG_CHECK_HIP( hipGetLastError() );
       }
-      rowsApply( ctx, tr.restrictRows, lc.bndFlag.p, flag, fp, (T*) cp + lc.nc * lc.sc, f.dtype, 0 );
+      if ( lc.nb > 0 )
+      {
+         kRestrictBnd< T ><<< gridFor( lc.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lc ), lc.nb, lc.nc,
+                                                                           lc.bndFlag.p, flag, st.clsInvN.p, lf.sc,
+                                                                           lf.w, lf.mm, (const T*) fp, (T*) cp );
+         HG_CHECK_HIP( hipGetLastError() );
+      }
    } );
    ctx->endTiming();
 }
@@ -627,24 +678,28 @@ void p1Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update 
 {
    if ( coarseL < f.minL || coarseL + 1 > f.maxL )
       throw Error( -1, "prolongation source level outside the function's level range" );
-   Storage&    st = *f.st;
-   Lattice&    lf = st.lattice( ctx, coarseL + 1 );
-   Lattice&    lc = st.lattice( ctx, coarseL );
-   P1Transfer& tr = transferFor( ctx, st, coarseL + 1 );
-   void*       fp = f.ptr( coarseL + 1 );
-   void*       cp = f.ptr( coarseL );
+   Storage& st = *f.st;
+   Lattice& lf = st.lattice( ctx, coarseL + 1 );
+   Lattice& lc = st.lattice( ctx, coarseL );
+   void*    fp = f.ptr( coarseL + 1 );
+   void*    cp = f.ptr( coarseL );
    ctx->beginTiming();
    pullHalos( ctx, st, lc, cp, f.dtype );
    dispatchType( f.dtype, [&]( auto tag ) {
       using T = decltype( tag );
       if ( lf.nrows > 0 )
       {
-         dim3 grid( unsigned( lf.nrows ), unsigned( lf.nc ) );
-         kProlongCell< T ><<< grid, kRowBlock, 0, ctx->stream >>>( lf.rows.p, lf.sc, lf.w, lf.mm, lc.sc, lc.w,
-                                                                 (const T*) cp, (T*) fp, update );
+         kProlongCell< T ><<< xcdGrid( lf.nrows * lf.nc ), kRowBlock, 0, ctx->stream >>>(
+             lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update );
          HG_CHECK_HIP( hipGetLastError() );
       }
-      rowsApply( ctx, tr.prolongRows, lf.bndFlag.p, flag, cp, (T*) fp + lf.nc * lf.sc, f.dtype, update );
+      if ( lf.nb > 0 )
+      {
+         kProlongBnd< T ><<< gridFor( lf.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lf ), lf.nb, lf.nc,
+                                                                          lf.bndFlag.p, flag, lc.sc, lc.w,
+                                                                          (const T*) cp, (T*) fp, update );
+         HG_CHECK_HIP( hipGetLastError() );
+      }
    } );
    ctx->endTiming();
 }

@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "abi_util.hpp"
+#include "halo.hpp"
 #include "hyteg_amd.h"
 #include "runtime.hpp"
 
@@ -104,6 +105,17 @@ void Storage::init( Context* ctx )
       ptr.upload( ctx, p );
       ent.upload( ctx, e );
    };
+   std::vector< double > inv( 14 * size_t( t.nc ) );
+   for ( int c = 0; c < t.nc; ++c )
+   {
+      for ( int k = 0; k < 4; ++k )
+         inv[14 * c + k] = 1.0 / t.fnc[t.cf[c][k]];
+      for ( int k = 0; k < 6; ++k )
+         inv[14 * c + 4 + k] = 1.0 / t.enc[t.ce[c][k]];
+      for ( int k = 0; k < 4; ++k )
+         inv[14 * c + 10 + k] = 1.0 / t.vnc[t.cv[c][k]];
+   }
+   clsInvN.upload( ctx, inv );
    csr( t.nf, [&]( int c, int k ) { return t.cf[c][k]; }, 4, faceCellPtr, faceCells );
    csr( t.ne, [&]( int c, int k ) { return t.ce[c][k]; }, 6, edgeCellPtr, edgeCells );
    csr( t.nv, [&]( int c, int k ) { return t.cv[c][k]; }, 4, vertCellPtr, vertCells );
@@ -161,47 +173,17 @@ Lattice& Storage::lattice( Context* ctx, int L )
    l.rowGroup[4] = int64_t( rows.size() / 2 );
    l.nrows       = l.rowGroup[4];
    l.rows.upload( ctx, rows );
+   rows.clear();
+   for ( int64_t z = 1; z <= l.mm - 3; ++z )
+      for ( int64_t y = 1; y + z <= l.mm - 2; ++y )
+      {
+         rows.push_back( int32_t( y ) );
+         rows.push_back( int32_t( z ) );
+      }
+   l.rowsNat.upload( ctx, rows );
 
    lattices[L] = std::move( lp );
    return l;
-}
-
-// ------------------------------------------------------------------------
-// halo geometry (device)
-// ------------------------------------------------------------------------
-__device__ __forceinline__ void decodeTri( int64_t n, int64_t i, int64_t& x, int64_t& y )
-{
-   const double b  = 2.0 * double( n ) + 1.0;
-   int64_t      yy = int64_t( floor( 0.5 * ( b - sqrt( b * b - 8.0 * double( i ) ) ) ) );
-   if ( yy < 0 )
-      yy = 0;
-   while ( yy > 0 && triIndex( n, 0, yy ) > i )
-      --yy;
-   while ( triIndex( n, 0, yy + 1 ) <= i )
-      ++yy;
-   y = yy;
-   x = i - triIndex( n, 0, yy );
-}
-
-// lattice position in cell c of the point with barycentric weights t (t[1..3] = x,y,z)
-__device__ __forceinline__ int64_t latticePos( int64_t c, int64_t sc, int64_t w, const int64_t t[4] )
-{
-   return c * sc + tetIndex( w, t[1], t[2], t[3] );
-}
-
-// kind 0: face-interior point (a,b) (0-based inside the interior triangle) of local face k
-__device__ __forceinline__ void faceToCell( const CellMeta& m, int k, int64_t mm, int64_t fx, int64_t fy, int64_t t[4] )
-{
-   t[0] = t[1] = t[2] = t[3] = 0;
-   t[m.fperm[k][0]] = mm - fx - fy;
-   t[m.fperm[k][1]] = fx;
-   t[m.fperm[k][2]] = fy;
-}
-__device__ __forceinline__ void edgeToCell( const CellMeta& m, int k, int64_t mm, int64_t s, int64_t t[4] )
-{
-   t[0] = t[1] = t[2] = t[3] = 0;
-   t[m.eperm[k][0]] = mm - s;
-   t[m.eperm[k][1]] = s;
 }
 
 template < class T >
@@ -241,63 +223,22 @@ __global__ void kPull( const CellMeta* __restrict__ meta, int64_t nc, int64_t sc
       t[k]                      = mm;
       src                       = base + m.vert[k];
    }
-   v[latticePos( c, sc, w, t )] = v[src];
+   v[c * sc + tetIndex( w, t[1], t[2], t[3] )] = v[src];
 }
 
 template < class T >
-__global__ void kPushAdditive( const CellMeta* __restrict__ meta, const int32_t* __restrict__ fptr,
-                               const int32_t* __restrict__ fcells, const int32_t* __restrict__ eptr,
-                               const int32_t* __restrict__ ecells, const int32_t* __restrict__ vptr,
-                               const int32_t* __restrict__ vcells, const uint8_t* __restrict__ bndFlag, int64_t nb,
-                               int64_t nc, int64_t sc, int64_t w, int64_t mm, int64_t fin, int64_t ein, int64_t eoff,
-                               int64_t foff, int excludeFlag, int zeroOut, T* __restrict__ v )
+__global__ void kPushAdditive( BndGeom g, const uint8_t* __restrict__ bndFlag, int64_t nb, int64_t nc,
+                               int excludeFlag, int zeroOut, T* __restrict__ v )
 {
-   using A           = typename Acc< T >::type;
-   const int64_t b   = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   if ( b >= nb )
+   using A         = typename Acc< T >::type;
+   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nb || ( bndFlag[b] & excludeFlag ) )
       return;
-   if ( bndFlag[b] & excludeFlag )
-      return;
-   const int64_t base = nc * sc;
+   const int64_t base = nc * g.sc;
    A             s    = zeroOut ? A( 0 ) : toAcc( v[base + b] );
-   int64_t       t[4];
-   if ( b < eoff )
-   {
-      for ( int32_t e = vptr[b]; e < vptr[b + 1]; ++e )
-      {
-         const int64_t c = vcells[2 * e];
-         const int     k = vcells[2 * e + 1];
-         t[0] = t[1] = t[2] = t[3] = 0;
-         t[k]                      = mm;
-         s += toAcc( v[latticePos( c, sc, w, t )] );
-      }
-   }
-   else if ( b < foff )
-   {
-      const int64_t eid = ( b - eoff ) / ein;
-      const int64_t sidx = ( b - eoff ) - eid * ein + 1;
-      for ( int32_t e = eptr[eid]; e < eptr[eid + 1]; ++e )
-      {
-         const int64_t c = ecells[2 * e];
-         const int     k = ecells[2 * e + 1];
-         edgeToCell( meta[c], k, mm, sidx, t );
-         s += toAcc( v[latticePos( c, sc, w, t )] );
-      }
-   }
-   else
-   {
-      const int64_t f = ( b - foff ) / fin;
-      const int64_t i = ( b - foff ) - f * fin;
-      int64_t       a, bb;
-      decodeTri( mm - 2, i, a, bb );
-      for ( int32_t e = fptr[f]; e < fptr[f + 1]; ++e )
-      {
-         const int64_t c = fcells[2 * e];
-         const int     k = fcells[2 * e + 1];
-         faceToCell( meta[c], k, mm, a + 1, bb + 1, t );
-         s += toAcc( v[latticePos( c, sc, w, t )] );
-      }
-   }
+   forEachIncident( g, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
+      s += toAcc( v[c * g.sc + tetIndex( g.w, x, y, z )] );
+   } );
    v[base + b] = fromAcc< T >( s );
 }
 
@@ -321,9 +262,7 @@ void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtyp
    dispatchType( dtype, [&]( auto tag ) {
       using T = decltype( tag );
       kPushAdditive< T ><<< gridFor( lat.nb ), kBlock, 0, ctx->stream >>>(
-          st.meta.p, st.faceCellPtr.p, st.faceCells.p, st.edgeCellPtr.p, st.edgeCells.p, st.vertCellPtr.p,
-          st.vertCells.p, lat.bndFlag.p, lat.nb, lat.nc, lat.sc, lat.w, lat.mm, lat.num.fin, lat.num.ein, lat.num.eoff,
-          lat.num.foff, excludeFlag, zeroOut ? 1 : 0, (T*) data );
+          BndGeom::of( st, lat ), lat.bndFlag.p, lat.nb, lat.nc, excludeFlag, zeroOut ? 1 : 0, (T*) data );
    } );
    HG_CHECK_HIP( hipGetLastError() );
 }

@@ -95,6 +95,7 @@ struct Lattice
    // interior rows (y,z) of a cell lattice, y,z >= 1, y+z <= mm-2, grouped by
    // (y%2, z%2); group g = (y%2) + 2*(z%2) occupies [rowGroup[g], rowGroup[g+1])
    DevBuf< int32_t > rows; // 2 ints per row
+   DevBuf< int32_t > rowsNat; // the same rows in natural order (z-major, then y)
    int64_t           rowGroup[5] = { 0, 0, 0, 0, 0 };
    int64_t           nrows       = 0;
 
@@ -120,6 +121,9 @@ struct Storage
    std::map< int, std::unique_ptr< OperatorBase > > transfers;
    std::map< int, std::unique_ptr< Lattice > >  lattices;
    DevBuf< CellMeta >                           meta;
+   // 1 / (number of cells) of each cell's local faces (0..3), edges (4..9) and
+   // vertices (10..13): nc*14
+   DevBuf< double >                             clsInvN;
    // cells incident to each face / edge / vertex: CSR of (cell, local index)
    DevBuf< int32_t > faceCellPtr, faceCells, edgeCellPtr, edgeCells, vertCellPtr, vertCells;
    Lattice&                                     lattice( Context* ctx, int L );

@@ -145,7 +145,8 @@ int hg_p1_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int lev
 int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre, int post,
                      int cycles, double tol, int coarseMaxIter, double coarseTol, double* res_hist, int* cycles_done );
 
-/* ---- timing of the last entry point, device time in milliseconds --------- */
+/* ---- device time (ms, HIP events on the context stream) of the dominant
+ * kernel of the last entry point, recorded while event timing is enabled ---- */
 int hg_ctx_event_timing( hg_ctx* ctx, int enable );
 int hg_ctx_last_kernel_ms( hg_ctx* ctx, double* ms );
 
