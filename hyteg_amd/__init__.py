@@ -9,11 +9,12 @@ CPU fallback: if the library is missing, importing `lib()` raises.
 """
 from ._lib import (ALL, DIRICHLET, F16, F32, F64, FREESLIP, GMG_FLAG, INNER, NEUMANN, ADD, REPLACE,
                    HygError, lib, library_path)
-from .api import (Context, P1ConstantLaplaceOperator, P1Function, PrimitiveStorage, cg_solve,
-                  gmg_solve, prolongate, restrict)
+from .api import (Context, P1ConstantLaplaceOperator, P1Function, P2DivKGradOperator, P2Function,
+                  PrimitiveStorage, cg_solve, gmg_solve, prolongate, restrict)
 
 __all__ = [
     "ALL", "DIRICHLET", "F16", "F32", "F64", "FREESLIP", "GMG_FLAG", "INNER", "NEUMANN", "ADD", "REPLACE",
     "HygError", "lib", "library_path", "Context", "P1ConstantLaplaceOperator", "P1Function",
+    "P2DivKGradOperator", "P2Function",
     "PrimitiveStorage", "cg_solve", "gmg_solve", "prolongate", "restrict",
 ]

@@ -42,6 +42,18 @@ SIGNATURES = {
     "hg_function_copy_from": [_p, _i64, _i64, _i],
     "hg_function_communicate": [_p, _i64, _i],
     "hg_p1_laplace_create": [_p, _i64, _i, _i, _p],
+    "hg_storage_shell": [_p, _i, _i, _d, _d, _p],
+    "hg_storage_blend": [_p, _i64, _p],
+    "hg_storage_cell_vertices": [_p, _i64, _p],
+    "hg_storage_vertex_coords": [_p, _i64, _p],
+    "hg_p2_ndofs": [_p, _i64, _i, _p],
+    "hg_p2_dof_coords": [_p, _i64, _i, _p],
+    "hg_p2_dof_flags": [_p, _i64, _i, _p],
+    "hg_p2_function_create": [_p, _i64, _i, _i, _i, _p],
+    "hg_function_mult": [_p, _i64, _i64, _i64, _i, _i],
+    "hg_p2_divkgrad_create": [_p, _i64, _i, _i, _i, _p, _p, _p, _p],
+    "hg_p2_apply": [_p, _i64, _i64, _i64, _i, _i, _i],
+    "hg_p2_inverse_diagonal": [_p, _i64, _i, _i64],
     "hg_operator_destroy": [_p, _i64],
     "hg_p1_apply": [_p, _i64, _i64, _i64, _i, _i, _i],
     "hg_p1_smooth_sor": [_p, _i64, _i64, _i64, _d, _i, _i],

@@ -67,6 +67,30 @@ class PrimitiveStorage:
         return PrimitiveStorage(ctx, h.value)
 
     @staticmethod
+    def shell(ctx, ntan=5, nrad=3, rmin=0.55, rmax=1.0):
+        h = C.c_int64()
+        ctx.check(lib().hg_storage_shell(ctx.h, ntan, nrad, rmin, rmax, C.byref(h)), "meshSphericalShell")
+        return PrimitiveStorage(ctx, h.value)
+
+    def blend(self):
+        out = np.zeros((self.counts()[3], 14))
+        self.ctx.check(lib().hg_storage_blend(self.ctx.h, self.h, _ptr(out)), "blend")
+        return out
+
+    def cell_vertices(self):
+        out = np.zeros((self.counts()[3], 4), dtype=np.int64)
+        self.ctx.check(lib().hg_storage_cell_vertices(self.ctx.h, self.h, _ptr(out)), "cell_vertices")
+        return out
+
+    def vertex_coords(self):
+        out = np.zeros((self.counts()[0], 3))
+        self.ctx.check(lib().hg_storage_vertex_coords(self.ctx.h, self.h, _ptr(out)), "vertex_coords")
+        return out
+
+    def p2_ndofs(self, level):
+        return self.ndofs(level + 1)
+
+    @staticmethod
     def from_cells(ctx, coords, cells):
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         cells = np.ascontiguousarray(cells, dtype=np.int64)
@@ -97,15 +121,20 @@ class PrimitiveStorage:
 
 
 class P1Function:
+    OFFSET = 0
+
     def __init__(self, name, storage, min_level, max_level, dtype=F64):
         self.name = name
         self.storage = storage
         self.ctx = storage.ctx
         self.min_level, self.max_level = min_level, max_level
         h = C.c_int64()
-        self.ctx.check(lib().hg_p1_function_create(self.ctx.h, storage.h, min_level, max_level, dtype, C.byref(h)),
-                       f"P1Function {name}")
+        create = lib().hg_p2_function_create if self.OFFSET else lib().hg_p1_function_create
+        self.ctx.check(create(self.ctx.h, storage.h, min_level, max_level, dtype, C.byref(h)), f"Function {name}")
         self.h = h.value
+
+    def ndofs(self, level):
+        return self.storage.ndofs(level + self.OFFSET)
 
     def destroy(self):
         if getattr(self, "h", None) and self.ctx.h:
@@ -114,12 +143,12 @@ class P1Function:
 
     def upload(self, level, values):
         v = np.ascontiguousarray(values, dtype=np.float64)
-        if v.shape != (self.storage.ndofs(level),):
+        if v.shape != (self.ndofs(level),):
             raise ValueError("wrong number of values")
         self.ctx.check(lib().hg_function_upload(self.ctx.h, self.h, level, _ptr(v)), "upload")
 
     def download(self, level):
-        out = np.zeros(self.storage.ndofs(level))
+        out = np.zeros(self.ndofs(level))
         self.ctx.check(lib().hg_function_download(self.ctx.h, self.h, level, _ptr(out)), "download")
         return out
 
@@ -148,6 +177,39 @@ class P1Function:
 
     def communicate(self, level):
         self.ctx.check(lib().hg_function_communicate(self.ctx.h, self.h, level), "communicate")
+
+    def mult(self, a, b, level, flag=ALL):
+        self.ctx.check(lib().hg_function_mult(self.ctx.h, self.h, a.h, b.h, level, flag), "multElementwise")
+
+
+class P2Function(P1Function):
+    """P2 DoFs of level L, numbered like the P1 DoFs of level L+1"""
+    OFFSET = 1
+
+
+class P2DivKGradOperator:
+    """P2ElementwiseOperator with the blended div(k grad) form,
+    k(x) = exp(sum_m a_m sin(omega_m . x + phi_m))"""
+
+    def __init__(self, storage, min_level, max_level, a=(), omega=(), phi=()):
+        self.storage = storage
+        self.ctx = storage.ctx
+        self.a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        self.omega = np.ascontiguousarray(omega, dtype=np.float64).reshape(-1, 3)
+        self.phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        h = C.c_int64()
+        e = np.zeros(1)
+        nm = len(self.a)
+        pa, po, pp = (_ptr(self.a), _ptr(self.omega), _ptr(self.phi)) if nm else (_ptr(e), _ptr(e), _ptr(e))
+        self.ctx.check(lib().hg_p2_divkgrad_create(self.ctx.h, storage.h, min_level, max_level, nm, pa, po, pp,
+                                                   C.byref(h)), "P2DivKGradOperator")
+        self.h = h.value
+
+    def apply(self, src, dst, level, flag, update=REPLACE):
+        self.ctx.check(lib().hg_p2_apply(self.ctx.h, self.h, src.h, dst.h, level, flag, update), "apply")
+
+    def inverse_diagonal(self, level, dst):
+        self.ctx.check(lib().hg_p2_inverse_diagonal(self.ctx.h, self.h, level, dst.h), "inverse diagonal")
 
 
 class P1ConstantLaplaceOperator:

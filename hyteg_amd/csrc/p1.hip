@@ -570,8 +570,8 @@ static P1OpLevel& levelOf( Context* ctx, P1Operator& op, int L )
 
 void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L, int flag, int update )
 {
-   if ( src.st != op.st || dst.st != op.st || src.dtype != dst.dtype )
-      throw Error( -1, "apply needs src/dst on the operator's storage with one value type" );
+   if ( src.st != op.st || dst.st != op.st || src.dtype != dst.dtype || src.off != 0 || dst.off != 0 )
+      throw Error( -1, "apply needs P1 src/dst on the operator's storage with one value type" );
    if ( &src == &dst )
       throw Error( -1, "apply needs distinct src and dst" );
    P1OpLevel& o   = levelOf( ctx, op, L );
@@ -601,7 +601,7 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
 
 void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax, int L, int flag )
 {
-   if ( x.st != op.st || b.st != op.st || x.dtype != b.dtype || &x == &b )
+   if ( x.st != op.st || b.st != op.st || x.dtype != b.dtype || &x == &b || x.off != 0 || b.off != 0 )
       throw Error( -1, "smooth_sor needs distinct x/b on the operator's storage with one value type" );
    P1OpLevel& o   = levelOf( ctx, op, L );
    Lattice&   lat = op.st->lattice( ctx, L );
@@ -646,8 +646,8 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
 
 void p1Restrict( Context* ctx, Function& f, int fineL, int flag )
 {
-   if ( fineL - 1 < f.minL || fineL > f.maxL )
-      throw Error( -1, "restriction source level outside the function's level range" );
+   if ( fineL - 1 < f.minL || fineL > f.maxL || f.off != 0 )
+      throw Error( -1, "P1 restriction source level outside the function's level range" );
    Storage& st = *f.st;
    Lattice& lf = st.lattice( ctx, fineL );
    Lattice& lc = st.lattice( ctx, fineL - 1 );
@@ -676,8 +676,8 @@ void p1Restrict( Context* ctx, Function& f, int fineL, int flag )
 
 void p1Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update )
 {
-   if ( coarseL < f.minL || coarseL + 1 > f.maxL )
-      throw Error( -1, "prolongation source level outside the function's level range" );
+   if ( coarseL < f.minL || coarseL + 1 > f.maxL || f.off != 0 )
+      throw Error( -1, "P1 prolongation source level outside the function's level range" );
    Storage& st = *f.st;
    Lattice& lf = st.lattice( ctx, coarseL + 1 );
    Lattice& lc = st.lattice( ctx, coarseL );

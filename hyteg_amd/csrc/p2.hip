@@ -1,0 +1,660 @@
+// P2 elementwise div(k grad) operator with icosahedral-shell blending:
+// host set-up, HIP kernels and C-ABI entry points.
+//
+// A P2 function of level L is stored on the lattice of level L+1.  One thread
+// handles one micro-cell of level L: it gathers the 10 P2 values, evaluates
+// the element action with the 11-point Xiao-Gimbutas rule at blended points
+// (p2_div_k_grad_blending_q4, P2ElementwiseOperator::gemv,
+// P2ElementwiseOperator.cpp:67-233) and adds the 10 results into the cell's
+// lattice; contributions on macro-faces/edges/vertices are then summed over
+// the incident cells (communicateAdditively<Cell, Face/Edge/Vertex>).
+//
+// The element action is evaluated as  y_i = sum_q w_q k(F(x_q)) |det M_q|
+// (M_q^-T grad phi_i) . (sum_j x_j M_q^-T grad phi_j),  M_q = DF(x_q) J, which is
+// the element matrix of the reference applied to the element vector without
+// forming the 10x10 matrix.
+#include <cmath>
+
+#include "abi_util.hpp"
+#include "halo.hpp"
+#include "hyteg_amd.h"
+#include "runtime.hpp"
+
+namespace hg {
+
+// quadrature points (barycentric w.r.t. the element vertices in the order
+// getMicroVerticesFromMicroCell lists them) and weights (reference tet, sum 1/6)
+__constant__ double cQP[11][4];
+__constant__ double cQW[11];
+__constant__ int    cVof[6][4][3] = {
+    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
+    { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
+    { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } }, { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
+__constant__ int cP2E[6][2] = { { 0, 1 }, { 0, 2 }, { 0, 3 }, { 1, 2 }, { 1, 3 }, { 2, 3 } };
+
+// Xiao-Gimbutas degree-4 rule on the tetrahedron, 11 points; the weights are
+// fixed by exactness for all polynomials of degree <= 4
+static const double kQP[11][4] = {
+    { 0.17469405869723054, 0.040490506727590428, 0.01356070187980288, 0.77125473269537614 },
+    { 0.081404918402859225, 0.75250850700965499, 0.068099370938206658, 0.097987203649279112 },
+    { 0.30444840243449711, 0.24027666492807259, 0.12680172591539199, 0.32847320672203839 },
+    { 0.74122888209362259, 0.067223294893383398, 0.035183929773598722, 0.15636389323939531 },
+    { 0.053341239535745266, 0.41926631387951302, 0.047781435559086663, 0.47961101102565512 },
+    { 0.43295349048135578, 0.45076587609127677, 0.059456616299433829, 0.056824017127933668 },
+    { 0.53800720391618573, 0.12941137378891041, 0.33019041483746447, 0.0023910074574393651 },
+    { 0.008991260093335729, 0.1215419913339278, 0.30649398842969028, 0.56297276014304609 },
+    { 0.10660417256199362, 0.097204644587583267, 0.68439041545304002, 0.11180076739738309 },
+    { 0.32923295974264682, 0.029569495206479609, 0.31790356021339461, 0.32329398483747901 },
+    { 0.10384411641099309, 0.43271023904776862, 0.35382323920929709, 0.1096224053319412 },
+};
+
+// weights from the moment equations sum_q w_q xi^a = a1! a2! a3! / (|a| + 3)!,
+// |a| <= 4 (35 equations, 11 unknowns, consistent), solved in the least-squares
+// sense with Householder reflections
+static void quadratureWeights( double w[11] )
+{
+   auto fact = []( int n ) {
+      double f = 1;
+      for ( int i = 2; i <= n; ++i )
+         f *= i;
+      return f;
+   };
+   std::vector< std::array< double, 12 > > A;
+   for ( int a = 0; a <= 4; ++a )
+      for ( int b = 0; a + b <= 4; ++b )
+         for ( int c = 0; a + b + c <= 4; ++c )
+         {
+            std::array< double, 12 > r{};
+            for ( int q = 0; q < 11; ++q )
+               r[q] = std::pow( kQP[q][1], a ) * std::pow( kQP[q][2], b ) * std::pow( kQP[q][3], c );
+            r[11] = fact( a ) * fact( b ) * fact( c ) / fact( a + b + c + 3 );
+            A.push_back( r );
+         }
+   const size_t m = A.size();
+   for ( int k = 0; k < 11; ++k )
+   {
+      double nrm = 0.0;
+      for ( size_t r = k; r < m; ++r )
+         nrm += A[r][k] * A[r][k];
+      nrm = std::sqrt( nrm );
+      std::vector< double > v( m, 0.0 );
+      for ( size_t r = k; r < m; ++r )
+         v[r] = A[r][k];
+      v[k] += A[k][k] > 0 ? nrm : -nrm;
+      double vv = 0.0;
+      for ( size_t r = k; r < m; ++r )
+         vv += v[r] * v[r];
+      for ( int j = k; j < 12; ++j )
+      {
+         double d = 0.0;
+         for ( size_t r = k; r < m; ++r )
+            d += v[r] * A[r][j];
+         d *= 2.0 / vv;
+         for ( size_t r = k; r < m; ++r )
+            A[r][j] -= d * v[r];
+      }
+   }
+   for ( int i = 10; i >= 0; --i )
+   {
+      double s = A[i][11];
+      for ( int j = i + 1; j < 11; ++j )
+         s -= A[i][j] * w[j];
+      w[i] = s / A[i][i];
+   }
+}
+
+struct CellGeom
+{
+   double v0[3], e1[3], e2[3], e3[3]; // v0 and edge vectors v_k - v0 of the macro-cell
+   double blend[14];                  // ray, ref, thr, for, radRay, radRef
+   int    kind;                       // 0 affine, 1 shell
+};
+
+struct KParams
+{
+   int    nm;
+   double a[32], om[32][3], ph[32];
+};
+
+struct P2Operator : OperatorBase
+{
+   Storage*               st = nullptr;
+   int                    minL = 0, maxL = 0;
+   DevBuf< CellGeom >     geom;
+   DevBuf< KParams >      kpar;
+   std::map< int, DevBuf< double > > invDiag; // by P2 level, lattice L+1 storage
+};
+
+__device__ __forceinline__ double kEval( const KParams& K, const double x[3] )
+{
+   if ( K.nm == 0 )
+      return 1.0;
+   double s = 0.0;
+   for ( int m = 0; m < K.nm; ++m )
+      s += K.a[m] * sin( K.om[m][0] * x[0] + K.om[m][1] * x[1] + K.om[m][2] * x[2] + K.ph[m] );
+   return exp( s );
+}
+
+// shell blending (IcosahedralShellMap::evalF / evalDF): radial projection with
+// the radius linear in the barycentric coordinate of refVertex
+__device__ __forceinline__ void shellMap( const double* B, const double x[3], double F[3], double DF[9] )
+{
+   const double a[3] = { B[3] - B[0], B[4] - B[1], B[5] - B[2] };
+   const double t[3] = { B[6] - B[0], B[7] - B[1], B[8] - B[2] };
+   const double f[3] = { B[9] - B[0], B[10] - B[1], B[11] - B[2] };
+   const double d[3] = { x[0] - B[0], x[1] - B[1], x[2] - B[2] };
+   const double n[3] = { t[1] * f[2] - t[2] * f[1], t[2] * f[0] - t[0] * f[2], t[0] * f[1] - t[1] * f[0] };
+   const double volT = a[0] * n[0] + a[1] * n[1] + a[2] * n[2];
+   const double lam  = ( d[0] * n[0] + d[1] * n[1] + d[2] * n[2] ) / volT;
+   const double rho  = sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
+   const double rad  = B[12] + fabs( lam ) * ( B[13] - B[12] );
+   const double q    = rad / rho;
+   const double s    = ( lam < 0 ? -1.0 : 1.0 ) * ( B[13] - B[12] ) / volT / rho;
+   for ( int i = 0; i < 3; ++i )
+   {
+      F[i] = x[i] * q;
+      for ( int j = 0; j < 3; ++j )
+         DF[3 * i + j] = q * ( ( i == j ? 1.0 : 0.0 ) - x[i] * x[j] / ( rho * rho ) ) + x[i] * s * n[j];
+   }
+}
+
+// micro-cell number j of type t in a macro-cell lattice of 2^L micro-edges
+__device__ __forceinline__ void decodeMicro( int64_t wt, int64_t j, int64_t& x, int64_t& y, int64_t& z )
+{
+   // slice z holds tri(wt - z) micro-cells
+   int64_t lo = 0, hi = wt - 1;
+   const int64_t total = tet( wt );
+   while ( lo < hi )
+   {
+      const int64_t mid = ( lo + hi + 1 ) / 2;
+      if ( total - tet( wt - mid ) <= j )
+         lo = mid;
+      else
+         hi = mid - 1;
+   }
+   z = lo;
+   decodeTri( wt - z, j - ( total - tet( wt - z ) ), x, y );
+}
+
+// locate micro-cell gid: returns cell c, type t and anchor (x,y,z)
+__device__ __forceinline__ bool microOf( int64_t gid, int64_t mm, int64_t nc, int64_t& c, int& t, int64_t a[3] )
+{
+   int64_t per = 0;
+   for ( int k = 0; k < 6; ++k )
+      per += tet( mm - MicroCells::rowShift[k] );
+   if ( gid >= per * nc )
+      return false;
+   c         = gid / per;
+   int64_t j = gid - c * per;
+   t         = 0;
+   while ( j >= tet( mm - MicroCells::rowShift[t] ) )
+   {
+      j -= tet( mm - MicroCells::rowShift[t] );
+      ++t;
+   }
+   decodeMicro( mm - MicroCells::rowShift[t], j, a[0], a[1], a[2] );
+   return true;
+}
+
+// physical gradients of the 4 barycentric coordinates D[k] and |det M| at a
+// quadrature point, plus the blended point F
+__device__ __forceinline__ void elementGeometry( const CellGeom& g, const double p[4][3], const double lam[4],
+                                                 double D[4][3], double& absdet, double F[3] )
+{
+   double J[9];
+   for ( int r = 0; r < 3; ++r )
+      for ( int k = 0; k < 3; ++k )
+         J[3 * r + k] = p[k + 1][r] - p[0][r];
+   double x[3];
+   for ( int d = 0; d < 3; ++d )
+      x[d] = lam[0] * p[0][d] + lam[1] * p[1][d] + lam[2] * p[2][d] + lam[3] * p[3][d];
+   double M[9];
+   if ( g.kind )
+   {
+      double DF[9];
+      shellMap( g.blend, x, F, DF );
+      for ( int i = 0; i < 3; ++i )
+         for ( int j = 0; j < 3; ++j )
+            M[3 * i + j] = DF[3 * i] * J[j] + DF[3 * i + 1] * J[3 + j] + DF[3 * i + 2] * J[6 + j];
+   }
+   else
+   {
+      for ( int i = 0; i < 9; ++i )
+         M[i] = J[i];
+      F[0] = x[0], F[1] = x[1], F[2] = x[2];
+   }
+   const double c00 = M[4] * M[8] - M[5] * M[7], c01 = M[5] * M[6] - M[3] * M[8], c02 = M[3] * M[7] - M[4] * M[6];
+   const double det = M[0] * c00 + M[1] * c01 + M[2] * c02;
+   const double id  = 1.0 / det;
+   // rows of M^-1 are the gradients of barycentric coordinates 1..3
+   D[1][0] = c00 * id;
+   D[1][1] = ( M[2] * M[7] - M[1] * M[8] ) * id;
+   D[1][2] = ( M[1] * M[5] - M[2] * M[4] ) * id;
+   D[2][0] = c01 * id;
+   D[2][1] = ( M[0] * M[8] - M[2] * M[6] ) * id;
+   D[2][2] = ( M[2] * M[3] - M[0] * M[5] ) * id;
+   D[3][0] = c02 * id;
+   D[3][1] = ( M[1] * M[6] - M[0] * M[7] ) * id;
+   D[3][2] = ( M[0] * M[4] - M[1] * M[3] ) * id;
+   for ( int d = 0; d < 3; ++d )
+      D[0][d] = -( D[1][d] + D[2][d] + D[3][d] );
+   absdet = fabs( det );
+}
+
+// physical gradients of the 10 P2 basis functions at barycentric point lam
+__device__ __forceinline__ void p2Gradients( const double lam[4], const double D[4][3], double G[10][3] )
+{
+   for ( int i = 0; i < 4; ++i )
+      for ( int d = 0; d < 3; ++d )
+         G[i][d] = ( 4.0 * lam[i] - 1.0 ) * D[i][d];
+   for ( int e = 0; e < 6; ++e )
+   {
+      const int a = cP2E[e][0], b = cP2E[e][1];
+      for ( int d = 0; d < 3; ++d )
+         G[4 + e][d] = 4.0 * ( lam[a] * D[b][d] + lam[b] * D[a][d] );
+   }
+}
+
+// vertex coordinates (computational space) and the 10 lattice positions (level L+1)
+__device__ __forceinline__ void microElement( const CellGeom& g, int t, const int64_t a[3], int64_t mm, int64_t c,
+                                              int64_t sc1, int64_t w1, double p[4][3], int64_t pos[10] )
+{
+   const double h = 1.0 / double( mm );
+   int64_t      v[4][3];
+   for ( int k = 0; k < 4; ++k )
+   {
+      for ( int d = 0; d < 3; ++d )
+         v[k][d] = a[d] + cVof[t][k][d];
+      for ( int d = 0; d < 3; ++d )
+         p[k][d] = g.v0[d] + g.e1[d] * ( h * double( v[k][0] ) ) + g.e2[d] * ( h * double( v[k][1] ) ) +
+                   g.e3[d] * ( h * double( v[k][2] ) );
+      pos[k] = c * sc1 + tetIndex( w1, 2 * v[k][0], 2 * v[k][1], 2 * v[k][2] );
+   }
+   for ( int e = 0; e < 6; ++e )
+   {
+      const int64_t* va = v[cP2E[e][0]];
+      const int64_t* vb = v[cP2E[e][1]];
+      pos[4 + e]        = c * sc1 + tetIndex( w1, va[0] + vb[0], va[1] + vb[1], va[2] + vb[2] );
+   }
+}
+
+__global__ void kP2ElementApply( const CellGeom* __restrict__ geom, const KParams* __restrict__ kp, int64_t mm,
+                                 int64_t nc, int64_t sc1, int64_t w1, const double* __restrict__ src,
+                                 double* __restrict__ dst )
+{
+   int64_t c, a[3];
+   int     t;
+   if ( !microOf( int64_t( blockIdx.x ) * blockDim.x + threadIdx.x, mm, nc, c, t, a ) )
+      return;
+   const CellGeom& g = geom[c];
+   double          p[4][3];
+   int64_t         pos[10];
+   microElement( g, t, a, mm, c, sc1, w1, p, pos );
+   double xe[10], ye[10];
+   for ( int i = 0; i < 10; ++i )
+   {
+      xe[i] = src[pos[i]];
+      ye[i] = 0.0;
+   }
+   for ( int q = 0; q < 11; ++q )
+   {
+      const double lam[4] = { cQP[q][0], cQP[q][1], cQP[q][2], cQP[q][3] };
+      double       D[4][3], absdet, F[3], G[10][3];
+      elementGeometry( g, p, lam, D, absdet, F );
+      p2Gradients( lam, D, G );
+      double gu[3] = { 0.0, 0.0, 0.0 };
+      for ( int j = 0; j < 10; ++j )
+         for ( int d = 0; d < 3; ++d )
+            gu[d] += xe[j] * G[j][d];
+      const double wq = cQW[q] * kEval( *kp, F ) * absdet;
+      for ( int i = 0; i < 10; ++i )
+         ye[i] += wq * ( G[i][0] * gu[0] + G[i][1] * gu[1] + G[i][2] * gu[2] );
+   }
+   for ( int i = 0; i < 10; ++i )
+      atomicAdd( dst + pos[i], ye[i] );
+}
+
+__global__ void kP2ElementDiag( const CellGeom* __restrict__ geom, const KParams* __restrict__ kp, int64_t mm,
+                                int64_t nc, int64_t sc1, int64_t w1, double* __restrict__ dst )
+{
+   int64_t c, a[3];
+   int     t;
+   if ( !microOf( int64_t( blockIdx.x ) * blockDim.x + threadIdx.x, mm, nc, c, t, a ) )
+      return;
+   const CellGeom& g = geom[c];
+   double          p[4][3];
+   int64_t         pos[10];
+   microElement( g, t, a, mm, c, sc1, w1, p, pos );
+   double ye[10] = {};
+   for ( int q = 0; q < 11; ++q )
+   {
+      const double lam[4] = { cQP[q][0], cQP[q][1], cQP[q][2], cQP[q][3] };
+      double       D[4][3], absdet, F[3], G[10][3];
+      elementGeometry( g, p, lam, D, absdet, F );
+      p2Gradients( lam, D, G );
+      const double wq = cQW[q] * kEval( *kp, F ) * absdet;
+      for ( int i = 0; i < 10; ++i )
+         ye[i] += wq * ( G[i][0] * G[i][0] + G[i][1] * G[i][1] + G[i][2] * G[i][2] );
+   }
+   for ( int i = 0; i < 10; ++i )
+      atomicAdd( dst + pos[i], ye[i] );
+}
+
+// halo positions of every cell lattice set to zero
+template < class T >
+__global__ void kZeroHalos( BndGeom g, int64_t nc, T* __restrict__ v )
+{
+   const int64_t per = 4 * g.fin + 6 * g.ein + 4;
+   const int64_t gid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( gid >= per * nc )
+      return;
+   const int64_t c = gid / per;
+   int64_t       j = gid - c * per;
+   int64_t       t[4];
+   if ( j < 4 * g.fin )
+   {
+      const int     k = int( j / g.fin );
+      int64_t       a, b;
+      decodeTri( g.mm - 2, j - k * g.fin, a, b );
+      faceToCell( g.meta[c], k, g.mm, a + 1, b + 1, t );
+   }
+   else if ( ( j -= 4 * g.fin ) < 6 * g.ein )
+   {
+      const int k = int( j / g.ein );
+      edgeToCell( g.meta[c], k, g.mm, j - k * g.ein + 1, t );
+   }
+   else
+   {
+      t[0] = t[1] = t[2] = t[3] = 0;
+      t[j - 6 * g.ein]          = g.mm;
+   }
+   v[c * g.sc + tetIndex( g.w, t[1], t[2], t[3] )] = fromAcc< T >( typename Acc< T >::type( 0 ) );
+}
+
+template < class T >
+__global__ void kFillRange( int64_t n, T* __restrict__ v, double value )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( i < n )
+      v[i] = fromAcc< T >( typename Acc< T >::type( value ) );
+}
+
+__global__ void kInvertNonZero( int64_t n, double* __restrict__ v )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( i < n && v[i] != 0.0 )
+      v[i] = 1.0 / v[i];
+}
+
+static int64_t microCellsPerCell( int64_t mm )
+{
+   int64_t per = 0;
+   for ( int k = 0; k < 6; ++k )
+      per += tet( mm - MicroCells::rowShift[k] );
+   return per;
+}
+
+static void p2CheckFunction( const P2Operator& op, const Function& f, int L )
+{
+   if ( f.st != op.st || f.off != 1 )
+      throw Error( -1, "P2 operator needs P2 functions on its storage" );
+   if ( L < op.minL || L > op.maxL || L < f.minL || L > f.maxL )
+      throw Error( -1, "level outside the operator's or function's level range" );
+}
+
+void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L, int flag, int update )
+{
+   p2CheckFunction( op, src, L );
+   p2CheckFunction( op, dst, L );
+   if ( src.dtype != 0 || dst.dtype != 0 )
+      throw Error( HG_ERR_UNSUPPORT, "the blended P2 div(k grad) operator runs in FP64" );
+   if ( &src == &dst )
+      throw Error( -1, "apply needs distinct src and dst" );
+   Storage&      st  = *op.st;
+   Lattice&      lat = st.lattice( ctx, L + 1 );
+   double*       s   = (double*) src.ptr( L );
+   double*       d   = (double*) dst.ptr( L );
+   const int64_t mm  = int64_t( 1 ) << L;
+   pullHalos( ctx, st, lat, s, 0 );
+   // zero the halos (all) and, for Replace with Inner in flag, the cell interiors
+   if ( update == HG_REPLACE && ( flag & kInner ) )
+      kFillRange< double ><<< gridFor( lat.nc * lat.sc ), kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, d, 0.0 );
+   else if ( lat.haloPerCell() > 0 )
+      kZeroHalos< double ><<< gridFor( lat.haloPerCell() * lat.nc ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lat ),
+                                                                                              lat.nc, d );
+   HG_CHECK_HIP( hipGetLastError() );
+   const int64_t n = microCellsPerCell( mm ) * lat.nc;
+   ctx->beginTiming();
+   kP2ElementApply<<< gridFor( n, 128 ), 128, 0, ctx->stream >>>( op.geom.p, op.kpar.p, mm, lat.nc, lat.sc, lat.w, s, d );
+   HG_CHECK_HIP( hipGetLastError() );
+   ctx->endTiming();
+   pushAdditive( ctx, st, lat, d, 0, kAll ^ flag, update == HG_REPLACE );
+}
+
+double* p2InverseDiagonal( Context* ctx, P2Operator& op, int L )
+{
+   auto it = op.invDiag.find( L );
+   if ( it != op.invDiag.end() )
+      return it->second.p;
+   Storage& st  = *op.st;
+   Lattice& lat = st.lattice( ctx, L + 1 );
+   auto&    buf = op.invDiag[L];
+   buf.alloc( ctx, lat.nstore );
+   const int64_t mm = int64_t( 1 ) << L;
+   const int64_t n  = microCellsPerCell( mm ) * lat.nc;
+   kP2ElementDiag<<< gridFor( n, 128 ), 128, 0, ctx->stream >>>( op.geom.p, op.kpar.p, mm, lat.nc, lat.sc, lat.w, buf.p );
+   HG_CHECK_HIP( hipGetLastError() );
+   pushAdditive( ctx, st, lat, buf.p, 0, 0, true );
+   kInvertNonZero<<< gridFor( lat.nstore ), kBlock, 0, ctx->stream >>>( lat.nstore, buf.p );
+   HG_CHECK_HIP( hipGetLastError() );
+   return buf.p;
+}
+
+template < class T >
+__global__ void kMult( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag,
+                       const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ dst )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t n = ( flag & kInner ? ncell : 0 ) + nb;
+   if ( i >= n )
+      return;
+   const int64_t p = ( flag & kInner ) ? i : ncell + i;
+   if ( p >= ncell && !( bndFlag[p - ncell] & flag ) )
+      return;
+   dst[p] = fromAcc< T >( toAcc( a[p] ) * toAcc( b[p] ) );
+}
+
+} // namespace hg
+
+using namespace hg;
+
+extern "C" {
+
+int hg_storage_shell( hg_ctx* h, int ntan, int nrad, double rmin, double rmax, hg_handle* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      auto st  = std::make_unique< Storage >();
+      st->topo = Topology::shell( ntan, nrad, rmin, rmax );
+      st->init( ctx );
+      *out                 = ctx->nextHandle++;
+      ctx->storages[*out] = std::move( st );
+   } );
+}
+
+int hg_storage_blend( hg_ctx* h, hg_handle s, double* out14 )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      const Topology& t = ctx->storage( s ).topo;
+      if ( !out14 || t.blend.empty() )
+         throw Error( HG_ERR_ARG, "storage has no blending" );
+      for ( int c = 0; c < t.nc; ++c )
+      {
+         const ShellCellMap& b = t.blend[c];
+         double*             o = out14 + 14 * size_t( c );
+         for ( int d = 0; d < 3; ++d )
+         {
+            o[d]     = b.rayVertex[d];
+            o[3 + d] = b.refVertex[d];
+            o[6 + d] = b.thrVertex[d];
+            o[9 + d] = b.forVertex[d];
+         }
+         o[12] = b.radRayVertex;
+         o[13] = b.radRefVertex;
+      }
+   } );
+}
+
+int hg_storage_cell_vertices( hg_ctx* h, hg_handle s, int64_t* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      const Topology& t = ctx->storage( s ).topo;
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      for ( int c = 0; c < t.nc; ++c )
+         for ( int k = 0; k < 4; ++k )
+            out[4 * size_t( c ) + k] = t.cv[c][k];
+   } );
+}
+
+int hg_storage_vertex_coords( hg_ctx* h, hg_handle s, double* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      const Topology& t = ctx->storage( s ).topo;
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      std::copy( t.X.begin(), t.X.end(), out );
+   } );
+}
+
+int hg_p2_ndofs( hg_ctx* h, hg_handle s, int level, int64_t* out )
+{
+   return hg_p1_ndofs( h, s, level + 1, out );
+}
+
+int hg_p2_dof_coords( hg_ctx* h, hg_handle s, int level, double* xyz )
+{
+   return hg_p1_dof_coords( h, s, level + 1, xyz );
+}
+
+int hg_p2_dof_flags( hg_ctx* h, hg_handle s, int level, int32_t* flags )
+{
+   return hg_p1_dof_flags( h, s, level + 1, flags );
+}
+
+int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int nm, const double* a, const double* omega,
+                           const double* phi, hg_handle* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out || minLevel < 0 || maxLevel < minLevel || maxLevel > 11 || nm < 0 || nm > 32 ||
+           ( nm > 0 && ( !a || !omega || !phi ) ) )
+         throw Error( HG_ERR_ARG, "invalid operator parameters" );
+      static bool quadReady = false;
+      if ( !quadReady )
+      {
+         double w[11];
+         quadratureWeights( w );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQP ), kQP, sizeof( kQP ) ) );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQW ), w, sizeof( w ) ) );
+         quadReady = true;
+      }
+      Storage&        st = ctx->storage( s );
+      const Topology& t  = st.topo;
+      auto            op = std::make_unique< P2Operator >();
+      op->st             = &st;
+      op->minL           = minLevel;
+      op->maxL           = maxLevel;
+      std::vector< CellGeom > g( t.nc );
+      for ( int c = 0; c < t.nc; ++c )
+      {
+         const double* v[4];
+         for ( int k = 0; k < 4; ++k )
+            v[k] = &t.X[3 * t.cv[c][k]];
+         for ( int d = 0; d < 3; ++d )
+         {
+            g[c].v0[d] = v[0][d];
+            g[c].e1[d] = v[1][d] - v[0][d];
+            g[c].e2[d] = v[2][d] - v[0][d];
+            g[c].e3[d] = v[3][d] - v[0][d];
+         }
+         g[c].kind = t.blend.empty() ? 0 : 1;
+         if ( g[c].kind )
+         {
+            const ShellCellMap& b = t.blend[c];
+            for ( int d = 0; d < 3; ++d )
+            {
+               g[c].blend[d]     = b.rayVertex[d];
+               g[c].blend[3 + d] = b.refVertex[d];
+               g[c].blend[6 + d] = b.thrVertex[d];
+               g[c].blend[9 + d] = b.forVertex[d];
+            }
+            g[c].blend[12] = b.radRayVertex;
+            g[c].blend[13] = b.radRefVertex;
+         }
+      }
+      op->geom.upload( ctx, g );
+      std::vector< KParams > kp( 1 );
+      kp[0]    = KParams{};
+      kp[0].nm = nm;
+      for ( int m = 0; m < nm; ++m )
+      {
+         kp[0].a[m]  = a[m];
+         kp[0].ph[m] = phi[m];
+         for ( int d = 0; d < 3; ++d )
+            kp[0].om[m][d] = omega[3 * m + d];
+      }
+      op->kpar.upload( ctx, kp );
+      for ( int L = minLevel; L <= maxLevel; ++L )
+         st.lattice( ctx, L + 1 );
+      *out                  = ctx->nextHandle++;
+      ctx->operators[*out] = std::move( op );
+   } );
+}
+
+int hg_p2_apply( hg_ctx* h, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( update != HG_REPLACE && update != HG_ADD )
+         throw Error( HG_ERR_ARG, "invalid update type" );
+      p2Apply( ctx, ctx->op< P2Operator >( op ), ctx->function( src ), ctx->function( dst ), level, flag, update );
+   } );
+}
+
+int hg_p2_inverse_diagonal( hg_ctx* h, hg_handle op, int level, hg_handle dst )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P2Operator& o = ctx->op< P2Operator >( op );
+      Function&   f = ctx->function( dst );
+      p2CheckFunction( o, f, level );
+      if ( f.dtype != 0 )
+         throw Error( HG_ERR_UNSUPPORT, "inverse diagonal is FP64" );
+      const double* inv = p2InverseDiagonal( ctx, o, level );
+      Lattice&      lat = o.st->lattice( ctx, level + 1 );
+      HG_CHECK_HIP( hipMemcpyAsync( f.ptr( level ), inv, lat.nstore * sizeof( double ), hipMemcpyDeviceToDevice, ctx->stream ) );
+   } );
+}
+
+int hg_function_mult( hg_ctx* h, hg_handle dst, hg_handle a, hg_handle b, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& d  = ctx->function( dst );
+      Function& fa = ctx->function( a );
+      Function& fb = ctx->function( b );
+      for ( Function* f : { &d, &fa, &fb } )
+         if ( f->st != d.st || f->dtype != d.dtype || f->off != d.off || level < f->minL || level > f->maxL )
+            throw Error( HG_ERR_ARG, "multElementwise needs functions of one storage, space and value type" );
+      Lattice&      l = d.lat( ctx, level );
+      const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
+      if ( n == 0 )
+         return;
+      dispatchType( d.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         kMult< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, (const T*) fa.ptr( level ),
+                                                                (const T*) fb.ptr( level ), (T*) d.ptr( level ) );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+} // extern "C"

@@ -14,49 +14,6 @@ namespace hg {
 // ------------------------------------------------------------------------
 // device buffers
 // ------------------------------------------------------------------------
-template < class T >
-void DevBuf< T >::alloc( Context* c, size_t count )
-{
-   release();
-   ctx = c;
-   n   = count;
-   if ( count == 0 )
-      return;
-   HG_CHECK_HIP( hipMalloc( (void**) &p, count * sizeof( T ) ) );
-   HG_CHECK_HIP( hipMemsetAsync( p, 0, count * sizeof( T ), c->stream ) );
-   c->bytes += int64_t( count * sizeof( T ) );
-}
-template < class T >
-void DevBuf< T >::upload( Context* c, const std::vector< T >& h )
-{
-   alloc( c, h.size() );
-   if ( !h.empty() )
-   {
-      HG_CHECK_HIP( hipMemcpyAsync( p, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice, c->stream ) );
-      HG_CHECK_HIP( hipStreamSynchronize( c->stream ) );
-   }
-}
-template < class T >
-void DevBuf< T >::release()
-{
-   if ( p )
-   {
-      (void) hipFree( p );
-      if ( ctx )
-         ctx->bytes -= int64_t( n * sizeof( T ) );
-   }
-   p = nullptr;
-   n = 0;
-}
-template struct DevBuf< double >;
-template struct DevBuf< float >;
-template struct DevBuf< int64_t >;
-template struct DevBuf< int32_t >;
-template struct DevBuf< uint32_t >;
-template struct DevBuf< uint8_t >;
-template struct DevBuf< int8_t >;
-template struct DevBuf< CellMeta >;
-
 // ------------------------------------------------------------------------
 // storage / lattice set-up
 // ------------------------------------------------------------------------
@@ -648,26 +605,37 @@ int hg_p1_dof_flags( hg_ctx* h, hg_handle s, int level, int32_t* flags )
    } );
 }
 
-int hg_p1_function_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, hg_handle* out )
+static int functionCreate( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, int off, hg_handle* out )
 {
    return hg_guard( h, [&]( Context* ctx ) {
-      if ( !out || minLevel < 0 || maxLevel < minLevel || maxLevel > 12 || dtype < 0 || dtype > 2 )
+      if ( !out || minLevel < 0 || maxLevel < minLevel || maxLevel + off > 12 || dtype < 0 || dtype > 2 )
          throw Error( HG_ERR_ARG, "invalid function parameters" );
       Storage& st = ctx->storage( s );
       auto     f  = std::make_unique< Function >();
       f->st       = &st;
       f->dtype    = dtype;
+      f->off      = off;
       f->minL     = minLevel;
       f->maxL     = maxLevel;
       for ( int L = minLevel; L <= maxLevel; ++L )
       {
-         Lattice& l = st.lattice( ctx, L );
-         f->data[L].alloc( ctx, size_t( l.nstore ) * f->elemSize() );
+         Lattice& l = st.lattice( ctx, L + off );
+         f->data[L + off].alloc( ctx, size_t( l.nstore ) * f->elemSize() );
       }
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
       *out                  = ctx->nextHandle++;
       ctx->functions[*out] = std::move( f );
    } );
+}
+
+int hg_p1_function_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, hg_handle* out )
+{
+   return functionCreate( h, s, minLevel, maxLevel, dtype, 0, out );
+}
+
+int hg_p2_function_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, hg_handle* out )
+{
+   return functionCreate( h, s, minLevel, maxLevel, dtype, 1, out );
 }
 
 int hg_function_destroy( hg_ctx* h, hg_handle fn )
@@ -702,7 +670,7 @@ int hg_function_upload( hg_ctx* h, hg_handle fn, int level, const double* canoni
       checkLevel( f, level );
       if ( !canonical )
          throw Error( HG_ERR_ARG, "null data" );
-      Lattice& l = f.st->lattice( ctx, level );
+      Lattice& l = f.lat( ctx, level );
       dispatchType( f.dtype, [&]( auto tag ) {
          using T = decltype( tag );
          std::vector< T > hbuf( l.nstore, hostFrom< T >( 0.0 ) );
@@ -722,7 +690,7 @@ int hg_function_download( hg_ctx* h, hg_handle fn, int level, double* canonical 
       checkLevel( f, level );
       if ( !canonical )
          throw Error( HG_ERR_ARG, "null data" );
-      Lattice& l = f.st->lattice( ctx, level );
+      Lattice& l = f.lat( ctx, level );
       dispatchType( f.dtype, [&]( auto tag ) {
          using T = decltype( tag );
          std::vector< T > hbuf( l.nstore );
@@ -747,11 +715,11 @@ static int linComb( hg_ctx* h, hg_handle dst, int nsrc, const double* scalars, c
       {
          Function& s = ctx->function( srcs[i] );
          checkLevel( s, level );
-         if ( s.st != d.st || s.dtype != d.dtype )
+         if ( s.st != d.st || s.dtype != d.dtype || s.off != d.off )
             throw Error( HG_ERR_ARG, "assign/add need functions of one storage and one value type" );
          src[i] = s.ptr( level );
       }
-      Lattice&      l = d.st->lattice( ctx, level );
+      Lattice&      l = d.lat( ctx, level );
       const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
       if ( n == 0 )
          return;
@@ -783,7 +751,7 @@ int hg_function_interpolate_const( hg_ctx* h, hg_handle fn, double value, int le
    return hg_guard( h, [&]( Context* ctx ) {
       Function& f = ctx->function( fn );
       checkLevel( f, level );
-      Lattice&      l = f.st->lattice( ctx, level );
+      Lattice&      l = f.lat( ctx, level );
       const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
       if ( n == 0 )
          return;
@@ -805,9 +773,9 @@ int hg_function_dot( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, d
       Function& fb = ctx->function( b );
       checkLevel( fa, level );
       checkLevel( fb, level );
-      if ( fa.st != fb.st || fa.dtype != fb.dtype )
+      if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off )
          throw Error( HG_ERR_ARG, "dot needs functions of one storage and one value type" );
-      *out = dotDevice( ctx, fa.st->lattice( ctx, level ), fa.ptr( level ), fb.ptr( level ), fa.dtype, flag );
+      *out = dotDevice( ctx, fa.lat( ctx, level ), fa.ptr( level ), fb.ptr( level ), fa.dtype, flag );
    } );
 }
 
@@ -818,9 +786,9 @@ int hg_function_copy_from( hg_ctx* h, hg_handle dst, hg_handle src, int level )
       Function& s = ctx->function( src );
       checkLevel( d, level );
       checkLevel( s, level );
-      if ( d.st != s.st )
-         throw Error( HG_ERR_ARG, "copyFrom needs functions of one storage" );
-      Lattice&      l = d.st->lattice( ctx, level );
+      if ( d.st != s.st || d.off != s.off )
+         throw Error( HG_ERR_ARG, "copyFrom needs functions of one storage and one space" );
+      Lattice&      l = d.lat( ctx, level );
       const int64_t n = l.nstore;
       dispatchType( d.dtype, [&]( auto td ) {
          dispatchType( s.dtype, [&]( auto ts ) {
@@ -839,7 +807,7 @@ int hg_function_communicate( hg_ctx* h, hg_handle fn, int level )
    return hg_guard( h, [&]( Context* ctx ) {
       Function& f = ctx->function( fn );
       checkLevel( f, level );
-      pullHalos( ctx, *f.st, f.st->lattice( ctx, level ), f.ptr( level ), f.dtype );
+      pullHalos( ctx, *f.st, f.lat( ctx, level ), f.ptr( level ), f.dtype );
    } );
 }
 

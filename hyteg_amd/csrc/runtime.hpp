@@ -130,20 +130,26 @@ struct Storage
    void                                         init( Context* ctx );
 };
 
+// A function on levels [minL, maxL] of one storage.  P1 functions of level L
+// live on lattice level L; P2 functions of level L on lattice level L+1 (their
+// nodes are the micro-vertices and micro-edge midpoints of level L), so
+// `off` = 0 for P1 and 1 for P2.
 struct Function
 {
-   Storage*              st    = nullptr;
-   int                   dtype = 0; // 0 f64, 1 f32, 2 f16
-   int                   minL = 0, maxL = 0;
-   std::map< int, DevBuf< uint8_t > > data; // raw bytes, nstore elements
-   size_t                elemSize() const { return dtype == 0 ? 8 : ( dtype == 1 ? 4 : 2 ); }
-   void*                 ptr( int L )
+   Storage*                           st    = nullptr;
+   int                                dtype = 0; // 0 f64, 1 f32, 2 f16
+   int                                off   = 0;
+   int                                minL = 0, maxL = 0;
+   std::map< int, DevBuf< uint8_t > > data; // by lattice level, nstore elements
+   size_t                             elemSize() const { return dtype == 0 ? 8 : ( dtype == 1 ? 4 : 2 ); }
+   void*                              ptr( int L )
    {
-      auto it = data.find( L );
+      auto it = data.find( L + off );
       if ( it == data.end() )
          throw Error( -1, "function has no data on level " + std::to_string( L ) );
       return it->second.p;
    }
+   Lattice& lat( Context* ctx, int L ) { return st->lattice( ctx, L + off ); }
 };
 
 struct Context
@@ -181,6 +187,42 @@ struct Context
    void beginTiming();
    void endTiming();
 };
+
+// ---- DevBuf members (need the complete Context) ------------------------
+template < class T >
+inline void DevBuf< T >::alloc( Context* c, size_t count )
+{
+   release();
+   ctx = c;
+   n   = count;
+   if ( count == 0 )
+      return;
+   HG_CHECK_HIP( hipMalloc( (void**) &p, count * sizeof( T ) ) );
+   HG_CHECK_HIP( hipMemsetAsync( p, 0, count * sizeof( T ), c->stream ) );
+   c->bytes += int64_t( count * sizeof( T ) );
+}
+template < class T >
+inline void DevBuf< T >::upload( Context* c, const std::vector< T >& h )
+{
+   alloc( c, h.size() );
+   if ( !h.empty() )
+   {
+      HG_CHECK_HIP( hipMemcpyAsync( p, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice, c->stream ) );
+      HG_CHECK_HIP( hipStreamSynchronize( c->stream ) );
+   }
+}
+template < class T >
+inline void DevBuf< T >::release()
+{
+   if ( p )
+   {
+      (void) hipFree( p );
+      if ( ctx )
+         ctx->bytes -= int64_t( n * sizeof( T ) );
+   }
+   p = nullptr;
+   n = 0;
+}
 
 // ---- device helpers used across translation units ----------------------
 constexpr int kBlock      = 256;

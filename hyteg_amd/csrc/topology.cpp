@@ -259,3 +259,240 @@ int LevelNumbering::numNeighbourCells( const Topology& t, int64_t id ) const
 }
 
 } // namespace hg
+
+// ---------------------------------------------------------------------------
+// icosahedral spherical shell (mesh/MeshGenSphericalShell.cpp:38-700)
+// ---------------------------------------------------------------------------
+#include <cmath>
+
+namespace hg {
+namespace {
+
+using Vec3 = std::array< double, 3 >;
+
+struct UnitSphereDiamonds
+{
+   int                  ntan;
+   Vec3                 ico[12];
+   int                  diamond[10][4];
+   std::vector< Vec3 >  node; // [is1][is2][id]
+   Vec3&                at( int i1, int i2, int id ) { return node[( size_t( i1 ) * ntan + i2 ) * 10 + id]; }
+
+   explicit UnitSphereDiamonds( int n )
+   : ntan( n )
+   , node( size_t( n ) * n * 10 )
+   {
+      const double fifthpi = 0.4 * std::asin( 1.0 );
+      const double w       = 2.0 * std::acos( 1.0 / ( 2.0 * std::sin( fifthpi ) ) );
+      ico[0]               = { 0.0, 0.0, 1.0 };
+      ico[11]              = { 0.0, 0.0, -1.0 };
+      for ( int k = 1; k <= 5; ++k )
+      {
+         const double a = 2.0 * ( k - 0.5 ) * fifthpi, b = 2.0 * ( k - 1 ) * fifthpi;
+         ico[k]         = { std::sin( w ) * std::cos( a ), std::sin( w ) * std::sin( a ), std::cos( w ) };
+         ico[k + 5]     = { std::sin( w ) * std::cos( b ), std::sin( w ) * std::sin( b ), -std::cos( w ) };
+      }
+      const int d[10][4] = { { 0, 5, 6, 1 },  { 0, 1, 7, 2 },  { 0, 2, 8, 3 },  { 0, 3, 9, 4 },  { 0, 4, 10, 5 },
+                             { 11, 7, 1, 6 }, { 11, 8, 2, 7 }, { 11, 9, 3, 8 }, { 11, 10, 4, 9 }, { 11, 6, 5, 10 } };
+      std::copy( &d[0][0], &d[0][0] + 40, &diamond[0][0] );
+      for ( int id = 0; id < 10; ++id )
+      {
+         const int left = id < 5 ? 1 : 3, right = id < 5 ? 3 : 1;
+         at( 0, 0, id )               = ico[diamond[id][0]];
+         at( n - 1, 0, id )           = ico[diamond[id][left]];
+         at( n - 1, n - 1, id )       = ico[diamond[id][2]];
+         at( 0, n - 1, id )           = ico[diamond[id][right]];
+         // recursive midpoint refinement: rows, columns, diagonals per step
+         for ( int m = 1; m < n - 1; m *= 2 )
+         {
+            const int l = ( n - 1 ) / m, h = l / 2;
+            auto      mid = [&]( int i1, int i2, int a1, int a2, int b1, int b2 ) {
+               const Vec3& A = at( a1, a2, id );
+               const Vec3& B = at( b1, b2, id );
+               Vec3        s{ A[0] + B[0], A[1] + B[1], A[2] + B[2] };
+               const double r = std::sqrt( s[0] * s[0] + s[1] * s[1] + s[2] * s[2] );
+               at( i1, i2, id ) = { s[0] / r, s[1] / r, s[2] / r };
+            };
+            for ( int j1 = 0; j1 <= m; ++j1 )
+               for ( int j2 = 0; j2 < m; ++j2 )
+                  mid( j1 * l, j2 * l + h, j1 * l, j2 * l, j1 * l, j2 * l + l );
+            for ( int j1 = 0; j1 <= m; ++j1 )
+               for ( int j2 = 0; j2 < m; ++j2 )
+                  mid( j2 * l + h, j1 * l, j2 * l, j1 * l, j2 * l + l, j1 * l );
+            for ( int j1 = 0; j1 < m; ++j1 )
+               for ( int j2 = 0; j2 < m; ++j2 )
+                  mid( j1 * l + h, j2 * l + h, j1 * l, j2 * l + l, j1 * l + l, j2 * l );
+         }
+      }
+   }
+};
+
+// global vertex index of the tuple (is1, is2, diamond, layer), folding the
+// diamond's left edges onto their owning neighbours
+int shellVertexIndex( int ntan, int nrad, int is1, int is2, int id, int ir )
+{
+   if ( is1 == 0 && is2 == 0 )
+      return id < 5 ? ir : nrad + ir;
+   if ( is2 == 0 )
+   {
+      is2 = is1;
+      is1 = 0;
+      id  = id < 5 ? ( id + 4 ) % 5 : ( id + 4 ) % 5 + 5;
+   }
+   else if ( is1 == ntan - 1 )
+   {
+      is1 = ( ntan - 1 ) - is2;
+      is2 = ntan - 1;
+      id  = id < 5 ? ( id + 4 ) % 5 + 5 : id % 5;
+   }
+   const int q = ( ntan - 1 ) * ( ntan - 1 );
+   return 2 * nrad + ir * q * 10 + id * q + ( is2 - 1 ) * ( ntan - 1 ) + is1;
+}
+
+} // namespace
+
+Topology Topology::shell( int ntan, int nrad, double rmin, double rmax )
+{
+   if ( ntan < 2 || nrad < 2 || !( rmin > 0.0 && rmin < rmax ) )
+      throw std::invalid_argument( "invalid shell parameters" );
+   if ( ( ( ntan - 1 ) & ( ntan - 2 ) ) != 0 )
+      throw std::invalid_argument( "ntan - 1 must be a power of two" );
+   UnitSphereDiamonds  sph( ntan );
+   const int           q  = ( ntan - 1 ) * ( ntan - 1 );
+   const int           nv = ( 10 * q + 2 ) * nrad;
+   std::vector< double > X( 3 * size_t( nv ) );
+   for ( int j = 0; j < nv; ++j )
+   {
+      int is1 = 0, is2 = 0, id, ir;
+      if ( j < nrad )
+         id = 0, ir = j;
+      else if ( j < 2 * nrad )
+         id = 5, ir = j - nrad;
+      else
+      {
+         int r = j - 2 * nrad;
+         ir    = r / ( 10 * q );
+         r -= ir * 10 * q;
+         id = r / q;
+         r -= id * q;
+         is2 = r / ( ntan - 1 ) + 1;
+         is1 = r % ( ntan - 1 );
+      }
+      const double radius = rmin + ( ( rmax - rmin ) / ( nrad - 1 ) ) * ir;
+      const Vec3&  p      = sph.at( is1, is2, id );
+      for ( int d = 0; d < 3; ++d )
+         X[3 * j + d] = p[d] * radius;
+   }
+   // six tetrahedra per radial prism cell, node tables for both hemispheres
+   constexpr int corner[8][3] = { { 0, 0, 1 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 1, 1 },
+                                  { 0, 0, 0 }, { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 } };
+   constexpr int north[6][4]  = { { 0, 1, 3, 7 }, { 4, 7, 5, 0 }, { 0, 5, 1, 7 }, { 1, 2, 3, 6 }, { 5, 7, 6, 1 }, { 3, 6, 7, 1 } };
+   constexpr int south[6][4]  = { { 4, 7, 5, 1 }, { 0, 1, 3, 4 }, { 4, 3, 7, 1 }, { 7, 6, 5, 2 }, { 3, 1, 2, 7 }, { 5, 2, 1, 7 } };
+   std::vector< int64_t > cells;
+   const int              ncell = q * ( nrad - 1 ) * 60;
+   cells.reserve( 4 * size_t( ncell ) );
+   for ( int e = 0; e < ncell; ++e )
+   {
+      int r        = e;
+      const int ir = r / ( 60 * q );
+      r -= ir * 60 * q;
+      const int id = r / ( 6 * q );
+      r -= id * 6 * q;
+      const int is2 = r / ( 6 * ( ntan - 1 ) );
+      r -= is2 * 6 * ( ntan - 1 );
+      const int is1 = r / 6, it = r % 6;
+      int       v8[8];
+      for ( int k = 0; k < 8; ++k )
+      {
+         const int a = id <= 4 ? corner[k][0] : corner[k][1];
+         const int b = id <= 4 ? corner[k][1] : corner[k][0];
+         v8[k]       = shellVertexIndex( ntan, nrad, is1 + a, is2 + b, id, ir + corner[k][2] );
+      }
+      for ( int k = 0; k < 4; ++k )
+         cells.push_back( v8[id <= 4 ? north[it][k] : south[it][k]] );
+   }
+   Topology t = fromCells( nv, X.data(), ncell, cells.data() );
+   t.computeShellBlend();
+   return t;
+}
+
+// IcosahedralShellMap::classifyVertices (geometry/IcosahedralShellMap.hpp:529-780):
+// the two vertices on one radial ray, ordered by tet type; a "skew" tet (two
+// vertices on each sphere) takes the map of the tet across its first face that
+// does not contain both radial vertices.
+void Topology::computeShellBlend()
+{
+   blend.assign( nc, ShellCellMap{} );
+   auto classify = [&]( int c, ShellCellMap& out, auto& self, int depth ) -> void {
+      const auto& cvv = cv[c];
+      double      P[4][3], r[4], rin = 1e300, rout = 0.0;
+      for ( int k = 0; k < 4; ++k )
+      {
+         for ( int d = 0; d < 3; ++d )
+            P[k][d] = X[3 * cvv[k] + d];
+         r[k] = std::sqrt( P[k][0] * P[k][0] + P[k][1] * P[k][1] + P[k][2] * P[k][2] );
+         rin  = std::min( rin, r[k] );
+         rout = std::max( rout, r[k] );
+      }
+      int outer = 0;
+      for ( int k = 0; k < 4; ++k )
+         outer += ( rout - r[k] ) < ( rout - rin ) * 0.001 ? 1 : 0;
+      int ref = -1, ray = -1;
+      for ( int k = 0; k < 4 && ref < 0; ++k )
+         for ( int j = k + 1; j < 4 && ref < 0; ++j )
+         {
+            const double s = r[k] * r[j];
+            const double cr[3] = { P[k][1] * P[j][2] - P[k][2] * P[j][1], P[k][2] * P[j][0] - P[k][0] * P[j][2],
+                                   P[k][0] * P[j][1] - P[k][1] * P[j][0] };
+            if ( std::fabs( cr[0] ) / s > 1e-14 || std::fabs( cr[1] ) / s > 1e-14 || std::fabs( cr[2] ) / s > 1e-14 )
+               continue;
+            ref = k;
+            ray = j;
+         }
+      if ( ref < 0 )
+         throw std::runtime_error( "shell map: no radial vertex pair in cell" );
+      if ( outer == 2 )
+      {
+         if ( depth > 0 )
+            throw std::runtime_error( "shell map: skew neighbour of a skew cell" );
+         for ( int k = 0; k < 4; ++k )
+         {
+            const auto& f = fv[cf[c][k]];
+            const bool  hasRef = std::count( f.begin(), f.end(), cvv[ref] ) > 0;
+            const bool  hasRay = std::count( f.begin(), f.end(), cvv[ray] ) > 0;
+            if ( hasRef && hasRay )
+               continue;
+            for ( int o = 0; o < nc; ++o )
+               if ( o != c && std::count( cf[o].begin(), cf[o].end(), cf[c][k] ) > 0 )
+               {
+                  self( o, out, self, depth + 1 );
+                  return;
+               }
+         }
+         throw std::runtime_error( "shell map: no non-skew neighbour" );
+      }
+      int thr = -1, fo = -1;
+      for ( int k = 0; k < 4; ++k )
+         if ( k != ref && k != ray )
+            thr = k;
+      for ( int k = 0; k < 4; ++k )
+         if ( k != ref && k != ray && k != thr )
+            fo = k;
+      if ( ( outer == 1 && r[ref] < r[ray] ) || ( outer == 3 && r[ref] > r[ray] ) )
+         std::swap( ref, ray );
+      out.kind = 1;
+      for ( int d = 0; d < 3; ++d )
+      {
+         out.rayVertex[d] = P[ray][d];
+         out.refVertex[d] = P[ref][d];
+         out.thrVertex[d] = P[thr][d];
+         out.forVertex[d] = P[fo][d];
+      }
+      out.radRayVertex = r[ray];
+      out.radRefVertex = r[ref];
+   };
+   for ( int c = 0; c < nc; ++c )
+      classify( c, blend[c], classify, 0 );
+}
+
+} // namespace hg

@@ -52,6 +52,10 @@ struct Topology
 
    static Topology fromCells( int nv, const double* coords, int nc, const int64_t* cells );
    static Topology cuboid( const double lo[3], const double hi[3], int nx, int ny, int nz );
+   // MeshInfo::meshSphericalShell(ntan, nrad, rmin, rmax) (SHELLMESH_CLASSIC) and
+   // IcosahedralShellMap::setMap on every cell
+   static Topology shell( int ntan, int nrad, double rmin, double rmax );
+   void            computeShellBlend();
 
    int findEdge( int a, int b ) const;
    int findFace( int a, int b, int c ) const;

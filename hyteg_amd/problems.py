@@ -1,0 +1,30 @@
+"""Synthetic, seeded problem data of the benchmark configurations.
+
+* cube:  unit cube split into 6 macro-tetrahedra, RHS uniform[-1,1] on inner DoFs.
+* shell: thick spherical shell r in [0.55, 1], ntan = 5, 3 radial layers
+  (1920 macro-tetrahedra); coefficient
+      k(x) = exp( sum_{m=1}^{16} a_m sin(omega_m . x + phi_m) ),
+  a_m ~ N(0, 0.5^2), omega_m ~ N(0, 3^2 I), phi_m ~ U[0, 2 pi); RHS uniform[-1,1].
+"""
+import numpy as np
+
+SHELL = dict(ntan=5, nrad=3, rmin=0.55, rmax=1.0)
+
+
+def shell_coefficient(seed=0, modes=16, amplitude=0.5, frequency=3.0):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(0.0, amplitude, modes)
+    omega = rng.normal(0.0, frequency, (modes, 3))
+    phi = rng.uniform(0.0, 2.0 * np.pi, modes)
+    return a, omega, phi
+
+
+def k_values(a, omega, phi, x):
+    x = np.atleast_2d(x)
+    return np.exp((np.asarray(a)[None, :] * np.sin(x @ np.asarray(omega).T + np.asarray(phi)[None, :])).sum(axis=1))
+
+
+def uniform_rhs(flags, seed=0, inner=1):
+    """uniform[-1,1] on DoFs of type `inner`, 0 elsewhere"""
+    rng = np.random.default_rng(seed)
+    return np.where(flags == inner, rng.uniform(-1.0, 1.0, len(flags)), 0.0)

@@ -76,6 +76,16 @@ int hg_storage_cuboid( hg_ctx* ctx, const double* lower3, const double* upper3, 
  * given; vertex ids 0..nv-1 in MeshInfo id order), e.g. a Gmsh file
  * (mesh/GmshReaderForMSH22.hpp:60-170), same boundary flagging. */
 int hg_storage_create( hg_ctx* ctx, int nv, const double* coords, int nc, const int64_t* cells, hg_handle* out );
+/* MeshInfo::meshSphericalShell(ntan, nrad, rmin, rmax) with SHELLMESH_CLASSIC
+ * (mesh/MeshInfo.hpp:512-516, mesh/MeshGenSphericalShell.cpp:642-700) and
+ * IcosahedralShellMap::setMap on all cells (geometry/IcosahedralShellMap.hpp:491) */
+int hg_storage_shell( hg_ctx* ctx, int ntan, int nrad, double rmin, double rmax, hg_handle* out );
+/* per macro-cell shell map data: rayVertex, refVertex, thrVertex, forVertex,
+ * radRayVertex, radRefVertex (14 doubles per cell, IcosahedralShellMap.hpp:608-731) */
+int hg_storage_blend( hg_ctx* ctx, hg_handle storage, double* out14 );
+/* macro-cell vertex ids in local order (nc*4) and vertex coordinates (nv*3) */
+int hg_storage_cell_vertices( hg_ctx* ctx, hg_handle storage, int64_t* out );
+int hg_storage_vertex_coords( hg_ctx* ctx, hg_handle storage, double* out );
 /* nv, ne, nf, nc of the macro mesh (PrimitiveStorage::getNumberOf*) */
 int hg_storage_counts( hg_ctx* ctx, hg_handle storage, int64_t* out4 );
 /* number of P1 DoFs on a level in canonical order (numberOfGlobalDoFs,
@@ -85,9 +95,18 @@ int hg_p1_ndofs( hg_ctx* ctx, hg_handle storage, int level, int64_t* out );
 int hg_p1_dof_coords( hg_ctx* ctx, hg_handle storage, int level, double* xyz );
 int hg_p1_dof_flags( hg_ctx* ctx, hg_handle storage, int level, int32_t* flags );
 
+/* P2 DoFs of level L: the micro-vertices and micro-edge midpoints of level L,
+ * numbered like the P1 DoFs of level L+1 (P2Function = vertex + edge DoFs,
+ * p2functionspace/P2Function.hpp:35-74) */
+int hg_p2_ndofs( hg_ctx* ctx, hg_handle storage, int level, int64_t* out );
+int hg_p2_dof_coords( hg_ctx* ctx, hg_handle storage, int level, double* xyz );
+int hg_p2_dof_flags( hg_ctx* ctx, hg_handle storage, int level, int32_t* flags );
+
 /* ---- functions: P1Function<T> (p1functionspace/VertexDoFFunction.hpp:58-512) */
 /* P1Function<T>(name, storage, minLevel, maxLevel), all values zero */
 int hg_p1_function_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, hg_handle* out );
+/* P2Function<T>(name, storage, minLevel, maxLevel) (p2functionspace/P2Function.hpp:35-74) */
+int hg_p2_function_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, hg_handle* out );
 int hg_function_destroy( hg_ctx* ctx, hg_handle fn );
 /* storage handle a function lives on (Function::getStorage, functions/Function.hpp:35-80) */
 int hg_function_storage( hg_ctx* ctx, hg_handle fn, hg_handle* storage );
@@ -102,6 +121,8 @@ int hg_function_add( hg_ctx* ctx, hg_handle dst, int nsrc, const double* scalars
 int hg_function_interpolate_const( hg_ctx* ctx, hg_handle fn, double value, int level, int flag );
 /* dotGlobal(other, level, flag) -- FP64 accumulation (VertexDoFFunction.cpp:1710-1716) */
 int hg_function_dot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
+/* multElementwise({a, b}, level, flag): dst = a * b (VertexDoFFunction.cpp:1408) */
+int hg_function_mult( hg_ctx* ctx, hg_handle dst, hg_handle a, hg_handle b, int level, int flag );
 /* copyFrom<otherT>(other, level): cast copy between storage types (VertexDoFFunction.hpp:151) */
 int hg_function_copy_from( hg_ctx* ctx, hg_handle dst, hg_handle src, int level );
 /* communicate<Vertex,Edge>..<Face,Cell>: refresh every macro-cell halo copy from
@@ -123,6 +144,20 @@ int hg_p1_smooth_sor( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, doubl
 /* stencil weights of the cell interior (15, same order as
  * apply_3D_macrocell_vertexdof_to_vertexdof_replace.cpp:36-50) */
 int hg_p1_cell_stencil( hg_ctx* ctx, hg_handle op, int level, int cell, double* out15 );
+
+/* ---- P2 elementwise div(k grad) operator with blending -------------------- */
+/* P2ElementwiseOperator<forms::p2_div_k_grad_blending_q4>(storage, minLevel, maxLevel, form)
+ * (elementwiseoperators/P2ElementwiseOperator.hpp:61,475; form
+ * forms/form_hyteg_generated/p2/p2_div_k_grad_blending_q4.hpp) with the
+ * coefficient k(x) = exp(sum_m a[m] sin(omega[3m..3m+2] . x + phi[m])), nm <= 32
+ * (nm = 0: k = 1), evaluated at the blended quadrature points. */
+int hg_p2_divkgrad_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int nm, const double* a,
+                           const double* omega, const double* phi, hg_handle* out );
+/* Operator::apply = gemv(1, src, 0|1, dst, level, flag) (P2ElementwiseOperator.cpp:110-328) */
+int hg_p2_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update );
+/* OperatorWithInverseDiagonal::getInverseDiagonalValues on `level`, written
+ * into the P2 function dst (P2ElementwiseOperator.cpp:418-568) */
+int hg_p2_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, hg_handle dst );
 
 /* ---- grid transfer ------------------------------------------------------ */
 /* P1toP1LinearRestriction::restrict(f, sourceLevel, flag)

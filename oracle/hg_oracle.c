@@ -22,40 +22,12 @@
  * null-space and energy identities, R = P^T adjointness, linear reproduction,
  * direct-solve agreement, P1GMG3DConvergenceTest's rate bound.
  */
-#include "hg_oracle.h"
+#include "hg_oracle_internal.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
-/* ------------------------------------------------------------------------- */
-/* lattice helpers                                                            */
-/* ------------------------------------------------------------------------- */
-static inline int64_t tri( int64_t n ) { return n <= 0 ? 0 : n * ( n + 1 ) / 2; }
-static inline int64_t tet( int64_t n ) { return n <= 0 ? 0 : n * ( n + 1 ) * ( n + 2 ) / 6; }
-/* MacroCellIndexing.hpp:40-53, width n (points per edge) */
-static inline int64_t tri_index( int64_t n, int64_t x, int64_t y ) { return y * n - y * ( y - 1 ) / 2 + x; }
-static inline int64_t tet_index( int64_t n, int64_t x, int64_t y, int64_t z )
-{
-   return tet( n ) - tet( n - z ) + y * ( n - z ) - y * ( y - 1 ) / 2 + x;
-}
-
-/* 6 micro-cell types (CellDoFIndexing.hpp:38-60, 150-200) */
-static const int VOF[6][4][3] = {
-    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, /* WHITE_UP   */
-    { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } }, /* BLUE_UP    */
-    { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, /* GREEN_UP   */
-    { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } }, /* WHITE_DOWN */
-    { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } }, /* BLUE_DOWN  */
-    { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } }, /* GREEN_DOWN */
-};
-/* micro-cells per row by type (CellDoFIndexing.hpp:64-83): m - WSHIFT */
-static const int WSHIFT[6] = { 0, 1, 1, 2, 1, 1 };
-
-/* 15-point stencil offsets (apply_3D_macrocell_vertexdof_to_vertexdof_replace.cpp:36-50) */
-static const int OFF15[15][3] = { { 0, 0, 0 },  { -1, 0, 0 }, { -1, 0, 1 }, { -1, 1, -1 }, { -1, 1, 0 },
-                                  { 0, -1, 0 }, { 0, -1, 1 }, { 0, 0, -1 }, { 0, 0, 1 },   { 0, 1, -1 },
-                                  { 0, 1, 0 },  { 1, -1, 0 }, { 1, -1, 1 }, { 1, 0, -1 },  { 1, 0, 0 } };
 static int off_slot( int dx, int dy, int dz )
 {
    for ( int k = 0; k < 15; ++k )
@@ -63,26 +35,6 @@ static int off_slot( int dx, int dy, int dz )
          return k;
    return -1;
 }
-
-/* local edges / faces of a cell (SetupPrimitiveStorage.cpp:290-300) */
-static const int LEDGE[6][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 0, 3 }, { 1, 3 }, { 2, 3 } };
-static const int LFACE[4][3] = { { 0, 1, 2 }, { 0, 1, 3 }, { 0, 2, 3 }, { 1, 2, 3 } };
-
-/* ------------------------------------------------------------------------- */
-/* mesh                                                                       */
-/* ------------------------------------------------------------------------- */
-struct or_mesh
-{
-   int     nv, ne, nf, nc;
-   double* X;
-   int*    ev; /* ne*2, sorted */
-   int*    fv; /* nf*3, sorted */
-   int*    cv; /* nc*4, local order */
-   int*    ce; /* nc*6 */
-   int*    cf; /* nc*4 */
-   int *   vflag, *eflag, *fflag;
-   int *   vnc, *enc, *fnc;
-};
 
 static int cmp_i2( const void* a, const void* b )
 {
@@ -313,6 +265,7 @@ void or_mesh_free( or_mesh* m )
    free( m->vnc );
    free( m->enc );
    free( m->fnc );
+   free( m->blend );
    free( m );
 }
 
@@ -332,14 +285,7 @@ void or_mesh_cell_vertices( const or_mesh* m, int64_t* out )
 /* ------------------------------------------------------------------------- */
 /* canonical numbering                                                        */
 /* ------------------------------------------------------------------------- */
-typedef struct
-{
-   int64_t mm;                         /* 2^L micro-edges per macro-edge */
-   int64_t eoff, foff, coff, n;        /* block offsets and total */
-   int64_t ein, fin, cin;              /* interior points per edge/face/cell */
-} lvl_t;
-
-static lvl_t lvl_info( const or_mesh* m, int L )
+lvl_t or_lvl_info( const or_mesh* m, int L )
 {
    lvl_t l;
    l.mm   = (int64_t) 1 << L;
@@ -353,8 +299,8 @@ static lvl_t lvl_info( const or_mesh* m, int L )
    return l;
 }
 
-int64_t or_p1_ndofs( const or_mesh* m, int level ) { return lvl_info( m, level ).n; }
-int64_t or_p1_nbnd( const or_mesh* m, int level ) { return lvl_info( m, level ).coff; }
+int64_t or_p1_ndofs( const or_mesh* m, int level ) { return or_lvl_info( m, level ).n; }
+int64_t or_p1_nbnd( const or_mesh* m, int level ) { return or_lvl_info( m, level ).coff; }
 
 static int local_of( const int* cv, int g )
 {
@@ -364,7 +310,7 @@ static int local_of( const int* cv, int g )
    return -1;
 }
 
-static int64_t locate_l( const or_mesh* m, const lvl_t* l, int c, int64_t x, int64_t y, int64_t z )
+int64_t or_locate_l( const or_mesh* m, const lvl_t* l, int c, int64_t x, int64_t y, int64_t z )
 {
    const int64_t t[4] = { l->mm - x - y - z, x, y, z };
    int           S[4], ns = 0;
@@ -400,8 +346,8 @@ static int64_t locate_l( const or_mesh* m, const lvl_t* l, int c, int64_t x, int
 
 int64_t or_p1_locate( const or_mesh* m, int level, int c, int x, int y, int z )
 {
-   lvl_t l = lvl_info( m, level );
-   return locate_l( m, &l, c, x, y, z );
+   lvl_t l = or_lvl_info( m, level );
+   return or_locate_l( m, &l, c, x, y, z );
 }
 
 /* owner kind of a canonical id: 0 vertex, 1 edge, 2 face, 3 cell; *prim = index */
@@ -426,7 +372,7 @@ static int owner_of( const or_mesh* m, const lvl_t* l, int64_t id, int64_t* prim
    return 3;
 }
 
-static int owner_flag( const or_mesh* m, const lvl_t* l, int64_t id )
+int or_owner_flag( const or_mesh* m, const lvl_t* l, int64_t id )
 {
    int64_t p;
    switch ( owner_of( m, l, id, &p ) )
@@ -443,7 +389,7 @@ static int owner_flag( const or_mesh* m, const lvl_t* l, int64_t id )
 }
 
 /* 1 / number of cells adjacent to the primitive owning id */
-static double inv_ncells( const or_mesh* m, const lvl_t* l, int64_t id )
+double or_inv_ncells( const or_mesh* m, const lvl_t* l, int64_t id )
 {
    int64_t p;
    switch ( owner_of( m, l, id, &p ) )
@@ -488,7 +434,7 @@ static int owner_vertices( const or_mesh* m, const lvl_t* l, int64_t id, int* vs
 
 void or_p1_coords( const or_mesh* m, int level, double* xyz )
 {
-   lvl_t         l  = lvl_info( m, level );
+   lvl_t         l  = or_lvl_info( m, level );
    const double* X  = m->X;
    double        mm = (double) l.mm;
    for ( int v = 0; v < m->nv; ++v )
@@ -535,9 +481,9 @@ void or_p1_coords( const or_mesh* m, int level, double* xyz )
 
 void or_p1_flags( const or_mesh* m, int level, int32_t* flags )
 {
-   lvl_t l = lvl_info( m, level );
+   lvl_t l = or_lvl_info( m, level );
    for ( int64_t i = 0; i < l.n; ++i )
-      flags[i] = owner_flag( m, &l, i );
+      flags[i] = or_owner_flag( m, &l, i );
 }
 
 /* ------------------------------------------------------------------------- */
@@ -649,7 +595,7 @@ or_p1op* or_p1op_create( const or_mesh* m, int level )
    or_p1op* op = calloc( 1, sizeof( or_p1op ) );
    op->m       = m;
    op->L       = level;
-   op->l       = lvl_info( m, level );
+   op->l       = or_lvl_info( m, level );
    const int64_t mm = op->l.mm;
    op->w            = mm + 1;
    op->sc           = tet( op->w );
@@ -658,11 +604,11 @@ or_p1op* or_p1op_create( const or_mesh* m, int level )
       for ( int64_t z = 0; z <= mm; ++z )
          for ( int64_t y = 0; y + z <= mm; ++y )
             for ( int64_t x = 0; x + y + z <= mm; ++x )
-               op->cmap[c * op->sc + lat( op->w, x, y, z )] = locate_l( m, &op->l, c, x, y, z );
+               op->cmap[c * op->sc + lat( op->w, x, y, z )] = or_locate_l( m, &op->l, c, x, y, z );
 
    /* cell stencils: sum element matrices of the 24 micro-cells around an inner
     * point (P1Elements.hpp:627-700) */
-   op->st = calloc( 15 * m->nc, sizeof( double ) );
+   op->st = calloc( 15 * (size_t) m->nc, sizeof( double ) );
    for ( int c = 0; c < m->nc; ++c )
    {
       double K[6][4][4];
@@ -915,9 +861,9 @@ void or_p1_sor( const or_p1op* op, double* x, const double* b, int flag, double 
 /* ------------------------------------------------------------------------- */
 /* transfer                                                                   */
 /* ------------------------------------------------------------------------- */
-static int64_t* build_cmap( const or_mesh* m, int L, int64_t* sc_out )
+int64_t* or_build_cmap( const or_mesh* m, int L, int64_t* sc_out )
 {
-   lvl_t         l  = lvl_info( m, L );
+   lvl_t         l  = or_lvl_info( m, L );
    const int64_t w  = l.mm + 1;
    const int64_t sc = tet( w );
    int64_t*      cm = malloc( sizeof( int64_t ) * m->nc * sc );
@@ -925,7 +871,7 @@ static int64_t* build_cmap( const or_mesh* m, int L, int64_t* sc_out )
       for ( int64_t z = 0; z <= l.mm; ++z )
          for ( int64_t y = 0; y + z <= l.mm; ++y )
             for ( int64_t x = 0; x + y + z <= l.mm; ++x )
-               cm[c * sc + lat( w, x, y, z )] = locate_l( m, &l, c, x, y, z );
+               cm[c * sc + lat( w, x, y, z )] = or_locate_l( m, &l, c, x, y, z );
    *sc_out = sc;
    return cm;
 }
@@ -933,17 +879,17 @@ static int64_t* build_cmap( const or_mesh* m, int L, int64_t* sc_out )
 void or_p1_restrict( const or_mesh* m, int fineLevel, const double* fine, double* coarse, int flag )
 {
    const int     Lc = fineLevel - 1;
-   lvl_t         lf = lvl_info( m, fineLevel ), lc = lvl_info( m, Lc );
+   lvl_t         lf = or_lvl_info( m, fineLevel ), lc = or_lvl_info( m, Lc );
    int64_t       scf, scc;
-   int64_t*      cmf = build_cmap( m, fineLevel, &scf );
-   int64_t*      cmc = build_cmap( m, Lc, &scc );
+   int64_t*      cmf = or_build_cmap( m, fineLevel, &scf );
+   int64_t*      cmc = or_build_cmap( m, Lc, &scc );
    const int64_t wf = lf.mm + 1, wc = lc.mm + 1;
    double*       acc = calloc( lc.n, sizeof( double ) );
    double*       invn = malloc( sizeof( double ) * scf );
    for ( int c = 0; c < m->nc; ++c )
    {
       for ( int64_t i = 0; i < scf; ++i )
-         invn[i] = inv_ncells( m, &lf, cmf[c * scf + i] );
+         invn[i] = or_inv_ncells( m, &lf, cmf[c * scf + i] );
       for ( int64_t z = 0; z <= lc.mm; ++z )
          for ( int64_t y = 0; y + z <= lc.mm; ++y )
             for ( int64_t x = 0; x + y + z <= lc.mm; ++x )
@@ -963,7 +909,7 @@ void or_p1_restrict( const or_mesh* m, int fineLevel, const double* fine, double
             }
    }
    for ( int64_t p = 0; p < lc.n; ++p )
-      if ( p >= lc.coff || ( owner_flag( m, &lc, p ) & flag ) )
+      if ( p >= lc.coff || ( or_owner_flag( m, &lc, p ) & flag ) )
          coarse[p] = acc[p];
    free( acc );
    free( invn );
@@ -974,10 +920,10 @@ void or_p1_restrict( const or_mesh* m, int fineLevel, const double* fine, double
 void or_p1_prolongate( const or_mesh* m, int coarseLevel, const double* coarse, double* fine, int flag, int add )
 {
    const int     Lf = coarseLevel + 1;
-   lvl_t         lf = lvl_info( m, Lf ), lc = lvl_info( m, coarseLevel );
+   lvl_t         lf = or_lvl_info( m, Lf ), lc = or_lvl_info( m, coarseLevel );
    int64_t       scf, scc;
-   int64_t*      cmf = build_cmap( m, Lf, &scf );
-   int64_t*      cmc = build_cmap( m, coarseLevel, &scc );
+   int64_t*      cmf = or_build_cmap( m, Lf, &scf );
+   int64_t*      cmc = or_build_cmap( m, coarseLevel, &scc );
    const int64_t wf = lf.mm + 1, wc = lc.mm + 1;
    double*       acc = calloc( lf.n, sizeof( double ) );
    for ( int c = 0; c < m->nc; ++c )
@@ -988,18 +934,18 @@ void or_p1_prolongate( const or_mesh* m, int coarseLevel, const double* coarse, 
                const double  v  = coarse[cmc[c * scc + lat( wc, x, y, z )]];
                const int64_t fx = 2 * x, fy = 2 * y, fz = 2 * z;
                int64_t       id = cmf[c * scf + lat( wf, fx, fy, fz )];
-               acc[id] += inv_ncells( m, &lf, id ) * v;
+               acc[id] += or_inv_ncells( m, &lf, id ) * v;
                for ( int s = 1; s < 15; ++s )
                {
                   const int64_t qx = fx + OFF15[s][0], qy = fy + OFF15[s][1], qz = fz + OFF15[s][2];
                   if ( qx < 0 || qy < 0 || qz < 0 || qx + qy + qz > lf.mm )
                      continue;
                   id = cmf[c * scf + lat( wf, qx, qy, qz )];
-                  acc[id] += 0.5 * inv_ncells( m, &lf, id ) * v;
+                  acc[id] += 0.5 * or_inv_ncells( m, &lf, id ) * v;
                }
             }
    for ( int64_t p = 0; p < lf.n; ++p )
-      if ( p >= lf.coff || ( owner_flag( m, &lf, p ) & flag ) )
+      if ( p >= lf.coff || ( or_owner_flag( m, &lf, p ) & flag ) )
          fine[p] = add ? fine[p] + acc[p] : acc[p];
    free( acc );
    free( cmf );
@@ -1018,7 +964,7 @@ double or_dot( int64_t n, const double* a, const double* b, const int32_t* flags
    return s;
 }
 
-static void axpby_f( int64_t n, double alpha, const double* x, double beta, double* y, const int32_t* flags, int flag )
+void or_axpby_f( int64_t n, double alpha, const double* x, double beta, double* y, const int32_t* flags, int flag )
 {
    for ( int64_t i = 0; i < n; ++i )
       if ( flags[i] & flag )
@@ -1055,8 +1001,8 @@ int or_p1_cg( const or_p1op* op, double* x, const double* b, int flag, int maxIt
       or_p1_apply( op, p, ap, flag, 0 );
       double pAp   = or_dot( n, p, ap, fl, flag );
       double alpha = prsold / pAp;
-      axpby_f( n, alpha, p, 1.0, x, fl, flag );
-      axpby_f( n, -alpha, ap, 1.0, r, fl, flag );
+      or_axpby_f( n, alpha, p, 1.0, x, fl, flag );
+      or_axpby_f( n, -alpha, ap, 1.0, r, fl, flag );
       double rsnew  = or_dot( n, r, r, fl, flag );
       double relRes = sqrt( rsnew ) / res_start;
       if ( relRes < relTol || sqrt( rsnew ) < absTol )
@@ -1066,7 +1012,7 @@ int or_p1_cg( const or_p1op* op, double* x, const double* b, int flag, int maxIt
       }
       double prsnew = rsnew;
       double beta   = prsnew / prsold;
-      axpby_f( n, 1.0, r, beta, p, fl, flag );
+      or_axpby_f( n, 1.0, r, beta, p, fl, flag );
       prsold = prsnew;
    }
    free( p );
@@ -1094,10 +1040,10 @@ static void gmg_rec( const or_mesh* m, glevel_t* lv, int minL, int L, int pre, i
    for ( int i = 0; i < pre; ++i )
       or_p1_sor( c->op, c->x, c->b, flag, 1.0 );
    or_p1_apply( c->op, c->x, c->tmp, flag, 0 );
-   axpby_f( c->n, 1.0, c->b, -1.0, c->tmp, c->op->flags, flag );
+   or_axpby_f( c->n, 1.0, c->b, -1.0, c->tmp, c->op->flags, flag );
    glevel_t* d = &lv[L - 1];
    or_p1_restrict( m, L, c->tmp, d->tmp, flag );
-   axpby_f( d->n, 1.0, d->tmp, 0.0, d->b, d->op->flags, flag );
+   or_axpby_f( d->n, 1.0, d->tmp, 0.0, d->b, d->op->flags, flag );
    memset( d->x, 0, sizeof( double ) * d->n );
    gmg_rec( m, lv, minL, L - 1, pre, post, cmaxit, ctol );
    or_p1_prolongate( m, L - 1, d->x, c->x, flag, 1 );
@@ -1123,7 +1069,7 @@ int or_p1_gmg( const or_mesh* m, int minLevel, int maxLevel, double* x, const do
    /* residual r = b - A x on Inner DoFs */
 #define RESID()                                                                   \
    ( or_p1_apply( f->op, f->x, r, OR_INNER, 0 ),                                  \
-     axpby_f( f->n, 1.0, f->b, -1.0, r, f->op->flags, OR_INNER ),                 \
+     or_axpby_f( f->n, 1.0, f->b, -1.0, r, f->op->flags, OR_INNER ),                 \
      sqrt( or_dot( f->n, r, r, f->op->flags, OR_INNER ) ) )
    double r0 = RESID();
    if ( res_hist )

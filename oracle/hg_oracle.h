@@ -81,6 +81,32 @@ int or_p1_cg( const or_p1op* op, double* x, const double* b, int flag, int maxIt
 int or_p1_gmg( const or_mesh* m, int minLevel, int maxLevel, double* x, const double* b, int pre, int post,
                int cycles, double tol, double* res_hist, int coarseMaxIter, double coarseTol );
 
+/* ---- P2 on the icosahedral shell ----------------------------------------- */
+typedef struct or_p2op or_p2op;
+
+/* MeshInfo::meshSphericalShell(ntan, nrad, rmin, rmax) (SHELLMESH_CLASSIC) with
+ * IcosahedralShellMap blending on every cell */
+or_mesh* or_mesh_shell( int ntan, int nrad, double rmin, double rmax );
+/* per cell: ray[3], ref[3], thr[3], for[3], radRay, radRef */
+void or_mesh_blend( const or_mesh* m, double* out14 );
+/* F(x) and DF(x) (row-major 3x3, may be NULL) of one cell's shell map */
+void or_shell_map( const double* blend14, const double* x, double* Fx, double* DF );
+/* the 11 quadrature points (barycentric, 11x4) and weights (reference tet, sum 1/6) */
+void or_quadrature( double* pts44, double* w11 );
+/* 10x10 element matrix of div(k grad) with k = exp(sum_m a_m sin(om_m.x + ph_m))
+ * (nm = 0: k = 1); p = 4 vertex coordinates in computational space; blend may be NULL */
+void or_p2_element_matrix( const double* p, const double* blend, int nm, const double* ka, const double* kom,
+                           const double* kph, double* A );
+
+/* P2ElementwiseOperator<p2_div_k_grad_blending_q4> on P2 level `level`; its
+ * DoFs are the canonical P1 DoFs of level+1 */
+or_p2op* or_p2op_create( const or_mesh* m, int level, int nm, const double* ka, const double* kom, const double* kph );
+void     or_p2op_free( or_p2op* op );
+void     or_p2_apply( const or_p2op* op, const double* src, double* dst, int flag, int add );
+void     or_p2_inv_diag( const or_p2op* op, double* out );
+int64_t  or_p2op_nelem( const or_p2op* op );
+void     or_p2op_coo( const or_p2op* op, int64_t* rows, int64_t* cols, double* vals );
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,17 @@ def lib():
             "or_dot": (_d, [_i64, _p, _p, _p, _i]),
             "or_p1_cg": (_i, [_p, _p, _p, _i, _i, _d, _d]),
             "or_p1_gmg": (_i, [_p, _i, _i, _p, _p, _i, _i, _i, _d, _p, _i, _d]),
+            "or_mesh_shell": (_p, [_i, _i, _d, _d]),
+            "or_mesh_blend": (None, [_p, _p]),
+            "or_shell_map": (None, [_p, _p, _p, _p]),
+            "or_quadrature": (None, [_p, _p]),
+            "or_p2_element_matrix": (None, [_p, _p, _i, _p, _p, _p, _p]),
+            "or_p2op_create": (_p, [_p, _i, _i, _p, _p, _p]),
+            "or_p2op_free": (None, [_p]),
+            "or_p2_apply": (None, [_p, _p, _p, _i, _i]),
+            "or_p2_inv_diag": (None, [_p, _p]),
+            "or_p2op_nelem": (_i64, [_p]),
+            "or_p2op_coo": (None, [_p, _p, _p, _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -76,6 +87,15 @@ class Mesh:
     @staticmethod
     def cuboid(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(1, 1, 1)):
         return Mesh(lib().or_mesh_cuboid(*lo, *hi, *n))
+
+    @staticmethod
+    def shell(ntan=5, nrad=3, rmin=0.55, rmax=1.0):
+        return Mesh(lib().or_mesh_shell(ntan, nrad, rmin, rmax))
+
+    def blend(self):
+        out = np.zeros((self.counts()[3], 14))
+        lib().or_mesh_blend(self.h, _ptr(out))
+        return out
 
     @staticmethod
     def from_arrays(coords, cells):
@@ -166,6 +186,86 @@ class P1Op:
         vals = np.zeros(nnz)
         lib().or_p1op_csr(self.h, _ptr(rp), _ptr(cols), _ptr(vals))
         return sp.csr_matrix((vals, cols, rp), shape=(self.n, self.n))
+
+
+class KCoeff:
+    """k(x) = exp(sum_m a_m sin(omega_m . x + phi_m)); empty means k = 1"""
+
+    def __init__(self, a=(), omega=(), phi=()):
+        self.a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        self.omega = np.ascontiguousarray(omega, dtype=np.float64).reshape(-1, 3)
+        self.phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        self.nm = len(self.a)
+
+    def __call__(self, x):
+        x = np.atleast_2d(x)
+        if self.nm == 0:
+            return np.ones(len(x))
+        return np.exp((self.a[None, :] * np.sin(x @ self.omega.T + self.phi[None, :])).sum(axis=1))
+
+    def args(self):
+        e = np.zeros(1)
+        if self.nm == 0:
+            return 0, _ptr(e), _ptr(e), _ptr(e)
+        return self.nm, _ptr(self.a), _ptr(self.omega), _ptr(self.phi)
+
+
+def shell_map(blend14, x):
+    blend14 = np.ascontiguousarray(blend14, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    F = np.zeros(3)
+    DF = np.zeros(9)
+    lib().or_shell_map(_ptr(blend14), _ptr(x), _ptr(F), _ptr(DF))
+    return F, DF.reshape(3, 3)
+
+
+def quadrature():
+    pts = np.zeros((11, 4))
+    w = np.zeros(11)
+    lib().or_quadrature(_ptr(pts), _ptr(w))
+    return pts, w
+
+
+def p2_element_matrix(p, blend=None, k=None):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    k = k or KCoeff()
+    A = np.zeros((10, 10))
+    b = None if blend is None else np.ascontiguousarray(blend, dtype=np.float64)
+    lib().or_p2_element_matrix(_ptr(p), None if b is None else _ptr(b), *k.args(), _ptr(A))
+    return A
+
+
+class P2Op:
+    def __init__(self, mesh, level, k=None):
+        self.mesh = mesh
+        self.level = level
+        self.k = k or KCoeff()
+        self.h = lib().or_p2op_create(mesh.h, level, *self.k.args())
+        self.n = mesh.ndofs(level + 1)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.or_p2op_free(self.h)
+            self.h = None
+
+    def apply(self, src, dst, flag, add=False):
+        lib().or_p2_apply(self.h, _ptr(src), _ptr(dst), flag, int(add))
+
+    def inv_diag(self):
+        out = np.zeros(self.n)
+        lib().or_p2_inv_diag(self.h, _ptr(out))
+        return out
+
+    def matrix(self):
+        import scipy.sparse as sp
+        ne = int(lib().or_p2op_nelem(self.h))
+        r = np.zeros(100 * ne, dtype=np.int64)
+        c = np.zeros(100 * ne, dtype=np.int64)
+        v = np.zeros(100 * ne)
+        lib().or_p2op_coo(self.h, _ptr(r), _ptr(c), _ptr(v))
+        A = sp.csr_matrix((v, (r, c)), shape=(self.n, self.n))
+        A.sum_duplicates()
+        return A
 
 
 def dot(a, b, flags, flag):
