@@ -1,13 +1,15 @@
-"""Benchmark of the matrix-free P1 operator / multigrid hot path on one MI355X.
+"""Benchmark of the matrix-free operator / multigrid hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--level L]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload p2_shell|p1_cube] [--level L]
 
-Step = one P1 constant-stencil Laplace apply (FP64) on the unit cube split into
-6 macro-tetrahedra at refinement level L (default 9: 135,005,697 DoFs), inputs
-resident in HBM.  Prints ONE JSON line: DoF/s per apply, the roofline of the
-dominant kernel (cell-interior stencil sweep, HIP-event timed on the library's
-stream), the CPU oracle timed on this host, and the FP64 GMG time-to-1e-8 on the
-cube at level 5 (V(3,3), GS, CG coarse).
+Default step: one application of the P2 variable-coefficient diffusion
+operator div(k grad) (P2ElementwiseOperator with the blended
+p2_div_k_grad_blending_q4 form, FP64, exact reference formulation) on the thick
+spherical shell r in [0.55, 1] (ntan 5, 3 radial layers, 1920 macro-cells) at
+level 6: 673,710,594 DoFs, inputs resident in HBM.  Prints ONE JSON line with
+DoF/s per apply, the roofline of the dominant kernel (HIP-event timed on the
+library's stream), the CPU oracle timed on this host, and secondary results
+(P1 apply on the cube, FP64 GMG time-to-1e-8 on the cube at level 5).
 """
 import argparse
 import json
@@ -21,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "DoF/s per apply; time-to-1e-8 residual mixed vs FP64; solves/s + p99 latency"
 
 
 def tet(n):
@@ -28,19 +31,93 @@ def tet(n):
 
 
 def load_traffic(key):
-    path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
-        return json.load(open(path)).get(key)
+        return json.load(open(os.path.join(ROOT, "profiles", "traffic.json"))).get(key)
     except (OSError, ValueError):
         return None
 
 
-def cpu_baseline(level, seconds):
-    """oracle (C restatement of the reference's CPU path), all host cores"""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_applies(ctx, apply, steps, warmup):
+    for _ in range(warmup):
+        apply()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        apply()
+    ctx.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def kernel_ms(ctx, apply, reps):
+    ctx.event_timing(True)
+    ms = []
+    for _ in range(reps):
+        apply()
+        ms.append(ctx.last_ms())
+    ctx.event_timing(False)
+    return float(np.mean(ms))
+
+
+def p2_shell(hg, ctx, L, steps, warmup):
+    from hyteg_amd import problems
+    st = hg.PrimitiveStorage.shell(ctx, **problems.SHELL)
+    a, om, ph = problems.shell_coefficient(0)
+    A = hg.P2DivKGradOperator(st, L, L, a, om, ph)
+    n = st.p2_ndofs(L)
+    src = hg.P2Function("src", st, L, L)
+    dst = hg.P2Function("dst", st, L, L)
+    src.interpolate_random(1, -1.0, 1.0, L)
+    log(f"P2 shell level {L}: {n} DoFs")
+    apply = lambda: A.apply(src, dst, L, hg.ALL)  # noqa: E731
+    sec = timed_applies(ctx, apply, steps, warmup)
+    kms = kernel_ms(ctx, apply, max(3, min(steps, 5)))
+    # P2 DoFs updated by kP2ElementApply: all lattice points of level L+1 in the cells
+    # (src read + dst written once each, FP64); k is evaluated, not streamed
+    alg = 2 * 8 * n
+    return n, sec, kms, alg, {"workload": f"P2 div(k grad) apply (blended shell map, 11-point quadrature, "
+                                          f"k = exp(16 sine modes), FP64), spherical shell r 0.55..1, ntan 5, "
+                                          f"3 layers (1920 macro-cells), level {L} ({n} DoFs)",
+                              "level": L, "dofs": n, "mesh": "meshSphericalShell(5, 3, 0.55, 1.0)"}, \
+        "kP2ElementApply", f"p2_apply_shell_L{L}_f64"
+
+
+def p1_cube(hg, ctx, L, steps, warmup):
+    st = hg.PrimitiveStorage.cuboid(ctx)
+    A = hg.P1ConstantLaplaceOperator(st, L, L)
+    n = st.ndofs(L)
+    src = hg.P1Function("src", st, L, L)
+    dst = hg.P1Function("dst", st, L, L)
+    src.interpolate_random(1, -1.0, 1.0, L)
+    log(f"P1 cube level {L}: {n} DoFs")
+    apply = lambda: A.apply(src, dst, L, hg.ALL)  # noqa: E731
+    sec = timed_applies(ctx, apply, steps, warmup)
+    kms = kernel_ms(ctx, apply, max(5, min(steps, 20)))
+    alg = 2 * 8 * 6 * tet(2 ** L - 3)
+    return n, sec, kms, alg, {"workload": f"P1ConstantLaplaceOperator apply, unit cube 6 macro-tets, level {L} "
+                                          f"({n} DoFs), FP64", "level": L, "dofs": n, "mesh": "meshCuboid 1x1x1"}, \
+        "kCellApply<double>", f"p1_apply_cube_L{L}_f64"
+
+
+def cpu_baseline(workload, seconds):
+    """oracle (C restatement of the reference's CPU path), all host cores, bounded sample"""
     from oracle import pyoracle as po
-    m = po.Mesh.cuboid()
-    op = po.P1Op(m, level)
-    n = m.ndofs(level)
+    from hyteg_amd import problems
+    if workload == "p2_shell":
+        L = 3
+        m = po.Mesh.shell(**problems.SHELL)
+        op = po.P2Op(m, L, po.KCoeff(*problems.shell_coefficient(0)))
+        n = m.ndofs(L + 1)
+        desc = f"oracle P2 div(k grad) apply, shell level {L} ({n} DoFs)"
+    else:
+        L = 8
+        m = po.Mesh.cuboid()
+        op = po.P1Op(m, L)
+        n = m.ndofs(L)
+        desc = f"oracle P1 apply, cube level {L} ({n} DoFs)"
     x = np.random.default_rng(1).uniform(-1, 1, n)
     y = np.zeros(n)
     op.apply(x, y, po.ALL)
@@ -51,7 +128,7 @@ def cpu_baseline(level, seconds):
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": n * reps / dt, "unit": "DoF/s", "cores": cores, "kind": "port",
-            "sample": f"oracle P1 apply, cube level {level} ({n} DoFs), {reps} applies in {dt:.1f} s, OpenMP"}
+            "sample": f"{desc}, {reps} applies in {dt:.1f} s, OpenMP over macro-cells"}
 
 
 def gmg_cfg1(hg, ctx):
@@ -65,7 +142,7 @@ def gmg_cfg1(hg, ctx):
     x = hg.P1Function("x", st, 0, L)
     fb = hg.P1Function("b", st, 0, L)
     fb.upload(L, b)
-    hg.gmg_solve(A, x, fb, 0, L, 3, 3, cycles=2)  # warm-up (set-up of transfer data)
+    hg.gmg_solve(A, x, fb, 0, L, 3, 3, cycles=2)
     x.interpolate(0.0, L)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -87,69 +164,43 @@ def gmg_cfg1(hg, ctx):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--level", type=int, default=9)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="p2_shell", choices=["p2_shell", "p1_cube"])
+    ap.add_argument("--level", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--cpu-level", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-apply", action="store_true",
                     help="run the apply measurement only (for counter collection)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this path shards by "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard by "
               "macro-cell, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
     import hyteg_amd as hg
     ctx = hg.Context(0)
-    L = args.level
-    st = hg.PrimitiveStorage.cuboid(ctx)
-    A = hg.P1ConstantLaplaceOperator(st, L, L)
-    n = st.ndofs(L)
-    src = hg.P1Function("src", st, L, L)
-    dst = hg.P1Function("dst", st, L, L)
-    src.upload(L, np.random.default_rng(0).uniform(-1, 1, n))
-    print(f"[bench] level {L}: {n} DoFs, warmup {args.warmup}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        A.apply(src, dst, L, hg.ALL)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        A.apply(src, dst, L, hg.ALL)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    ms = 1e3 * dt / args.steps
-    value = n / (dt / args.steps)
-    print(f"[bench] timed {args.steps} applies: {ms:.3f} ms/apply", file=sys.stderr, flush=True)
-
-    # dominant kernel: cell-interior 15-point sweep, HIP events on the library stream
-    ctx.event_timing(True)
-    kms = []
-    for _ in range(max(5, min(args.steps, 20))):
-        A.apply(src, dst, L, hg.ALL)
-        kms.append(ctx.last_ms())
-    ctx.event_timing(False)
-    kavg = float(np.mean(kms))
-    m = 2 ** L
-    n_cell = 6 * tet(m - 3)
-    alg_bytes = 2 * 8 * n_cell
-    achieved = alg_bytes / (kavg * 1e-3) / 1e9
+    run = p2_shell if args.workload == "p2_shell" else p1_cube
+    L = args.level if args.level is not None else (6 if args.workload == "p2_shell" else 9)
+    n, sec, kms, alg, config, kname, tkey = run(hg, ctx, L, args.steps, args.warmup)
+    log(f"{args.steps} applies: {1e3 * sec:.3f} ms/apply, dominant kernel {kms:.3f} ms")
+    achieved = alg / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(f"p1_apply_cube_L{L}_f64"),
-                "kernel": "kCellApply<double>", "kernel_avg_ms": kavg,
-                "algorithmic_bytes_per_launch": alg_bytes}
-
-    gmg = None if args.only_apply else gmg_cfg1(hg, ctx)
-    cpu = None if (args.no_cpu or args.only_apply) else cpu_baseline(min(args.cpu_level, L), args.cpu_seconds)
+                "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(tkey), "kernel": kname,
+                "kernel_avg_ms": kms, "algorithmic_bytes_per_launch": alg}
+    extra = {}
+    if not args.only_apply:
+        if args.workload == "p2_shell":
+            n1, sec1, kms1, alg1, cfg1, _, _ = p1_cube(hg, ctx, 9, 20, 3)
+            extra["p1_apply_cube_L9_f64"] = {"dofs": n1, "ms_per_apply": 1e3 * sec1, "dof_per_s": n1 / sec1,
+                                             "kernel_ms": kms1, "kernel_gbs": alg1 / (kms1 * 1e-3) / 1e9}
+        extra["gmg_p1_cube_L5_f64"] = gmg_cfg1(hg, ctx)
+    cpu = None if (args.no_cpu or args.only_apply) else cpu_baseline(args.workload, args.cpu_seconds)
     out = {
-        "metric": "DoF/s per apply; time-to-1e-8 residual mixed vs FP64; solves/s + p99 latency",
-        "value": value, "unit": "DoF/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (uniform[-1,1], seed 0)",
-        "config": {"workload": f"P1ConstantLaplaceOperator apply, unit cube 6 macro-tets, level {L} "
-                               f"({n} DoFs), FP64", "level": L, "dofs": n, "mesh": "meshCuboid 1x1x1"},
-        "roofline": roofline, "cpu_baseline": cpu, "gmg": gmg,
+        "metric": METRIC, "value": n / sec, "unit": "DoF/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform[-1,1] input, seeded k)",
+        "config": config, "roofline": roofline, "cpu_baseline": cpu, **extra,
     }
     print(json.dumps(out))
 

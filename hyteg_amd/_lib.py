@@ -39,6 +39,7 @@ SIGNATURES = {
     "hg_function_add": [_p, _i64, _i, _p, _p, _i, _i],
     "hg_function_interpolate_const": [_p, _i64, _d, _i, _i],
     "hg_function_dot": [_p, _i64, _i64, _i, _i, _p],
+    "hg_function_interpolate_random": [_p, _i64, C.c_uint64, _d, _d, _i, _i],
     "hg_function_copy_from": [_p, _i64, _i64, _i],
     "hg_function_communicate": [_p, _i64, _i],
     "hg_p1_laplace_create": [_p, _i64, _i, _i, _p],

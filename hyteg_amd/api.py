@@ -167,6 +167,10 @@ class P1Function:
         self.ctx.check(lib().hg_function_interpolate_const(self.ctx.h, self.h, float(value), level, flag),
                        "interpolate")
 
+    def interpolate_random(self, seed, lo, hi, level, flag=ALL):
+        self.ctx.check(lib().hg_function_interpolate_random(self.ctx.h, self.h, seed, lo, hi, level, flag),
+                       "interpolate_random")
+
     def dot(self, other, level, flag=ALL):
         out = C.c_double()
         self.ctx.check(lib().hg_function_dot(self.ctx.h, self.h, other.h, level, flag, C.byref(out)), "dot")

@@ -268,6 +268,40 @@ __global__ void kFill( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bn
    dst[p] = fromAcc< T >( typename Acc< T >::type( value ) );
 }
 
+// SplitMix64 of (seed, canonical DoF id) -> uniform [lo, hi): independent of
+// the storage layout and of the traversal order
+__device__ __forceinline__ double canonicalUniform( uint64_t seed, uint64_t id, double lo, double hi )
+{
+   uint64_t z = seed + ( id + 1 ) * 0x9E3779B97F4A7C15ull;
+   z          = ( z ^ ( z >> 30 ) ) * 0xBF58476D1CE4E5B9ull;
+   z          = ( z ^ ( z >> 27 ) ) * 0x94D049BB133111EBull;
+   z ^= z >> 31;
+   return lo + ( hi - lo ) * ( double( z >> 11 ) * ( 1.0 / 9007199254740992.0 ) );
+}
+
+// cell-interior points: one block per (cell, interior row)
+template < class T >
+__global__ void kRandomCells( const int32_t* __restrict__ rows, int64_t nrows, int64_t sc, int64_t w, int64_t mm,
+                              int64_t coff, int64_t cin, uint64_t seed, double lo, double hi, T* __restrict__ v )
+{
+   const int64_t c = blockIdx.y;
+   const int64_t r = blockIdx.x;
+   const int64_t y = rows[2 * r], z = rows[2 * r + 1];
+   const int64_t out = c * sc + tetRowStart( w, y, z );
+   const int64_t cid = coff + c * cin + tetIndex( mm - 3, 0, y - 1, z - 1 );
+   for ( int64_t x = 1 + threadIdx.x; x + y + z < mm; x += blockDim.x )
+      v[out + x] = fromAcc< T >( typename Acc< T >::type( canonicalUniform( seed, uint64_t( cid + x - 1 ), lo, hi ) ) );
+}
+
+template < class T >
+__global__ void kRandomOwned( int64_t nb, int64_t base, const uint8_t* __restrict__ bndFlag, int flag, uint64_t seed,
+                              double lo, double hi, T* __restrict__ v )
+{
+   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b < nb && ( bndFlag[b] & flag ) )
+      v[base + b] = fromAcc< T >( typename Acc< T >::type( canonicalUniform( seed, uint64_t( b ), lo, hi ) ) );
+}
+
 template < class TD, class TS >
 __global__ void kCast( int64_t n, const TS* __restrict__ src, TD* __restrict__ dst )
 {
@@ -759,6 +793,28 @@ int hg_function_interpolate_const( hg_ctx* h, hg_handle fn, double value, int le
          using T = decltype( tag );
          kFill< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, value,
                                                                 (T*) f.ptr( level ) );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+int hg_function_interpolate_random( hg_ctx* h, hg_handle fn, uint64_t seed, double lo, double hi, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& f = ctx->function( fn );
+      checkLevel( f, level );
+      Lattice& l = f.lat( ctx, level );
+      dispatchType( f.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         if ( ( flag & kInner ) && l.nrows > 0 )
+         {
+            dim3 grid( unsigned( l.nrows ), unsigned( l.nc ) );
+            kRandomCells< T ><<< grid, 64, 0, ctx->stream >>>( l.rowsNat.p, l.nrows, l.sc, l.w, l.mm, l.num.coff,
+                                                              l.num.cin, seed, lo, hi, (T*) f.ptr( level ) );
+         }
+         if ( l.nb > 0 )
+            kRandomOwned< T ><<< gridFor( l.nb ), kBlock, 0, ctx->stream >>>( l.nb, l.nc * l.sc, l.bndFlag.p, flag, seed,
+                                                                             lo, hi, (T*) f.ptr( level ) );
       } );
       HG_CHECK_HIP( hipGetLastError() );
    } );

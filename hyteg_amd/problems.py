@@ -24,6 +24,17 @@ def k_values(a, omega, phi, x):
     return np.exp((np.asarray(a)[None, :] * np.sin(x @ np.asarray(omega).T + np.asarray(phi)[None, :])).sum(axis=1))
 
 
+def canonical_uniform(n, seed, lo=-1.0, hi=1.0):
+    """value of canonical DoF i: SplitMix64(seed + (i+1) * golden) -> [lo, hi) (the
+    device generator of hg_function_interpolate_random)"""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.arange(n, dtype=np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z ^= z >> np.uint64(31)
+    return lo + (hi - lo) * ((z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0))
+
+
 def uniform_rhs(flags, seed=0, inner=1):
     """uniform[-1,1] on DoFs of type `inner`, 0 elsewhere"""
     rng = np.random.default_rng(seed)

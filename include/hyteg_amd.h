@@ -119,6 +119,11 @@ int hg_function_assign( hg_ctx* ctx, hg_handle dst, int nsrc, const double* scal
 int hg_function_add( hg_ctx* ctx, hg_handle dst, int nsrc, const double* scalars, const hg_handle* srcs, int level, int flag );
 /* interpolate(constant, level, flag) (VertexDoFFunction.cpp:384) */
 int hg_function_interpolate_const( hg_ctx* ctx, hg_handle fn, double value, int level, int flag );
+/* interpolate seeded uniform[lo, hi) values on DoFs of matching type: the value
+ * of canonical DoF i is SplitMix64(seed + (i+1) * 0x9E3779B97F4A7C15) mapped to
+ * [lo, hi), independent of the storage layout (random interpolation,
+ * cf. VertexDoFFunction::interpolate with a random expression) */
+int hg_function_interpolate_random( hg_ctx* ctx, hg_handle fn, uint64_t seed, double lo, double hi, int level, int flag );
 /* dotGlobal(other, level, flag) -- FP64 accumulation (VertexDoFFunction.cpp:1710-1716) */
 int hg_function_dot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
 /* multElementwise({a, b}, level, flag): dst = a * b (VertexDoFFunction.cpp:1408) */

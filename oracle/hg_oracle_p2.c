@@ -566,11 +566,11 @@ static void comp_coord( const or_mesh* m, int c, int64_t mm, const int64_t q[3],
 /* loop over all micro-cells of level L; calls f with the 10 canonical ids and the element matrix */
 typedef void ( *elem_cb )( void* ctx, const int64_t* ids, const double* A );
 
-static void for_each_element( const or_p2op* op, elem_cb f, void* ctx )
+static void for_each_element_in( const or_p2op* op, int c0, int c1, elem_cb f, void* ctx )
 {
    const or_mesh* m  = op->m;
    const int64_t  mm = (int64_t) 1 << op->L;
-   for ( int c = 0; c < m->nc; ++c )
+   for ( int c = c0; c < c1; ++c )
    {
       const double* B = m->blend ? m->blend + 14 * c : NULL;
       for ( int t = 0; t < 6; ++t )
@@ -604,6 +604,8 @@ static void for_each_element( const or_p2op* op, elem_cb f, void* ctx )
    }
 }
 
+static void for_each_element( const or_p2op* op, elem_cb f, void* ctx ) { for_each_element_in( op, 0, op->m->nc, f, ctx ); }
+
 typedef struct
 {
    const double* src;
@@ -621,6 +623,7 @@ static void apply_cb( void* vctx, const int64_t* ids, const double* A )
       double s = 0.0;
       for ( int j = 0; j < 10; ++j )
          s += A[10 * i + j] * xe[j];
+#pragma omp atomic
       c->acc[ids[i]] += s;
    }
 }
@@ -634,7 +637,10 @@ void or_p2_apply( const or_p2op* op, const double* src, double* dst, int flag, i
    const int64_t n   = op->l1.n;
    double*       acc = calloc( n, sizeof( double ) );
    apply_ctx     c   = { src, acc };
-   for_each_element( op, apply_cb, &c );
+   /* macro-cells in parallel; contributions to shared DoFs are added atomically */
+#pragma omp parallel for schedule( dynamic, 1 )
+   for ( int cell = 0; cell < op->m->nc; ++cell )
+      for_each_element_in( op, cell, cell + 1, apply_cb, &c );
    for ( int64_t i = 0; i < n; ++i )
    {
       if ( i >= op->l1.coff )

@@ -86,3 +86,18 @@ def test_mult_elementwise(ctx, shell_pair):
     fc.mult(fa, fb, L, hg.GMG_FLAG)
     ref = np.where(fl & hg.GMG_FLAG, a * b, c)
     assert relmax(fc.download(L), ref) < 1e-15
+
+
+def test_interpolate_random_matches_canonical_generator(ctx, shell_pair):
+    st, om = shell_pair
+    for L, F in ((2, hg.P1Function), (1, hg.P2Function)):
+        f = F("r", st, L, L)
+        f.interpolate_random(123, -1.0, 1.0, L, hg.ALL)
+        n = f.ndofs(L)
+        assert np.array_equal(f.download(L), problems.canonical_uniform(n, 123))
+        g = F("g", st, L, L)
+        g.interpolate(5.0, L)
+        g.interpolate_random(7, 0.0, 2.0, L, hg.INNER)
+        fl = st.flags(L + F.OFFSET)
+        ref = np.where(fl == hg.INNER, problems.canonical_uniform(n, 7, 0.0, 2.0), 5.0)
+        assert np.array_equal(g.download(L), ref)
