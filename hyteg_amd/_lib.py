@@ -55,6 +55,7 @@ SIGNATURES = {
     "hg_p2_divkgrad_create": [_p, _i64, _i, _i, _i, _p, _p, _p, _p],
     "hg_p2_apply": [_p, _i64, _i64, _i64, _i, _i, _i],
     "hg_p2_inverse_diagonal": [_p, _i64, _i, _i64],
+    "hg_p2_store_coefficients": [_p, _i64, _i],
     "hg_operator_destroy": [_p, _i64],
     "hg_p1_apply": [_p, _i64, _i64, _i64, _i, _i, _i],
     "hg_p1_smooth_sor": [_p, _i64, _i64, _i64, _d, _i, _i],

@@ -215,6 +215,10 @@ class P2DivKGradOperator:
     def inverse_diagonal(self, level, dst):
         self.ctx.check(lib().hg_p2_inverse_diagonal(self.ctx.h, self.h, level, dst.h), "inverse diagonal")
 
+    def store_coefficients(self, enable=True):
+        """keep k at the quadrature points (default) or evaluate it in every apply"""
+        self.ctx.check(lib().hg_p2_store_coefficients(self.ctx.h, self.h, int(enable)), "store_coefficients")
+
 
 class P1ConstantLaplaceOperator:
     def __init__(self, storage, min_level, max_level):

@@ -33,6 +33,14 @@ HG_HD int64_t tetIndex( int64_t n, int64_t x, int64_t y, int64_t z )
 {
    return tet( n ) - tet( n - z ) + y * ( n - z ) - y * ( y - 1 ) / 2 + x;
 }
+// 32-bit variant of tetRowStart for in-lattice offsets of lattices with
+// n <= 1024 (tet(n) < 2^31 and all intermediate products < 2^31)
+HG_HD int32_t tetRowStart32( int32_t n, int32_t y, int32_t z )
+{
+   const int32_t m = n - z; // tet(n) - tet(n - z) = z (3 n^2 - 3 n z + 6 n + z^2 - 3 z + 2) / 6
+   return int32_t( ( int64_t( z ) * ( 3 * n * n - 3 * n * z + 6 * n + z * z - 3 * z + 2 ) ) / 6 ) + y * m -
+          y * ( y - 1 ) / 2;
+}
 // start of row (y, z) in a tetrahedral lattice of width n (x = 0)
 HG_HD int64_t tetRowStart( int64_t n, int64_t y, int64_t z )
 {

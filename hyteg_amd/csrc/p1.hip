@@ -273,19 +273,6 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
 // Block b of a 1D launch of 8*ceil(n/8) blocks -> logical block such that each
 // XCD (blocks are dispatched round-robin over the 8 XCDs) sweeps one contiguous
 // range of rows and keeps the neighbour rows it shares in its own L2.
-__device__ __forceinline__ int64_t xcdBlock( int64_t n )
-{
-   const int64_t per = ( n + 7 ) / 8;
-   return int64_t( blockIdx.x % 8 ) * per + int64_t( blockIdx.x / 8 );
-}
-static unsigned xcdGrid( int64_t n )
-{
-   const int64_t g = 8 * ( ( n + 7 ) / 8 );
-   if ( g > 2147483647LL )
-      throw Error( -1, "grid too large" );
-   return unsigned( g );
-}
-
 // 15-point stencil on cell interiors; one block per (cell, interior row)
 template < class T >
 __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,

@@ -26,6 +26,8 @@ namespace hg {
 // getMicroVerticesFromMicroCell lists them) and weights (reference tet, sum 1/6)
 __constant__ double cQP[11][4];
 __constant__ double cQW[11];
+__constant__ double cQ4[11][4];  // 4 lam_k at quadrature point q
+__constant__ double cQ4m[11][4]; // 4 lam_k - 1
 __constant__ int    cVof[6][4][3] = {
     { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
     { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
@@ -116,13 +118,35 @@ struct KParams
    double a[32], om[32][3], ph[32];
 };
 
+// Per macro-cell data of the tiled element kernels.  Micro-cell of type t with
+// anchor a has the computational-space vertices p_k = v0 + hE (a + vof_t[k]);
+// J[t] = hE V_t (columns p_k - p_0), Dc[t] = rows of J[t]^-1 (gradients of the
+// barycentric coordinates 1..3), adet[t] = |det J[t]|.  The shell map of the cell
+// (IcosahedralShellMap.hpp:109-146,183-236) is F(x) = q(x) x with
+// q = (r0 + |lam| dr) / |x|, lam = ((x - B0) . n) / volT, so that
+// DF = q I + x w^T,  w = s n - (q / |x|^2) x,  s = sign(lam) dr / (volT |x|):
+// a rank-one update of a multiple of the identity, inverted in closed form.
+struct P2CellGeom
+{
+   double v0[3], hE[9];
+   double J[6][9], Dc[6][9], adet[6];
+   double XQ[6][11][4]; // J[t] (lam_1, lam_2, lam_3)_q and its lam-increment nl . XQ
+   double nl[3], lam0;  // lam(x) = nl . x - lam0,  nl = n / volT,  lam0 = nl . B0
+   double r0, dr;       // radius at the ray vertex and its increment to the ref vertex
+   int    kind;         // 0 affine, 1 shell
+};
+
 struct P2Operator : OperatorBase
 {
    Storage*               st = nullptr;
    int                    minL = 0, maxL = 0;
+   bool                   storeCoefficients = true;
+   int                    nm                = 0; // number of sine modes of k (0: k = 1)
    DevBuf< CellGeom >     geom;
+   std::map< int, DevBuf< P2CellGeom > > tgeom; // by P2 level
    DevBuf< KParams >      kpar;
    std::map< int, DevBuf< double > > invDiag; // by P2 level, lattice L+1 storage
+   std::map< int, DevBuf< double > > kq;      // k at the quadrature points, by level (tile-major)
 };
 
 __device__ __forceinline__ double kEval( const KParams& K, const double x[3] )
@@ -278,40 +302,283 @@ __device__ __forceinline__ void microElement( const CellGeom& g, int t, const in
    }
 }
 
-__global__ void kP2ElementApply( const CellGeom* __restrict__ geom, const KParams* __restrict__ kp, int64_t mm,
-                                 int64_t nc, int64_t sc1, int64_t w1, const double* __restrict__ src,
-                                 double* __restrict__ dst )
+// ---- tiled element kernels ------------------------------------------------
+// A workgroup handles one tile of kTile^3 micro-cell anchors of one macro-cell
+// (all 6 micro-cell types: 384 threads, one micro-cell each, one type per
+// wavefront).  The tile's P2 values sit on a (2 kTile + 1)^3 box of the level
+// L+1 lattice; the box is staged through LDS, element results are summed in
+// LDS, and the box is written back once: box-interior points belong to this
+// tile only (plain read-modify-write), box-surface points are shared with
+// neighbouring tiles (atomic add).
+constexpr int kTile        = 4;
+constexpr int kBox         = 2 * kTile + 1;
+constexpr int kBox3        = kBox * kBox * kBox;
+constexpr int kTileThreads = 6 * kTile * kTile * kTile;
+
+__host__ __device__ inline int64_t tileEdges( int64_t mm ) { return ( mm - 1 ) / kTile + 1; }
+__host__ __device__ inline int64_t tilesPerCell( int64_t mm ) { return tet( tileEdges( mm ) ); }
+
+struct TileCoord
 {
-   int64_t c, a[3];
-   int     t;
-   if ( !microOf( int64_t( blockIdx.x ) * blockDim.x + threadIdx.x, mm, nc, c, t, a ) )
-      return;
-   const CellGeom& g = geom[c];
-   double          p[4][3];
-   int64_t         pos[10];
-   microElement( g, t, a, mm, c, sc1, w1, p, pos );
-   double xe[10], ye[10];
-   for ( int i = 0; i < 10; ++i )
+   int64_t c, x0, y0, z0;
+};
+
+__device__ __forceinline__ TileCoord tileOf( int64_t lb, int64_t mm )
+{
+   const int64_t tpc = tilesPerCell( mm );
+   TileCoord     tc;
+   tc.c       = lb / tpc;
+   int64_t ti, tj, tk;
+   decodeMicro( tileEdges( mm ), lb - tc.c * tpc, ti, tj, tk );
+   tc.x0 = kTile * ti;
+   tc.y0 = kTile * tj;
+   tc.z0 = kTile * tk;
+   return tc;
+}
+
+// 1/b and 1/sqrt(b) from the hardware estimates (relative error below 2^-22)
+// refined by two Newton steps (error below 2^-52 after rounding)
+__device__ __forceinline__ double rcpNewton( double b )
+{
+   double r = __builtin_amdgcn_rcp( b );
+   r        = fma( r, fma( -b, r, 1.0 ), r );
+   return fma( r, fma( -b, r, 1.0 ), r );
+}
+__device__ __forceinline__ double rsqrtNewton( double b )
+{
+   double       y = __builtin_amdgcn_rsq( b );
+   const double h = 0.5 * b;
+   y              = y * fma( -h * y, y, 1.5 );
+   return y * fma( -h * y, y, 1.5 );
+}
+
+// shell-map scalars at x with lam(x) = ls: q = |F(x)| / |x|, w (DF = q I + x w^T),
+// beta = q + w . x
+__device__ __forceinline__ void shellScalars( const P2CellGeom& g, const double x[3], double ls, double& qq, double w[3],
+                                              double& beta )
+{
+   const double r2   = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+   const double irho = rsqrtNewton( r2 );
+   qq                = ( g.r0 + fabs( ls ) * g.dr ) * irho;
+   const double s    = ( ls < 0 ? -g.dr : g.dr ) * irho; // DF = q I - (q / rho^2) x x^T + s x nl^T
+   const double qr2  = qq * irho * irho;
+#pragma unroll
+   for ( int d = 0; d < 3; ++d )
+      w[d] = s * g.nl[d] - qr2 * x[d];
+   beta = qq + w[0] * x[0] + w[1] * x[1] + w[2] * x[2];
+}
+
+__device__ __forceinline__ double kAt( const P2CellGeom& g, const KParams& K, const double x[3], double ls )
+{
+   if ( g.kind )
    {
-      xe[i] = src[pos[i]];
-      ye[i] = 0.0;
+      double qq, w[3], beta;
+      shellScalars( g, x, ls, qq, w, beta );
+      const double F[3] = { qq * x[0], qq * x[1], qq * x[2] };
+      return kEval( K, F );
    }
+   return kEval( K, x );
+}
+
+struct TileThread
+{
+   int     t;          // micro-cell type (uniform per wavefront)
+   int     ax, ay, az; // anchor inside the tile
+   bool    valid;
+   double  p0[3];      // computational-space vertex 0
+   double  ls0;        // lam(p0) of the shell map
+};
+
+__device__ __forceinline__ TileThread tileThread( const P2CellGeom& g, const TileCoord& tc, int64_t mm )
+{
+   TileThread th;
+   th.t        = __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
+   const int a = int( threadIdx.x ) & 63;
+   th.ax       = a & 3;
+   th.ay       = ( a >> 2 ) & 3;
+   th.az       = a >> 4;
+   const int64_t gx = tc.x0 + th.ax, gy = tc.y0 + th.ay, gz = tc.z0 + th.az;
+   th.valid         = gx + gy + gz <= mm - 1 - MicroCells::rowShift[th.t];
+   const double vx = double( gx + cVof[th.t][0][0] ), vy = double( gy + cVof[th.t][0][1] ),
+                vz = double( gz + cVof[th.t][0][2] );
+#pragma unroll
+   for ( int d = 0; d < 3; ++d )
+      th.p0[d] = g.v0[d] + g.hE[3 * d] * vx + g.hE[3 * d + 1] * vy + g.hE[3 * d + 2] * vz;
+   th.ls0 = g.nl[0] * th.p0[0] + g.nl[1] * th.p0[1] + g.nl[2] * th.p0[2] - g.lam0;
+   return th;
+}
+
+// LDS box positions of the 10 P2 DoFs of the thread's micro-cell (vertices in
+// the order of getMicroVerticesFromMicroCell, then the edges (0,1), (0,2),
+// (0,3), (1,2), (1,3), (2,3))
+__device__ __forceinline__ void elementBoxPositions( const TileThread& th, int pos[10] )
+{
+#pragma unroll
+   for ( int k = 0; k < 4; ++k )
+      pos[k] = 2 * ( th.ax + cVof[th.t][k][0] ) +
+               kBox * ( 2 * ( th.ay + cVof[th.t][k][1] ) + kBox * 2 * ( th.az + cVof[th.t][k][2] ) );
+#pragma unroll
+   for ( int e = 0; e < 6; ++e )
+      pos[4 + e] = ( pos[cP2E[e][0]] + pos[cP2E[e][1]] ) / 2;
+}
+
+// k(F(x_q)) for every quadrature point of every micro-cell, stored tile-major:
+// table[(tile * 11 + q) * 384 + thread] (zero for threads outside the macro-cell)
+__global__ __launch_bounds__( kTileThreads ) void kP2TileCoefficients( const P2CellGeom* __restrict__ geom,
+                                                                      const KParams* __restrict__ kp, int64_t mm,
+                                                                      int64_t ntiles, double* __restrict__ table )
+{
+   const int64_t lb = blockIdx.x;
+   if ( lb >= ntiles )
+      return;
+   const TileCoord   tc = tileOf( lb, mm );
+   const P2CellGeom& g  = geom[tc.c];
+   const TileThread  th = tileThread( g, tc, mm );
    for ( int q = 0; q < 11; ++q )
    {
-      const double lam[4] = { cQP[q][0], cQP[q][1], cQP[q][2], cQP[q][3] };
-      double       D[4][3], absdet, F[3], G[10][3];
-      elementGeometry( g, p, lam, D, absdet, F );
-      p2Gradients( lam, D, G );
-      double gu[3] = { 0.0, 0.0, 0.0 };
-      for ( int j = 0; j < 10; ++j )
-         for ( int d = 0; d < 3; ++d )
-            gu[d] += xe[j] * G[j][d];
-      const double wq = cQW[q] * kEval( *kp, F ) * absdet;
-      for ( int i = 0; i < 10; ++i )
-         ye[i] += wq * ( G[i][0] * gu[0] + G[i][1] * gu[1] + G[i][2] * gu[2] );
+      const double* xq   = g.XQ[th.t][q];
+      const double  x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
+      table[( lb * 11 + q ) * kTileThreads + threadIdx.x] = th.valid ? kAt( g, *kp, x, th.ls0 + xq[3] ) : 0.0;
    }
-   for ( int i = 0; i < 10; ++i )
-      atomicAdd( dst + pos[i], ye[i] );
+}
+
+// y = A x on the cell lattices of level L+1 (halo positions included): the
+// element action of p2_div_k_grad_blending_q4 evaluated in barycentric form,
+//   gamma_k = sum_j u_j dphi_j/dlam_k,  g = sum_k gamma_k grad_c lam_k,
+//   y_i += sum_q w_q k_q |det DF J| dphi_i/dlam_k grad_c lam_k . (DF^-1 DF^-T g)
+// (grad_c: computational space), equal to the reference's element matrix times
+// the element vector.
+template < bool STORED >
+__global__ __launch_bounds__( kTileThreads, 6 ) void kP2TileApply( const P2CellGeom* __restrict__ geom,
+                                                               const KParams* __restrict__ kp,
+                                                               const double* __restrict__ table, int64_t mm,
+                                                               int64_t ntiles, int64_t sc1, int64_t w1,
+                                                               const double* __restrict__ src, double* __restrict__ dst )
+{
+   __shared__ double su[kBox3];
+   __shared__ double sy[kBox3];
+   __shared__ double sk[STORED ? 11 * kTileThreads : 1];
+   const int64_t lb = xcdBlock( ntiles );
+   if ( lb >= ntiles )
+      return;
+   const TileCoord tc   = tileOf( lb, mm );
+   const int64_t   base = tc.c * sc1;
+   const int32_t   top  = int32_t( 2 * mm );
+   // the coefficient loads are issued first (their latency overlaps the box
+   // load) and parked in LDS, which keeps them out of the registers of the
+   // quadrature loop
+   if constexpr ( STORED )
+   {
+      double kv[11];
+#pragma unroll
+      for ( int q = 0; q < 11; ++q )
+         kv[q] = table[( lb * 11 + q ) * kTileThreads + threadIdx.x];
+#pragma unroll
+      for ( int q = 0; q < 11; ++q )
+         sk[q * kTileThreads + threadIdx.x] = kv[q];
+   }
+   const double* srcC = src + base;
+   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+   {
+      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
+      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
+      su[p]           = ( X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : 0.0;
+      sy[p]           = 0.0;
+   }
+   __syncthreads();
+   const P2CellGeom& g  = geom[tc.c];
+   const TileThread  th = tileThread( g, tc, mm );
+   if ( th.valid )
+   {
+      double u[10], y[10];
+      {
+         int pos[10];
+         elementBoxPositions( th, pos );
+#pragma unroll
+         for ( int i = 0; i < 10; ++i )
+         {
+            u[i] = su[pos[i]];
+            y[i] = 0.0;
+         }
+      }
+      const double* Dc   = g.Dc[th.t];
+      const double  adet = g.adet[th.t];
+#pragma unroll 1
+      for ( int q = 0; q < 11; ++q )
+      {
+         const double* a4 = cQ4[q];
+         const double* b4 = cQ4m[q];
+         const double  g0 = b4[0] * u[0] + a4[1] * u[4] + a4[2] * u[5] + a4[3] * u[6];
+         const double  g1 = b4[1] * u[1] + a4[0] * u[4] + a4[2] * u[7] + a4[3] * u[8];
+         const double  g2 = b4[2] * u[2] + a4[0] * u[5] + a4[1] * u[7] + a4[3] * u[9];
+         const double  g3 = b4[3] * u[3] + a4[0] * u[6] + a4[1] * u[8] + a4[2] * u[9];
+         const double  d1 = g1 - g0, d2 = g2 - g0, d3 = g3 - g0;
+         double        v[3];
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            v[d] = d1 * Dc[d] + d2 * Dc[3 + d] + d3 * Dc[6 + d];
+         const double* xq   = g.XQ[th.t][q];
+         const double  x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
+         double        kqv;
+         if constexpr ( STORED )
+            kqv = sk[q * kTileThreads + threadIdx.x];
+         else
+            kqv = kAt( g, *kp, x, th.ls0 + xq[3] );
+         double cc = cQW[q] * adet * kqv;
+         if ( g.kind )
+         {
+            double qq, w[3], beta;
+            shellScalars( g, x, th.ls0 + xq[3], qq, w, beta );
+            const double ib = rcpNewton( beta );
+            const double t1 = ( x[0] * v[0] + x[1] * v[1] + x[2] * v[2] ) * ib;
+#pragma unroll
+            for ( int d = 0; d < 3; ++d )
+               v[d] -= w[d] * t1;
+            const double t2 = ( w[0] * v[0] + w[1] * v[1] + w[2] * v[2] ) * ib;
+#pragma unroll
+            for ( int d = 0; d < 3; ++d )
+               v[d] -= x[d] * t2;
+            cc *= fabs( beta );
+         }
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            v[d] *= cc;
+         const double e1 = Dc[0] * v[0] + Dc[1] * v[1] + Dc[2] * v[2];
+         const double e2 = Dc[3] * v[0] + Dc[4] * v[1] + Dc[5] * v[2];
+         const double e3 = Dc[6] * v[0] + Dc[7] * v[1] + Dc[8] * v[2];
+         const double e0 = -( e1 + e2 + e3 );
+         y[0] += b4[0] * e0;
+         y[1] += b4[1] * e1;
+         y[2] += b4[2] * e2;
+         y[3] += b4[3] * e3;
+         y[4] += a4[1] * e0 + a4[0] * e1;
+         y[5] += a4[2] * e0 + a4[0] * e2;
+         y[6] += a4[3] * e0 + a4[0] * e3;
+         y[7] += a4[2] * e1 + a4[1] * e2;
+         y[8] += a4[3] * e1 + a4[1] * e3;
+         y[9] += a4[3] * e2 + a4[2] * e3;
+      }
+      int pos[10];
+      elementBoxPositions( th, pos );
+#pragma unroll
+      for ( int i = 0; i < 10; ++i )
+         atomicAdd( &sy[pos[i]], y[i] );
+   }
+   __syncthreads();
+   double* dstC = dst + base;
+   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+   {
+      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
+      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
+      if ( X + Y + Z > top )
+         continue;
+      double*    o      = dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
+      const bool shared = lx == 0 || ly == 0 || lz == 0 || lx == kBox - 1 || ly == kBox - 1 || lz == kBox - 1;
+      if ( shared )
+         atomicAdd( o, sy[p] );
+      else
+         *o += sy[p];
+   }
 }
 
 __global__ void kP2ElementDiag( const CellGeom* __restrict__ geom, const KParams* __restrict__ kp, int64_t mm,
@@ -402,6 +669,25 @@ static void p2CheckFunction( const P2Operator& op, const Function& f, int L )
       throw Error( -1, "level outside the operator's or function's level range" );
 }
 
+// k at the quadrature points of level L (built on first use when the operator
+// stores coefficients; nullptr when it evaluates k in the apply kernel)
+static const double* p2Coefficients( Context* ctx, P2Operator& op, int L )
+{
+   if ( !op.storeCoefficients || op.nm == 0 )
+      return nullptr;
+   auto it = op.kq.find( L );
+   if ( it != op.kq.end() )
+      return it->second.p;
+   const int64_t mm     = int64_t( 1 ) << L;
+   const int64_t ntiles = tilesPerCell( mm ) * int64_t( op.st->topo.nc );
+   auto&         buf    = op.kq[L];
+   buf.alloc( ctx, size_t( ntiles ) * 11 * kTileThreads );
+   kP2TileCoefficients<<< gridFor( ntiles, 1 ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, mm, ntiles,
+                                                                                buf.p );
+   HG_CHECK_HIP( hipGetLastError() );
+   return buf.p;
+}
+
 void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L, int flag, int update )
 {
    p2CheckFunction( op, src, L );
@@ -423,9 +709,17 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
       kZeroHalos< double ><<< gridFor( lat.haloPerCell() * lat.nc ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lat ),
                                                                                               lat.nc, d );
    HG_CHECK_HIP( hipGetLastError() );
-   const int64_t n = microCellsPerCell( mm ) * lat.nc;
+   if ( lat.w > 1024 )
+      throw Error( HG_ERR_UNSUPPORT, "P2 apply supports levels up to 9 (32-bit lattice offsets)" );
+   const int64_t ntiles = tilesPerCell( mm ) * lat.nc;
+   const double* table  = p2Coefficients( ctx, op, L );
    ctx->beginTiming();
-   kP2ElementApply<<< gridFor( n, 128 ), 128, 0, ctx->stream >>>( op.geom.p, op.kpar.p, mm, lat.nc, lat.sc, lat.w, s, d );
+   if ( table )
+      kP2TileApply< true ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, table, mm,
+                                                                                  ntiles, lat.sc, lat.w, s, d );
+   else
+      kP2TileApply< false ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, nullptr,
+                                                                                   mm, ntiles, lat.sc, lat.w, s, d );
    HG_CHECK_HIP( hipGetLastError() );
    ctx->endTiming();
    pushAdditive( ctx, st, lat, d, 0, kAll ^ flag, update == HG_REPLACE );
@@ -462,6 +756,102 @@ __global__ void kMult( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bn
    if ( p >= ncell && !( bndFlag[p - ncell] & flag ) )
       return;
    dst[p] = fromAcc< T >( toAcc( a[p] ) * toAcc( b[p] ) );
+}
+
+static const int kVofHost[6][4][3] = {
+    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
+    { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
+    { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } }, { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
+
+// per macro-cell data of the tiled kernels for micro-edge length 1 (hE holds the
+// macro-cell's edge vectors; tileGeometryLevel scales them to a level)
+static std::vector< P2CellGeom > tileGeometry( const Topology& t )
+{
+   std::vector< P2CellGeom > out( t.nc );
+   for ( int c = 0; c < t.nc; ++c )
+   {
+      P2CellGeom&   g = out[c];
+      const double* v[4];
+      for ( int k = 0; k < 4; ++k )
+         v[k] = &t.X[3 * t.cv[c][k]];
+      for ( int d = 0; d < 3; ++d )
+      {
+         g.v0[d] = v[0][d];
+         for ( int k = 0; k < 3; ++k )
+            g.hE[3 * d + k] = v[k + 1][d] - v[0][d];
+      }
+      g.kind = t.blend.empty() ? 0 : 1;
+      if ( g.kind )
+      {
+         const ShellCellMap& b = t.blend[c];
+         double              a[3], tt[3], f[3];
+         for ( int d = 0; d < 3; ++d )
+         {
+            a[d]    = b.refVertex[d] - b.rayVertex[d];
+            tt[d]   = b.thrVertex[d] - b.rayVertex[d];
+            f[d]    = b.forVertex[d] - b.rayVertex[d];
+         }
+         const double n[3]  = { tt[1] * f[2] - tt[2] * f[1], tt[2] * f[0] - tt[0] * f[2], tt[0] * f[1] - tt[1] * f[0] };
+         const double volT  = a[0] * n[0] + a[1] * n[1] + a[2] * n[2];
+         for ( int d = 0; d < 3; ++d )
+            g.nl[d] = n[d] / volT;
+         g.lam0 = g.nl[0] * b.rayVertex[0] + g.nl[1] * b.rayVertex[1] + g.nl[2] * b.rayVertex[2];
+         g.r0   = b.radRayVertex;
+         g.dr   = b.radRefVertex - b.radRayVertex;
+      }
+      else
+      {
+         for ( int d = 0; d < 3; ++d )
+            g.nl[d] = 0.0;
+         g.lam0 = g.r0 = g.dr = 0.0;
+      }
+   }
+   return out;
+}
+
+// scale the level-independent cell data to micro-edge length 1/mm and fill the
+// per-type Jacobians, their inverses and determinants
+static void tileGeometryLevel( std::vector< P2CellGeom >& gs, int64_t mm )
+{
+   const double h = 1.0 / double( mm );
+   for ( auto& g : gs )
+   {
+      for ( double& e : g.hE )
+         e *= h;
+      for ( int ty = 0; ty < 6; ++ty )
+      {
+         double* J = g.J[ty];
+         for ( int r = 0; r < 3; ++r )
+            for ( int k = 0; k < 3; ++k )
+            {
+               double s = 0.0;
+               for ( int m = 0; m < 3; ++m )
+                  s += g.hE[3 * r + m] * double( kVofHost[ty][k + 1][m] - kVofHost[ty][0][m] );
+               J[3 * r + k] = s;
+            }
+         const double c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
+         const double det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+         const double id  = 1.0 / det;
+         double*      D   = g.Dc[ty];
+         D[0]             = c00 * id;
+         D[1]             = ( J[2] * J[7] - J[1] * J[8] ) * id;
+         D[2]             = ( J[1] * J[5] - J[2] * J[4] ) * id;
+         D[3]             = c01 * id;
+         D[4]             = ( J[0] * J[8] - J[2] * J[6] ) * id;
+         D[5]             = ( J[2] * J[3] - J[0] * J[5] ) * id;
+         D[6]             = c02 * id;
+         D[7]             = ( J[1] * J[6] - J[0] * J[7] ) * id;
+         D[8]             = ( J[0] * J[4] - J[1] * J[3] ) * id;
+         g.adet[ty]       = std::fabs( det );
+         for ( int q = 0; q < 11; ++q )
+         {
+            double* xq = g.XQ[ty][q];
+            for ( int d = 0; d < 3; ++d )
+               xq[d] = J[3 * d] * kQP[q][1] + J[3 * d + 1] * kQP[q][2] + J[3 * d + 2] * kQP[q][3];
+            xq[3] = g.nl[0] * xq[0] + g.nl[1] * xq[1] + g.nl[2] * xq[2];
+         }
+      }
+   }
 }
 
 } // namespace hg
@@ -557,6 +947,15 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
          quadratureWeights( w );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQP ), kQP, sizeof( kQP ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQW ), w, sizeof( w ) ) );
+         double q4[11][4], q4m[11][4];
+         for ( int q = 0; q < 11; ++q )
+            for ( int k = 0; k < 4; ++k )
+            {
+               q4[q][k]  = 4.0 * kQP[q][k];
+               q4m[q][k] = 4.0 * kQP[q][k] - 1.0;
+            }
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4 ), q4, sizeof( q4 ) ) );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4m ), q4m, sizeof( q4m ) ) );
          quadReady = true;
       }
       Storage&        st = ctx->storage( s );
@@ -594,6 +993,13 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
          }
       }
       op->geom.upload( ctx, g );
+      const std::vector< P2CellGeom > tg = tileGeometry( t );
+      for ( int L = minLevel; L <= maxLevel; ++L )
+      {
+         std::vector< P2CellGeom > gl = tg;
+         tileGeometryLevel( gl, int64_t( 1 ) << L );
+         op->tgeom[L].upload( ctx, gl );
+      }
       std::vector< KParams > kp( 1 );
       kp[0]    = KParams{};
       kp[0].nm = nm;
@@ -605,6 +1011,7 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
             kp[0].om[m][d] = omega[3 * m + d];
       }
       op->kpar.upload( ctx, kp );
+      op->nm = nm;
       for ( int L = minLevel; L <= maxLevel; ++L )
          st.lattice( ctx, L + 1 );
       *out                  = ctx->nextHandle++;
@@ -618,6 +1025,16 @@ int hg_p2_apply( hg_ctx* h, hg_handle op, hg_handle src, hg_handle dst, int leve
       if ( update != HG_REPLACE && update != HG_ADD )
          throw Error( HG_ERR_ARG, "invalid update type" );
       p2Apply( ctx, ctx->op< P2Operator >( op ), ctx->function( src ), ctx->function( dst ), level, flag, update );
+   } );
+}
+
+int hg_p2_store_coefficients( hg_ctx* h, hg_handle op, int enable )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P2Operator& o = ctx->op< P2Operator >( op );
+      o.storeCoefficients = enable != 0;
+      if ( !o.storeCoefficients )
+         o.kq.clear();
    } );
 }
 

@@ -228,6 +228,24 @@ inline void DevBuf< T >::release()
 constexpr int kBlock      = 256;
 constexpr int kDotBlocks  = 1024;
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): blockIdx b runs on XCD b % 8.  xcdBlock maps the hardware block id to a
+// logical one so that each XCD receives a contiguous range of logical blocks
+// (neighbouring rows / tiles share L2 lines); the grid is padded to a multiple
+// of 8 by xcdGrid and logical ids >= n must return.
+__device__ __forceinline__ int64_t xcdBlock( int64_t n )
+{
+   const int64_t per = ( n + 7 ) / 8;
+   return int64_t( blockIdx.x % 8 ) * per + int64_t( blockIdx.x / 8 );
+}
+inline unsigned xcdGrid( int64_t n )
+{
+   const int64_t g = 8 * ( ( n + 7 ) / 8 );
+   if ( g > 2147483647LL )
+      throw Error( -1, "grid too large" );
+   return unsigned( g );
+}
+
 inline unsigned gridFor( int64_t n, int block = kBlock )
 {
    int64_t g = ( n + block - 1 ) / block;

@@ -160,6 +160,13 @@ int hg_p2_divkgrad_create( hg_ctx* ctx, hg_handle storage, int minLevel, int max
                            const double* omega, const double* phi, hg_handle* out );
 /* Operator::apply = gemv(1, src, 0|1, dst, level, flag) (P2ElementwiseOperator.cpp:110-328) */
 int hg_p2_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update );
+/* P2ElementwiseOperator::computeAndStoreLocalElementMatrices counterpart
+ * (elementwiseoperators/P2ElementwiseOperator.hpp:169,375-385): instead of the
+ * 10x10 element matrices (100 values per micro-cell) the operator keeps k at the
+ * 11 quadrature points of every micro-cell (11 values, FP64), built on the first
+ * apply of a level.  enable = 1 (default) stores, 0 evaluates k in every apply
+ * and frees the tables.  Results are identical either way. */
+int hg_p2_store_coefficients( hg_ctx* ctx, hg_handle op, int enable );
 /* OperatorWithInverseDiagonal::getInverseDiagonalValues on `level`, written
  * into the P2 function dst (P2ElementwiseOperator.cpp:418-568) */
 int hg_p2_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, hg_handle dst );

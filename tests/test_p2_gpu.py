@@ -30,8 +30,9 @@ def test_shell_storage_matches_oracle(ctx, shell_pair):
         assert np.abs(st.coords(L) - om.coords(L)).max() < 1e-14
 
 
-def _parity(ctx, st, om, levels, k_args):
+def _parity(ctx, st, om, levels, k_args, store=True):
     A = hg.P2DivKGradOperator(st, min(levels), max(levels), *k_args)
+    A.store_coefficients(store)
     for L in levels:
         print(f"P2 apply parity L{L}", flush=True)
         op = po.P2Op(om, L, po.KCoeff(*k_args) if k_args else None)
@@ -70,6 +71,28 @@ def test_p2_apply_parity_cube_variable_k(ctx):
 def test_p2_apply_parity_shell_variable_k(ctx, shell_pair):
     st, om = shell_pair
     _parity(ctx, st, om, [0, 1, 2], problems.shell_coefficient(0))
+
+
+def test_p2_apply_parity_shell_k_evaluated_in_kernel(ctx, shell_pair):
+    st, om = shell_pair
+    _parity(ctx, st, om, [1, 3], problems.shell_coefficient(3), store=False)
+
+
+def test_p2_apply_shell_level3_stored_equals_evaluated(ctx, shell_pair):
+    """stored quadrature-point coefficients reproduce the in-kernel evaluation (up to the
+    order of the atomic sums)"""
+    st, _ = shell_pair
+    L = 3
+    A = hg.P2DivKGradOperator(st, L, L, *problems.shell_coefficient(0))
+    src = hg.P2Function("src", st, L, L)
+    src.interpolate_random(9, -1.0, 1.0, L)
+    out = []
+    for store in (True, False):
+        A.store_coefficients(store)
+        dst = hg.P2Function("dst", st, L, L)
+        A.apply(src, dst, L, hg.ALL)
+        out.append(dst.download(L))
+    assert relmax(out[0], out[1]) < 1e-14
 
 
 def test_mult_elementwise(ctx, shell_pair):
