@@ -107,6 +107,32 @@ void     or_p2_inv_diag( const or_p2op* op, double* out );
 int64_t  or_p2op_nelem( const or_p2op* op );
 void     or_p2op_coo( const or_p2op* op, int64_t* rows, int64_t* cols, double* vals );
 
+/* ---- P2 multigrid ------------------------------------------------------------- */
+/* P2toP2QuadraticProlongation::prolongate (add = 0) / prolongateAndAdd (add = 1):
+ * fine = level coarseLevel+1; fine cell-interior DoFs always, macro
+ * vertex/edge/face DoFs where their flag is in `flag` */
+void or_p2_prolongate( const or_mesh* m, int coarseLevel, const double* coarse, double* fine, int flag, int add );
+/* P2toP2QuadraticRestriction::restrict: coarse = P^T fine on coarse interior DoFs and
+ * flagged macro vertex/edge/face DoFs */
+void or_p2_restrict( const or_mesh* m, int fineLevel, const double* fine, double* coarse, int flag );
+/* ChebyshevSmoother::setupCoefficientsInternal(order, lower, upper): coeffs[order] */
+int or_chebyshev_coefficients( int order, double lower, double upper, double* coeffs );
+/* one ChebyshevSmoother::solve step with the operator's inverse diagonal */
+void or_p2_chebyshev_smooth( const or_p2op* op, const double* invDiag, int order, const double* coeffs, double* x,
+                             const double* b, int flag );
+/* chebyshev::estimateRadius: power iteration on D^-1 A (flag All); x is the start
+ * vector and returns the last iterate */
+double or_p2_power_iteration( const or_p2op* op, const double* invDiag, double* x, int its );
+/* CGSolver::solve with identity (jacobi 0) or inverse-diagonal (jacobi 1) preconditioner */
+int or_p2_cg( const or_p2op* op, double* x, const double* b, int flag, int maxIter, double relTol, double absTol,
+              int jacobi );
+/* V(pre,post) cycles with Chebyshev smoothing of `order` (bounds 0.3 rho, 1.2 rho on
+ * every level) and CG at minLevel, until the Inner residual drops by `tol`;
+ * res_hist[0..k] (k = cycles done, returned) */
+int or_p2_gmg( const or_mesh* m, int minLevel, int maxLevel, int nm, const double* ka, const double* kom,
+               const double* kph, double* x, const double* b, int pre, int post, int order, double rho, int cycles,
+               double tol, double* res_hist, int coarseMaxIter, double coarseTol, int coarseJacobi );
+
 #ifdef __cplusplus
 }
 #endif

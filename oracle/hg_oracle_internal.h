@@ -66,6 +66,26 @@ typedef struct
 } lvl_t;
 
 
+/* k(x) = exp( sum_m a_m sin(omega_m . x + phi_m) ); nm == 0 means k == 1 */
+typedef struct
+{
+   int     nm;
+   double* a;     /* nm */
+   double* omega; /* nm*3 */
+   double* phi;   /* nm */
+} kfun_t;
+
+struct or_p2op
+{
+   const or_mesh* m;
+   int            L; /* P2 level; storage lattice is level L+1 */
+   lvl_t          l1;
+   int64_t        sc1, w1;
+   int64_t*       cmap; /* nc * sc1 */
+   kfun_t         K;
+   int32_t*       flags;
+};
+
 lvl_t    or_lvl_info( const or_mesh* m, int L );
 int64_t  or_locate_l( const or_mesh* m, const lvl_t* l, int c, int64_t x, int64_t y, int64_t z );
 int      or_owner_flag( const or_mesh* m, const lvl_t* l, int64_t id );

@@ -411,14 +411,6 @@ void or_quadrature( double* pts44, double* w11 )
    memcpy( w11, QW, sizeof( QW ) );
 }
 
-/* k(x) = exp( sum_m a_m sin(omega_m . x + phi_m) ); nm == 0 means k == 1 */
-typedef struct
-{
-   int     nm;
-   double* a;     /* nm */
-   double* omega; /* nm*3 */
-   double* phi;   /* nm */
-} kfun_t;
 
 static double kfun( const kfun_t* K, const double* x )
 {
@@ -505,16 +497,6 @@ void or_p2_element_matrix( const double* p /*4x3*/, const double* blend /*14 or 
 /* ------------------------------------------------------------------------- */
 /* P2 elementwise operator                                                    */
 /* ------------------------------------------------------------------------- */
-struct or_p2op
-{
-   const or_mesh* m;
-   int            L; /* P2 level; storage lattice is level L+1 */
-   lvl_t          l1;
-   int64_t        sc1, w1;
-   int64_t*       cmap; /* nc * sc1 */
-   kfun_t         K;
-   int32_t*       flags;
-};
 
 or_p2op* or_p2op_create( const or_mesh* m, int level, int nm, const double* ka, const double* kom, const double* kph )
 {

@@ -67,6 +67,13 @@ def lib():
             "or_p2_inv_diag": (None, [_p, _p]),
             "or_p2op_nelem": (_i64, [_p]),
             "or_p2op_coo": (None, [_p, _p, _p, _p]),
+            "or_p2_prolongate": (None, [_p, _i, _p, _p, _i, _i]),
+            "or_p2_restrict": (None, [_p, _i, _p, _p, _i]),
+            "or_chebyshev_coefficients": (_i, [_i, _d, _d, _p]),
+            "or_p2_chebyshev_smooth": (None, [_p, _p, _i, _p, _p, _p, _i]),
+            "or_p2_power_iteration": (_d, [_p, _p, _p, _i]),
+            "or_p2_cg": (_i, [_p, _p, _p, _i, _i, _d, _d, _i]),
+            "or_p2_gmg": (_i, [_p, _i, _i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _d, _i, _d, _p, _i, _d, _i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -137,6 +144,22 @@ class Mesh:
 
     def locate(self, level, c, x, y, z):
         return int(lib().or_p1_locate(self.h, level, c, x, y, z))
+
+    # P2 functions of P2 level L live on the P1 DoFs of level L+1
+    def p2_prolongate(self, coarse_level, coarse, fine, flag, add=False):
+        lib().or_p2_prolongate(self.h, coarse_level, _ptr(coarse), _ptr(fine), flag, int(add))
+
+    def p2_restrict(self, fine_level, fine, coarse, flag):
+        lib().or_p2_restrict(self.h, fine_level, _ptr(fine), _ptr(coarse), flag)
+
+    def p2_gmg(self, min_level, max_level, k, x, b, pre, post, order, rho, cycles=50, tol=1e-8,
+               coarse_max_iter=10000, coarse_tol=1e-14, coarse_jacobi=True):
+        hist = np.zeros(cycles + 1)
+        kk = lib().or_p2_gmg(self.h, min_level, max_level, *(k or KCoeff()).args(), _ptr(x), _ptr(b), pre, post,
+                             order, rho, cycles, tol, _ptr(hist), coarse_max_iter, coarse_tol, int(coarse_jacobi))
+        if kk < 0:
+            raise ValueError("unsupported Chebyshev order")
+        return kk, hist[:kk + 1]
 
     def restrict(self, fine_level, fine, coarse, flag):
         lib().or_p1_restrict(self.h, fine_level, _ptr(fine), _ptr(coarse), flag)
@@ -256,6 +279,16 @@ class P2Op:
         lib().or_p2_inv_diag(self.h, _ptr(out))
         return out
 
+    def chebyshev_smooth(self, inv_diag, coeffs, x, b, flag=GMG_FLAG):
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        lib().or_p2_chebyshev_smooth(self.h, _ptr(inv_diag), len(coeffs), _ptr(coeffs), _ptr(x), _ptr(b), flag)
+
+    def power_iteration(self, inv_diag, x, its):
+        return float(lib().or_p2_power_iteration(self.h, _ptr(inv_diag), _ptr(x), its))
+
+    def cg(self, x, b, flag=GMG_FLAG, max_iter=10000, rel_tol=1e-14, abs_tol=1e-300, jacobi=True):
+        return int(lib().or_p2_cg(self.h, _ptr(x), _ptr(b), flag, max_iter, rel_tol, abs_tol, int(jacobi)))
+
     def matrix(self):
         import scipy.sparse as sp
         ne = int(lib().or_p2op_nelem(self.h))
@@ -266,6 +299,13 @@ class P2Op:
         A = sp.csr_matrix((v, (r, c)), shape=(self.n, self.n))
         A.sum_duplicates()
         return A
+
+
+def chebyshev_coefficients(order, lower, upper):
+    out = np.zeros(order)
+    if lib().or_chebyshev_coefficients(order, lower, upper, _ptr(out)):
+        raise ValueError("unsupported Chebyshev order")
+    return out
 
 
 def dot(a, b, flags, flag):
