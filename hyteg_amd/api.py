@@ -245,11 +245,52 @@ class P1ConstantLaplaceOperator:
 
 
 def restrict(f, source_level, flag):
-    f.ctx.check(lib().hg_p1_restrict(f.ctx.h, f.h, source_level, flag), "restrict")
+    """P1toP1LinearRestriction / P2toP2QuadraticRestriction by the function's space"""
+    fn = lib().hg_p2_restrict if isinstance(f, P2Function) else lib().hg_p1_restrict
+    f.ctx.check(fn(f.ctx.h, f.h, source_level, flag), "restrict")
 
 
 def prolongate(f, source_level, flag, update):
-    f.ctx.check(lib().hg_p1_prolongate(f.ctx.h, f.h, source_level, flag, update), "prolongate")
+    """P1toP1LinearProlongation / P2toP2QuadraticProlongation by the function's space"""
+    fn = lib().hg_p2_prolongate if isinstance(f, P2Function) else lib().hg_p1_prolongate
+    f.ctx.check(fn(f.ctx.h, f.h, source_level, flag, update), "prolongate")
+
+
+def p2_transfer_stencil(kind, cls):
+    """(offsets (n,3), weights (n,)) of a P2 transfer stencil (kind 0 pull, 1 restriction)"""
+    n = C.c_int()
+    off = np.zeros((128, 3), dtype=np.int32)
+    w = np.zeros(128)
+    if lib().hg_p2_transfer_stencil(kind, cls, 128, _ptr(off), _ptr(w), C.byref(n)) != 0:
+        raise ValueError("invalid stencil class")
+    return off[: n.value].copy(), w[: n.value].copy()
+
+
+def p2_cg_solve(A, x, b, level, max_iter=10000, rel_tol=1e-14, abs_tol=1e-300, jacobi=True):
+    it = C.c_int()
+    A.ctx.check(lib().hg_p2_cg_solve(A.ctx.h, A.h, x.h, b.h, level, max_iter, rel_tol, abs_tol, int(jacobi),
+                                     C.byref(it)), "p2 cg")
+    return it.value
+
+
+def p2_chebyshev_smooth(A, x, b, level, order, rho, steps=1):
+    A.ctx.check(lib().hg_p2_chebyshev_smooth(A.ctx.h, A.h, x.h, b.h, level, order, rho, steps), "chebyshev")
+
+
+def p2_estimate_radius(A, x, level, iterations):
+    r = C.c_double()
+    A.ctx.check(lib().hg_p2_estimate_radius(A.ctx.h, A.h, x.h, level, iterations, C.byref(r)), "estimate radius")
+    return r.value
+
+
+def p2_gmg_solve(A, x, b, min_level, max_level, rho, pre=2, post=2, order=3, cycles=20, tol=0.0,
+                 coarse_max_iter=10000, coarse_tol=1e-14, coarse_jacobi=True):
+    hist = np.zeros(cycles + 1)
+    done = C.c_int()
+    A.ctx.check(lib().hg_p2_gmg_solve(A.ctx.h, A.h, x.h, b.h, min_level, max_level, pre, post, order, rho, cycles,
+                                      tol, coarse_max_iter, coarse_tol, int(coarse_jacobi), _ptr(hist),
+                                      C.byref(done)), "p2 gmg")
+    return done.value, hist[: done.value + 1]
 
 
 def cg_solve(A, x, b, level, max_iter, rel_tol, abs_tol):

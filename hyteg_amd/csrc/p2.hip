@@ -449,14 +449,15 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileCoefficients( const P2C
 // (grad_c: computational space), equal to the reference's element matrix times
 // the element vector.
 template < bool STORED >
-__global__ __launch_bounds__( kTileThreads, 6 ) void kP2TileApply( const P2CellGeom* __restrict__ geom,
+__global__ __launch_bounds__( kTileThreads ) void kP2TileApply( const P2CellGeom* __restrict__ geom,
                                                                const KParams* __restrict__ kp,
                                                                const double* __restrict__ table, int64_t mm,
                                                                int64_t ntiles, int64_t sc1, int64_t w1,
                                                                const double* __restrict__ src, double* __restrict__ dst )
 {
-   __shared__ double su[kBox3];
-   __shared__ double sy[kBox3];
+   // one LDS box: first the tile's source values, then (after every thread has
+   // read its 10 values) the accumulator of the element results
+   __shared__ double sb[kBox3];
    __shared__ double sk[STORED ? 11 * kTileThreads : 1];
    const int64_t lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
@@ -482,25 +483,29 @@ __global__ __launch_bounds__( kTileThreads, 6 ) void kP2TileApply( const P2CellG
    {
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      su[p]           = ( X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : 0.0;
-      sy[p]           = 0.0;
+      sb[p]           = ( X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : 0.0;
    }
    __syncthreads();
    const P2CellGeom& g  = geom[tc.c];
    const TileThread  th = tileThread( g, tc, mm );
+   double            u[10];
+   {
+      int pos[10];
+      elementBoxPositions( th, pos );
+#pragma unroll
+      for ( int i = 0; i < 10; ++i )
+         u[i] = th.valid ? sb[pos[i]] : 0.0;
+   }
+   __syncthreads();
+   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+      sb[p] = 0.0;
+   __syncthreads();
    if ( th.valid )
    {
-      double u[10], y[10];
-      {
-         int pos[10];
-         elementBoxPositions( th, pos );
+      double y[10];
 #pragma unroll
-         for ( int i = 0; i < 10; ++i )
-         {
-            u[i] = su[pos[i]];
-            y[i] = 0.0;
-         }
-      }
+      for ( int i = 0; i < 10; ++i )
+         y[i] = 0.0;
       const double* Dc   = g.Dc[th.t];
       const double  adet = g.adet[th.t];
 #pragma unroll 1
@@ -562,7 +567,7 @@ __global__ __launch_bounds__( kTileThreads, 6 ) void kP2TileApply( const P2CellG
       elementBoxPositions( th, pos );
 #pragma unroll
       for ( int i = 0; i < 10; ++i )
-         atomicAdd( &sy[pos[i]], y[i] );
+         atomicAdd( &sb[pos[i]], y[i] );
    }
    __syncthreads();
    double* dstC = dst + base;
@@ -575,9 +580,9 @@ __global__ __launch_bounds__( kTileThreads, 6 ) void kP2TileApply( const P2CellG
       double*    o      = dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
       const bool shared = lx == 0 || ly == 0 || lz == 0 || lx == kBox - 1 || ly == kBox - 1 || lz == kBox - 1;
       if ( shared )
-         atomicAdd( o, sy[p] );
+         atomicAdd( o, sb[p] );
       else
-         *o += sy[p];
+         *o += sb[p];
    }
 }
 

@@ -1,0 +1,421 @@
+// Quadratic P2 grid transfer: P2toP2QuadraticProlongation (prolongate /
+// prolongateAndAdd, P2toP2QuadraticProlongation.cpp:37-63,217-425) and
+// P2toP2QuadraticRestriction (restrict, P2toP2QuadraticRestriction.cpp:131-290).
+//
+// A P2 function of level L lives on the lattice of level L+1, so the fine P2
+// nodes of level L+1 are the points F of lattice L+2 and the coarse nodes the
+// points C of lattice L+1.  Prolongation is the interpolation of the coarse
+// quadratic at every fine node: F lies in the coarse micro-cell found by a
+// fixed rule (anchor floor(F/4), the containing micro-cell type of least row
+// shift), whose 10 nodes and basis values depend only on F mod 4.  That gives
+// 64 translation-invariant pull stencils.  Restriction is the transpose: for a
+// coarse node C the fine nodes whose located micro-cell has C as a node, with
+// the basis value as weight, form one stencil per parity class of C (vertex
+// node or one of the 7 edge orientations).  Both tables are built once on the
+// host by enumerating a lattice patch and live in constant memory.
+//
+// Macro-boundary DoFs are handled like the reference's additive kernels:
+// a fine point shared by several macro-cells contributes 1/(number of cells
+// sharing its primitive) from each, and restricted values of coarse DoFs on
+// macro vertices/edges/faces are the sum over the incident cells.
+#include <array>
+#include <vector>
+
+#include "abi_util.hpp"
+#include "halo.hpp"
+#include "hyteg_amd.h"
+#include "runtime.hpp"
+
+namespace hg {
+
+constexpr int kPMax = 10;  // nonzero weights of one fine node
+constexpr int kRMax = 128; // fine nodes contributing to one coarse node
+
+__constant__ int8_t cPOff[64][kPMax][3];
+__constant__ double cPW[64][kPMax];
+__constant__ int    cPN[64];
+__constant__ int8_t cROff[8][kRMax][3];
+__constant__ double cRW[8][kRMax];
+__constant__ int    cRN[8];
+
+namespace {
+
+const int kVof[6][4][3] = {
+    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
+    { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
+    { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } }, { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
+const int kShift[6]   = { 0, 1, 1, 2, 1, 1 };
+const int kEdge[6][2] = { { 0, 1 }, { 0, 2 }, { 0, 3 }, { 1, 2 }, { 1, 3 }, { 2, 3 } };
+
+struct Located
+{
+   int    node[10][3]; // lattice L+1 coordinates of the 10 nodes
+   double phi[10];     // basis values at the fine point
+};
+
+// coarse micro-cell containing fine point F (lattice L+2 units) by the fixed
+// rule; returns its nodes and the basis values at F
+Located locate( const int F[3] )
+{
+   int A[3], r[3];
+   for ( int d = 0; d < 3; ++d )
+   {
+      A[d] = F[d] >= 0 ? F[d] / 4 : -( ( -F[d] + 3 ) / 4 );
+      r[d] = F[d] - 4 * A[d];
+   }
+   int order[6] = { 0, 1, 2, 4, 5, 3 }; // types by increasing row shift
+   for ( int oi = 0; oi < 6; ++oi )
+   {
+      const int t = order[oi];
+      // 4 lam' = M^-1 (r - 4 vof0), M columns vof_k - vof_0 (det +-1)
+      int M[3][3];
+      for ( int k = 0; k < 3; ++k )
+         for ( int d = 0; d < 3; ++d )
+            M[d][k] = kVof[t][k + 1][d] - kVof[t][0][d];
+      const int det = M[0][0] * ( M[1][1] * M[2][2] - M[1][2] * M[2][1] ) -
+                      M[0][1] * ( M[1][0] * M[2][2] - M[1][2] * M[2][0] ) +
+                      M[0][2] * ( M[1][0] * M[2][1] - M[1][1] * M[2][0] );
+      int adj[3][3];
+      adj[0][0] = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+      adj[0][1] = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+      adj[0][2] = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+      adj[1][0] = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+      adj[1][1] = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+      adj[1][2] = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+      adj[2][0] = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+      adj[2][1] = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+      adj[2][2] = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+      int rel[3], l4[4];
+      for ( int d = 0; d < 3; ++d )
+         rel[d] = r[d] - 4 * kVof[t][0][d];
+      int s = 0;
+      for ( int k = 0; k < 3; ++k )
+      {
+         l4[k + 1] = ( adj[k][0] * rel[0] + adj[k][1] * rel[1] + adj[k][2] * rel[2] ) * det;
+         s += l4[k + 1];
+      }
+      l4[0] = 4 - s;
+      if ( l4[0] < 0 || l4[1] < 0 || l4[2] < 0 || l4[3] < 0 )
+         continue;
+      Located out;
+      double  lam[4];
+      for ( int k = 0; k < 4; ++k )
+      {
+         lam[k] = 0.25 * l4[k];
+         for ( int d = 0; d < 3; ++d )
+            out.node[k][d] = 2 * ( A[d] + kVof[t][k][d] );
+         out.phi[k] = lam[k] * ( 2.0 * lam[k] - 1.0 );
+      }
+      for ( int e = 0; e < 6; ++e )
+      {
+         const int a = kEdge[e][0], b = kEdge[e][1];
+         for ( int d = 0; d < 3; ++d )
+            out.node[4 + e][d] = ( out.node[a][d] + out.node[b][d] ) / 2;
+         out.phi[4 + e] = 4.0 * lam[a] * lam[b];
+      }
+      return out;
+   }
+   throw Error( -1, "P2 transfer tables: point not located" );
+}
+
+struct Tables
+{
+   int8_t pOff[64][kPMax][3]{};
+   double pW[64][kPMax]{};
+   int    pN[64]{};
+   int8_t rOff[8][kRMax][3]{};
+   double rW[8][kRMax]{};
+   int    rN[8]{};
+};
+
+const Tables& tables()
+{
+   static Tables T;
+   static bool   built = false;
+   if ( built )
+      return T;
+   (void) kShift;
+   // pull stencils: fine point F = 4 A + r, offsets of the coarse nodes from 2 A
+   for ( int rz = 0; rz < 4; ++rz )
+      for ( int ry = 0; ry < 4; ++ry )
+         for ( int rx = 0; rx < 4; ++rx )
+         {
+            const int     cls  = rx + 4 * ry + 16 * rz;
+            const int     F[3] = { 64 + rx, 64 + ry, 64 + rz };
+            const Located l    = locate( F );
+            for ( int i = 0; i < 10; ++i )
+               if ( l.phi[i] != 0.0 )
+               {
+                  const int n = T.pN[cls]++;
+                  for ( int d = 0; d < 3; ++d )
+                     T.pOff[cls][n][d] = int8_t( l.node[i][d] - 32 );
+                  T.pW[cls][n] = l.phi[i];
+               }
+         }
+   // transpose stencils: coarse node C0 = 32 + q (lattice L+1), fine points F near 2 C0
+   for ( int q = 0; q < 8; ++q )
+   {
+      const int C0[3] = { 32 + ( q & 1 ), 32 + ( ( q >> 1 ) & 1 ), 32 + ( ( q >> 2 ) & 1 ) };
+      for ( int dz = -8; dz <= 8; ++dz )
+         for ( int dy = -8; dy <= 8; ++dy )
+            for ( int dx = -8; dx <= 8; ++dx )
+            {
+               const int     F[3] = { 2 * C0[0] + dx, 2 * C0[1] + dy, 2 * C0[2] + dz };
+               const Located l    = locate( F );
+               for ( int i = 0; i < 10; ++i )
+                  if ( l.phi[i] != 0.0 && l.node[i][0] == C0[0] && l.node[i][1] == C0[1] && l.node[i][2] == C0[2] )
+                  {
+                     const int n = T.rN[q]++;
+                     if ( n >= kRMax )
+                        throw Error( -1, "P2 restriction stencil too large" );
+                     T.rOff[q][n][0] = int8_t( dx );
+                     T.rOff[q][n][1] = int8_t( dy );
+                     T.rOff[q][n][2] = int8_t( dz );
+                     T.rW[q][n]      = l.phi[i];
+                  }
+            }
+   }
+   built = true;
+   return T;
+}
+
+void uploadTables()
+{
+   static bool done = false;
+   if ( done )
+      return;
+   const Tables& T = tables();
+   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPOff ), T.pOff, sizeof( T.pOff ) ) );
+   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPW ), T.pW, sizeof( T.pW ) ) );
+   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPN ), T.pN, sizeof( T.pN ) ) );
+   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cROff ), T.rOff, sizeof( T.rOff ) ) );
+   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRW ), T.rW, sizeof( T.rW ) ) );
+   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRN ), T.rN, sizeof( T.rN ) ) );
+   done = true;
+}
+
+} // namespace
+
+// value of the coarse quadratic at fine point (x,y,z) of cell c
+template < class T >
+__device__ __forceinline__ typename Acc< T >::type p2Interp( const T* __restrict__ coarse, int64_t c, int64_t scc,
+                                                             int64_t wc, int64_t x, int64_t y, int64_t z )
+{
+   using A         = typename Acc< T >::type;
+   const int cls   = int( x & 3 ) + 4 * int( y & 3 ) + 16 * int( z & 3 );
+   const int64_t bx = 2 * ( x >> 2 ), by = 2 * ( y >> 2 ), bz = 2 * ( z >> 2 );
+   A             v  = A( 0 );
+   for ( int e = 0; e < cPN[cls]; ++e )
+   {
+      const int64_t cx = bx + cPOff[cls][e][0], cy = by + cPOff[cls][e][1], cz = bz + cPOff[cls][e][2];
+      if ( cx < 0 || cy < 0 || cz < 0 || cx + cy + cz > wc - 1 )
+         continue;
+      v += A( cPW[cls][e] ) * toAcc( coarse[c * scc + tetIndex( wc, cx, cy, cz )] );
+   }
+   return v;
+}
+
+template < class T >
+__global__ void kP2ProlongCell( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t scf, int64_t wf,
+                                int64_t mmf, int64_t scc, int64_t wc, const T* __restrict__ coarse,
+                                T* __restrict__ fine, int add )
+{
+   using A          = typename Acc< T >::type;
+   const int64_t lb = xcdBlock( nrows * nc );
+   if ( lb >= nrows * nc )
+      return;
+   const int64_t c   = lb / nrows;
+   const int64_t r   = lb - c * nrows;
+   const int64_t y   = rows[2 * r];
+   const int64_t z   = rows[2 * r + 1];
+   const int64_t out = c * scf + tetRowStart( wf, y, z );
+   for ( int64_t x = 1 + threadIdx.x; x + y + z < mmf; x += blockDim.x )
+   {
+      A v = p2Interp( coarse, c, scc, wc, x, y, z );
+      if ( add )
+         v += toAcc( fine[out + x] );
+      fine[out + x] = fromAcc< T >( v );
+   }
+}
+
+template < class T >
+__global__ void kP2ProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
+                               int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine, int add )
+{
+   using A         = typename Acc< T >::type;
+   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nbf || !( bndFlag[b] & flag ) )
+      return;
+   bool done = false;
+   A    v    = A( 0 );
+   forEachIncident( gf, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
+      if ( done )
+         return;
+      done = true;
+      v    = p2Interp( coarse, c, scc, wc, x, y, z );
+   } );
+   const int64_t o = nc * gf.sc + b;
+   if ( add )
+      v += toAcc( fine[o] );
+   fine[o] = fromAcc< T >( v );
+}
+
+template < class T >
+__global__ void kP2RestrictCell( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t scc, int64_t wc,
+                                 int64_t mmc, int64_t scf, int64_t wf, const T* __restrict__ fine,
+                                 T* __restrict__ coarse )
+{
+   using A          = typename Acc< T >::type;
+   const int64_t lb = xcdBlock( nrows * nc );
+   if ( lb >= nrows * nc )
+      return;
+   const int64_t c   = lb / nrows;
+   const int64_t r   = lb - c * nrows;
+   const int64_t y   = rows[2 * r];
+   const int64_t z   = rows[2 * r + 1];
+   const int64_t out = c * scc + tetRowStart( wc, y, z );
+   const int64_t mmf = wf - 1;
+   for ( int64_t x = 1 + threadIdx.x; x + y + z < mmc; x += blockDim.x )
+   {
+      const int q = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
+      A         v = A( 0 );
+      for ( int e = 0; e < cRN[q]; ++e )
+      {
+         const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + cROff[q][e][1], fz = 2 * z + cROff[q][e][2];
+         if ( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf )
+            continue;
+         v += A( cRW[q][e] ) * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
+      }
+      coarse[out + x] = fromAcc< T >( v );
+   }
+}
+
+template < class T >
+__global__ void kP2RestrictBnd( BndGeom gc, int64_t nbc, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
+                                const double* __restrict__ clsInvN, int64_t scf, int64_t wf, int64_t mmf,
+                                const T* __restrict__ fine, T* __restrict__ coarse )
+{
+   using A         = typename Acc< T >::type;
+   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nbc || !( bndFlag[b] & flag ) )
+      return;
+   A s = A( 0 );
+   forEachIncident( gc, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
+      const int q = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
+      for ( int e = 0; e < cRN[q]; ++e )
+      {
+         const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + cROff[q][e][1], fz = 2 * z + cROff[q][e][2];
+         if ( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf )
+            continue;
+         const int cl = pointClass( mmf, fx, fy, fz );
+         const A   w  = A( cRW[q][e] ) * ( cl < 0 ? A( 1 ) : A( clsInvN[14 * c + cl] ) );
+         s += w * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
+      }
+   } );
+   coarse[nc * gc.sc + b] = fromAcc< T >( s );
+}
+
+static constexpr int kRowBlockT = 64;
+
+void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
+{
+   if ( f.off != 1 || fineL - 1 < f.minL || fineL > f.maxL )
+      throw Error( HG_ERR_ARG, "P2 restriction needs a P2 function holding levels fineLevel-1 and fineLevel" );
+   uploadTables();
+   Storage& st = *f.st;
+   Lattice& lf = st.lattice( ctx, fineL + 1 );
+   Lattice& lc = st.lattice( ctx, fineL );
+   void*    fp = f.ptr( fineL );
+   void*    cp = f.ptr( fineL - 1 );
+   pullHalos( ctx, st, lf, fp, f.dtype );
+   dispatchType( f.dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      if ( lc.nrows > 0 )
+      {
+         kP2RestrictCell< T ><<< xcdGrid( lc.nrows * lc.nc ), kRowBlockT, 0, ctx->stream >>>(
+             lc.rowsNat.p, lc.nrows, lc.nc, lc.sc, lc.w, lc.mm, lf.sc, lf.w, (const T*) fp, (T*) cp );
+         HG_CHECK_HIP( hipGetLastError() );
+      }
+      if ( lc.nb > 0 )
+      {
+         kP2RestrictBnd< T ><<< gridFor( lc.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lc ), lc.nb, lc.nc,
+                                                                             lc.bndFlag.p, flag, st.clsInvN.p, lf.sc,
+                                                                             lf.w, lf.mm, (const T*) fp, (T*) cp );
+         HG_CHECK_HIP( hipGetLastError() );
+      }
+   } );
+}
+
+void p2Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update )
+{
+   if ( f.off != 1 || coarseL < f.minL || coarseL + 1 > f.maxL )
+      throw Error( HG_ERR_ARG, "P2 prolongation needs a P2 function holding levels sourceLevel and sourceLevel+1" );
+   uploadTables();
+   Storage& st = *f.st;
+   Lattice& lf = st.lattice( ctx, coarseL + 2 );
+   Lattice& lc = st.lattice( ctx, coarseL + 1 );
+   void*    fp = f.ptr( coarseL + 1 );
+   void*    cp = f.ptr( coarseL );
+   pullHalos( ctx, st, lc, cp, f.dtype );
+   dispatchType( f.dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      if ( lf.nrows > 0 )
+      {
+         kP2ProlongCell< T ><<< xcdGrid( lf.nrows * lf.nc ), kRowBlockT, 0, ctx->stream >>>(
+             lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update );
+         HG_CHECK_HIP( hipGetLastError() );
+      }
+      if ( lf.nb > 0 )
+      {
+         kP2ProlongBnd< T ><<< gridFor( lf.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lf ), lf.nb, lf.nc,
+                                                                            lf.bndFlag.p, flag, lc.sc, lc.w,
+                                                                            (const T*) cp, (T*) fp, update );
+         HG_CHECK_HIP( hipGetLastError() );
+      }
+   } );
+}
+
+} // namespace hg
+
+using namespace hg;
+
+extern "C" {
+
+int hg_p2_restrict( hg_ctx* h, hg_handle fn, int sourceLevel, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) { p2Restrict( ctx, ctx->function( fn ), sourceLevel, flag ); } );
+}
+
+int hg_p2_prolongate( hg_ctx* h, hg_handle fn, int sourceLevel, int flag, int update )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( update != HG_REPLACE && update != HG_ADD )
+         throw Error( HG_ERR_ARG, "invalid update type" );
+      p2Prolongate( ctx, ctx->function( fn ), sourceLevel, flag, update == HG_ADD );
+   } );
+}
+
+int hg_p2_transfer_stencil( int kind, int cls, int max, int32_t* offsets, double* weights, int* count )
+{
+   try
+   {
+      const Tables& T = tables();
+      if ( !count || ( kind != 0 && kind != 1 ) || cls < 0 || cls >= ( kind == 0 ? 64 : 8 ) )
+         return HG_ERR_ARG;
+      const int n = kind == 0 ? T.pN[cls] : T.rN[cls];
+      *count      = n;
+      for ( int e = 0; e < n && e < max; ++e )
+      {
+         for ( int d = 0; d < 3; ++d )
+            offsets[3 * e + d] = kind == 0 ? T.pOff[cls][e][d] : T.rOff[cls][e][d];
+         weights[e] = kind == 0 ? T.pW[cls][e] : T.rW[cls][e];
+      }
+      return HG_OK;
+   }
+   catch ( ... )
+   {
+      return HG_ERR_ARG;
+   }
+}
+
+} // extern "C"

@@ -459,6 +459,14 @@ using namespace hg;
 // ==========================================================================
 // C ABI
 // ==========================================================================
+namespace hg {
+void setContextError( hg_ctx* h, const char* msg )
+{
+   if ( h )
+      reinterpret_cast< Context* >( h )->err = msg ? msg : "";
+}
+} // namespace hg
+
 extern "C" {
 
 int hg_ctx_create( int device, hg_ctx** out )

@@ -7,6 +7,11 @@
 
 #include "hyteg_amd/hyteg_amd.hpp"
 
+namespace hg {
+// records `msg` as the context's last error (defined with the context in runtime.hip)
+void setContextError( hg_ctx* ctx, const char* msg );
+} // namespace hg
+
 using namespace hyteg_amd;
 
 namespace {
@@ -18,25 +23,48 @@ std::shared_ptr< PrimitiveStorage > storageOf( hg_ctx* ctx, hg_handle fn )
    return PrimitiveStorage::borrow( ctx, s );
 }
 
-double residualNorm( const P1ConstantLaplaceOperator& A, const P1Function< double >& x, const P1Function< double >& b,
-                     const P1Function< double >& r, uint_t level )
+template < class Op, class F >
+double residualNorm( const Op& A, const F& x, const F& b, const F& r, uint_t level )
 {
    A.apply( x, r, level, Inner, Replace );
    r.assign( { 1.0, -1.0 }, { b, r }, level, Inner );
    return std::sqrt( r.dotGlobal( r, level, Inner ) );
 }
 
+// runs a solver body; a failure is reported as HG_ERR_STATE with its message
+// in hg_last_error (the failing ABI call inside already recorded its own)
 template < class F >
-int guarded( F&& f )
+int guarded( hg_ctx* ctx, F&& f )
 {
    try
    {
       f();
       return HG_OK;
-   } catch ( const std::exception& )
+   } catch ( const std::exception& e )
    {
+      hg::setContextError( ctx, e.what() );
       return HG_ERR_STATE;
    }
+}
+
+// V-cycles of `gmg` until the Inner residual drops by `tol` (0: run all cycles)
+template < class Op, class F, class G >
+int cycle( const Op& A, G& gmg, const F& x, const F& b, const F& r, uint_t level, int cycles, double tol, double* hist )
+{
+   const double r0 = residualNorm( A, x, b, r, level );
+   if ( hist )
+      hist[0] = r0;
+   int k = 0;
+   for ( ; k < cycles; ++k )
+   {
+      gmg.solve( A, x, b, level );
+      const double rk = residualNorm( A, x, b, r, level );
+      if ( hist )
+         hist[k + 1] = rk;
+      if ( tol > 0 && rk <= tol * r0 )
+         return k + 1;
+   }
+   return k;
 }
 
 } // namespace
@@ -48,7 +76,7 @@ int hg_p1_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int lev
 {
    if ( !ctx || maxIter < 0 )
       return HG_ERR_ARG;
-   return guarded( [&] {
+   return guarded( ctx, [&] {
       auto storage = storageOf( ctx, x );
       auto A       = P1ConstantLaplaceOperator::borrow( ctx, op );
       auto X       = P1Function< double >::borrow( ctx, x );
@@ -65,7 +93,7 @@ int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int mi
 {
    if ( !ctx || minLevel < 0 || maxLevel < minLevel || pre < 0 || post < 0 || cycles < 0 || coarseMaxIter < 0 )
       return HG_ERR_ARG;
-   return guarded( [&] {
+   return guarded( ctx, [&] {
       auto storage = storageOf( ctx, x );
       auto A       = P1ConstantLaplaceOperator::borrow( ctx, op );
       auto X       = P1Function< double >::borrow( ctx, x );
@@ -78,22 +106,91 @@ int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int mi
       GeometricMultigridSolver< P1ConstantLaplaceOperator > gmg( storage, smoother, coarse, R, P, uint_t( minLevel ),
                                                                 uint_t( maxLevel ), uint_t( pre ), uint_t( post ) );
       P1Function< double > r( "residual", storage, uint_t( maxLevel ), uint_t( maxLevel ) );
-      const double         r0 = residualNorm( *A, *X, *B, r, uint_t( maxLevel ) );
-      if ( res_hist )
-         res_hist[0] = r0;
-      int k = 0;
-      for ( ; k < cycles; ++k )
-      {
-         gmg.solve( *A, *X, *B, uint_t( maxLevel ) );
-         const double rk = residualNorm( *A, *X, *B, r, uint_t( maxLevel ) );
-         if ( res_hist )
-            res_hist[k + 1] = rk;
-         if ( tol > 0 && rk <= tol * r0 )
-         {
-            ++k;
-            break;
-         }
-      }
+      const int            k = cycle( *A, gmg, *X, *B, r, uint_t( maxLevel ), cycles, tol, res_hist );
+      if ( cycles_done )
+         *cycles_done = k;
+   } );
+}
+
+int hg_p2_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int maxIter, double relTol,
+                    double absTol, int jacobi, int* iterations )
+{
+   if ( !ctx || maxIter < 0 || level < 0 )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      using Op     = P2ElementwiseBlendingDivKGradOperator;
+      auto storage = storageOf( ctx, x );
+      auto A       = Op::borrow( storage, op, uint_t( level ), uint_t( level ) );
+      auto X       = P2Function< double >::borrow( ctx, x );
+      auto B       = P2Function< double >::borrow( ctx, b );
+      std::shared_ptr< Solver< Op > > prec;
+      if ( jacobi )
+         prec = std::make_shared< InverseDiagonalPreconditioner< Op > >();
+      CGSolver< Op > cg( storage, uint_t( level ), uint_t( level ), uint_t( maxIter ), relTol, absTol, prec );
+      cg.solve( *A, *X, *B, uint_t( level ) );
+      if ( iterations )
+         *iterations = int( cg.getIterations() );
+   } );
+}
+
+int hg_p2_chebyshev_smooth( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int order,
+                            double spectralRadius, int steps )
+{
+   if ( !ctx || level < 0 || order < 1 || order > 16 || steps < 0 || !( spectralRadius > 0 ) )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      using Op     = P2ElementwiseBlendingDivKGradOperator;
+      auto storage = storageOf( ctx, x );
+      auto A       = Op::borrow( storage, op, uint_t( level ), uint_t( level ) );
+      auto X       = P2Function< double >::borrow( ctx, x );
+      auto B       = P2Function< double >::borrow( ctx, b );
+      ChebyshevSmoother< Op > smoother( storage, uint_t( level ), uint_t( level ) );
+      smoother.setupCoefficients( uint_t( order ), spectralRadius );
+      for ( int i = 0; i < steps; ++i )
+         smoother.solve( *A, *X, *B, uint_t( level ) );
+   } );
+}
+
+int hg_p2_estimate_radius( hg_ctx* ctx, hg_handle op, hg_handle x, int level, int iterations, double* radius )
+{
+   if ( !ctx || !radius || iterations < 1 || level < 0 )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      using Op     = P2ElementwiseBlendingDivKGradOperator;
+      auto storage = storageOf( ctx, x );
+      auto A       = Op::borrow( storage, op, uint_t( level ), uint_t( level ) );
+      auto X       = P2Function< double >::borrow( ctx, x );
+      P2Function< double > tmp( "radius_tmp", storage, uint_t( level ), uint_t( level ) );
+      *radius = chebyshev::estimateRadius( *A, uint_t( level ), uint_t( iterations ), *X, tmp );
+   } );
+}
+
+int hg_p2_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre, int post,
+                     int chebyshevOrder, double spectralRadius, int cycles, double tol, int coarseMaxIter,
+                     double coarseTol, int coarseJacobi, double* res_hist, int* cycles_done )
+{
+   if ( !ctx || minLevel < 0 || maxLevel < minLevel || pre < 0 || post < 0 || cycles < 0 || coarseMaxIter < 0 ||
+        chebyshevOrder < 1 || chebyshevOrder > 16 || !( spectralRadius > 0 ) )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      using Op      = P2ElementwiseBlendingDivKGradOperator;
+      auto storage  = storageOf( ctx, x );
+      auto A        = Op::borrow( storage, op, uint_t( minLevel ), uint_t( maxLevel ) );
+      auto X        = P2Function< double >::borrow( ctx, x );
+      auto B        = P2Function< double >::borrow( ctx, b );
+      auto smoother = std::make_shared< ChebyshevSmoother< Op > >( storage, uint_t( minLevel ), uint_t( maxLevel ) );
+      smoother->setupCoefficients( uint_t( chebyshevOrder ), spectralRadius );
+      std::shared_ptr< Solver< Op > > prec;
+      if ( coarseJacobi )
+         prec = std::make_shared< InverseDiagonalPreconditioner< Op > >();
+      auto coarse = std::make_shared< CGSolver< Op > >( storage, uint_t( minLevel ), uint_t( minLevel ),
+                                                        uint_t( coarseMaxIter ), coarseTol, 1e-300, prec );
+      auto R      = std::make_shared< P2toP2QuadraticRestriction<> >();
+      auto P      = std::make_shared< P2toP2QuadraticProlongation<> >();
+      GeometricMultigridSolver< Op > gmg( storage, smoother, coarse, R, P, uint_t( minLevel ), uint_t( maxLevel ),
+                                          uint_t( pre ), uint_t( post ) );
+      P2Function< double > r( "residual", storage, uint_t( maxLevel ), uint_t( maxLevel ) );
+      const int            k = cycle( *A, gmg, *X, *B, r, uint_t( maxLevel ), cycles, tol, res_hist );
       if ( cycles_done )
          *cycles_done = k;
    } );

@@ -179,6 +179,20 @@ int hg_p1_restrict( hg_ctx* ctx, hg_handle fn, int sourceLevel, int flag );
  * (gridtransferoperators/P1toP1LinearProlongation.hpp:34-58, .cpp:194-411) */
 int hg_p1_prolongate( hg_ctx* ctx, hg_handle fn, int sourceLevel, int flag, int update );
 
+/* P2toP2QuadraticRestriction::restrict(f, sourceLevel, flag)
+ * (gridtransferoperators/P2toP2QuadraticRestriction.cpp:131-290) */
+int hg_p2_restrict( hg_ctx* ctx, hg_handle fn, int sourceLevel, int flag );
+/* P2toP2QuadraticProlongation::prolongate / prolongateAndAdd(f, sourceLevel, flag)
+ * (gridtransferoperators/P2toP2QuadraticProlongation.cpp:37-63,217-425) */
+int hg_p2_prolongate( hg_ctx* ctx, hg_handle fn, int sourceLevel, int flag, int update );
+/* Inspection of the translation-invariant P2 transfer stencils (no
+ * counterpart in the reference; used by tests).  kind 0: pull stencil of the
+ * fine class cls = (x&3) + 4(y&3) + 16(z&3), offsets from 2*(p>>2) in coarse
+ * lattice units; kind 1: restriction stencil of the coarse class
+ * cls = (x&1) + 2(y&1) + 4(z&1), offsets from 2*p in fine lattice units.
+ * Writes min(count, max) entries. */
+int hg_p2_transfer_stencil( int kind, int cls, int max, int32_t* offsets, double* weights, int* count );
+
 /* ---- solvers (host control flow, device kernels) ------------------------ */
 /* CGSolver(storage, minLevel, maxLevel, maxIter, relTol, absTol).solve(A, x, b, level)
  * (solvers/CGSolver.hpp:36-203).  *iterations receives the iteration count. */
@@ -196,6 +210,29 @@ int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int mi
  * kernel of the last entry point, recorded while event timing is enabled ---- */
 int hg_ctx_event_timing( hg_ctx* ctx, int enable );
 int hg_ctx_last_kernel_ms( hg_ctx* ctx, double* ms );
+
+/* CGSolver<P2ElementwiseOperator>(storage, level, level, maxIter, relTol, absTol,
+ * prec) on `level` (solvers/CGSolver.hpp:94-203); jacobi = 1 uses the operator's
+ * inverse diagonal as preconditioner (ApplyPreconditioner over the
+ * inverse-diagonal multiplication, SubstitutePreconditioner.hpp:120-158) */
+int hg_p2_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int maxIter, double relTol,
+                    double absTol, int jacobi, int* iterations );
+/* `steps` ChebyshevSmoother::solve calls on `level` (solvers/ChebyshevSmoother.hpp:112-232)
+ * after setupCoefficients(order, spectralRadius) with bounds 0.3/1.2 x radius */
+int hg_p2_chebyshev_smooth( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int order,
+                            double spectralRadius, int steps );
+/* chebyshev::estimateRadius(A, level, iterations, storage, x, tmp)
+ * (ChebyshevSmoother.hpp:656-665, numerictools/SpectrumEstimation.hpp:56-90):
+ * power iteration on D^-1 A from the start vector x (overwritten) */
+int hg_p2_estimate_radius( hg_ctx* ctx, hg_handle op, hg_handle x, int level, int iterations, double* radius );
+/* GeometricMultigridSolver<P2ElementwiseOperator> with ChebyshevSmoother
+ * (setupCoefficients(order, spectralRadius), bounds 0.3/1.2 x radius on every
+ * level), P2 quadratic transfer and (preconditioned) CG on minLevel, run as V(pre,
+ * post) cycles until the Inner residual norm drops by `tol` or `cycles` are done;
+ * res_hist[0..done] receives the residual norms */
+int hg_p2_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre, int post,
+                     int chebyshevOrder, double spectralRadius, int cycles, double tol, int coarseMaxIter,
+                     double coarseTol, int coarseJacobi, double* res_hist, int* cycles_done );
 
 #ifdef __cplusplus
 }

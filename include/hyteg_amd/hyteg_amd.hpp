@@ -12,6 +12,10 @@
 //   Solver<Op>                          solvers/Solver.hpp:27-34
 //   GaussSeidelSmoother, SORSmoother    solvers/GaussSeidelSmoother.hpp:31-55, SORSmoother.hpp
 //   CGSolver                            solvers/CGSolver.hpp:36-203
+//   P2Function<T>                       p2functionspace/P2Function.hpp
+//   P2ElementwiseOperator (div k grad)  elementwiseoperators/P2ElementwiseOperator.hpp:61-475
+//   P2toP2Quadratic{Restriction,Prolongation} gridtransferoperators/P2toP2Quadratic*.cpp
+//   ChebyshevSmoother, estimateRadius   solvers/ChebyshevSmoother.hpp:40-665
 //   GeometricMultigridSolver            solvers/GeometricMultigridSolver.hpp:38-308
 #pragma once
 
@@ -137,50 +141,56 @@ struct DTypeOf< float >
    static constexpr int value = HG_F32;
 };
 
-// P1Function<ValueType>.  Wraps an existing handle without owning it when
-// constructed through `borrow`.
-template < typename ValueType >
-class P1Function
+// P1Function<ValueType> (Space 1) and P2Function<ValueType> (Space 2): one
+// device array per level; a P2 function of level L holds the values of the
+// level-(L+1) vertex lattice (vertex DoFs at even, edge DoFs at odd points).
+// Wraps an existing handle without owning it when constructed through `borrow`.
+template < typename ValueType, int Space >
+class LatticeFunction
 {
  public:
    using valueType = ValueType;
 
-   P1Function( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
+   LatticeFunction( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
+                    uint_t maxLevel )
    : name_( name )
    , storage_( storage )
    , ctx_( storage->ctx() )
    , owned_( true )
    {
-      check( ctx_, hg_p1_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), DTypeOf< ValueType >::value, &h_ ),
-             "P1Function" );
+      const int st = Space == 1 ? hg_p1_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ),
+                                                         DTypeOf< ValueType >::value, &h_ ) :
+                                  hg_p2_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ),
+                                                         DTypeOf< ValueType >::value, &h_ );
+      check( ctx_, st, Space == 1 ? "P1Function" : "P2Function" );
    }
-   ~P1Function()
+   ~LatticeFunction()
    {
       if ( owned_ && h_ )
          hg_function_destroy( ctx_, h_ );
    }
-   P1Function( const P1Function& )            = delete;
-   P1Function& operator=( const P1Function& ) = delete;
+   LatticeFunction( const LatticeFunction& )            = delete;
+   LatticeFunction& operator=( const LatticeFunction& ) = delete;
 
-   static std::unique_ptr< P1Function > borrow( hg_ctx* ctx, hg_handle h )
+   static std::unique_ptr< LatticeFunction > borrow( hg_ctx* ctx, hg_handle h )
    {
-      return std::unique_ptr< P1Function >( new P1Function( ctx, h ) );
+      return std::unique_ptr< LatticeFunction >( new LatticeFunction( ctx, h ) );
    }
 
    hg_handle handle() const { return h_; }
    hg_ctx*   ctx() const { return ctx_; }
 
-   void assign( const std::vector< ValueType >&                                       scalars,
-                const std::vector< std::reference_wrapper< const P1Function > >& functions,
-                uint_t                                                            level,
-                DoFType                                                           flag = All ) const
+   void assign( const std::vector< ValueType >&                                            scalars,
+                const std::vector< std::reference_wrapper< const LatticeFunction > >& functions,
+                uint_t                                                                 level,
+                DoFType                                                                flag = All ) const
    {
       linComb( scalars, functions, level, flag, false );
    }
-   void add( const std::vector< ValueType >&                                       scalars,
-             const std::vector< std::reference_wrapper< const P1Function > >& functions,
-             uint_t                                                            level,
-             DoFType                                                           flag = All ) const
+   void add( const std::vector< ValueType >&                                            scalars,
+             const std::vector< std::reference_wrapper< const LatticeFunction > >& functions,
+             uint_t                                                                 level,
+             DoFType                                                                flag = All ) const
    {
       linComb( scalars, functions, level, flag, true );
    }
@@ -189,30 +199,39 @@ class P1Function
       check( ctx_, hg_function_interpolate_const( ctx_, h_, double( constant ), int( level ), flag ), "interpolate" );
    }
    void setToZero( uint_t level ) const { interpolate( ValueType( 0 ), level, All ); }
-   ValueType dotGlobal( const P1Function& other, uint_t level, DoFType flag = All ) const
+   ValueType dotGlobal( const LatticeFunction& other, uint_t level, DoFType flag = All ) const
    {
       double r = 0.0;
       check( ctx_, hg_function_dot( ctx_, h_, other.h_, int( level ), flag, &r ), "dotGlobal" );
       return ValueType( r );
    }
    template < typename OtherT >
-   void copyFrom( const P1Function< OtherT >& other, uint_t level ) const
+   void copyFrom( const LatticeFunction< OtherT, Space >& other, uint_t level ) const
    {
       check( ctx_, hg_function_copy_from( ctx_, h_, other.handle(), int( level ) ), "copyFrom" );
+   }
+   // multElementwise({a, b}, level, flag): this = a * b on flagged DoFs
+   void multElementwise( const std::vector< std::reference_wrapper< const LatticeFunction > >& functions, uint_t level,
+                         DoFType flag = All ) const
+   {
+      if ( functions.size() != 2 )
+         throw std::invalid_argument( "multElementwise takes two functions" );
+      check( ctx_, hg_function_mult( ctx_, h_, functions[0].get().h_, functions[1].get().h_, int( level ), flag ),
+             "multElementwise" );
    }
    void communicate( uint_t level ) const { check( ctx_, hg_function_communicate( ctx_, h_, int( level ) ), "communicate" ); }
 
  private:
-   P1Function( hg_ctx* ctx, hg_handle h )
+   LatticeFunction( hg_ctx* ctx, hg_handle h )
    : ctx_( ctx )
    , h_( h )
    , owned_( false )
    {}
-   void linComb( const std::vector< ValueType >&                                       scalars,
-                 const std::vector< std::reference_wrapper< const P1Function > >& functions,
-                 uint_t                                                            level,
-                 DoFType                                                           flag,
-                 bool                                                              add ) const
+   void linComb( const std::vector< ValueType >&                                            scalars,
+                 const std::vector< std::reference_wrapper< const LatticeFunction > >& functions,
+                 uint_t                                                                 level,
+                 DoFType                                                                flag,
+                 bool                                                                   add ) const
    {
       if ( scalars.size() != functions.size() )
          throw std::invalid_argument( "assign: sizes differ" );
@@ -241,6 +260,11 @@ class P1Function
    hg_handle                           h_     = 0;
    bool                                owned_ = true;
 };
+
+template < typename ValueType >
+using P1Function = LatticeFunction< ValueType, 1 >;
+template < typename ValueType >
+using P2Function = LatticeFunction< ValueType, 2 >;
 
 // Operator<Src,Dst>
 template < class SrcT, class DstT >
@@ -354,6 +378,115 @@ class P1toP1LinearProlongation : public ProlongationOperator< P1Function< ValueT
    }
 };
 
+// OperatorWithInverseDiagonal<F> (solvers/Smoothables.hpp:96-120)
+template < class F >
+class OperatorWithInverseDiagonal
+{
+ public:
+   virtual ~OperatorWithInverseDiagonal()                                = default;
+   virtual std::shared_ptr< F > getInverseDiagonalValues() const          = 0;
+   virtual void                 computeInverseDiagonalOperatorValues() = 0;
+};
+
+// P2ElementwiseOperator<p2_div_k_grad_blending_q4> with
+// k(x) = exp(sum_m a_m sin(omega_m . x + phi_m))
+// (elementwiseoperators/P2ElementwiseOperator.hpp:61-475)
+class P2ElementwiseBlendingDivKGradOperator : public Operator< P2Function< double >, P2Function< double > >,
+                                              public OperatorWithInverseDiagonal< P2Function< double > >
+{
+ public:
+   P2ElementwiseBlendingDivKGradOperator( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
+                                          uint_t maxLevel, const std::vector< double >& a = {},
+                                          const std::vector< double >& omega = {}, const std::vector< double >& phi = {} )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , owned_( true )
+   {
+      if ( omega.size() != 3 * a.size() || phi.size() != a.size() )
+         throw std::invalid_argument( "k: omega needs 3 entries per mode, phi one" );
+      check( ctx_,
+             hg_p2_divkgrad_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), int( a.size() ), a.data(),
+                                    omega.data(), phi.data(), &h_ ),
+             "P2ElementwiseBlendingDivKGradOperator" );
+   }
+   static std::unique_ptr< P2ElementwiseBlendingDivKGradOperator >
+       borrow( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel, uint_t maxLevel )
+   {
+      return std::unique_ptr< P2ElementwiseBlendingDivKGradOperator >(
+          new P2ElementwiseBlendingDivKGradOperator( storage, h, minLevel, maxLevel ) );
+   }
+   ~P2ElementwiseBlendingDivKGradOperator() override
+   {
+      if ( owned_ && h_ )
+         hg_operator_destroy( ctx_, h_ );
+   }
+   void apply( const P2Function< double >& src, const P2Function< double >& dst, uint_t level, DoFType flag,
+               UpdateType updateType = Replace ) const override
+   {
+      check( ctx_, hg_p2_apply( ctx_, h_, src.handle(), dst.handle(), int( level ), flag, updateType ), "apply" );
+   }
+   std::shared_ptr< P2Function< double > > getInverseDiagonalValues() const override
+   {
+      if ( !invDiag_ )
+         const_cast< P2ElementwiseBlendingDivKGradOperator* >( this )->computeInverseDiagonalOperatorValues();
+      return invDiag_;
+   }
+   void computeInverseDiagonalOperatorValues() override
+   {
+      invDiag_ = std::make_shared< P2Function< double > >( "inverseDiagonal", storage_, minLevel_, maxLevel_ );
+      for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
+         check( ctx_, hg_p2_inverse_diagonal( ctx_, h_, int( L ), invDiag_->handle() ), "inverse diagonal" );
+   }
+   hg_handle handle() const { return h_; }
+   uint_t    getMinLevel() const { return minLevel_; }
+   uint_t    getMaxLevel() const { return maxLevel_; }
+   const std::shared_ptr< PrimitiveStorage >& getStorage() const { return storage_; }
+
+ private:
+   P2ElementwiseBlendingDivKGradOperator( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel,
+                                          uint_t maxLevel )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , h_( h )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , owned_( false )
+   {}
+   std::shared_ptr< PrimitiveStorage >           storage_;
+   hg_ctx*                                       ctx_ = nullptr;
+   hg_handle                                     h_   = 0;
+   uint_t                                        minLevel_, maxLevel_;
+   bool                                          owned_;
+   std::shared_ptr< P2Function< double > >       invDiag_;
+};
+
+// P2toP2QuadraticRestriction / P2toP2QuadraticProlongation
+// (gridtransferoperators/P2toP2QuadraticRestriction.cpp, P2toP2QuadraticProlongation.cpp)
+template < typename ValueType = double >
+class P2toP2QuadraticRestriction : public RestrictionOperator< P2Function< ValueType > >
+{
+ public:
+   void restrict( const P2Function< ValueType >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      check( f.ctx(), hg_p2_restrict( f.ctx(), f.handle(), int( sourceLevel ), flag ), "restrict" );
+   }
+};
+template < typename ValueType = double >
+class P2toP2QuadraticProlongation : public ProlongationOperator< P2Function< ValueType > >
+{
+ public:
+   void prolongate( const P2Function< ValueType >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      check( f.ctx(), hg_p2_prolongate( f.ctx(), f.handle(), int( sourceLevel ), flag, HG_REPLACE ), "prolongate" );
+   }
+   void prolongateAndAdd( const P2Function< ValueType >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      check( f.ctx(), hg_p2_prolongate( f.ctx(), f.handle(), int( sourceLevel ), flag, HG_ADD ), "prolongateAndAdd" );
+   }
+};
+
 // Solver<Op>
 template < class OperatorType >
 class Solver
@@ -414,11 +547,13 @@ class CGSolver : public Solver< OperatorType >
 
    CGSolver( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel,
              uint_t maxIter = std::numeric_limits< uint_t >::max(), ValueType relativeTolerance = 1e-16,
-             ValueType absoluteTolerance = 1e-16 )
+             ValueType absoluteTolerance = 1e-16, std::shared_ptr< Solver< OperatorType > > preconditioner = nullptr )
    : maxIter_( maxIter )
    , relTol_( relativeTolerance )
    , absTol_( absoluteTolerance )
+   , prec_( std::move( preconditioner ) )
    , p_( "p", storage, minLevel, maxLevel )
+   , z_( "z", storage, minLevel, maxLevel )
    , ap_( "ap", storage, minLevel, maxLevel )
    , r_( "r", storage, minLevel, maxLevel )
    {}
@@ -428,15 +563,16 @@ class CGSolver : public Solver< OperatorType >
       if ( maxIter_ == 0 )
          return;
       p_.setToZero( level );
+      z_.setToZero( level );
       ap_.setToZero( level );
       r_.setToZero( level );
       A.apply( x, p_, level, flag_, Replace );
       r_.assign( { 1.0, -1.0 }, { b, p_ }, level, flag_ );
-      // identity preconditioner: z = r
-      p_.assign( { 1.0 }, { r_ }, level, flag_ );
-      ValueType prsold    = r_.dotGlobal( r_, level, flag_ );
-      ValueType res_start = std::sqrt( prsold );
-      iterations_         = 0;
+      precondition( A, level );
+      p_.assign( { 1.0 }, { z_ }, level, flag_ );
+      ValueType       prsold    = r_.dotGlobal( z_, level, flag_ );
+      const ValueType res_start = std::sqrt( r_.dotGlobal( r_, level, flag_ ) );
+      iterations_               = 0;
       if ( res_start < absTol_ )
          return;
       for ( uint_t i = 0; i < maxIter_; ++i )
@@ -451,20 +587,171 @@ class CGSolver : public Solver< OperatorType >
          iterations_            = i;
          if ( relRes < relTol_ || std::sqrt( rsnew ) < absTol_ )
             break;
-         const ValueType beta = rsnew / prsold;
-         p_.assign( { 1.0, beta }, { r_, p_ }, level, flag_ );
-         prsold = rsnew;
+         z_.interpolate( ValueType( 0 ), level, All );
+         precondition( A, level );
+         const ValueType prsnew = r_.dotGlobal( z_, level, flag_ );
+         const ValueType beta   = prsnew / prsold;
+         p_.assign( { 1.0, beta }, { z_, p_ }, level, flag_ );
+         prsold = prsnew;
       }
    }
    uint_t getIterations() const { return iterations_; }
 
  private:
-   uint_t       maxIter_;
-   ValueType    relTol_, absTol_;
-   FunctionType p_, ap_, r_;
-   DoFType      flag_       = Inner | NeumannBoundary | FreeslipBoundary;
-   uint_t       iterations_ = 0;
+   // z = M^-1 r (identity when no preconditioner is given)
+   void precondition( const OperatorType& A, uint_t level )
+   {
+      if ( prec_ )
+         prec_->solve( A, z_, r_, level );
+      else
+         z_.assign( { 1.0 }, { r_ }, level, flag_ );
+   }
+   uint_t                                    maxIter_;
+   ValueType                                 relTol_, absTol_;
+   std::shared_ptr< Solver< OperatorType > > prec_;
+   FunctionType                              p_, z_, ap_, r_;
+   DoFType                                   flag_       = Inner | NeumannBoundary | FreeslipBoundary;
+   uint_t                                    iterations_ = 0;
 };
+
+// Preconditioner z = D^-1 r with the operator's inverse diagonal
+// (ApplyPreconditioner over ApplyFunctionMultiplicationWrapper,
+// solvers/SubstitutePreconditioner.hpp:120-158)
+template < class OperatorType >
+class InverseDiagonalPreconditioner : public Solver< OperatorType >
+{
+ public:
+   using FunctionType = typename OperatorType::srcType;
+   void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      const auto* withDiag = dynamic_cast< const OperatorWithInverseDiagonal< FunctionType >* >( &A );
+      if ( !withDiag )
+         throw std::runtime_error( "the preconditioner requires the OperatorWithInverseDiagonal interface" );
+      x.multElementwise( { *withDiag->getInverseDiagonalValues(), b }, level, flag_ );
+   }
+
+ private:
+   DoFType flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+// ChebyshevSmoother (solvers/ChebyshevSmoother.hpp:40-420): x += p(D^-1 A) D^-1 (b - A x)
+// with the polynomial p of degree order-1 for which
+// 1 - t p(t) = T_order((theta - t)/delta) / T_order(theta/delta) on [lower, upper].
+template < class OperatorType >
+class ChebyshevSmoother : public Solver< OperatorType >
+{
+ public:
+   using FunctionType = typename OperatorType::srcType;
+   using ValueType    = typename FunctionType::valueType;
+
+   ChebyshevSmoother( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
+   : minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , coefficients_( maxLevel - minLevel + 1 )
+   , tmp1_( "cheby_tmp1", storage, minLevel, maxLevel )
+   , tmp2_( "cheby_tmp2", storage, minLevel, maxLevel )
+   {}
+
+   // setupCoefficients(order, spectralRadius, upperFactor, lowerFactor)
+   void setupCoefficients( uint_t order, double spectralRadius, double upperFactor = 1.2, double lowerFactor = 0.3 )
+   {
+      for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
+         coefficients_[L - minLevel_] = coefficientsFor( order, lowerFactor * spectralRadius, upperFactor * spectralRadius );
+   }
+   void setupCoefficientsOnLevel( uint_t order, double spectralRadius, uint_t level, double upperFactor = 1.2,
+                                  double lowerFactor = 0.3 )
+   {
+      coefficients_.at( level - minLevel_ ) =
+          coefficientsFor( order, lowerFactor * spectralRadius, upperFactor * spectralRadius );
+   }
+   const std::vector< double >& coefficients( uint_t level ) const { return coefficients_.at( level - minLevel_ ); }
+
+   // monomial coefficients of p from the Chebyshev recurrence T_{j+1} = 2 s T_j - T_{j-1}
+   // in s = (theta - t) / delta
+   static std::vector< double > coefficientsFor( uint_t order, double lower, double upper )
+   {
+      if ( order < 1 || order > 16 )
+         throw std::invalid_argument( "Chebyshev order must be 1..16" );
+      const double          theta = 0.5 * ( upper + lower ), delta = 0.5 * ( upper - lower );
+      const double          a = theta / delta, b = -1.0 / delta;
+      std::vector< double > prev( order + 2, 0.0 ), cur( order + 2, 0.0 ), next( order + 2, 0.0 );
+      prev[0] = 1.0;
+      cur[0]  = a;
+      cur[1]  = b;
+      for ( uint_t j = 1; j < order; ++j )
+      {
+         for ( uint_t i = 0; i <= j + 1; ++i )
+            next[i] = 2.0 * ( a * cur[i] + ( i > 0 ? b * cur[i - 1] : 0.0 ) ) - prev[i];
+         prev = cur;
+         cur  = next;
+      }
+      std::vector< double > c( order );
+      for ( uint_t i = 0; i < order; ++i )
+         c[i] = -cur[i + 1] / cur[0];
+      return c;
+   }
+
+   void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      const auto* withDiag = dynamic_cast< const OperatorWithInverseDiagonal< FunctionType >* >( &A );
+      if ( !withDiag )
+         throw std::runtime_error( "The Chebyshev-Smoother requires the OperatorWithInverseDiagonal interface." );
+      const auto&                  invDiag = *withDiag->getInverseDiagonalValues();
+      const std::vector< double >& c       = coefficients_.at( level - minLevel_ );
+      if ( c.empty() )
+         throw std::runtime_error( "Chebyshev coefficients are not set up" );
+      A.apply( x, tmp2_, level, flag_ );
+      tmp2_.assign( { 1.0, -1.0 }, { b, tmp2_ }, level, flag_ );
+      tmp1_.setToZero( level );
+      tmp1_.multElementwise( { invDiag, tmp2_ }, level, flag_ );
+      x.add( { ValueType( c[0] ) }, { tmp1_ }, level, flag_ );
+      for ( size_t k = 1; k < c.size(); ++k )
+      {
+         A.apply( tmp1_, tmp2_, level, flag_ );
+         tmp1_.setToZero( level );
+         tmp1_.multElementwise( { invDiag, tmp2_ }, level, flag_ );
+         x.add( { ValueType( c[k] ) }, { tmp1_ }, level, flag_ );
+      }
+   }
+
+ private:
+   uint_t                               minLevel_, maxLevel_;
+   std::vector< std::vector< double > > coefficients_;
+   FunctionType                         tmp1_, tmp2_;
+   DoFType                              flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+namespace chebyshev {
+// estimateRadius (ChebyshevSmoother.hpp:656-665): power iteration on D^-1 A with
+// flag All (numerictools/SpectrumEstimation.hpp:56-90); x is the start vector and
+// holds the last iterate on return
+template < class OperatorType >
+double estimateRadius( const OperatorType& A, uint_t level, uint_t maxIter, const typename OperatorType::srcType& x,
+                       const typename OperatorType::srcType& tmp )
+{
+   using FunctionType   = typename OperatorType::srcType;
+   const auto* withDiag = dynamic_cast< const OperatorWithInverseDiagonal< FunctionType >* >( &A );
+   if ( !withDiag )
+      throw std::runtime_error( "estimateRadius requires the OperatorWithInverseDiagonal interface" );
+   const auto& invDiag = *withDiag->getInverseDiagonalValues();
+   auto        applyDA = [&]( const FunctionType& src, const FunctionType& dst ) {
+      A.apply( src, dst, level, All );
+      dst.multElementwise( { invDiag, dst }, level, All );
+   };
+   double norm = std::sqrt( double( x.dotGlobal( x, level, All ) ) );
+   x.assign( { 1.0 / norm }, { x }, level, All );
+   applyDA( x, tmp );
+   double radius = 0.0;
+   for ( uint_t it = 1; it <= maxIter; ++it )
+   {
+      norm = std::sqrt( double( tmp.dotGlobal( tmp, level, All ) ) );
+      x.assign( { 1.0 / norm }, { tmp }, level, All );
+      applyDA( x, tmp );
+      radius = double( x.dotGlobal( tmp, level, All ) );
+   }
+   return radius;
+}
+} // namespace chebyshev
 
 // GeometricMultigridSolver (GeometricMultigridSolver.hpp:126-308)
 template < class OperatorType >
