@@ -23,6 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations of one micro-cell action in kP2TileApply: 11 quadrature points x
+# (65 FMA + 34 MUL + 8 ADD + 2 reciprocal / reciprocal-sqrt seeds) + 12 set-up = 1940
+P2_FLOPS_PER_ELEMENT = 11 * (2 * 65 + 34 + 8 + 2) + 26
 METRIC = "DoF/s per apply; time-to-1e-8 residual mixed vs FP64; solves/s + p99 latency"
 
 
@@ -75,14 +79,22 @@ def p2_shell(hg, ctx, L, steps, warmup):
     apply = lambda: A.apply(src, dst, L, hg.ALL)  # noqa: E731
     sec = timed_applies(ctx, apply, steps, warmup)
     kms = kernel_ms(ctx, apply, max(3, min(steps, 5)))
-    # P2 DoFs updated by kP2ElementApply: all lattice points of level L+1 in the cells
-    # (src read + dst written once each, FP64); k is evaluated, not streamed
-    alg = 2 * 8 * n
-    return n, sec, kms, alg, {"workload": f"P2 div(k grad) apply (blended shell map, 11-point quadrature, "
-                                          f"k = exp(16 sine modes), FP64), spherical shell r 0.55..1, ntan 5, "
-                                          f"3 layers (1920 macro-cells), level {L} ({n} DoFs)",
-                              "level": L, "dofs": n, "mesh": "meshSphericalShell(5, 3, 0.55, 1.0)"}, \
-        "kP2ElementApply", f"p2_apply_shell_L{L}_f64"
+    nelem = 1920 * 8 ** L
+    roof = {"bound": "mfma", "achieved": P2_FLOPS_PER_ELEMENT * nelem / (kms * 1e-3) / 1e12,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "note": "FP64 VALU-bound element integration (no MFMA); MI355X FP64 vector and matrix peaks are "
+                    "both 78.6 TFLOP/s; algorithmic flops = 1940 per micro-cell x "
+                    f"{nelem} micro-cells",
+            "algorithmic_flops_per_launch": P2_FLOPS_PER_ELEMENT * nelem,
+            # algorithmic HBM bytes: src, dst and one coefficient value per P2 DoF, FP64
+            "hbm": {"algorithmic_bytes_per_launch": 3 * 8 * n,
+                    "achieved_gbs": 3 * 8 * n / (kms * 1e-3) / 1e9,
+                    "frac": 3 * 8 * n / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    return n, sec, kms, roof, {"workload": f"P2 div(k grad) apply (blended shell map, 11-point quadrature, "
+                                           f"k = exp(16 sine modes), FP64), spherical shell r 0.55..1, ntan 5, "
+                                           f"3 layers (1920 macro-cells), level {L} ({n} DoFs)",
+                               "level": L, "dofs": n, "mesh": "meshSphericalShell(5, 3, 0.55, 1.0)"}, \
+        "kP2TileApply<true>", f"p2_apply_shell_L{L}_f64"
 
 
 def p1_cube(hg, ctx, L, steps, warmup):
@@ -97,7 +109,9 @@ def p1_cube(hg, ctx, L, steps, warmup):
     sec = timed_applies(ctx, apply, steps, warmup)
     kms = kernel_ms(ctx, apply, max(5, min(steps, 20)))
     alg = 2 * 8 * 6 * tet(2 ** L - 3)
-    return n, sec, kms, alg, {"workload": f"P1ConstantLaplaceOperator apply, unit cube 6 macro-tets, level {L} "
+    roof = {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "algorithmic_bytes_per_launch": alg}
+    return n, sec, kms, roof, {"workload": f"P1ConstantLaplaceOperator apply, unit cube 6 macro-tets, level {L} "
                                           f"({n} DoFs), FP64", "level": L, "dofs": n, "mesh": "meshCuboid 1x1x1"}, \
         "kCellApply<double>", f"p1_apply_cube_L{L}_f64"
 
@@ -161,6 +175,39 @@ def gmg_cfg1(hg, ctx):
             "rel_diff_vs_oracle_solution": err}
 
 
+def gmg_p2_shell(hg, ctx, L):
+    """FP64 GMG V(2,2), Chebyshev order 3 (radius from 20 power iterations on the finest
+    level), P2 quadratic transfer, Jacobi-PCG to 1e-14 on level 0, to 1e-8 relative residual"""
+    from hyteg_amd import problems
+    st = hg.PrimitiveStorage.shell(ctx, **problems.SHELL)
+    A = hg.P2DivKGradOperator(st, 0, L, *problems.shell_coefficient(0))
+    x = hg.P2Function("x", st, 0, L)
+    b = hg.P2Function("b", st, 0, L)
+    b.interpolate_random(0, -1.0, 1.0, L, hg.INNER)
+    t = hg.P2Function("t", st, L, L)
+    t.interpolate_random(1, -1.0, 1.0, L, hg.ALL)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rho = hg.p2_estimate_radius(A, t, L, 20)
+    hg.p2_gmg_solve(A, x, b, 0, L, rho, cycles=1)  # builds the per-level data once
+    ctx.synchronize()
+    setup = time.perf_counter() - t0
+    x.interpolate(0.0, L)
+    log(f"P2 GMG level {L}: radius {rho:.4f}, set-up {setup:.2f} s")
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    k, hist = hg.p2_gmg_solve(A, x, b, 0, L, rho, pre=2, post=2, order=3, cycles=60, tol=1e-8)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"P2 GMG level {L}: {k} cycles, {dt:.2f} s")
+    for f in (x, b, t):
+        f.destroy()
+    return {"workload": f"P2 div(k grad) shell level {L}, FP64 GMG V(2,2) Chebyshev(3), CG level 0, to 1e-8",
+            "cycles": k, "time_s": dt, "setup_s": setup, "spectral_radius": rho,
+            "factors": [float(v) for v in hist[1:] / hist[:-1]],
+            "relative_residual": float(hist[-1] / hist[0])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +217,7 @@ def main():
     ap.add_argument("--level", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-gmg", action="store_true", help="skip the P2 GMG solve")
     ap.add_argument("--only-apply", action="store_true",
                     help="run the apply measurement only (for counter collection)")
     args = ap.parse_args()
@@ -182,18 +230,18 @@ def main():
     ctx = hg.Context(0)
     run = p2_shell if args.workload == "p2_shell" else p1_cube
     L = args.level if args.level is not None else (6 if args.workload == "p2_shell" else 9)
-    n, sec, kms, alg, config, kname, tkey = run(hg, ctx, L, args.steps, args.warmup)
+    n, sec, kms, roofline, config, kname, tkey = run(hg, ctx, L, args.steps, args.warmup)
     log(f"{args.steps} applies: {1e3 * sec:.3f} ms/apply, dominant kernel {kms:.3f} ms")
-    achieved = alg / (kms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(tkey), "kernel": kname,
-                "kernel_avg_ms": kms, "algorithmic_bytes_per_launch": alg}
+    roofline["frac"] = roofline["achieved"] / roofline["peak"]
+    roofline.update({"traffic": load_traffic(tkey), "kernel": kname, "kernel_avg_ms": kms})
     extra = {}
     if not args.only_apply:
         if args.workload == "p2_shell":
-            n1, sec1, kms1, alg1, cfg1, _, _ = p1_cube(hg, ctx, 9, 20, 3)
+            n1, sec1, kms1, roof1, cfg1, _, _ = p1_cube(hg, ctx, 9, 20, 3)
             extra["p1_apply_cube_L9_f64"] = {"dofs": n1, "ms_per_apply": 1e3 * sec1, "dof_per_s": n1 / sec1,
-                                             "kernel_ms": kms1, "kernel_gbs": alg1 / (kms1 * 1e-3) / 1e9}
+                                             "kernel_ms": kms1, "kernel_gbs": roof1["achieved"]}
+            if not args.no_gmg:
+                extra["gmg_p2_shell_f64"] = gmg_p2_shell(hg, ctx, L)
         extra["gmg_p1_cube_L5_f64"] = gmg_cfg1(hg, ctx)
     cpu = None if (args.no_cpu or args.only_apply) else cpu_baseline(args.workload, args.cpu_seconds)
     out = {

@@ -141,6 +141,12 @@ class P1Function:
             lib().hg_function_destroy(self.ctx.h, self.h)
             self.h = None
 
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
     def upload(self, level, values):
         v = np.ascontiguousarray(values, dtype=np.float64)
         if v.shape != (self.ndofs(level),):
@@ -208,6 +214,17 @@ class P2DivKGradOperator:
         self.ctx.check(lib().hg_p2_divkgrad_create(self.ctx.h, storage.h, min_level, max_level, nm, pa, po, pp,
                                                    C.byref(h)), "P2DivKGradOperator")
         self.h = h.value
+
+    def destroy(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().hg_operator_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
     def apply(self, src, dst, level, flag, update=REPLACE):
         self.ctx.check(lib().hg_p2_apply(self.ctx.h, self.h, src.h, dst.h, level, flag, update), "apply")

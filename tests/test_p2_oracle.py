@@ -37,10 +37,8 @@ def test_shell_cells_have_positive_volume_and_tile_the_shell(shell):
     P = X[C]
     vol = np.einsum("ij,ij->i", P[:, 1] - P[:, 0], np.cross(P[:, 2] - P[:, 0], P[:, 3] - P[:, 0])) / 6
     assert np.all(np.abs(vol) > 1e-6)
-    # the polyhedral shell volume: both spheres are triangulated by the same
-    # faces scaled radially, so the volume is (1 - 0.55^3) * (volume of the unit polyhedron)
+    # the flat-faced shell lies inside the curved one and fills most of it
     bnd = np.abs(np.linalg.norm(X, axis=1) - 1.0) < 1e-12
-    assert np.isclose(np.abs(vol).sum(), (1 - 0.55 ** 3) / (1 - 0.775 ** 3) * np.abs(vol)[:960].sum()) or True
     assert np.abs(vol).sum() < 4 / 3 * np.pi * (1 - 0.55 ** 3)
     assert np.abs(vol).sum() > 0.9 * 4 / 3 * np.pi * (1 - 0.55 ** 3)
     assert bnd.sum() == 162
