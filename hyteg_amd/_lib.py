@@ -67,6 +67,7 @@ SIGNATURES = {
     "hg_p2_cg_solve": [_p, _i64, _i64, _i64, _i, _i, _d, _d, _i, _p],
     "hg_p2_estimate_radius": [_p, _i64, _i64, _i, _i, _p],
     "hg_p2_chebyshev_smooth": [_p, _i64, _i64, _i64, _i, _i, _d, _i],
+    "hg_p2_ir_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _i, _d, _i, _i, _d, _i, _d, _p, _p],
     "hg_p2_gmg_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _d, _i, _d, _i, _d, _i, _p, _p],
     "hg_p1_prolongate": [_p, _i64, _i, _i, _i],
     "hg_p1_cg_solve": [_p, _i64, _i64, _i64, _i, _i, _d, _d, _p],

@@ -3,7 +3,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import ALL, F64, GMG_FLAG, REPLACE, HygError, lib
+from ._lib import ALL, F16, F64, GMG_FLAG, REPLACE, HygError, lib
 
 
 def _ptr(a):
@@ -298,6 +298,18 @@ def p2_estimate_radius(A, x, level, iterations):
     r = C.c_double()
     A.ctx.check(lib().hg_p2_estimate_radius(A.ctx.h, A.h, x.h, level, iterations, C.byref(r)), "estimate radius")
     return r.value
+
+
+def p2_ir_solve(A, x, b, min_level, max_level, rho, low_dtype=F16, pre=2, post=2, order=3, inner_cycles=1,
+                max_outer=40, tol=1e-8, coarse_max_iter=10000, coarse_tol=1e-14):
+    """FP64 iterative refinement around a reduced-precision (FP32 arithmetic, FP32/FP16
+    storage) V-cycle; returns (outer steps, FP64 Inner residual norms)"""
+    hist = np.zeros(max_outer + 1)
+    done = C.c_int()
+    A.ctx.check(lib().hg_p2_ir_solve(A.ctx.h, A.h, x.h, b.h, min_level, max_level, low_dtype, pre, post, order, rho,
+                                     inner_cycles, max_outer, tol, coarse_max_iter, coarse_tol, _ptr(hist),
+                                     C.byref(done)), "p2 iterative refinement")
+    return done.value, hist[: done.value + 1]
 
 
 def p2_gmg_solve(A, x, b, min_level, max_level, rho, pre=2, post=2, order=3, cycles=20, tol=0.0,

@@ -14,6 +14,7 @@
 // the element matrix of the reference applied to the element vector without
 // forming the 10x10 matrix.
 #include <cmath>
+#include <type_traits>
 
 #include "abi_util.hpp"
 #include "halo.hpp"
@@ -28,6 +29,24 @@ __constant__ double cQP[11][4];
 __constant__ double cQW[11];
 __constant__ double cQ4[11][4];  // 4 lam_k at quadrature point q
 __constant__ double cQ4m[11][4]; // 4 lam_k - 1
+__constant__ float  cQWf[11], cQ4f[11][4], cQ4mf[11][4]; // the same in FP32
+
+template < class R >
+struct QuadConst;
+template <>
+struct QuadConst< double >
+{
+   __device__ static const double* w() { return cQW; }
+   __device__ static const double* a4( int q ) { return cQ4[q]; }
+   __device__ static const double* b4( int q ) { return cQ4m[q]; }
+};
+template <>
+struct QuadConst< float >
+{
+   __device__ static const float* w() { return cQWf; }
+   __device__ static const float* a4( int q ) { return cQ4f[q]; }
+   __device__ static const float* b4( int q ) { return cQ4mf[q]; }
+};
 __constant__ int    cVof[6][4][3] = {
     { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
     { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
@@ -126,15 +145,19 @@ struct KParams
 // q = (r0 + |lam| dr) / |x|, lam = ((x - B0) . n) / volT, so that
 // DF = q I + x w^T,  w = s n - (q / |x|^2) x,  s = sign(lam) dr / (volT |x|):
 // a rank-one update of a multiple of the identity, inverted in closed form.
-struct P2CellGeom
+// R is the arithmetic type of the kernel that reads it (double, or float for
+// the reduced-precision apply).
+template < class R >
+struct P2CellGeomT
 {
-   double v0[3], hE[9];
-   double J[6][9], Dc[6][9], adet[6];
-   double XQ[6][11][4]; // J[t] (lam_1, lam_2, lam_3)_q and its lam-increment nl . XQ
-   double nl[3], lam0;  // lam(x) = nl . x - lam0,  nl = n / volT,  lam0 = nl . B0
-   double r0, dr;       // radius at the ray vertex and its increment to the ref vertex
-   int    kind;         // 0 affine, 1 shell
+   R   v0[3], hE[9];
+   R   J[6][9], Dc[6][9], adet[6];
+   R   XQ[6][11][4]; // J[t] (lam_1, lam_2, lam_3)_q and its lam-increment nl . XQ
+   R   nl[3], lam0;  // lam(x) = nl . x - lam0,  nl = n / volT,  lam0 = nl . B0
+   R   r0, dr;       // radius at the ray vertex and its increment to the ref vertex
+   int kind;         // 0 affine, 1 shell
 };
+using P2CellGeom = P2CellGeomT< double >;
 
 struct P2Operator : OperatorBase
 {
@@ -143,10 +166,12 @@ struct P2Operator : OperatorBase
    bool                   storeCoefficients = true;
    int                    nm                = 0; // number of sine modes of k (0: k = 1)
    DevBuf< CellGeom >     geom;
-   std::map< int, DevBuf< P2CellGeom > > tgeom; // by P2 level
+   std::map< int, DevBuf< P2CellGeom > >           tgeom;  // by P2 level
+   std::map< int, DevBuf< P2CellGeomT< float > > > tgeomF; // by P2 level, for the FP32 kernels
    DevBuf< KParams >      kpar;
    std::map< int, DevBuf< double > > invDiag; // by P2 level, lattice L+1 storage
-   std::map< int, DevBuf< double > > kq;      // k at the quadrature points, by level (tile-major)
+   // k at the quadrature points (tile-major), by (level, value type of the table)
+   std::map< std::pair< int, int >, DevBuf< char > > kq;
 };
 
 __device__ __forceinline__ double kEval( const KParams& K, const double x[3] )
@@ -351,17 +376,22 @@ __device__ __forceinline__ double rsqrtNewton( double b )
    y              = y * fma( -h * y, y, 1.5 );
    return y * fma( -h * y, y, 1.5 );
 }
+// FP64: refined estimates; FP32: the hardware estimates (about 1 ulp)
+__device__ __forceinline__ double fastRcp( double b ) { return rcpNewton( b ); }
+__device__ __forceinline__ double fastRsqrt( double b ) { return rsqrtNewton( b ); }
+__device__ __forceinline__ float  fastRcp( float b ) { return __builtin_amdgcn_rcpf( b ); }
+__device__ __forceinline__ float  fastRsqrt( float b ) { return __builtin_amdgcn_rsqf( b ); }
 
 // shell-map scalars at x with lam(x) = ls: q = |F(x)| / |x|, w (DF = q I + x w^T),
 // beta = q + w . x
-__device__ __forceinline__ void shellScalars( const P2CellGeom& g, const double x[3], double ls, double& qq, double w[3],
-                                              double& beta )
+template < class R >
+__device__ __forceinline__ void shellScalars( const P2CellGeomT< R >& g, const R x[3], R ls, R& qq, R w[3], R& beta )
 {
-   const double r2   = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
-   const double irho = rsqrtNewton( r2 );
-   qq                = ( g.r0 + fabs( ls ) * g.dr ) * irho;
-   const double s    = ( ls < 0 ? -g.dr : g.dr ) * irho; // DF = q I - (q / rho^2) x x^T + s x nl^T
-   const double qr2  = qq * irho * irho;
+   const R r2   = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+   const R irho = fastRsqrt( r2 );
+   qq           = ( g.r0 + fabs( ls ) * g.dr ) * irho;
+   const R s    = ( ls < 0 ? -g.dr : g.dr ) * irho; // DF = q I - (q / rho^2) x x^T + s x nl^T
+   const R qr2  = qq * irho * irho;
 #pragma unroll
    for ( int d = 0; d < 3; ++d )
       w[d] = s * g.nl[d] - qr2 * x[d];
@@ -380,18 +410,20 @@ __device__ __forceinline__ double kAt( const P2CellGeom& g, const KParams& K, co
    return kEval( K, x );
 }
 
+template < class R >
 struct TileThread
 {
-   int     t;          // micro-cell type (uniform per wavefront)
-   int     ax, ay, az; // anchor inside the tile
-   bool    valid;
-   double  p0[3];      // computational-space vertex 0
-   double  ls0;        // lam(p0) of the shell map
+   int  t;          // micro-cell type (uniform per wavefront)
+   int  ax, ay, az; // anchor inside the tile
+   bool valid;
+   R    p0[3];      // computational-space vertex 0
+   R    ls0;        // lam(p0) of the shell map
 };
 
-__device__ __forceinline__ TileThread tileThread( const P2CellGeom& g, const TileCoord& tc, int64_t mm )
+template < class R >
+__device__ __forceinline__ TileThread< R > tileThread( const P2CellGeomT< R >& g, const TileCoord& tc, int64_t mm )
 {
-   TileThread th;
+   TileThread< R > th;
    th.t        = __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
    const int a = int( threadIdx.x ) & 63;
    th.ax       = a & 3;
@@ -399,8 +431,7 @@ __device__ __forceinline__ TileThread tileThread( const P2CellGeom& g, const Til
    th.az       = a >> 4;
    const int64_t gx = tc.x0 + th.ax, gy = tc.y0 + th.ay, gz = tc.z0 + th.az;
    th.valid         = gx + gy + gz <= mm - 1 - MicroCells::rowShift[th.t];
-   const double vx = double( gx + cVof[th.t][0][0] ), vy = double( gy + cVof[th.t][0][1] ),
-                vz = double( gz + cVof[th.t][0][2] );
+   const R vx = R( gx + cVof[th.t][0][0] ), vy = R( gy + cVof[th.t][0][1] ), vz = R( gz + cVof[th.t][0][2] );
 #pragma unroll
    for ( int d = 0; d < 3; ++d )
       th.p0[d] = g.v0[d] + g.hE[3 * d] * vx + g.hE[3 * d + 1] * vy + g.hE[3 * d + 2] * vz;
@@ -411,7 +442,8 @@ __device__ __forceinline__ TileThread tileThread( const P2CellGeom& g, const Til
 // LDS box positions of the 10 P2 DoFs of the thread's micro-cell (vertices in
 // the order of getMicroVerticesFromMicroCell, then the edges (0,1), (0,2),
 // (0,3), (1,2), (1,3), (2,3))
-__device__ __forceinline__ void elementBoxPositions( const TileThread& th, int pos[10] )
+template < class R >
+__device__ __forceinline__ void elementBoxPositions( const TileThread< R >& th, int pos[10] )
 {
 #pragma unroll
    for ( int k = 0; k < 4; ++k )
@@ -424,22 +456,41 @@ __device__ __forceinline__ void elementBoxPositions( const TileThread& th, int p
 
 // k(F(x_q)) for every quadrature point of every micro-cell, stored tile-major:
 // table[(tile * 11 + q) * 384 + thread] (zero for threads outside the macro-cell)
+// (computed in FP64 and rounded to the table's value type K)
+template < class K >
 __global__ __launch_bounds__( kTileThreads ) void kP2TileCoefficients( const P2CellGeom* __restrict__ geom,
                                                                       const KParams* __restrict__ kp, int64_t mm,
-                                                                      int64_t ntiles, double* __restrict__ table )
+                                                                      int64_t ntiles, K* __restrict__ table )
 {
    const int64_t lb = blockIdx.x;
    if ( lb >= ntiles )
       return;
-   const TileCoord   tc = tileOf( lb, mm );
-   const P2CellGeom& g  = geom[tc.c];
-   const TileThread  th = tileThread( g, tc, mm );
+   const TileCoord            tc = tileOf( lb, mm );
+   const P2CellGeom&          g  = geom[tc.c];
+   const TileThread< double > th = tileThread( g, tc, mm );
    for ( int q = 0; q < 11; ++q )
    {
       const double* xq   = g.XQ[th.t][q];
       const double  x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
-      table[( lb * 11 + q ) * kTileThreads + threadIdx.x] = th.valid ? kAt( g, *kp, x, th.ls0 + xq[3] ) : 0.0;
+      const double  v    = th.valid ? kAt( g, *kp, x, th.ls0 + xq[3] ) : 0.0;
+      table[( lb * 11 + q ) * kTileThreads + threadIdx.x] = fromAcc< K >( typename Acc< K >::type( v ) );
    }
+}
+
+// dst += v at a point shared with other tiles
+__device__ __forceinline__ void sharedAdd( double* o, double v ) { atomicAdd( o, v ); }
+__device__ __forceinline__ void sharedAdd( float* o, float v ) { atomicAdd( o, v ); }
+__device__ __forceinline__ void sharedAdd( __half* o, float v )
+{
+   // packed FP16 atomic add on the aligned pair holding the value, with +0 for
+   // the other half (function buffers are padded, so the pair never reaches past
+   // the allocation)
+   typedef _Float16 h2 __attribute__( ( ext_vector_type( 2 ) ) );
+   const uintptr_t a   = reinterpret_cast< uintptr_t >( o );
+   h2*             w   = reinterpret_cast< h2* >( a & ~uintptr_t( 3 ) );
+   const _Float16  hv  = _Float16( v );
+   const h2        inc = ( a & 2 ) ? h2{ _Float16( 0.0f ), hv } : h2{ hv, _Float16( 0.0f ) };
+   __builtin_amdgcn_global_atomic_fadd_v2f16( (__attribute__( ( address_space( 1 ) ) ) h2*) w, inc );
 }
 
 // y = A x on the cell lattices of level L+1 (halo positions included): the
@@ -448,17 +499,20 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileCoefficients( const P2C
 //   y_i += sum_q w_q k_q |det DF J| dphi_i/dlam_k grad_c lam_k . (DF^-1 DF^-T g)
 // (grad_c: computational space), equal to the reference's element matrix times
 // the element vector.
-template < bool STORED >
-__global__ __launch_bounds__( kTileThreads ) void kP2TileApply( const P2CellGeom* __restrict__ geom,
-                                                               const KParams* __restrict__ kp,
-                                                               const double* __restrict__ table, int64_t mm,
-                                                               int64_t ntiles, int64_t sc1, int64_t w1,
-                                                               const double* __restrict__ src, double* __restrict__ dst )
+// T: value type of src / dst; K: value type of the coefficient table; the
+// arithmetic runs in Acc<T> (FP64 for double, FP32 for float and half).
+template < class T, class K, bool STORED >
+__global__ __launch_bounds__( kTileThreads ) void kP2TileApply(
+    const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
+    const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
+    const T* __restrict__ src, T* __restrict__ dst )
 {
+   using R  = typename Acc< T >::type;
+   using QC = QuadConst< R >;
    // one LDS box: first the tile's source values, then (after every thread has
    // read its 10 values) the accumulator of the element results
-   __shared__ double sb[kBox3];
-   __shared__ double sk[STORED ? 11 * kTileThreads : 1];
+   __shared__ R sb[kBox3];
+   __shared__ R sk[STORED ? 11 * kTileThreads : 1];
    const int64_t lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
       return;
@@ -470,76 +524,83 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApply( const P2CellGeom
    // quadrature loop
    if constexpr ( STORED )
    {
-      double kv[11];
+      K kv[11];
 #pragma unroll
       for ( int q = 0; q < 11; ++q )
          kv[q] = table[( lb * 11 + q ) * kTileThreads + threadIdx.x];
 #pragma unroll
       for ( int q = 0; q < 11; ++q )
-         sk[q * kTileThreads + threadIdx.x] = kv[q];
+         sk[q * kTileThreads + threadIdx.x] = R( toAcc( kv[q] ) );
    }
-   const double* srcC = src + base;
+   const T* srcC = src + base;
    for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
    {
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      sb[p]           = ( X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : 0.0;
+      sb[p]           = ( X + Y + Z <= top ) ? R( toAcc( srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] ) ) : R( 0 );
    }
    __syncthreads();
-   const P2CellGeom& g  = geom[tc.c];
-   const TileThread  th = tileThread( g, tc, mm );
-   double            u[10];
+   const P2CellGeomT< R >& g  = geom[tc.c];
+   const TileThread< R >   th = tileThread( g, tc, mm );
+   R                       u[10];
    {
       int pos[10];
       elementBoxPositions( th, pos );
 #pragma unroll
       for ( int i = 0; i < 10; ++i )
-         u[i] = th.valid ? sb[pos[i]] : 0.0;
+         u[i] = th.valid ? sb[pos[i]] : R( 0 );
    }
    __syncthreads();
    for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
-      sb[p] = 0.0;
+      sb[p] = R( 0 );
    __syncthreads();
    if ( th.valid )
    {
-      double y[10];
+      R y[10];
 #pragma unroll
       for ( int i = 0; i < 10; ++i )
-         y[i] = 0.0;
-      const double* Dc   = g.Dc[th.t];
-      const double  adet = g.adet[th.t];
+         y[i] = R( 0 );
+      const R* Dc   = g.Dc[th.t];
+      const R  adet = g.adet[th.t];
 #pragma unroll 1
       for ( int q = 0; q < 11; ++q )
       {
-         const double* a4 = cQ4[q];
-         const double* b4 = cQ4m[q];
-         const double  g0 = b4[0] * u[0] + a4[1] * u[4] + a4[2] * u[5] + a4[3] * u[6];
-         const double  g1 = b4[1] * u[1] + a4[0] * u[4] + a4[2] * u[7] + a4[3] * u[8];
-         const double  g2 = b4[2] * u[2] + a4[0] * u[5] + a4[1] * u[7] + a4[3] * u[9];
-         const double  g3 = b4[3] * u[3] + a4[0] * u[6] + a4[1] * u[8] + a4[2] * u[9];
-         const double  d1 = g1 - g0, d2 = g2 - g0, d3 = g3 - g0;
-         double        v[3];
+         const R* a4 = QC::a4( q );
+         const R* b4 = QC::b4( q );
+         const R  g0 = b4[0] * u[0] + a4[1] * u[4] + a4[2] * u[5] + a4[3] * u[6];
+         const R  g1 = b4[1] * u[1] + a4[0] * u[4] + a4[2] * u[7] + a4[3] * u[8];
+         const R  g2 = b4[2] * u[2] + a4[0] * u[5] + a4[1] * u[7] + a4[3] * u[9];
+         const R  g3 = b4[3] * u[3] + a4[0] * u[6] + a4[1] * u[8] + a4[2] * u[9];
+         const R  d1 = g1 - g0, d2 = g2 - g0, d3 = g3 - g0;
+         R        v[3];
 #pragma unroll
          for ( int d = 0; d < 3; ++d )
             v[d] = d1 * Dc[d] + d2 * Dc[3 + d] + d3 * Dc[6 + d];
-         const double* xq   = g.XQ[th.t][q];
-         const double  x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
-         double        kqv;
+         const R* xq   = g.XQ[th.t][q];
+         const R  x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
+         R        kqv;
          if constexpr ( STORED )
             kqv = sk[q * kTileThreads + threadIdx.x];
          else
-            kqv = kAt( g, *kp, x, th.ls0 + xq[3] );
-         double cc = cQW[q] * adet * kqv;
+         {
+            // k from the FP64 geometry of the same point
+            const P2CellGeom&          g64  = geom64[tc.c];
+            const TileThread< double > t64  = tileThread( g64, tc, mm );
+            const double*              xq64 = g64.XQ[th.t][q];
+            const double x64[3] = { t64.p0[0] + xq64[0], t64.p0[1] + xq64[1], t64.p0[2] + xq64[2] };
+            kqv                 = R( kAt( g64, *kp, x64, t64.ls0 + xq64[3] ) );
+         }
+         R cc = QC::w()[q] * adet * kqv;
          if ( g.kind )
          {
-            double qq, w[3], beta;
+            R qq, w[3], beta;
             shellScalars( g, x, th.ls0 + xq[3], qq, w, beta );
-            const double ib = rcpNewton( beta );
-            const double t1 = ( x[0] * v[0] + x[1] * v[1] + x[2] * v[2] ) * ib;
+            const R ib = fastRcp( beta );
+            const R t1 = ( x[0] * v[0] + x[1] * v[1] + x[2] * v[2] ) * ib;
 #pragma unroll
             for ( int d = 0; d < 3; ++d )
                v[d] -= w[d] * t1;
-            const double t2 = ( w[0] * v[0] + w[1] * v[1] + w[2] * v[2] ) * ib;
+            const R t2 = ( w[0] * v[0] + w[1] * v[1] + w[2] * v[2] ) * ib;
 #pragma unroll
             for ( int d = 0; d < 3; ++d )
                v[d] -= x[d] * t2;
@@ -548,10 +609,10 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApply( const P2CellGeom
 #pragma unroll
          for ( int d = 0; d < 3; ++d )
             v[d] *= cc;
-         const double e1 = Dc[0] * v[0] + Dc[1] * v[1] + Dc[2] * v[2];
-         const double e2 = Dc[3] * v[0] + Dc[4] * v[1] + Dc[5] * v[2];
-         const double e3 = Dc[6] * v[0] + Dc[7] * v[1] + Dc[8] * v[2];
-         const double e0 = -( e1 + e2 + e3 );
+         const R e1 = Dc[0] * v[0] + Dc[1] * v[1] + Dc[2] * v[2];
+         const R e2 = Dc[3] * v[0] + Dc[4] * v[1] + Dc[5] * v[2];
+         const R e3 = Dc[6] * v[0] + Dc[7] * v[1] + Dc[8] * v[2];
+         const R e0 = -( e1 + e2 + e3 );
          y[0] += b4[0] * e0;
          y[1] += b4[1] * e1;
          y[2] += b4[2] * e2;
@@ -570,19 +631,19 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApply( const P2CellGeom
          atomicAdd( &sb[pos[i]], y[i] );
    }
    __syncthreads();
-   double* dstC = dst + base;
+   T* dstC = dst + base;
    for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
    {
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
       if ( X + Y + Z > top )
          continue;
-      double*    o      = dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
+      T*         o      = dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
       const bool shared = lx == 0 || ly == 0 || lz == 0 || lx == kBox - 1 || ly == kBox - 1 || lz == kBox - 1;
       if ( shared )
-         atomicAdd( o, sb[p] );
+         sharedAdd( o, sb[p] );
       else
-         *o += sb[p];
+         *o = fromAcc< T >( toAcc( *o ) + sb[p] );
    }
 }
 
@@ -676,58 +737,77 @@ static void p2CheckFunction( const P2Operator& op, const Function& f, int L )
 
 // k at the quadrature points of level L (built on first use when the operator
 // stores coefficients; nullptr when it evaluates k in the apply kernel)
-static const double* p2Coefficients( Context* ctx, P2Operator& op, int L )
+// k at the quadrature points of level L in value type K (built on first use
+// when the operator stores coefficients; nullptr when it evaluates k in the
+// apply kernel)
+template < class K >
+static const K* p2Coefficients( Context* ctx, P2Operator& op, int L, int dtype )
 {
    if ( !op.storeCoefficients || op.nm == 0 )
       return nullptr;
-   auto it = op.kq.find( L );
+   auto it = op.kq.find( { L, dtype } );
    if ( it != op.kq.end() )
-      return it->second.p;
+      return (const K*) it->second.p;
    const int64_t mm     = int64_t( 1 ) << L;
    const int64_t ntiles = tilesPerCell( mm ) * int64_t( op.st->topo.nc );
-   auto&         buf    = op.kq[L];
-   buf.alloc( ctx, size_t( ntiles ) * 11 * kTileThreads );
-   kP2TileCoefficients<<< gridFor( ntiles, 1 ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, mm, ntiles,
-                                                                                buf.p );
+   auto&         buf    = op.kq[{ L, dtype }];
+   buf.alloc( ctx, size_t( ntiles ) * 11 * kTileThreads * sizeof( K ) );
+   kP2TileCoefficients< K ><<< gridFor( ntiles, 1 ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, mm,
+                                                                                     ntiles, (K*) buf.p );
    HG_CHECK_HIP( hipGetLastError() );
-   return buf.p;
+   return (const K*) buf.p;
+}
+
+template < class T >
+static const P2CellGeomT< typename Acc< T >::type >* p2Geometry( P2Operator& op, int L )
+{
+   if constexpr ( std::is_same< typename Acc< T >::type, double >::value )
+      return op.tgeom.at( L ).p;
+   else
+      return op.tgeomF.at( L ).p;
 }
 
 void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L, int flag, int update )
 {
    p2CheckFunction( op, src, L );
    p2CheckFunction( op, dst, L );
-   if ( src.dtype != 0 || dst.dtype != 0 )
-      throw Error( HG_ERR_UNSUPPORT, "the blended P2 div(k grad) operator runs in FP64" );
+   if ( src.dtype != dst.dtype )
+      throw Error( HG_ERR_ARG, "apply needs src and dst of one value type" );
    if ( &src == &dst )
       throw Error( -1, "apply needs distinct src and dst" );
    Storage&      st  = *op.st;
    Lattice&      lat = st.lattice( ctx, L + 1 );
-   double*       s   = (double*) src.ptr( L );
-   double*       d   = (double*) dst.ptr( L );
+   void*         s   = src.ptr( L );
+   void*         d   = dst.ptr( L );
    const int64_t mm  = int64_t( 1 ) << L;
-   pullHalos( ctx, st, lat, s, 0 );
-   // zero the halos (all) and, for Replace with Inner in flag, the cell interiors
-   if ( update == HG_REPLACE && ( flag & kInner ) )
-      kFillRange< double ><<< gridFor( lat.nc * lat.sc ), kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, d, 0.0 );
-   else if ( lat.haloPerCell() > 0 )
-      kZeroHalos< double ><<< gridFor( lat.haloPerCell() * lat.nc ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lat ),
-                                                                                              lat.nc, d );
-   HG_CHECK_HIP( hipGetLastError() );
    if ( lat.w > 1024 )
       throw Error( HG_ERR_UNSUPPORT, "P2 apply supports levels up to 9 (32-bit lattice offsets)" );
-   const int64_t ntiles = tilesPerCell( mm ) * lat.nc;
-   const double* table  = p2Coefficients( ctx, op, L );
-   ctx->beginTiming();
-   if ( table )
-      kP2TileApply< true ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, table, mm,
-                                                                                  ntiles, lat.sc, lat.w, s, d );
-   else
-      kP2TileApply< false ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, nullptr,
-                                                                                   mm, ntiles, lat.sc, lat.w, s, d );
-   HG_CHECK_HIP( hipGetLastError() );
-   ctx->endTiming();
-   pushAdditive( ctx, st, lat, d, 0, kAll ^ flag, update == HG_REPLACE );
+   pullHalos( ctx, st, lat, s, src.dtype );
+   dispatchType( src.dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      // FP64 functions use an FP64 table, FP32 an FP32 one, FP16 an FP16 one
+      using K = T;
+      // zero the halos (all) and, for Replace with Inner in flag, the cell interiors
+      if ( update == HG_REPLACE && ( flag & kInner ) )
+         kFillRange< T ><<< gridFor( lat.nc * lat.sc ), kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, (T*) d, 0.0 );
+      else if ( lat.haloPerCell() > 0 )
+         kZeroHalos< T ><<< gridFor( lat.haloPerCell() * lat.nc ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lat ),
+                                                                                            lat.nc, (T*) d );
+      HG_CHECK_HIP( hipGetLastError() );
+      const int64_t ntiles = tilesPerCell( mm ) * lat.nc;
+      const K*      table  = p2Coefficients< K >( ctx, op, L, src.dtype );
+      const auto*   geom   = p2Geometry< T >( op, L );
+      ctx->beginTiming();
+      if ( table )
+         kP2TileApply< T, K, true ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+             geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d );
+      else
+         kP2TileApply< T, K, false ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+             geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d );
+      HG_CHECK_HIP( hipGetLastError() );
+      ctx->endTiming();
+   } );
+   pushAdditive( ctx, st, lat, d, dst.dtype, kAll ^ flag, update == HG_REPLACE );
 }
 
 double* p2InverseDiagonal( Context* ctx, P2Operator& op, int L )
@@ -859,6 +939,32 @@ static void tileGeometryLevel( std::vector< P2CellGeom >& gs, int64_t mm )
    }
 }
 
+static std::vector< P2CellGeomT< float > > toFloat( const std::vector< P2CellGeom >& gs )
+{
+   std::vector< P2CellGeomT< float > > out( gs.size() );
+   for ( size_t c = 0; c < gs.size(); ++c )
+   {
+      const P2CellGeom&      g = gs[c];
+      P2CellGeomT< float >& f = out[c];
+      auto cp = []( float* dst, const double* src, int n ) {
+         for ( int i = 0; i < n; ++i )
+            dst[i] = float( src[i] );
+      };
+      cp( f.v0, g.v0, 3 );
+      cp( f.hE, g.hE, 9 );
+      cp( &f.J[0][0], &g.J[0][0], 54 );
+      cp( &f.Dc[0][0], &g.Dc[0][0], 54 );
+      cp( f.adet, g.adet, 6 );
+      cp( &f.XQ[0][0][0], &g.XQ[0][0][0], 6 * 11 * 4 );
+      cp( f.nl, g.nl, 3 );
+      f.lam0 = float( g.lam0 );
+      f.r0   = float( g.r0 );
+      f.dr   = float( g.dr );
+      f.kind = g.kind;
+   }
+   return out;
+}
+
 } // namespace hg
 
 using namespace hg;
@@ -961,6 +1067,19 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
             }
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4 ), q4, sizeof( q4 ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4m ), q4m, sizeof( q4m ) ) );
+         float wf[11], q4f[11][4], q4mf[11][4];
+         for ( int q = 0; q < 11; ++q )
+         {
+            wf[q] = float( w[q] );
+            for ( int k = 0; k < 4; ++k )
+            {
+               q4f[q][k]  = float( q4[q][k] );
+               q4mf[q][k] = float( q4m[q][k] );
+            }
+         }
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQWf ), wf, sizeof( wf ) ) );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4f ), q4f, sizeof( q4f ) ) );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4mf ), q4mf, sizeof( q4mf ) ) );
          quadReady = true;
       }
       Storage&        st = ctx->storage( s );
@@ -1004,6 +1123,7 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
          std::vector< P2CellGeom > gl = tg;
          tileGeometryLevel( gl, int64_t( 1 ) << L );
          op->tgeom[L].upload( ctx, gl );
+         op->tgeomF[L].upload( ctx, toFloat( gl ) );
       }
       std::vector< KParams > kp( 1 );
       kp[0]    = KParams{};
@@ -1049,11 +1169,13 @@ int hg_p2_inverse_diagonal( hg_ctx* h, hg_handle op, int level, hg_handle dst )
       P2Operator& o = ctx->op< P2Operator >( op );
       Function&   f = ctx->function( dst );
       p2CheckFunction( o, f, level );
-      if ( f.dtype != 0 )
-         throw Error( HG_ERR_UNSUPPORT, "inverse diagonal is FP64" );
       const double* inv = p2InverseDiagonal( ctx, o, level );
       Lattice&      lat = o.st->lattice( ctx, level + 1 );
-      HG_CHECK_HIP( hipMemcpyAsync( f.ptr( level ), inv, lat.nstore * sizeof( double ), hipMemcpyDeviceToDevice, ctx->stream ) );
+      if ( f.dtype == 0 )
+         HG_CHECK_HIP( hipMemcpyAsync( f.ptr( level ), inv, lat.nstore * sizeof( double ), hipMemcpyDeviceToDevice,
+                                       ctx->stream ) );
+      else
+         castCopy( ctx, inv, 0, f.ptr( level ), f.dtype, lat.nstore );
    } );
 }
 

@@ -320,6 +320,18 @@ __global__ void kCast( int64_t n, const TS* __restrict__ src, TD* __restrict__ d
    }
 }
 
+void castCopy( Context* ctx, const void* src, int sdtype, void* dst, int ddtype, int64_t n )
+{
+   dispatchType( ddtype, [&]( auto td ) {
+      dispatchType( sdtype, [&]( auto ts ) {
+         using TD = decltype( td );
+         using TS = decltype( ts );
+         kCast< TD, TS ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( n, (const TS*) src, (TD*) dst );
+      } );
+   } );
+   HG_CHECK_HIP( hipGetLastError() );
+}
+
 template < class T >
 __global__ void kDotPartial( int64_t ncell, int64_t nb, const uint32_t* __restrict__ cellMask,
                              const uint8_t* __restrict__ bndFlag, int flag, const T* __restrict__ a,
@@ -662,7 +674,9 @@ static int functionCreate( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
       for ( int L = minLevel; L <= maxLevel; ++L )
       {
          Lattice& l = st.lattice( ctx, L + off );
-         f->data[L + off].alloc( ctx, size_t( l.nstore ) * f->elemSize() );
+         // two spare values: a 16-bit value is updated atomically through its
+         // aligned 32-bit word, which for the last value may extend past nstore
+         f->data[L + off].alloc( ctx, size_t( l.nstore + 2 ) * f->elemSize() );
       }
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
       *out                  = ctx->nextHandle++;
@@ -852,17 +866,8 @@ int hg_function_copy_from( hg_ctx* h, hg_handle dst, hg_handle src, int level )
       checkLevel( s, level );
       if ( d.st != s.st || d.off != s.off )
          throw Error( HG_ERR_ARG, "copyFrom needs functions of one storage and one space" );
-      Lattice&      l = d.lat( ctx, level );
-      const int64_t n = l.nstore;
-      dispatchType( d.dtype, [&]( auto td ) {
-         dispatchType( s.dtype, [&]( auto ts ) {
-            using TD = decltype( td );
-            using TS = decltype( ts );
-            kCast< TD, TS ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( n, (const TS*) s.ptr( level ),
-                                                                        (TD*) d.ptr( level ) );
-         } );
-      } );
-      HG_CHECK_HIP( hipGetLastError() );
+      Lattice& l = d.lat( ctx, level );
+      castCopy( ctx, s.ptr( level ), s.dtype, d.ptr( level ), d.dtype, l.nstore );
    } );
 }
 

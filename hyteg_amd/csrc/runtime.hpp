@@ -259,6 +259,8 @@ inline unsigned gridFor( int64_t n, int block = kBlock )
 // Refresh every halo copy in the cell lattices from its owner (the
 // Vertex->Edge->Face->Cell communicate chain of VertexDoFFunction).
 void pullHalos( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype );
+// dst[i] = src[i] converted between value types (HG_F64 / HG_F32 / HG_F16)
+void castCopy( Context* ctx, const void* src, int sdtype, void* dst, int ddtype, int64_t n );
 // Sum the halo copies of every owned vertex/edge/face DoF into the owner
 // (communicateAdditively<Cell, Face/Edge/Vertex>).  Owners whose type matches
 // excludeFlag are left untouched; zeroOut replaces instead of adding.

@@ -67,6 +67,71 @@ int cycle( const Op& A, G& gmg, const F& x, const F& b, const F& r, uint_t level
    return k;
 }
 
+// Mixed-precision iterative refinement (mixedPrecisionIterativeRefinement.cpp:163-172):
+// FP64 residual r = b - A x, scaled by the RMS of D^-1 r and cast to the low
+// precision; `innerCycles` V-cycles of the low-precision hierarchy (FP32
+// arithmetic, storage type L, FP64 PCG on the coarsest level) solve A e = r;
+// x += scale * e in FP64.  Stops when the FP64 Inner residual drops by `tol`.
+template < typename L >
+int iterativeRefinement( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre,
+                         int post, int order, double rho, int innerCycles, int maxOuter, double tol, int coarseMaxIter,
+                         double coarseTol, double* hist )
+{
+   using Op64   = P2ElementwiseBlendingDivKGradOperator;
+   using OpLo   = P2ElementwiseBlendingDivKGradOperatorT< L >;
+   auto storage = storageOf( ctx, x );
+   auto A       = Op64::borrow( storage, op, uint_t( minLevel ), uint_t( maxLevel ) );
+   auto Alo     = OpLo::borrow( storage, op, uint_t( minLevel ), uint_t( maxLevel ) );
+   auto X       = P2Function< double >::borrow( ctx, x );
+   auto B       = P2Function< double >::borrow( ctx, b );
+   const uint_t top = uint_t( maxLevel ), bottom = uint_t( minLevel );
+
+   auto smoother = std::make_shared< ChebyshevSmoother< OpLo > >( storage, bottom, top );
+   smoother->setupCoefficients( uint_t( order ), rho );
+   auto pcg    = std::make_shared< CGSolver< Op64 > >( storage, bottom, bottom, uint_t( coarseMaxIter ), coarseTol, 1e-300,
+                                                    std::make_shared< InverseDiagonalPreconditioner< Op64 > >() );
+   auto coarse = std::make_shared< FP64CoarseSolver< OpLo, Op64 > >( storage, bottom, *A, pcg );
+   GeometricMultigridSolver< OpLo > gmg( storage, smoother, coarse, std::make_shared< P2toP2QuadraticRestriction< L > >(),
+                                         std::make_shared< P2toP2QuadraticProlongation< L > >(), bottom, top,
+                                         uint_t( pre ), uint_t( post ) );
+   P2Function< L >      xlo( "ir_x", storage, bottom, top ), blo( "ir_b", storage, bottom, top );
+   P2Function< double > r( "ir_r", storage, top, top ), e( "ir_e", storage, top, top );
+   int64_t              n = 0;
+   check( ctx, hg_p2_ndofs( ctx, storage->handle(), maxLevel, &n ), "ndofs" );
+   const DoFType flag = Inner | NeumannBoundary | FreeslipBoundary;
+
+   const auto invDiag = A->getInverseDiagonalValues();
+   double     r0      = 0.0;
+   int        k       = 0;
+   for ( ;; ++k )
+   {
+      // r = b - A x on the solver's DoFs; its Inner norm is the convergence measure
+      A->apply( *X, r, top, flag, Replace );
+      r.assign( { 1.0, -1.0 }, { *B, r }, top, flag );
+      const double rk = std::sqrt( r.dotGlobal( r, top, Inner ) );
+      if ( k == 0 )
+         r0 = rk;
+      if ( hist )
+         hist[k] = rk;
+      if ( k == maxOuter || ( k > 0 && tol > 0 && rk <= tol * r0 ) )
+         return k;
+      // scaled so that its Jacobi correction D^-1 r (the size of the unknown
+      // correction) has unit RMS: keeps the low-precision vectors of every level
+      // inside the FP16 range
+      e.multElementwise( { *invDiag, r }, top, flag );
+      const double scale = std::sqrt( e.dotGlobal( e, top, flag ) / double( n ) );
+      if ( !( scale > 0 ) )
+         return k;
+      r.assign( { 1.0 / scale }, { r }, top, flag );
+      blo.copyFrom( r, top );
+      xlo.setToZero( top );
+      for ( int j = 0; j < innerCycles; ++j )
+         gmg.solve( *Alo, xlo, blo, top );
+      e.copyFrom( xlo, top );
+      X->add( { scale }, { e }, top, flag );
+   }
+}
+
 } // namespace
 
 extern "C" {
@@ -193,6 +258,26 @@ int hg_p2_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int mi
       const int            k = cycle( *A, gmg, *X, *B, r, uint_t( maxLevel ), cycles, tol, res_hist );
       if ( cycles_done )
          *cycles_done = k;
+   } );
+}
+
+int hg_p2_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int lowDtype,
+                    int pre, int post, int chebyshevOrder, double spectralRadius, int innerCycles, int maxOuter,
+                    double tol, int coarseMaxIter, double coarseTol, double* res_hist, int* outer_done )
+{
+   if ( !ctx || minLevel < 0 || maxLevel < minLevel || pre < 0 || post < 0 || innerCycles < 1 || maxOuter < 0 ||
+        coarseMaxIter < 0 || chebyshevOrder < 1 || chebyshevOrder > 16 || !( spectralRadius > 0 ) ||
+        ( lowDtype != HG_F32 && lowDtype != HG_F16 ) )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      if ( lowDtype == HG_F32 )
+         *outer_done = iterativeRefinement< float >( ctx, op, x, b, minLevel, maxLevel, pre, post, chebyshevOrder,
+                                                     spectralRadius, innerCycles, maxOuter, tol, coarseMaxIter,
+                                                     coarseTol, res_hist );
+      else
+         *outer_done = iterativeRefinement< f16 >( ctx, op, x, b, minLevel, maxLevel, pre, post, chebyshevOrder,
+                                                   spectralRadius, innerCycles, maxOuter, tol, coarseMaxIter, coarseTol,
+                                                   res_hist );
    } );
 }
 

@@ -234,6 +234,19 @@ int hg_p2_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int mi
                      int chebyshevOrder, double spectralRadius, int cycles, double tol, int coarseMaxIter,
                      double coarseTol, int coarseJacobi, double* res_hist, int* cycles_done );
 
+/* Mixed-precision iterative refinement around the P2 V-cycle
+ * (tests/hyteg/mixedPrecision/mixedPrecisionIterativeRefinement.cpp:163-172):
+ * each outer step forms r = b - A x in FP64, casts it (scaled so that D^-1 r has
+ * unit RMS) to
+ * lowDtype (HG_F32 or HG_F16 storage, FP32 arithmetic), runs innerCycles
+ * V(pre,post) cycles of the low-precision hierarchy (Chebyshev(order) with the
+ * given radius, quadratic transfer, FP64 Jacobi-PCG on minLevel to coarseTol) and
+ * adds the correction in FP64.  Stops when the FP64 Inner residual norm drops by
+ * `tol` or after maxOuter steps; res_hist[0..done] gets the residual norms. */
+int hg_p2_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int lowDtype,
+                    int pre, int post, int chebyshevOrder, double spectralRadius, int innerCycles, int maxOuter,
+                    double tol, int coarseMaxIter, double coarseTol, double* res_hist, int* outer_done );
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,6 +128,13 @@ class PrimitiveStorage
    hg_handle                  h_   = 0;
 };
 
+// FP16 storage tag: functions of this value type hold IEEE half values on the
+// device; their scalars (assign coefficients, dot products) are float.
+struct f16
+{
+   unsigned short bits;
+};
+
 template < typename T >
 struct DTypeOf;
 template <>
@@ -140,16 +147,33 @@ struct DTypeOf< float >
 {
    static constexpr int value = HG_F32;
 };
+template <>
+struct DTypeOf< f16 >
+{
+   static constexpr int value = HG_F16;
+};
+template < typename T >
+struct ScalarOf
+{
+   using type = T;
+};
+template <>
+struct ScalarOf< f16 >
+{
+   using type = float;
+};
 
 // P1Function<ValueType> (Space 1) and P2Function<ValueType> (Space 2): one
 // device array per level; a P2 function of level L holds the values of the
 // level-(L+1) vertex lattice (vertex DoFs at even, edge DoFs at odd points).
 // Wraps an existing handle without owning it when constructed through `borrow`.
-template < typename ValueType, int Space >
+template < typename StorageType, int Space >
 class LatticeFunction
 {
  public:
-   using valueType = ValueType;
+   using valueType   = typename ScalarOf< StorageType >::type;
+   using storageType = StorageType;
+   using ValueType   = valueType;
 
    LatticeFunction( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
                     uint_t maxLevel )
@@ -159,9 +183,9 @@ class LatticeFunction
    , owned_( true )
    {
       const int st = Space == 1 ? hg_p1_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ),
-                                                         DTypeOf< ValueType >::value, &h_ ) :
+                                                         DTypeOf< StorageType >::value, &h_ ) :
                                   hg_p2_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ),
-                                                         DTypeOf< ValueType >::value, &h_ );
+                                                         DTypeOf< StorageType >::value, &h_ );
       check( ctx_, st, Space == 1 ? "P1Function" : "P2Function" );
    }
    ~LatticeFunction()
@@ -391,13 +415,19 @@ class OperatorWithInverseDiagonal
 // P2ElementwiseOperator<p2_div_k_grad_blending_q4> with
 // k(x) = exp(sum_m a_m sin(omega_m . x + phi_m))
 // (elementwiseoperators/P2ElementwiseOperator.hpp:61-475)
-class P2ElementwiseBlendingDivKGradOperator : public Operator< P2Function< double >, P2Function< double > >,
-                                              public OperatorWithInverseDiagonal< P2Function< double > >
+//
+// VT is the value type of the functions the operator acts on: double (FP64
+// arithmetic), float or f16 (FP32 arithmetic, FP32 / FP16 storage of vectors
+// and of the quadrature-point coefficients).  Operators of several value types
+// may share one device operator (`borrow`).
+template < typename VT >
+class P2ElementwiseBlendingDivKGradOperatorT : public Operator< P2Function< VT >, P2Function< VT > >,
+                                               public OperatorWithInverseDiagonal< P2Function< VT > >
 {
  public:
-   P2ElementwiseBlendingDivKGradOperator( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
-                                          uint_t maxLevel, const std::vector< double >& a = {},
-                                          const std::vector< double >& omega = {}, const std::vector< double >& phi = {} )
+   P2ElementwiseBlendingDivKGradOperatorT( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
+                                           uint_t maxLevel, const std::vector< double >& a = {},
+                                           const std::vector< double >& omega = {}, const std::vector< double >& phi = {} )
    : storage_( storage )
    , ctx_( storage->ctx() )
    , minLevel_( minLevel )
@@ -411,31 +441,31 @@ class P2ElementwiseBlendingDivKGradOperator : public Operator< P2Function< doubl
                                     omega.data(), phi.data(), &h_ ),
              "P2ElementwiseBlendingDivKGradOperator" );
    }
-   static std::unique_ptr< P2ElementwiseBlendingDivKGradOperator >
+   static std::unique_ptr< P2ElementwiseBlendingDivKGradOperatorT >
        borrow( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel, uint_t maxLevel )
    {
-      return std::unique_ptr< P2ElementwiseBlendingDivKGradOperator >(
-          new P2ElementwiseBlendingDivKGradOperator( storage, h, minLevel, maxLevel ) );
+      return std::unique_ptr< P2ElementwiseBlendingDivKGradOperatorT >(
+          new P2ElementwiseBlendingDivKGradOperatorT( storage, h, minLevel, maxLevel ) );
    }
-   ~P2ElementwiseBlendingDivKGradOperator() override
+   ~P2ElementwiseBlendingDivKGradOperatorT() override
    {
       if ( owned_ && h_ )
          hg_operator_destroy( ctx_, h_ );
    }
-   void apply( const P2Function< double >& src, const P2Function< double >& dst, uint_t level, DoFType flag,
+   void apply( const P2Function< VT >& src, const P2Function< VT >& dst, uint_t level, DoFType flag,
                UpdateType updateType = Replace ) const override
    {
       check( ctx_, hg_p2_apply( ctx_, h_, src.handle(), dst.handle(), int( level ), flag, updateType ), "apply" );
    }
-   std::shared_ptr< P2Function< double > > getInverseDiagonalValues() const override
+   std::shared_ptr< P2Function< VT > > getInverseDiagonalValues() const override
    {
       if ( !invDiag_ )
-         const_cast< P2ElementwiseBlendingDivKGradOperator* >( this )->computeInverseDiagonalOperatorValues();
+         const_cast< P2ElementwiseBlendingDivKGradOperatorT* >( this )->computeInverseDiagonalOperatorValues();
       return invDiag_;
    }
    void computeInverseDiagonalOperatorValues() override
    {
-      invDiag_ = std::make_shared< P2Function< double > >( "inverseDiagonal", storage_, minLevel_, maxLevel_ );
+      invDiag_ = std::make_shared< P2Function< VT > >( "inverseDiagonal", storage_, minLevel_, maxLevel_ );
       for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
          check( ctx_, hg_p2_inverse_diagonal( ctx_, h_, int( L ), invDiag_->handle() ), "inverse diagonal" );
    }
@@ -445,8 +475,8 @@ class P2ElementwiseBlendingDivKGradOperator : public Operator< P2Function< doubl
    const std::shared_ptr< PrimitiveStorage >& getStorage() const { return storage_; }
 
  private:
-   P2ElementwiseBlendingDivKGradOperator( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel,
-                                          uint_t maxLevel )
+   P2ElementwiseBlendingDivKGradOperatorT( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h,
+                                           uint_t minLevel, uint_t maxLevel )
    : storage_( storage )
    , ctx_( storage->ctx() )
    , h_( h )
@@ -459,8 +489,9 @@ class P2ElementwiseBlendingDivKGradOperator : public Operator< P2Function< doubl
    hg_handle                                     h_   = 0;
    uint_t                                        minLevel_, maxLevel_;
    bool                                          owned_;
-   std::shared_ptr< P2Function< double > >       invDiag_;
+   std::shared_ptr< P2Function< VT > >           invDiag_;
 };
+using P2ElementwiseBlendingDivKGradOperator = P2ElementwiseBlendingDivKGradOperatorT< double >;
 
 // P2toP2QuadraticRestriction / P2toP2QuadraticProlongation
 // (gridtransferoperators/P2toP2QuadraticRestriction.cpp, P2toP2QuadraticProlongation.cpp)
@@ -752,6 +783,36 @@ double estimateRadius( const OperatorType& A, uint_t level, uint_t maxIter, cons
    return radius;
 }
 } // namespace chebyshev
+
+// Coarse solver of a reduced-precision multigrid hierarchy that solves in FP64:
+// b is cast up, `solver` (e.g. PCG to 1e-14) runs on the FP64 operator, and the
+// result is cast back.
+template < class OperatorType, class OperatorType64 >
+class FP64CoarseSolver : public Solver< OperatorType >
+{
+ public:
+   using FunctionType   = typename OperatorType::srcType;
+   using FunctionType64 = typename OperatorType64::srcType;
+   FP64CoarseSolver( const std::shared_ptr< PrimitiveStorage >& storage, uint_t level, const OperatorType64& A64,
+                     std::shared_ptr< Solver< OperatorType64 > > solver )
+   : A64_( A64 )
+   , solver_( std::move( solver ) )
+   , x64_( "coarse_x64", storage, level, level )
+   , b64_( "coarse_b64", storage, level, level )
+   {}
+   void solve( const OperatorType&, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      b64_.copyFrom( b, level );
+      x64_.setToZero( level );
+      solver_->solve( A64_, x64_, b64_, level );
+      x.copyFrom( x64_, level );
+   }
+
+ private:
+   const OperatorType64&                       A64_;
+   std::shared_ptr< Solver< OperatorType64 > > solver_;
+   FunctionType64                              x64_, b64_;
+};
 
 // GeometricMultigridSolver (GeometricMultigridSolver.hpp:126-308)
 template < class OperatorType >
