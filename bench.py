@@ -200,12 +200,77 @@ def gmg_p2_shell(hg, ctx, L):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     log(f"P2 GMG level {L}: {k} cycles, {dt:.2f} s")
-    for f in (x, b, t):
+    out = {"workload": f"P2 div(k grad) shell level {L}, FP64 GMG V(2,2) Chebyshev(3), Jacobi-PCG to 1e-14 on "
+                       f"level 0, to 1e-8 relative residual",
+           "cycles": k, "time_s": dt, "setup_s": setup, "spectral_radius": rho,
+           "factors": [float(v) for v in hist[1:] / hist[:-1]],
+           "relative_residual": float(hist[-1] / hist[0])}
+    t.destroy()
+    # mixed precision: FP32 arithmetic, FP16 storage inside FP64 iterative refinement
+    xm = hg.P2Function("xm", st, 0, L)
+    hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=hg.F16, max_outer=1)  # builds the FP16 tables once
+    xm.interpolate(0.0, L)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    km, hm = hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=hg.F16, inner_cycles=1, max_outer=60, tol=1e-8)
+    ctx.synchronize()
+    dtm = time.perf_counter() - t0
+    # || x_mixed - x_fp64 ||_2 / || x_fp64 ||_2 on the device
+    xm.assign([1.0, -1.0], [xm, x], L, hg.ALL)
+    err = (xm.dot(xm, L, hg.ALL) / x.dot(x, L, hg.ALL)) ** 0.5
+    log(f"P2 mixed IR level {L}: {km} outer steps, {dtm:.2f} s, speed-up {dt / dtm:.2f}, error vs FP64 {err:.2e}")
+    out["mixed_fp32_compute_fp16_storage"] = {
+        "workload": "FP64 iterative refinement, one V(2,2) Chebyshev(3) cycle per step in FP32 arithmetic with "
+                    "FP16 vectors and FP16 quadrature-point coefficients, FP64 PCG on level 0, to 1e-8",
+        "outer_steps": km, "time_s": dtm, "speedup_vs_fp64": dt / dtm,
+        "relative_residual": float(hm[-1] / hm[0]),
+        "rel_l2_difference_to_fp64_solution": err}
+    for f in (x, b, xm):
         f.destroy()
-    return {"workload": f"P2 div(k grad) shell level {L}, FP64 GMG V(2,2) Chebyshev(3), CG level 0, to 1e-8",
-            "cycles": k, "time_s": dt, "setup_s": setup, "spectral_radius": rho,
-            "factors": [float(v) for v in hist[1:] / hist[:-1]],
-            "relative_residual": float(hist[-1] / hist[0])}
+    A.destroy()
+    return out
+
+
+def batched_cfg4(hg, ctx, L=6, nvec=64):
+    """64 P1 Laplace systems on the cube at level 6, RHS uniform[-1,1] on inner DoFs from
+    seeds 0..63: one batched FP16-storage V(3,3) Gauss-Seidel hierarchy inside FP64
+    iterative refinement (interleaved vectors) versus 64 sequential FP64 GMG solves,
+    all to 1e-10 relative residual"""
+    st = hg.PrimitiveStorage.cuboid(ctx)
+    A = hg.P1ConstantLaplaceOperator(st, 0, L)
+    xb = hg.P1Function("xb", st, 0, L, nvec=nvec)
+    bb = hg.P1Function("bb", st, 0, L, nvec=nvec)
+    bb.interpolate_random(0, -1.0, 1.0, L, hg.INNER)
+    hg.batched_ir_solve(A, xb, bb, 0, L, low_dtype=hg.F16, max_outer=1)
+    xb.interpolate(0.0, L)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    kir, hir = hg.batched_ir_solve(A, xb, bb, 0, L, low_dtype=hg.F16, max_outer=60, tol=1e-10)
+    ctx.synchronize()
+    t_ir = time.perf_counter() - t0
+    log(f"batched FP16 refinement: {kir} steps, {t_ir:.3f} s for {nvec} systems")
+    # 64 sequential FP64 solves, each from its own seed
+    x = hg.P1Function("x", st, 0, L)
+    b = hg.P1Function("b", st, 0, L)
+    e = hg.P1Function("e", st, L, L)
+    errs, cycles, t_seq = [], [], 0.0
+    xs = xb.download(L)
+    for k in range(nvec):
+        b.interpolate_random(k, -1.0, 1.0, L, hg.INNER)
+        x.interpolate(0.0, L)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        kk, _ = hg.gmg_solve(A, x, b, 0, L, 3, 3, cycles=60, tol=1e-10)
+        ctx.synchronize()
+        t_seq += time.perf_counter() - t0
+        cycles.append(kk)
+        xk = x.download(L)
+        errs.append(float(np.abs(xs[:, k] - xk).max() / np.abs(xk).max()))
+    log(f"sequential FP64: {t_seq:.3f} s for {nvec} systems")
+    return {"workload": f"{nvec} x P1 Laplace, unit cube level {L} ({st.ndofs(L)} DoFs each), RHS seeds 0..{nvec - 1}, "
+                        "to 1e-10", "batched_fp16_ir": {"outer_steps": kir, "time_s": t_ir, "solves_per_s": nvec / t_ir},
+            "sequential_fp64_gmg": {"cycles_per_solve": cycles[0], "time_s": t_seq, "solves_per_s": nvec / t_seq},
+            "speedup": t_seq / t_ir, "max_rel_error_vs_fp64": max(errs)}
 
 
 def main():
@@ -218,6 +283,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gmg", action="store_true", help="skip the P2 GMG solve")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched 64-system P1 solve")
     ap.add_argument("--only-apply", action="store_true",
                     help="run the apply measurement only (for counter collection)")
     args = ap.parse_args()
@@ -243,6 +309,8 @@ def main():
             if not args.no_gmg:
                 extra["gmg_p2_shell_f64"] = gmg_p2_shell(hg, ctx, L)
         extra["gmg_p1_cube_L5_f64"] = gmg_cfg1(hg, ctx)
+        if args.workload == "p2_shell" and not args.no_batched:
+            extra["batched_p1_cube_L6_x64"] = batched_cfg4(hg, ctx)
     cpu = None if (args.no_cpu or args.only_apply) else cpu_baseline(args.workload, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": n / sec, "unit": "DoF/s", "n_gpus": 1, "steps": args.steps,

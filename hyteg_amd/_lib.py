@@ -61,6 +61,12 @@ SIGNATURES = {
     "hg_p1_smooth_sor": [_p, _i64, _i64, _i64, _d, _i, _i],
     "hg_p1_cell_stencil": [_p, _i64, _i, _i, _p],
     "hg_p1_restrict": [_p, _i64, _i, _i],
+    "hg_p1_function_create_batched": [_p, _i64, _i, _i, _i, _i, _p],
+    "hg_function_nvec": [_p, _i64, _p],
+    "hg_function_dot_batched": [_p, _i64, _i64, _i, _i, _p],
+    "hg_function_scale_batched": [_p, _i64, _p, _i, _i],
+    "hg_p1_batched_gmg_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _i, _d, _p, _p],
+    "hg_p1_batched_ir_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _i, _i, _i, _d, _p, _p],
     "hg_p2_restrict": [_p, _i64, _i, _i],
     "hg_p2_prolongate": [_p, _i64, _i, _i, _i],
     "hg_p2_transfer_stencil": [_i, _i, _i, _p, _p, _p],

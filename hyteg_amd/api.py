@@ -123,14 +123,22 @@ class PrimitiveStorage:
 class P1Function:
     OFFSET = 0
 
-    def __init__(self, name, storage, min_level, max_level, dtype=F64):
+    def __init__(self, name, storage, min_level, max_level, dtype=F64, nvec=1):
+        """nvec > 1 (P1 only): that many independent vectors interleaved per DoF"""
         self.name = name
         self.storage = storage
         self.ctx = storage.ctx
         self.min_level, self.max_level = min_level, max_level
+        self.nvec = nvec
         h = C.c_int64()
-        create = lib().hg_p2_function_create if self.OFFSET else lib().hg_p1_function_create
-        self.ctx.check(create(self.ctx.h, storage.h, min_level, max_level, dtype, C.byref(h)), f"Function {name}")
+        if nvec != 1:
+            if self.OFFSET:
+                raise ValueError("P2 functions hold one vector")
+            self.ctx.check(lib().hg_p1_function_create_batched(self.ctx.h, storage.h, min_level, max_level, dtype,
+                                                               nvec, C.byref(h)), f"Function {name}")
+        else:
+            create = lib().hg_p2_function_create if self.OFFSET else lib().hg_p1_function_create
+            self.ctx.check(create(self.ctx.h, storage.h, min_level, max_level, dtype, C.byref(h)), f"Function {name}")
         self.h = h.value
 
     def ndofs(self, level):
@@ -147,14 +155,18 @@ class P1Function:
         except Exception:
             pass
 
+    def _shape(self, level):
+        return (self.ndofs(level),) if self.nvec == 1 else (self.ndofs(level), self.nvec)
+
     def upload(self, level, values):
+        """values: (ndofs,) or, for interleaved functions, (ndofs, nvec)"""
         v = np.ascontiguousarray(values, dtype=np.float64)
-        if v.shape != (self.ndofs(level),):
+        if v.shape != self._shape(level):
             raise ValueError("wrong number of values")
         self.ctx.check(lib().hg_function_upload(self.ctx.h, self.h, level, _ptr(v)), "upload")
 
     def download(self, level):
-        out = np.zeros(self.ndofs(level))
+        out = np.zeros(self._shape(level))
         self.ctx.check(lib().hg_function_download(self.ctx.h, self.h, level, _ptr(out)), "download")
         return out
 
@@ -178,9 +190,20 @@ class P1Function:
                        "interpolate_random")
 
     def dot(self, other, level, flag=ALL):
+        if self.nvec != 1:
+            out = np.zeros(self.nvec)
+            self.ctx.check(lib().hg_function_dot_batched(self.ctx.h, self.h, other.h, level, flag, _ptr(out)), "dot")
+            return out
         out = C.c_double()
         self.ctx.check(lib().hg_function_dot(self.ctx.h, self.h, other.h, level, flag, C.byref(out)), "dot")
         return out.value
+
+    def scale(self, scales, level, flag=ALL):
+        """vector k *= scales[k] (interleaved functions)"""
+        sc = np.ascontiguousarray(scales, dtype=np.float64)
+        if sc.shape != (self.nvec,):
+            raise ValueError("one scale per vector")
+        self.ctx.check(lib().hg_function_scale_batched(self.ctx.h, self.h, _ptr(sc), level, flag), "scale")
 
     def copy_from(self, other, level):
         self.ctx.check(lib().hg_function_copy_from(self.ctx.h, self.h, other.h, level), "copyFrom")
@@ -298,6 +321,27 @@ def p2_estimate_radius(A, x, level, iterations):
     r = C.c_double()
     A.ctx.check(lib().hg_p2_estimate_radius(A.ctx.h, A.h, x.h, level, iterations, C.byref(r)), "estimate radius")
     return r.value
+
+
+def batched_gmg_solve(A, x, b, min_level, max_level, pre=3, post=3, coarse_sweeps=10, cycles=30, tol=0.0):
+    """P1 GMG V(pre,post) with Gauss-Seidel for all interleaved vectors of x, b; returns
+    (cycles, residual norms of shape (cycles+1, nvec))"""
+    hist = np.zeros((cycles + 1, x.nvec))
+    done = C.c_int()
+    A.ctx.check(lib().hg_p1_batched_gmg_solve(A.ctx.h, A.h, x.h, b.h, min_level, max_level, pre, post, coarse_sweeps,
+                                              cycles, tol, _ptr(hist), C.byref(done)), "batched gmg")
+    return done.value, hist[: done.value + 1]
+
+
+def batched_ir_solve(A, x, b, min_level, max_level, low_dtype=F16, pre=3, post=3, coarse_sweeps=10, inner_cycles=1,
+                     max_outer=40, tol=1e-10):
+    """FP64 iterative refinement around the low-precision batched V-cycle"""
+    hist = np.zeros((max_outer + 1, x.nvec))
+    done = C.c_int()
+    A.ctx.check(lib().hg_p1_batched_ir_solve(A.ctx.h, A.h, x.h, b.h, min_level, max_level, low_dtype, pre, post,
+                                             coarse_sweeps, inner_cycles, max_outer, tol, _ptr(hist),
+                                             C.byref(done)), "batched ir")
+    return done.value, hist[: done.value + 1]
 
 
 def p2_ir_solve(A, x, b, min_level, max_level, rho, low_dtype=F16, pre=2, post=2, order=3, inner_cycles=1,

@@ -274,10 +274,13 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
 // XCD (blocks are dispatched round-robin over the 8 XCDs) sweeps one contiguous
 // range of rows and keeps the neighbour rows it shares in its own L2.
 // 15-point stencil on cell interiors; one block per (cell, interior row)
+// Functions may interleave 2^lv vectors per lattice point (value k of point p
+// at (p << lv) + k); the cell kernels walk a row as (x, k) pairs with k fastest,
+// so a wavefront reads 2^lv consecutive values per stencil entry.
 template < class T >
 __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
                             int64_t mm, const double* __restrict__ stencil, const T* __restrict__ src,
-                            T* __restrict__ dst, int add )
+                            T* __restrict__ dst, int add, int lv )
 {
    using A             = typename Acc< T >::type;
    const int64_t lb    = xcdBlock( nrows * nc );
@@ -297,15 +300,20 @@ __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int
       rb[s] = base + tetRowStart( w, y + cOff15[s][1], z + cOff15[s][2] ) + cOff15[s][0];
    }
    const int64_t out = base + tetRowStart( w, y, z );
-   for ( int64_t x = 1 + threadIdx.x; x + y + z < mm; x += blockDim.x )
+   const int64_t msk = ( int64_t( 1 ) << lv ) - 1;
+   for ( int64_t t = threadIdx.x;; t += blockDim.x )
    {
+      const int64_t x = 1 + ( t >> lv ), k = t & msk;
+      if ( x + y + z >= mm )
+         break;
       A v = A( 0 );
 #pragma unroll
       for ( int s = 0; s < 15; ++s )
-         v += st[s] * toAcc( src[rb[s] + x] );
+         v += st[s] * toAcc( src[( ( rb[s] + x ) << lv ) + k] );
+      const int64_t o = ( ( out + x ) << lv ) + k;
       if ( add )
-         v += toAcc( dst[out + x] );
-      dst[out + x] = fromAcc< T >( v );
+         v += toAcc( dst[o] );
+      dst[o] = fromAcc< T >( v );
    }
 }
 
@@ -313,7 +321,7 @@ __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int
 template < class T >
 __global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
                           int64_t mm, const double* __restrict__ stencil, int cx, double relax,
-                          const T* __restrict__ b, T* __restrict__ x )
+                          const T* __restrict__ b, T* __restrict__ x, int lv )
 {
    using A          = typename Acc< T >::type;
    const int64_t lb = xcdBlock( nrows * nc );
@@ -334,14 +342,19 @@ __global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows, int64
    }
    const A       invc = A( 1 ) / st[0];
    const int64_t x0   = cx ? 1 : 2;
-   for ( int64_t xx = x0 + 2 * int64_t( threadIdx.x ); xx + y + z < mm; xx += 2 * int64_t( blockDim.x ) )
+   const int64_t msk  = ( int64_t( 1 ) << lv ) - 1;
+   for ( int64_t t = threadIdx.x;; t += blockDim.x )
    {
-      A v = toAcc( b[rb[0] + xx] );
+      const int64_t xx = x0 + 2 * ( t >> lv ), k = t & msk;
+      if ( xx + y + z >= mm )
+         break;
+      const int64_t o = ( ( rb[0] + xx ) << lv ) + k;
+      A             v = toAcc( b[o] );
 #pragma unroll
       for ( int s = 1; s < 15; ++s )
-         v -= st[s] * toAcc( x[rb[s] + xx] );
-      const A xo    = toAcc( x[rb[0] + xx] );
-      x[rb[0] + xx] = fromAcc< T >( ( A( 1 ) - A( relax ) ) * xo + A( relax ) * invc * v );
+         v -= st[s] * toAcc( x[( ( rb[s] + xx ) << lv ) + k] );
+      const A xo = toAcc( x[o] );
+      x[o]       = fromAcc< T >( ( A( 1 ) - A( relax ) ) * xo + A( relax ) * invc * v );
    }
 }
 
@@ -349,7 +362,8 @@ __global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows, int64
 template < class T >
 __device__ __forceinline__ typename Acc< T >::type bndStencilSum( const BndGeom& g, const double* __restrict__ cls,
                                                                   const T* __restrict__ v, int64_t c, int64_t x,
-                                                                  int64_t y, int64_t z, int k, int skip )
+                                                                  int64_t y, int64_t z, int k, int skip, int lv,
+                                                                  int64_t vec )
 {
    using A           = typename Acc< T >::type;
    const double* W   = cls + ( 14 * c + k ) * 15;
@@ -363,7 +377,7 @@ __device__ __forceinline__ typename Acc< T >::type bndStencilSum( const BndGeom&
       const int64_t nx = x + cOff15[q][0], ny = y + cOff15[q][1], nz = z + cOff15[q][2];
       if ( nx < 0 || ny < 0 || nz < 0 || nx + ny + nz > g.mm )
          continue;
-      s += A( wq ) * toAcc( v[base + tetIndex( g.w, nx, ny, nz )] );
+      s += A( wq ) * toAcc( v[( ( base + tetIndex( g.w, nx, ny, nz ) ) << lv ) + vec] );
    }
    return s;
 }
@@ -372,17 +386,19 @@ __device__ __forceinline__ typename Acc< T >::type bndStencilSum( const BndGeom&
 // cell's share, read from the cell lattices (halos refreshed before)
 template < class T >
 __global__ void kBndApply( BndGeom g, int64_t nb, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
-                           const double* __restrict__ cls, const T* __restrict__ src, T* __restrict__ dst, int add )
+                           const double* __restrict__ cls, const T* __restrict__ src, T* __restrict__ dst, int add,
+                           int lv )
 {
-   using A         = typename Acc< T >::type;
-   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t b   = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
    if ( b >= nb || !( bndFlag[b] & flag ) )
       return;
    A s = A( 0 );
    forEachIncident( g, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int k ) {
-      s += bndStencilSum( g, cls, src, c, x, y, z, k, 0 );
+      s += bndStencilSum( g, cls, src, c, x, y, z, k, 0, lv, vec );
    } );
-   const int64_t o = nc * g.sc + b;
+   const int64_t o = ( ( nc * g.sc + b ) << lv ) + vec;
    if ( add )
       s += toAcc( dst[o] );
    dst[o] = fromAcc< T >( s );
@@ -394,31 +410,33 @@ template < class T >
 __global__ void kBndSOR( BndGeom g, int64_t n, const int32_t* __restrict__ list, int64_t nc,
                          const uint8_t* __restrict__ bndFlag, int flag, const double* __restrict__ cls,
                          const double* __restrict__ invDiag, double relax, const T* __restrict__ rhs,
-                         T* __restrict__ x )
+                         T* __restrict__ x, int lv )
 {
-   using A         = typename Acc< T >::type;
-   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
    if ( i >= n )
       return;
    const int64_t b = list[i];
    if ( !( bndFlag[b] & flag ) )
       return;
-   const int64_t o = nc * g.sc + b;
+   const int64_t o = ( ( nc * g.sc + b ) << lv ) + vec;
    A             s = toAcc( rhs[o] );
    forEachIncident( g, b, [&]( int64_t c, int64_t xx, int64_t yy, int64_t zz, int k ) {
-      s -= bndStencilSum( g, cls, x, c, xx, yy, zz, k, 1 );
+      s -= bndStencilSum( g, cls, x, c, xx, yy, zz, k, 1, lv, vec );
    } );
    const T nv = fromAcc< T >( ( A( 1 ) - A( relax ) ) * toAcc( x[o] ) + A( relax ) * A( invDiag[b] ) * s );
    x[o]       = nv;
    forEachIncident( g, b, [&]( int64_t c, int64_t xx, int64_t yy, int64_t zz, int ) {
-      x[c * g.sc + tetIndex( g.w, xx, yy, zz )] = nv;
+      x[( ( c * g.sc + tetIndex( g.w, xx, yy, zz ) ) << lv ) + vec] = nv;
    } );
 }
 
 // restriction to coarse cell-interior points (full weighting, R = P^T)
 template < class T >
 __global__ void kRestrictCell( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t scc, int64_t wc,
-                               int64_t mmc, int64_t scf, int64_t wf, const T* __restrict__ fine, T* __restrict__ coarse )
+                               int64_t mmc, int64_t scf, int64_t wf, const T* __restrict__ fine, T* __restrict__ coarse,
+                               int lv )
 {
    using A          = typename Acc< T >::type;
    const int64_t lb = xcdBlock( nrows * nc );
@@ -433,14 +451,18 @@ __global__ void kRestrictCell( const int32_t* __restrict__ rows, int64_t nrows, 
    for ( int s = 0; s < 15; ++s )
       rb[s] = c * scf + tetRowStart( wf, 2 * y + cOff15[s][1], 2 * z + cOff15[s][2] ) + cOff15[s][0];
    const int64_t out = c * scc + tetRowStart( wc, y, z );
-   for ( int64_t x = 1 + threadIdx.x; x + y + z < mmc; x += blockDim.x )
+   const int64_t msk = ( int64_t( 1 ) << lv ) - 1;
+   for ( int64_t t = threadIdx.x;; t += blockDim.x )
    {
+      const int64_t x = 1 + ( t >> lv ), k = t & msk;
+      if ( x + y + z >= mmc )
+         break;
       A v = A( 0 );
 #pragma unroll
       for ( int s = 1; s < 15; ++s )
-         v += toAcc( fine[rb[s] + 2 * x] );
-      v               = toAcc( fine[rb[0] + 2 * x] ) + A( 0.5 ) * v;
-      coarse[out + x] = fromAcc< T >( v );
+         v += toAcc( fine[( ( rb[s] + 2 * x ) << lv ) + k] );
+      v                              = toAcc( fine[( ( rb[0] + 2 * x ) << lv ) + k] ) + A( 0.5 ) * v;
+      coarse[( ( out + x ) << lv ) + k] = fromAcc< T >( v );
    }
 }
 
@@ -450,10 +472,11 @@ __global__ void kRestrictCell( const int32_t* __restrict__ rows, int64_t nrows, 
 template < class T >
 __global__ void kRestrictBnd( BndGeom gc, int64_t nbc, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
                               const double* __restrict__ clsInvN, int64_t scf, int64_t wf, int64_t mmf,
-                              const T* __restrict__ fine, T* __restrict__ coarse )
+                              const T* __restrict__ fine, T* __restrict__ coarse, int lv )
 {
-   using A         = typename Acc< T >::type;
-   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t b   = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
    if ( b >= nbc || !( bndFlag[b] & flag ) )
       return;
    A s = A( 0 );
@@ -465,17 +488,17 @@ __global__ void kRestrictBnd( BndGeom gc, int64_t nbc, int64_t nc, const uint8_t
             continue;
          const int cl  = pointClass( mmf, fx, fy, fz );
          const A   wgt = ( q == 0 ? A( 1 ) : A( 0.5 ) ) * ( cl < 0 ? A( 1 ) : A( clsInvN[14 * c + cl] ) );
-         s += wgt * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
+         s += wgt * toAcc( fine[( ( c * scf + tetIndex( wf, fx, fy, fz ) ) << lv ) + vec] );
       }
    } );
-   coarse[nc * gc.sc + b] = fromAcc< T >( s );
+   coarse[( ( nc * gc.sc + b ) << lv ) + vec] = fromAcc< T >( s );
 }
 
 // linear interpolation to fine cell-interior points
 template < class T >
 __global__ void kProlongCell( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t scf, int64_t wf,
                               int64_t mmf, int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine,
-                              int add )
+                              int add, int lv )
 {
    using A          = typename Acc< T >::type;
    const int64_t lb = xcdBlock( nrows * nc );
@@ -486,21 +509,27 @@ __global__ void kProlongCell( const int32_t* __restrict__ rows, int64_t nrows, i
    const int64_t y   = rows[2 * r];
    const int64_t z   = rows[2 * r + 1];
    const int64_t out = c * scf + tetRowStart( wf, y, z );
-   for ( int64_t x = 1 + threadIdx.x; x + y + z < mmf; x += blockDim.x )
+   const int64_t msk = ( int64_t( 1 ) << lv ) - 1;
+   for ( int64_t t = threadIdx.x;; t += blockDim.x )
    {
+      const int64_t x = 1 + ( t >> lv ), k = t & msk;
+      if ( x + y + z >= mmf )
+         break;
       const int p = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
       A         v;
       if ( p == 0 )
-         v = toAcc( coarse[c * scc + tetIndex( wc, x / 2, y / 2, z / 2 )] );
+         v = toAcc( coarse[( ( c * scc + tetIndex( wc, x / 2, y / 2, z / 2 ) ) << lv ) + k] );
       else
       {
          const int dx = cMidDir[p][0], dy = cMidDir[p][1], dz = cMidDir[p][2];
-         v = A( 0.5 ) * ( toAcc( coarse[c * scc + tetIndex( wc, ( x - dx ) / 2, ( y - dy ) / 2, ( z - dz ) / 2 )] ) +
-                          toAcc( coarse[c * scc + tetIndex( wc, ( x + dx ) / 2, ( y + dy ) / 2, ( z + dz ) / 2 )] ) );
+         v = A( 0.5 ) *
+             ( toAcc( coarse[( ( c * scc + tetIndex( wc, ( x - dx ) / 2, ( y - dy ) / 2, ( z - dz ) / 2 ) ) << lv ) + k] ) +
+               toAcc( coarse[( ( c * scc + tetIndex( wc, ( x + dx ) / 2, ( y + dy ) / 2, ( z + dz ) / 2 ) ) << lv ) + k] ) );
       }
+      const int64_t o = ( ( out + x ) << lv ) + k;
       if ( add )
-         v += toAcc( fine[out + x] );
-      fine[out + x] = fromAcc< T >( v );
+         v += toAcc( fine[o] );
+      fine[o] = fromAcc< T >( v );
    }
 }
 
@@ -508,10 +537,12 @@ __global__ void kProlongCell( const int32_t* __restrict__ rows, int64_t nrows, i
 // the coarse lattice of the first incident cell (halos refreshed before)
 template < class T >
 __global__ void kProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
-                             int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine, int add )
+                             int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine, int add,
+                             int lv )
 {
-   using A         = typename Acc< T >::type;
-   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t b   = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
    if ( b >= nbf || !( bndFlag[b] & flag ) )
       return;
    bool done = false;
@@ -522,15 +553,16 @@ __global__ void kProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_t*
       done        = true;
       const int p = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
       if ( p == 0 )
-         v = toAcc( coarse[c * scc + tetIndex( wc, x / 2, y / 2, z / 2 )] );
+         v = toAcc( coarse[( ( c * scc + tetIndex( wc, x / 2, y / 2, z / 2 ) ) << lv ) + vec] );
       else
       {
          const int dx = cMidDir[p][0], dy = cMidDir[p][1], dz = cMidDir[p][2];
-         v = A( 0.5 ) * ( toAcc( coarse[c * scc + tetIndex( wc, ( x - dx ) / 2, ( y - dy ) / 2, ( z - dz ) / 2 )] ) +
-                          toAcc( coarse[c * scc + tetIndex( wc, ( x + dx ) / 2, ( y + dy ) / 2, ( z + dz ) / 2 )] ) );
+         v = A( 0.5 ) *
+             ( toAcc( coarse[( ( c * scc + tetIndex( wc, ( x - dx ) / 2, ( y - dy ) / 2, ( z - dz ) / 2 ) ) << lv ) + vec] ) +
+               toAcc( coarse[( ( c * scc + tetIndex( wc, ( x + dx ) / 2, ( y + dy ) / 2, ( z + dz ) / 2 ) ) << lv ) + vec] ) );
       }
    } );
-   const int64_t o = nc * gf.sc + b;
+   const int64_t o = ( ( nc * gf.sc + b ) << lv ) + vec;
    if ( add )
       v += toAcc( fine[o] );
    fine[o] = fromAcc< T >( v );
@@ -557,29 +589,32 @@ static P1OpLevel& levelOf( Context* ctx, P1Operator& op, int L )
 
 void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L, int flag, int update )
 {
-   if ( src.st != op.st || dst.st != op.st || src.dtype != dst.dtype || src.off != 0 || dst.off != 0 )
-      throw Error( -1, "apply needs P1 src/dst on the operator's storage with one value type" );
+   if ( src.st != op.st || dst.st != op.st || src.dtype != dst.dtype || src.off != 0 || dst.off != 0 ||
+        src.lv != dst.lv )
+      throw Error( -1, "apply needs P1 src/dst on the operator's storage with one value type and vector count" );
+   const int lv = src.lv;
    if ( &src == &dst )
       throw Error( -1, "apply needs distinct src and dst" );
    P1OpLevel& o   = levelOf( ctx, op, L );
    Lattice&   lat = op.st->lattice( ctx, L );
    void*      s   = src.ptr( L );
    void*      d   = dst.ptr( L );
-   pullHalos( ctx, *op.st, lat, s, src.dtype );
+   pullHalos( ctx, *op.st, lat, s, src.dtype, lv );
    dispatchType( src.dtype, [&]( auto tag ) {
       using T = decltype( tag );
       if ( lat.nb > 0 )
       {
-         kBndApply< T ><<< gridFor( lat.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( *op.st, lat ), lat.nb, lat.nc,
-                                                                         lat.bndFlag.p, flag, o.clsStencil.p,
-                                                                         (const T*) s, (T*) d, update );
+         kBndApply< T ><<< gridFor( lat.nb << lv ), kBlock, 0, ctx->stream >>>( BndGeom::of( *op.st, lat ), lat.nb,
+                                                                               lat.nc, lat.bndFlag.p, flag,
+                                                                               o.clsStencil.p, (const T*) s, (T*) d,
+                                                                               update, lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
       if ( ( flag & kInner ) && lat.nrows > 0 )
       {
          ctx->beginTiming();
          kCellApply< T ><<< xcdGrid( lat.nrows * lat.nc ), kRowBlock, 0, ctx->stream >>>(
-             lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update );
+             lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update, lv );
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
       }
@@ -588,14 +623,15 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
 
 void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax, int L, int flag )
 {
-   if ( x.st != op.st || b.st != op.st || x.dtype != b.dtype || &x == &b || x.off != 0 || b.off != 0 )
-      throw Error( -1, "smooth_sor needs distinct x/b on the operator's storage with one value type" );
+   if ( x.st != op.st || b.st != op.st || x.dtype != b.dtype || &x == &b || x.off != 0 || b.off != 0 || x.lv != b.lv )
+      throw Error( -1, "smooth_sor needs distinct x/b on the operator's storage with one value type and vector count" );
+   const int lv = x.lv;
    P1OpLevel& o   = levelOf( ctx, op, L );
    Lattice&   lat = op.st->lattice( ctx, L );
    void*      xp  = x.ptr( L );
    const void* bp = b.ptr( L );
    ctx->beginTiming();
-   pullHalos( ctx, *op.st, lat, xp, x.dtype );
+   pullHalos( ctx, *op.st, lat, xp, x.dtype, lv );
    dispatchType( x.dtype, [&]( auto tag ) {
       using T = decltype( tag );
       // owned vertex/edge/face DoFs colour by colour (each update also refreshes
@@ -607,9 +643,10 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
          const int64_t n = o.colorStart[cc + 1] - o.colorStart[cc];
          if ( n == 0 )
             continue;
-         kBndSOR< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( g, n, o.colorRows.p + o.colorStart[cc], lat.nc,
-                                                                  lat.bndFlag.p, flag, o.clsStencil.p, o.invDiag.p,
-                                                                  relax, (const T*) bp, (T*) xp );
+         kBndSOR< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>( g, n, o.colorRows.p + o.colorStart[cc],
+                                                                        lat.nc, lat.bndFlag.p, flag, o.clsStencil.p,
+                                                                        o.invDiag.p, relax, (const T*) bp, (T*) xp,
+                                                                        lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
       if ( !( flag & kInner ) || lat.nrows == 0 )
@@ -624,7 +661,7 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
                   continue;
                kCellSOR< T ><<< xcdGrid( n * lat.nc ), kRowBlock, 0, ctx->stream >>>(
                    lat.rows.p + 2 * lat.rowGroup[grp], n, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, cx, relax,
-                   (const T*) bp, (T*) xp );
+                   (const T*) bp, (T*) xp, lv );
                HG_CHECK_HIP( hipGetLastError() );
             }
    } );
@@ -641,20 +678,20 @@ void p1Restrict( Context* ctx, Function& f, int fineL, int flag )
    void*    fp = f.ptr( fineL );
    void*    cp = f.ptr( fineL - 1 );
    ctx->beginTiming();
-   pullHalos( ctx, st, lf, fp, f.dtype );
+   pullHalos( ctx, st, lf, fp, f.dtype, f.lv );
    dispatchType( f.dtype, [&]( auto tag ) {
       using T = decltype( tag );
       if ( lc.nrows > 0 )
       {
          kRestrictCell< T ><<< xcdGrid( lc.nrows * lc.nc ), kRowBlock, 0, ctx->stream >>>(
-             lc.rowsNat.p, lc.nrows, lc.nc, lc.sc, lc.w, lc.mm, lf.sc, lf.w, (const T*) fp, (T*) cp );
+             lc.rowsNat.p, lc.nrows, lc.nc, lc.sc, lc.w, lc.mm, lf.sc, lf.w, (const T*) fp, (T*) cp, f.lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
       if ( lc.nb > 0 )
       {
-         kRestrictBnd< T ><<< gridFor( lc.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lc ), lc.nb, lc.nc,
-                                                                           lc.bndFlag.p, flag, st.clsInvN.p, lf.sc,
-                                                                           lf.w, lf.mm, (const T*) fp, (T*) cp );
+         kRestrictBnd< T ><<< gridFor( lc.nb << f.lv ), kBlock, 0, ctx->stream >>>(
+             BndGeom::of( st, lc ), lc.nb, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp,
+             (T*) cp, f.lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
    } );
@@ -671,20 +708,19 @@ void p1Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update 
    void*    fp = f.ptr( coarseL + 1 );
    void*    cp = f.ptr( coarseL );
    ctx->beginTiming();
-   pullHalos( ctx, st, lc, cp, f.dtype );
+   pullHalos( ctx, st, lc, cp, f.dtype, f.lv );
    dispatchType( f.dtype, [&]( auto tag ) {
       using T = decltype( tag );
       if ( lf.nrows > 0 )
       {
          kProlongCell< T ><<< xcdGrid( lf.nrows * lf.nc ), kRowBlock, 0, ctx->stream >>>(
-             lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update );
+             lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update, f.lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
       if ( lf.nb > 0 )
       {
-         kProlongBnd< T ><<< gridFor( lf.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lf ), lf.nb, lf.nc,
-                                                                          lf.bndFlag.p, flag, lc.sc, lc.w,
-                                                                          (const T*) cp, (T*) fp, update );
+         kProlongBnd< T ><<< gridFor( lf.nb << f.lv ), kBlock, 0, ctx->stream >>>(
+             BndGeom::of( st, lf ), lf.nb, lf.nc, lf.bndFlag.p, flag, lc.sc, lc.w, (const T*) cp, (T*) fp, update, f.lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
    } );

@@ -729,8 +729,8 @@ static int64_t microCellsPerCell( int64_t mm )
 
 static void p2CheckFunction( const P2Operator& op, const Function& f, int L )
 {
-   if ( f.st != op.st || f.off != 1 )
-      throw Error( -1, "P2 operator needs P2 functions on its storage" );
+   if ( f.st != op.st || f.off != 1 || f.lv != 0 )
+      throw Error( -1, "P2 operator needs (single-vector) P2 functions on its storage" );
    if ( L < op.minL || L > op.maxL || L < f.minL || L > f.maxL )
       throw Error( -1, "level outside the operator's or function's level range" );
 }
@@ -831,16 +831,18 @@ double* p2InverseDiagonal( Context* ctx, P2Operator& op, int L )
 
 template < class T >
 __global__ void kMult( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag,
-                       const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ dst )
+                       const T* __restrict__ a, const T* __restrict__ b, int lv, T* __restrict__ dst )
 {
-   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   const int64_t n = ( flag & kInner ? ncell : 0 ) + nb;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv;
+   const int64_t n   = ( flag & kInner ? ncell : 0 ) + nb;
    if ( i >= n )
       return;
-   const int64_t p = ( flag & kInner ) ? i : ncell + i;
-   if ( p >= ncell && !( bndFlag[p - ncell] & flag ) )
+   const int64_t pt = ( flag & kInner ) ? i : ncell + i;
+   if ( pt >= ncell && !( bndFlag[pt - ncell] & flag ) )
       return;
-   dst[p] = fromAcc< T >( toAcc( a[p] ) * toAcc( b[p] ) );
+   const int64_t p = ( pt << lv ) + ( tid & ( ( int64_t( 1 ) << lv ) - 1 ) );
+   dst[p]          = fromAcc< T >( toAcc( a[p] ) * toAcc( b[p] ) );
 }
 
 static const int kVofHost[6][4][3] = {
@@ -1186,16 +1188,19 @@ int hg_function_mult( hg_ctx* h, hg_handle dst, hg_handle a, hg_handle b, int le
       Function& fa = ctx->function( a );
       Function& fb = ctx->function( b );
       for ( Function* f : { &d, &fa, &fb } )
-         if ( f->st != d.st || f->dtype != d.dtype || f->off != d.off || level < f->minL || level > f->maxL )
-            throw Error( HG_ERR_ARG, "multElementwise needs functions of one storage, space and value type" );
+         if ( f->st != d.st || f->dtype != d.dtype || f->off != d.off || f->lv != d.lv || level < f->minL ||
+              level > f->maxL )
+            throw Error( HG_ERR_ARG, "multElementwise needs functions of one storage, space, value type and vector count" );
       Lattice&      l = d.lat( ctx, level );
       const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
       if ( n == 0 )
          return;
       dispatchType( d.dtype, [&]( auto tag ) {
          using T = decltype( tag );
-         kMult< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, (const T*) fa.ptr( level ),
-                                                                (const T*) fb.ptr( level ), (T*) d.ptr( level ) );
+         kMult< T ><<< gridFor( n << d.lv ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag,
+                                                                         (const T*) fa.ptr( level ),
+                                                                         (const T*) fb.ptr( level ), d.lv,
+                                                                         (T*) d.ptr( level ) );
       } );
       HG_CHECK_HIP( hipGetLastError() );
    } );

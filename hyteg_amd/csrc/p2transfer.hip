@@ -319,7 +319,7 @@ static constexpr int kRowBlockT = 64;
 
 void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
 {
-   if ( f.off != 1 || fineL - 1 < f.minL || fineL > f.maxL )
+   if ( f.off != 1 || f.lv != 0 || fineL - 1 < f.minL || fineL > f.maxL )
       throw Error( HG_ERR_ARG, "P2 restriction needs a P2 function holding levels fineLevel-1 and fineLevel" );
    uploadTables();
    Storage& st = *f.st;
@@ -348,7 +348,7 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
 
 void p2Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update )
 {
-   if ( f.off != 1 || coarseL < f.minL || coarseL + 1 > f.maxL )
+   if ( f.off != 1 || f.lv != 0 || coarseL < f.minL || coarseL + 1 > f.maxL )
       throw Error( HG_ERR_ARG, "P2 prolongation needs a P2 function holding levels sourceLevel and sourceLevel+1" );
    uploadTables();
    Storage& st = *f.st;

@@ -145,10 +145,12 @@ Lattice& Storage::lattice( Context* ctx, int L )
 
 template < class T >
 __global__ void kPull( const CellMeta* __restrict__ meta, int64_t nc, int64_t sc, int64_t w, int64_t mm, int64_t fin,
-                       int64_t ein, int64_t eoff, int64_t foff, T* __restrict__ v )
+                       int64_t ein, int64_t eoff, int64_t foff, int lv, T* __restrict__ v )
 {
    const int64_t per = 4 * fin + 6 * ein + 4;
-   const int64_t gid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t gid = tid >> lv;
+   const int64_t vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
    if ( gid >= per * nc )
       return;
    const int64_t   c = gid / per;
@@ -180,46 +182,49 @@ __global__ void kPull( const CellMeta* __restrict__ meta, int64_t nc, int64_t sc
       t[k]                      = mm;
       src                       = base + m.vert[k];
    }
-   v[c * sc + tetIndex( w, t[1], t[2], t[3] )] = v[src];
+   v[( ( c * sc + tetIndex( w, t[1], t[2], t[3] ) ) << lv ) + vec] = v[( src << lv ) + vec];
 }
 
 template < class T >
 __global__ void kPushAdditive( BndGeom g, const uint8_t* __restrict__ bndFlag, int64_t nb, int64_t nc,
-                               int excludeFlag, int zeroOut, T* __restrict__ v )
+                               int excludeFlag, int zeroOut, int lv, T* __restrict__ v )
 {
-   using A         = typename Acc< T >::type;
-   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t b   = tid >> lv;
+   const int64_t vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
    if ( b >= nb || ( bndFlag[b] & excludeFlag ) )
       return;
    const int64_t base = nc * g.sc;
-   A             s    = zeroOut ? A( 0 ) : toAcc( v[base + b] );
+   A             s    = zeroOut ? A( 0 ) : toAcc( v[( ( base + b ) << lv ) + vec] );
    forEachIncident( g, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
-      s += toAcc( v[c * g.sc + tetIndex( g.w, x, y, z )] );
+      s += toAcc( v[( ( c * g.sc + tetIndex( g.w, x, y, z ) ) << lv ) + vec] );
    } );
-   v[base + b] = fromAcc< T >( s );
+   v[( ( base + b ) << lv ) + vec] = fromAcc< T >( s );
 }
 
-void pullHalos( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype )
+void pullHalos( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int lv )
 {
    const int64_t n = lat.haloPerCell() * lat.nc;
    if ( n == 0 )
       return;
    dispatchType( dtype, [&]( auto tag ) {
       using T = decltype( tag );
-      kPull< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( st.meta.p, lat.nc, lat.sc, lat.w, lat.mm, lat.num.fin,
-                                                             lat.num.ein, lat.num.eoff, lat.num.foff, (T*) data );
+      kPull< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>( st.meta.p, lat.nc, lat.sc, lat.w, lat.mm,
+                                                                   lat.num.fin, lat.num.ein, lat.num.eoff,
+                                                                   lat.num.foff, lv, (T*) data );
    } );
    HG_CHECK_HIP( hipGetLastError() );
 }
 
-void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int excludeFlag, bool zeroOut )
+void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int excludeFlag, bool zeroOut, int lv )
 {
    if ( lat.nb == 0 )
       return;
    dispatchType( dtype, [&]( auto tag ) {
       using T = decltype( tag );
-      kPushAdditive< T ><<< gridFor( lat.nb ), kBlock, 0, ctx->stream >>>(
-          BndGeom::of( st, lat ), lat.bndFlag.p, lat.nb, lat.nc, excludeFlag, zeroOut ? 1 : 0, (T*) data );
+      kPushAdditive< T ><<< gridFor( lat.nb << lv ), kBlock, 0, ctx->stream >>>(
+          BndGeom::of( st, lat ), lat.bndFlag.p, lat.nb, lat.nc, excludeFlag, zeroOut ? 1 : 0, lv, (T*) data );
    } );
    HG_CHECK_HIP( hipGetLastError() );
 }
@@ -229,19 +234,22 @@ void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtyp
 // copies included: they are refreshed before any read anyway); the owned
 // vertex/edge/face part is filtered by DoFType.
 // ------------------------------------------------------------------------
+// (2^lv interleaved vectors: thread i handles value i & (2^lv - 1) of point i >> lv)
 template < class T >
 __global__ void kLinComb( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, int nsrc,
                           double s0, double s1, double s2, const T* __restrict__ f0, const T* __restrict__ f1,
-                          const T* __restrict__ f2, int add, T* __restrict__ dst )
+                          const T* __restrict__ f2, int add, int lv, T* __restrict__ dst )
 {
    using A           = typename Acc< T >::type;
-   const int64_t i   = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv;
    const int64_t n   = ( flag & kInner ? ncell : 0 ) + nb;
    if ( i >= n )
       return;
-   const int64_t p = ( flag & kInner ) ? i : ncell + i;
-   if ( p >= ncell && !( bndFlag[p - ncell] & flag ) )
+   const int64_t pt = ( flag & kInner ) ? i : ncell + i;
+   if ( pt >= ncell && !( bndFlag[pt - ncell] & flag ) )
       return;
+   const int64_t p = ( pt << lv ) + ( tid & ( ( int64_t( 1 ) << lv ) - 1 ) );
    A v = A( 0 );
    if ( nsrc > 0 )
       v += A( s0 ) * toAcc( f0[p] );
@@ -255,17 +263,36 @@ __global__ void kLinComb( int64_t ncell, int64_t nb, const uint8_t* __restrict__
 }
 
 template < class T >
-__global__ void kFill( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, double value,
+__global__ void kFill( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, double value, int lv,
                        T* __restrict__ dst )
 {
-   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   const int64_t n = ( flag & kInner ? ncell : 0 ) + nb;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv;
+   const int64_t n   = ( flag & kInner ? ncell : 0 ) + nb;
    if ( i >= n )
       return;
-   const int64_t p = ( flag & kInner ) ? i : ncell + i;
-   if ( p >= ncell && !( bndFlag[p - ncell] & flag ) )
+   const int64_t pt = ( flag & kInner ) ? i : ncell + i;
+   if ( pt >= ncell && !( bndFlag[pt - ncell] & flag ) )
       return;
-   dst[p] = fromAcc< T >( typename Acc< T >::type( value ) );
+   dst[( pt << lv ) + ( tid & ( ( int64_t( 1 ) << lv ) - 1 ) )] = fromAcc< T >( typename Acc< T >::type( value ) );
+}
+
+// per-vector scaling v[p][k] *= scale[k] on flagged DoFs
+template < class T >
+__global__ void kScaleVectors( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag,
+                               const double* __restrict__ scale, int lv, T* __restrict__ v )
+{
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv;
+   const int64_t n   = ( flag & kInner ? ncell : 0 ) + nb;
+   if ( i >= n )
+      return;
+   const int64_t pt = ( flag & kInner ) ? i : ncell + i;
+   if ( pt >= ncell && !( bndFlag[pt - ncell] & flag ) )
+      return;
+   const int64_t k = tid & ( ( int64_t( 1 ) << lv ) - 1 );
+   const int64_t p = ( pt << lv ) + k;
+   v[p]            = fromAcc< T >( typename Acc< T >::type( scale[k] ) * toAcc( v[p] ) );
 }
 
 // SplitMix64 of (seed, canonical DoF id) -> uniform [lo, hi): independent of
@@ -280,26 +307,37 @@ __device__ __forceinline__ double canonicalUniform( uint64_t seed, uint64_t id, 
 }
 
 // cell-interior points: one block per (cell, interior row)
+// (vector k of an interleaved function uses seed + k)
 template < class T >
 __global__ void kRandomCells( const int32_t* __restrict__ rows, int64_t nrows, int64_t sc, int64_t w, int64_t mm,
-                              int64_t coff, int64_t cin, uint64_t seed, double lo, double hi, T* __restrict__ v )
+                              int64_t coff, int64_t cin, uint64_t seed, double lo, double hi, int lv,
+                              T* __restrict__ v )
 {
-   const int64_t c = blockIdx.y;
-   const int64_t r = blockIdx.x;
-   const int64_t y = rows[2 * r], z = rows[2 * r + 1];
+   const int64_t c   = blockIdx.y;
+   const int64_t r   = blockIdx.x;
+   const int64_t y   = rows[2 * r], z = rows[2 * r + 1];
    const int64_t out = c * sc + tetRowStart( w, y, z );
    const int64_t cid = coff + c * cin + tetIndex( mm - 3, 0, y - 1, z - 1 );
-   for ( int64_t x = 1 + threadIdx.x; x + y + z < mm; x += blockDim.x )
-      v[out + x] = fromAcc< T >( typename Acc< T >::type( canonicalUniform( seed, uint64_t( cid + x - 1 ), lo, hi ) ) );
+   const int64_t msk = ( int64_t( 1 ) << lv ) - 1;
+   for ( int64_t t = threadIdx.x;; t += blockDim.x )
+   {
+      const int64_t x = 1 + ( t >> lv ), k = t & msk;
+      if ( x + y + z >= mm )
+         break;
+      v[( ( out + x ) << lv ) + k] = fromAcc< T >(
+          typename Acc< T >::type( canonicalUniform( seed + uint64_t( k ), uint64_t( cid + x - 1 ), lo, hi ) ) );
+   }
 }
 
 template < class T >
 __global__ void kRandomOwned( int64_t nb, int64_t base, const uint8_t* __restrict__ bndFlag, int flag, uint64_t seed,
-                              double lo, double hi, T* __restrict__ v )
+                              double lo, double hi, int lv, T* __restrict__ v )
 {
-   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t b   = tid >> lv, k = tid & ( ( int64_t( 1 ) << lv ) - 1 );
    if ( b < nb && ( bndFlag[b] & flag ) )
-      v[base + b] = fromAcc< T >( typename Acc< T >::type( canonicalUniform( seed, uint64_t( b ), lo, hi ) ) );
+      v[( ( base + b ) << lv ) + k] =
+          fromAcc< T >( typename Acc< T >::type( canonicalUniform( seed + uint64_t( k ), uint64_t( b ), lo, hi ) ) );
 }
 
 template < class TD, class TS >
@@ -332,39 +370,44 @@ void castCopy( Context* ctx, const void* src, int sdtype, void* dst, int ddtype,
    HG_CHECK_HIP( hipGetLastError() );
 }
 
+// per-block partial sums; with 2^lv interleaved vectors thread t always sees
+// vector t & (2^lv - 1), and partials[block * 2^lv + k] holds vector k's sum
 template < class T >
 __global__ void kDotPartial( int64_t ncell, int64_t nb, const uint32_t* __restrict__ cellMask,
                              const uint8_t* __restrict__ bndFlag, int flag, const T* __restrict__ a,
-                             const T* __restrict__ b, double* __restrict__ partials )
+                             const T* __restrict__ b, int lv, double* __restrict__ partials )
 {
    __shared__ double red[kBlock];
    double            s      = 0.0;
-   const int64_t     stride = int64_t( gridDim.x ) * blockDim.x;
+   const int64_t     k      = threadIdx.x & ( ( 1 << lv ) - 1 );
+   const int64_t     stride = ( int64_t( gridDim.x ) * blockDim.x ) >> lv;
+   const int64_t     first  = ( int64_t( blockIdx.x ) * blockDim.x + threadIdx.x ) >> lv;
    if ( flag & kInner )
-      for ( int64_t p = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x; p < ncell; p += stride )
+      for ( int64_t p = first; p < ncell; p += stride )
          if ( ( cellMask[p >> 5] >> ( p & 31 ) ) & 1u )
-            s += double( toAcc( a[p] ) ) * double( toAcc( b[p] ) );
-   for ( int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x; i < nb; i += stride )
+            s += double( toAcc( a[( p << lv ) + k] ) ) * double( toAcc( b[( p << lv ) + k] ) );
+   for ( int64_t i = first; i < nb; i += stride )
       if ( bndFlag[i] & flag )
-         s += double( toAcc( a[ncell + i] ) ) * double( toAcc( b[ncell + i] ) );
+         s += double( toAcc( a[( ( ncell + i ) << lv ) + k] ) ) * double( toAcc( b[( ( ncell + i ) << lv ) + k] ) );
    red[threadIdx.x] = s;
    __syncthreads();
-   for ( int o = kBlock / 2; o > 0; o >>= 1 )
+   for ( int o = kBlock / 2; o >= ( 1 << lv ); o >>= 1 )
    {
       if ( threadIdx.x < o )
          red[threadIdx.x] += red[threadIdx.x + o];
       __syncthreads();
    }
-   if ( threadIdx.x == 0 )
-      partials[blockIdx.x] = red[0];
+   if ( threadIdx.x < ( 1 << lv ) )
+      partials[( int64_t( blockIdx.x ) << lv ) + threadIdx.x] = red[threadIdx.x];
 }
 
-__global__ void kDotFinal( const double* __restrict__ partials, int n, double* __restrict__ out )
+// out[k] = sum over blocks of partials[block * 2^lv + k], one block per vector
+__global__ void kDotFinal( const double* __restrict__ partials, int n, int lv, double* __restrict__ out )
 {
    __shared__ double red[kBlock];
    double            s = 0.0;
    for ( int i = threadIdx.x; i < n; i += blockDim.x )
-      s += partials[i];
+      s += partials[( int64_t( i ) << lv ) + blockIdx.x];
    red[threadIdx.x] = s;
    __syncthreads();
    for ( int o = kBlock / 2; o > 0; o >>= 1 )
@@ -374,23 +417,32 @@ __global__ void kDotFinal( const double* __restrict__ partials, int n, double* _
       __syncthreads();
    }
    if ( threadIdx.x == 0 )
-      out[0] = red[0];
+      out[blockIdx.x] = red[0];
+}
+
+void dotDeviceBatched( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv, double* out )
+{
+   const int nv = 1 << lv;
+   dispatchType( dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      kDotPartial< T ><<< kDotBlocks, kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, lat.nb, lat.cellMask.p,
+                                                                lat.bndFlag.p, flag, (const T*) a, (const T*) b, lv,
+                                                                ctx->partials.p );
+   } );
+   double* res = ctx->partials.p + ( int64_t( kDotBlocks ) << kMaxLv );
+   kDotFinal<<< nv, kBlock, 0, ctx->stream >>>( ctx->partials.p, kDotBlocks, lv, res );
+   HG_CHECK_HIP( hipGetLastError() );
+   HG_CHECK_HIP( hipMemcpyAsync( ctx->hostResult, res, sizeof( double ) * nv, hipMemcpyDeviceToHost, ctx->stream ) );
+   HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+   for ( int k = 0; k < nv; ++k )
+      out[k] = ctx->hostResult[k];
 }
 
 double dotDevice( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag )
 {
-   dispatchType( dtype, [&]( auto tag ) {
-      using T = decltype( tag );
-      kDotPartial< T ><<< kDotBlocks, kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, lat.nb, lat.cellMask.p,
-                                                                lat.bndFlag.p, flag, (const T*) a, (const T*) b,
-                                                                ctx->partials.p );
-   } );
-   kDotFinal<<< 1, kBlock, 0, ctx->stream >>>( ctx->partials.p, kDotBlocks, ctx->partials.p + kDotBlocks );
-   HG_CHECK_HIP( hipGetLastError() );
-   HG_CHECK_HIP( hipMemcpyAsync( ctx->hostResult, ctx->partials.p + kDotBlocks, sizeof( double ), hipMemcpyDeviceToHost,
-                                 ctx->stream ) );
-   HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
-   return *ctx->hostResult;
+   double r;
+   dotDeviceBatched( ctx, lat, a, b, dtype, flag, 0, &r );
+   return r;
 }
 
 // ------------------------------------------------------------------------
@@ -492,10 +544,10 @@ int hg_ctx_create( int device, hg_ctx** out )
       ctx->device = device;
       HG_CHECK_HIP( hipSetDevice( device ) );
       HG_CHECK_HIP( hipStreamCreateWithFlags( &ctx->stream, hipStreamNonBlocking ) );
-      HG_CHECK_HIP( hipHostMalloc( (void**) &ctx->hostResult, sizeof( double ) * 8, hipHostMallocDefault ) );
+      HG_CHECK_HIP( hipHostMalloc( (void**) &ctx->hostResult, sizeof( double ) * ( 1 << kMaxLv ), hipHostMallocDefault ) );
       HG_CHECK_HIP( hipEventCreate( &ctx->ev0 ) );
       HG_CHECK_HIP( hipEventCreate( &ctx->ev1 ) );
-      ctx->partials.alloc( ctx, kDotBlocks + 8 );
+      ctx->partials.alloc( ctx, ( size_t( kDotBlocks ) << kMaxLv ) + ( 1 << kMaxLv ) );
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
    } catch ( const std::exception& e )
    {
@@ -659,16 +711,23 @@ int hg_p1_dof_flags( hg_ctx* h, hg_handle s, int level, int32_t* flags )
    } );
 }
 
-static int functionCreate( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, int off, hg_handle* out )
+static int functionCreate( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, int off, int nvec,
+                           hg_handle* out )
 {
    return hg_guard( h, [&]( Context* ctx ) {
       if ( !out || minLevel < 0 || maxLevel < minLevel || maxLevel + off > 12 || dtype < 0 || dtype > 2 )
          throw Error( HG_ERR_ARG, "invalid function parameters" );
+      int lv = 0;
+      while ( ( 1 << lv ) < nvec )
+         ++lv;
+      if ( nvec < 1 || ( 1 << lv ) != nvec || lv > kMaxLv )
+         throw Error( HG_ERR_ARG, "the number of interleaved vectors must be a power of two up to 64" );
       Storage& st = ctx->storage( s );
       auto     f  = std::make_unique< Function >();
       f->st       = &st;
       f->dtype    = dtype;
       f->off      = off;
+      f->lv       = lv;
       f->minL     = minLevel;
       f->maxL     = maxLevel;
       for ( int L = minLevel; L <= maxLevel; ++L )
@@ -676,7 +735,7 @@ static int functionCreate( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
          Lattice& l = st.lattice( ctx, L + off );
          // two spare values: a 16-bit value is updated atomically through its
          // aligned 32-bit word, which for the last value may extend past nstore
-         f->data[L + off].alloc( ctx, size_t( l.nstore + 2 ) * f->elemSize() );
+         f->data[L + off].alloc( ctx, size_t( l.nstore + 2 ) * f->elemSize() << lv );
       }
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
       *out                  = ctx->nextHandle++;
@@ -686,12 +745,12 @@ static int functionCreate( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
 
 int hg_p1_function_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, hg_handle* out )
 {
-   return functionCreate( h, s, minLevel, maxLevel, dtype, 0, out );
+   return functionCreate( h, s, minLevel, maxLevel, dtype, 0, 1, out );
 }
 
 int hg_p2_function_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, hg_handle* out )
 {
-   return functionCreate( h, s, minLevel, maxLevel, dtype, 1, out );
+   return functionCreate( h, s, minLevel, maxLevel, dtype, 1, 1, out );
 }
 
 int hg_function_destroy( hg_ctx* h, hg_handle fn )
@@ -726,14 +785,18 @@ int hg_function_upload( hg_ctx* h, hg_handle fn, int level, const double* canoni
       checkLevel( f, level );
       if ( !canonical )
          throw Error( HG_ERR_ARG, "null data" );
-      Lattice& l = f.lat( ctx, level );
+      Lattice&  l  = f.lat( ctx, level );
+      const int nv = f.nvec();
       dispatchType( f.dtype, [&]( auto tag ) {
          using T = decltype( tag );
-         std::vector< T > hbuf( l.nstore, hostFrom< T >( 0.0 ) );
-         forEachCanonical( l, [&]( int64_t id, int64_t pos ) { hbuf[pos] = hostFrom< T >( canonical[id] ); } );
-         HG_CHECK_HIP( hipMemcpyAsync( f.ptr( level ), hbuf.data(), l.nstore * sizeof( T ), hipMemcpyHostToDevice,
+         std::vector< T > hbuf( size_t( l.nstore ) * nv, hostFrom< T >( 0.0 ) );
+         forEachCanonical( l, [&]( int64_t id, int64_t pos ) {
+            for ( int k = 0; k < nv; ++k )
+               hbuf[pos * nv + k] = hostFrom< T >( canonical[id * nv + k] );
+         } );
+         HG_CHECK_HIP( hipMemcpyAsync( f.ptr( level ), hbuf.data(), hbuf.size() * sizeof( T ), hipMemcpyHostToDevice,
                                        ctx->stream ) );
-         pullHalos( ctx, *f.st, l, f.ptr( level ), f.dtype );
+         pullHalos( ctx, *f.st, l, f.ptr( level ), f.dtype, f.lv );
          HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
       } );
    } );
@@ -746,14 +809,18 @@ int hg_function_download( hg_ctx* h, hg_handle fn, int level, double* canonical 
       checkLevel( f, level );
       if ( !canonical )
          throw Error( HG_ERR_ARG, "null data" );
-      Lattice& l = f.lat( ctx, level );
+      Lattice&  l  = f.lat( ctx, level );
+      const int nv = f.nvec();
       dispatchType( f.dtype, [&]( auto tag ) {
          using T = decltype( tag );
-         std::vector< T > hbuf( l.nstore );
-         HG_CHECK_HIP( hipMemcpyAsync( hbuf.data(), f.ptr( level ), l.nstore * sizeof( T ), hipMemcpyDeviceToHost,
+         std::vector< T > hbuf( size_t( l.nstore ) * nv );
+         HG_CHECK_HIP( hipMemcpyAsync( hbuf.data(), f.ptr( level ), hbuf.size() * sizeof( T ), hipMemcpyDeviceToHost,
                                        ctx->stream ) );
          HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
-         forEachCanonical( l, [&]( int64_t id, int64_t pos ) { canonical[id] = hostTo< T >( hbuf[pos] ); } );
+         forEachCanonical( l, [&]( int64_t id, int64_t pos ) {
+            for ( int k = 0; k < nv; ++k )
+               canonical[id * nv + k] = hostTo< T >( hbuf[pos * nv + k] );
+         } );
       } );
    } );
 }
@@ -771,8 +838,8 @@ static int linComb( hg_ctx* h, hg_handle dst, int nsrc, const double* scalars, c
       {
          Function& s = ctx->function( srcs[i] );
          checkLevel( s, level );
-         if ( s.st != d.st || s.dtype != d.dtype || s.off != d.off )
-            throw Error( HG_ERR_ARG, "assign/add need functions of one storage and one value type" );
+         if ( s.st != d.st || s.dtype != d.dtype || s.off != d.off || s.lv != d.lv )
+            throw Error( HG_ERR_ARG, "assign/add need functions of one storage, value type and vector count" );
          src[i] = s.ptr( level );
       }
       Lattice&      l = d.lat( ctx, level );
@@ -784,9 +851,9 @@ static int linComb( hg_ctx* h, hg_handle dst, int nsrc, const double* scalars, c
          s[i] = scalars[i];
       dispatchType( d.dtype, [&]( auto tag ) {
          using T = decltype( tag );
-         kLinComb< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, nsrc, s[0],
-                                                                   s[1], s[2], (const T*) src[0], (const T*) src[1],
-                                                                   (const T*) src[2], add, (T*) d.ptr( level ) );
+         kLinComb< T ><<< gridFor( n << d.lv ), kBlock, 0, ctx->stream >>>(
+             l.nc * l.sc, l.nb, l.bndFlag.p, flag, nsrc, s[0], s[1], s[2], (const T*) src[0], (const T*) src[1],
+             (const T*) src[2], add, d.lv, (T*) d.ptr( level ) );
       } );
       HG_CHECK_HIP( hipGetLastError() );
    } );
@@ -813,8 +880,8 @@ int hg_function_interpolate_const( hg_ctx* h, hg_handle fn, double value, int le
          return;
       dispatchType( f.dtype, [&]( auto tag ) {
          using T = decltype( tag );
-         kFill< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, value,
-                                                                (T*) f.ptr( level ) );
+         kFill< T ><<< gridFor( n << f.lv ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, value,
+                                                                         f.lv, (T*) f.ptr( level ) );
       } );
       HG_CHECK_HIP( hipGetLastError() );
    } );
@@ -832,11 +899,11 @@ int hg_function_interpolate_random( hg_ctx* h, hg_handle fn, uint64_t seed, doub
          {
             dim3 grid( unsigned( l.nrows ), unsigned( l.nc ) );
             kRandomCells< T ><<< grid, 64, 0, ctx->stream >>>( l.rowsNat.p, l.nrows, l.sc, l.w, l.mm, l.num.coff,
-                                                              l.num.cin, seed, lo, hi, (T*) f.ptr( level ) );
+                                                              l.num.cin, seed, lo, hi, f.lv, (T*) f.ptr( level ) );
          }
          if ( l.nb > 0 )
-            kRandomOwned< T ><<< gridFor( l.nb ), kBlock, 0, ctx->stream >>>( l.nb, l.nc * l.sc, l.bndFlag.p, flag, seed,
-                                                                             lo, hi, (T*) f.ptr( level ) );
+            kRandomOwned< T ><<< gridFor( l.nb << f.lv ), kBlock, 0, ctx->stream >>>(
+                l.nb, l.nc * l.sc, l.bndFlag.p, flag, seed, lo, hi, f.lv, (T*) f.ptr( level ) );
       } );
       HG_CHECK_HIP( hipGetLastError() );
    } );
@@ -851,9 +918,65 @@ int hg_function_dot( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, d
       Function& fb = ctx->function( b );
       checkLevel( fa, level );
       checkLevel( fb, level );
-      if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off )
-         throw Error( HG_ERR_ARG, "dot needs functions of one storage and one value type" );
+      if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off || fa.lv != fb.lv )
+         throw Error( HG_ERR_ARG, "dot needs functions of one storage, value type and vector count" );
+      if ( fa.lv != 0 )
+         throw Error( HG_ERR_ARG, "dot of interleaved vectors: use hg_function_dot_batched" );
       *out = dotDevice( ctx, fa.lat( ctx, level ), fa.ptr( level ), fb.ptr( level ), fa.dtype, flag );
+   } );
+}
+
+int hg_function_dot_batched( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, double* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      Function& fa = ctx->function( a );
+      Function& fb = ctx->function( b );
+      checkLevel( fa, level );
+      checkLevel( fb, level );
+      if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off || fa.lv != fb.lv )
+         throw Error( HG_ERR_ARG, "dot needs functions of one storage, value type and vector count" );
+      dotDeviceBatched( ctx, fa.lat( ctx, level ), fa.ptr( level ), fb.ptr( level ), fa.dtype, flag, fa.lv, out );
+   } );
+}
+
+int hg_function_scale_batched( hg_ctx* h, hg_handle fn, const double* scales, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& f = ctx->function( fn );
+      checkLevel( f, level );
+      if ( !scales )
+         throw Error( HG_ERR_ARG, "null scales" );
+      Lattice&      l = f.lat( ctx, level );
+      const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
+      if ( n == 0 )
+         return;
+      double* dev = ctx->partials.p + ( int64_t( kDotBlocks ) << kMaxLv );
+      HG_CHECK_HIP( hipMemcpyAsync( dev, scales, sizeof( double ) * f.nvec(), hipMemcpyHostToDevice, ctx->stream ) );
+      dispatchType( f.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         kScaleVectors< T ><<< gridFor( n << f.lv ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag,
+                                                                                 dev, f.lv, (T*) f.ptr( level ) );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+      // the scales live in the reduction scratch: wait before the next reduction reuses it
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+   } );
+}
+
+int hg_p1_function_create_batched( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, int nvec,
+                                   hg_handle* out )
+{
+   return functionCreate( h, s, minLevel, maxLevel, dtype, 0, nvec, out );
+}
+
+int hg_function_nvec( hg_ctx* h, hg_handle fn, int* nvec )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !nvec )
+         throw Error( HG_ERR_ARG, "null output" );
+      *nvec = ctx->function( fn ).nvec();
    } );
 }
 
@@ -864,10 +987,10 @@ int hg_function_copy_from( hg_ctx* h, hg_handle dst, hg_handle src, int level )
       Function& s = ctx->function( src );
       checkLevel( d, level );
       checkLevel( s, level );
-      if ( d.st != s.st || d.off != s.off )
-         throw Error( HG_ERR_ARG, "copyFrom needs functions of one storage and one space" );
+      if ( d.st != s.st || d.off != s.off || d.lv != s.lv )
+         throw Error( HG_ERR_ARG, "copyFrom needs functions of one storage, space and vector count" );
       Lattice& l = d.lat( ctx, level );
-      castCopy( ctx, s.ptr( level ), s.dtype, d.ptr( level ), d.dtype, l.nstore );
+      castCopy( ctx, s.ptr( level ), s.dtype, d.ptr( level ), d.dtype, l.nstore << d.lv );
    } );
 }
 
@@ -876,7 +999,7 @@ int hg_function_communicate( hg_ctx* h, hg_handle fn, int level )
    return hg_guard( h, [&]( Context* ctx ) {
       Function& f = ctx->function( fn );
       checkLevel( f, level );
-      pullHalos( ctx, *f.st, f.lat( ctx, level ), f.ptr( level ), f.dtype );
+      pullHalos( ctx, *f.st, f.lat( ctx, level ), f.ptr( level ), f.dtype, f.lv );
    } );
 }
 

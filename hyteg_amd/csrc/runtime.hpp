@@ -139,9 +139,11 @@ struct Function
    Storage*                           st    = nullptr;
    int                                dtype = 0; // 0 f64, 1 f32, 2 f16
    int                                off   = 0;
+   int                                lv    = 0; // log2 of the number of interleaved vectors
    int                                minL = 0, maxL = 0;
    std::map< int, DevBuf< uint8_t > > data; // by lattice level, nstore elements
    size_t                             elemSize() const { return dtype == 0 ? 8 : ( dtype == 1 ? 4 : 2 ); }
+   int                                nvec() const { return 1 << lv; }
    void*                              ptr( int L )
    {
       auto it = data.find( L + off );
@@ -227,6 +229,7 @@ inline void DevBuf< T >::release()
 // ---- device helpers used across translation units ----------------------
 constexpr int kBlock      = 256;
 constexpr int kDotBlocks  = 1024;
+constexpr int kMaxLv      = 6; // at most 64 interleaved vectors per function
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2): blockIdx b runs on XCD b % 8.  xcdBlock maps the hardware block id to a
@@ -258,15 +261,19 @@ inline unsigned gridFor( int64_t n, int block = kBlock )
 
 // Refresh every halo copy in the cell lattices from its owner (the
 // Vertex->Edge->Face->Cell communicate chain of VertexDoFFunction).
-void pullHalos( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype );
+// lv: log2 of the number of vectors interleaved per lattice point (0: one vector)
+void pullHalos( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int lv = 0 );
 // dst[i] = src[i] converted between value types (HG_F64 / HG_F32 / HG_F16)
 void castCopy( Context* ctx, const void* src, int sdtype, void* dst, int ddtype, int64_t n );
 // Sum the halo copies of every owned vertex/edge/face DoF into the owner
 // (communicateAdditively<Cell, Face/Edge/Vertex>).  Owners whose type matches
 // excludeFlag are left untouched; zeroOut replaces instead of adding.
-void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int excludeFlag, bool zeroOut );
+void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int excludeFlag, bool zeroOut,
+                   int lv = 0 );
 // FP64 dot over owned DoFs of matching type
 double dotDevice( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag );
+// per-vector dot products of two functions with 2^lv interleaved vectors
+void dotDeviceBatched( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv, double* out );
 // storage type dispatch
 template < class F >
 void dispatchType( int dtype, F&& f )

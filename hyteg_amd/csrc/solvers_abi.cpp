@@ -132,9 +132,151 @@ int iterativeRefinement( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, in
    }
 }
 
+// ---- interleaved multi-vector P1 solves -------------------------------------
+// per-vector Inner residual norms of b - A x (r is scratch)
+template < class Op, class F >
+std::vector< double > residualNorms( const Op& A, const F& x, const F& b, const F& r, uint_t level )
+{
+   A.apply( x, r, level, Inner, Replace );
+   r.assign( { 1.0, -1.0 }, { b, r }, level, Inner );
+   std::vector< double > n = r.dotGlobalBatched( r, level, Inner );
+   for ( double& v : n )
+      v = std::sqrt( v );
+   return n;
+}
+
+bool allBelow( const std::vector< double >& r, const std::vector< double >& r0, double tol )
+{
+   for ( size_t k = 0; k < r.size(); ++k )
+      if ( r[k] > tol * r0[k] )
+         return false;
+   return true;
+}
+
+void storeHist( double* hist, int k, const std::vector< double >& r )
+{
+   if ( hist )
+      for ( size_t j = 0; j < r.size(); ++j )
+         hist[size_t( k ) * r.size() + j] = r[j];
+}
+
+// GMG V(pre,post) with Gauss-Seidel smoothing on functions of one value type
+// holding nvec interleaved vectors; the coarsest level gets coarseSweeps sweeps
+template < typename VT >
+std::unique_ptr< GeometricMultigridSolver< P1ConstantLaplaceOperatorT< VT > > >
+    batchedGMG( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel, int pre, int post,
+                int coarseSweeps, int nvec )
+{
+   using Op      = P1ConstantLaplaceOperatorT< VT >;
+   auto smoother = std::make_shared< GaussSeidelSmoother< Op > >();
+   auto coarse   = std::make_shared< SmootherSweeps< Op > >( smoother, uint_t( coarseSweeps ) );
+   return std::make_unique< GeometricMultigridSolver< Op > >(
+       storage, smoother, coarse, std::make_shared< P1toP1LinearRestriction< VT > >(),
+       std::make_shared< P1toP1LinearProlongation< VT > >(), uint_t( minLevel ), uint_t( maxLevel ), uint_t( pre ),
+       uint_t( post ), 0, CycleType::VCYCLE, nvec );
+}
+
+// FP64 iterative refinement around the low-precision batched V-cycle; every
+// vector's residual is scaled by its own RMS / (stencil centre) so that the
+// correction is O(1) in FP16
+template < typename L >
+int batchedIR( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre, int post,
+               int coarseSweeps, int innerCycles, int maxOuter, double tol, double* hist )
+{
+   auto storage = storageOf( ctx, x );
+   auto A       = P1ConstantLaplaceOperator::borrow( ctx, op );
+   auto Alo     = P1ConstantLaplaceOperatorT< L >::borrow( ctx, op );
+   auto X       = P1Function< double >::borrow( ctx, x );
+   auto B       = P1Function< double >::borrow( ctx, b );
+   const int    nv  = X->nvec();
+   const uint_t top = uint_t( maxLevel );
+   auto         gmg = batchedGMG< L >( storage, minLevel, maxLevel, pre, post, coarseSweeps, nv );
+   P1Function< L >      xlo( "bir_x", storage, uint_t( minLevel ), top, nv ), blo( "bir_b", storage, uint_t( minLevel ), top, nv );
+   P1Function< double > r( "bir_r", storage, top, top, nv ), e( "bir_e", storage, top, top, nv );
+   int64_t              n = 0;
+   check( ctx, hg_p1_ndofs( ctx, storage->handle(), maxLevel, &n ), "ndofs" );
+   double centre[15];
+   check( ctx, hg_p1_cell_stencil( ctx, op, maxLevel, 0, centre ), "stencil" );
+   const DoFType         flag = Inner | NeumannBoundary | FreeslipBoundary;
+   std::vector< double > r0;
+   for ( int k = 0;; ++k )
+   {
+      A->apply( *X, r, top, flag, Replace );
+      r.assign( { 1.0, -1.0 }, { *B, r }, top, flag );
+      std::vector< double > rk = r.dotGlobalBatched( r, top, Inner );
+      for ( double& v : rk )
+         v = std::sqrt( v );
+      if ( k == 0 )
+         r0 = rk;
+      storeHist( hist, k, rk );
+      if ( k == maxOuter || ( k > 0 && tol > 0 && allBelow( rk, r0, tol ) ) )
+         return k;
+      std::vector< double > scale( static_cast< size_t >( nv ) ), inv( static_cast< size_t >( nv ) );
+      const std::vector< double > ms = r.dotGlobalBatched( r, top, flag );
+      for ( int j = 0; j < nv; ++j )
+      {
+         scale[j] = std::sqrt( ms[j] / double( n ) ) / std::fabs( centre[0] );
+         inv[j]   = scale[j] > 0 ? 1.0 / scale[j] : 0.0;
+      }
+      r.scaleBatched( inv, top, flag );
+      blo.copyFrom( r, top );
+      xlo.setToZero( top );
+      for ( int j = 0; j < innerCycles; ++j )
+         gmg->solve( *Alo, xlo, blo, top );
+      e.copyFrom( xlo, top );
+      e.scaleBatched( scale, top, flag );
+      X->add( { 1.0 }, { e }, top, flag );
+   }
+}
+
 } // namespace
 
 extern "C" {
+
+int hg_p1_batched_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre,
+                             int post, int coarseSweeps, int cycles, double tol, double* res_hist, int* cycles_done )
+{
+   if ( !ctx || minLevel < 0 || maxLevel < minLevel || pre < 0 || post < 0 || cycles < 0 || coarseSweeps < 0 )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      auto storage = storageOf( ctx, x );
+      auto A       = P1ConstantLaplaceOperator::borrow( ctx, op );
+      auto X       = P1Function< double >::borrow( ctx, x );
+      auto B       = P1Function< double >::borrow( ctx, b );
+      auto gmg     = batchedGMG< double >( storage, minLevel, maxLevel, pre, post, coarseSweeps, X->nvec() );
+      P1Function< double >        r( "residual", storage, uint_t( maxLevel ), uint_t( maxLevel ), X->nvec() );
+      const std::vector< double > r0 = residualNorms( *A, *X, *B, r, uint_t( maxLevel ) );
+      storeHist( res_hist, 0, r0 );
+      int k = 0;
+      while ( k < cycles )
+      {
+         gmg->solve( *A, *X, *B, uint_t( maxLevel ) );
+         ++k;
+         const std::vector< double > rk = residualNorms( *A, *X, *B, r, uint_t( maxLevel ) );
+         storeHist( res_hist, k, rk );
+         if ( tol > 0 && allBelow( rk, r0, tol ) )
+            break;
+      }
+      if ( cycles_done )
+         *cycles_done = k;
+   } );
+}
+
+int hg_p1_batched_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel,
+                            int lowDtype, int pre, int post, int coarseSweeps, int innerCycles, int maxOuter, double tol,
+                            double* res_hist, int* outer_done )
+{
+   if ( !ctx || minLevel < 0 || maxLevel < minLevel || pre < 0 || post < 0 || coarseSweeps < 0 || innerCycles < 1 ||
+        maxOuter < 0 || ( lowDtype != HG_F32 && lowDtype != HG_F16 ) || !outer_done )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      *outer_done = lowDtype == HG_F32 ?
+                        batchedIR< float >( ctx, op, x, b, minLevel, maxLevel, pre, post, coarseSweeps, innerCycles,
+                                            maxOuter, tol, res_hist ) :
+                        batchedIR< f16 >( ctx, op, x, b, minLevel, maxLevel, pre, post, coarseSweeps, innerCycles,
+                                          maxOuter, tol, res_hist );
+   } );
+}
 
 int hg_p1_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int maxIter, double relTol, double absTol,
                     int* iterations )

@@ -105,6 +105,16 @@ int hg_p2_dof_flags( hg_ctx* ctx, hg_handle storage, int level, int32_t* flags )
 /* ---- functions: P1Function<T> (p1functionspace/VertexDoFFunction.hpp:58-512) */
 /* P1Function<T>(name, storage, minLevel, maxLevel), all values zero */
 int hg_p1_function_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, hg_handle* out );
+/* A P1 function holding nvec (a power of two up to 64) independent vectors
+ * interleaved per DoF: the values of DoF i are stored together (multi-vector
+ * layout for batched solves; no counterpart in the reference, which solves one
+ * system per call).  upload/download then take host arrays [ndofs][nvec];
+ * assign/add/mult/interpolate/copyFrom act on all vectors; interpolate_random
+ * gives vector k the seed seed+k. */
+int hg_p1_function_create_batched( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, int nvec,
+                                   hg_handle* out );
+/* number of interleaved vectors of a function */
+int hg_function_nvec( hg_ctx* ctx, hg_handle fn, int* nvec );
 /* P2Function<T>(name, storage, minLevel, maxLevel) (p2functionspace/P2Function.hpp:35-74) */
 int hg_p2_function_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, hg_handle* out );
 int hg_function_destroy( hg_ctx* ctx, hg_handle fn );
@@ -126,6 +136,10 @@ int hg_function_interpolate_const( hg_ctx* ctx, hg_handle fn, double value, int 
 int hg_function_interpolate_random( hg_ctx* ctx, hg_handle fn, uint64_t seed, double lo, double hi, int level, int flag );
 /* dotGlobal(other, level, flag) -- FP64 accumulation (VertexDoFFunction.cpp:1710-1716) */
 int hg_function_dot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
+/* per-vector dotGlobal of interleaved functions: out[nvec] */
+int hg_function_dot_batched( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
+/* vector k of fn scaled by scales[k] on flagged DoFs */
+int hg_function_scale_batched( hg_ctx* ctx, hg_handle fn, const double* scales, int level, int flag );
 /* multElementwise({a, b}, level, flag): dst = a * b (VertexDoFFunction.cpp:1408) */
 int hg_function_mult( hg_ctx* ctx, hg_handle dst, hg_handle a, hg_handle b, int level, int flag );
 /* copyFrom<otherT>(other, level): cast copy between storage types (VertexDoFFunction.hpp:151) */
@@ -246,6 +260,19 @@ int hg_p2_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int mi
 int hg_p2_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int lowDtype,
                     int pre, int post, int chebyshevOrder, double spectralRadius, int innerCycles, int maxOuter,
                     double tol, int coarseMaxIter, double coarseTol, double* res_hist, int* outer_done );
+
+/* Batched solves of the P1 Laplacian for all vectors of interleaved functions x, b:
+ * GMG V(pre,post) cycles with Gauss-Seidel smoothing (GeometricMultigridSolver,
+ * P1toP1Linear transfer), coarseSweeps Gauss-Seidel sweeps on minLevel, until every
+ * vector's Inner residual drops by tol; res_hist[(cycle) * nvec + k] */
+int hg_p1_batched_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre,
+                             int post, int coarseSweeps, int cycles, double tol, double* res_hist, int* cycles_done );
+/* The same inside FP64 iterative refinement, the V-cycles running on lowDtype
+ * (HG_F32 / HG_F16 storage, FP32 arithmetic) copies; each vector's residual is
+ * scaled separately before the cast */
+int hg_p1_batched_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel,
+                            int lowDtype, int pre, int post, int coarseSweeps, int innerCycles, int maxOuter, double tol,
+                            double* res_hist, int* outer_done );
 
 #ifdef __cplusplus
 }

@@ -175,18 +175,24 @@ class LatticeFunction
    using storageType = StorageType;
    using ValueType   = valueType;
 
+   // nvec > 1 (P1 only): that many independent vectors interleaved per DoF
    LatticeFunction( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
-                    uint_t maxLevel )
+                    uint_t maxLevel, int nvec = 1 )
    : name_( name )
    , storage_( storage )
    , ctx_( storage->ctx() )
    , owned_( true )
+   , nvec_( nvec )
    {
-      const int st = Space == 1 ? hg_p1_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ),
-                                                         DTypeOf< StorageType >::value, &h_ ) :
-                                  hg_p2_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ),
-                                                         DTypeOf< StorageType >::value, &h_ );
+      const int dt = DTypeOf< StorageType >::value;
+      const int st = Space == 1 ? ( nvec == 1 ? hg_p1_function_create( ctx_, storage->handle(), int( minLevel ),
+                                                                       int( maxLevel ), dt, &h_ ) :
+                                                hg_p1_function_create_batched( ctx_, storage->handle(), int( minLevel ),
+                                                                               int( maxLevel ), dt, nvec, &h_ ) ) :
+                                  hg_p2_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), dt, &h_ );
       check( ctx_, st, Space == 1 ? "P1Function" : "P2Function" );
+      if ( Space != 1 && nvec != 1 )
+         throw std::invalid_argument( "P2 functions hold one vector" );
    }
    ~LatticeFunction()
    {
@@ -203,6 +209,21 @@ class LatticeFunction
 
    hg_handle handle() const { return h_; }
    hg_ctx*   ctx() const { return ctx_; }
+   int       nvec() const { return nvec_; }
+   // per-vector dot products of interleaved functions
+   std::vector< double > dotGlobalBatched( const LatticeFunction& other, uint_t level, DoFType flag = All ) const
+   {
+      std::vector< double > r( static_cast< size_t >( nvec_ ) );
+      check( ctx_, hg_function_dot_batched( ctx_, h_, other.h_, int( level ), flag, r.data() ), "dotGlobalBatched" );
+      return r;
+   }
+   // v_k *= scales[k] for every interleaved vector k
+   void scaleBatched( const std::vector< double >& scales, uint_t level, DoFType flag = All ) const
+   {
+      if ( scales.size() != size_t( nvec_ ) )
+         throw std::invalid_argument( "scaleBatched needs one scale per vector" );
+      check( ctx_, hg_function_scale_batched( ctx_, h_, scales.data(), int( level ), flag ), "scaleBatched" );
+   }
 
    void assign( const std::vector< ValueType >&                                            scalars,
                 const std::vector< std::reference_wrapper< const LatticeFunction > >& functions,
@@ -250,7 +271,9 @@ class LatticeFunction
    : ctx_( ctx )
    , h_( h )
    , owned_( false )
-   {}
+   {
+      check( ctx_, hg_function_nvec( ctx_, h_, &nvec_ ), "nvec" );
+   }
    void linComb( const std::vector< ValueType >&                                            scalars,
                  const std::vector< std::reference_wrapper< const LatticeFunction > >& functions,
                  uint_t                                                                 level,
@@ -283,6 +306,7 @@ class LatticeFunction
    hg_ctx*                             ctx_   = nullptr;
    hg_handle                           h_     = 0;
    bool                                owned_ = true;
+   int                                 nvec_  = 1;
 };
 
 template < typename ValueType >
@@ -316,44 +340,50 @@ class SORSmoothable
    virtual void smooth_sor( const F& dst, const F& rhs, typename F::valueType relax, uint_t level, DoFType flag ) const = 0;
 };
 
-class P1ConstantLaplaceOperator : public Operator< P1Function< double >, P1Function< double > >,
-                                  public GSSmoothable< P1Function< double > >,
-                                  public SORSmoothable< P1Function< double > >
+// P1ConstantLaplaceOperator; VT is the value type of the functions it acts on
+// (double, float, or f16 storage with FP32 arithmetic); one device operator
+// serves every value type and any number of interleaved vectors
+template < typename VT >
+class P1ConstantLaplaceOperatorT : public Operator< P1Function< VT >, P1Function< VT > >,
+                                   public GSSmoothable< P1Function< VT > >,
+                                   public SORSmoothable< P1Function< VT > >
 {
  public:
-   P1ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
+   P1ConstantLaplaceOperatorT( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
    : ctx_( storage->ctx() )
    , owned_( true )
    {
-      check( ctx_, hg_p1_laplace_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), &h_ ), "P1ConstantLaplaceOperator" );
+      check( ctx_, hg_p1_laplace_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), &h_ ),
+             "P1ConstantLaplaceOperator" );
    }
-   static std::unique_ptr< P1ConstantLaplaceOperator > borrow( hg_ctx* ctx, hg_handle h )
+   static std::unique_ptr< P1ConstantLaplaceOperatorT > borrow( hg_ctx* ctx, hg_handle h )
    {
-      return std::unique_ptr< P1ConstantLaplaceOperator >( new P1ConstantLaplaceOperator( ctx, h ) );
+      return std::unique_ptr< P1ConstantLaplaceOperatorT >( new P1ConstantLaplaceOperatorT( ctx, h ) );
    }
-   ~P1ConstantLaplaceOperator() override
+   ~P1ConstantLaplaceOperatorT() override
    {
       if ( owned_ && h_ )
          hg_operator_destroy( ctx_, h_ );
    }
-   void apply( const P1Function< double >& src, const P1Function< double >& dst, uint_t level, DoFType flag,
+   void apply( const P1Function< VT >& src, const P1Function< VT >& dst, uint_t level, DoFType flag,
                UpdateType updateType = Replace ) const override
    {
       check( ctx_, hg_p1_apply( ctx_, h_, src.handle(), dst.handle(), int( level ), flag, updateType ), "apply" );
    }
-   void smooth_gs( const P1Function< double >& dst, const P1Function< double >& rhs, uint_t level, DoFType flag ) const override
+   void smooth_gs( const P1Function< VT >& dst, const P1Function< VT >& rhs, uint_t level, DoFType flag ) const override
    {
       smooth_sor( dst, rhs, 1.0, level, flag );
    }
-   void smooth_sor( const P1Function< double >& dst, const P1Function< double >& rhs, double relax, uint_t level,
-                    DoFType flag ) const override
+   void smooth_sor( const P1Function< VT >& dst, const P1Function< VT >& rhs, typename P1Function< VT >::valueType relax,
+                    uint_t level, DoFType flag ) const override
    {
-      check( ctx_, hg_p1_smooth_sor( ctx_, h_, dst.handle(), rhs.handle(), relax, int( level ), flag ), "smooth_sor" );
+      check( ctx_, hg_p1_smooth_sor( ctx_, h_, dst.handle(), rhs.handle(), double( relax ), int( level ), flag ),
+             "smooth_sor" );
    }
    hg_handle handle() const { return h_; }
 
  private:
-   P1ConstantLaplaceOperator( hg_ctx* ctx, hg_handle h )
+   P1ConstantLaplaceOperatorT( hg_ctx* ctx, hg_handle h )
    : ctx_( ctx )
    , h_( h )
    , owned_( false )
@@ -362,6 +392,7 @@ class P1ConstantLaplaceOperator : public Operator< P1Function< double >, P1Funct
    hg_handle h_   = 0;
    bool      owned_;
 };
+using P1ConstantLaplaceOperator = P1ConstantLaplaceOperatorT< double >;
 
 template < class F >
 class RestrictionOperator
@@ -752,6 +783,28 @@ class ChebyshevSmoother : public Solver< OperatorType >
    DoFType                              flag_ = Inner | NeumannBoundary | FreeslipBoundary;
 };
 
+// Coarse "solver" made of a fixed number of smoother sweeps (used for
+// interleaved multi-vector hierarchies, whose CG would need per-vector scalars)
+template < class OperatorType >
+class SmootherSweeps : public Solver< OperatorType >
+{
+ public:
+   SmootherSweeps( std::shared_ptr< Solver< OperatorType > > smoother, uint_t sweeps )
+   : smoother_( std::move( smoother ) )
+   , sweeps_( sweeps )
+   {}
+   void solve( const OperatorType& A, const typename OperatorType::srcType& x, const typename OperatorType::dstType& b,
+               uint_t level ) override
+   {
+      for ( uint_t i = 0; i < sweeps_; ++i )
+         smoother_->solve( A, x, b, level );
+   }
+
+ private:
+   std::shared_ptr< Solver< OperatorType > > smoother_;
+   uint_t                                    sweeps_;
+};
+
 namespace chebyshev {
 // estimateRadius (ChebyshevSmoother.hpp:656-665): power iteration on D^-1 A with
 // flag All (numerictools/SpectrumEstimation.hpp:56-90); x is the start vector and
@@ -827,7 +880,8 @@ class GeometricMultigridSolver : public Solver< OperatorType >
                              std::shared_ptr< RestrictionOperator< FunctionType > >  restrictionOperator,
                              std::shared_ptr< ProlongationOperator< FunctionType > > prolongationOperator,
                              uint_t minLevel, uint_t maxLevel, uint_t preSmoothSteps = 3, uint_t postSmoothSteps = 3,
-                             uint_t smoothIncrementOnCoarserGrids = 0, CycleType cycleType = CycleType::VCYCLE )
+                             uint_t smoothIncrementOnCoarserGrids = 0, CycleType cycleType = CycleType::VCYCLE,
+                             int nvec = 1 )
    : minLevel_( minLevel )
    , maxLevel_( maxLevel )
    , pre_( preSmoothSteps )
@@ -838,7 +892,7 @@ class GeometricMultigridSolver : public Solver< OperatorType >
    , coarse_( std::move( coarseSolver ) )
    , restriction_( std::move( restrictionOperator ) )
    , prolongation_( std::move( prolongationOperator ) )
-   , tmp_( "gmg_tmp", storage, minLevel, maxLevel )
+   , tmp_( "gmg_tmp", storage, minLevel, maxLevel, nvec )
    {}
 
    void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
