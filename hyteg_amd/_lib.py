@@ -21,6 +21,12 @@ SIGNATURES = {
     "hg_ctx_create": [_i, _p],
     "hg_ctx_destroy": [_p],
     "hg_ctx_synchronize": [_p],
+    "hg_graph_begin_capture": [_p],
+    "hg_graph_end_capture": [_p, _p],
+    "hg_graph_launch": [_p, _i64],
+    "hg_graph_destroy": [_p, _i64],
+    "hg_ctx_set_graphs": [_p, _i],
+    "hg_ctx_graphs_enabled": [_p, _p],
     "hg_ctx_memory": [_p, _p],
     "hg_ctx_event_timing": [_p, _i],
     "hg_ctx_last_kernel_ms": [_p, _p],
@@ -63,6 +69,8 @@ SIGNATURES = {
     "hg_p1_restrict": [_p, _i64, _i, _i],
     "hg_p1_function_create_batched": [_p, _i64, _i, _i, _i, _i, _p],
     "hg_function_nvec": [_p, _i64, _p],
+    "hg_serve_stream": [_i, _i64, _p, _p, _p, _p, _d, _i, _d, _p, _p, _p, _p, _p],
+    "hg_function_copy_vector": [_p, _i64, _i, _i64, _i, _i],
     "hg_function_dot_batched": [_p, _i64, _i64, _i, _i, _p],
     "hg_function_scale_batched": [_p, _i64, _p, _i, _i],
     "hg_p1_batched_gmg_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _i, _d, _p, _p],
@@ -111,5 +119,7 @@ def lib():
             f.restype = _i
         L.hg_last_error.argtypes = [_p]
         L.hg_last_error.restype = C.c_char_p
+        L.hg_serve_last_error.argtypes = []
+        L.hg_serve_last_error.restype = C.c_char_p
         _LIB = L
     return _LIB

@@ -41,6 +41,10 @@ class Context:
         self.check(lib().hg_ctx_last_kernel_ms(self.h, C.byref(out)), "last_ms")
         return out.value
 
+    def set_graphs(self, enable=True):
+        """let the batched multigrid solvers replay their V-cycle from a captured HIP graph"""
+        self.check(lib().hg_ctx_set_graphs(self.h, int(enable)), "set_graphs")
+
     def close(self):
         if getattr(self, "h", None):
             lib().hg_ctx_destroy(self.h)

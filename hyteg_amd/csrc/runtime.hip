@@ -358,6 +358,15 @@ __global__ void kCast( int64_t n, const TS* __restrict__ src, TD* __restrict__ d
    }
 }
 
+// dst vector kd (of 2^ld interleaved) = src vector ks (of 2^ls), n points
+template < class T >
+__global__ void kCopyVector( int64_t n, const T* __restrict__ src, int ls, int ks, T* __restrict__ dst, int ld, int kd )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( i < n )
+      dst[( i << ld ) + kd] = src[( i << ls ) + ks];
+}
+
 void castCopy( Context* ctx, const void* src, int sdtype, void* dst, int ddtype, int64_t n )
 {
    dispatchType( ddtype, [&]( auto td ) {
@@ -566,6 +575,9 @@ int hg_ctx_destroy( hg_ctx* h )
    (void) hipSetDevice( ctx->device );
    if ( ctx->stream )
       (void) hipStreamSynchronize( ctx->stream );
+   for ( auto& g : ctx->graphs )
+      (void) hipGraphExecDestroy( g.second );
+   ctx->graphs.clear();
    ctx->operators.clear();
    ctx->functions.clear();
    ctx->storages.clear();
@@ -587,6 +599,65 @@ const char* hg_last_error( const hg_ctx* h )
    if ( !h )
       return "null context";
    return reinterpret_cast< const Context* >( h )->err.c_str();
+}
+
+int hg_graph_begin_capture( hg_ctx* h )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      HG_CHECK_HIP( hipStreamBeginCapture( ctx->stream, hipStreamCaptureModeThreadLocal ) );
+   } );
+}
+
+int hg_graph_end_capture( hg_ctx* h, hg_handle* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      hipGraph_t g = nullptr;
+      HG_CHECK_HIP( hipStreamEndCapture( ctx->stream, &g ) );
+      hipGraphExec_t exec = nullptr;
+      const hipError_t e  = hipGraphInstantiate( &exec, g, nullptr, nullptr, 0 );
+      (void) hipGraphDestroy( g );
+      HG_CHECK_HIP( e );
+      *out               = ctx->nextHandle++;
+      ctx->graphs[*out] = exec;
+   } );
+}
+
+int hg_graph_launch( hg_ctx* h, hg_handle graph )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      auto it = ctx->graphs.find( graph );
+      if ( it == ctx->graphs.end() )
+         throw Error( HG_ERR_HANDLE, "unknown graph handle" );
+      HG_CHECK_HIP( hipGraphLaunch( it->second, ctx->stream ) );
+   } );
+}
+
+int hg_graph_destroy( hg_ctx* h, hg_handle graph )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      auto it = ctx->graphs.find( graph );
+      if ( it == ctx->graphs.end() )
+         throw Error( HG_ERR_HANDLE, "unknown graph handle" );
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+      HG_CHECK_HIP( hipGraphExecDestroy( it->second ) );
+      ctx->graphs.erase( it );
+   } );
+}
+
+int hg_ctx_set_graphs( hg_ctx* h, int enable )
+{
+   return hg_guard( h, [&]( Context* ctx ) { ctx->useGraphs = enable != 0; } );
+}
+
+int hg_ctx_graphs_enabled( hg_ctx* h, int* enabled )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !enabled )
+         throw Error( HG_ERR_ARG, "null output" );
+      *enabled = ctx->useGraphs ? 1 : 0;
+   } );
 }
 
 int hg_ctx_synchronize( hg_ctx* h )
@@ -969,6 +1040,28 @@ int hg_p1_function_create_batched( hg_ctx* h, hg_handle s, int minLevel, int max
                                    hg_handle* out )
 {
    return functionCreate( h, s, minLevel, maxLevel, dtype, 0, nvec, out );
+}
+
+int hg_function_copy_vector( hg_ctx* h, hg_handle dst, int kdst, hg_handle src, int ksrc, int level )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& d = ctx->function( dst );
+      Function& s = ctx->function( src );
+      checkLevel( d, level );
+      checkLevel( s, level );
+      if ( d.st != s.st || d.off != s.off || d.dtype != s.dtype )
+         throw Error( HG_ERR_ARG, "copy_vector needs functions of one storage, space and value type" );
+      if ( kdst < 0 || kdst >= d.nvec() || ksrc < 0 || ksrc >= s.nvec() )
+         throw Error( HG_ERR_ARG, "vector index out of range" );
+      Lattice&      l = d.lat( ctx, level );
+      const int64_t n = l.nstore;
+      dispatchType( d.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         kCopyVector< T ><<< gridFor( n ), kBlock, 0, ctx->stream >>>( n, (const T*) s.ptr( level ), s.lv, ksrc,
+                                                                      (T*) d.ptr( level ), d.lv, kdst );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
 }
 
 int hg_function_nvec( hg_ctx* h, hg_handle fn, int* nvec )

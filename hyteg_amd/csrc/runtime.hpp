@@ -162,6 +162,8 @@ struct Context
    int64_t                                            bytes      = 0;
    int64_t                                            nextHandle = 1;
    std::map< int64_t, std::unique_ptr< Storage > >      storages;
+   std::map< int64_t, hipGraphExec_t >                  graphs; // instantiated captures
+   bool                                                 useGraphs = true;
    std::map< int64_t, std::unique_ptr< Function > >     functions;
    std::map< int64_t, std::unique_ptr< OperatorBase > > operators;
 

@@ -1,0 +1,476 @@
+// Serving loop: a stream of multigrid solve requests (P1 Poisson and P2
+// variable-coefficient diffusion, cube and shell meshes, several levels) on
+// one GPU, with continuous batching.  No counterpart in the reference, whose
+// Solver::solve handles one system per call (solvers/Solver.hpp:27-34).
+//
+// Every request class (kind, level) is served by a worker thread with its own
+// context (one HIP stream), so the work of different classes overlaps on the
+// device.  A class worker is started when its first request arrives and its
+// hierarchy fits the device-memory budget next to the live ones; an idle
+// worker releases its hierarchy.
+//
+// P1 classes batch requests: the worker keeps one interleaved multi-vector
+// hierarchy with `slots` vectors and runs V(3,3) Gauss-Seidel cycles (replayed
+// from a captured HIP graph) on all of them.  After every cycle each active
+// slot's residual is checked; converged requests leave and waiting ones take
+// free slots at the next cycle boundary.  Vectors never interact, so a
+// request's result does not depend on what else is in the batch.  P2 classes
+// solve one request at a time (Chebyshev-smoothed V(2,2), PCG coarse solve),
+// with the coefficient k generated from the request's seed.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hyteg_amd/hyteg_amd.hpp"
+
+using namespace hyteg_amd;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum Kind
+{
+   P1Cube  = 0,
+   P1Shell = 1,
+   P2Shell = 2,
+   P2Cube  = 3
+};
+
+struct Request
+{
+   int64_t  idx;
+   int      kind, level;
+   uint64_t seed;
+   double   arrival;
+};
+
+struct Results
+{
+   double* latency;
+   double* relres;
+   double* checksum;
+   int*    cycles;
+};
+
+thread_local std::string tlsError;
+
+// SplitMix64 stream for the per-request coefficients
+struct SplitMix
+{
+   uint64_t s;
+   uint64_t next()
+   {
+      uint64_t z = ( s += 0x9E3779B97F4A7C15ull );
+      z          = ( z ^ ( z >> 30 ) ) * 0xBF58476D1CE4E5B9ull;
+      z          = ( z ^ ( z >> 27 ) ) * 0x94D049BB133111EBull;
+      return z ^ ( z >> 31 );
+   }
+   double uniform() { return double( next() >> 11 ) * ( 1.0 / 9007199254740992.0 ); }
+   double normal()
+   {
+      const double u1 = std::max( uniform(), 1e-300 ), u2 = uniform();
+      return std::sqrt( -2.0 * std::log( u1 ) ) * std::cos( 6.283185307179586 * u2 );
+   }
+};
+
+// k(x) = exp(sum_{m<16} a_m sin(omega_m . x + phi_m)), a ~ N(0, 0.5^2),
+// omega ~ N(0, 3^2 I), phi ~ U[0, 2 pi), drawn from the request's seed
+void coefficientOf( uint64_t seed, std::vector< double >& a, std::vector< double >& om, std::vector< double >& ph )
+{
+   SplitMix g{ seed ^ 0x5DEECE66Dull };
+   a.resize( 16 );
+   om.resize( 48 );
+   ph.resize( 16 );
+   for ( int m = 0; m < 16; ++m )
+   {
+      a[m] = 0.5 * g.normal();
+      for ( int d = 0; d < 3; ++d )
+         om[3 * m + d] = 3.0 * g.normal();
+      ph[m] = 6.283185307179586 * g.uniform();
+   }
+}
+
+int64_t latticeSize( int64_t w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
+
+// device bytes of a class hierarchy (functions of all levels plus operator
+// tables), generous estimate used for admission
+int64_t footprint( int kind, int level, int slots )
+{
+   const int64_t cells = ( kind == P1Cube || kind == P2Cube ) ? 6 : 1920;
+   const int     L     = ( kind == P2Shell || kind == P2Cube ) ? level + 1 : level;
+   int64_t       pts   = 0;
+   for ( int l = 0; l <= L; ++l )
+      pts += cells * latticeSize( ( int64_t( 1 ) << l ) + 1 );
+   if ( kind == P1Cube || kind == P1Shell )
+      return pts * 8 * int64_t( slots ) * 5 + ( int64_t( 64 ) << 20 );
+   // P2: about ten FP64 functions and the quadrature-point coefficients (~8 values per DoF)
+   return pts * 8 * 18 + ( int64_t( 64 ) << 20 );
+}
+
+struct Server
+{
+   std::mutex                                            m;
+   std::condition_variable                               cv;
+   std::map< std::pair< int, int >, std::deque< Request > > queues;
+   std::map< std::pair< int, int >, int >                live; // workers per class
+   int64_t                                               memBudget = 0, memUsed = 0;
+   bool                                                  allReleased = false;
+   Clock::time_point                                     t0;
+   Results                                               res{};
+   double                                                tol     = 1e-8;
+   int                                                   slots   = 32;
+   int                                                   device  = 0;
+   std::string                                           failure;
+   std::vector< std::thread >                            threads;
+
+   double now() const { return std::chrono::duration< double >( Clock::now() - t0 ).count(); }
+
+   void finish( const Request& r, double relres, double checksum, int cycles )
+   {
+      res.latency[r.idx]  = now() - r.arrival;
+      res.relres[r.idx]   = relres;
+      res.checksum[r.idx] = checksum;
+      res.cycles[r.idx]   = cycles;
+   }
+
+   // pops up to `max` requests of a class; waits (bounded) when none is queued.
+   // Returns false when the worker should exit (idle and everything released).
+   bool take( std::pair< int, int > cls, size_t max, bool mayWait, std::vector< Request >& out )
+   {
+      std::unique_lock< std::mutex > lk( m );
+      auto&                          q = queues[cls];
+      if ( q.empty() && mayWait )
+         cv.wait_for( lk, std::chrono::milliseconds( 200 ), [&] { return !q.empty() || allReleased; } );
+      while ( !q.empty() && out.size() < max )
+      {
+         out.push_back( q.front() );
+         q.pop_front();
+      }
+      return !( q.empty() && out.empty() && mayWait );
+   }
+
+   void release( std::pair< int, int > cls, int64_t bytes )
+   {
+      std::lock_guard< std::mutex > lk( m );
+      memUsed -= bytes;
+      live[cls] -= 1;
+      cv.notify_all();
+   }
+
+   void fail( const std::string& what )
+   {
+      std::lock_guard< std::mutex > lk( m );
+      if ( failure.empty() )
+         failure = what;
+      cv.notify_all();
+   }
+};
+
+// ---- P1 class: continuous batching over interleaved slots ------------------
+void p1Worker( Server& sv, int kind, int level, int64_t bytes )
+{
+   const std::pair< int, int > cls{ kind, level };
+   try
+   {
+      auto ctx = std::make_shared< Context >( sv.device );
+      std::shared_ptr< PrimitiveStorage > st;
+      if ( kind == P1Cube )
+      {
+         const double lo[3] = { 0, 0, 0 }, hi[3] = { 1, 1, 1 };
+         st                 = PrimitiveStorage::cuboid( ctx, lo, hi, 1, 1, 1 );
+      }
+      else
+      {
+         hg_handle h;
+         check( ctx->get(), hg_storage_shell( ctx->get(), 5, 3, 0.55, 1.0, &h ), "shell" );
+         st = std::make_shared< PrimitiveStorage >( ctx, h );
+      }
+      const int    S = sv.slots;
+      const uint_t L = uint_t( level );
+      P1ConstantLaplaceOperator A( st, 0, L );
+      P1Function< double >      X( "x", st, 0, L, S ), B( "b", st, 0, L, S ), R( "r", st, L, L, S );
+      P1Function< double >      one( "rhs", st, L, L ), zero( "zero", st, L, L );
+      auto smoother = std::make_shared< GaussSeidelSmoother< P1ConstantLaplaceOperator > >();
+      auto coarse   = std::make_shared< SmootherSweeps< P1ConstantLaplaceOperator > >( smoother, 30 );
+      GeometricMultigridSolver< P1ConstantLaplaceOperator > gmg(
+          st, smoother, coarse, std::make_shared< P1toP1LinearRestriction<> >(),
+          std::make_shared< P1toP1LinearProlongation<> >(), 0, L, 3, 3, 0, CycleType::VCYCLE, S );
+      hg_handle graph  = 0;
+      bool      warmed = false;
+      std::vector< Request > slot( S );
+      std::vector< bool >    active( S, false );
+      std::vector< double >  r0( S, 0.0 );
+      std::vector< int >     cyc( S, 0 );
+      const int              maxCycles = 60;
+      for ( ;; )
+      {
+         int nActive = 0;
+         for ( int k = 0; k < S; ++k )
+            nActive += active[k];
+         std::vector< Request > in;
+         if ( !sv.take( cls, size_t( S - nActive ), nActive == 0, in ) && nActive == 0 )
+            break;
+         if ( !in.empty() )
+         {
+            int k = 0;
+            for ( const Request& r : in )
+            {
+               while ( active[k] )
+                  ++k;
+               one.interpolate( 0.0, L, All );
+               check( ctx->get(), hg_function_interpolate_random( ctx->get(), one.handle(), r.seed, -1.0, 1.0, int( L ), Inner ),
+                      "rhs" );
+               B.copyVector( k, one, 0, L );
+               X.copyVector( k, zero, 0, L );
+               slot[k]   = r;
+               active[k] = true;
+               cyc[k]    = 0;
+            }
+            const std::vector< double > bb = B.dotGlobalBatched( B, L, Inner );
+            for ( const Request& r : in )
+               for ( int j = 0; j < S; ++j )
+                  if ( active[j] && slot[j].idx == r.idx )
+                     r0[j] = std::sqrt( bb[j] );
+         }
+         nActive = 0;
+         for ( int k = 0; k < S; ++k )
+            nActive += active[k];
+         if ( nActive == 0 )
+            continue;
+         // one V-cycle on every slot (idle slots hold b = 0, x = 0 and stay there)
+         if ( !warmed )
+         {
+            gmg.solve( A, X, B, L );
+            warmed = true;
+         }
+         else
+         {
+            if ( !graph )
+            {
+               check( ctx->get(), hg_graph_begin_capture( ctx->get() ), "capture" );
+               gmg.solve( A, X, B, L );
+               check( ctx->get(), hg_graph_end_capture( ctx->get(), &graph ), "capture" );
+            }
+            check( ctx->get(), hg_graph_launch( ctx->get(), graph ), "graph launch" );
+         }
+         A.apply( X, R, L, Inner, Replace );
+         R.assign( { 1.0, -1.0 }, { B, R }, L, Inner );
+         const std::vector< double > rr = R.dotGlobalBatched( R, L, Inner );
+         std::vector< int >          done;
+         for ( int k = 0; k < S; ++k )
+         {
+            if ( !active[k] )
+               continue;
+            ++cyc[k];
+            const double rel = r0[k] > 0 ? std::sqrt( rr[k] ) / r0[k] : 0.0;
+            if ( rel <= sv.tol || cyc[k] >= maxCycles )
+               done.push_back( k );
+         }
+         if ( !done.empty() )
+         {
+            const std::vector< double > xx = X.dotGlobalBatched( X, L, All );
+            for ( int k : done )
+            {
+               sv.finish( slot[k], r0[k] > 0 ? std::sqrt( rr[k] ) / r0[k] : 0.0, xx[k], cyc[k] );
+               B.copyVector( k, zero, 0, L );
+               X.copyVector( k, zero, 0, L );
+               active[k] = false;
+            }
+         }
+      }
+      if ( graph )
+         (void) hg_graph_destroy( ctx->get(), graph );
+   } catch ( const std::exception& e )
+   {
+      sv.fail( std::string( "P1 worker: " ) + e.what() );
+   }
+   sv.release( cls, bytes );
+}
+
+// ---- P2 class: one request at a time -----------------------------------------
+void p2Worker( Server& sv, int kind, int level, int64_t bytes )
+{
+   const std::pair< int, int > cls{ kind, level };
+   try
+   {
+      auto ctx = std::make_shared< Context >( sv.device );
+      std::shared_ptr< PrimitiveStorage > st;
+      if ( kind == P2Cube )
+      {
+         const double lo[3] = { 0, 0, 0 }, hi[3] = { 1, 1, 1 };
+         st                 = PrimitiveStorage::cuboid( ctx, lo, hi, 1, 1, 1 );
+      }
+      else
+      {
+         hg_handle h;
+         check( ctx->get(), hg_storage_shell( ctx->get(), 5, 3, 0.55, 1.0, &h ), "shell" );
+         st = std::make_shared< PrimitiveStorage >( ctx, h );
+      }
+      const uint_t         L = uint_t( level );
+      P2Function< double > x( "x", st, 0, L ), b( "b", st, 0, L ), t( "t", st, L, L ), t2( "t2", st, L, L ),
+          r( "r", st, L, L );
+      for ( ;; )
+      {
+         std::vector< Request > in;
+         if ( !sv.take( cls, 1, true, in ) )
+            break;
+         if ( in.empty() )
+            continue;
+         const Request&        rq = in[0];
+         std::vector< double > a, om, ph;
+         coefficientOf( rq.seed, a, om, ph );
+         P2ElementwiseBlendingDivKGradOperator A( st, 0, L, a, om, ph );
+         x.interpolate( 0.0, L, All );
+         b.interpolate( 0.0, L, All );
+         check( ctx->get(), hg_function_interpolate_random( ctx->get(), b.handle(), rq.seed, -1.0, 1.0, int( L ), Inner ),
+                "rhs" );
+         check( ctx->get(), hg_function_interpolate_random( ctx->get(), t.handle(), rq.seed + 1, -1.0, 1.0, int( L ), All ),
+                "start vector" );
+         const double rho      = chebyshev::estimateRadius( A, L, 15, t, t2 );
+         auto         smoother = std::make_shared< ChebyshevSmoother< decltype( A ) > >( st, 0, L );
+         smoother->setupCoefficients( 3, rho );
+         auto coarse = std::make_shared< CGSolver< decltype( A ) > >(
+             st, 0, 0, 10000, 1e-14, 1e-300, std::make_shared< InverseDiagonalPreconditioner< decltype( A ) > >() );
+         GeometricMultigridSolver< decltype( A ) > gmg( st, smoother, coarse,
+                                                       std::make_shared< P2toP2QuadraticRestriction<> >(),
+                                                       std::make_shared< P2toP2QuadraticProlongation<> >(), 0, L, 2, 2 );
+         auto resid = [&] {
+            A.apply( x, r, L, Inner, Replace );
+            r.assign( { 1.0, -1.0 }, { b, r }, L, Inner );
+            return std::sqrt( r.dotGlobal( r, L, Inner ) );
+         };
+         const double r0 = resid();
+         double       rk = r0;
+         int          k  = 0;
+         while ( k < 60 && rk > sv.tol * r0 )
+         {
+            gmg.solve( A, x, b, L );
+            rk = resid();
+            ++k;
+         }
+         sv.finish( rq, r0 > 0 ? rk / r0 : 0.0, x.dotGlobal( x, L, All ), k );
+      }
+   } catch ( const std::exception& e )
+   {
+      sv.fail( std::string( "P2 worker: " ) + e.what() );
+   }
+   sv.release( cls, bytes );
+}
+
+// starts a worker for every class with queued requests and no live worker,
+// as long as its hierarchy fits the memory budget (call with the lock held)
+void startWorkers( Server& sv )
+{
+   for ( auto& [cls, q] : sv.queues )
+   {
+      if ( q.empty() )
+         continue;
+      const int want = ( cls.first == P2Shell || cls.first == P2Cube ) ? 2 : 1;
+      while ( sv.live[cls] < want && ( sv.live[cls] == 0 || q.size() > size_t( sv.live[cls] ) ) )
+      {
+         const int64_t bytes = footprint( cls.first, cls.second, sv.slots );
+         if ( sv.memUsed + bytes > sv.memBudget && sv.memUsed > 0 )
+            break;
+         sv.memUsed += bytes;
+         sv.live[cls] += 1;
+         if ( cls.first == P1Cube || cls.first == P1Shell )
+            sv.threads.emplace_back( p1Worker, std::ref( sv ), cls.first, cls.second, bytes );
+         else
+            sv.threads.emplace_back( p2Worker, std::ref( sv ), cls.first, cls.second, bytes );
+      }
+   }
+}
+
+} // namespace
+
+extern "C" {
+
+const char* hg_serve_last_error( void )
+{
+   return tlsError.c_str();
+}
+
+int hg_serve_stream( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
+                     const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
+                     double* rel_residual, double* checksum, int* cycles, double* wall_s )
+{
+   tlsError.clear();
+   if ( n < 0 || !kind || !level || !seed || !arrival_s || !latency_s || !rel_residual || !checksum || !cycles ||
+        slots < 1 || slots > 64 || ( slots & ( slots - 1 ) ) || !( tol > 0 ) || !( memBudgetGB > 0 ) )
+   {
+      tlsError = "invalid arguments";
+      return HG_ERR_ARG;
+   }
+   for ( int64_t i = 0; i < n; ++i )
+   {
+      const bool p1 = kind[i] == P1Cube || kind[i] == P1Shell;
+      const bool p2 = kind[i] == P2Shell || kind[i] == P2Cube;
+      if ( ( !p1 && !p2 ) || level[i] < 1 || level[i] > ( p1 ? 8 : 6 ) )
+      {
+         tlsError = "request " + std::to_string( i ) + ": unknown kind or level out of range";
+         return HG_ERR_ARG;
+      }
+   }
+   Server sv;
+   sv.tol       = tol;
+   sv.slots     = slots;
+   sv.device    = device;
+   sv.memBudget = int64_t( memBudgetGB * 1e9 );
+   sv.res       = Results{ latency_s, rel_residual, checksum, cycles };
+   std::vector< int64_t > order( static_cast< size_t >( n ) );
+   for ( int64_t i = 0; i < n; ++i )
+      order[size_t( i )] = i;
+   std::stable_sort( order.begin(), order.end(), [&]( int64_t a, int64_t b ) { return arrival_s[a] < arrival_s[b]; } );
+   sv.t0 = Clock::now();
+   for ( int64_t i : order )
+   {
+      const double wait = arrival_s[i] - sv.now();
+      if ( wait > 0 )
+         std::this_thread::sleep_for( std::chrono::duration< double >( wait ) );
+      std::lock_guard< std::mutex > lk( sv.m );
+      if ( !sv.failure.empty() )
+         break;
+      sv.queues[{ kind[i], level[i] }].push_back( Request{ i, kind[i], level[i], seed[i], arrival_s[i] } );
+      startWorkers( sv );
+      sv.cv.notify_all();
+   }
+   // keep starting workers for classes that waited for memory until all queues drain
+   for ( ;; )
+   {
+      std::unique_lock< std::mutex > lk( sv.m );
+      sv.allReleased = true;
+      startWorkers( sv );
+      bool pending = false;
+      for ( auto& [c, q] : sv.queues )
+         pending |= !q.empty();
+      if ( ( !pending || !sv.failure.empty() ) )
+         break;
+      sv.cv.wait_for( lk, std::chrono::milliseconds( 20 ) );
+   }
+   {
+      std::lock_guard< std::mutex > lk( sv.m );
+      sv.cv.notify_all();
+   }
+   for ( size_t i = 0; i < sv.threads.size(); ++i )
+      sv.threads[i].join();
+   if ( wall_s )
+      *wall_s = sv.now();
+   if ( !sv.failure.empty() )
+   {
+      tlsError = sv.failure;
+      return HG_ERR_STATE;
+   }
+   return HG_OK;
+}
+
+} // extern "C"

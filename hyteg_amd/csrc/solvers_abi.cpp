@@ -132,6 +132,54 @@ int iterativeRefinement( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, in
    }
 }
 
+// Replays `body` (a V-cycle: kernels only) from a HIP graph captured on its
+// first use when the context allows graphs; runs it directly otherwise.
+class GraphedCycle
+{
+ public:
+   explicit GraphedCycle( hg_ctx* ctx )
+   : ctx_( ctx )
+   {
+      int en = 0;
+      check( ctx_, hg_ctx_graphs_enabled( ctx_, &en ), "graphs" );
+      enabled_ = en != 0;
+   }
+   ~GraphedCycle()
+   {
+      if ( graph_ )
+         (void) hg_graph_destroy( ctx_, graph_ );
+   }
+   template < class F >
+   void run( F&& body )
+   {
+      if ( !enabled_ )
+      {
+         body();
+         return;
+      }
+      if ( !graph_ )
+      {
+         check( ctx_, hg_graph_begin_capture( ctx_ ), "begin capture" );
+         try
+         {
+            body();
+         } catch ( ... )
+         {
+            hg_handle dummy = 0;
+            (void) hg_graph_end_capture( ctx_, &dummy );
+            throw;
+         }
+         check( ctx_, hg_graph_end_capture( ctx_, &graph_ ), "end capture" );
+      }
+      check( ctx_, hg_graph_launch( ctx_, graph_ ), "graph launch" );
+   }
+
+ private:
+   hg_ctx*   ctx_;
+   hg_handle graph_   = 0;
+   bool      enabled_ = true;
+};
+
 // ---- interleaved multi-vector P1 solves -------------------------------------
 // per-vector Inner residual norms of b - A x (r is scratch)
 template < class Op, class F >
@@ -199,6 +247,7 @@ int batchedIR( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel
    check( ctx, hg_p1_cell_stencil( ctx, op, maxLevel, 0, centre ), "stencil" );
    const DoFType         flag = Inner | NeumannBoundary | FreeslipBoundary;
    std::vector< double > r0;
+   GraphedCycle          cycleGraph( ctx );
    for ( int k = 0;; ++k )
    {
       A->apply( *X, r, top, flag, Replace );
@@ -221,8 +270,12 @@ int batchedIR( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel
       r.scaleBatched( inv, top, flag );
       blo.copyFrom( r, top );
       xlo.setToZero( top );
-      for ( int j = 0; j < innerCycles; ++j )
-         gmg->solve( *Alo, xlo, blo, top );
+      if ( k == 0 )
+         gmg->solve( *Alo, xlo, blo, top ); // first cycle eager: creates every lazily built level
+      else
+         cycleGraph.run( [&] { gmg->solve( *Alo, xlo, blo, top ); } );
+      for ( int j = 1; j < innerCycles; ++j )
+         cycleGraph.run( [&] { gmg->solve( *Alo, xlo, blo, top ); } );
       e.copyFrom( xlo, top );
       e.scaleBatched( scale, top, flag );
       X->add( { 1.0 }, { e }, top, flag );
@@ -247,10 +300,14 @@ int hg_p1_batched_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b
       P1Function< double >        r( "residual", storage, uint_t( maxLevel ), uint_t( maxLevel ), X->nvec() );
       const std::vector< double > r0 = residualNorms( *A, *X, *B, r, uint_t( maxLevel ) );
       storeHist( res_hist, 0, r0 );
-      int k = 0;
+      int          k = 0;
+      GraphedCycle cycleGraph( ctx );
       while ( k < cycles )
       {
-         gmg->solve( *A, *X, *B, uint_t( maxLevel ) );
+         if ( k == 0 )
+            gmg->solve( *A, *X, *B, uint_t( maxLevel ) ); // eager: creates every lazily built level
+         else
+            cycleGraph.run( [&] { gmg->solve( *A, *X, *B, uint_t( maxLevel ) ); } );
          ++k;
          const std::vector< double > rk = residualNorms( *A, *X, *B, r, uint_t( maxLevel ) );
          storeHist( res_hist, k, rk );

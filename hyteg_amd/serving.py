@@ -1,0 +1,79 @@
+"""Serving-stream driver: seeded Poisson-arrival request streams for
+hg_serve_stream and their latency / throughput summary.
+
+    python -m hyteg_amd.serving [--rates 10,50,200,500] [--requests 2000] [--tol 1e-8]
+
+Request mix (kind, levels, share): P1 Poisson on the unit cube, levels 3-7,
+40 %; P1 Poisson on the shell macro mesh, levels 3-5, 25 %; P2 div(k grad) on
+the cube, levels 3-5, 20 %; P2 div(k grad) on the blended shell, levels 2-3,
+15 %.  Coefficients and right-hand sides come from each request's seed.
+"""
+import argparse
+import ctypes as C
+import json
+
+import numpy as np
+
+from ._lib import HygError, lib
+
+P1_CUBE, P1_SHELL, P2_SHELL, P2_CUBE = 0, 1, 2, 3
+MIX = [  # (kind, levels, level weights, share)
+    (P1_CUBE, (3, 4, 5, 6, 7), (0.25, 0.25, 0.25, 0.15, 0.10), 0.40),
+    (P1_SHELL, (3, 4, 5), (0.4, 0.4, 0.2), 0.25),
+    (P2_CUBE, (3, 4, 5), (0.4, 0.4, 0.2), 0.20),
+    (P2_SHELL, (2, 3), (0.6, 0.4), 0.15),
+]
+
+
+def make_stream(n, rate, seed=0):
+    """n requests, exponential inter-arrival times of mean 1/rate (seeded)"""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    arrivals -= arrivals[0]
+    shares = np.array([m[3] for m in MIX])
+    which = rng.choice(len(MIX), n, p=shares / shares.sum())
+    kinds = np.array([MIX[w][0] for w in which], dtype=np.int32)
+    levels = np.array([rng.choice(MIX[w][1], p=np.array(MIX[w][2]) / sum(MIX[w][2])) for w in which], dtype=np.int32)
+    seeds = rng.integers(0, 2 ** 62, n, dtype=np.uint64)
+    return kinds, levels, seeds, arrivals
+
+
+def serve_stream(kinds, levels, seeds, arrivals, tol=1e-8, slots=32, mem_budget_gb=200.0, device=0):
+    n = len(kinds)
+    kinds = np.ascontiguousarray(kinds, dtype=np.int32)
+    levels = np.ascontiguousarray(levels, dtype=np.int32)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    arrivals = np.ascontiguousarray(arrivals, dtype=np.float64)
+    lat, rel, chk = np.zeros(n), np.zeros(n), np.zeros(n)
+    cyc = np.zeros(n, dtype=np.int32)
+    wall = C.c_double()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    st = lib().hg_serve_stream(device, n, p(kinds), p(levels), p(seeds), p(arrivals), tol, slots, mem_budget_gb,
+                               p(lat), p(rel), p(chk), p(cyc), C.byref(wall))
+    if st != 0:
+        raise HygError(f"hg_serve_stream: status {st}: {lib().hg_serve_last_error().decode()}")
+    return {"latency_s": lat, "rel_residual": rel, "checksum": chk, "cycles": cyc, "wall_s": wall.value}
+
+
+def summary(out, n):
+    lat = out["latency_s"]
+    return {"requests": n, "wall_s": out["wall_s"], "solves_per_s": n / out["wall_s"],
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "max_rel_residual": float(out["rel_residual"].max())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rates", default="10,50,200,500")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--slots", type=int, default=32)
+    args = ap.parse_args()
+    for rate in [float(r) for r in args.rates.split(",")]:
+        stream = make_stream(args.requests, rate, seed=0)
+        out = serve_stream(*stream, tol=args.tol, slots=args.slots)
+        print(json.dumps({"rate_per_s": rate, **summary(out, args.requests)}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

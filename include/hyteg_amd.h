@@ -63,6 +63,18 @@ int         hg_ctx_create( int device, hg_ctx** out );
 int         hg_ctx_destroy( hg_ctx* ctx );
 const char* hg_last_error( const hg_ctx* ctx );
 int         hg_ctx_synchronize( hg_ctx* ctx );
+/* HIP graph capture of the work a context enqueues (no counterpart in the
+ * reference): everything called between begin and end on this host thread is
+ * recorded instead of run (calls that return scalars to the host cannot be
+ * captured); the instantiated graph replays it with one launch.  The batched
+ * multigrid solvers capture their V-cycle this way unless disabled with
+ * hg_ctx_set_graphs(ctx, 0). */
+int hg_graph_begin_capture( hg_ctx* ctx );
+int hg_graph_end_capture( hg_ctx* ctx, hg_handle* graph );
+int hg_graph_launch( hg_ctx* ctx, hg_handle graph );
+int hg_graph_destroy( hg_ctx* ctx, hg_handle graph );
+int hg_ctx_set_graphs( hg_ctx* ctx, int enable );
+int hg_ctx_graphs_enabled( hg_ctx* ctx, int* enabled );
 /* bytes of device memory held by the context */
 int hg_ctx_memory( hg_ctx* ctx, int64_t* bytes );
 
@@ -115,6 +127,8 @@ int hg_p1_function_create_batched( hg_ctx* ctx, hg_handle storage, int minLevel,
                                    hg_handle* out );
 /* number of interleaved vectors of a function */
 int hg_function_nvec( hg_ctx* ctx, hg_handle fn, int* nvec );
+/* vector kdst of dst = vector ksrc of src (all DoFs of `level`, halos included) */
+int hg_function_copy_vector( hg_ctx* ctx, hg_handle dst, int kdst, hg_handle src, int ksrc, int level );
 /* P2Function<T>(name, storage, minLevel, maxLevel) (p2functionspace/P2Function.hpp:35-74) */
 int hg_p2_function_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, hg_handle* out );
 int hg_function_destroy( hg_ctx* ctx, hg_handle fn );
@@ -273,6 +287,24 @@ int hg_p1_batched_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b
 int hg_p1_batched_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel,
                             int lowDtype, int pre, int post, int coarseSweeps, int innerCycles, int maxOuter, double tol,
                             double* res_hist, int* outer_done );
+
+/* ---- serving --------------------------------------------------------------- */
+/* Runs a stream of n solve requests on `device` with continuous batching (no
+ * counterpart in the reference).  Request i of kind[i] (0: P1 Poisson on the unit
+ * cube, 1: P1 Poisson on the shell macro mesh, 2: P2 div(k grad) on the blended
+ * shell, 3: P2 div(k grad) on the cube; k and the RHS generated from seed[i]) at
+ * level[i] arrives arrival_s[i] seconds after the call starts and is solved to
+ * relative residual `tol`.  P1 requests of one (kind, level) share an interleaved
+ * hierarchy of `slots` vectors (a power of two up to 64) and join / leave it at
+ * V-cycle boundaries; every class has its own stream; a class hierarchy is built
+ * only while the estimated device memory of the live ones stays below
+ * memBudgetGB.  Outputs per request: latency (arrival to completion), reached
+ * relative residual, checksum x.x of the solution, V-cycles; wall_s = total time. */
+int hg_serve_stream( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
+                     const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
+                     double* rel_residual, double* checksum, int* cycles, double* wall_s );
+/* message of the last hg_serve_stream failure on this thread */
+const char* hg_serve_last_error( void );
 
 #ifdef __cplusplus
 }

@@ -217,6 +217,12 @@ class LatticeFunction
       check( ctx_, hg_function_dot_batched( ctx_, h_, other.h_, int( level ), flag, r.data() ), "dotGlobalBatched" );
       return r;
    }
+   // vector k of this function = vector ksrc of other
+   template < typename OtherT >
+   void copyVector( int k, const LatticeFunction< OtherT, Space >& other, int ksrc, uint_t level ) const
+   {
+      check( ctx_, hg_function_copy_vector( ctx_, h_, k, other.handle(), ksrc, int( level ) ), "copyVector" );
+   }
    // v_k *= scales[k] for every interleaved vector k
    void scaleBatched( const std::vector< double >& scales, uint_t level, DoFType flag = All ) const
    {

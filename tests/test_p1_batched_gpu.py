@@ -134,3 +134,23 @@ def test_batched_gmg_and_fp16_refinement_reach_single_solutions(ctx):
     print(f"batched: fp64 {k64} cycles, fp16 refinement {kir} steps", flush=True)
     assert (hir[-1] <= 1e-10 * hir[0]).all()
     assert relmax(xb.download(L), ref) < 1e-8
+
+
+def test_graph_captured_cycles_equal_eager_cycles(ctx):
+    """the V-cycle replayed from a captured HIP graph gives the eager results bit for bit"""
+    st = hg.PrimitiveStorage.cuboid(ctx)
+    L = 4
+    A = hg.P1ConstantLaplaceOperator(st, 0, L)
+    out = []
+    for graphs in (True, False):
+        ctx.set_graphs(graphs)
+        x = hg.P1Function("x", st, 0, L, nvec=4)
+        b = hg.P1Function("b", st, 0, L, nvec=4)
+        b.interpolate_random(3, -1.0, 1.0, L, hg.INNER)
+        k1, h1 = hg.batched_gmg_solve(A, x, b, 0, L, cycles=6)
+        x2 = hg.P1Function("x2", st, 0, L, nvec=4)
+        k2, h2 = hg.batched_ir_solve(A, x2, b, 0, L, low_dtype=hg.F16, max_outer=5, tol=0.0)
+        out.append((h1, x.download(L), h2, x2.download(L)))
+    ctx.set_graphs(True)
+    for a, b_ in zip(out[0], out[1]):
+        assert np.array_equal(a, b_)
