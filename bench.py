@@ -9,7 +9,8 @@ spherical shell r in [0.55, 1] (ntan 5, 3 radial layers, 1920 macro-cells) at
 level 6: 673,710,594 DoFs, inputs resident in HBM.  Prints ONE JSON line with
 DoF/s per apply, the roofline of the dominant kernel (HIP-event timed on the
 library's stream), the CPU oracle timed on this host, and secondary results
-(P1 apply on the cube, FP64 GMG time-to-1e-8 on the cube at level 5).
+(P1 apply on the cube, FP64 and mixed-precision GMG time-to-1e-8, 64 batched
+P1 solves, and a mixed request stream through the serving engine).
 """
 import argparse
 import json
@@ -273,6 +274,19 @@ def batched_cfg4(hg, ctx, L=6, nvec=64):
             "speedup": t_seq / t_ir, "max_rel_error_vs_fp64": max(errs)}
 
 
+def serving_cfg5(n=400, rate=200.0):
+    """seeded Poisson stream of mixed P1/P2 solve requests (to 1e-8) through the
+    continuous-batching server; latency = completion - arrival"""
+    from hyteg_amd import serving
+    stream = serving.make_stream(n, rate, seed=0)
+    out = serving.serve_stream(*stream, tol=1e-8, slots=32)
+    res = serving.summary(out, n, stream[0])
+    log(f"serving {n} requests at {rate:g}/s: {res['solves_per_s']:.1f} solves/s, p50 {res['p50_latency_s'] * 1e3:.1f} ms, "
+        f"p99 {res['p99_latency_s'] * 1e3:.1f} ms")
+    return {"workload": f"{n} requests, Poisson arrivals at {rate:g}/s, mix P1 cube L3-7 40%, P1 shell L3-5 25%, "
+                        "P2 cube L3-5 20%, P2 shell L2-3 15%, each solved to 1e-8", "rate_per_s": rate, **res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +298,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gmg", action="store_true", help="skip the P2 GMG solve")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 64-system P1 solve")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
     ap.add_argument("--only-apply", action="store_true",
                     help="run the apply measurement only (for counter collection)")
     args = ap.parse_args()
@@ -311,6 +326,8 @@ def main():
         extra["gmg_p1_cube_L5_f64"] = gmg_cfg1(hg, ctx)
         if args.workload == "p2_shell" and not args.no_batched:
             extra["batched_p1_cube_L6_x64"] = batched_cfg4(hg, ctx)
+        if args.workload == "p2_shell" and not args.no_serving:
+            extra["serving_mixed_stream"] = serving_cfg5()
     cpu = None if (args.no_cpu or args.only_apply) else cpu_baseline(args.workload, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": n / sec, "unit": "DoF/s", "n_gpus": 1, "steps": args.steps,

@@ -125,6 +125,7 @@ struct Server
    std::map< std::pair< int, int >, int >                live; // workers per class
    int64_t                                               memBudget = 0, memUsed = 0;
    bool                                                  allReleased = false;
+   bool                                                  starved     = false; // a queued class waits for memory
    Clock::time_point                                     t0;
    Results                                               res{};
    double                                                tol     = 1e-8;
@@ -144,19 +145,23 @@ struct Server
    }
 
    // pops up to `max` requests of a class; waits (bounded) when none is queued.
-   // Returns false when the worker should exit (idle and everything released).
+   // An idle worker keeps its hierarchy resident while requests may still
+   // arrive; it returns false (exit) once the stream is fully released or
+   // another class waits for the memory it holds.
    bool take( std::pair< int, int > cls, size_t max, bool mayWait, std::vector< Request >& out )
    {
       std::unique_lock< std::mutex > lk( m );
       auto&                          q = queues[cls];
       if ( q.empty() && mayWait )
-         cv.wait_for( lk, std::chrono::milliseconds( 200 ), [&] { return !q.empty() || allReleased; } );
+         cv.wait_for( lk, std::chrono::milliseconds( 200 ),
+                      [&] { return !q.empty() || allReleased || starved || !failure.empty(); } );
       while ( !q.empty() && out.size() < max )
       {
          out.push_back( q.front() );
          q.pop_front();
       }
-      return !( q.empty() && out.empty() && mayWait );
+      const bool idle = q.empty() && out.empty() && mayWait;
+      return !( idle && ( allReleased || starved || !failure.empty() ) );
    }
 
    void release( std::pair< int, int > cls, int64_t bytes )
@@ -191,9 +196,18 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
       }
       else
       {
-         hg_handle h;
+         // the constant-stencil P1 operator works on the affine (unblended) shell mesh
+         hg_handle h, a;
+         int64_t   cnt[4];
          check( ctx->get(), hg_storage_shell( ctx->get(), 5, 3, 0.55, 1.0, &h ), "shell" );
-         st = std::make_shared< PrimitiveStorage >( ctx, h );
+         check( ctx->get(), hg_storage_counts( ctx->get(), h, cnt ), "shell" );
+         std::vector< double >  X( static_cast< size_t >( 3 * cnt[0] ) );
+         std::vector< int64_t > cells( static_cast< size_t >( 4 * cnt[3] ) );
+         check( ctx->get(), hg_storage_vertex_coords( ctx->get(), h, X.data() ), "shell" );
+         check( ctx->get(), hg_storage_cell_vertices( ctx->get(), h, cells.data() ), "shell" );
+         check( ctx->get(), hg_storage_create( ctx->get(), int( cnt[0] ), X.data(), int( cnt[3] ), cells.data(), &a ),
+                "shell" );
+         st = std::make_shared< PrimitiveStorage >( ctx, a );
       }
       const int    S = sv.slots;
       const uint_t L = uint_t( level );
@@ -371,6 +385,7 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
 // as long as its hierarchy fits the memory budget (call with the lock held)
 void startWorkers( Server& sv )
 {
+   sv.starved = false;
    for ( auto& [cls, q] : sv.queues )
    {
       if ( q.empty() )
@@ -380,7 +395,14 @@ void startWorkers( Server& sv )
       {
          const int64_t bytes = footprint( cls.first, cls.second, sv.slots );
          if ( sv.memUsed + bytes > sv.memBudget && sv.memUsed > 0 )
+         {
+            if ( sv.live[cls] == 0 )
+            {
+               sv.starved = true; // idle workers of other classes give their memory back
+               sv.cv.notify_all();
+            }
             break;
+         }
          sv.memUsed += bytes;
          sv.live[cls] += 1;
          if ( cls.first == P1Cube || cls.first == P1Shell )

@@ -55,11 +55,21 @@ def serve_stream(kinds, levels, seeds, arrivals, tol=1e-8, slots=32, mem_budget_
     return {"latency_s": lat, "rel_residual": rel, "checksum": chk, "cycles": cyc, "wall_s": wall.value}
 
 
-def summary(out, n):
+KIND_NAMES = {P1_CUBE: "p1_cube", P1_SHELL: "p1_shell", P2_SHELL: "p2_shell", P2_CUBE: "p2_cube"}
+
+
+def summary(out, n, kinds=None):
+    """throughput and latency percentiles, overall and (given kinds) per request kind"""
     lat = out["latency_s"]
-    return {"requests": n, "wall_s": out["wall_s"], "solves_per_s": n / out["wall_s"],
-            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "max_rel_residual": float(out["rel_residual"].max())}
+    res = {"requests": n, "wall_s": out["wall_s"], "solves_per_s": n / out["wall_s"],
+           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+           "max_rel_residual": float(out["rel_residual"].max())}
+    if kinds is not None:
+        res["by_kind"] = {KIND_NAMES[k]: {"requests": int((kinds == k).sum()),
+                                          "p50_latency_s": float(np.percentile(lat[kinds == k], 50)),
+                                          "p99_latency_s": float(np.percentile(lat[kinds == k], 99))}
+                          for k in sorted(set(int(v) for v in kinds))}
+    return res
 
 
 def main():
@@ -72,7 +82,7 @@ def main():
     for rate in [float(r) for r in args.rates.split(",")]:
         stream = make_stream(args.requests, rate, seed=0)
         out = serve_stream(*stream, tol=args.tol, slots=args.slots)
-        print(json.dumps({"rate_per_s": rate, **summary(out, args.requests)}), flush=True)
+        print(json.dumps({"rate_per_s": rate, **summary(out, args.requests, stream[0])}), flush=True)
 
 
 if __name__ == "__main__":

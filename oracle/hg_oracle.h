@@ -133,6 +133,23 @@ int or_p2_gmg( const or_mesh* m, int minLevel, int maxLevel, int nm, const doubl
                const double* kph, double* x, const double* b, int pre, int post, int order, double rho, int cycles,
                double tol, double* res_hist, int coarseMaxIter, double coarseTol, int coarseJacobi );
 
+/* ---- P2-P1 Stokes (Taylor-Hood) ------------------------------------------------ */
+typedef struct or_stokesop or_stokesop;
+/* element matrices of the blended full-viscous operator, eta = exp(c0 + c1 |F(x)|):
+ * A 30x30 (index 10 c + i: component c, P2 function i; may be NULL) and
+ * B 4x30 = -(psi_i, div u) (may be NULL); p = 4 computational vertices, blend may be NULL */
+void         or_stokes_element( const double* p, const double* blend, double c0, double c1, double* A, double* B );
+or_stokesop* or_stokesop_create( const or_mesh* m, int level, double c0, double c1 );
+void         or_stokesop_free( or_stokesop* op );
+/* mask bit 1: yu = A u, 2: yu += B^T p, 4: yp = B u; u = [ux|uy|uz] (level+1 numbering),
+ * p on level; add: UpdateType Add */
+void or_stokes_apply( const or_stokesop* op, const double* u, const double* p, double* yu, double* yp, int flag, int mask,
+                      int add );
+void or_stokes_inv_diag( const or_stokesop* op, double* out );
+/* P1PSPGInvDiagOperator and P1LumpedInvMassOperator (affine geometry) of one level */
+void or_p1_pspg_inv_diag( const or_mesh* m, int level, double* out );
+void or_p1_lumped_inv_mass( const or_mesh* m, int level, double* out );
+
 #ifdef __cplusplus
 }
 #endif

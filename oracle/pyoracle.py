@@ -74,6 +74,13 @@ def lib():
             "or_p2_power_iteration": (_d, [_p, _p, _p, _i]),
             "or_p2_cg": (_i, [_p, _p, _p, _i, _i, _d, _d, _i]),
             "or_p2_gmg": (_i, [_p, _i, _i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _d, _i, _d, _p, _i, _d, _i]),
+            "or_stokes_element": (None, [_p, _p, _d, _d, _p, _p]),
+            "or_stokesop_create": (_p, [_p, _i, _d, _d]),
+            "or_stokesop_free": (None, [_p]),
+            "or_stokes_apply": (None, [_p, _p, _p, _p, _p, _i, _i, _i]),
+            "or_stokes_inv_diag": (None, [_p, _p]),
+            "or_p1_pspg_inv_diag": (None, [_p, _i, _p]),
+            "or_p1_lumped_inv_mass": (None, [_p, _i, _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -310,3 +317,265 @@ def chebyshev_coefficients(order, lower, upper):
 
 def dot(a, b, flags, flag):
     return float(lib().or_dot(len(a), _ptr(a), _ptr(b), _ptr(flags), flag))
+
+
+# ---- P2-P1 Stokes ------------------------------------------------------------------
+def stokes_element(p, blend=None, c0=0.0, c1=0.0):
+    """(A 30x30, B 4x30) of the blended full-viscous Stokes element, eta = exp(c0 + c1 |F(x)|)"""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    A = np.zeros((30, 30))
+    B = np.zeros((4, 30))
+    b = None if blend is None else np.ascontiguousarray(blend, dtype=np.float64)
+    lib().or_stokes_element(_ptr(p), None if b is None else _ptr(b), c0, c1, _ptr(A), _ptr(B))
+    return A, B
+
+
+def p1_pspg_inv_diag(mesh, level):
+    out = np.zeros(mesh.ndofs(level))
+    lib().or_p1_pspg_inv_diag(mesh.h, level, _ptr(out))
+    return out
+
+
+def p1_lumped_inv_mass(mesh, level):
+    out = np.zeros(mesh.ndofs(level))
+    lib().or_p1_lumped_inv_mass(mesh.h, level, _ptr(out))
+    return out
+
+
+class StokesOp:
+    """P2P1ElementwiseBlendingFullViscousStokesOperator on P2 level `level`: velocity
+    [ux|uy|uz] in the P1 numbering of level+1, pressure in that of level (all Inner)"""
+
+    A_U, BT_P, B_U = 1, 2, 4
+
+    def __init__(self, mesh, level, c0=0.0, c1=0.0):
+        self.mesh = mesh
+        self.level = level
+        self.h = lib().or_stokesop_create(mesh.h, level, c0, c1)
+        self.n2 = mesh.ndofs(level + 1)
+        self.n1 = mesh.ndofs(level)
+        self.fl2 = np.tile(mesh.flags(level + 1), 3)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.or_stokesop_free(self.h)
+            self.h = None
+
+    def apply(self, u, p, yu, yp, flag, mask=7, add=False):
+        lib().or_stokes_apply(self.h, None if u is None else _ptr(u), None if p is None else _ptr(p),
+                              None if yu is None else _ptr(yu), None if yp is None else _ptr(yp), flag, mask, int(add))
+
+    def inv_diag(self):
+        out = np.zeros(3 * self.n2)
+        lib().or_stokes_inv_diag(self.h, _ptr(out))
+        return out
+
+    def vmask(self, flag):
+        return (self.fl2 & flag) != 0
+
+    @staticmethod
+    def pmask(p, flag):
+        return np.full(len(p), bool(flag & INNER))
+
+
+def project_mean(p):
+    """vertexdof::projectMean: subtract the arithmetic mean over all DoFs"""
+    p -= p.sum() / len(p)
+
+
+def stokes_chebyshev(op, inv_diag, coeffs, u, bu, flag=GMG_FLAG):
+    """ChebyshevSmoother::solve on the velocity operator (order = len(coeffs))"""
+    m = op.vmask(flag)
+    t2 = np.zeros_like(u)
+    op.apply(u, None, t2, None, flag, StokesOp.A_U)
+    t2[m] = bu[m] - t2[m]
+    t1 = np.zeros_like(u)
+    t1[m] = inv_diag[m] * t2[m]
+    u[m] += coeffs[0] * t1[m]
+    for k in range(1, len(coeffs)):
+        op.apply(t1, None, t2, None, flag, StokesOp.A_U)
+        t1[:] = 0.0
+        t1[m] = inv_diag[m] * t2[m]
+        u[m] += coeffs[k] * t1[m]
+
+
+def stokes_power_iteration(op, inv_diag, x, its):
+    """chebyshev::estimateRadius on the velocity operator: power iteration on D^-1 A (flag All)"""
+    x /= np.sqrt(x @ x)
+    aux = np.zeros_like(x)
+    op.apply(x, None, aux, None, ALL, StokesOp.A_U)
+    aux *= inv_diag
+    radius = 0.0
+    for _ in range(its):
+        x[:] = aux / np.sqrt(aux @ aux)
+        op.apply(x, None, aux, None, ALL, StokesOp.A_U)
+        aux *= inv_diag
+        radius = float(x @ aux)
+    return radius
+
+
+def uzawa_smooth(op, u, p, bu, bp, inv_diag, coeffs, pspg_inv, omega, vel_iter=1, flag=GMG_FLAG, project=True):
+    """UzawaSmoother (block variant without stabilisation, UzawaSmoother.hpp:330-470) with a
+    Chebyshev velocity smoother"""
+    m = op.vmask(flag)
+    ru = np.zeros_like(u)
+    op.apply(None, p, ru, None, flag, StokesOp.BT_P)
+    ru[m] = bu[m] - ru[m]
+    for _ in range(vel_iter):
+        stokes_chebyshev(op, inv_diag, coeffs, u, ru, flag)
+    if not flag & INNER:
+        return
+    rp = np.zeros_like(p)
+    op.apply(u, None, None, rp, flag, StokesOp.B_U)
+    rp = omega * (bp - rp)
+    if project:
+        project_mean(rp)
+    p += pspg_inv * rp
+    if project:
+        project_mean(p)
+
+
+def stokes_residual(op, u, p, bu, bp, flag):
+    ru = np.zeros_like(u)
+    rp = np.zeros_like(p)
+    op.apply(u, p, ru, rp, flag, 7)
+    m = op.vmask(flag)
+    ru[~m] = 0.0
+    ru[m] = bu[m] - ru[m]
+    rp = bp - rp if flag & INNER else np.zeros_like(p)
+    return ru, rp
+
+
+def stokes_minres(op, u, p, bu, bp, minv, max_iter, rel_tol, abs_tol=0.0, flag=GMG_FLAG):
+    """MinResSolver (MinresSolver.hpp:117-245) with StokesPressureBlockPreconditioner<P1LumpedInvMass>;
+    returns the iteration index at exit"""
+    n2 = len(u)
+    m = np.concatenate([op.vmask(flag), np.full(len(p), bool(flag & INNER))])
+    A = lambda z: np.concatenate(stokes_apply_all(op, z[:n2], z[n2:], flag))  # noqa: E731
+
+    def prec(v):
+        z = np.zeros_like(v)
+        z[m] = v[m]
+        if flag & INNER:
+            z[n2:] = minv * v[n2:]
+        return z
+
+    x = np.concatenate([u, p])
+    b = np.concatenate([bu, bp])
+    v = np.zeros_like(x)
+    v[m] = b[m] - A(x)[m]
+    vm = np.zeros_like(x)
+    wm = np.zeros_like(x)
+    w = np.zeros_like(x)
+    z = prec(v)
+    gamma_old, gamma_new = 1.0, np.sqrt(z[m] @ v[m])
+    res_start = eta = gamma_new
+    s_old = s_new = 0.0
+    c_old = c_new = 1.0
+    it = max_iter
+    if gamma_new < abs_tol:
+        return 0
+    for i in range(max_iter):
+        z[m] /= gamma_new
+        vp = np.zeros_like(x)
+        vp[m] = A(z)[m]
+        delta = vp[m] @ z[m]
+        vp[m] = vp[m] - (delta / gamma_new) * v[m] - (gamma_new / gamma_old) * vm[m]
+        zp = prec(vp)
+        gamma_old, gamma_new = gamma_new, np.sqrt(zp[m] @ vp[m])
+        a0 = c_new * delta - c_old * s_new * gamma_old
+        a1 = np.sqrt(a0 * a0 + gamma_new * gamma_new)
+        a2 = s_new * delta + c_old * c_new * gamma_old
+        a3 = s_old * gamma_old
+        c_old, c_new = c_new, a0 / a1
+        s_old, s_new = s_new, gamma_new / a1
+        wp = np.zeros_like(x)
+        wp[m] = z[m] / a1 - (a3 / a1) * wm[m] - (a2 / a1) * w[m]
+        x[m] += c_new * eta * wp[m]
+        eta = -s_new * eta
+        vm, v = v, vp
+        wm, w = w, wp
+        z = zp
+        if abs(eta) / res_start < rel_tol or abs(eta) < abs_tol:
+            it = i
+            break
+    u[:] = x[:n2]
+    p[:] = x[n2:]
+    return it
+
+
+def stokes_apply_all(op, u, p, flag):
+    yu = np.zeros_like(u)
+    yp = np.zeros_like(p)
+    op.apply(u, p, yu, yp, flag, 7)
+    return yu, yp
+
+
+class StokesGMG:
+    """GeometricMultigridSolver<P2P1 Stokes> with the Uzawa smoother (Chebyshev velocity
+    smoother, PSPG inverse diagonal), P2P1StokesToP2P1Stokes restriction / prolongation and
+    pressure-preconditioned MINRES on the coarsest level (stokesGMGUzawaFSSolver's layout,
+    StokesFSGMGUzawaSolverTemplate.hpp:63-190, without free-slip projections)"""
+
+    def __init__(self, mesh, min_level, max_level, c0=0.0, c1=0.0, order=3, rho=None, omega=0.3, pre=3, post=3,
+                 inc=2, vel_iter=1, coarse_iter=10, coarse_tol=1e-6, power_its=25, seed=42, project=True):
+        self.m, self.minL, self.maxL = mesh, min_level, max_level
+        self.pre, self.post, self.inc, self.omega, self.vel_iter = pre, post, inc, omega, vel_iter
+        self.coarse_iter, self.coarse_tol, self.project = coarse_iter, coarse_tol, project
+        self.ops = {L: StokesOp(mesh, L, c0, c1) for L in range(min_level, max_level + 1)}
+        self.invd = {L: self.ops[L].inv_diag() for L in self.ops}
+        self.pspg = {L: p1_pspg_inv_diag(mesh, L) for L in self.ops}
+        self.minv = p1_lumped_inv_mass(mesh, min_level)
+        if rho is None:
+            f = self.ops[max_level]
+            x = np.random.default_rng(seed).uniform(-1, 1, 3 * f.n2)
+            rho = stokes_power_iteration(f, self.invd[max_level], x, power_its)
+        self.rho = rho
+        self.coeffs = chebyshev_coefficients(order, 0.3 * rho, 1.2 * rho)
+
+    def smooth(self, L, u, p, bu, bp):
+        uzawa_smooth(self.ops[L], u, p, bu, bp, self.invd[L], self.coeffs, self.pspg[L], self.omega, self.vel_iter,
+                     GMG_FLAG, self.project)
+
+    def cycle(self, L, u, p, bu, bp):
+        op = self.ops[L]
+        if L == self.minL:
+            stokes_minres(op, u, p, bu, bp, self.minv, self.coarse_iter, self.coarse_tol, 0.0, GMG_FLAG)
+            if self.project:
+                project_mean(p)
+            return
+        for _ in range(self.pre + self.inc * (self.maxL - L)):
+            self.smooth(L, u, p, bu, bp)
+        ru, rp = stokes_residual(op, u, p, bu, bp, GMG_FLAG)
+        c = self.ops[L - 1]
+        cu = np.zeros(3 * c.n2)
+        for k in range(3):
+            coarse = np.zeros(c.n2)
+            self.m.p2_restrict(L, np.ascontiguousarray(ru[k * op.n2:(k + 1) * op.n2]), coarse, GMG_FLAG)
+            cu[k * c.n2:(k + 1) * c.n2] = coarse
+        cp = np.zeros(c.n1)
+        self.m.restrict(L, rp, cp, ALL)
+        xu, xp = np.zeros_like(cu), np.zeros_like(cp)
+        self.cycle(L - 1, xu, xp, cu, cp)
+        for k in range(3):
+            fine = np.ascontiguousarray(u[k * op.n2:(k + 1) * op.n2])
+            self.m.p2_prolongate(L - 1, np.ascontiguousarray(xu[k * c.n2:(k + 1) * c.n2]), fine, GMG_FLAG, True)
+            u[k * op.n2:(k + 1) * op.n2] = fine
+        self.m.prolongate(L - 1, xp, p, ALL, True)
+        for _ in range(self.post + self.inc * (self.maxL - L)):
+            self.smooth(L, u, p, bu, bp)
+
+    def residual_norm(self, u, p, bu, bp):
+        ru, rp = stokes_residual(self.ops[self.maxL], u, p, bu, bp, INNER | NEUMANN | FREESLIP)
+        return float(np.sqrt(ru @ ru + rp @ rp))
+
+    def solve(self, u, p, bu, bp, cycles=20, tol=1e-7):
+        hist = [self.residual_norm(u, p, bu, bp)]
+        k = 0
+        while k < cycles and hist[-1] > tol * hist[0]:
+            self.cycle(self.maxL, u, p, bu, bp)
+            if self.project:
+                project_mean(p)
+            hist.append(self.residual_norm(u, p, bu, bp))
+            k += 1
+        return k, np.array(hist)
