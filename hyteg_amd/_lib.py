@@ -70,6 +70,12 @@ SIGNATURES = {
     "hg_p1_function_create_batched": [_p, _i64, _i, _i, _i, _i, _p],
     "hg_function_nvec": [_p, _i64, _p],
     "hg_serve_stream": [_i, _i64, _p, _p, _p, _p, _d, _i, _d, _p, _p, _p, _p, _p],
+    "hg_function_sum": [_p, _i64, _i, _i, _p],
+    "hg_function_add_const": [_p, _i64, _d, _i, _i],
+    "hg_stokes_create": [_p, _i64, _i, _i, _d, _d, _p],
+    "hg_stokes_apply": [_p, _i64, _p, _p, _i, _i, _i, _i],
+    "hg_stokes_velocity_inverse_diagonal": [_p, _i64, _i, _p],
+    "hg_stokes_pressure_diagonal": [_p, _i64, _i, _i, _i64],
     "hg_function_copy_vector": [_p, _i64, _i, _i64, _i, _i],
     "hg_function_dot_batched": [_p, _i64, _i64, _i, _i, _p],
     "hg_function_scale_batched": [_p, _i64, _p, _i, _i],

@@ -218,6 +218,15 @@ class P1Function:
     def mult(self, a, b, level, flag=ALL):
         self.ctx.check(lib().hg_function_mult(self.ctx.h, self.h, a.h, b.h, level, flag), "multElementwise")
 
+    def sum(self, level, flag=ALL):
+        """sumGlobal (FP64 single-vector functions)"""
+        out = C.c_double()
+        self.ctx.check(lib().hg_function_sum(self.ctx.h, self.h, level, flag, C.byref(out)), "sum")
+        return out.value
+
+    def add_const(self, value, level, flag=ALL):
+        self.ctx.check(lib().hg_function_add_const(self.ctx.h, self.h, float(value), level, flag), "add")
+
 
 class P2Function(P1Function):
     """P2 DoFs of level L, numbered like the P1 DoFs of level L+1"""
@@ -383,3 +392,104 @@ def gmg_solve(A, x, b, min_level, max_level, pre=3, post=3, cycles=10, tol=0.0, 
     A.ctx.check(lib().hg_p1_gmg_solve(A.ctx.h, A.h, x.h, b.h, min_level, max_level, pre, post, cycles, tol,
                                       coarse_max_iter, coarse_tol, _ptr(hist), C.byref(done)), "gmg")
     return done.value, hist[: done.value + 1]
+
+
+def _pressure_flag(flag):
+    """the pressure of a Taylor-Hood function has no boundary DoFs (createAllInnerBC)"""
+    return ALL if flag & INNER else 0
+
+
+class P2P1TaylorHoodFunction:
+    """velocity uvw (three P2 functions) and pressure p (P1, every DoF Inner)"""
+
+    def __init__(self, name, storage, min_level, max_level):
+        self.storage = storage
+        self.ctx = storage.ctx
+        self.uvw = [P2Function(f"{name}_{c}", storage, min_level, max_level) for c in "uvw"]
+        self.p = P1Function(f"{name}_p", storage, min_level, max_level)
+
+    def handles(self):
+        return np.ascontiguousarray([f.h for f in self.uvw] + [self.p.h], dtype=np.int64)
+
+    def destroy(self):
+        for f in self.uvw + [self.p]:
+            f.destroy()
+
+    def upload(self, level, u, p):
+        """u: [u_x | u_y | u_z] (3 * P2 DoFs), p: P1 DoFs"""
+        n2 = self.uvw[0].ndofs(level)
+        u = np.asarray(u, dtype=np.float64).reshape(3, n2)
+        for c in range(3):
+            self.uvw[c].upload(level, u[c])
+        self.p.upload(level, p)
+
+    def download(self, level):
+        return np.concatenate([f.download(level) for f in self.uvw]), self.p.download(level)
+
+    def assign(self, scalars, funcs, level, flag=ALL):
+        for c in range(3):
+            self.uvw[c].assign(scalars, [f.uvw[c] for f in funcs], level, flag)
+        if _pressure_flag(flag):
+            self.p.assign(scalars, [f.p for f in funcs], level, ALL)
+
+    def add(self, scalars, funcs, level, flag=ALL):
+        for c in range(3):
+            self.uvw[c].add(scalars, [f.uvw[c] for f in funcs], level, flag)
+        if _pressure_flag(flag):
+            self.p.add(scalars, [f.p for f in funcs], level, ALL)
+
+    def interpolate(self, value, level, flag=ALL):
+        for c in range(3):
+            self.uvw[c].interpolate(value, level, flag)
+        if _pressure_flag(flag):
+            self.p.interpolate(value, level, ALL)
+
+    def dot(self, other, level, flag=ALL):
+        d = sum(self.uvw[c].dot(other.uvw[c], level, flag) for c in range(3))
+        return d + (self.p.dot(other.p, level, ALL) if _pressure_flag(flag) else 0.0)
+
+    def project_mean_pressure(self, level):
+        """vertexdof::projectMean of the pressure"""
+        n = self.storage.ndofs(level)
+        self.p.add_const(-self.p.sum(level, ALL) / n, level, ALL)
+
+
+class P2P1StokesOperator:
+    """P2P1ElementwiseBlendingFullViscousStokesOperator with viscosity
+    eta(x) = exp(c0 + c1 |x|) at the physical (blended) point"""
+
+    A_U, BT_P, B_U = 1, 2, 4
+
+    def __init__(self, storage, min_level, max_level, visc_c0=0.0, visc_c1=0.0):
+        self.storage = storage
+        self.ctx = storage.ctx
+        h = C.c_int64()
+        self.ctx.check(lib().hg_stokes_create(self.ctx.h, storage.h, min_level, max_level, float(visc_c0),
+                                              float(visc_c1), C.byref(h)), "P2P1StokesOperator")
+        self.h = h.value
+
+    def destroy(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().hg_operator_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def apply(self, src, dst, level, flag, update=REPLACE, mask=7):
+        """mask bit 1: dst.uvw = A src.uvw, 2: dst.uvw += B^T src.p, 4: dst.p = B src.uvw"""
+        self.ctx.check(lib().hg_stokes_apply(self.ctx.h, self.h, _ptr(src.handles()), _ptr(dst.handles()), level,
+                                             flag, mask, update), "Stokes apply")
+
+    def velocity_inverse_diagonal(self, level, dst):
+        self.ctx.check(lib().hg_stokes_velocity_inverse_diagonal(self.ctx.h, self.h, level, _ptr(dst.handles())),
+                       "velocity inverse diagonal")
+
+    def pspg_inverse_diagonal(self, level, dst):
+        self.ctx.check(lib().hg_stokes_pressure_diagonal(self.ctx.h, self.h, level, 0, dst.h), "PSPG diagonal")
+
+    def lumped_inverse_mass(self, level, dst):
+        self.ctx.check(lib().hg_stokes_pressure_diagonal(self.ctx.h, self.h, level, 1, dst.h), "lumped mass")

@@ -25,7 +25,8 @@ COMMON = [
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-SOURCES = ["topology.cpp", "runtime.hip", "p1.hip", "p2.hip", "p2transfer.hip", "solvers_abi.cpp", "serving.cpp"]
+SOURCES = ["topology.cpp", "runtime.hip", "p1.hip", "p2.hip", "p2transfer.hip", "stokes.hip", "solvers_abi.cpp",
+           "serving.cpp"]
 
 
 def _compile(src):

@@ -19,6 +19,7 @@
 #include "abi_util.hpp"
 #include "halo.hpp"
 #include "hyteg_amd.h"
+#include "p2tile.hpp"
 #include "runtime.hpp"
 
 namespace hg {
@@ -47,11 +48,6 @@ struct QuadConst< float >
    __device__ static const float* a4( int q ) { return cQ4f[q]; }
    __device__ static const float* b4( int q ) { return cQ4mf[q]; }
 };
-__constant__ int    cVof[6][4][3] = {
-    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
-    { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
-    { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } }, { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
-__constant__ int cP2E[6][2] = { { 0, 1 }, { 0, 2 }, { 0, 3 }, { 1, 2 }, { 1, 3 }, { 2, 3 } };
 
 // Xiao-Gimbutas degree-4 rule on the tetrahedron, 11 points; the weights are
 // fixed by exactness for all polynomials of degree <= 4
@@ -137,27 +133,6 @@ struct KParams
    double a[32], om[32][3], ph[32];
 };
 
-// Per macro-cell data of the tiled element kernels.  Micro-cell of type t with
-// anchor a has the computational-space vertices p_k = v0 + hE (a + vof_t[k]);
-// J[t] = hE V_t (columns p_k - p_0), Dc[t] = rows of J[t]^-1 (gradients of the
-// barycentric coordinates 1..3), adet[t] = |det J[t]|.  The shell map of the cell
-// (IcosahedralShellMap.hpp:109-146,183-236) is F(x) = q(x) x with
-// q = (r0 + |lam| dr) / |x|, lam = ((x - B0) . n) / volT, so that
-// DF = q I + x w^T,  w = s n - (q / |x|^2) x,  s = sign(lam) dr / (volT |x|):
-// a rank-one update of a multiple of the identity, inverted in closed form.
-// R is the arithmetic type of the kernel that reads it (double, or float for
-// the reduced-precision apply).
-template < class R >
-struct P2CellGeomT
-{
-   R   v0[3], hE[9];
-   R   J[6][9], Dc[6][9], adet[6];
-   R   XQ[6][11][4]; // J[t] (lam_1, lam_2, lam_3)_q and its lam-increment nl . XQ
-   R   nl[3], lam0;  // lam(x) = nl . x - lam0,  nl = n / volT,  lam0 = nl . B0
-   R   r0, dr;       // radius at the ray vertex and its increment to the ref vertex
-   int kind;         // 0 affine, 1 shell
-};
-using P2CellGeom = P2CellGeomT< double >;
 
 struct P2Operator : OperatorBase
 {
@@ -207,23 +182,6 @@ __device__ __forceinline__ void shellMap( const double* B, const double x[3], do
    }
 }
 
-// micro-cell number j of type t in a macro-cell lattice of 2^L micro-edges
-__device__ __forceinline__ void decodeMicro( int64_t wt, int64_t j, int64_t& x, int64_t& y, int64_t& z )
-{
-   // slice z holds tri(wt - z) micro-cells
-   int64_t lo = 0, hi = wt - 1;
-   const int64_t total = tet( wt );
-   while ( lo < hi )
-   {
-      const int64_t mid = ( lo + hi + 1 ) / 2;
-      if ( total - tet( wt - mid ) <= j )
-         lo = mid;
-      else
-         hi = mid - 1;
-   }
-   z = lo;
-   decodeTri( wt - z, j - ( total - tet( wt - z ) ), x, y );
-}
 
 // locate micro-cell gid: returns cell c, type t and anchor (x,y,z)
 __device__ __forceinline__ bool microOf( int64_t gid, int64_t mm, int64_t nc, int64_t& c, int& t, int64_t a[3] )
@@ -327,76 +285,6 @@ __device__ __forceinline__ void microElement( const CellGeom& g, int t, const in
    }
 }
 
-// ---- tiled element kernels ------------------------------------------------
-// A workgroup handles one tile of kTile^3 micro-cell anchors of one macro-cell
-// (all 6 micro-cell types: 384 threads, one micro-cell each, one type per
-// wavefront).  The tile's P2 values sit on a (2 kTile + 1)^3 box of the level
-// L+1 lattice; the box is staged through LDS, element results are summed in
-// LDS, and the box is written back once: box-interior points belong to this
-// tile only (plain read-modify-write), box-surface points are shared with
-// neighbouring tiles (atomic add).
-constexpr int kTile        = 4;
-constexpr int kBox         = 2 * kTile + 1;
-constexpr int kBox3        = kBox * kBox * kBox;
-constexpr int kTileThreads = 6 * kTile * kTile * kTile;
-
-__host__ __device__ inline int64_t tileEdges( int64_t mm ) { return ( mm - 1 ) / kTile + 1; }
-__host__ __device__ inline int64_t tilesPerCell( int64_t mm ) { return tet( tileEdges( mm ) ); }
-
-struct TileCoord
-{
-   int64_t c, x0, y0, z0;
-};
-
-__device__ __forceinline__ TileCoord tileOf( int64_t lb, int64_t mm )
-{
-   const int64_t tpc = tilesPerCell( mm );
-   TileCoord     tc;
-   tc.c       = lb / tpc;
-   int64_t ti, tj, tk;
-   decodeMicro( tileEdges( mm ), lb - tc.c * tpc, ti, tj, tk );
-   tc.x0 = kTile * ti;
-   tc.y0 = kTile * tj;
-   tc.z0 = kTile * tk;
-   return tc;
-}
-
-// 1/b and 1/sqrt(b) from the hardware estimates (relative error below 2^-22)
-// refined by two Newton steps (error below 2^-52 after rounding)
-__device__ __forceinline__ double rcpNewton( double b )
-{
-   double r = __builtin_amdgcn_rcp( b );
-   r        = fma( r, fma( -b, r, 1.0 ), r );
-   return fma( r, fma( -b, r, 1.0 ), r );
-}
-__device__ __forceinline__ double rsqrtNewton( double b )
-{
-   double       y = __builtin_amdgcn_rsq( b );
-   const double h = 0.5 * b;
-   y              = y * fma( -h * y, y, 1.5 );
-   return y * fma( -h * y, y, 1.5 );
-}
-// FP64: refined estimates; FP32: the hardware estimates (about 1 ulp)
-__device__ __forceinline__ double fastRcp( double b ) { return rcpNewton( b ); }
-__device__ __forceinline__ double fastRsqrt( double b ) { return rsqrtNewton( b ); }
-__device__ __forceinline__ float  fastRcp( float b ) { return __builtin_amdgcn_rcpf( b ); }
-__device__ __forceinline__ float  fastRsqrt( float b ) { return __builtin_amdgcn_rsqf( b ); }
-
-// shell-map scalars at x with lam(x) = ls: q = |F(x)| / |x|, w (DF = q I + x w^T),
-// beta = q + w . x
-template < class R >
-__device__ __forceinline__ void shellScalars( const P2CellGeomT< R >& g, const R x[3], R ls, R& qq, R w[3], R& beta )
-{
-   const R r2   = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
-   const R irho = fastRsqrt( r2 );
-   qq           = ( g.r0 + fabs( ls ) * g.dr ) * irho;
-   const R s    = ( ls < 0 ? -g.dr : g.dr ) * irho; // DF = q I - (q / rho^2) x x^T + s x nl^T
-   const R qr2  = qq * irho * irho;
-#pragma unroll
-   for ( int d = 0; d < 3; ++d )
-      w[d] = s * g.nl[d] - qr2 * x[d];
-   beta = qq + w[0] * x[0] + w[1] * x[1] + w[2] * x[2];
-}
 
 __device__ __forceinline__ double kAt( const P2CellGeom& g, const KParams& K, const double x[3], double ls )
 {
@@ -410,49 +298,6 @@ __device__ __forceinline__ double kAt( const P2CellGeom& g, const KParams& K, co
    return kEval( K, x );
 }
 
-template < class R >
-struct TileThread
-{
-   int  t;          // micro-cell type (uniform per wavefront)
-   int  ax, ay, az; // anchor inside the tile
-   bool valid;
-   R    p0[3];      // computational-space vertex 0
-   R    ls0;        // lam(p0) of the shell map
-};
-
-template < class R >
-__device__ __forceinline__ TileThread< R > tileThread( const P2CellGeomT< R >& g, const TileCoord& tc, int64_t mm )
-{
-   TileThread< R > th;
-   th.t        = __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
-   const int a = int( threadIdx.x ) & 63;
-   th.ax       = a & 3;
-   th.ay       = ( a >> 2 ) & 3;
-   th.az       = a >> 4;
-   const int64_t gx = tc.x0 + th.ax, gy = tc.y0 + th.ay, gz = tc.z0 + th.az;
-   th.valid         = gx + gy + gz <= mm - 1 - MicroCells::rowShift[th.t];
-   const R vx = R( gx + cVof[th.t][0][0] ), vy = R( gy + cVof[th.t][0][1] ), vz = R( gz + cVof[th.t][0][2] );
-#pragma unroll
-   for ( int d = 0; d < 3; ++d )
-      th.p0[d] = g.v0[d] + g.hE[3 * d] * vx + g.hE[3 * d + 1] * vy + g.hE[3 * d + 2] * vz;
-   th.ls0 = g.nl[0] * th.p0[0] + g.nl[1] * th.p0[1] + g.nl[2] * th.p0[2] - g.lam0;
-   return th;
-}
-
-// LDS box positions of the 10 P2 DoFs of the thread's micro-cell (vertices in
-// the order of getMicroVerticesFromMicroCell, then the edges (0,1), (0,2),
-// (0,3), (1,2), (1,3), (2,3))
-template < class R >
-__device__ __forceinline__ void elementBoxPositions( const TileThread< R >& th, int pos[10] )
-{
-#pragma unroll
-   for ( int k = 0; k < 4; ++k )
-      pos[k] = 2 * ( th.ax + cVof[th.t][k][0] ) +
-               kBox * ( 2 * ( th.ay + cVof[th.t][k][1] ) + kBox * 2 * ( th.az + cVof[th.t][k][2] ) );
-#pragma unroll
-   for ( int e = 0; e < 6; ++e )
-      pos[4 + e] = ( pos[cP2E[e][0]] + pos[cP2E[e][1]] ) / 2;
-}
 
 // k(F(x_q)) for every quadrature point of every micro-cell, stored tile-major:
 // table[(tile * 11 + q) * 384 + thread] (zero for threads outside the macro-cell)
@@ -477,21 +322,6 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileCoefficients( const P2C
    }
 }
 
-// dst += v at a point shared with other tiles
-__device__ __forceinline__ void sharedAdd( double* o, double v ) { atomicAdd( o, v ); }
-__device__ __forceinline__ void sharedAdd( float* o, float v ) { atomicAdd( o, v ); }
-__device__ __forceinline__ void sharedAdd( __half* o, float v )
-{
-   // packed FP16 atomic add on the aligned pair holding the value, with +0 for
-   // the other half (function buffers are padded, so the pair never reaches past
-   // the allocation)
-   typedef _Float16 h2 __attribute__( ( ext_vector_type( 2 ) ) );
-   const uintptr_t a   = reinterpret_cast< uintptr_t >( o );
-   h2*             w   = reinterpret_cast< h2* >( a & ~uintptr_t( 3 ) );
-   const _Float16  hv  = _Float16( v );
-   const h2        inc = ( a & 2 ) ? h2{ _Float16( 0.0f ), hv } : h2{ hv, _Float16( 0.0f ) };
-   __builtin_amdgcn_global_atomic_fadd_v2f16( (__attribute__( ( address_space( 1 ) ) ) h2*) w, inc );
-}
 
 // y = A x on the cell lattices of level L+1 (halo positions included): the
 // element action of p2_div_k_grad_blending_q4 evaluated in barycentric form,
@@ -673,51 +503,6 @@ __global__ void kP2ElementDiag( const CellGeom* __restrict__ geom, const KParams
       atomicAdd( dst + pos[i], ye[i] );
 }
 
-// halo positions of every cell lattice set to zero
-template < class T >
-__global__ void kZeroHalos( BndGeom g, int64_t nc, T* __restrict__ v )
-{
-   const int64_t per = 4 * g.fin + 6 * g.ein + 4;
-   const int64_t gid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   if ( gid >= per * nc )
-      return;
-   const int64_t c = gid / per;
-   int64_t       j = gid - c * per;
-   int64_t       t[4];
-   if ( j < 4 * g.fin )
-   {
-      const int     k = int( j / g.fin );
-      int64_t       a, b;
-      decodeTri( g.mm - 2, j - k * g.fin, a, b );
-      faceToCell( g.meta[c], k, g.mm, a + 1, b + 1, t );
-   }
-   else if ( ( j -= 4 * g.fin ) < 6 * g.ein )
-   {
-      const int k = int( j / g.ein );
-      edgeToCell( g.meta[c], k, g.mm, j - k * g.ein + 1, t );
-   }
-   else
-   {
-      t[0] = t[1] = t[2] = t[3] = 0;
-      t[j - 6 * g.ein]          = g.mm;
-   }
-   v[c * g.sc + tetIndex( g.w, t[1], t[2], t[3] )] = fromAcc< T >( typename Acc< T >::type( 0 ) );
-}
-
-template < class T >
-__global__ void kFillRange( int64_t n, T* __restrict__ v, double value )
-{
-   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   if ( i < n )
-      v[i] = fromAcc< T >( typename Acc< T >::type( value ) );
-}
-
-__global__ void kInvertNonZero( int64_t n, double* __restrict__ v )
-{
-   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   if ( i < n && v[i] != 0.0 )
-      v[i] = 1.0 / v[i];
-}
 
 static int64_t microCellsPerCell( int64_t mm )
 {
@@ -845,127 +630,6 @@ __global__ void kMult( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bn
    dst[p]          = fromAcc< T >( toAcc( a[p] ) * toAcc( b[p] ) );
 }
 
-static const int kVofHost[6][4][3] = {
-    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
-    { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
-    { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } }, { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
-
-// per macro-cell data of the tiled kernels for micro-edge length 1 (hE holds the
-// macro-cell's edge vectors; tileGeometryLevel scales them to a level)
-static std::vector< P2CellGeom > tileGeometry( const Topology& t )
-{
-   std::vector< P2CellGeom > out( t.nc );
-   for ( int c = 0; c < t.nc; ++c )
-   {
-      P2CellGeom&   g = out[c];
-      const double* v[4];
-      for ( int k = 0; k < 4; ++k )
-         v[k] = &t.X[3 * t.cv[c][k]];
-      for ( int d = 0; d < 3; ++d )
-      {
-         g.v0[d] = v[0][d];
-         for ( int k = 0; k < 3; ++k )
-            g.hE[3 * d + k] = v[k + 1][d] - v[0][d];
-      }
-      g.kind = t.blend.empty() ? 0 : 1;
-      if ( g.kind )
-      {
-         const ShellCellMap& b = t.blend[c];
-         double              a[3], tt[3], f[3];
-         for ( int d = 0; d < 3; ++d )
-         {
-            a[d]    = b.refVertex[d] - b.rayVertex[d];
-            tt[d]   = b.thrVertex[d] - b.rayVertex[d];
-            f[d]    = b.forVertex[d] - b.rayVertex[d];
-         }
-         const double n[3]  = { tt[1] * f[2] - tt[2] * f[1], tt[2] * f[0] - tt[0] * f[2], tt[0] * f[1] - tt[1] * f[0] };
-         const double volT  = a[0] * n[0] + a[1] * n[1] + a[2] * n[2];
-         for ( int d = 0; d < 3; ++d )
-            g.nl[d] = n[d] / volT;
-         g.lam0 = g.nl[0] * b.rayVertex[0] + g.nl[1] * b.rayVertex[1] + g.nl[2] * b.rayVertex[2];
-         g.r0   = b.radRayVertex;
-         g.dr   = b.radRefVertex - b.radRayVertex;
-      }
-      else
-      {
-         for ( int d = 0; d < 3; ++d )
-            g.nl[d] = 0.0;
-         g.lam0 = g.r0 = g.dr = 0.0;
-      }
-   }
-   return out;
-}
-
-// scale the level-independent cell data to micro-edge length 1/mm and fill the
-// per-type Jacobians, their inverses and determinants
-static void tileGeometryLevel( std::vector< P2CellGeom >& gs, int64_t mm )
-{
-   const double h = 1.0 / double( mm );
-   for ( auto& g : gs )
-   {
-      for ( double& e : g.hE )
-         e *= h;
-      for ( int ty = 0; ty < 6; ++ty )
-      {
-         double* J = g.J[ty];
-         for ( int r = 0; r < 3; ++r )
-            for ( int k = 0; k < 3; ++k )
-            {
-               double s = 0.0;
-               for ( int m = 0; m < 3; ++m )
-                  s += g.hE[3 * r + m] * double( kVofHost[ty][k + 1][m] - kVofHost[ty][0][m] );
-               J[3 * r + k] = s;
-            }
-         const double c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
-         const double det = J[0] * c00 + J[1] * c01 + J[2] * c02;
-         const double id  = 1.0 / det;
-         double*      D   = g.Dc[ty];
-         D[0]             = c00 * id;
-         D[1]             = ( J[2] * J[7] - J[1] * J[8] ) * id;
-         D[2]             = ( J[1] * J[5] - J[2] * J[4] ) * id;
-         D[3]             = c01 * id;
-         D[4]             = ( J[0] * J[8] - J[2] * J[6] ) * id;
-         D[5]             = ( J[2] * J[3] - J[0] * J[5] ) * id;
-         D[6]             = c02 * id;
-         D[7]             = ( J[1] * J[6] - J[0] * J[7] ) * id;
-         D[8]             = ( J[0] * J[4] - J[1] * J[3] ) * id;
-         g.adet[ty]       = std::fabs( det );
-         for ( int q = 0; q < 11; ++q )
-         {
-            double* xq = g.XQ[ty][q];
-            for ( int d = 0; d < 3; ++d )
-               xq[d] = J[3 * d] * kQP[q][1] + J[3 * d + 1] * kQP[q][2] + J[3 * d + 2] * kQP[q][3];
-            xq[3] = g.nl[0] * xq[0] + g.nl[1] * xq[1] + g.nl[2] * xq[2];
-         }
-      }
-   }
-}
-
-static std::vector< P2CellGeomT< float > > toFloat( const std::vector< P2CellGeom >& gs )
-{
-   std::vector< P2CellGeomT< float > > out( gs.size() );
-   for ( size_t c = 0; c < gs.size(); ++c )
-   {
-      const P2CellGeom&      g = gs[c];
-      P2CellGeomT< float >& f = out[c];
-      auto cp = []( float* dst, const double* src, int n ) {
-         for ( int i = 0; i < n; ++i )
-            dst[i] = float( src[i] );
-      };
-      cp( f.v0, g.v0, 3 );
-      cp( f.hE, g.hE, 9 );
-      cp( &f.J[0][0], &g.J[0][0], 54 );
-      cp( &f.Dc[0][0], &g.Dc[0][0], 54 );
-      cp( f.adet, g.adet, 6 );
-      cp( &f.XQ[0][0][0], &g.XQ[0][0][0], 6 * 11 * 4 );
-      cp( f.nl, g.nl, 3 );
-      f.lam0 = float( g.lam0 );
-      f.r0   = float( g.r0 );
-      f.dr   = float( g.dr );
-      f.kind = g.kind;
-   }
-   return out;
-}
 
 } // namespace hg
 
@@ -1123,7 +787,7 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
       for ( int L = minLevel; L <= maxLevel; ++L )
       {
          std::vector< P2CellGeom > gl = tg;
-         tileGeometryLevel( gl, int64_t( 1 ) << L );
+         tileGeometryLevel( gl, int64_t( 1 ) << L, kQP, 11 );
          op->tgeom[L].upload( ctx, gl );
          op->tgeomF[L].upload( ctx, toFloat( gl ) );
       }

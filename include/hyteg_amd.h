@@ -158,6 +158,11 @@ int hg_function_scale_batched( hg_ctx* ctx, hg_handle fn, const double* scales, 
 int hg_function_mult( hg_ctx* ctx, hg_handle dst, hg_handle a, hg_handle b, int level, int flag );
 /* copyFrom<otherT>(other, level): cast copy between storage types (VertexDoFFunction.hpp:151) */
 int hg_function_copy_from( hg_ctx* ctx, hg_handle dst, hg_handle src, int level );
+/* sumGlobal(level, flag) of a single-vector FP64 function (VertexDoFFunction.hpp:310) */
+int hg_function_sum( hg_ctx* ctx, hg_handle fn, int level, int flag, double* out );
+/* add(scalar, level, flag): fn += value on flagged DoFs (VertexDoFFunction.hpp:195);
+ * with hg_function_sum this is vertexdof::projectMean (VertexDoFFunction.hpp:586-592) */
+int hg_function_add_const( hg_ctx* ctx, hg_handle fn, double value, int level, int flag );
 /* communicate<Vertex,Edge>..<Face,Cell>: refresh every macro-cell halo copy from
  * its owner (VertexDoFFunction.hpp:366-392, VertexDoFPackInfo.hpp:534-610) */
 int hg_function_communicate( hg_ctx* ctx, hg_handle fn, int level );
@@ -305,6 +310,30 @@ int hg_serve_stream( int device, int64_t n, const int* kind, const int* level, c
                      double* rel_residual, double* checksum, int* cycles, double* wall_s );
 /* message of the last hg_serve_stream failure on this thread */
 const char* hg_serve_last_error( void );
+
+/* ---- P2-P1 Taylor-Hood Stokes -------------------------------------------------------
+ * P2P1ElementwiseBlendingFullViscousStokesOperator(storage, minLevel, maxLevel, viscosity)
+ * (elementwiseoperators/P2P1ElementwiseBlendingStokesOperator.hpp:125-216) with
+ * viscosity eta(x) = exp(viscC0 + viscC1 |x|) at the physical (blended) point. */
+int hg_stokes_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, double viscC0, double viscC1,
+                      hg_handle* out );
+/* Blocks of the Stokes apply: src/dst = {u_x, u_y, u_z, p} (FP64 P2 velocity
+ * components and FP64 P1 pressure of one storage); mask bit 1: dst.uvw = A src.uvw
+ * (P2ElementwiseBlendingFullViscousOperator::apply), bit 2: dst.uvw += B^T src.p
+ * (P1ToP2ElementwiseBlendingDivTOperator), bit 4: dst.p = B src.uvw
+ * (P2ToP1ElementwiseBlendingDivOperator); mask 7 is the operator's apply
+ * (P2P1ElementwiseBlendingStokesOperator.hpp:176-184).  Handles of unused
+ * components are ignored.  The pressure has no boundary DoFs (createAllInnerBC):
+ * it is written when flag contains Inner. */
+int hg_stokes_apply( hg_ctx* ctx, hg_handle op, const hg_handle* src, const hg_handle* dst, int level, int flag, int mask,
+                     int update );
+/* inverse diagonal of the velocity block (VectorToVectorOperator::
+ * computeInverseDiagonalOperatorValues) into dst = {d_x, d_y, d_z} */
+int hg_stokes_velocity_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, const hg_handle* dst );
+/* kind 0: P1PSPGInvDiagOperator values (the Uzawa smoother's pressure scaling),
+ * kind 1: P1LumpedInvMassOperator values (the MINRES pressure preconditioner)
+ * (constant_stencil_operator/P1ConstantOperator.hpp:195-215), affine geometry */
+int hg_stokes_pressure_diagonal( hg_ctx* ctx, hg_handle op, int level, int kind, hg_handle dst );
 
 #ifdef __cplusplus
 }

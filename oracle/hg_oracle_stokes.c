@@ -292,7 +292,9 @@ static void sapply_cb( void* vctx, const double* p, const double* blend, const i
  * velocity; 4: B u into the pressure), with the gemv data flow of the
  * elementwise operators (or_p2_apply): velocity DoFs of vertices, edges and
  * faces are written where their flag matches, cell-interior DoFs always
- * (zeroed first on Replace with Inner in flag); pressure DoFs are all Inner. */
+ * (zeroed first on Replace with Inner in flag); pressure DoFs are all Inner,
+ * so pressure DoFs of vertices, edges and faces are written when flag
+ * contains Inner. */
 void or_stokes_apply( const or_stokesop* op, const double* u, const double* p, double* yu, double* yp, int flag, int mask,
                       int add )
 {
@@ -313,9 +315,16 @@ void or_stokes_apply( const or_stokesop* op, const double* u, const double* p, d
             else if ( op->flags1[i] & flag )
                *d = add ? *d + au[cc * n2 + i] : au[cc * n2 + i];
          }
-   if ( ( mask & 4 ) && ( flag & OR_INNER ) )
+   /* pressure: every DoF has type Inner; cell-interior DoFs accumulate as in the
+    * velocity (zeroed first on Replace with Inner in flag) */
+   if ( mask & 4 )
       for ( int64_t i = 0; i < n1; ++i )
-         yp[i] = add ? yp[i] + ap[i] : ap[i];
+      {
+         if ( i >= op->l0.coff )
+            yp[i] = ( ( !add && ( flag & OR_INNER ) ) ? 0.0 : yp[i] ) + ap[i];
+         else if ( flag & OR_INNER )
+            yp[i] = add ? yp[i] + ap[i] : ap[i];
+      }
    free( au );
    free( ap );
 }

@@ -446,6 +446,7 @@ def stokes_residual(op, u, p, bu, bp, flag):
     return ru, rp
 
 
+
 def stokes_minres(op, u, p, bu, bp, minv, max_iter, rel_tol, abs_tol=0.0, flag=GMG_FLAG):
     """MinResSolver (MinresSolver.hpp:117-245) with StokesPressureBlockPreconditioner<P1LumpedInvMass>;
     returns the iteration index at exit"""

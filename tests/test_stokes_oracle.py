@@ -1,0 +1,164 @@
+"""Pins of the CPU oracle's P2-P1 Stokes operator (blended full-viscous block,
+divergence blocks, PSPG and lumped-mass diagonals) by closed forms and
+invariants that a wrong term, sign, index or quadrature rule breaks."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+SHELL_VISC = (np.log(1e4) / 0.45, -np.log(1e4) / 0.45)  # eta = exp(ln(1e4) (1 - r) / 0.45)
+
+
+@pytest.fixture(scope="module")
+def cube():
+    return po.Mesh.cuboid()
+
+
+@pytest.fixture(scope="module")
+def shell():
+    return po.Mesh.shell()
+
+
+def _linear(X, G):
+    return np.concatenate([X @ G[c] for c in range(3)])
+
+
+def test_linear_velocity_energy_is_the_closed_form_stress(cube):
+    """u = G x is represented exactly and its stress
+    sigma = eta (G + G^T) - 2/3 eta tr(G) I is constant, so for v = x_k e_i
+    v^T A u = int sigma : grad v = |Omega| sigma_ik (the 2/3 term, the symmetric
+    gradient and the block layout all enter)"""
+    L = 2
+    X = cube.coords(L + 1)
+    G = np.random.default_rng(0).normal(size=(3, 3))
+    for c0, scale in ((0.0, 1.0), (np.log(3.0), 3.0)):
+        op = po.StokesOp(cube, L, c0, 0.0)
+        y = np.zeros(3 * op.n2)
+        op.apply(_linear(X, G), None, y, None, po.ALL, op.A_U)
+        sigma = scale * ((G + G.T) - (2.0 / 3.0) * np.trace(G) * np.eye(3))
+        for i in range(3):
+            for k in range(3):
+                v = np.zeros(3 * op.n2)
+                v[i * op.n2:(i + 1) * op.n2] = X[:, k]
+                assert abs(v @ y - sigma[i, k]) < 1e-12 * max(1.0, abs(sigma).max())
+
+
+def test_null_space_of_the_viscous_block(cube, shell):
+    """rigid rotations and the isotropic expansion u = x (eps = I, div = 3) have
+    zero full-viscous stress on the affine cube; constants are annihilated on the
+    blended shell for any viscosity"""
+    L = 2
+    op = po.StokesOp(cube, L)
+    X = cube.coords(L + 1)
+    y = np.zeros(3 * op.n2)
+    for u in (np.concatenate([X[:, 0], X[:, 1], X[:, 2]]),
+              np.concatenate([-X[:, 1], X[:, 0], 0 * X[:, 0]]),
+              np.concatenate([0 * X[:, 0], -X[:, 2], X[:, 1]])):
+        op.apply(u, None, y, None, po.ALL, op.A_U)
+        assert np.abs(y).max() < 1e-12
+    ops = po.StokesOp(shell, 1, *SHELL_VISC)
+    y = np.zeros(3 * ops.n2)
+    for c in range(3):
+        u = np.zeros(3 * ops.n2)
+        u[c * ops.n2:(c + 1) * ops.n2] = 1.0
+        ops.apply(u, None, y, None, po.ALL, ops.A_U)
+        assert np.abs(y).max() < 1e-9 * np.exp(SHELL_VISC[0])
+
+
+def test_divergence_closed_form(cube):
+    """p^T B u = -int p div u: with u = G x, div u = tr(G); p = 1 gives -tr(G),
+    p = x gives -tr(G) / 2 on the unit cube"""
+    L = 2
+    op = po.StokesOp(cube, L)
+    X = cube.coords(L + 1)
+    P = cube.coords(L)
+    for G in (np.eye(3), np.random.default_rng(1).normal(size=(3, 3))):
+        yp = np.zeros(op.n1)
+        op.apply(_linear(X, G), None, None, yp, po.ALL, op.B_U)
+        assert abs(yp.sum() + np.trace(G)) < 1e-13
+        for k in range(3):
+            assert abs(P[:, k] @ yp + 0.5 * np.trace(G)) < 1e-13
+
+
+def test_transpose_and_symmetry_on_the_blended_shell(shell):
+    op = po.StokesOp(shell, 1, *SHELL_VISC)
+    rng = np.random.default_rng(2)
+    u, w = rng.normal(size=3 * op.n2), rng.normal(size=3 * op.n2)
+    p = rng.normal(size=op.n1)
+    yp = np.zeros(op.n1)
+    yu = np.zeros(3 * op.n2)
+    op.apply(u, None, None, yp, po.ALL, op.B_U)
+    op.apply(None, p, yu, None, po.ALL, op.BT_P)
+    assert abs(p @ yp - u @ yu) < 1e-12 * abs(p @ yp)
+    op.apply(u, None, yu, None, po.ALL, op.A_U)
+    a = w @ yu
+    op.apply(w, None, yu, None, po.ALL, op.A_U)
+    assert abs(a - u @ yu) < 1e-11 * abs(a)
+
+
+def test_viscosity_enters_at_the_degree_3_points():
+    """u = G x, v = x_k e_i: v^T A u = sigma_ik(eta = 1) * Q(eta) with Q the
+    composite degree-3 rule applied to eta; Q converges to int eta like h^4, which
+    a misplaced point or weight breaks (cube [1,2]^3, where eta is smooth)"""
+    cube = po.Mesh.cuboid((1.0, 1.0, 1.0), (2.0, 2.0, 2.0))
+    c0, c1 = 0.3, 1.1
+    G = np.random.default_rng(3).normal(size=(3, 3))
+    sigma = (G + G.T) - (2.0 / 3.0) * np.trace(G) * np.eye(3)
+    # int_cube exp(c0 + c1 |x|) by a tensor Gauss rule (exact to far below the test tolerance)
+    g, w = np.polynomial.legendre.leggauss(64)
+    g, w = 0.5 * (g + 1) + 1.0, 0.5 * w
+    X3 = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    W3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    exact = float(W3 @ np.exp(c0 + c1 * np.linalg.norm(X3, axis=1)))
+    errs = []
+    for L in (1, 2, 3):
+        op = po.StokesOp(cube, L, c0, c1)
+        X = cube.coords(L + 1)
+        y = np.zeros(3 * op.n2)
+        op.apply(_linear(X, G), None, y, None, po.ALL, op.A_U)
+        Q = []
+        for i in range(3):
+            for k in range(3):
+                v = np.zeros(3 * op.n2)
+                v[i * op.n2:(i + 1) * op.n2] = X[:, k]
+                Q.append((v @ y) / sigma[i, k])
+        assert np.ptp(Q) < 1e-12 * abs(np.mean(Q))  # one weight for all nine entries
+        errs.append(abs(np.mean(Q) - exact))
+    assert errs[1] / errs[2] > 12 and errs[0] / errs[1] > 8, errs
+
+
+def test_pspg_diagonal_is_the_scaled_laplace_stencil(cube):
+    """interior points: PSPG diagonal = -(|T|^(2/3)/12) x (P1 stiffness stencil
+    centre), |T| the micro-cell volume"""
+    L = 3
+    inv = po.p1_pspg_inv_diag(cube, L)
+    lap = po.P1Op(cube, L)
+    vm = (1.0 / 6.0) / 8 ** L  # each macro-tetrahedron of the unit cube has volume 1/6
+    m = 2 ** L
+    for c in range(6):
+        centre = lap.cell_stencil(c)[0]
+        for (x, y, z) in ((1, 1, 1), (2, 1, 1), (1, 2, 3)):
+            if x + y + z < m:
+                idx = cube.locate(L, c, x, y, z)
+                assert abs(1.0 / inv[idx] + vm ** (2.0 / 3.0) / 12.0 * centre) < 1e-13 * abs(1.0 / inv[idx])
+
+
+def test_lumped_mass_sums_to_the_macro_volume(cube, shell):
+    assert abs((1.0 / po.p1_lumped_inv_mass(cube, 2)).sum() - 1.0) < 1e-13
+    X = shell.coords(0)
+    cells = shell.cell_vertices()
+    vol = sum(abs(np.linalg.det(np.stack([X[c[k]] - X[c[0]] for k in (1, 2, 3)]))) / 6.0 for c in cells)
+    for L in (0, 1):
+        assert abs((1.0 / po.p1_lumped_inv_mass(shell, L)).sum() - vol) < 1e-12 * vol
+
+
+def test_velocity_inverse_diagonal_matches_the_element_diagonals(shell):
+    op = po.StokesOp(shell, 0, *SHELL_VISC)
+    inv = op.inv_diag()
+    rng = np.random.default_rng(4)
+    for j in rng.choice(3 * op.n2, 20, replace=False):
+        e = np.zeros(3 * op.n2)
+        e[j] = 1.0
+        y = np.zeros(3 * op.n2)
+        op.apply(e, None, y, None, po.ALL, op.A_U)
+        assert abs(y[j] * inv[j] - 1.0) < 1e-12
