@@ -76,6 +76,10 @@ SIGNATURES = {
     "hg_stokes_apply": [_p, _i64, _p, _p, _i, _i, _i, _i],
     "hg_stokes_velocity_inverse_diagonal": [_p, _i64, _i, _p],
     "hg_stokes_pressure_diagonal": [_p, _i64, _i, _i, _i64],
+    "hg_stokes_estimate_radius": [_p, _i64, _p, _p, _i, _i, _p],
+    "hg_stokes_uzawa_smooth": [_p, _i64, _p, _p, _i, _d, _i, _i, _d, _i],
+    "hg_stokes_minres_solve": [_p, _i64, _p, _p, _i, _i, _d, _d, _p],
+    "hg_stokes_gmg_solve": [_p, _i64, _p, _p, _i, _i, _i, _i, _i, _d, _i, _i, _d, _i, _d, _i, _i, _d, _p, _p],
     "hg_function_copy_vector": [_p, _i64, _i, _i64, _i, _i],
     "hg_function_dot_batched": [_p, _i64, _i64, _i, _i, _p],
     "hg_function_scale_batched": [_p, _i64, _p, _i, _i],

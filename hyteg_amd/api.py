@@ -493,3 +493,36 @@ class P2P1StokesOperator:
 
     def lumped_inverse_mass(self, level, dst):
         self.ctx.check(lib().hg_stokes_pressure_diagonal(self.ctx.h, self.h, level, 1, dst.h), "lumped mass")
+
+
+def stokes_estimate_radius(A, x, tmp, level, iterations=25):
+    """power iteration on D^-1 A of the velocity block; x.uvw is the start vector"""
+    r = C.c_double()
+    A.ctx.check(lib().hg_stokes_estimate_radius(A.ctx.h, A.h, _ptr(x.handles()), _ptr(tmp.handles()), level,
+                                                iterations, C.byref(r)), "Stokes estimateRadius")
+    return r.value
+
+
+def stokes_uzawa_smooth(A, x, b, level, rho, omega=0.3, velocity_iterations=1, order=3, project=True):
+    A.ctx.check(lib().hg_stokes_uzawa_smooth(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), level, omega,
+                                             velocity_iterations, order, rho, int(project)), "Uzawa smoother")
+
+
+def stokes_minres_solve(A, x, b, level, max_iter=10, rel_tol=1e-6, abs_tol=0.0):
+    its = C.c_int()
+    A.ctx.check(lib().hg_stokes_minres_solve(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), level, max_iter,
+                                             rel_tol, abs_tol, C.byref(its)), "MINRES")
+    return its.value
+
+
+def stokes_gmg_solve(A, x, b, min_level, max_level, rho, pre=3, post=3, smooth_increment=2, omega=0.3,
+                     velocity_iterations=1, order=3, coarse_max_iter=10, coarse_tol=1e-6, project=True, cycles=10,
+                     tol=0.0):
+    """stokesGMGUzawaFSSolver layout (no free-slip); returns (cycles done, residual history)"""
+    hist = np.zeros(cycles + 1)
+    done = C.c_int()
+    A.ctx.check(lib().hg_stokes_gmg_solve(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), min_level, max_level,
+                                          pre, post, smooth_increment, omega, velocity_iterations, order, rho,
+                                          coarse_max_iter, coarse_tol, int(project), cycles, tol, _ptr(hist),
+                                          C.byref(done)), "Stokes GMG")
+    return done.value, hist[:done.value + 1]

@@ -16,6 +16,8 @@ using namespace hyteg_amd;
 
 namespace {
 
+using StokesOp = P2P1ElementwiseBlendingFullViscousStokesOperator;
+
 std::shared_ptr< PrimitiveStorage > storageOf( hg_ctx* ctx, hg_handle fn )
 {
    hg_handle s;
@@ -284,6 +286,24 @@ int batchedIR( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel
 
 } // namespace
 
+namespace {
+// the Uzawa smoother of the free-slip GMG template (StokesFSGMGUzawaSolverTemplate.hpp:113-170):
+// Chebyshev smoothing of the velocity block with coefficients from the spectral
+// radius, inside FullStokesVelocityBlockBlockDiagonalPreconditioner
+std::shared_ptr< UzawaSmoother< StokesOp > > stokesUzawa( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel,
+                                                          int maxLevel, double omega, int velocityIterations,
+                                                          int chebyshevOrder, double spectralRadius, bool project )
+{
+   auto cheby = std::make_shared< ChebyshevSmoother< StokesOp::VelocityOperator_T > >( storage, uint_t( minLevel ),
+                                                                                        uint_t( maxLevel ) );
+   cheby->setupCoefficients( uint_t( chebyshevOrder ), spectralRadius );
+   auto velocity = std::make_shared< FullStokesVelocityBlockBlockDiagonalPreconditioner< StokesOp > >( cheby );
+   return std::make_shared< UzawaSmoother< StokesOp > >( storage, velocity, uint_t( minLevel ), uint_t( maxLevel ), omega,
+                                                         Inner | NeumannBoundary | FreeslipBoundary,
+                                                         uint_t( velocityIterations ), project );
+}
+} // namespace
+
 extern "C" {
 
 int hg_p1_batched_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre,
@@ -477,6 +497,94 @@ int hg_p2_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int min
          *outer_done = iterativeRefinement< f16 >( ctx, op, x, b, minLevel, maxLevel, pre, post, chebyshevOrder,
                                                    spectralRadius, innerCycles, maxOuter, tol, coarseMaxIter, coarseTol,
                                                    res_hist );
+   } );
+}
+
+int hg_stokes_estimate_radius( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* tmp, int level,
+                               int iterations, double* radius )
+{
+   if ( !ctx || !x || !tmp || !radius || level < 0 || iterations < 1 )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      auto storage = storageOf( ctx, x[0] );
+      auto A       = StokesOp::borrow( storage, op, uint_t( level ), uint_t( level ) );
+      P2VectorFunction< double > X( ctx, x ), T( ctx, tmp );
+      *radius = chebyshev::estimateRadius( A->getA(), uint_t( level ), uint_t( iterations ), X, T );
+   } );
+}
+
+int hg_stokes_uzawa_smooth( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int level, double omega,
+                            int velocityIterations, int chebyshevOrder, double spectralRadius, int projectPressureMean )
+{
+   if ( !ctx || !x || !b || level < 0 || velocityIterations < 0 || chebyshevOrder < 1 || chebyshevOrder > 16 ||
+        !( spectralRadius > 0 ) )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      auto storage = storageOf( ctx, x[0] );
+      auto A       = StokesOp::borrow( storage, op, uint_t( level ), uint_t( level ) );
+      P2P1TaylorHoodFunction< double > X( ctx, x ), B( ctx, b );
+      auto uzawa = stokesUzawa( storage, level, level, omega, velocityIterations, chebyshevOrder, spectralRadius,
+                                projectPressureMean != 0 );
+      uzawa->solve( *A, X, B, uint_t( level ) );
+   } );
+}
+
+int hg_stokes_minres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int level, int maxIter,
+                            double relTol, double absTol, int* iterations )
+{
+   if ( !ctx || !x || !b || level < 0 || maxIter < 0 )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      auto storage = storageOf( ctx, x[0] );
+      auto A       = StokesOp::borrow( storage, op, uint_t( level ), uint_t( level ) );
+      P2P1TaylorHoodFunction< double > X( ctx, x ), B( ctx, b );
+      MinResSolver< StokesOp >         minres( storage, uint_t( level ), uint_t( level ), uint_t( maxIter ), relTol, absTol,
+                                               std::make_shared< StokesPressureBlockPreconditioner< StokesOp > >() );
+      minres.solve( *A, X, B, uint_t( level ) );
+      if ( iterations )
+         *iterations = int( minres.getIterations() );
+   } );
+}
+
+int hg_stokes_gmg_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel, int maxLevel,
+                         int pre, int post, int smoothIncrement, double omega, int velocityIterations, int chebyshevOrder,
+                         double spectralRadius, int coarseMaxIter, double coarseTol, int projectPressureMean, int cycles,
+                         double tol, double* res_hist, int* cycles_done )
+{
+   if ( !ctx || !x || !b || minLevel < 0 || maxLevel < minLevel || pre < 0 || post < 0 || smoothIncrement < 0 ||
+        velocityIterations < 0 || chebyshevOrder < 1 || chebyshevOrder > 16 || !( spectralRadius > 0 ) ||
+        coarseMaxIter < 0 || cycles < 0 )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      auto storage = storageOf( ctx, x[0] );
+      auto A       = StokesOp::borrow( storage, op, uint_t( minLevel ), uint_t( maxLevel ) );
+      P2P1TaylorHoodFunction< double > X( ctx, x ), B( ctx, b );
+      auto uzawa  = stokesUzawa( storage, minLevel, maxLevel, omega, velocityIterations, chebyshevOrder, spectralRadius,
+                                 projectPressureMean != 0 );
+      auto coarse = std::make_shared< MinResSolver< StokesOp > >(
+          storage, uint_t( minLevel ), uint_t( minLevel ), uint_t( coarseMaxIter ), coarseTol, 0.0,
+          std::make_shared< StokesPressureBlockPreconditioner< StokesOp > >() );
+      GeometricMultigridSolver< StokesOp > gmg( storage, uzawa, coarse, std::make_shared< P2P1StokesToP2P1StokesRestriction >(),
+                                                std::make_shared< P2P1StokesToP2P1StokesProlongation >(), uint_t( minLevel ),
+                                                uint_t( maxLevel ), uint_t( pre ), uint_t( post ), uint_t( smoothIncrement ) );
+      P2P1TaylorHoodFunction< double > r( "residual", storage, uint_t( maxLevel ), uint_t( maxLevel ) );
+      const uint_t                     L  = uint_t( maxLevel );
+      const double                     r0 = residualNorm( *A, X, B, r, L );
+      if ( res_hist )
+         res_hist[0] = r0;
+      int k = 0;
+      for ( double rk = r0; k < cycles && !( tol > 0 && rk <= tol * r0 ); )
+      {
+         gmg.solve( *A, X, B, L );
+         if ( projectPressureMean )
+            vertexdof::projectMean( X.p(), L );
+         rk = residualNorm( *A, X, B, r, L );
+         ++k;
+         if ( res_hist )
+            res_hist[k] = rk;
+      }
+      if ( cycles_done )
+         *cycles_done = k;
    } );
 }
 

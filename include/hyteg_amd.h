@@ -334,6 +334,29 @@ int hg_stokes_velocity_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, c
  * kind 1: P1LumpedInvMassOperator values (the MINRES pressure preconditioner)
  * (constant_stencil_operator/P1ConstantOperator.hpp:195-215), affine geometry */
 int hg_stokes_pressure_diagonal( hg_ctx* ctx, hg_handle op, int level, int kind, hg_handle dst );
+/* chebyshev::estimateRadius on the velocity block (ChebyshevSmoother.hpp:656-665):
+ * x = {x_x, x_y, x_z} is the start vector and returns the last iterate */
+int hg_stokes_estimate_radius( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* tmp, int level,
+                               int iterations, double* radius );
+/* one UzawaSmoother::solve (solvers/UzawaSmoother.hpp:330-470, UzawaSmootherWithFreeSlipProjection
+ * without projection) with a Chebyshev velocity smoother of the given order and
+ * radius inside FullStokesVelocityBlockBlockDiagonalPreconditioner */
+int hg_stokes_uzawa_smooth( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int level, double omega,
+                            int velocityIterations, int chebyshevOrder, double spectralRadius, int projectPressureMean );
+/* MinResSolver with StokesPressureBlockPreconditioner<P1LumpedInvMassOperator>
+ * (solvers/MinresSolver.hpp:117-245, solvertemplates/StokesSolverTemplates.hpp:54-69) */
+int hg_stokes_minres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int level, int maxIter,
+                            double relTol, double absTol, int* iterations );
+/* GeometricMultigridSolver<Stokes> as stokesGMGUzawaFSSolver builds it
+ * (solvertemplates/StokesFSGMGUzawaSolverTemplate.hpp:63-190, without free-slip
+ * projections): Uzawa smoothing, P2P1StokesToP2P1Stokes transfers, V(pre,post)
+ * with smoothIncrement more steps per coarser level, MINRES on minLevel.  Runs
+ * V-cycles (projecting the pressure mean after each when asked) until the Inner
+ * residual 2-norm drops by tol or `cycles` ran; res_hist[0..done] */
+int hg_stokes_gmg_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel, int maxLevel,
+                         int pre, int post, int smoothIncrement, double omega, int velocityIterations, int chebyshevOrder,
+                         double spectralRadius, int coarseMaxIter, double coarseTol, int projectPressureMean, int cycles,
+                         double tol, double* res_hist, int* cycles_done );
 
 #ifdef __cplusplus
 }

@@ -20,10 +20,12 @@
 #pragma once
 
 #include <cmath>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../hyteg_amd.h"
@@ -939,6 +941,539 @@ class GeometricMultigridSolver : public Solver< OperatorType >
    std::shared_ptr< ProlongationOperator< FunctionType > > prolongation_;
    FunctionType                                            tmp_;
    DoFType                                                 flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+// ---- P2-P1 Taylor-Hood Stokes --------------------------------------------------
+
+// the pressure of a Taylor-Hood function has only Inner DoFs
+// (P2P1TaylorHoodFunction.hpp:61, createAllInnerBC)
+inline DoFType pressureFlag( DoFType flag )
+{
+   return ( flag & Inner ) ? All : DoFType( 0 );
+}
+
+namespace vertexdof {
+// projectMean (p1functionspace/VertexDoFFunction.hpp:586-592): p -= sum(p) / #DoFs
+inline void projectMean( const P1Function< double >& p, uint_t level )
+{
+   hg_handle s;
+   int64_t   n   = 0;
+   double    sum = 0.0;
+   check( p.ctx(), hg_function_storage( p.ctx(), p.handle(), &s ), "projectMean" );
+   check( p.ctx(), hg_p1_ndofs( p.ctx(), s, int( level ), &n ), "projectMean" );
+   check( p.ctx(), hg_function_sum( p.ctx(), p.handle(), int( level ), All, &sum ), "projectMean" );
+   check( p.ctx(), hg_function_add_const( p.ctx(), p.handle(), -sum / double( n ), int( level ), All ), "projectMean" );
+}
+} // namespace vertexdof
+
+// P2VectorFunction<double>: the three components of a P2 vector field
+template < typename VT >
+class P2VectorFunction
+{
+ public:
+   using valueType = VT;
+   using ValueType = VT;
+
+   P2VectorFunction( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
+                     uint_t maxLevel, int nvec = 1 )
+   {
+      if ( nvec != 1 )
+         throw std::invalid_argument( "P2 vector functions hold one vector" );
+      for ( int k = 0; k < 3; ++k )
+         c_[k] = std::make_unique< P2Function< VT > >( name + "_" + char( 'u' + k ), storage, minLevel, maxLevel );
+   }
+   // a view of three existing component handles (not owned)
+   P2VectorFunction( hg_ctx* ctx, const hg_handle h[3] )
+   {
+      for ( int k = 0; k < 3; ++k )
+         c_[k] = P2Function< VT >::borrow( ctx, h[k] );
+   }
+   const P2Function< VT >& operator[]( int k ) const { return *c_[k]; }
+   hg_ctx*                 ctx() const { return c_[0]->ctx(); }
+
+   void assign( const std::vector< VT >& scalars, const std::vector< std::reference_wrapper< const P2VectorFunction > >& fs,
+                uint_t level, DoFType flag = All ) const
+   {
+      for ( int k = 0; k < 3; ++k )
+         c_[k]->assign( scalars, component( fs, k ), level, flag );
+   }
+   void add( const std::vector< VT >& scalars, const std::vector< std::reference_wrapper< const P2VectorFunction > >& fs,
+             uint_t level, DoFType flag = All ) const
+   {
+      for ( int k = 0; k < 3; ++k )
+         c_[k]->add( scalars, component( fs, k ), level, flag );
+   }
+   void interpolate( VT constant, uint_t level, DoFType flag = All ) const
+   {
+      for ( int k = 0; k < 3; ++k )
+         c_[k]->interpolate( constant, level, flag );
+   }
+   void setToZero( uint_t level ) const { interpolate( VT( 0 ), level, All ); }
+   VT   dotGlobal( const P2VectorFunction& o, uint_t level, DoFType flag = All ) const
+   {
+      VT s = 0;
+      for ( int k = 0; k < 3; ++k )
+         s += c_[k]->dotGlobal( *o.c_[k], level, flag );
+      return s;
+   }
+   void multElementwise( const std::vector< std::reference_wrapper< const P2VectorFunction > >& fs, uint_t level,
+                         DoFType flag = All ) const
+   {
+      for ( int k = 0; k < 3; ++k )
+         c_[k]->multElementwise( component( fs, k ), level, flag );
+   }
+
+ private:
+   static std::vector< std::reference_wrapper< const P2Function< VT > > >
+       component( const std::vector< std::reference_wrapper< const P2VectorFunction > >& fs, int k )
+   {
+      std::vector< std::reference_wrapper< const P2Function< VT > > > out;
+      for ( const auto& f : fs )
+         out.push_back( f.get()[k] );
+      return out;
+   }
+   std::unique_ptr< P2Function< VT > > c_[3];
+};
+
+// P2P1TaylorHoodFunction<double> (composites/P2P1TaylorHoodFunction.hpp): P2
+// velocity uvw and P1 pressure p, whose DoFs are all Inner
+template < typename VT >
+class P2P1TaylorHoodFunction
+{
+ public:
+   using valueType = VT;
+   using ValueType = VT;
+
+   P2P1TaylorHoodFunction( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
+                           uint_t maxLevel, int nvec = 1 )
+   : uvw_( std::make_unique< P2VectorFunction< VT > >( name + "_uvw", storage, minLevel, maxLevel, nvec ) )
+   , p_( std::make_unique< P1Function< VT > >( name + "_p", storage, minLevel, maxLevel ) )
+   {}
+   // a view of existing handles {u_x, u_y, u_z, p} (not owned)
+   P2P1TaylorHoodFunction( hg_ctx* ctx, const hg_handle h[4] )
+   : uvw_( std::make_unique< P2VectorFunction< VT > >( ctx, h ) )
+   , p_( P1Function< VT >::borrow( ctx, h[3] ) )
+   {}
+   const P2VectorFunction< VT >& uvw() const { return *uvw_; }
+   const P1Function< VT >&       p() const { return *p_; }
+
+   void assign( const std::vector< VT >& scalars,
+                const std::vector< std::reference_wrapper< const P2P1TaylorHoodFunction > >& fs, uint_t level,
+                DoFType flag = All ) const
+   {
+      uvw_->assign( scalars, velocities( fs ), level, flag );
+      if ( pressureFlag( flag ) )
+         p_->assign( scalars, pressures( fs ), level, All );
+   }
+   void add( const std::vector< VT >& scalars, const std::vector< std::reference_wrapper< const P2P1TaylorHoodFunction > >& fs,
+             uint_t level, DoFType flag = All ) const
+   {
+      uvw_->add( scalars, velocities( fs ), level, flag );
+      if ( pressureFlag( flag ) )
+         p_->add( scalars, pressures( fs ), level, All );
+   }
+   void interpolate( VT constant, uint_t level, DoFType flag = All ) const
+   {
+      uvw_->interpolate( constant, level, flag );
+      if ( pressureFlag( flag ) )
+         p_->interpolate( constant, level, All );
+   }
+   void setToZero( uint_t level ) const { interpolate( VT( 0 ), level, All ); }
+   VT   dotGlobal( const P2P1TaylorHoodFunction& o, uint_t level, DoFType flag = All ) const
+   {
+      VT s = uvw_->dotGlobal( *o.uvw_, level, flag );
+      if ( pressureFlag( flag ) )
+         s += p_->dotGlobal( *o.p_, level, All );
+      return s;
+   }
+
+ private:
+   static std::vector< std::reference_wrapper< const P2VectorFunction< VT > > >
+       velocities( const std::vector< std::reference_wrapper< const P2P1TaylorHoodFunction > >& fs )
+   {
+      std::vector< std::reference_wrapper< const P2VectorFunction< VT > > > out;
+      for ( const auto& f : fs )
+         out.push_back( f.get().uvw() );
+      return out;
+   }
+   static std::vector< std::reference_wrapper< const P1Function< VT > > >
+       pressures( const std::vector< std::reference_wrapper< const P2P1TaylorHoodFunction > >& fs )
+   {
+      std::vector< std::reference_wrapper< const P1Function< VT > > > out;
+      for ( const auto& f : fs )
+         out.push_back( f.get().p() );
+      return out;
+   }
+   std::unique_ptr< P2VectorFunction< VT > > uvw_;
+   std::unique_ptr< P1Function< VT > >       p_;
+};
+
+namespace detail {
+inline void stokesApply( hg_ctx* ctx, hg_handle op, const hg_handle src[4], const hg_handle dst[4], uint_t level,
+                         DoFType flag, int mask, UpdateType update )
+{
+   check( ctx, hg_stokes_apply( ctx, op, src, dst, int( level ), flag, mask, update ), "Stokes apply" );
+}
+} // namespace detail
+
+// P2ElementwiseBlendingFullViscousOperator (the velocity block
+// 2 eta eps(u):eps(v) - 2/3 eta div u div v of a Stokes operator, whose device
+// operator it shares; elementwiseoperators/P2ElementwiseBlendingFullViscousOperator.hpp)
+class P2ElementwiseBlendingFullViscousOperator : public Operator< P2VectorFunction< double >, P2VectorFunction< double > >,
+                                                 public OperatorWithInverseDiagonal< P2VectorFunction< double > >
+{
+ public:
+   P2ElementwiseBlendingFullViscousOperator( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h,
+                                             uint_t minLevel, uint_t maxLevel )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , h_( h )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   {}
+   void apply( const P2VectorFunction< double >& src, const P2VectorFunction< double >& dst, uint_t level, DoFType flag,
+               UpdateType updateType = Replace ) const override
+   {
+      const hg_handle s[4] = { src[0].handle(), src[1].handle(), src[2].handle(), 0 };
+      const hg_handle d[4] = { dst[0].handle(), dst[1].handle(), dst[2].handle(), 0 };
+      detail::stokesApply( ctx_, h_, s, d, level, flag, 1, updateType );
+   }
+   std::shared_ptr< P2VectorFunction< double > > getInverseDiagonalValues() const override
+   {
+      if ( !invDiag_ )
+         const_cast< P2ElementwiseBlendingFullViscousOperator* >( this )->computeInverseDiagonalOperatorValues();
+      return invDiag_;
+   }
+   void computeInverseDiagonalOperatorValues() override
+   {
+      invDiag_ = std::make_shared< P2VectorFunction< double > >( "inverseDiagonal", storage_, minLevel_, maxLevel_ );
+      for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
+      {
+         const hg_handle d[3] = { ( *invDiag_ )[0].handle(), ( *invDiag_ )[1].handle(), ( *invDiag_ )[2].handle() };
+         check( ctx_, hg_stokes_velocity_inverse_diagonal( ctx_, h_, int( L ), d ), "inverse diagonal" );
+      }
+   }
+
+ private:
+   std::shared_ptr< PrimitiveStorage >           storage_;
+   hg_ctx*                                       ctx_;
+   hg_handle                                     h_;
+   uint_t                                        minLevel_, maxLevel_;
+   std::shared_ptr< P2VectorFunction< double > > invDiag_;
+};
+
+// P2P1ElementwiseBlendingFullViscousStokesOperator
+// (elementwiseoperators/P2P1ElementwiseBlendingStokesOperator.hpp:125-216) with
+// viscosity eta(x) = exp(viscC0 + viscC1 |x|)
+class P2P1ElementwiseBlendingFullViscousStokesOperator
+: public Operator< P2P1TaylorHoodFunction< double >, P2P1TaylorHoodFunction< double > >
+{
+ public:
+   using VelocityOperator_T = P2ElementwiseBlendingFullViscousOperator;
+
+   P2P1ElementwiseBlendingFullViscousStokesOperator( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
+                                                     uint_t maxLevel, double viscC0 = 0.0, double viscC1 = 0.0 )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , owned_( true )
+   {
+      check( ctx_, hg_stokes_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), viscC0, viscC1, &h_ ),
+             "P2P1ElementwiseBlendingFullViscousStokesOperator" );
+      velocity_ = std::make_unique< VelocityOperator_T >( storage, h_, minLevel, maxLevel );
+   }
+   static std::unique_ptr< P2P1ElementwiseBlendingFullViscousStokesOperator >
+       borrow( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel, uint_t maxLevel )
+   {
+      return std::unique_ptr< P2P1ElementwiseBlendingFullViscousStokesOperator >(
+          new P2P1ElementwiseBlendingFullViscousStokesOperator( storage, h, minLevel, maxLevel ) );
+   }
+   ~P2P1ElementwiseBlendingFullViscousStokesOperator() override
+   {
+      if ( owned_ && h_ )
+         hg_operator_destroy( ctx_, h_ );
+   }
+   void apply( const P2P1TaylorHoodFunction< double >& src, const P2P1TaylorHoodFunction< double >& dst, uint_t level,
+               DoFType flag, UpdateType updateType = Replace ) const override
+   {
+      const hg_handle s[4] = { src.uvw()[0].handle(), src.uvw()[1].handle(), src.uvw()[2].handle(), src.p().handle() };
+      const hg_handle d[4] = { dst.uvw()[0].handle(), dst.uvw()[1].handle(), dst.uvw()[2].handle(), dst.p().handle() };
+      detail::stokesApply( ctx_, h_, s, d, level, flag, 7, updateType );
+   }
+   const VelocityOperator_T& getA() const { return *velocity_; }
+   // divT.apply(p, uvw): B^T p into the velocity
+   void applyDivT( const P1Function< double >& p, const P2VectorFunction< double >& dst, uint_t level, DoFType flag,
+                   UpdateType updateType = Replace ) const
+   {
+      const hg_handle s[4] = { 0, 0, 0, p.handle() };
+      const hg_handle d[4] = { dst[0].handle(), dst[1].handle(), dst[2].handle(), 0 };
+      detail::stokesApply( ctx_, h_, s, d, level, flag, 2, updateType );
+   }
+   // div.apply(uvw, p): B u into the pressure
+   void applyDiv( const P2VectorFunction< double >& u, const P1Function< double >& dst, uint_t level, DoFType flag,
+                  UpdateType updateType = Replace ) const
+   {
+      const hg_handle s[4] = { u[0].handle(), u[1].handle(), u[2].handle(), 0 };
+      const hg_handle d[4] = { 0, 0, 0, dst.handle() };
+      detail::stokesApply( ctx_, h_, s, d, level, flag, 4, updateType );
+   }
+   // values of P1PSPGInvDiagOperator (pspg_inv_diag_) and P1LumpedInvMassOperator
+   const P1Function< double >& pspgInverseDiagonal() const { return pressureDiagonal( pspgInv_, 0 ); }
+   const P1Function< double >& lumpedInverseMass() const { return pressureDiagonal( lumpedInv_, 1 ); }
+   hg_handle                   handle() const { return h_; }
+   const std::shared_ptr< PrimitiveStorage >& getStorage() const { return storage_; }
+
+ private:
+   P2P1ElementwiseBlendingFullViscousStokesOperator( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h,
+                                                     uint_t minLevel, uint_t maxLevel )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , h_( h )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , owned_( false )
+   , velocity_( std::make_unique< VelocityOperator_T >( storage, h, minLevel, maxLevel ) )
+   {}
+   const P1Function< double >& pressureDiagonal( std::unique_ptr< P1Function< double > >& f, int kind ) const
+   {
+      if ( !f )
+      {
+         f = std::make_unique< P1Function< double > >( kind ? "lumpedInvMass" : "pspgInvDiag", storage_, minLevel_,
+                                                       maxLevel_ );
+         for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
+            check( ctx_, hg_stokes_pressure_diagonal( ctx_, h_, int( L ), kind, f->handle() ), "pressure diagonal" );
+      }
+      return *f;
+   }
+   std::shared_ptr< PrimitiveStorage >                  storage_;
+   hg_ctx*                                              ctx_ = nullptr;
+   hg_handle                                            h_   = 0;
+   uint_t                                               minLevel_, maxLevel_;
+   bool                                                 owned_;
+   std::unique_ptr< VelocityOperator_T >                velocity_;
+   mutable std::unique_ptr< P1Function< double > >      pspgInv_, lumpedInv_;
+};
+
+// FullStokesVelocityBlockBlockDiagonalPreconditioner
+// (solvers/preconditioners/stokes/FullStokesVelocityBlockBlockDiagonalPreconditioner.hpp):
+// the velocity solver applied to (A.getA(), x.uvw(), b.uvw())
+template < class OperatorType >
+class FullStokesVelocityBlockBlockDiagonalPreconditioner : public Solver< OperatorType >
+{
+ public:
+   explicit FullStokesVelocityBlockBlockDiagonalPreconditioner(
+       std::shared_ptr< Solver< typename OperatorType::VelocityOperator_T > > velocitySolver )
+   : velocity_( std::move( velocitySolver ) )
+   {}
+   void solve( const OperatorType& A, const typename OperatorType::srcType& x, const typename OperatorType::dstType& b,
+               uint_t level ) override
+   {
+      velocity_->solve( A.getA(), x.uvw(), b.uvw(), level );
+   }
+
+ private:
+   std::shared_ptr< Solver< typename OperatorType::VelocityOperator_T > > velocity_;
+};
+
+// UzawaSmoother, block variant without stabilisation (solvers/UzawaSmoother.hpp:330-470):
+//   r_u = b_u - B^T p;  velocity smoothing of A u = r_u;  r_p = omega (b_p - B u);
+//   p += D_pspg^-1 r_p  (with the mean of r_p and p projected out when asked)
+template < class OperatorType >
+class UzawaSmoother : public Solver< OperatorType >
+{
+ public:
+   using FunctionType = typename OperatorType::srcType;
+
+   UzawaSmoother( const std::shared_ptr< PrimitiveStorage >& storage, std::shared_ptr< Solver< OperatorType > > velocitySmoother,
+                  uint_t minLevel, uint_t maxLevel, double relaxParam, DoFType flag = Inner | NeumannBoundary | FreeslipBoundary,
+                  uint_t numVelocityIterations = 2, bool projectPressureMean = false )
+   : velocity_( std::move( velocitySmoother ) )
+   , relax_( relaxParam )
+   , flag_( flag )
+   , numVelocity_( numVelocityIterations )
+   , project_( projectPressureMean )
+   , r_( "uzawa_smoother_r", storage, minLevel, maxLevel )
+   , tmp_( "uzawa_smoother_tmp", storage, minLevel, maxLevel )
+   {}
+   void setRelaxationParameter( double omega ) { relax_ = omega; }
+   void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      A.applyDivT( x.p(), r_.uvw(), level, flag_, Replace );
+      r_.uvw().assign( { 1.0, -1.0 }, { b.uvw(), r_.uvw() }, level, flag_ );
+      for ( uint_t i = 0; i < numVelocity_; ++i )
+         velocity_->solve( A, x, r_, level );
+      const DoFType pflag = pressureFlag( flag_ );
+      if ( !pflag )
+         return;
+      A.applyDiv( x.uvw(), r_.p(), level, flag_, Replace );
+      r_.p().assign( { relax_, -relax_ }, { b.p(), r_.p() }, level, pflag );
+      if ( project_ )
+         vertexdof::projectMean( r_.p(), level );
+      tmp_.multElementwise( { A.pspgInverseDiagonal(), r_.p() }, level, pflag );
+      x.p().add( { 1.0 }, { tmp_ }, level, pflag );
+      if ( project_ )
+         vertexdof::projectMean( x.p(), level );
+   }
+
+ private:
+   std::shared_ptr< Solver< OperatorType > > velocity_;
+   double                                    relax_;
+   DoFType                                   flag_;
+   uint_t                                    numVelocity_;
+   bool                                      project_;
+   FunctionType                              r_;
+   P1Function< double >                      tmp_;
+};
+
+// StokesPressureBlockPreconditioner<Op, P1LumpedInvMassOperator>
+// (solvers/preconditioners/stokes/StokesPressureBlockPreconditioner.hpp): identity
+// on the velocity, inverse lumped mass on the pressure
+template < class OperatorType >
+class StokesPressureBlockPreconditioner : public Solver< OperatorType >
+{
+ public:
+   void solve( const OperatorType& A, const typename OperatorType::srcType& x, const typename OperatorType::dstType& b,
+               uint_t level ) override
+   {
+      x.assign( { 1.0 }, { b }, level, flag_ );
+      x.p().multElementwise( { A.lumpedInverseMass(), b.p() }, level, pressureFlag( flag_ ) );
+   }
+
+ private:
+   DoFType flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+// MinResSolver (solvers/MinresSolver.hpp:117-245)
+template < class OperatorType >
+class MinResSolver : public Solver< OperatorType >
+{
+ public:
+   using FunctionType = typename OperatorType::srcType;
+
+   MinResSolver( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel, uint_t maxIter,
+                 double relativeTolerance, double absoluteTolerance,
+                 std::shared_ptr< Solver< OperatorType > > preconditioner )
+   : maxIter_( maxIter )
+   , relTol_( relativeTolerance )
+   , absTol_( absoluteTolerance )
+   , prec_( std::move( preconditioner ) )
+   {
+      const char* names[10] = { "minres_vm", "minres_v",  "minres_vp", "minres_z",   "minres_zp",
+                                "minres_wm", "minres_w",  "minres_wp", "minres_tmp", "minres_r" };
+      for ( int i = 0; i < 10; ++i )
+         f_[i] = std::make_unique< FunctionType >( names[i], storage, minLevel, maxLevel );
+   }
+   uint_t getIterations() const { return iterations_; }
+   void   solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      auto &vm = f_[0], &v = f_[1], &vp = f_[2], &z = f_[3], &zp = f_[4], &wm = f_[5], &w = f_[6], &wp = f_[7],
+           &r = f_[9];
+      for ( auto& f : f_ )
+         f->setToZero( level );
+      A.apply( x, *r, level, flag_ );
+      v->assign( { 1.0, -1.0 }, { b, *r }, level, flag_ );
+      prec_->solve( A, *z, *v, level );
+      double       gammaOld = 1.0;
+      double       gammaNew = std::sqrt( z->dotGlobal( *v, level, flag_ ) );
+      const double resStart = gammaNew;
+      double       eta      = gammaNew;
+      double       sOld = 0.0, sNew = 0.0, cOld = 1.0, cNew = 1.0;
+      iterations_ = maxIter_;
+      if ( gammaNew < absTol_ )
+         return;
+      for ( uint_t i = 0; i < maxIter_; ++i )
+      {
+         z->assign( { 1.0 / gammaNew }, { *z }, level, flag_ );
+         A.apply( *z, *vp, level, flag_ );
+         const double delta = vp->dotGlobal( *z, level, flag_ );
+         vp->assign( { 1.0, -delta / gammaNew, -gammaNew / gammaOld }, { *vp, *v, *vm }, level, flag_ );
+         zp->interpolate( 0, level, flag_ );
+         prec_->solve( A, *zp, *vp, level );
+         gammaOld           = gammaNew;
+         gammaNew           = std::sqrt( zp->dotGlobal( *vp, level, flag_ ) );
+         const double a0    = cNew * delta - cOld * sNew * gammaOld;
+         const double a1    = std::sqrt( a0 * a0 + gammaNew * gammaNew );
+         const double a2    = sNew * delta + cOld * cNew * gammaOld;
+         const double a3    = sOld * gammaOld;
+         cOld               = cNew;
+         cNew               = a0 / a1;
+         sOld               = sNew;
+         sNew               = gammaNew / a1;
+         wp->assign( { 1.0 / a1, -a3 / a1, -a2 / a1 }, { *z, *wm, *w }, level, flag_ );
+         x.add( { cNew * eta }, { *wp }, level, flag_ );
+         eta = -sNew * eta;
+         // (vm, v, vp) <- (v, vp, vm); (wm, w, wp) <- (w, wp, wm); (z, zp) <- (zp, z)
+         std::swap( vm, v );
+         std::swap( v, vp );
+         std::swap( wm, w );
+         std::swap( w, wp );
+         std::swap( z, zp );
+         if ( std::abs( eta ) / resStart < relTol_ || std::abs( eta ) < absTol_ )
+         {
+            iterations_ = i;
+            break;
+         }
+      }
+   }
+
+ private:
+   uint_t                                    maxIter_, iterations_ = 0;
+   double                                    relTol_, absTol_;
+   std::shared_ptr< Solver< OperatorType > > prec_;
+   std::unique_ptr< FunctionType >           f_[10];
+   DoFType                                   flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+// P2P1StokesToP2P1StokesRestriction / Prolongation
+// (gridtransferoperators/P2P1StokesToP2P1StokesRestriction.hpp, ...Prolongation.hpp):
+// quadratic transfer of each velocity component, linear transfer of the pressure
+class P2P1StokesToP2P1StokesRestriction : public RestrictionOperator< P2P1TaylorHoodFunction< double > >
+{
+ public:
+   explicit P2P1StokesToP2P1StokesRestriction( bool projectMeanAfterRestriction = false )
+   : project_( projectMeanAfterRestriction )
+   {}
+   void restrict( const P2P1TaylorHoodFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      for ( int k = 0; k < 3; ++k )
+         check( f.p().ctx(), hg_p2_restrict( f.p().ctx(), f.uvw()[k].handle(), int( sourceLevel ), flag ), "restrict" );
+      if ( pressureFlag( flag ) )
+         check( f.p().ctx(), hg_p1_restrict( f.p().ctx(), f.p().handle(), int( sourceLevel ), All ), "restrict" );
+      if ( project_ )
+         vertexdof::projectMean( f.p(), sourceLevel - 1 );
+   }
+
+ private:
+   bool project_;
+};
+class P2P1StokesToP2P1StokesProlongation : public ProlongationOperator< P2P1TaylorHoodFunction< double > >
+{
+ public:
+   explicit P2P1StokesToP2P1StokesProlongation( bool projectMeanAfterProlongation = false )
+   : project_( projectMeanAfterProlongation )
+   {}
+   void prolongate( const P2P1TaylorHoodFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      transfer( f, sourceLevel, flag, HG_REPLACE );
+   }
+   void prolongateAndAdd( const P2P1TaylorHoodFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      transfer( f, sourceLevel, flag, HG_ADD );
+   }
+
+ private:
+   void transfer( const P2P1TaylorHoodFunction< double >& f, uint_t sourceLevel, DoFType flag, int update ) const
+   {
+      hg_ctx* ctx = f.p().ctx();
+      for ( int k = 0; k < 3; ++k )
+         check( ctx, hg_p2_prolongate( ctx, f.uvw()[k].handle(), int( sourceLevel ), flag, update ), "prolongate" );
+      if ( pressureFlag( flag ) )
+         check( ctx, hg_p1_prolongate( ctx, f.p().handle(), int( sourceLevel ), All, update ), "prolongate" );
+      if ( project_ )
+         vertexdof::projectMean( f.p(), sourceLevel + 1 );
+   }
+   bool project_;
 };
 
 } // namespace hyteg_amd

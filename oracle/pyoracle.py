@@ -542,8 +542,6 @@ class StokesGMG:
         op = self.ops[L]
         if L == self.minL:
             stokes_minres(op, u, p, bu, bp, self.minv, self.coarse_iter, self.coarse_tol, 0.0, GMG_FLAG)
-            if self.project:
-                project_mean(p)
             return
         for _ in range(self.pre + self.inc * (self.maxL - L)):
             self.smooth(L, u, p, bu, bp)
