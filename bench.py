@@ -287,6 +287,57 @@ def serving_cfg5(n=400, rate=200.0):
                         "P2 cube L3-5 20%, P2 shell L2-3 15%, each solved to 1e-8", "rate_per_s": rate, **res}
 
 
+def stokes_cfg3(hg, ctx, L=5, cycles=8):
+    """P2-P1 Stokes on the shell, eta = exp(ln(1e4)(1-r)/0.45), nodal buoyancy of 12
+    Gaussian blobs on inner velocity DoFs, no-slip on both surfaces; the Uzawa GMG of
+    the free-slip solver template: V(3,3) + 2 per coarser level, omega 0.3, Chebyshev(3)
+    velocity smoothing with the radius from 25 power iterations, 10 MINRES steps on
+    level 0, pressure mean projected; `cycles` V-cycles or 1e-7 relative residual"""
+    from hyteg_amd import problems
+    st = hg.PrimitiveStorage.shell(ctx, **problems.SHELL)
+    A = hg.P2P1StokesOperator(st, 0, L, *problems.STOKES_VISCOSITY)
+    X = hg.P2P1TaylorHoodFunction("x", st, 0, L)
+    B = hg.P2P1TaylorHoodFunction("b", st, 0, L)
+    n2, n1 = st.p2_ndofs(L), st.ndofs(L)
+    t0 = time.perf_counter()
+    xyz = st.coords(L + 1)
+    f = problems.buoyancy(xyz)
+    inner = st.flags(L + 1) == hg.INNER
+    del xyz
+    B.upload(L, np.concatenate([np.where(inner, f[:, c], 0.0) for c in range(3)]), np.zeros(n1))
+    del f
+    for c in range(3):
+        X.uvw[c].interpolate_random(7 + c, -1.0, 1.0, L, hg.INNER)
+    T = hg.P2P1TaylorHoodFunction("t", st, L, L)
+    rho = hg.stokes_estimate_radius(A, X, T, L, 25)
+    T.destroy()
+    hg.stokes_gmg_solve(A, X, B, 0, L, rho, cycles=1)  # builds the per-level data once
+    X.interpolate(0.0, L)
+    ctx.synchronize()
+    setup = time.perf_counter() - t0
+    log(f"Stokes level {L}: {3 * n2 + n1} DoFs, radius {rho:.4f}, set-up {setup:.2f} s")
+    Y = hg.P2P1TaylorHoodFunction("y", st, L, L)
+    kms = kernel_ms(ctx, lambda: A.apply(B, Y, L, hg.GMG_FLAG), 3)
+    Y.destroy()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    k, hist = hg.stokes_gmg_solve(A, X, B, 0, L, rho, cycles=cycles, tol=1e-7)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"Stokes GMG level {L}: {k} cycles, {dt:.2f} s, relative residual {hist[-1] / hist[0]:.2e}")
+    for fn in (X, B):
+        fn.destroy()
+    A.destroy()
+    return {"workload": f"P2-P1 Stokes, shell level {L} ({3 * n2} velocity + {n1} pressure DoFs), "
+                        "eta = exp(ln(1e4)(1-r)/0.45), 12-blob buoyancy, no-slip, FP64 Uzawa GMG V(3,3)+2, "
+                        "Chebyshev(3) velocity smoothing, omega 0.3, MINRES(10) on level 0",
+            "dofs": 3 * n2 + n1, "cycles": k, "time_s": dt, "s_per_cycle": dt / max(k, 1), "setup_s": setup,
+            "spectral_radius": rho, "relative_residual": float(hist[-1] / hist[0]),
+            "reached_1e-7": bool(hist[-1] <= 1e-7 * hist[0]),
+            "factors": [float(v) for v in hist[1:] / hist[:-1]],
+            "stokes_apply_kernel_ms": kms, "stokes_apply_dof_per_s": (3 * n2 + n1) / (kms * 1e-3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +350,7 @@ def main():
     ap.add_argument("--no-gmg", action="store_true", help="skip the P2 GMG solve")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 64-system P1 solve")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-stokes", action="store_true", help="skip the Stokes solve")
     ap.add_argument("--only-apply", action="store_true",
                     help="run the apply measurement only (for counter collection)")
     args = ap.parse_args()
@@ -326,6 +378,8 @@ def main():
         extra["gmg_p1_cube_L5_f64"] = gmg_cfg1(hg, ctx)
         if args.workload == "p2_shell" and not args.no_batched:
             extra["batched_p1_cube_L6_x64"] = batched_cfg4(hg, ctx)
+        if args.workload == "p2_shell" and not args.no_stokes:
+            extra["stokes_shell_L5_f64"] = stokes_cfg3(hg, ctx)
         if args.workload == "p2_shell" and not args.no_serving:
             extra["serving_mixed_stream"] = serving_cfg5()
     cpu = None if (args.no_cpu or args.only_apply) else cpu_baseline(args.workload, args.cpu_seconds)

@@ -3,7 +3,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import ALL, F16, F64, GMG_FLAG, REPLACE, HygError, lib
+from ._lib import ALL, F16, F64, GMG_FLAG, INNER, REPLACE, HygError, lib
 
 
 def _ptr(a):

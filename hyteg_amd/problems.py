@@ -5,6 +5,10 @@
   (1920 macro-tetrahedra); coefficient
       k(x) = exp( sum_{m=1}^{16} a_m sin(omega_m . x + phi_m) ),
   a_m ~ N(0, 0.5^2), omega_m ~ N(0, 3^2 I), phi_m ~ U[0, 2 pi); RHS uniform[-1,1].
+* Stokes on the shell: viscosity eta(r) = exp(ln(1e4) (1 - r) / 0.45) (10^4
+  contrast between the outer and the inner surface); buoyancy T(x) x/|x| from a
+  temperature anomaly of 12 Gaussian blobs (sigma 0.05, amplitude 1) whose
+  centres are uniform in the shell volume, seeded.
 """
 import numpy as np
 
@@ -39,3 +43,29 @@ def uniform_rhs(flags, seed=0, inner=1):
     """uniform[-1,1] on DoFs of type `inner`, 0 elsewhere"""
     rng = np.random.default_rng(seed)
     return np.where(flags == inner, rng.uniform(-1.0, 1.0, len(flags)), 0.0)
+
+
+STOKES_VISCOSITY = (np.log(1e4) / 0.45, -np.log(1e4) / 0.45)  # eta = exp(c0 + c1 r)
+
+
+def blob_centres(seed=0, n=12, rmin=0.55, rmax=1.0):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = (rmin ** 3 + rng.uniform(size=n) * (rmax ** 3 - rmin ** 3)) ** (1.0 / 3.0)
+    return d * r[:, None]
+
+
+def buoyancy(x, seed=0, sigma=0.05, amplitude=1.0, chunk=1 << 22):
+    """nodal buoyancy T(x) x/|x| at points x (n x 3), T = sum of the Gaussian blobs"""
+    c = blob_centres(seed)
+    out = np.empty_like(x)
+    for a in range(0, len(x), chunk):
+        xs = x[a:a + chunk]
+        T = np.zeros(len(xs))
+        for cb in c:
+            d2 = ((xs - cb) ** 2).sum(axis=1)
+            near = d2 < 36.0 * sigma * sigma
+            T[near] += amplitude * np.exp(-d2[near] / (2.0 * sigma * sigma))
+        out[a:a + chunk] = xs * (T / np.linalg.norm(xs, axis=1))[:, None]
+    return out
