@@ -461,19 +461,17 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApply(
          atomicAdd( &sb[pos[i]], y[i] );
    }
    __syncthreads();
+   // every box point is added with an atomic that returns nothing: box-surface
+   // points are shared with neighbouring tiles, and a read-modify-write of the
+   // interior points would make the wave wait for its outstanding atomics too
+   // (the vector-memory counter covers loads and atomics alike)
    T* dstC = dst + base;
    for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
    {
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      if ( X + Y + Z > top )
-         continue;
-      T*         o      = dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
-      const bool shared = lx == 0 || ly == 0 || lz == 0 || lx == kBox - 1 || ly == kBox - 1 || lz == kBox - 1;
-      if ( shared )
-         sharedAdd( o, sb[p] );
-      else
-         *o = fromAcc< T >( toAcc( *o ) + sb[p] );
+      if ( X + Y + Z <= top )
+         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, sb[p] );
    }
 }
 

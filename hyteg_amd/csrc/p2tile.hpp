@@ -66,9 +66,8 @@ __device__ __forceinline__ void decodeMicro( int64_t wt, int64_t j, int64_t& x, 
 // (all 6 micro-cell types: 384 threads, one micro-cell each, one type per
 // wavefront).  The tile's P2 values sit on a (2 kTile + 1)^3 box of the level
 // L+1 lattice; the box is staged through LDS, element results are summed in
-// LDS, and the box is written back once: box-interior points belong to this
-// tile only (plain read-modify-write), box-surface points are shared with
-// neighbouring tiles (atomic add).
+// LDS, and the box is added back once with atomics that return nothing
+// (box-surface points are shared with neighbouring tiles).
 constexpr int kTile        = 4;
 constexpr int kBox         = 2 * kTile + 1;
 constexpr int kBox3        = kBox * kBox * kBox;

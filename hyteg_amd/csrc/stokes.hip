@@ -315,16 +315,12 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
          const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
          if ( X + Y + Z > top1 )
             continue;
-         const int64_t idx    = base1 + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
-         const bool    shared = lx == 0 || ly == 0 || lz == 0 || lx == kBox - 1 || ly == kBox - 1 || lz == kBox - 1;
+         // atomics for every point: a read-modify-write of the box interior would
+         // wait for the outstanding atomics of the box surface
+         const int64_t idx = base1 + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
 #pragma unroll
          for ( int c = 0; c < 3; ++c )
-         {
-            if ( shared )
-               sharedAdd( yout[c] + idx, su[c][p] );
-            else
-               yout[c][idx] += su[c][p];
-         }
+            sharedAdd( yout[c] + idx, su[c][p] );
       }
    }
    if constexpr ( DO_B )
@@ -334,12 +330,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
          const int32_t X = int32_t( tc.x0 ) + lx, Y = int32_t( tc.y0 ) + ly, Z = int32_t( tc.z0 ) + lz;
          if ( X + Y + Z > top0 )
             continue;
-         const int64_t idx    = base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X;
-         const bool    shared = lx == 0 || ly == 0 || lz == 0 || lx == kPBox - 1 || ly == kPBox - 1 || lz == kPBox - 1;
-         if ( shared )
-            sharedAdd( yp + idx, sp[p] );
-         else
-            yp[idx] += sp[p];
+         sharedAdd( yp + base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X, sp[p] );
       }
 }
 
