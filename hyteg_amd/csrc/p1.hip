@@ -148,7 +148,18 @@ struct P1OpLevel
    DevBuf< double >       invDiag;    // per owned vertex/edge/face DoF
    std::vector< int64_t > colorStart; // owned boundary DoFs grouped by colour
    DevBuf< int32_t >      colorRows;
+   // assembled off-diagonal rows of the owned boundary DoFs (built when they fit,
+   // see kCsrRowBudget): storage positions of the neighbours -- the owner slot of
+   // boundary neighbours, the cell-lattice slot of cell-interior ones -- and weights
+   bool              csr = false;
+   DevBuf< int64_t > csrPtr, csrPos;
+   DevBuf< double >  csrVal;
 };
+
+// boundary rows are assembled into CSR up to this many owned boundary DoFs
+// (about 40 entries of 16 bytes each: below 2.7 GB); beyond it the kernels sum
+// the per-cell stencil shares in place
+static constexpr int64_t kCsrRowBudget = int64_t( 1 ) << 22;
 
 struct P1Operator : OperatorBase
 {
@@ -207,6 +218,9 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
    // diagonal and coupling graph of the owned boundary DoFs
    std::vector< double >                 diag( lat.nb, 0.0 );
    std::vector< std::vector< int32_t > > adj( lat.nb );
+   o.csr = lat.nb > 0 && lat.nb <= kCsrRowBudget;
+   std::vector< std::vector< std::pair< int64_t, double > > > row( o.csr ? lat.nb : 0 );
+   const int64_t                                             ownerBase = lat.nc * lat.sc;
    for ( int c = 0; c < t.nc; ++c )
       for ( int64_t z = 0; z <= mm; ++z )
          for ( int64_t y = 0; y + z <= mm; ++y )
@@ -228,11 +242,42 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
                         throw Error( -4, "boundary stencil reaches outside its cell" );
                      continue;
                   }
-                  if ( W[s] == 0.0 || qc < 0 )
+                  if ( W[s] == 0.0 )
+                     continue;
+                  if ( o.csr )
+                  {
+                     const int64_t pos = qc < 0 ? int64_t( c ) * lat.sc + tetIndex( lat.w, qx, qy, qz ) :
+                                                  ownerBase + lat.num.locate( t, c, qx, qy, qz );
+                     auto& rr = row[r];
+                     auto  it = std::find_if( rr.begin(), rr.end(), [&]( const auto& e ) { return e.first == pos; } );
+                     if ( it == rr.end() )
+                        rr.emplace_back( pos, W[s] );
+                     else
+                        it->second += W[s];
+                  }
+                  if ( qc < 0 )
                      continue;
                   adj[r].push_back( int32_t( lat.num.locate( t, c, qx, qy, qz ) ) );
                }
             }
+   if ( o.csr )
+   {
+      std::vector< int64_t > ptr( lat.nb + 1, 0 ), pos;
+      std::vector< double >  val;
+      for ( int64_t r = 0; r < lat.nb; ++r )
+      {
+         std::sort( row[r].begin(), row[r].end() );
+         for ( const auto& e : row[r] )
+         {
+            pos.push_back( e.first );
+            val.push_back( e.second );
+         }
+         ptr[r + 1] = int64_t( pos.size() );
+      }
+      o.csrPtr.upload( ctx, ptr );
+      o.csrPos.upload( ctx, pos );
+      o.csrVal.upload( ctx, val );
+   }
    std::vector< double > inv( lat.nb );
    for ( int64_t r = 0; r < lat.nb; ++r )
       inv[r] = 1.0 / diag[r];
@@ -277,14 +322,18 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
 // Functions may interleave 2^lv vectors per lattice point (value k of point p
 // at (p << lv) + k); the cell kernels walk a row as (x, k) pairs with k fastest,
 // so a wavefront reads 2^lv consecutive values per stencil entry.
+// Short rows (small levels) are packed 2^rpbLog per block, each handled by
+// blockDim.x >> rpbLog threads.
 template < class T >
 __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
                             int64_t mm, const double* __restrict__ stencil, const T* __restrict__ src,
-                            T* __restrict__ dst, int add, int lv )
+                            T* __restrict__ dst, int add, int lv, int rpbLog )
 {
    using A             = typename Acc< T >::type;
-   const int64_t lb    = xcdBlock( nrows * nc );
-   if ( lb >= nrows * nc )
+   const int64_t nrc   = nrows * nc;
+   const int64_t lb    = ( xcdBlock( ( nrc + ( int64_t( 1 ) << rpbLog ) - 1 ) >> rpbLog ) << rpbLog ) +
+                      ( threadIdx.x >> ( 6 - rpbLog ) );
+   if ( lb >= nrc )
       return;
    const int64_t c    = lb / nrows;
    const int64_t r    = lb - c * nrows;
@@ -301,7 +350,8 @@ __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int
    }
    const int64_t out = base + tetRowStart( w, y, z );
    const int64_t msk = ( int64_t( 1 ) << lv ) - 1;
-   for ( int64_t t = threadIdx.x;; t += blockDim.x )
+   const int     tpr = int( blockDim.x ) >> rpbLog;
+   for ( int64_t t = threadIdx.x & ( tpr - 1 );; t += tpr )
    {
       const int64_t x = 1 + ( t >> lv ), k = t & msk;
       if ( x + y + z >= mm )
@@ -321,11 +371,13 @@ __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int
 template < class T >
 __global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
                           int64_t mm, const double* __restrict__ stencil, int cx, double relax,
-                          const T* __restrict__ b, T* __restrict__ x, int lv )
+                          const T* __restrict__ b, T* __restrict__ x, int lv, int rpbLog )
 {
-   using A          = typename Acc< T >::type;
-   const int64_t lb = xcdBlock( nrows * nc );
-   if ( lb >= nrows * nc )
+   using A           = typename Acc< T >::type;
+   const int64_t nrc = nrows * nc;
+   const int64_t lb  = ( xcdBlock( ( nrc + ( int64_t( 1 ) << rpbLog ) - 1 ) >> rpbLog ) << rpbLog ) +
+                      ( threadIdx.x >> ( 6 - rpbLog ) );
+   if ( lb >= nrc )
       return;
    const int64_t c    = lb / nrows;
    const int64_t r    = lb - c * nrows;
@@ -343,7 +395,8 @@ __global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows, int64
    const A       invc = A( 1 ) / st[0];
    const int64_t x0   = cx ? 1 : 2;
    const int64_t msk  = ( int64_t( 1 ) << lv ) - 1;
-   for ( int64_t t = threadIdx.x;; t += blockDim.x )
+   const int     tpr  = int( blockDim.x ) >> rpbLog;
+   for ( int64_t t = threadIdx.x & ( tpr - 1 );; t += tpr )
    {
       const int64_t xx = x0 + 2 * ( t >> lv ), k = t & msk;
       if ( xx + y + z >= mm )
@@ -430,6 +483,51 @@ __global__ void kBndSOR( BndGeom g, int64_t n, const int32_t* __restrict__ list,
    forEachIncident( g, b, [&]( int64_t c, int64_t xx, int64_t yy, int64_t zz, int ) {
       x[( ( c * g.sc + tetIndex( g.w, xx, yy, zz ) ) << lv ) + vec] = nv;
    } );
+}
+
+// apply / SOR on owned boundary DoFs from their assembled rows: the neighbour
+// loads are independent, so a thread issues them together instead of walking
+// the incident cells one stencil entry at a time
+template < class T >
+__global__ void kBndApplyCsr( int64_t nb, int64_t ownerBase, const uint8_t* __restrict__ bndFlag, int flag,
+                              const int64_t* __restrict__ ptr, const int64_t* __restrict__ pos,
+                              const double* __restrict__ val, const double* __restrict__ invDiag,
+                              const T* __restrict__ src, T* __restrict__ dst, int add, int lv )
+{
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t b   = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
+   if ( b >= nb || !( bndFlag[b] & flag ) )
+      return;
+   const int64_t o = ( ( ownerBase + b ) << lv ) + vec;
+   A             s = A( 1.0 / invDiag[b] ) * toAcc( src[o] );
+   for ( int64_t j = ptr[b]; j < ptr[b + 1]; ++j )
+      s += A( val[j] ) * toAcc( src[( pos[j] << lv ) + vec] );
+   if ( add )
+      s += toAcc( dst[o] );
+   dst[o] = fromAcc< T >( s );
+}
+
+template < class T >
+__global__ void kBndSORCsr( int64_t n, const int32_t* __restrict__ list, int64_t ownerBase,
+                            const uint8_t* __restrict__ bndFlag, int flag, const int64_t* __restrict__ ptr,
+                            const int64_t* __restrict__ pos, const double* __restrict__ val,
+                            const double* __restrict__ invDiag, double relax, const T* __restrict__ rhs,
+                            T* __restrict__ x, int lv )
+{
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
+   if ( i >= n )
+      return;
+   const int64_t b = list[i];
+   if ( !( bndFlag[b] & flag ) )
+      return;
+   const int64_t o = ( ( ownerBase + b ) << lv ) + vec;
+   A             s = toAcc( rhs[o] );
+   for ( int64_t j = ptr[b]; j < ptr[b + 1]; ++j )
+      s -= A( val[j] ) * toAcc( x[( pos[j] << lv ) + vec] );
+   x[o] = fromAcc< T >( ( A( 1 ) - A( relax ) ) * toAcc( x[o] ) + A( relax ) * A( invDiag[b] ) * s );
 }
 
 // restriction to coarse cell-interior points (full weighting, R = P^T)
@@ -571,7 +669,22 @@ __global__ void kProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_t*
 // ------------------------------------------------------------------------
 // host drivers
 // ------------------------------------------------------------------------
-static constexpr int kRowBlock = 64;
+static constexpr int kRowBlock = 64; // the cell kernels assume 64 (log2 6)
+
+// log2 of the rows one 64-thread block of a cell kernel takes: enough lanes for
+// `lanes` values per row, at most 64 rows
+static int rowsPerBlockLog( int64_t lanes )
+{
+   int r = 0;
+   while ( r < 6 && ( int64_t( kRowBlock ) >> ( r + 1 ) ) >= lanes )
+      ++r;
+   return r;
+}
+
+static unsigned cellGrid( int64_t rowsTimesCells, int rpbLog )
+{
+   return xcdGrid( ( rowsTimesCells + ( int64_t( 1 ) << rpbLog ) - 1 ) >> rpbLog );
+}
 
 static P1OpLevel& levelOf( Context* ctx, P1Operator& op, int L )
 {
@@ -602,7 +715,14 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
    pullHalos( ctx, *op.st, lat, s, src.dtype, lv );
    dispatchType( src.dtype, [&]( auto tag ) {
       using T = decltype( tag );
-      if ( lat.nb > 0 )
+      if ( lat.nb > 0 && o.csr )
+      {
+         kBndApplyCsr< T ><<< gridFor( lat.nb << lv ), kBlock, 0, ctx->stream >>>(
+             lat.nb, lat.nc * lat.sc, lat.bndFlag.p, flag, o.csrPtr.p, o.csrPos.p, o.csrVal.p, o.invDiag.p,
+             (const T*) s, (T*) d, update, lv );
+         HG_CHECK_HIP( hipGetLastError() );
+      }
+      else if ( lat.nb > 0 )
       {
          kBndApply< T ><<< gridFor( lat.nb << lv ), kBlock, 0, ctx->stream >>>( BndGeom::of( *op.st, lat ), lat.nb,
                                                                                lat.nc, lat.bndFlag.p, flag,
@@ -613,8 +733,10 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
       if ( ( flag & kInner ) && lat.nrows > 0 )
       {
          ctx->beginTiming();
-         kCellApply< T ><<< xcdGrid( lat.nrows * lat.nc ), kRowBlock, 0, ctx->stream >>>(
-             lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update, lv );
+         const int rpb = rowsPerBlockLog( ( lat.mm - 3 ) << lv );
+         kCellApply< T ><<< cellGrid( lat.nrows * lat.nc, rpb ), kRowBlock, 0, ctx->stream >>>(
+             lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update, lv,
+             rpb );
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
       }
@@ -643,12 +765,21 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
          const int64_t n = o.colorStart[cc + 1] - o.colorStart[cc];
          if ( n == 0 )
             continue;
-         kBndSOR< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>( g, n, o.colorRows.p + o.colorStart[cc],
-                                                                        lat.nc, lat.bndFlag.p, flag, o.clsStencil.p,
-                                                                        o.invDiag.p, relax, (const T*) bp, (T*) xp,
-                                                                        lv );
+         if ( o.csr )
+            kBndSORCsr< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>(
+                n, o.colorRows.p + o.colorStart[cc], lat.nc * lat.sc, lat.bndFlag.p, flag, o.csrPtr.p, o.csrPos.p,
+                o.csrVal.p, o.invDiag.p, relax, (const T*) bp, (T*) xp, lv );
+         else
+            kBndSOR< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>( g, n, o.colorRows.p + o.colorStart[cc],
+                                                                           lat.nc, lat.bndFlag.p, flag, o.clsStencil.p,
+                                                                           o.invDiag.p, relax, (const T*) bp, (T*) xp,
+                                                                           lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
+      // the CSR sweep updated the owners only: refresh the cell copies the
+      // interior sweep reads
+      if ( o.csr && !o.colorStart.empty() )
+         pullHalos( ctx, *op.st, lat, xp, x.dtype, lv );
       if ( !( flag & kInner ) || lat.nrows == 0 )
          return;
       for ( int cz = 0; cz < 2; ++cz )
@@ -659,9 +790,10 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
                const int64_t n   = lat.rowGroup[grp + 1] - lat.rowGroup[grp];
                if ( n == 0 )
                   continue;
-               kCellSOR< T ><<< xcdGrid( n * lat.nc ), kRowBlock, 0, ctx->stream >>>(
+               const int rpb = rowsPerBlockLog( ( ( lat.mm - 2 ) / 2 ) << lv );
+               kCellSOR< T ><<< cellGrid( n * lat.nc, rpb ), kRowBlock, 0, ctx->stream >>>(
                    lat.rows.p + 2 * lat.rowGroup[grp], n, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, cx, relax,
-                   (const T*) bp, (T*) xp, lv );
+                   (const T*) bp, (T*) xp, lv, rpb );
                HG_CHECK_HIP( hipGetLastError() );
             }
    } );
