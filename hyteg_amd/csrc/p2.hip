@@ -14,6 +14,7 @@
 // the element matrix of the reference applied to the element vector without
 // forming the 10x10 matrix.
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "abi_util.hpp"
@@ -715,9 +716,10 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
       if ( !out || minLevel < 0 || maxLevel < minLevel || maxLevel > 11 || nm < 0 || nm > 32 ||
            ( nm > 0 && ( !a || !omega || !phi ) ) )
          throw Error( HG_ERR_ARG, "invalid operator parameters" );
-      static bool quadReady = false;
-      if ( !quadReady )
-      {
+      // the quadrature constants are uploaded once per process (several host
+      // threads may create operators at once)
+      static std::once_flag quadOnce;
+      std::call_once( quadOnce, [] {
          double w[11];
          quadratureWeights( w );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQP ), kQP, sizeof( kQP ) ) );
@@ -744,8 +746,7 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQWf ), wf, sizeof( wf ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4f ), q4f, sizeof( q4f ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4mf ), q4mf, sizeof( q4mf ) ) );
-         quadReady = true;
-      }
+      } );
       Storage&        st = ctx->storage( s );
       const Topology& t  = st.topo;
       auto            op = std::make_unique< P2Operator >();

@@ -18,6 +18,7 @@
 // a fine point shared by several macro-cells contributes 1/(number of cells
 // sharing its primitive) from each, and restricted values of coarse DoFs on
 // macro vertices/edges/faces are the sum over the incident cells.
+#include <mutex>
 #include <array>
 #include <vector>
 
@@ -128,12 +129,9 @@ struct Tables
    int    rN[8]{};
 };
 
-const Tables& tables()
+Tables buildTables()
 {
-   static Tables T;
-   static bool   built = false;
-   if ( built )
-      return T;
+   Tables T;
    (void) kShift;
    // pull stencils: fine point F = 4 A + r, offsets of the coarse nodes from 2 A
    for ( int rz = 0; rz < 4; ++rz )
@@ -175,23 +173,29 @@ const Tables& tables()
                   }
             }
    }
-   built = true;
+   return T;
+}
+
+// built once per process; contexts on several host threads may ask at once
+// (function-local statics are initialised exactly once, the others wait)
+const Tables& tables()
+{
+   static const Tables T = buildTables();
    return T;
 }
 
 void uploadTables()
 {
-   static bool done = false;
-   if ( done )
-      return;
-   const Tables& T = tables();
-   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPOff ), T.pOff, sizeof( T.pOff ) ) );
-   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPW ), T.pW, sizeof( T.pW ) ) );
-   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPN ), T.pN, sizeof( T.pN ) ) );
-   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cROff ), T.rOff, sizeof( T.rOff ) ) );
-   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRW ), T.rW, sizeof( T.rW ) ) );
-   HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRN ), T.rN, sizeof( T.rN ) ) );
-   done = true;
+   static std::once_flag once;
+   std::call_once( once, [] {
+      const Tables& T = tables();
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPOff ), T.pOff, sizeof( T.pOff ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPW ), T.pW, sizeof( T.pW ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPN ), T.pN, sizeof( T.pN ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cROff ), T.rOff, sizeof( T.rOff ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRW ), T.rW, sizeof( T.rW ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRN ), T.rN, sizeof( T.rN ) ) );
+   } );
 }
 
 } // namespace

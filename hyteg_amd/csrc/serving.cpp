@@ -102,6 +102,11 @@ void coefficientOf( uint64_t seed, std::vector< double >& a, std::vector< double
 
 int64_t latticeSize( int64_t w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
 
+// P2 requests are solved one at a time per worker; their small kernels leave
+// the device mostly idle, so a class runs several workers (own context and
+// stream each) while requests are queued
+constexpr int kP2Workers = 6;
+
 // device bytes of a class hierarchy (functions of all levels plus operator
 // tables), generous estimate used for admission
 int64_t footprint( int kind, int level, int slots )
@@ -390,7 +395,7 @@ void startWorkers( Server& sv )
    {
       if ( q.empty() )
          continue;
-      const int want = ( cls.first == P2Shell || cls.first == P2Cube ) ? 2 : 1;
+      const int want = ( cls.first == P2Shell || cls.first == P2Cube ) ? kP2Workers : 1;
       while ( sv.live[cls] < want && ( sv.live[cls] == 0 || q.size() > size_t( sv.live[cls] ) ) )
       {
          const int64_t bytes = footprint( cls.first, cls.second, sv.slots );

@@ -21,6 +21,7 @@
 // power remains in the divergence blocks.
 #include <array>
 #include <cmath>
+#include <mutex>
 
 #include "abi_util.hpp"
 #include "halo.hpp"
@@ -655,9 +656,9 @@ int hg_stokes_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, double
       if ( !out || minLevel < 0 || maxLevel < minLevel || maxLevel > 9 || !std::isfinite( viscC0 ) ||
            !std::isfinite( viscC1 ) )
          throw Error( HG_ERR_ARG, "invalid Stokes operator parameters" );
-      static bool quadReady = false;
-      if ( !quadReady )
-      {
+      // uploaded once per process (several host threads may create operators at once)
+      static std::once_flag quadOnce;
+      std::call_once( quadOnce, [] {
          double q4[9][4], q4m[9][4];
          for ( int q = 0; q < 9; ++q )
             for ( int k = 0; k < 4; ++k )
@@ -669,8 +670,7 @@ int hg_stokes_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, double
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQ4m ), q4m, sizeof( q4m ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQL ), kSQP, sizeof( kSQP ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQW ), kSQW, sizeof( kSQW ) ) );
-         quadReady = true;
-      }
+      } );
       Storage& st = ctx->storage( s );
       auto     op = std::make_unique< StokesOperator >();
       op->st      = &st;

@@ -66,3 +66,36 @@ def test_restriction_is_transpose_of_pull():
         C = k[1]
         if all(2 <= c < 6 for c in C) and all(0 <= f < 8 for f in k[0]):
             assert k in rest and abs(rest[k] - wt) < 1e-15, k
+
+
+_CONCURRENT_FIRST_USE = r"""
+import sys, threading
+sys.path.insert(0, sys.argv[1])
+import hyteg_amd as hg
+barrier = threading.Barrier(16)
+out = {}
+def work(i):
+    barrier.wait()
+    out[i] = [len(hg.p2_transfer_stencil(0, c)[1]) for c in range(64)] + \
+             [len(hg.p2_transfer_stencil(1, c)[1]) for c in range(8)]
+ts = [threading.Thread(target=work, args=(i,)) for i in range(16)]
+for t in ts: t.start()
+for t in ts: t.join()
+print(sorted(set(tuple(v) for v in out.values())))
+"""
+
+
+def test_tables_built_once_under_concurrent_first_use():
+    """the tables are built lazily; threads that ask for them at the same time
+    (one context per serving worker) must all see the single-threaded tables"""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    ref = [len(hg.p2_transfer_stencil(0, c)[1]) for c in range(64)] + \
+          [len(hg.p2_transfer_stencil(1, c)[1]) for c in range(8)]
+    for _ in range(3):
+        r = subprocess.run([sys.executable, "-c", _CONCURRENT_FIRST_USE, root], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout.strip() == str([tuple(ref)]), r.stdout
