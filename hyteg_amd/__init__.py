@@ -10,7 +10,7 @@ CPU fallback: if the library is missing, importing `lib()` raises.
 from ._lib import (ALL, DIRICHLET, F16, F32, F64, FREESLIP, GMG_FLAG, INNER, NEUMANN, ADD, REPLACE,
                    HygError, lib, library_path)
 from .api import (Context, P1ConstantLaplaceOperator, P1Function, P2DivKGradOperator, P2Function,
-                  P2P1StokesOperator, P2P1TaylorHoodFunction, PrimitiveStorage, batched_gmg_solve, batched_ir_solve, cg_solve, gmg_solve, p2_cg_solve, p2_chebyshev_smooth, p2_estimate_radius,
+                  P2P1StokesOperator, P2P1TaylorHoodFunction, P2ProjectNormalOperator, PrimitiveStorage, batched_gmg_solve, batched_ir_solve, cg_solve, gmg_solve, p2_cg_solve, p2_chebyshev_smooth, p2_estimate_radius,
                   p2_gmg_solve, p2_ir_solve, p2_transfer_stencil, prolongate, restrict, stokes_estimate_radius,
                   stokes_gmg_solve, stokes_minres_solve, stokes_uzawa_smooth)
 
@@ -18,7 +18,7 @@ __all__ = [
     "ALL", "DIRICHLET", "F16", "F32", "F64", "FREESLIP", "GMG_FLAG", "INNER", "NEUMANN", "ADD", "REPLACE",
     "HygError", "lib", "library_path", "Context", "P1ConstantLaplaceOperator", "P1Function",
     "P2DivKGradOperator", "P2Function", "P2P1StokesOperator", "P2P1TaylorHoodFunction",
-    "PrimitiveStorage", "batched_gmg_solve", "batched_ir_solve", "cg_solve", "gmg_solve", "p2_cg_solve", "p2_chebyshev_smooth", "p2_estimate_radius",
+    "P2ProjectNormalOperator", "PrimitiveStorage", "batched_gmg_solve", "batched_ir_solve", "cg_solve", "gmg_solve", "p2_cg_solve", "p2_chebyshev_smooth", "p2_estimate_radius",
     "p2_gmg_solve", "p2_ir_solve", "p2_transfer_stencil", "prolongate", "restrict", "stokes_estimate_radius",
     "stokes_gmg_solve", "stokes_minres_solve", "stokes_uzawa_smooth",
 ]

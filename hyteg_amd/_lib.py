@@ -50,6 +50,7 @@ SIGNATURES = {
     "hg_function_communicate": [_p, _i64, _i],
     "hg_p1_laplace_create": [_p, _i64, _i, _i, _p],
     "hg_storage_shell": [_p, _i, _i, _d, _d, _p],
+    "hg_storage_shell_bc": [_p, _i, _i, _d, _d, _i, _i, _p],
     "hg_storage_blend": [_p, _i64, _p],
     "hg_storage_cell_vertices": [_p, _i64, _p],
     "hg_storage_vertex_coords": [_p, _i64, _p],
@@ -76,10 +77,12 @@ SIGNATURES = {
     "hg_stokes_apply": [_p, _i64, _p, _p, _i, _i, _i, _i],
     "hg_stokes_velocity_inverse_diagonal": [_p, _i64, _i, _p],
     "hg_stokes_pressure_diagonal": [_p, _i64, _i, _i, _i64],
-    "hg_stokes_estimate_radius": [_p, _i64, _p, _p, _i, _i, _p],
-    "hg_stokes_uzawa_smooth": [_p, _i64, _p, _p, _i, _d, _i, _i, _d, _i],
-    "hg_stokes_minres_solve": [_p, _i64, _p, _p, _i, _i, _d, _d, _p],
-    "hg_stokes_gmg_solve": [_p, _i64, _p, _p, _i, _i, _i, _i, _i, _d, _i, _i, _d, _i, _d, _i, _i, _d, _p, _p],
+    "hg_p2_project_normal": [_p, _p, _i, _i],
+    "hg_stokes_freeslip_apply": [_p, _i64, _p, _p, _i, _i, _i, _i],
+    "hg_stokes_estimate_radius": [_p, _i64, _p, _p, _i, _i, _i, _p],
+    "hg_stokes_uzawa_smooth": [_p, _i64, _p, _p, _i, _d, _i, _i, _d, _i, _i],
+    "hg_stokes_minres_solve": [_p, _i64, _p, _p, _i, _i, _d, _d, _i, _p],
+    "hg_stokes_gmg_solve": [_p, _i64, _p, _p, _i, _i, _i, _i, _i, _d, _i, _i, _d, _i, _d, _i, _i, _i, _d, _p, _p],
     "hg_function_copy_vector": [_p, _i64, _i, _i64, _i, _i],
     "hg_function_dot_batched": [_p, _i64, _i64, _i, _i, _p],
     "hg_function_scale_batched": [_p, _i64, _p, _i, _i],

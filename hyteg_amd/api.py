@@ -3,7 +3,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import ALL, F16, F64, GMG_FLAG, INNER, REPLACE, HygError, lib
+from ._lib import ALL, DIRICHLET, F16, F64, FREESLIP, GMG_FLAG, INNER, REPLACE, HygError, lib
 
 
 def _ptr(a):
@@ -71,9 +71,15 @@ class PrimitiveStorage:
         return PrimitiveStorage(ctx, h.value)
 
     @staticmethod
-    def shell(ctx, ntan=5, nrad=3, rmin=0.55, rmax=1.0):
+    def shell(ctx, ntan=5, nrad=3, rmin=0.55, rmax=1.0, inner_flag=None, outer_flag=None):
+        """meshSphericalShell with the icosahedral shell map; inner_flag / outer_flag
+        (DIRICHLET, NEUMANN or FREESLIP) set the boundary type of each surface"""
         h = C.c_int64()
-        ctx.check(lib().hg_storage_shell(ctx.h, ntan, nrad, rmin, rmax, C.byref(h)), "meshSphericalShell")
+        if inner_flag is None and outer_flag is None:
+            ctx.check(lib().hg_storage_shell(ctx.h, ntan, nrad, rmin, rmax, C.byref(h)), "meshSphericalShell")
+        else:
+            ctx.check(lib().hg_storage_shell_bc(ctx.h, ntan, nrad, rmin, rmax, inner_flag or DIRICHLET,
+                                                outer_flag or DIRICHLET, C.byref(h)), "meshSphericalShell")
         return PrimitiveStorage(ctx, h.value)
 
     def blend(self):
@@ -454,15 +460,31 @@ class P2P1TaylorHoodFunction:
         self.p.add_const(-self.p.sum(level, ALL) / n, level, ALL)
 
 
+class P2ProjectNormalOperator:
+    """u -= (u.n) n with the radial normal n = x/|x| on the P2 DoFs whose boundary type is in flag"""
+
+    def __init__(self, storage):
+        self.ctx = storage.ctx
+
+    def project(self, f, level, flag=FREESLIP):
+        """f: a P2P1TaylorHoodFunction (its velocity is projected) or a list of three P2Functions"""
+        comps = f.uvw if hasattr(f, "uvw") else f
+        h = np.ascontiguousarray([c.h for c in comps], dtype=np.int64)
+        self.ctx.check(lib().hg_p2_project_normal(self.ctx.h, _ptr(h), level, flag), "P2ProjectNormalOperator")
+
+
 class P2P1StokesOperator:
     """P2P1ElementwiseBlendingFullViscousStokesOperator with viscosity
-    eta(x) = exp(c0 + c1 |x|) at the physical (blended) point"""
+    eta(x) = exp(c0 + c1 |x|) at the physical (blended) point.  freeslip=True
+    makes it the free-slip wrapper (P A P, P B^T, B P, pressure mean projected out
+    of apply) and selects the projections of the free-slip solver layout."""
 
     A_U, BT_P, B_U = 1, 2, 4
 
-    def __init__(self, storage, min_level, max_level, visc_c0=0.0, visc_c1=0.0):
+    def __init__(self, storage, min_level, max_level, visc_c0=0.0, visc_c1=0.0, freeslip=False):
         self.storage = storage
         self.ctx = storage.ctx
+        self.freeslip = bool(freeslip)
         h = C.c_int64()
         self.ctx.check(lib().hg_stokes_create(self.ctx.h, storage.h, min_level, max_level, float(visc_c0),
                                               float(visc_c1), C.byref(h)), "P2P1StokesOperator")
@@ -480,9 +502,11 @@ class P2P1StokesOperator:
             pass
 
     def apply(self, src, dst, level, flag, update=REPLACE, mask=7):
-        """mask bit 1: dst.uvw = A src.uvw, 2: dst.uvw += B^T src.p, 4: dst.p = B src.uvw"""
-        self.ctx.check(lib().hg_stokes_apply(self.ctx.h, self.h, _ptr(src.handles()), _ptr(dst.handles()), level,
-                                             flag, mask, update), "Stokes apply")
+        """mask bit 1: dst.uvw = A src.uvw, 2: dst.uvw += B^T src.p, 4: dst.p = B src.uvw
+        (free-slip wrapper: masks 1, 2, 4 or 7)"""
+        fn = lib().hg_stokes_freeslip_apply if self.freeslip else lib().hg_stokes_apply
+        self.ctx.check(fn(self.ctx.h, self.h, _ptr(src.handles()), _ptr(dst.handles()), level, flag, mask, update),
+                       "Stokes apply")
 
     def velocity_inverse_diagonal(self, level, dst):
         self.ctx.check(lib().hg_stokes_velocity_inverse_diagonal(self.ctx.h, self.h, level, _ptr(dst.handles())),
@@ -499,30 +523,32 @@ def stokes_estimate_radius(A, x, tmp, level, iterations=25):
     """power iteration on D^-1 A of the velocity block; x.uvw is the start vector"""
     r = C.c_double()
     A.ctx.check(lib().hg_stokes_estimate_radius(A.ctx.h, A.h, _ptr(x.handles()), _ptr(tmp.handles()), level,
-                                                iterations, C.byref(r)), "Stokes estimateRadius")
+                                                iterations, int(A.freeslip), C.byref(r)), "Stokes estimateRadius")
     return r.value
 
 
 def stokes_uzawa_smooth(A, x, b, level, rho, omega=0.3, velocity_iterations=1, order=3, project=True):
     A.ctx.check(lib().hg_stokes_uzawa_smooth(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), level, omega,
-                                             velocity_iterations, order, rho, int(project)), "Uzawa smoother")
+                                             velocity_iterations, order, rho, int(project), int(A.freeslip)),
+                "Uzawa smoother")
 
 
 def stokes_minres_solve(A, x, b, level, max_iter=10, rel_tol=1e-6, abs_tol=0.0):
     its = C.c_int()
     A.ctx.check(lib().hg_stokes_minres_solve(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), level, max_iter,
-                                             rel_tol, abs_tol, C.byref(its)), "MINRES")
+                                             rel_tol, abs_tol, int(A.freeslip), C.byref(its)), "MINRES")
     return its.value
 
 
 def stokes_gmg_solve(A, x, b, min_level, max_level, rho, pre=3, post=3, smooth_increment=2, omega=0.3,
                      velocity_iterations=1, order=3, coarse_max_iter=10, coarse_tol=1e-6, project=True, cycles=10,
                      tol=0.0):
-    """stokesGMGUzawaFSSolver layout (no free-slip); returns (cycles done, residual history)"""
+    """stokesGMGUzawaFSSolver layout (with its free-slip projections when A.freeslip);
+    returns (cycles done, residual history)"""
     hist = np.zeros(cycles + 1)
     done = C.c_int()
     A.ctx.check(lib().hg_stokes_gmg_solve(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), min_level, max_level,
                                           pre, post, smooth_increment, omega, velocity_iterations, order, rho,
-                                          coarse_max_iter, coarse_tol, int(project), cycles, tol, _ptr(hist),
-                                          C.byref(done)), "Stokes GMG")
+                                          coarse_max_iter, coarse_tol, int(project), int(A.freeslip), cycles, tol,
+                                          _ptr(hist), C.byref(done)), "Stokes GMG")
     return done.value, hist[:done.value + 1]

@@ -649,6 +649,23 @@ int hg_storage_shell( hg_ctx* h, int ntan, int nrad, double rmin, double rmax, h
    } );
 }
 
+int hg_storage_shell_bc( hg_ctx* h, int ntan, int nrad, double rmin, double rmax, int innerFlag, int outerFlag,
+                         hg_handle* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      const int bnd = kDirichlet | kNeumann | kFreeslip;
+      if ( !out || ( innerFlag & ~bnd ) || ( outerFlag & ~bnd ) || !innerFlag || !outerFlag ||
+           ( innerFlag & ( innerFlag - 1 ) ) || ( outerFlag & ( outerFlag - 1 ) ) )
+         throw Error( HG_ERR_ARG, "boundary flags must each be one of Dirichlet, Neumann, FreeSlip" );
+      auto st  = std::make_unique< Storage >();
+      st->topo = Topology::shell( ntan, nrad, rmin, rmax );
+      st->topo.setShellBoundaryFlags( 0.5 * ( rmin + rmax ), innerFlag, outerFlag );
+      st->init( ctx );
+      *out                 = ctx->nextHandle++;
+      ctx->storages[*out] = std::move( st );
+   } );
+}
+
 int hg_storage_blend( hg_ctx* h, hg_handle s, double* out14 )
 {
    return hg_guard( h, [&]( Context* ctx ) {

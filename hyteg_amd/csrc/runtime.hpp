@@ -91,6 +91,7 @@ struct Lattice
    DevBuf< uint8_t >  bndFlag;  // DoFType of each owned vertex/edge/face DoF
    DevBuf< uint32_t > cellMask; // bit set on cell-owned (interior) lattice points
    DevBuf< double >   ones;     // nstore ones (sums as dot products), built on first use
+   DevBuf< double >   bndNormal; // x/|x| of each owned boundary DoF (radial normals), built on first use
    std::vector< uint8_t > hBndFlag;
 
    // interior rows (y,z) of a cell lattice, y,z >= 1, y+z <= mm-2, grouped by

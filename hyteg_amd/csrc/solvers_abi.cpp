@@ -287,20 +287,58 @@ int batchedIR( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel
 } // namespace
 
 namespace {
+using TaylorHood = P2P1TaylorHoodFunction< double >;
+
 // the Uzawa smoother of the free-slip GMG template (StokesFSGMGUzawaSolverTemplate.hpp:113-170):
 // Chebyshev smoothing of the velocity block with coefficients from the spectral
-// radius, inside FullStokesVelocityBlockBlockDiagonalPreconditioner
-std::shared_ptr< UzawaSmoother< StokesOp > > stokesUzawa( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel,
-                                                          int maxLevel, double omega, int velocityIterations,
-                                                          int chebyshevOrder, double spectralRadius, bool project )
+// radius, inside FullStokesVelocityBlockBlockDiagonalPreconditioner; with a
+// projection, the Chebyshev corrections and the velocity residual are projected
+template < class Op >
+std::shared_ptr< UzawaSmoother< Op > > stokesUzawa( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel,
+                                                    int maxLevel, double omega, int velocityIterations, int chebyshevOrder,
+                                                    double spectralRadius, bool project,
+                                                    const std::shared_ptr< P2ProjectNormalOperator >& projection )
 {
-   auto cheby = std::make_shared< ChebyshevSmoother< StokesOp::VelocityOperator_T > >( storage, uint_t( minLevel ),
-                                                                                        uint_t( maxLevel ) );
+   using Cheby = ChebyshevSmoother< typename Op::VelocityOperator_T, P2ProjectNormalOperator >;
+   const uint_t minL = uint_t( minLevel ), maxL = uint_t( maxLevel );
+   auto         cheby = projection ? std::make_shared< Cheby >( storage, minL, maxL, false, projection, FreeslipBoundary ) :
+                                     std::make_shared< Cheby >( storage, minL, maxL );
    cheby->setupCoefficients( uint_t( chebyshevOrder ), spectralRadius );
-   auto velocity = std::make_shared< FullStokesVelocityBlockBlockDiagonalPreconditioner< StokesOp > >( cheby );
-   return std::make_shared< UzawaSmoother< StokesOp > >( storage, velocity, uint_t( minLevel ), uint_t( maxLevel ), omega,
-                                                         Inner | NeumannBoundary | FreeslipBoundary,
-                                                         uint_t( velocityIterations ), project );
+   auto          velocity = std::make_shared< FullStokesVelocityBlockBlockDiagonalPreconditioner< Op > >( cheby );
+   const DoFType flag     = DoFType( Inner | NeumannBoundary | FreeslipBoundary );
+   if ( projection )
+      return std::make_shared< UzawaSmootherWithFreeSlipProjection< Op > >( storage, velocity, minL, maxL, omega, flag,
+                                                                             projection, uint_t( velocityIterations ),
+                                                                             project );
+   return std::make_shared< UzawaSmoother< Op > >( storage, velocity, minL, maxL, omega, flag,
+                                                   uint_t( velocityIterations ), project );
+}
+
+// runs body(A, projection) on the elementwise Stokes operator, or (freeslip) on
+// its free-slip wrapper P2P1FullStokesFreeslipWrapper with the radial projection
+template < class Body >
+void withStokesOperator( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle op, uint_t minLevel,
+                         uint_t maxLevel, bool freeslip, Body&& body )
+{
+   std::shared_ptr< StokesOp > A( StokesOp::borrow( storage, op, minLevel, maxLevel ) );
+   if ( !freeslip )
+   {
+      body( *A, std::shared_ptr< P2ProjectNormalOperator >() );
+      return;
+   }
+   auto                          projection = std::make_shared< P2ProjectNormalOperator >();
+   P2P1FullStokesFreeslipWrapper W( storage, minLevel, maxLevel, A, projection );
+   body( W, projection );
+}
+
+// residual 2-norm of a Stokes system over Inner | Neumann | FreeSlip
+template < class Op >
+double stokesResidualNorm( const Op& A, const TaylorHood& x, const TaylorHood& b, const TaylorHood& r, uint_t level )
+{
+   const DoFType flag = DoFType( Inner | NeumannBoundary | FreeslipBoundary );
+   A.apply( x, r, level, flag, Replace );
+   r.assign( { 1.0, -1.0 }, { b, r }, level, flag );
+   return std::sqrt( r.dotGlobal( r, level, flag ) );
 }
 } // namespace
 
@@ -500,91 +538,138 @@ int hg_p2_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int min
    } );
 }
 
+int hg_stokes_freeslip_apply( hg_ctx* ctx, hg_handle op, const hg_handle* src, const hg_handle* dst, int level,
+                              int flag, int mask, int update )
+{
+   if ( !ctx || !src || !dst || level < 0 || ( mask != 1 && mask != 2 && mask != 4 && mask != 7 ) ||
+        ( update != HG_REPLACE && update != HG_ADD ) )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      const hg_handle any     = ( mask == 2 ) ? src[3] : src[0];
+      auto            storage = storageOf( ctx, any );
+      withStokesOperator( storage, op, uint_t( level ), uint_t( level ), true, [&]( const auto& A, const auto& ) {
+         const uint_t     L = uint_t( level );
+         const DoFType    f = DoFType( flag );
+         const UpdateType u = UpdateType( update );
+         if ( mask == 7 )
+            A.apply( TaylorHood( ctx, src ), TaylorHood( ctx, dst ), L, f, u );
+         else if ( mask == 1 )
+            A.getA().apply( P2VectorFunction< double >( ctx, src ), P2VectorFunction< double >( ctx, dst ), L, f, u );
+         else if ( mask == 2 )
+            A.applyDivT( *P1Function< double >::borrow( ctx, src[3] ), P2VectorFunction< double >( ctx, dst ), L, f, u );
+         else
+            A.applyDiv( P2VectorFunction< double >( ctx, src ), *P1Function< double >::borrow( ctx, dst[3] ), L, f, u );
+      } );
+   } );
+}
+
 int hg_stokes_estimate_radius( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* tmp, int level,
-                               int iterations, double* radius )
+                               int iterations, int freeslip, double* radius )
 {
    if ( !ctx || !x || !tmp || !radius || level < 0 || iterations < 1 )
       return HG_ERR_ARG;
    return guarded( ctx, [&] {
       auto storage = storageOf( ctx, x[0] );
-      auto A       = StokesOp::borrow( storage, op, uint_t( level ), uint_t( level ) );
-      P2VectorFunction< double > X( ctx, x ), T( ctx, tmp );
-      *radius = chebyshev::estimateRadius( A->getA(), uint_t( level ), uint_t( iterations ), X, T );
+      withStokesOperator( storage, op, uint_t( level ), uint_t( level ), freeslip != 0, [&]( const auto& A, const auto& ) {
+         P2VectorFunction< double > X( ctx, x ), T( ctx, tmp );
+         *radius = chebyshev::estimateRadius( A.getA(), uint_t( level ), uint_t( iterations ), X, T );
+      } );
    } );
 }
 
 int hg_stokes_uzawa_smooth( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int level, double omega,
-                            int velocityIterations, int chebyshevOrder, double spectralRadius, int projectPressureMean )
+                            int velocityIterations, int chebyshevOrder, double spectralRadius, int projectPressureMean,
+                            int freeslip )
 {
    if ( !ctx || !x || !b || level < 0 || velocityIterations < 0 || chebyshevOrder < 1 || chebyshevOrder > 16 ||
         !( spectralRadius > 0 ) )
       return HG_ERR_ARG;
    return guarded( ctx, [&] {
       auto storage = storageOf( ctx, x[0] );
-      auto A       = StokesOp::borrow( storage, op, uint_t( level ), uint_t( level ) );
-      P2P1TaylorHoodFunction< double > X( ctx, x ), B( ctx, b );
-      auto uzawa = stokesUzawa( storage, level, level, omega, velocityIterations, chebyshevOrder, spectralRadius,
-                                projectPressureMean != 0 );
-      uzawa->solve( *A, X, B, uint_t( level ) );
+      withStokesOperator( storage, op, uint_t( level ), uint_t( level ), freeslip != 0,
+                          [&]( const auto& A, const auto& projection ) {
+                             using Op = std::decay_t< decltype( A ) >;
+                             TaylorHood X( ctx, x ), B( ctx, b );
+                             auto       uzawa = stokesUzawa< Op >( storage, level, level, omega, velocityIterations,
+                                                             chebyshevOrder, spectralRadius, projectPressureMean != 0,
+                                                             projection );
+                             uzawa->solve( A, X, B, uint_t( level ) );
+                          } );
    } );
 }
 
 int hg_stokes_minres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int level, int maxIter,
-                            double relTol, double absTol, int* iterations )
+                            double relTol, double absTol, int freeslip, int* iterations )
 {
    if ( !ctx || !x || !b || level < 0 || maxIter < 0 )
       return HG_ERR_ARG;
    return guarded( ctx, [&] {
       auto storage = storageOf( ctx, x[0] );
-      auto A       = StokesOp::borrow( storage, op, uint_t( level ), uint_t( level ) );
-      P2P1TaylorHoodFunction< double > X( ctx, x ), B( ctx, b );
-      MinResSolver< StokesOp >         minres( storage, uint_t( level ), uint_t( level ), uint_t( maxIter ), relTol, absTol,
-                                               std::make_shared< StokesPressureBlockPreconditioner< StokesOp > >() );
-      minres.solve( *A, X, B, uint_t( level ) );
-      if ( iterations )
-         *iterations = int( minres.getIterations() );
+      withStokesOperator( storage, op, uint_t( level ), uint_t( level ), freeslip != 0, [&]( const auto& A, const auto& ) {
+         using Op = std::decay_t< decltype( A ) >;
+         TaylorHood          X( ctx, x ), B( ctx, b );
+         MinResSolver< Op > minres( storage, uint_t( level ), uint_t( level ), uint_t( maxIter ), relTol, absTol,
+                                    std::make_shared< StokesPressureBlockPreconditioner< Op > >() );
+         minres.solve( A, X, B, uint_t( level ) );
+         if ( iterations )
+            *iterations = int( minres.getIterations() );
+      } );
    } );
 }
 
 int hg_stokes_gmg_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel, int maxLevel,
                          int pre, int post, int smoothIncrement, double omega, int velocityIterations, int chebyshevOrder,
-                         double spectralRadius, int coarseMaxIter, double coarseTol, int projectPressureMean, int cycles,
-                         double tol, double* res_hist, int* cycles_done )
+                         double spectralRadius, int coarseMaxIter, double coarseTol, int projectPressureMean, int freeslip,
+                         int cycles, double tol, double* res_hist, int* cycles_done )
 {
    if ( !ctx || !x || !b || minLevel < 0 || maxLevel < minLevel || pre < 0 || post < 0 || smoothIncrement < 0 ||
         velocityIterations < 0 || chebyshevOrder < 1 || chebyshevOrder > 16 || !( spectralRadius > 0 ) ||
         coarseMaxIter < 0 || cycles < 0 )
       return HG_ERR_ARG;
    return guarded( ctx, [&] {
-      auto storage = storageOf( ctx, x[0] );
-      auto A       = StokesOp::borrow( storage, op, uint_t( minLevel ), uint_t( maxLevel ) );
-      P2P1TaylorHoodFunction< double > X( ctx, x ), B( ctx, b );
-      auto uzawa  = stokesUzawa( storage, minLevel, maxLevel, omega, velocityIterations, chebyshevOrder, spectralRadius,
-                                 projectPressureMean != 0 );
-      auto coarse = std::make_shared< MinResSolver< StokesOp > >(
-          storage, uint_t( minLevel ), uint_t( minLevel ), uint_t( coarseMaxIter ), coarseTol, 0.0,
-          std::make_shared< StokesPressureBlockPreconditioner< StokesOp > >() );
-      GeometricMultigridSolver< StokesOp > gmg( storage, uzawa, coarse, std::make_shared< P2P1StokesToP2P1StokesRestriction >(),
-                                                std::make_shared< P2P1StokesToP2P1StokesProlongation >(), uint_t( minLevel ),
-                                                uint_t( maxLevel ), uint_t( pre ), uint_t( post ), uint_t( smoothIncrement ) );
-      P2P1TaylorHoodFunction< double > r( "residual", storage, uint_t( maxLevel ), uint_t( maxLevel ) );
-      const uint_t                     L  = uint_t( maxLevel );
-      const double                     r0 = residualNorm( *A, X, B, r, L );
-      if ( res_hist )
-         res_hist[0] = r0;
-      int k = 0;
-      for ( double rk = r0; k < cycles && !( tol > 0 && rk <= tol * r0 ); )
-      {
-         gmg.solve( *A, X, B, L );
-         if ( projectPressureMean )
-            vertexdof::projectMean( X.p(), L );
-         rk = residualNorm( *A, X, B, r, L );
-         ++k;
+      auto         storage = storageOf( ctx, x[0] );
+      const uint_t minL = uint_t( minLevel ), L = uint_t( maxLevel );
+      withStokesOperator( storage, op, minL, L, freeslip != 0, [&]( const auto& A, const auto& projection ) {
+         using Op = std::decay_t< decltype( A ) >;
+         TaylorHood X( ctx, x ), B( ctx, b );
+         auto       uzawa  = stokesUzawa< Op >( storage, minLevel, maxLevel, omega, velocityIterations, chebyshevOrder,
+                                         spectralRadius, projectPressureMean != 0, projection );
+         auto       coarse = std::make_shared< MinResSolver< Op > >(
+             storage, minL, minL, uint_t( coarseMaxIter ), coarseTol, 0.0,
+             std::make_shared< StokesPressureBlockPreconditioner< Op > >() );
+         std::shared_ptr< RestrictionOperator< TaylorHood > >  restriction;
+         std::shared_ptr< ProlongationOperator< TaylorHood > > prolongation;
+         if ( projection )
+         {
+            restriction  = std::make_shared< P2P1StokesToP2P1StokesRestrictionWithFreeSlipProjection >( projection );
+            prolongation = std::make_shared< P2P1StokesToP2P1StokesProlongationWithFreeSlipProjection >(
+                std::make_shared< TaylorHood >( "tmpProlongation", storage, minL, L ), projection );
+         }
+         else
+         {
+            restriction  = std::make_shared< P2P1StokesToP2P1StokesRestriction >();
+            prolongation = std::make_shared< P2P1StokesToP2P1StokesProlongation >();
+         }
+         GeometricMultigridSolver< Op > gmg( storage, uzawa, coarse, restriction, prolongation, minL, L, uint_t( pre ),
+                                             uint_t( post ), uint_t( smoothIncrement ) );
+         TaylorHood   r( "residual", storage, L, L );
+         const double r0 = stokesResidualNorm( A, X, B, r, L );
          if ( res_hist )
-            res_hist[k] = rk;
-      }
-      if ( cycles_done )
-         *cycles_done = k;
+            res_hist[0] = r0;
+         int k = 0;
+         for ( double rk = r0; k < cycles && !( tol > 0 && rk <= tol * r0 ); )
+         {
+            gmg.solve( A, X, B, L );
+            if ( projectPressureMean )
+               vertexdof::projectMean( X.p(), L );
+            rk = stokesResidualNorm( A, X, B, r, L );
+            ++k;
+            if ( res_hist )
+               res_hist[k] = rk;
+         }
+         if ( cycles_done )
+            *cycles_done = k;
+      } );
    } );
 }
 

@@ -461,6 +461,23 @@ __global__ void kAddConst( int64_t ncell, int64_t nb, const uint8_t* __restrict_
    v[pt] += value;
 }
 
+// u -= (u . n) n on the owned boundary DoFs whose type is in flag
+// (P2ProjectNormalOperator::project, p1functionspace/freeslip/VertexDoFProjectNormal.hpp:43-120)
+__global__ void kProjectNormal( int64_t nb, int64_t ncell, const uint8_t* __restrict__ bndFlag, int flag,
+                                const double* __restrict__ nrm, double* __restrict__ u0, double* __restrict__ u1,
+                                double* __restrict__ u2 )
+{
+   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nb || !( bndFlag[b] & flag ) )
+      return;
+   const double* n = nrm + 3 * b;
+   const int64_t p = ncell + b;
+   const double  d = n[0] * u0[p] + n[1] * u1[p] + n[2] * u2[p];
+   u0[p] -= d * n[0];
+   u1[p] -= d * n[1];
+   u2[p] -= d * n[2];
+}
+
 struct StokesOperator : OperatorBase
 {
    Storage*                                            st   = nullptr;
@@ -638,6 +655,46 @@ const double* affineInverseDiagonal( Context* ctx, StokesOperator& op, int L, in
    return b.p;
 }
 
+// x/|x| of the owned boundary DoFs of a lattice: the outer normal of a sphere
+// centred at the origin, which the shell map keeps (it only scales radii)
+const double* boundaryNormals( Context* ctx, Storage& st, Lattice& lat )
+{
+   if ( lat.bndNormal.p )
+      return lat.bndNormal.p;
+   const Topology&       t  = st.topo;
+   const int64_t         mm = lat.mm;
+   std::vector< double > n( 3 * size_t( lat.nb ), 0.0 );
+   std::vector< char >   done( size_t( lat.nb ), 0 );
+   for ( int c = 0; c < t.nc; ++c )
+   {
+      const double* v[4];
+      for ( int k = 0; k < 4; ++k )
+         v[k] = &t.X[3 * t.cv[c][k]];
+      for ( int64_t z = 0; z <= mm; ++z )
+         for ( int64_t y = 0; y + z <= mm; ++y )
+            for ( int64_t x = 0; x + y + z <= mm; x = ( y > 0 && z > 0 && x == 0 && mm - y - z > 0 ) ? mm - y - z : x + 1 )
+            {
+               const int64_t id = lat.num.locate( t, c, x, y, z );
+               if ( id >= lat.nb || done[id] )
+                  continue;
+               done[id] = 1;
+               double p[3], r = 0.0;
+               for ( int d = 0; d < 3; ++d )
+               {
+                  p[d] = v[0][d] + ( double( x ) / double( mm ) ) * ( v[1][d] - v[0][d] ) +
+                         ( double( y ) / double( mm ) ) * ( v[2][d] - v[0][d] ) +
+                         ( double( z ) / double( mm ) ) * ( v[3][d] - v[0][d] );
+                  r += p[d] * p[d];
+               }
+               r = std::sqrt( r );
+               for ( int d = 0; d < 3; ++d )
+                  n[3 * id + d] = p[d] / r;
+            }
+   }
+   lat.bndNormal.upload( ctx, n );
+   return lat.bndNormal.p;
+}
+
 void copyInto( Context* ctx, const double* src, Function& f, int L, size_t n )
 {
    HG_CHECK_HIP( hipMemcpyAsync( f.ptr( L ), src, n * sizeof( double ), hipMemcpyDeviceToDevice, ctx->stream ) );
@@ -742,6 +799,35 @@ int hg_stokes_pressure_diagonal( hg_ctx* h, hg_handle op, int level, int kind, h
       checkPressure( o, f, level );
       const double* inv = affineInverseDiagonal( ctx, o, level, kind );
       copyInto( ctx, inv, f, level, size_t( o.st->lattice( ctx, level ).nstore ) );
+   } );
+}
+
+int hg_p2_project_normal( hg_ctx* h, const hg_handle* u, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !u )
+         throw Error( HG_ERR_ARG, "null function list" );
+      // the normal is defined on boundary DoFs only (P2ProjectNormalOperator aborts on a non-boundary face)
+      if ( flag & ( kInner | ~kAll ) )
+         throw Error( HG_ERR_ARG, "normal projection flag must select boundary types only" );
+      Function* f[3];
+      for ( int c = 0; c < 3; ++c )
+      {
+         f[c] = &ctx->function( u[c] );
+         if ( f[c]->off != 1 || f[c]->lv != 0 || f[c]->dtype != 0 || f[c]->st != f[0]->st || level < f[c]->minL ||
+              level > f[c]->maxL )
+            throw Error( HG_ERR_ARG, "normal projection needs three FP64 P2 functions of one storage" );
+      }
+      Storage& st  = *f[0]->st;
+      Lattice& lat = st.lattice( ctx, level + 1 );
+      if ( lat.nb == 0 )
+         return;
+      const double* nrm = boundaryNormals( ctx, st, lat );
+      kProjectNormal<<< gridFor( lat.nb ), kBlock, 0, ctx->stream >>>( lat.nb, lat.nc * lat.sc, lat.bndFlag.p, flag, nrm,
+                                                                       (double*) f[0]->ptr( level ),
+                                                                       (double*) f[1]->ptr( level ),
+                                                                       (double*) f[2]->ptr( level ) );
+      HG_CHECK_HIP( hipGetLastError() );
    } );
 }
 

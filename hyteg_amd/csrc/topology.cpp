@@ -351,6 +351,27 @@ int shellVertexIndex( int ntan, int nrad, int is1, int is2, int id, int ir )
 
 } // namespace
 
+void Topology::setShellBoundaryFlags( double rmid, int innerFlag, int outerFlag )
+{
+   for ( int f = 0; f < nf; ++f )
+   {
+      if ( fnc[f] != 1 )
+         continue;
+      const auto v = fv[f];
+      double     r = 0.0;
+      for ( int j = 0; j < 3; ++j )
+         r += std::sqrt( X[3 * v[j]] * X[3 * v[j]] + X[3 * v[j] + 1] * X[3 * v[j] + 1] + X[3 * v[j] + 2] * X[3 * v[j] + 2] ) /
+              3.0;
+      const int flag = r < rmid ? innerFlag : outerFlag;
+      fflag[f]       = flag;
+      for ( int j = 0; j < 3; ++j )
+         vflag[v[j]] = flag;
+      eflag[findEdge( v[0], v[1] )] = flag;
+      eflag[findEdge( v[0], v[2] )] = flag;
+      eflag[findEdge( v[1], v[2] )] = flag;
+   }
+}
+
 Topology Topology::shell( int ntan, int nrad, double rmin, double rmax )
 {
    if ( ntan < 2 || nrad < 2 || !( rmin > 0.0 && rmin < rmax ) )

@@ -55,6 +55,10 @@ struct Topology
    // MeshInfo::meshSphericalShell(ntan, nrad, rmin, rmax) (SHELLMESH_CLASSIC) and
    // IcosahedralShellMap::setMap on every cell
    static Topology shell( int ntan, int nrad, double rmin, double rmax );
+   // boundary flags of a shell by surface: primitives of boundary faces nearer
+   // the inner radius get innerFlag, the others outerFlag (the mesh boundary
+   // flags HyTeG applications set with setMeshBoundaryFlagsInner/Outer)
+   void setShellBoundaryFlags( double rmid, int innerFlag, int outerFlag );
    void            computeShellBlend();
 
    int findEdge( int a, int b ) const;

@@ -92,6 +92,13 @@ int hg_storage_create( hg_ctx* ctx, int nv, const double* coords, int nc, const 
  * (mesh/MeshInfo.hpp:512-516, mesh/MeshGenSphericalShell.cpp:642-700) and
  * IcosahedralShellMap::setMap on all cells (geometry/IcosahedralShellMap.hpp:491) */
 int hg_storage_shell( hg_ctx* ctx, int ntan, int nrad, double rmin, double rmax, hg_handle* out );
+/* the same shell with the boundary flags of its two surfaces: the primitives of
+ * the inner surface get innerFlag, those of the outer one outerFlag, each one of
+ * HG_DIRICHLET, HG_NEUMANN, HG_FREESLIP (the setMeshBoundaryFlagsOnBoundary pair
+ * with MeshInfo::hollowFlag::flagInnerBoundary / flagOuterBoundary and a
+ * BoundaryCondition mapping them, mesh/MeshInfo.hpp:52-58) */
+int hg_storage_shell_bc( hg_ctx* ctx, int ntan, int nrad, double rmin, double rmax, int innerFlag, int outerFlag,
+                         hg_handle* out );
 /* per macro-cell shell map data: rayVertex, refVertex, thrVertex, forVertex,
  * radRayVertex, radRefVertex (14 doubles per cell, IcosahedralShellMap.hpp:608-731) */
 int hg_storage_blend( hg_ctx* ctx, hg_handle storage, double* out14 );
@@ -334,29 +341,47 @@ int hg_stokes_velocity_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, c
  * kind 1: P1LumpedInvMassOperator values (the MINRES pressure preconditioner)
  * (constant_stencil_operator/P1ConstantOperator.hpp:195-215), affine geometry */
 int hg_stokes_pressure_diagonal( hg_ctx* ctx, hg_handle op, int level, int kind, hg_handle dst );
+/* P2ProjectNormalOperator::project (p2functionspace/P2ProjectNormalOperator.hpp:34-61)
+ * with the radial normal n = x/|x|: u -= (u.n) n on the P2 DoFs of u = {u_x, u_y, u_z}
+ * (FP64) whose boundary type is in flag */
+int hg_p2_project_normal( hg_ctx* ctx, const hg_handle* u, int level, int flag );
+/* the free-slip wrapper of the Stokes operator (P2P1FullStokesFreeslipWrapperTemplate
+ * with P2ProjectNormalOperator, terraneo/operators/P2P1StokesOperatorWithWrapper.hpp:41-386),
+ * P = the normal projection on FreeslipBoundary DoFs: mask 1: dst.uvw = P A P src.uvw,
+ * 2: dst.uvw = P B^T src.p, 4: dst.p = B P src.uvw, 7: the full apply on (P u, p),
+ * P on the velocity result, and the pressure mean of dst projected out */
+int hg_stokes_freeslip_apply( hg_ctx* ctx, hg_handle op, const hg_handle* src, const hg_handle* dst, int level,
+                              int flag, int mask, int update );
+/* In the solver calls below freeslip != 0 selects the free-slip wrapper as the
+ * operator, and with it the projections of stokesGMGUzawaFSSolver: Chebyshev
+ * corrections (ChebyshevSmoother<., P2ProjectNormalOperator>), the Uzawa velocity
+ * residual (UzawaSmootherWithFreeSlipProjection) and both grid transfers
+ * (P2P1StokesToP2P1Stokes{Restriction,Prolongation}WithFreeSlipProjection). */
 /* chebyshev::estimateRadius on the velocity block (ChebyshevSmoother.hpp:656-665):
  * x = {x_x, x_y, x_z} is the start vector and returns the last iterate */
 int hg_stokes_estimate_radius( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* tmp, int level,
-                               int iterations, double* radius );
-/* one UzawaSmoother::solve (solvers/UzawaSmoother.hpp:330-470, UzawaSmootherWithFreeSlipProjection
- * without projection) with a Chebyshev velocity smoother of the given order and
- * radius inside FullStokesVelocityBlockBlockDiagonalPreconditioner */
+                               int iterations, int freeslip, double* radius );
+/* one UzawaSmoother::solve (solvers/UzawaSmoother.hpp:330-470, or
+ * UzawaSmootherWithFreeSlipProjection, UzawaSmoother.hpp:480-613) with a Chebyshev
+ * velocity smoother of the given order and radius inside
+ * FullStokesVelocityBlockBlockDiagonalPreconditioner */
 int hg_stokes_uzawa_smooth( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int level, double omega,
-                            int velocityIterations, int chebyshevOrder, double spectralRadius, int projectPressureMean );
+                            int velocityIterations, int chebyshevOrder, double spectralRadius, int projectPressureMean,
+                            int freeslip );
 /* MinResSolver with StokesPressureBlockPreconditioner<P1LumpedInvMassOperator>
  * (solvers/MinresSolver.hpp:117-245, solvertemplates/StokesSolverTemplates.hpp:54-69) */
 int hg_stokes_minres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int level, int maxIter,
-                            double relTol, double absTol, int* iterations );
+                            double relTol, double absTol, int freeslip, int* iterations );
 /* GeometricMultigridSolver<Stokes> as stokesGMGUzawaFSSolver builds it
- * (solvertemplates/StokesFSGMGUzawaSolverTemplate.hpp:63-190, without free-slip
- * projections): Uzawa smoothing, P2P1StokesToP2P1Stokes transfers, V(pre,post)
- * with smoothIncrement more steps per coarser level, MINRES on minLevel.  Runs
- * V-cycles (projecting the pressure mean after each when asked) until the Inner
- * residual 2-norm drops by tol or `cycles` ran; res_hist[0..done] */
+ * (solvertemplates/StokesFSGMGUzawaSolverTemplate.hpp:63-190): Uzawa smoothing,
+ * P2P1StokesToP2P1Stokes transfers, V(pre,post) with smoothIncrement more steps
+ * per coarser level, MINRES on minLevel.  Runs V-cycles (projecting the pressure
+ * mean after each when asked) until the residual 2-norm over
+ * Inner | Neumann | FreeSlip drops by tol or `cycles` ran; res_hist[0..done] */
 int hg_stokes_gmg_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel, int maxLevel,
                          int pre, int post, int smoothIncrement, double omega, int velocityIterations, int chebyshevOrder,
-                         double spectralRadius, int coarseMaxIter, double coarseTol, int projectPressureMean, int cycles,
-                         double tol, double* res_hist, int* cycles_done );
+                         double spectralRadius, int coarseMaxIter, double coarseTol, int projectPressureMean,
+                         int freeslip, int cycles, double tol, double* res_hist, int* cycles_done );
 
 #ifdef __cplusplus
 }

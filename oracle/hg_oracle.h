@@ -149,6 +149,10 @@ void or_stokes_inv_diag( const or_stokesop* op, double* out );
 /* P1PSPGInvDiagOperator and P1LumpedInvMassOperator (affine geometry) of one level */
 void or_p1_pspg_inv_diag( const or_mesh* m, int level, double* out );
 void or_p1_lumped_inv_mass( const or_mesh* m, int level, double* out );
+/* boundary flags by surface: boundary faces with mean vertex radius < rmid get innerFlag */
+void or_mesh_set_shell_flags( or_mesh* m, double rmid, int innerFlag, int outerFlag );
+/* P2ProjectNormalOperator::project with the radial normal x/|x| on DoFs of type flag */
+void or_p2_project_normal( const or_mesh* m, int level, double* u, int flag );
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@
  */
 #include "hg_oracle_internal.h"
 
+#include <stdio.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -415,3 +416,66 @@ static void p1_affine_inv_diag( const or_mesh* m, int level, int kind, double* o
 
 void or_p1_pspg_inv_diag( const or_mesh* m, int level, double* out ) { p1_affine_inv_diag( m, level, 0, out ); }
 void or_p1_lumped_inv_mass( const or_mesh* m, int level, double* out ) { p1_affine_inv_diag( m, level, 1, out ); }
+
+/* boundary flags by surface (setMeshBoundaryFlagsInner/Outer of the shell
+ * applications): primitives of boundary faces whose mean vertex radius is
+ * below rmid get innerFlag, the others outerFlag */
+void or_mesh_set_shell_flags( or_mesh* m, double rmid, int innerFlag, int outerFlag )
+{
+   for ( int f = 0; f < m->nf; ++f )
+   {
+      if ( m->fnc[f] != 1 )
+         continue;
+      const int* v = m->fv + 3 * f;
+      double     r = 0.0;
+      for ( int j = 0; j < 3; ++j )
+      {
+         const double* x = m->X + 3 * v[j];
+         r += sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] ) / 3.0;
+      }
+      const int flag = r < rmid ? innerFlag : outerFlag;
+      m->fflag[f]    = flag;
+      for ( int j = 0; j < 3; ++j )
+         m->vflag[v[j]] = flag;
+      for ( int e = 0; e < m->ne; ++e )
+      {
+         const int a = m->ev[2 * e], b = m->ev[2 * e + 1];
+         int       na = 0;
+         for ( int j = 0; j < 3; ++j )
+            na += ( v[j] == a ) + ( v[j] == b );
+         if ( na == 2 )
+            m->eflag[e] = flag;
+      }
+   }
+}
+
+/* P2ProjectNormalOperator::project with the radial normal x/|x|
+ * (p1functionspace/freeslip/VertexDoFProjectNormal.hpp:43-120): u -= (u.n) n
+ * on the P2 DoFs of level `level` whose type is in flag; u = [ux|uy|uz] */
+void or_p2_project_normal( const or_mesh* m, int level, double* u, int flag )
+{
+   if ( flag & OR_INNER )
+   {
+      fprintf( stderr, "or_p2_project_normal: the flag must select boundary types only\n" );
+      abort();
+   }
+   const lvl_t   l  = or_lvl_info( m, level + 1 );
+   const int64_t n2 = l.n;
+   double*       X  = malloc( sizeof( double ) * 3 * n2 );
+   int32_t*      fl = malloc( sizeof( int32_t ) * n2 );
+   or_p1_coords( m, level + 1, X );
+   or_p1_flags( m, level + 1, fl );
+   for ( int64_t i = 0; i < l.coff; ++i )
+   {
+      if ( !( fl[i] & flag ) )
+         continue;
+      const double* x = X + 3 * i;
+      const double  r = sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
+      const double  n[3] = { x[0] / r, x[1] / r, x[2] / r };
+      const double  d    = n[0] * u[i] + n[1] * u[n2 + i] + n[2] * u[2 * n2 + i];
+      for ( int c = 0; c < 3; ++c )
+         u[c * n2 + i] -= d * n[c];
+   }
+   free( X );
+   free( fl );
+}

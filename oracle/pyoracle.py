@@ -81,6 +81,8 @@ def lib():
             "or_stokes_inv_diag": (None, [_p, _p]),
             "or_p1_pspg_inv_diag": (None, [_p, _i, _p]),
             "or_p1_lumped_inv_mass": (None, [_p, _i, _p]),
+            "or_mesh_set_shell_flags": (None, [_p, _d, _i, _i]),
+            "or_p2_project_normal": (None, [_p, _i, _p, _i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -103,8 +105,16 @@ class Mesh:
         return Mesh(lib().or_mesh_cuboid(*lo, *hi, *n))
 
     @staticmethod
-    def shell(ntan=5, nrad=3, rmin=0.55, rmax=1.0):
-        return Mesh(lib().or_mesh_shell(ntan, nrad, rmin, rmax))
+    def shell(ntan=5, nrad=3, rmin=0.55, rmax=1.0, inner_flag=None, outer_flag=None):
+        m = Mesh(lib().or_mesh_shell(ntan, nrad, rmin, rmax))
+        if inner_flag is not None or outer_flag is not None:
+            lib().or_mesh_set_shell_flags(m.h, 0.5 * (rmin + rmax), inner_flag or DIRICHLET, outer_flag or DIRICHLET)
+        return m
+
+    def p2_project_normal(self, level, u, flag=FREESLIP):
+        """u -= (u.n) n with n = x/|x| on P2 DoFs of type flag; u = [ux|uy|uz] (in place)"""
+        assert u.flags["C_CONTIGUOUS"] and u.dtype == np.float64
+        lib().or_p2_project_normal(self.h, level, _ptr(u), flag)
 
     def blend(self):
         out = np.zeros((self.counts()[3], 14))
@@ -348,9 +358,10 @@ class StokesOp:
 
     A_U, BT_P, B_U = 1, 2, 4
 
-    def __init__(self, mesh, level, c0=0.0, c1=0.0):
+    def __init__(self, mesh, level, c0=0.0, c1=0.0, freeslip=False):
         self.mesh = mesh
         self.level = level
+        self.freeslip = freeslip
         self.h = lib().or_stokesop_create(mesh.h, level, c0, c1)
         self.n2 = mesh.ndofs(level + 1)
         self.n1 = mesh.ndofs(level)
@@ -362,8 +373,41 @@ class StokesOp:
             self.h = None
 
     def apply(self, u, p, yu, yp, flag, mask=7, add=False):
+        if self.freeslip:
+            self._apply_freeslip(u, p, yu, yp, flag, mask, add)
+            return
+        self.apply_raw(u, p, yu, yp, flag, mask, add)
+
+    def apply_raw(self, u, p, yu, yp, flag, mask=7, add=False):
         lib().or_stokes_apply(self.h, None if u is None else _ptr(u), None if p is None else _ptr(p),
                               None if yu is None else _ptr(yu), None if yp is None else _ptr(yp), flag, mask, int(add))
+
+    def project(self, u, flag=FREESLIP):
+        """P2ProjectNormalOperator::project on [ux|uy|uz] (in place)"""
+        self.mesh.p2_project_normal(self.level, u, flag)
+
+    def _apply_freeslip(self, u, p, yu, yp, flag, mask, add):
+        """P2P1FullStokesFreeslipWrapperTemplate (terraneo/operators/P2P1StokesOperatorWithWrapper.hpp:41-386):
+        mask 1: P A P u, 2: P B^T p, 4: B P u, 7: the blocks on (P u, p), P on the velocity
+        result, pressure mean of the result projected out; results written on flag"""
+        assert mask in (1, 2, 4, 7)
+        tu = None
+        if mask & 5:
+            tu = u.copy()
+            self.project(tu)
+        if mask == 4:
+            self.apply_raw(tu, None, None, yp, flag, 4, add)
+            return
+        du = np.zeros(3 * self.n2)
+        dp = np.zeros(self.n1) if mask == 7 else None
+        self.apply_raw(tu, p if mask & 2 else None, du, dp, flag, mask, False)
+        self.project(du)
+        m = self.vmask(flag)
+        yu[m] = yu[m] + du[m] if add else du[m]
+        if mask == 7:
+            if flag & INNER:
+                yp[:] = yp + dp if add else dp
+            project_mean(yp)
 
     def inv_diag(self):
         out = np.zeros(3 * self.n2)
@@ -391,11 +435,15 @@ def stokes_chebyshev(op, inv_diag, coeffs, u, bu, flag=GMG_FLAG):
     t2[m] = bu[m] - t2[m]
     t1 = np.zeros_like(u)
     t1[m] = inv_diag[m] * t2[m]
+    if op.freeslip:  # ChebyshevSmoother<., P2ProjectNormalOperator> (ChebyshevSmoother.hpp:186-231)
+        op.project(t1)
     u[m] += coeffs[0] * t1[m]
     for k in range(1, len(coeffs)):
         op.apply(t1, None, t2, None, flag, StokesOp.A_U)
         t1[:] = 0.0
         t1[m] = inv_diag[m] * t2[m]
+        if op.freeslip:
+            op.project(t1)
         u[m] += coeffs[k] * t1[m]
 
 
@@ -421,6 +469,8 @@ def uzawa_smooth(op, u, p, bu, bp, inv_diag, coeffs, pspg_inv, omega, vel_iter=1
     ru = np.zeros_like(u)
     op.apply(None, p, ru, None, flag, StokesOp.BT_P)
     ru[m] = bu[m] - ru[m]
+    if op.freeslip:  # UzawaSmootherWithFreeSlipProjection (UzawaSmoother.hpp:549-611)
+        op.project(ru)
     for _ in range(vel_iter):
         stokes_chebyshev(op, inv_diag, coeffs, u, ru, flag)
     if not flag & INNER:
@@ -519,11 +569,13 @@ class StokesGMG:
     StokesFSGMGUzawaSolverTemplate.hpp:63-190, without free-slip projections)"""
 
     def __init__(self, mesh, min_level, max_level, c0=0.0, c1=0.0, order=3, rho=None, omega=0.3, pre=3, post=3,
-                 inc=2, vel_iter=1, coarse_iter=10, coarse_tol=1e-6, power_its=25, seed=42, project=True):
+                 inc=2, vel_iter=1, coarse_iter=10, coarse_tol=1e-6, power_its=25, seed=42, project=True,
+                 freeslip=False):
         self.m, self.minL, self.maxL = mesh, min_level, max_level
         self.pre, self.post, self.inc, self.omega, self.vel_iter = pre, post, inc, omega, vel_iter
         self.coarse_iter, self.coarse_tol, self.project = coarse_iter, coarse_tol, project
-        self.ops = {L: StokesOp(mesh, L, c0, c1) for L in range(min_level, max_level + 1)}
+        self.freeslip = freeslip
+        self.ops = {L: StokesOp(mesh, L, c0, c1, freeslip) for L in range(min_level, max_level + 1)}
         self.invd = {L: self.ops[L].inv_diag() for L in self.ops}
         self.pspg = {L: p1_pspg_inv_diag(mesh, L) for L in self.ops}
         self.minv = p1_lumped_inv_mass(mesh, min_level)
@@ -554,13 +606,31 @@ class StokesGMG:
             cu[k * c.n2:(k + 1) * c.n2] = coarse
         cp = np.zeros(c.n1)
         self.m.restrict(L, rp, cp, ALL)
+        if self.freeslip:  # P2P1StokesToP2P1StokesRestrictionWithFreeSlipProjection
+            c.project(cu)
+            project_mean(cp)
         xu, xp = np.zeros_like(cu), np.zeros_like(cp)
         self.cycle(L - 1, xu, xp, cu, cp)
-        for k in range(3):
-            fine = np.ascontiguousarray(u[k * op.n2:(k + 1) * op.n2])
-            self.m.p2_prolongate(L - 1, np.ascontiguousarray(xu[k * c.n2:(k + 1) * c.n2]), fine, GMG_FLAG, True)
-            u[k * op.n2:(k + 1) * op.n2] = fine
-        self.m.prolongate(L - 1, xp, p, ALL, True)
+        if self.freeslip:
+            # P2P1StokesToP2P1StokesProlongationWithFreeSlipProjection::prolongateAndAdd:
+            # prolongate a copy, project it, add it on the flag
+            tu, tp = np.zeros_like(u), np.zeros_like(p)
+            for k in range(3):
+                fine = np.zeros(op.n2)
+                self.m.p2_prolongate(L - 1, np.ascontiguousarray(xu[k * c.n2:(k + 1) * c.n2]), fine, GMG_FLAG)
+                tu[k * op.n2:(k + 1) * op.n2] = fine
+            self.m.prolongate(L - 1, xp, tp, ALL, False)
+            op.project(tu)
+            project_mean(tp)
+            m = op.vmask(GMG_FLAG)
+            u[m] += tu[m]
+            p += tp
+        else:
+            for k in range(3):
+                fine = np.ascontiguousarray(u[k * op.n2:(k + 1) * op.n2])
+                self.m.p2_prolongate(L - 1, np.ascontiguousarray(xu[k * c.n2:(k + 1) * c.n2]), fine, GMG_FLAG, True)
+                u[k * op.n2:(k + 1) * op.n2] = fine
+            self.m.prolongate(L - 1, xp, p, ALL, True)
         for _ in range(self.post + self.inc * (self.maxL - L)):
             self.smooth(L, u, p, bu, bp)
 

@@ -162,3 +162,107 @@ def test_velocity_inverse_diagonal_matches_the_element_diagonals(shell):
         y = np.zeros(3 * op.n2)
         op.apply(e, None, y, None, po.ALL, op.A_U)
         assert abs(y[j] * inv[j] - 1.0) < 1e-12
+
+
+# ---- free-slip: surface flags, normal projection, projected operator --------------
+@pytest.fixture(scope="module")
+def fs_shell():
+    """the shell with a free-slip inner and a no-slip outer surface"""
+    return po.Mesh.shell(ntan=3, nrad=2, rmin=0.5, rmax=1.0, inner_flag=po.FREESLIP, outer_flag=po.DIRICHLET)
+
+
+def test_shell_surface_flags(fs_shell):
+    """each surface of the shell (ntan = 3) is an icosahedral triangulation with
+    10 n^2 + 2 points, n = 2^(L+1); the inner one carries FreeSlip, the outer
+    one Dirichlet (the computational points lie on the flat macro faces, inside
+    their sphere, and the macro vertices on it)"""
+    for L in (0, 1, 2):
+        r = np.linalg.norm(fs_shell.coords(L), axis=1)
+        fl = fs_shell.flags(L)
+        n = 2 ** (L + 1)
+        assert (fl == po.FREESLIP).sum() == (fl == po.DIRICHLET).sum() == 10 * n * n + 2
+        assert np.all(r[fl == po.FREESLIP] <= 0.5 + 1e-12) and np.all(r[fl == po.FREESLIP] > 0.45)
+        assert np.all(r[fl == po.DIRICHLET] <= 1.0 + 1e-12) and np.all(r[fl == po.DIRICHLET] > 0.9)
+        assert np.all((fl == po.INNER) | (fl == po.FREESLIP) | (fl == po.DIRICHLET))
+        assert (np.abs(r - 0.5) < 1e-12).sum() == (np.abs(r - 1.0) < 1e-12).sum() == 42
+
+
+def test_project_normal_closed_forms(fs_shell):
+    """radial fields vanish on the FreeSlip DoFs, rigid rotations (tangential)
+    pass unchanged, a random field loses exactly its normal part there and
+    nothing elsewhere, and the projection is idempotent"""
+    L = 1
+    X = fs_shell.coords(L + 1)
+    n2 = len(X)
+    fs = fs_shell.flags(L + 1) == po.FREESLIP
+    nrm = X / np.linalg.norm(X, axis=1)[:, None]
+    radial = np.concatenate([X[:, c] * (1.0 + X[:, 0] ** 2) for c in range(3)])
+    fs_shell.p2_project_normal(L, radial)
+    assert np.abs(radial.reshape(3, n2).T[fs]).max() < 1e-14
+    w = np.array([0.3, -1.1, 0.7])
+    rot = np.cross(w, X).T.ravel().copy()
+    before = rot.copy()
+    fs_shell.p2_project_normal(L, rot)
+    assert np.abs(rot - before).max() < 1e-14
+    u = np.random.default_rng(3).normal(size=3 * n2)
+    v = u.copy()
+    fs_shell.p2_project_normal(L, v)
+    U, V = u.reshape(3, n2).T, v.reshape(3, n2).T
+    assert np.abs((V * nrm).sum(1)[fs]).max() < 1e-14
+    d = U - V
+    assert np.abs(d[~fs]).max() == 0.0
+    assert np.abs(d[fs] - (U[fs] * nrm[fs]).sum(1)[:, None] * nrm[fs]).max() < 1e-14
+    v2 = v.copy()
+    fs_shell.p2_project_normal(L, v2)
+    assert np.abs(v2 - v).max() < 1e-15
+
+
+def test_freeslip_wrapper_is_symmetric(fs_shell):
+    """on fields that vanish on the Dirichlet DoFs, P A P is symmetric and
+    B P, P B^T are transposes of each other (the wrapper composes the
+    projection on both sides of the unprojected blocks)"""
+    L = 1
+    op = po.StokesOp(fs_shell, L, *SHELL_VISC, freeslip=True)
+    rng = np.random.default_rng(4)
+    m = op.vmask(po.GMG_FLAG)
+    u, v = rng.normal(size=3 * op.n2), rng.normal(size=3 * op.n2)
+    u[~m] = 0.0
+    v[~m] = 0.0
+    q = rng.normal(size=op.n1)
+    Au, Av = np.zeros_like(u), np.zeros_like(v)
+    op.apply(u, None, Au, None, po.GMG_FLAG, op.A_U)
+    op.apply(v, None, Av, None, po.GMG_FLAG, op.A_U)
+    assert abs(v @ Au - u @ Av) < 1e-12 * abs(v @ Au)
+    Bu = np.zeros(op.n1)
+    BTq = np.zeros_like(u)
+    op.apply(u, None, None, Bu, po.GMG_FLAG, op.B_U)
+    op.apply(None, q, BTq, None, po.GMG_FLAG, op.BT_P)
+    assert abs(q @ Bu - u @ BTq) < 1e-12 * np.abs(q).sum() * np.abs(Bu).max()
+    fs = np.tile(fs_shell.flags(L + 1) == po.FREESLIP, 3)
+    n2 = op.n2
+    X = fs_shell.coords(L + 1)
+    nrm = X / np.linalg.norm(X, axis=1)[:, None]
+    for y in (Au, BTq):
+        Y = y.reshape(3, n2).T
+        assert np.abs((Y * nrm).sum(1)[fs[:n2]]).max() < 1e-12 * np.abs(y).max()
+    # the full apply projects the pressure mean out of its result
+    yu, yp = np.zeros_like(u), np.zeros(op.n1)
+    op.apply(u, q, yu, yp, po.GMG_FLAG)
+    assert abs(yp.sum()) < 1e-12 * np.abs(yp).sum()
+
+
+def test_freeslip_gmg_keeps_the_velocity_tangential(fs_shell):
+    """V-cycles of the free-slip Uzawa GMG (constant viscosity) reduce the
+    residual and keep u.n = 0 on the FreeSlip DoFs"""
+    g = po.StokesGMG(fs_shell, 0, 2, freeslip=True)
+    op = g.ops[2]
+    bu = np.random.default_rng(5).normal(size=3 * op.n2)
+    bu[~op.vmask(po.GMG_FLAG)] = 0.0
+    op.project(bu)
+    u, p = np.zeros_like(bu), np.zeros(op.n1)
+    k, hist = g.solve(u, p, bu, np.zeros(op.n1), cycles=4)
+    assert k == 4 and hist[-1] < 0.02 * hist[0], hist
+    X = fs_shell.coords(3)
+    U = u.reshape(3, op.n2).T
+    fs = fs_shell.flags(3) == po.FREESLIP
+    assert np.abs((U * X).sum(1)[fs]).max() < 1e-14 * np.abs(U).max()
