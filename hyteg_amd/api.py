@@ -515,6 +515,13 @@ class P2P1StokesOperator:
     def pspg_inverse_diagonal(self, level, dst):
         self.ctx.check(lib().hg_stokes_pressure_diagonal(self.ctx.h, self.h, level, 0, dst.h), "PSPG diagonal")
 
+    def kmass_apply(self, src, dst, level, flag=ALL, update=REPLACE):
+        """dst = M_k src on pressure functions, the P1 mass with k = 1/eta"""
+        s = np.ascontiguousarray([0, 0, 0, src.h], dtype=np.int64)
+        d = np.ascontiguousarray([0, 0, 0, dst.h], dtype=np.int64)
+        self.ctx.check(lib().hg_stokes_apply(self.ctx.h, self.h, _ptr(s), _ptr(d), level, flag, 8, update),
+                       "K-mass apply")
+
     def lumped_inverse_mass(self, level, dst):
         self.ctx.check(lib().hg_stokes_pressure_diagonal(self.ctx.h, self.h, level, 1, dst.h), "lumped mass")
 
@@ -538,6 +545,20 @@ def stokes_minres_solve(A, x, b, level, max_iter=10, rel_tol=1e-6, abs_tol=0.0):
     A.ctx.check(lib().hg_stokes_minres_solve(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), level, max_iter,
                                              rel_tol, abs_tol, int(A.freeslip), C.byref(its)), "MINRES")
     return its.value
+
+
+def stokes_fgmres_solve(A, x, b, min_level, max_level, rho, a_pre=3, a_post=3, order=3, a_coarse_iter=10,
+                        a_coarse_tol=1e-6, schur_iter=50, schur_tol=1e-6, max_iter=100, restart=50, tol=1e-7):
+    """FGMRES with the block-factorisation preconditioner (velocity block: one GMG V-cycle with
+    Chebyshev smoothing, Schur complement: CG on the 1/eta pressure mass); returns
+    (iterations, approximate residual history)"""
+    hist = np.zeros(max_iter + 1)
+    its = C.c_int()
+    A.ctx.check(lib().hg_stokes_fgmres_solve(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), min_level, max_level,
+                                             a_pre, a_post, order, rho, a_coarse_iter, a_coarse_tol, schur_iter,
+                                             schur_tol, int(A.freeslip), max_iter, restart, tol, _ptr(hist),
+                                             C.byref(its)), "Stokes FGMRES")
+    return its.value, hist[:its.value + 1]
 
 
 def stokes_gmg_solve(A, x, b, min_level, max_level, rho, pre=3, post=3, smooth_increment=2, omega=0.3,

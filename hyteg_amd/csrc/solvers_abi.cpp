@@ -673,4 +673,68 @@ int hg_stokes_gmg_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg
    } );
 }
 
+int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel,
+                            int maxLevel, int aPre, int aPost, int chebyshevOrder, double spectralRadius,
+                            int aCoarseMaxIter, double aCoarseTol, int schurMaxIter, double schurTol, int freeslip,
+                            int maxIter, int restart, double relTol, double* res_hist, int* iterations )
+{
+   if ( !ctx || !x || !b || minLevel < 0 || maxLevel < minLevel || aPre < 0 || aPost < 0 || chebyshevOrder < 1 ||
+        chebyshevOrder > 16 || !( spectralRadius > 0 ) || aCoarseMaxIter < 0 || schurMaxIter < 0 || maxIter < 1 ||
+        restart < 2 )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      auto         storage = storageOf( ctx, x[0] );
+      const uint_t minL = uint_t( minLevel ), L = uint_t( maxLevel );
+      withStokesOperator( storage, op, minL, L, freeslip != 0, [&]( const auto& A, const auto& projection ) {
+         using Op  = std::decay_t< decltype( A ) >;
+         using VOp = typename Op::VelocityOperator_T;
+         using SOp = P1ElementwiseKMassOperator;
+         TaylorHood X( ctx, x ), B( ctx, b );
+         // velocity block: GMG with Chebyshev smoothing on [0.1 rho, 1.2 rho] (the TerraNeo
+         // interval factors, terraneo/helpers/TerraNeoParameters.hpp:82-83) and MINRES on minLevel
+         auto cheby = std::make_shared< ChebyshevSmoother< VOp > >( storage, minL, L );
+         cheby->setupCoefficients( uint_t( chebyshevOrder ), spectralRadius, 1.2, 0.1 );
+         auto coarse = std::make_shared< MinResSolver< VOp > >( storage, minL, minL, uint_t( aCoarseMaxIter ), aCoarseTol,
+                                                                0.0, std::make_shared< IdentityPreconditioner< VOp > >() );
+         // with free-slip the transfers project their result: the coarse P A P is singular
+         // in the normal directions, and an unprojected coarse right-hand side makes the
+         // velocity V-cycle diverge
+         std::shared_ptr< RestrictionOperator< P2VectorFunction< double > > >  restriction;
+         std::shared_ptr< ProlongationOperator< P2VectorFunction< double > > > prolongation;
+         if ( projection )
+         {
+            restriction  = std::make_shared< P2toP2QuadraticVectorRestrictionWithProjection >( projection );
+            prolongation = std::make_shared< P2toP2QuadraticVectorProlongationWithProjection >( projection );
+         }
+         else
+         {
+            restriction  = std::make_shared< P2toP2QuadraticVectorRestriction >();
+            prolongation = std::make_shared< P2toP2QuadraticVectorProlongation >();
+         }
+         auto gmgA = std::make_shared< GeometricMultigridSolver< VOp > >( storage, cheby, coarse, restriction, prolongation,
+                                                                          minL, L, uint_t( aPre ), uint_t( aPost ) );
+         // Schur block: CG on the 1/eta pressure mass, preconditioned with its inverse lumped diagonal
+         SOp  S( ctx, op );
+         auto cgS = std::make_shared< CGSolver< SOp > >(
+             storage, L, L, uint_t( schurMaxIter ), schurTol, 1e-16,
+             std::make_shared< FunctionMultiplicationPreconditioner< SOp > >( S.inverseLumpedDiagonal( storage, L, L ),
+                                                                              All ) );
+         // the Krylov vectors carry the whole hierarchy: the velocity V-cycle runs on them
+         auto prec = std::make_shared< BlockFactorisationPreconditioner< Op, VOp, SOp > >( storage, minL, L, S, gmgA, cgS,
+                                                                                           1, projection );
+         FGMRESSolver< Op > fgmres( storage, minL, L, uint_t( maxIter ) + 1, relTol, 0.0, prec, uint_t( restart ) );
+         fgmres.setIterationHook( [&]( uint_t j, double r ) {
+            if ( res_hist )
+               res_hist[j] = r;
+            return false;
+         } );
+         fgmres.solve( A, X, B, L );
+         if ( res_hist )
+            res_hist[0] = fgmres.getInitialResidual();
+         if ( iterations )
+            *iterations = int( fgmres.getIterations() );
+      } );
+   } );
+}
+
 } // extern "C"

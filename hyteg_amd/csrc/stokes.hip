@@ -126,7 +126,9 @@ __device__ __forceinline__ QPoint qpoint( const P2CellGeom& g, const TileThread<
 }
 
 // y (=|+=) the selected blocks on the cell lattices (halo positions included):
-// MASK bit 1: yu = A u, bit 2: yu += B^T p, bit 4: yp = B u
+// MASK bit 1: yu = A u, bit 2: yu += B^T p, bit 4: yp = B u; MASK 8 alone:
+// yp = M_k p, the P1 mass with k = 1/eta (the Schur-complement approximation
+// of the block-factorisation preconditioner), 4-point degree-2 rule
 template < int MASK >
 __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
     const P2CellGeom* __restrict__ geom, double c0, double c1, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
@@ -134,8 +136,8 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
     const double* __restrict__ pin, double* __restrict__ y0, double* __restrict__ y1, double* __restrict__ y2,
     double* __restrict__ yp )
 {
-   constexpr bool DO_A = MASK & 1, DO_BT = MASK & 2, DO_B = MASK & 4;
-   constexpr bool NEED_U = DO_A || DO_B, OUT_U = DO_A || DO_BT;
+   constexpr bool DO_A = MASK & 1, DO_BT = MASK & 2, DO_B = MASK & 4, DO_M = MASK & 8;
+   constexpr bool NEED_U = DO_A || DO_B, OUT_U = DO_A || DO_BT, NEED_P = DO_BT || DO_M, OUT_P = DO_B || DO_M;
    // LDS boxes: first the tile's input values, then the accumulators
    __shared__ double su[3][kBox3];
    __shared__ double sp[kPBox3];
@@ -157,7 +159,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
          for ( int c = 0; c < 3; ++c )
             su[c][p] = in ? uin[c][idx] : 0.0;
       }
-   if constexpr ( DO_BT )
+   if constexpr ( NEED_P )
       for ( int p = threadIdx.x; p < kPBox3; p += kTileThreads )
       {
          const int32_t lx = p % kPBox, ly = ( p / kPBox ) % kPBox, lz = p / ( kPBox * kPBox );
@@ -181,7 +183,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
          for ( int i = 0; i < 10; ++i )
             u[c][i] = th.valid ? su[c][pos[i]] : 0.0;
    }
-   if constexpr ( DO_BT )
+   if constexpr ( NEED_P )
    {
 #pragma unroll
       for ( int k = 0; k < 4; ++k )
@@ -191,7 +193,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
    if constexpr ( OUT_U )
       for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
          su[0][p] = su[1][p] = su[2][p] = 0.0;
-   if constexpr ( DO_B )
+   if constexpr ( OUT_P )
       for ( int p = threadIdx.x; p < kPBox3; p += kTileThreads )
          sp[p] = 0.0;
    __syncthreads();
@@ -248,6 +250,21 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
             }
          }
       }
+      if constexpr ( DO_M )
+      {
+#pragma unroll 1
+         for ( int q = 5; q < 9; ++q )
+         {
+            const double* lam = cSQL[q];
+            const QPoint  P   = qpoint( g, th, q );
+            // |det D(F J)| = q^2 |beta| adet; k = 1/eta at the blended point
+            const double cc = cSQW[q] * adet * fabs( P.beta ) * P.qq * P.qq * exp( -( c0 + c1 * P.r ) );
+            const double s  = cc * ( lam[0] * pv[0] + lam[1] * pv[1] + lam[2] * pv[2] + lam[3] * pv[3] );
+#pragma unroll
+            for ( int k = 0; k < 4; ++k )
+               yq[k] += lam[k] * s;
+         }
+      }
       if constexpr ( DO_B || DO_BT )
       {
 #pragma unroll 1
@@ -299,7 +316,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
             for ( int i = 0; i < 10; ++i )
                atomicAdd( &su[c][pos[i]], y[c][i] );
       }
-      if constexpr ( DO_B )
+      if constexpr ( OUT_P )
       {
 #pragma unroll
          for ( int k = 0; k < 4; ++k )
@@ -324,7 +341,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
             sharedAdd( yout[c] + idx, su[c][p] );
       }
    }
-   if constexpr ( DO_B )
+   if constexpr ( OUT_P )
       for ( int p = threadIdx.x; p < kPBox3; p += kTileThreads )
       {
          const int32_t lx = p % kPBox, ly = ( p / kPBox ) % kPBox, lz = p / ( kPBox * kPBox );
@@ -461,6 +478,21 @@ __global__ void kAddConst( int64_t ncell, int64_t nb, const uint8_t* __restrict_
    v[pt] += value;
 }
 
+// v = 1/v on the flagged DoFs of a single-vector FP64 function (zeros stay)
+__global__ void kInvertFlagged( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag,
+                                double* __restrict__ v )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t n = ( flag & kInner ? ncell : 0 ) + nb;
+   if ( i >= n )
+      return;
+   const int64_t pt = ( flag & kInner ) ? i : ncell + i;
+   if ( pt >= ncell && !( bndFlag[pt - ncell] & flag ) )
+      return;
+   if ( v[pt] != 0.0 )
+      v[pt] = 1.0 / v[pt];
+}
+
 // u -= (u . n) n on the owned boundary DoFs whose type is in flag
 // (P2ProjectNormalOperator::project, p1functionspace/freeslip/VertexDoFProjectNormal.hpp:43-120)
 __global__ void kProjectNormal( int64_t nb, int64_t ncell, const uint8_t* __restrict__ bndFlag, int flag,
@@ -533,9 +565,12 @@ void launchStokes( Context* ctx, StokesOperator& op, int L, int64_t ntiles, Latt
 void stokesApply( Context* ctx, StokesOperator& op, Function* const src[4], Function* const dst[4], int L, int flag,
                   int mask, int update )
 {
-   if ( mask < 1 || mask > 7 )
-      throw Error( HG_ERR_ARG, "block mask must be 1..7 (1: A u, 2: B^T p, 4: B u)" );
-   const bool needU = mask & 5, needP = mask & 2, outU = mask & 3, outP = mask & 4;
+   if ( mask < 1 || mask > 8 )
+      throw Error( HG_ERR_ARG, "block mask must be 1..7 (1: A u, 2: B^T p, 4: B u) or 8 (the 1/eta pressure mass)" );
+   const bool needU = mask & 5, needP = mask & 10, outU = mask & 3, outP = mask & 12;
+   for ( int c = 0; c < 4; ++c )
+      if ( ( ( c < 3 ? needU : needP ) && !src[c] ) || ( ( c < 3 ? outU : outP ) && !dst[c] ) )
+         throw Error( HG_ERR_ARG, "a function the block mask needs is missing" );
    for ( int c = 0; c < 3; ++c )
    {
       if ( needU )
@@ -596,7 +631,8 @@ void stokesApply( Context* ctx, StokesOperator& op, Function* const src[4], Func
    case 4: launchStokes< 4 >( ctx, op, L, ntiles, l1, l0, u, p, y, yp ); break;
    case 5: launchStokes< 5 >( ctx, op, L, ntiles, l1, l0, u, p, y, yp ); break;
    case 6: launchStokes< 6 >( ctx, op, L, ntiles, l1, l0, u, p, y, yp ); break;
-   default: launchStokes< 7 >( ctx, op, L, ntiles, l1, l0, u, p, y, yp ); break;
+   case 7: launchStokes< 7 >( ctx, op, L, ntiles, l1, l0, u, p, y, yp ); break;
+   default: launchStokes< 8 >( ctx, op, L, ntiles, l1, l0, u, p, y, yp ); break;
    }
    ctx->endTiming();
    if ( outU )
@@ -761,8 +797,8 @@ int hg_stokes_apply( hg_ctx* h, hg_handle op, const hg_handle* src, const hg_han
       Function* d[4] = { nullptr, nullptr, nullptr, nullptr };
       for ( int k = 0; k < 4; ++k )
       {
-         const bool needS = k < 3 ? ( mask & 5 ) : ( mask & 2 );
-         const bool needD = k < 3 ? ( mask & 3 ) : ( mask & 4 );
+         const bool needS = k < 3 ? ( mask & 5 ) : ( mask & 10 );
+         const bool needD = k < 3 ? ( mask & 3 ) : ( mask & 12 );
          if ( needS )
             s[k] = &ctx->function( src[k] );
          if ( needD )
@@ -862,6 +898,22 @@ int hg_function_add_const( hg_ctx* h, hg_handle fn, double value, int level, int
          return;
       kAddConst<<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, value,
                                                              (double*) f.ptr( level ) );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+int hg_function_invert( hg_ctx* h, hg_handle fn, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& f = ctx->function( fn );
+      if ( f.lv != 0 || f.dtype != 0 || level < f.minL || level > f.maxL )
+         throw Error( HG_ERR_ARG, "invertElementwise needs a single-vector FP64 function and a level in its range" );
+      Lattice&      l = f.lat( ctx, level );
+      const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
+      if ( n == 0 )
+         return;
+      kInvertFlagged<<< gridFor( n ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag,
+                                                                  (double*) f.ptr( level ) );
       HG_CHECK_HIP( hipGetLastError() );
    } );
 }

@@ -170,6 +170,9 @@ int hg_function_sum( hg_ctx* ctx, hg_handle fn, int level, int flag, double* out
 /* add(scalar, level, flag): fn += value on flagged DoFs (VertexDoFFunction.hpp:195);
  * with hg_function_sum this is vertexdof::projectMean (VertexDoFFunction.hpp:586-592) */
 int hg_function_add_const( hg_ctx* ctx, hg_handle fn, double value, int level, int flag );
+/* v = 1/v on the flagged DoFs, zeros kept (VertexDoFFunction::invertElementwise,
+ * VertexDoFFunction.hpp:252); single-vector FP64 */
+int hg_function_invert( hg_ctx* ctx, hg_handle fn, int level, int flag );
 /* communicate<Vertex,Edge>..<Face,Cell>: refresh every macro-cell halo copy from
  * its owner (VertexDoFFunction.hpp:366-392, VertexDoFPackInfo.hpp:534-610) */
 int hg_function_communicate( hg_ctx* ctx, hg_handle fn, int level );
@@ -329,7 +332,11 @@ int hg_stokes_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel
  * (P2ElementwiseBlendingFullViscousOperator::apply), bit 2: dst.uvw += B^T src.p
  * (P1ToP2ElementwiseBlendingDivTOperator), bit 4: dst.p = B src.uvw
  * (P2ToP1ElementwiseBlendingDivOperator); mask 7 is the operator's apply
- * (P2P1ElementwiseBlendingStokesOperator.hpp:176-184).  Handles of unused
+ * (P2P1ElementwiseBlendingStokesOperator.hpp:176-184).  Mask 8 (alone):
+ * dst.p = M_k src.p, the P1 mass with k = 1/eta at the blended points (the
+ * Schur-complement operator of the free-slip wrapper, a P1 "K-mass",
+ * terraneo/operators/P2P1StokesOperatorWithWrapper.hpp:388-395), 4-point
+ * degree-2 rule.  Handles of unused
  * components are ignored.  The pressure has no boundary DoFs (createAllInnerBC):
  * it is written when flag contains Inner. */
 int hg_stokes_apply( hg_ctx* ctx, hg_handle op, const hg_handle* src, const hg_handle* dst, int level, int flag, int mask,
@@ -382,6 +389,21 @@ int hg_stokes_gmg_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg
                          int pre, int post, int smoothIncrement, double omega, int velocityIterations, int chebyshevOrder,
                          double spectralRadius, int coarseMaxIter, double coarseTol, int projectPressureMean,
                          int freeslip, int cycles, double tol, double* res_hist, int* cycles_done );
+/* FGMRESSolver (solvers/FGMRESSolver.hpp:55-500) with the block-factorisation
+ * preconditioner of StokesMCFGMRESSolver (terraneo/solvers/StokesMCFGMRESSolver.hpp:50-160,
+ * BlockFactorisationPreconditioner, StokesBlockPreconditioners.hpp:366-448):
+ * velocity block by one GMG V(aPre, aPost) cycle (Chebyshev of the given order
+ * on [0.1, 1.2] x radius, P2toP2QuadraticVector transfers (...WithProjection
+ * on the FreeSlip DoFs when freeslip), MINRES with identity
+ * preconditioner on minLevel), Schur complement by CG on the 1/eta pressure
+ * mass (mask 8 of hg_stokes_apply) preconditioned with its inverse lumped
+ * diagonal.  Iterates until the approximate residual drops by relTol or maxIter
+ * iterations ran (restart every `restart`); res_hist[0..iterations] holds the
+ * initial and the approximate residual 2-norms over Inner | Neumann | FreeSlip. */
+int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel,
+                            int maxLevel, int aPre, int aPost, int chebyshevOrder, double spectralRadius,
+                            int aCoarseMaxIter, double aCoarseTol, int schurMaxIter, double schurTol, int freeslip,
+                            int maxIter, int restart, double relTol, double* res_hist, int* iterations );
 
 #ifdef __cplusplus
 }

@@ -217,11 +217,17 @@ class LatticeFunction
    hg_handle handle() const { return h_; }
    hg_ctx*   ctx() const { return ctx_; }
    int       nvec() const { return nvec_; }
+   // BoundaryCondition::createAllInnerBC (boundary/BoundaryConditions.cpp): every
+   // DoF counts as Inner, so a flag selects all DoFs when it contains Inner and
+   // none otherwise (the pressure of a Taylor-Hood function)
+   void setAllInner( bool allInner = true ) { allInner_ = allInner; }
+   bool isAllInner() const { return allInner_; }
    // per-vector dot products of interleaved functions
    std::vector< double > dotGlobalBatched( const LatticeFunction& other, uint_t level, DoFType flag = All ) const
    {
       std::vector< double > r( static_cast< size_t >( nvec_ ) );
-      check( ctx_, hg_function_dot_batched( ctx_, h_, other.h_, int( level ), flag, r.data() ), "dotGlobalBatched" );
+      check( ctx_, hg_function_dot_batched( ctx_, h_, other.h_, int( level ), dofs( flag ), r.data() ),
+             "dotGlobalBatched" );
       return r;
    }
    // vector k of this function = vector ksrc of other
@@ -235,7 +241,7 @@ class LatticeFunction
    {
       if ( scales.size() != size_t( nvec_ ) )
          throw std::invalid_argument( "scaleBatched needs one scale per vector" );
-      check( ctx_, hg_function_scale_batched( ctx_, h_, scales.data(), int( level ), flag ), "scaleBatched" );
+      check( ctx_, hg_function_scale_batched( ctx_, h_, scales.data(), int( level ), dofs( flag ) ), "scaleBatched" );
    }
 
    void assign( const std::vector< ValueType >&                                            scalars,
@@ -254,13 +260,19 @@ class LatticeFunction
    }
    void interpolate( ValueType constant, uint_t level, DoFType flag = All ) const
    {
-      check( ctx_, hg_function_interpolate_const( ctx_, h_, double( constant ), int( level ), flag ), "interpolate" );
+      check( ctx_, hg_function_interpolate_const( ctx_, h_, double( constant ), int( level ), dofs( flag ) ),
+             "interpolate" );
+   }
+   // invertElementwise(level, flag): v = 1/v (zeros kept), FP64
+   void invertElementwise( uint_t level, DoFType flag = All ) const
+   {
+      check( ctx_, hg_function_invert( ctx_, h_, int( level ), dofs( flag ) ), "invertElementwise" );
    }
    void setToZero( uint_t level ) const { interpolate( ValueType( 0 ), level, All ); }
    ValueType dotGlobal( const LatticeFunction& other, uint_t level, DoFType flag = All ) const
    {
       double r = 0.0;
-      check( ctx_, hg_function_dot( ctx_, h_, other.h_, int( level ), flag, &r ), "dotGlobal" );
+      check( ctx_, hg_function_dot( ctx_, h_, other.h_, int( level ), dofs( flag ), &r ), "dotGlobal" );
       return ValueType( r );
    }
    template < typename OtherT >
@@ -274,7 +286,8 @@ class LatticeFunction
    {
       if ( functions.size() != 2 )
          throw std::invalid_argument( "multElementwise takes two functions" );
-      check( ctx_, hg_function_mult( ctx_, h_, functions[0].get().h_, functions[1].get().h_, int( level ), flag ),
+      check( ctx_,
+             hg_function_mult( ctx_, h_, functions[0].get().h_, functions[1].get().h_, int( level ), dofs( flag ) ),
              "multElementwise" );
    }
    void communicate( uint_t level ) const { check( ctx_, hg_function_communicate( ctx_, h_, int( level ) ), "communicate" ); }
@@ -287,12 +300,14 @@ class LatticeFunction
    {
       check( ctx_, hg_function_nvec( ctx_, h_, &nvec_ ), "nvec" );
    }
+   int dofs( DoFType flag ) const { return allInner_ ? ( ( flag & Inner ) ? int( All ) : 0 ) : int( flag ); }
    void linComb( const std::vector< ValueType >&                                            scalars,
                  const std::vector< std::reference_wrapper< const LatticeFunction > >& functions,
                  uint_t                                                                 level,
-                 DoFType                                                                flag,
+                 DoFType                                                                flagIn,
                  bool                                                                   add ) const
    {
+      const int flag = dofs( flagIn );
       if ( scalars.size() != functions.size() )
          throw std::invalid_argument( "assign: sizes differ" );
       std::vector< double >    s( scalars.begin(), scalars.end() );
@@ -317,10 +332,23 @@ class LatticeFunction
    std::string                         name_;
    std::shared_ptr< PrimitiveStorage > storage_;
    hg_ctx*                             ctx_   = nullptr;
-   hg_handle                           h_     = 0;
-   bool                                owned_ = true;
-   int                                 nvec_  = 1;
+   hg_handle                           h_        = 0;
+   bool                                owned_    = true;
+   int                                 nvec_     = 1;
+   bool                                allInner_ = false;
 };
+
+// copyBCs(src, dst): dst takes src's boundary-condition mode (the solvers'
+// temporaries follow the function they solve for, as copyBCs does in the
+// reference's solvers); a no-op for composite functions
+template < class F >
+inline void copyBCs( const F&, F& )
+{}
+template < typename T, int S >
+inline void copyBCs( const LatticeFunction< T, S >& src, LatticeFunction< T, S >& dst )
+{
+   dst.setAllInner( src.isAllInner() );
+}
 
 template < typename ValueType >
 using P1Function = LatticeFunction< ValueType, 1 >;
@@ -637,6 +665,10 @@ class CGSolver : public Solver< OperatorType >
    {
       if ( maxIter_ == 0 )
          return;
+      copyBCs( x, p_ );
+      copyBCs( x, z_ );
+      copyBCs( x, ap_ );
+      copyBCs( x, r_ );
       p_.setToZero( level );
       z_.setToZero( level );
       ap_.setToZero( level );
@@ -1077,12 +1109,16 @@ class P2P1TaylorHoodFunction
                            uint_t maxLevel, int nvec = 1 )
    : uvw_( std::make_unique< P2VectorFunction< VT > >( name + "_uvw", storage, minLevel, maxLevel, nvec ) )
    , p_( std::make_unique< P1Function< VT > >( name + "_p", storage, minLevel, maxLevel ) )
-   {}
+   {
+      p_->setAllInner();
+   }
    // a view of existing handles {u_x, u_y, u_z, p} (not owned)
    P2P1TaylorHoodFunction( hg_ctx* ctx, const hg_handle h[4] )
    : uvw_( std::make_unique< P2VectorFunction< VT > >( ctx, h ) )
    , p_( P1Function< VT >::borrow( ctx, h[3] ) )
-   {}
+   {
+      p_->setAllInner();
+   }
    const P2VectorFunction< VT >& uvw() const { return *uvw_; }
    const P1Function< VT >&       p() const { return *p_; }
 
@@ -1733,6 +1769,362 @@ class P2P1StokesToP2P1StokesProlongationWithFreeSlipProjection : public P2P1Stok
  private:
    std::shared_ptr< P2P1TaylorHoodFunction< double > > temp_;
    std::shared_ptr< P2ProjectNormalOperator >          projection_;
+};
+
+// P2toP2QuadraticVectorRestriction / Prolongation
+// (gridtransferoperators/P2toP2QuadraticVectorRestriction.hpp, ...Prolongation.hpp):
+// the quadratic transfers of each velocity component
+class P2toP2QuadraticVectorRestriction : public RestrictionOperator< P2VectorFunction< double > >
+{
+ public:
+   void restrict( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      for ( int k = 0; k < 3; ++k )
+         check( f.ctx(), hg_p2_restrict( f.ctx(), f[k].handle(), int( sourceLevel ), flag ), "restrict" );
+   }
+};
+class P2toP2QuadraticVectorProlongation : public ProlongationOperator< P2VectorFunction< double > >
+{
+ public:
+   void prolongate( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      for ( int k = 0; k < 3; ++k )
+         check( f.ctx(), hg_p2_prolongate( f.ctx(), f[k].handle(), int( sourceLevel ), flag, HG_REPLACE ), "prolongate" );
+   }
+   void prolongateAndAdd( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      for ( int k = 0; k < 3; ++k )
+         check( f.ctx(), hg_p2_prolongate( f.ctx(), f[k].handle(), int( sourceLevel ), flag, HG_ADD ), "prolongate" );
+   }
+};
+
+// P2toP2QuadraticVector{Restriction,Prolongation}WithProjection with their default
+// configuration (P2toP2QuadraticVectorRestriction.hpp:54-150,
+// P2toP2QuadraticVectorProlongation.hpp:64-230: no pre-projection, post-projection
+// allowed to change the destination): transfer, then project the result on the
+// projectionFlag DoFs of the target level
+class P2toP2QuadraticVectorRestrictionWithProjection : public P2toP2QuadraticVectorRestriction
+{
+ public:
+   explicit P2toP2QuadraticVectorRestrictionWithProjection( std::shared_ptr< P2ProjectNormalOperator > projection,
+                                                            DoFType projectionFlag = FreeslipBoundary )
+   : projection_( std::move( projection ) )
+   , projectionFlag_( projectionFlag )
+   {}
+   void restrict( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      P2toP2QuadraticVectorRestriction::restrict( f, sourceLevel, flag );
+      projection_->project( f, sourceLevel - 1, projectionFlag_ );
+   }
+
+ private:
+   std::shared_ptr< P2ProjectNormalOperator > projection_;
+   DoFType                                    projectionFlag_;
+};
+class P2toP2QuadraticVectorProlongationWithProjection : public P2toP2QuadraticVectorProlongation
+{
+ public:
+   explicit P2toP2QuadraticVectorProlongationWithProjection( std::shared_ptr< P2ProjectNormalOperator > projection,
+                                                             DoFType projectionFlag = FreeslipBoundary )
+   : projection_( std::move( projection ) )
+   , projectionFlag_( projectionFlag )
+   {}
+   void prolongate( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      P2toP2QuadraticVectorProlongation::prolongate( f, sourceLevel, flag );
+      projection_->project( f, sourceLevel + 1, projectionFlag_ );
+   }
+   void prolongateAndAdd( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   {
+      P2toP2QuadraticVectorProlongation::prolongateAndAdd( f, sourceLevel, flag );
+      projection_->project( f, sourceLevel + 1, projectionFlag_ );
+   }
+
+ private:
+   std::shared_ptr< P2ProjectNormalOperator > projection_;
+   DoFType                                    projectionFlag_;
+};
+
+// IdentityPreconditioner (solvers/preconditioners/IdentityPreconditioner.hpp): x = b
+template < class OperatorType >
+class IdentityPreconditioner : public Solver< OperatorType >
+{
+ public:
+   void solve( const OperatorType&, const typename OperatorType::srcType& x, const typename OperatorType::dstType& b,
+               uint_t level ) override
+   {
+      x.assign( { 1.0 }, { b }, level, flag_ );
+   }
+
+ private:
+   DoFType flag_ = Inner | NeumannBoundary | FreeslipBoundary;
+};
+
+// FunctionMultiplicationPreconditioner (solvers/FunctionMultiplicationPreconditioner.hpp):
+// x = f * b elementwise with a fixed function f
+template < class OperatorType >
+class FunctionMultiplicationPreconditioner : public Solver< OperatorType >
+{
+ public:
+   FunctionMultiplicationPreconditioner( std::shared_ptr< typename OperatorType::dstType > multFunction,
+                                         DoFType flag = Inner | NeumannBoundary | FreeslipBoundary )
+   : f_( std::move( multFunction ) )
+   , flag_( flag )
+   {}
+   void solve( const OperatorType&, const typename OperatorType::srcType& x, const typename OperatorType::dstType& b,
+               uint_t level ) override
+   {
+      x.multElementwise( { *f_, b }, level, flag_ );
+   }
+
+ private:
+   std::shared_ptr< typename OperatorType::dstType > f_;
+   DoFType                                           flag_;
+};
+
+// The Schur-complement operator of the free-slip Stokes wrapper (its
+// SchurOperator_T, a P1 "K-mass" with k = 1/viscosity;
+// terraneo/operators/P2P1StokesOperatorWithWrapper.hpp:233-288,388-395):
+// (1/eta p, q) with eta the Stokes operator's viscosity at the blended points
+// (4-point degree-2 rule), sharing the Stokes operator's device data.
+class P1ElementwiseKMassOperator : public Operator< P1Function< double >, P1Function< double > >
+{
+ public:
+   P1ElementwiseKMassOperator( hg_ctx* ctx, hg_handle stokesOperator )
+   : ctx_( ctx )
+   , h_( stokesOperator )
+   {}
+   void apply( const P1Function< double >& src, const P1Function< double >& dst, uint_t level, DoFType flag,
+               UpdateType updateType = Replace ) const override
+   {
+      const hg_handle s[4] = { 0, 0, 0, src.handle() };
+      const hg_handle d[4] = { 0, 0, 0, dst.handle() };
+      detail::stokesApply( ctx_, h_, s, d, level, flag, 8, updateType );
+   }
+   // 1 / (M_k 1): the inverse lumped (row-sum) diagonal, for a
+   // FunctionMultiplicationPreconditioner
+   std::shared_ptr< P1Function< double > > inverseLumpedDiagonal( const std::shared_ptr< PrimitiveStorage >& storage,
+                                                                  uint_t minLevel, uint_t maxLevel ) const
+   {
+      auto                 d = std::make_shared< P1Function< double > >( "kmass_lumped_inv", storage, minLevel, maxLevel );
+      P1Function< double > one( "kmass_one", storage, minLevel, maxLevel );
+      d->setAllInner();
+      one.setAllInner();
+      for ( uint_t L = minLevel; L <= maxLevel; ++L )
+      {
+         one.interpolate( 1.0, L, All );
+         apply( one, *d, L, All, Replace );
+         d->invertElementwise( L, All );
+      }
+      return d;
+   }
+
+ private:
+   hg_ctx*   ctx_;
+   hg_handle h_;
+};
+
+// BlockFactorisationPreconditioner (solvers/preconditioners/stokes/StokesBlockPreconditioners.hpp:366-448):
+//   x.p = -S^-1 b.p;  x.uvw = A^-1 (b.uvw - B^T x.p)  (b.uvw projected when a projection is given)
+// with S^-1 and A^-1 the given approximate solvers of the Schur operator and the velocity block
+template < class OperatorType, class AOperatorType, class SchurOperatorType >
+class BlockFactorisationPreconditioner : public Solver< OperatorType >
+{
+ public:
+   using FunctionType = typename OperatorType::srcType;
+
+   BlockFactorisationPreconditioner( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel,
+                                     const SchurOperatorType&                              schurOp,
+                                     std::shared_ptr< Solver< AOperatorType > >            ABlockSolver,
+                                     std::shared_ptr< Solver< SchurOperatorType > >        schurSolver,
+                                     uint_t                                                velocityIterations = 1,
+                                     std::shared_ptr< P2ProjectNormalOperator >            projection         = nullptr,
+                                     DoFType flag = Inner | NeumannBoundary | FreeslipBoundary )
+   : schurOp_( schurOp )
+   , A_( std::move( ABlockSolver ) )
+   , S_( std::move( schurSolver ) )
+   , velocityIterations_( velocityIterations )
+   , projection_( std::move( projection ) )
+   , flag_( flag )
+   , tmp_( "BlockFactorisationPreconditioner_tmp", storage, minLevel, maxLevel )
+   {}
+   void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      S_->solve( schurOp_, x.p(), b.p(), level );
+      A.applyDivT( x.p(), tmp_.uvw(), level, flag_, Replace );
+      tmp_.uvw().assign( { 1.0, 1.0 }, { b.uvw(), tmp_.uvw() }, level, flag_ );
+      x.p().assign( { -1.0 }, { x.p() }, level, flag_ );
+      if ( projection_ )
+         projection_->project( tmp_.uvw(), level, FreeslipBoundary );
+      for ( uint_t i = 0; i < velocityIterations_; ++i )
+         A_->solve( A.getA(), x.uvw(), tmp_.uvw(), level );
+   }
+
+ private:
+   const SchurOperatorType&                       schurOp_;
+   std::shared_ptr< Solver< AOperatorType > >     A_;
+   std::shared_ptr< Solver< SchurOperatorType > > S_;
+   uint_t                                         velocityIterations_;
+   std::shared_ptr< P2ProjectNormalOperator >     projection_;
+   DoFType                                        flag_;
+   FunctionType                                   tmp_;
+};
+
+// FGMRESSolver (solvers/FGMRESSolver.hpp:55-500): flexible GMRES, right
+// preconditioned (the preconditioner may change between iterations), modified
+// Gram-Schmidt Arnoldi, Givens rotations on the Hessenberg matrix; restarts
+// every restartLength iterations; stops on the approximate residual (relative
+// or absolute), on the Arnoldi breakdown tolerance, on the iteration hook, or
+// after maxKrylowDim - 1 iterations.
+template < class OperatorType >
+class FGMRESSolver : public Solver< OperatorType >
+{
+ public:
+   using FunctionType = typename OperatorType::srcType;
+   using Hook         = std::function< bool( uint_t, double ) >; // (iteration, approximate residual) -> stop
+
+   FGMRESSolver( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel,
+                 uint_t maxKrylowDim = 1000, double approxRelativeTOL = 1e-16, double approxAbsoluteTOL = 1e-16,
+                 std::shared_ptr< Solver< OperatorType > > preconditioner = nullptr, uint_t restartLength = 1000,
+                 double arnoldiTOL = 1e-16 )
+   : storage_( storage )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , maxKrylowDim_( maxKrylowDim )
+   , restartLength_( restartLength )
+   , arnoldiTOL_( arnoldiTOL )
+   , relTol_( approxRelativeTOL )
+   , absTol_( approxAbsoluteTOL )
+   , prec_( preconditioner ? std::move( preconditioner ) : std::make_shared< IdentityPreconditioner< OperatorType > >() )
+   , r0_( "r0", storage, minLevel, maxLevel )
+   {}
+   void   setIterationHook( Hook hook ) { hook_ = std::move( hook ); }
+   uint_t getIterations() const { return iterations_; }
+   double getInitialResidual() const { return initRes_; }
+
+   void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
+   {
+      r0_.setToZero( level );
+      init( A, x, b, level, false );
+      iterations_   = maxKrylowDim_ > 0 ? maxKrylowDim_ - 1 : 0; // iterations performed without an early stop
+      double residual = absTol_ + 1.0;
+      for ( uint_t j = 1; j < maxKrylowDim_; ++j )
+      {
+         if ( j % restartLength_ == 0 )
+         {
+            generateApproximation( x, level );
+            init( A, x, b, level, true );
+            continue;
+         }
+         const size_t k = j % restartLength_; // the Krylov index of this iteration
+         while ( Z_.size() < k )
+            Z_.push_back( std::make_unique< FunctionType >( "z", storage_, minLevel_, maxLevel_ ) );
+         while ( V_.size() <= k )
+            V_.push_back( std::make_unique< FunctionType >( "w", storage_, minLevel_, maxLevel_ ) );
+         Z_[k - 1]->setToZero( level );
+         prec_->solve( A, *Z_[k - 1], *V_[k - 1], level );
+         A.apply( *Z_[k - 1], *V_[k], level, flag_ );
+         // H is (k+1) x k, column major in rows of length k+1
+         H_.resize( k );
+         H_[k - 1].assign( k + 1, 0.0 );
+         for ( size_t i = 0; i < k; ++i )
+         {
+            const double h = V_[k]->dotGlobal( *V_[i], level, flag_ );
+            H_[k - 1][i]   = h;
+            V_[k]->add( { -h }, { *V_[i] }, level, flag_ );
+         }
+         const double wNorm = std::sqrt( V_[k]->dotGlobal( *V_[k], level, flag_ ) );
+         H_[k - 1][k]       = wNorm;
+         residual           = hessenbergMinimizer();
+         V_[k]->assign( { 1.0 / wNorm }, { *V_[k] }, level, flag_ );
+         if ( hook_ && hook_( j, residual ) )
+         {
+            iterations_ = j;
+            break;
+         }
+         if ( wNorm <= arnoldiTOL_ || residual <= absTol_ || residual / initRes_ <= relTol_ )
+         {
+            iterations_ = j;
+            break;
+         }
+      }
+      generateApproximation( x, level );
+   }
+
+ private:
+   void init( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level, bool restart )
+   {
+      A.apply( x, r0_, level, flag_ );
+      r0_.assign( { 1.0, -1.0 }, { b, r0_ }, level, flag_ );
+      beta_ = std::sqrt( r0_.dotGlobal( r0_, level, flag_ ) );
+      if ( !restart )
+         initRes_ = beta_;
+      H_.clear();
+      rot_.clear();
+      g_.assign( 1, beta_ );
+      y_.clear();
+      if ( V_.empty() )
+         V_.push_back( std::make_unique< FunctionType >( "v0", storage_, minLevel_, maxLevel_ ) );
+      V_[0]->assign( { 1.0 / beta_ }, { r0_ }, level, flag_ );
+      if ( Z_.empty() )
+         Z_.push_back( std::make_unique< FunctionType >( "z0", storage_, minLevel_, maxLevel_ ) );
+   }
+   // applies the stored rotations and a new one to the last column, returns
+   // |residual| of the least-squares problem and sets y (back substitution)
+   double hessenbergMinimizer()
+   {
+      const size_t k   = H_.size();
+      auto&        col = H_[k - 1];
+      for ( size_t i = 0; i + 1 < k; ++i )
+      {
+         const double c = rot_[i].first, s = rot_[i].second;
+         const double a = col[i], b = col[i + 1];
+         col[i]         = c * a + s * b;
+         col[i + 1]     = -s * a + c * b;
+      }
+      double c = 1.0, s = 0.0;
+      if ( col[k] != 0.0 )
+      {
+         const double r = std::sqrt( col[k - 1] * col[k - 1] + col[k] * col[k] );
+         s              = col[k] / r;
+         c              = col[k - 1] / r;
+      }
+      rot_.emplace_back( c, s );
+      const double a = col[k - 1], b = col[k];
+      col[k - 1]     = c * a + s * b;
+      col[k]         = -s * a + c * b;
+      g_.push_back( -s * g_[k - 1] );
+      g_[k - 1] = c * g_[k - 1];
+      y_.assign( k, 0.0 );
+      for ( size_t i = k; i-- > 0; )
+      {
+         double t = g_[i];
+         for ( size_t m = i + 1; m < k; ++m )
+            t -= H_[m][i] * y_[m];
+         y_[i] = t / H_[i][i];
+      }
+      return std::abs( g_[k] );
+   }
+   void generateApproximation( const FunctionType& x, uint_t level )
+   {
+      for ( size_t i = 0; i < y_.size(); ++i )
+         x.add( { y_[i] }, { *Z_[i] }, level, flag_ );
+      y_.clear();
+   }
+
+   std::shared_ptr< PrimitiveStorage >                 storage_;
+   uint_t                                              minLevel_, maxLevel_, maxKrylowDim_, restartLength_;
+   double                                              arnoldiTOL_, relTol_, absTol_;
+   std::shared_ptr< Solver< OperatorType > >           prec_;
+   FunctionType                                        r0_;
+   std::vector< std::unique_ptr< FunctionType > >      V_, Z_;
+   std::vector< std::vector< double > >                H_;
+   std::vector< std::pair< double, double > >          rot_;
+   std::vector< double >                               g_, y_;
+   double                                              beta_ = 0.0, initRes_ = 0.0;
+   uint_t                                              iterations_ = 0;
+   Hook                                                hook_;
+   DoFType                                             flag_ = Inner | NeumannBoundary | FreeslipBoundary;
 };
 
 } // namespace hyteg_amd

@@ -139,6 +139,8 @@ typedef struct or_stokesop or_stokesop;
  * A 30x30 (index 10 c + i: component c, P2 function i; may be NULL) and
  * B 4x30 = -(psi_i, div u) (may be NULL); p = 4 computational vertices, blend may be NULL */
 void         or_stokes_element( const double* p, const double* blend, double c0, double c1, double* A, double* B );
+/* 4x4 P1 mass with k = 1/eta (4-point degree-2 rule, blended) */
+void         or_stokes_kmass_element( const double* p, const double* blend, double c0, double c1, double* M );
 or_stokesop* or_stokesop_create( const or_mesh* m, int level, double c0, double c1 );
 void         or_stokesop_free( or_stokesop* op );
 /* mask bit 1: yu = A u, 2: yu += B^T p, 4: yp = B u; u = [ux|uy|uz] (level+1 numbering),

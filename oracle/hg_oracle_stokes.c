@@ -154,6 +154,23 @@ void or_stokes_element( const double* p, const double* blend, double c0, double 
    }
 }
 
+/* M (4x4) of int k psi_i psi_j with k = 1/eta = exp(-(c0 + c1 |F(x)|)), the
+ * 4-point degree-2 rule on the blended element (the P1 "K-mass" Schur operator
+ * of the free-slip Stokes wrapper, terraneo/operators/P2P1StokesOperatorWithWrapper.hpp:388-395) */
+void or_stokes_kmass_element( const double* p, const double* blend, double c0, double c1, double* M )
+{
+   memset( M, 0, 16 * sizeof( double ) );
+   for ( int q = 0; q < 4; ++q )
+   {
+      double pg[10][3], adet, F[3];
+      point_geometry( p, blend, Q2P[q], pg, &adet, F );
+      const double wq = Q2W * adet / viscosity( c0, c1, F );
+      for ( int i = 0; i < 4; ++i )
+         for ( int j = 0; j < 4; ++j )
+            M[4 * i + j] += wq * Q2P[q][i] * Q2P[q][j];
+   }
+}
+
 struct or_stokesop
 {
    const or_mesh* m;
@@ -274,6 +291,14 @@ static void sapply_cb( void* vctx, const double* p, const double* blend, const i
       for ( int i = 0; i < 4; ++i )
          for ( int s = 0; s < 30; ++s )
             yp[i] += B[30 * i + s] * ue[s];
+   if ( c->mask & 8 )
+   {
+      double M[16];
+      or_stokes_kmass_element( p, blend, c->op->c0, c->op->c1, M );
+      for ( int i = 0; i < 4; ++i )
+         for ( int j = 0; j < 4; ++j )
+            yp[i] += M[4 * i + j] * pe[j];
+   }
    if ( c->mask & 3 )
       for ( int cc = 0; cc < 3; ++cc )
          for ( int i = 0; i < 10; ++i )
@@ -281,7 +306,7 @@ static void sapply_cb( void* vctx, const double* p, const double* blend, const i
 #pragma omp atomic
             c->au[cc * n2 + uid[i]] += yu[10 * cc + i];
          }
-   if ( c->mask & 4 )
+   if ( c->mask & 12 )
       for ( int i = 0; i < 4; ++i )
       {
 #pragma omp atomic
@@ -290,7 +315,7 @@ static void sapply_cb( void* vctx, const double* p, const double* blend, const i
 }
 
 /* P2P1 Stokes apply of selected blocks (mask bit 1: A u, 2: B^T p into the
- * velocity; 4: B u into the pressure), with the gemv data flow of the
+ * velocity; 4: B u into the pressure; mask 8: the 1/eta pressure mass), with the gemv data flow of the
  * elementwise operators (or_p2_apply): velocity DoFs of vertices, edges and
  * faces are written where their flag matches, cell-interior DoFs always
  * (zeroed first on Replace with Inner in flag); pressure DoFs are all Inner,
@@ -301,7 +326,7 @@ void or_stokes_apply( const or_stokesop* op, const double* u, const double* p, d
 {
    const int64_t n2 = op->l1.n, n1 = op->l0.n;
    double*       au = ( mask & 3 ) ? calloc( 3 * n2, sizeof( double ) ) : NULL;
-   double*       ap = ( mask & 4 ) ? calloc( n1, sizeof( double ) ) : NULL;
+   double*       ap = ( mask & 12 ) ? calloc( n1, sizeof( double ) ) : NULL;
    sapply_ctx    c  = { op, u, p, au, ap, mask };
 #pragma omp parallel for schedule( dynamic, 1 )
    for ( int cell = 0; cell < op->m->nc; ++cell )
@@ -318,7 +343,7 @@ void or_stokes_apply( const or_stokesop* op, const double* u, const double* p, d
          }
    /* pressure: every DoF has type Inner; cell-interior DoFs accumulate as in the
     * velocity (zeroed first on Replace with Inner in flag) */
-   if ( mask & 4 )
+   if ( mask & 12 )
       for ( int64_t i = 0; i < n1; ++i )
       {
          if ( i >= op->l0.coff )

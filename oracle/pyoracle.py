@@ -356,7 +356,7 @@ class StokesOp:
     """P2P1ElementwiseBlendingFullViscousStokesOperator on P2 level `level`: velocity
     [ux|uy|uz] in the P1 numbering of level+1, pressure in that of level (all Inner)"""
 
-    A_U, BT_P, B_U = 1, 2, 4
+    A_U, BT_P, B_U, K_MASS = 1, 2, 4, 8
 
     def __init__(self, mesh, level, c0=0.0, c1=0.0, freeslip=False):
         self.mesh = mesh
@@ -427,22 +427,25 @@ def project_mean(p):
     p -= p.sum() / len(p)
 
 
-def stokes_chebyshev(op, inv_diag, coeffs, u, bu, flag=GMG_FLAG):
-    """ChebyshevSmoother::solve on the velocity operator (order = len(coeffs))"""
+def stokes_chebyshev(op, inv_diag, coeffs, u, bu, flag=GMG_FLAG, project=None):
+    """ChebyshevSmoother::solve on the velocity operator (order = len(coeffs)); the
+    corrections are projected when `project` (default: the operator is free-slip)"""
+    if project is None:
+        project = op.freeslip
     m = op.vmask(flag)
     t2 = np.zeros_like(u)
     op.apply(u, None, t2, None, flag, StokesOp.A_U)
     t2[m] = bu[m] - t2[m]
     t1 = np.zeros_like(u)
     t1[m] = inv_diag[m] * t2[m]
-    if op.freeslip:  # ChebyshevSmoother<., P2ProjectNormalOperator> (ChebyshevSmoother.hpp:186-231)
+    if project:  # ChebyshevSmoother<., P2ProjectNormalOperator> (ChebyshevSmoother.hpp:186-231)
         op.project(t1)
     u[m] += coeffs[0] * t1[m]
     for k in range(1, len(coeffs)):
         op.apply(t1, None, t2, None, flag, StokesOp.A_U)
         t1[:] = 0.0
         t1[m] = inv_diag[m] * t2[m]
-        if op.freeslip:
+        if project:
             op.project(t1)
         u[m] += coeffs[k] * t1[m]
 
@@ -648,3 +651,268 @@ class StokesGMG:
             hist.append(self.residual_norm(u, p, bu, bp))
             k += 1
         return k, np.array(hist)
+
+
+# ---- FGMRES with the block-factorisation preconditioner ---------------------------
+def cg_solve(apply, prec, x, b, m, max_iter, rel_tol, abs_tol=0.0):
+    """CGSolver (CGSolver.hpp:94-203) on the DoFs of mask m; prec(r) -> z; returns the
+    iteration index at exit"""
+    if max_iter == 0:
+        return 0
+    r = np.zeros_like(x)
+    r[m] = b[m] - apply(x)[m]
+    z = np.zeros_like(x)
+    z[m] = prec(r)[m]
+    p = z.copy()
+    prsold = r[m] @ z[m]
+    res_start = np.sqrt(r[m] @ r[m])
+    it = 0
+    if res_start < abs_tol:
+        return 0
+    for i in range(max_iter):
+        ap = apply(p)
+        alpha = prsold / (p[m] @ ap[m])
+        x[m] += alpha * p[m]
+        r[m] -= alpha * ap[m]
+        rsnew = r[m] @ r[m]
+        it = i
+        if np.sqrt(rsnew) / res_start < rel_tol or np.sqrt(rsnew) < abs_tol:
+            break
+        z[:] = 0.0
+        z[m] = prec(r)[m]
+        prsnew = r[m] @ z[m]
+        p[m] = z[m] + (prsnew / prsold) * p[m]
+        prsold = prsnew
+    return it
+
+
+def minres_solve(apply, prec, x, b, m, max_iter, rel_tol, abs_tol=0.0):
+    """MinResSolver (MinresSolver.hpp:117-245) on the DoFs of mask m; prec(v) -> z"""
+    def pre(v):
+        z = np.zeros_like(v)
+        z[m] = prec(v)[m]
+        return z
+
+    v = np.zeros_like(x)
+    v[m] = b[m] - apply(x)[m]
+    vm, wm, w = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    z = pre(v)
+    gamma_old, gamma_new = 1.0, np.sqrt(z[m] @ v[m])
+    res_start = eta = gamma_new
+    s_old = s_new = 0.0
+    c_old = c_new = 1.0
+    it = max_iter
+    if gamma_new < abs_tol:
+        return 0
+    for i in range(max_iter):
+        z[m] /= gamma_new
+        vp = np.zeros_like(x)
+        vp[m] = apply(z)[m]
+        delta = vp[m] @ z[m]
+        vp[m] = vp[m] - (delta / gamma_new) * v[m] - (gamma_new / gamma_old) * vm[m]
+        zp = pre(vp)
+        gamma_old, gamma_new = gamma_new, np.sqrt(zp[m] @ vp[m])
+        a0 = c_new * delta - c_old * s_new * gamma_old
+        a1 = np.sqrt(a0 * a0 + gamma_new * gamma_new)
+        a2 = s_new * delta + c_old * c_new * gamma_old
+        a3 = s_old * gamma_old
+        c_old, c_new = c_new, a0 / a1
+        s_old, s_new = s_new, gamma_new / a1
+        wp = np.zeros_like(x)
+        wp[m] = z[m] / a1 - (a3 / a1) * wm[m] - (a2 / a1) * w[m]
+        x[m] += c_new * eta * wp[m]
+        eta = -s_new * eta
+        vm, v = v, vp
+        wm, w = w, wp
+        z = zp
+        if abs(eta) / res_start < rel_tol or abs(eta) < abs_tol:
+            it = i
+            break
+    return it
+
+
+def fgmres(apply, prec, x, b, m, max_krylov, rel_tol, abs_tol=0.0, restart=1000, arnoldi_tol=1e-16, hist=None):
+    """FGMRESSolver (FGMRESSolver.hpp:55-500): right-preconditioned flexible GMRES with
+    modified Gram-Schmidt and Givens rotations; hist collects the approximate
+    residuals; returns the iteration count"""
+    state = {}
+
+    def init(reset):
+        r0 = np.zeros_like(x)
+        r0[m] = b[m] - apply(x)[m]
+        beta = np.sqrt(r0[m] @ r0[m])
+        if not reset:
+            state["init"] = beta
+            if hist is not None:
+                hist.append(beta)
+        v0 = np.zeros_like(x)
+        v0[m] = r0[m] / beta
+        state.update(V=[v0], Z=[], H=[], rot=[], g=[beta], y=[])
+
+    def approximation():
+        for yi, zi in zip(state["y"], state["Z"]):
+            x[m] += yi * zi[m]
+        state["y"] = []
+
+    init(False)
+    its = max_krylov - 1
+    for j in range(1, max_krylov):
+        if j % restart == 0:
+            approximation()
+            init(True)
+            continue
+        k = j % restart
+        V, Z, H, rot, g = state["V"], state["Z"], state["H"], state["rot"], state["g"]
+        z = np.zeros_like(x)
+        z[m] = prec(V[k - 1])[m]
+        Z.append(z)
+        w = np.zeros_like(x)
+        w[m] = apply(z)[m]
+        col = np.zeros(k + 1)
+        for i in range(k):
+            col[i] = w[m] @ V[i][m]
+            w[m] -= col[i] * V[i][m]
+        wn = np.sqrt(w[m] @ w[m])
+        col[k] = wn
+        for i, (c, s_) in enumerate(rot):
+            a, bb = col[i], col[i + 1]
+            col[i], col[i + 1] = c * a + s_ * bb, -s_ * a + c * bb
+        c, s_ = 1.0, 0.0
+        if col[k] != 0.0:
+            r = np.sqrt(col[k - 1] ** 2 + col[k] ** 2)
+            s_, c = col[k] / r, col[k - 1] / r
+        rot.append((c, s_))
+        a, bb = col[k - 1], col[k]
+        col[k - 1], col[k] = c * a + s_ * bb, -s_ * a + c * bb
+        g.append(-s_ * g[k - 1])
+        g[k - 1] = c * g[k - 1]
+        H.append(col)
+        y = np.zeros(k)
+        for i in range(k - 1, -1, -1):
+            t = g[i] - sum(H[mm][i] * y[mm] for mm in range(i + 1, k))
+            y[i] = t / H[i][i]
+        state["y"] = list(y)
+        residual = abs(g[k])
+        if hist is not None:
+            hist.append(residual)
+        v = np.zeros_like(x)
+        v[m] = w[m] / wn
+        V.append(v)
+        if wn <= arnoldi_tol or residual <= abs_tol or residual / state["init"] <= rel_tol:
+            its = j
+            break
+    approximation()
+    return its
+
+
+class VelocityGMG:
+    """GeometricMultigridSolver on the velocity block (StokesMCFGMRESSolver's A-block
+    solver): Chebyshev smoothing, P2toP2QuadraticVector transfers, MINRES with
+    identity preconditioner on the coarsest level"""
+
+    def __init__(self, mesh, ops, invd, coeffs, min_level, max_level, pre=3, post=3, coarse_iter=10, coarse_tol=1e-6,
+                 project=False):
+        self.m, self.ops, self.invd, self.coeffs = mesh, ops, invd, coeffs
+        self.minL, self.maxL, self.pre, self.post = min_level, max_level, pre, post
+        self.coarse_iter, self.coarse_tol = coarse_iter, coarse_tol
+        # P2toP2QuadraticVector{Restriction,Prolongation}WithProjection (post-projection of
+        # the transferred function on the FreeSlip DoFs)
+        self.project = project
+
+    def cycle(self, L, u, b):
+        op = self.ops[L]
+        m = op.vmask(GMG_FLAG)
+
+        def A(z):
+            y = np.zeros_like(z)
+            op.apply(z, None, y, None, GMG_FLAG, StokesOp.A_U)
+            return y
+        if L == self.minL:
+            minres_solve(A, lambda v: v.copy(), u, b, m, self.coarse_iter, self.coarse_tol)
+            return
+        for _ in range(self.pre):
+            stokes_chebyshev(op, self.invd[L], self.coeffs, u, b, GMG_FLAG, project=False)
+        r = np.zeros_like(u)
+        r[m] = b[m] - A(u)[m]
+        c = self.ops[L - 1]
+        cb = np.zeros(3 * c.n2)
+        for k in range(3):
+            coarse = np.zeros(c.n2)
+            self.m.p2_restrict(L, np.ascontiguousarray(r[k * op.n2:(k + 1) * op.n2]), coarse, GMG_FLAG)
+            cb[k * c.n2:(k + 1) * c.n2] = coarse
+        if self.project:
+            c.project(cb)
+        cx = np.zeros_like(cb)
+        self.cycle(L - 1, cx, cb)
+        for k in range(3):
+            fine = np.ascontiguousarray(u[k * op.n2:(k + 1) * op.n2])
+            self.m.p2_prolongate(L - 1, np.ascontiguousarray(cx[k * c.n2:(k + 1) * c.n2]), fine, GMG_FLAG, True)
+            u[k * op.n2:(k + 1) * op.n2] = fine
+        if self.project:
+            op.project(u)
+        for _ in range(self.post):
+            stokes_chebyshev(op, self.invd[L], self.coeffs, u, b, GMG_FLAG, project=False)
+
+
+class StokesFGMRES:
+    """FGMRESSolver<Stokes> with BlockFactorisationPreconditioner (StokesMCFGMRESSolver,
+    terraneo/solvers/StokesMCFGMRESSolver.hpp:50-160; StokesBlockPreconditioners.hpp:366-448):
+    x.p = -S^-1 b.p (CG on the 1/eta pressure mass, inverse lumped diagonal as
+    preconditioner), x.u = one velocity GMG V-cycle on b.u - B^T x.p"""
+
+    def __init__(self, mesh, min_level, max_level, c0=0.0, c1=0.0, rho=None, order=3, a_pre=3, a_post=3,
+                 a_coarse_iter=10, a_coarse_tol=1e-6, schur_iter=50, schur_tol=1e-6, freeslip=False, power_its=25,
+                 seed=42):
+        self.m, self.minL, self.maxL, self.freeslip = mesh, min_level, max_level, freeslip
+        self.ops = {L: StokesOp(mesh, L, c0, c1, freeslip) for L in range(min_level, max_level + 1)}
+        self.invd = {L: self.ops[L].inv_diag() for L in self.ops}
+        if rho is None:
+            x = np.random.default_rng(seed).uniform(-1, 1, 3 * self.ops[max_level].n2)
+            rho = stokes_power_iteration(self.ops[max_level], self.invd[max_level], x, power_its)
+        self.rho = rho
+        # the TerraNeo interval factors (TerraNeoParameters.hpp:82-83): [0.1 rho, 1.2 rho]
+        coeffs = chebyshev_coefficients(order, 0.1 * rho, 1.2 * rho)
+        self.vgmg = VelocityGMG(mesh, self.ops, self.invd, coeffs, min_level, max_level, a_pre, a_post, a_coarse_iter,
+                                a_coarse_tol, project=freeslip)
+        self.schur_iter, self.schur_tol = schur_iter, schur_tol
+        op = self.ops[max_level]
+        one, d = np.ones(op.n1), np.zeros(op.n1)
+        op.apply_raw(None, one, None, d, ALL, StokesOp.K_MASS)
+        self.lumped_inv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0)
+
+    def kmass(self, p):
+        y = np.zeros_like(p)
+        self.ops[self.maxL].apply_raw(None, p, None, y, ALL, StokesOp.K_MASS)
+        return y
+
+    def precondition(self, vu, vp):
+        op = self.ops[self.maxL]
+        m = op.vmask(GMG_FLAG)
+        zp = np.zeros_like(vp)
+        cg_solve(self.kmass, lambda r: self.lumped_inv * r, zp, vp, np.ones(len(vp), bool), self.schur_iter,
+                 self.schur_tol, 1e-16)
+        tu = np.zeros_like(vu)
+        op.apply(None, zp, tu, None, GMG_FLAG, StokesOp.BT_P)
+        tu[m] = vu[m] + tu[m]
+        zp = -zp
+        if self.freeslip:
+            op.project(tu)
+        zu = np.zeros_like(vu)
+        self.vgmg.cycle(self.maxL, zu, tu)
+        return zu, zp
+
+    def solve(self, u, p, bu, bp, max_iter=100, restart=50, tol=1e-7):
+        op = self.ops[self.maxL]
+        n2 = len(u)
+        m = np.concatenate([op.vmask(GMG_FLAG), np.ones(len(p), bool)])
+        A = lambda z: np.concatenate(stokes_apply_all(op, z[:n2], z[n2:], GMG_FLAG))  # noqa: E731
+
+        def prec(v):
+            zu, zp = self.precondition(v[:n2].copy(), v[n2:].copy())
+            return np.concatenate([zu, zp])
+        x = np.concatenate([u, p])
+        hist = []
+        its = fgmres(A, prec, x, np.concatenate([bu, bp]), m, max_iter + 1, tol, 0.0, restart, hist=hist)
+        u[:] = x[:n2]
+        p[:] = x[n2:]
+        return its, np.array(hist)

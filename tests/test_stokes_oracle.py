@@ -266,3 +266,91 @@ def test_freeslip_gmg_keeps_the_velocity_tangential(fs_shell):
     U = u.reshape(3, op.n2).T
     fs = fs_shell.flags(3) == po.FREESLIP
     assert np.abs((U * X).sum(1)[fs]).max() < 1e-14 * np.abs(U).max()
+
+
+# ---- 1/eta pressure mass, CG, FGMRES ------------------------------------------------
+def test_kmass_closed_forms(cube, shell):
+    """the P1 mass with k = 1/eta: on the cube with constant k it integrates linear
+    fields and their squares exactly; on the shell 1^T M_k 1 converges at O(h^4) to
+    4 pi int r^2 / eta(r) dr"""
+    L = 2
+    op = po.StokesOp(cube, L, 0.7, 0.0)
+    k = np.exp(-0.7)
+    X = cube.coords(L)
+    f = 0.3 + X @ np.array([1.0, -2.0, 0.5])
+    Mf = np.zeros(op.n1)
+    op.apply_raw(None, f, None, Mf, po.ALL, op.K_MASS)
+    assert abs(Mf.sum() - k * (0.3 + 0.5 * (1.0 - 2.0 + 0.5))) < 1e-13
+    # int_cube (0.3 + a.x)^2 = 0.09 + 0.6 (a.1)/2 + sum_i a_i^2 / 3 + sum_{i<j} a_i a_j / 2
+    a = np.array([1.0, -2.0, 0.5])
+    exact = 0.09 + 0.3 * a.sum() + (a * a).sum() / 3 + (a[0] * a[1] + a[0] * a[2] + a[1] * a[2]) / 2
+    assert abs(f @ Mf - k * exact) < 1e-13
+    from scipy.integrate import quad
+    c0, c1 = SHELL_VISC
+    ex = 4 * np.pi * quad(lambda r: r * r * np.exp(-(c0 + c1 * r)), 0.55, 1.0)[0]
+    errs = []
+    for L in (1, 2, 3):
+        ops = po.StokesOp(shell, L, c0, c1)
+        y = np.zeros(ops.n1)
+        ops.apply_raw(None, np.ones(ops.n1), None, y, po.ALL, ops.K_MASS)
+        errs.append(abs(y.sum() / ex - 1))
+    assert errs[2] < 3e-5 and errs[0] / errs[1] > 10 and errs[1] / errs[2] > 10, errs
+
+
+def test_cg_solve_matches_a_direct_solve():
+    """preconditioned CG on a small SPD system reaches the direct solution"""
+    rng = np.random.default_rng(8)
+    Q = rng.normal(size=(30, 30))
+    A = Q @ Q.T + 30 * np.eye(30)
+    b = rng.normal(size=30)
+    x = np.zeros(30)
+    m = np.ones(30, bool)
+    po.cg_solve(lambda v: A @ v, lambda r: r / np.diag(A), x, b, m, 200, 1e-14)
+    assert np.abs(x - np.linalg.solve(A, b)).max() < 1e-10
+
+
+def test_fgmres_residuals_are_the_krylov_minimum():
+    """with a fixed linear preconditioner M, the approximate residual after k
+    FGMRES steps is min ||b - A(x0 + M y)|| over the right-preconditioned Krylov
+    space of dimension k (brute-force least squares), and the returned x has it"""
+    rng = np.random.default_rng(9)
+    n = 25
+    A = rng.normal(size=(n, n)) + 6 * np.eye(n)
+    M = np.diag(1.0 / np.diag(A)) + 0.01 * rng.normal(size=(n, n))
+    b = rng.normal(size=n)
+    m = np.ones(n, bool)
+    hist = []
+    x = np.zeros(n)
+    its = po.fgmres(lambda v: A @ v, lambda v: M @ v, x, b, m, 12, 1e-30, hist=hist)
+    assert its == 11 and len(hist) == 12
+    AM = A @ M
+    K = [b / np.linalg.norm(b)]
+    for k in range(1, 12):
+        Kmat = np.array(K).T
+        y = np.linalg.lstsq(AM @ Kmat, b, rcond=None)[0]
+        assert abs(np.linalg.norm(b - AM @ Kmat @ y) - hist[k]) < 1e-10 * np.linalg.norm(b), k
+        v = AM @ K[-1]
+        for q in K:
+            v -= (v @ q) * q
+        K.append(v / np.linalg.norm(v))
+    assert abs(np.linalg.norm(b - A @ x) - hist[-1]) < 1e-10 * np.linalg.norm(b)
+    # restarted: the true residual never grows over restarts
+    x2, h2 = np.zeros(n), []
+    po.fgmres(lambda v: A @ v, lambda v: M @ v, x2, b, m, 20, 1e-30, restart=5, hist=h2)
+    assert np.linalg.norm(b - A @ x2) < hist[3]
+
+
+def test_block_factorisation_fgmres_converges(fs_shell):
+    """FGMRES with the block-factorisation preconditioner on the free-slip shell
+    (constant viscosity): the approximate residual falls below 1e-6 and agrees
+    with the true residual of the returned iterate"""
+    s = po.StokesFGMRES(fs_shell, 0, 2, freeslip=True)
+    op = s.ops[2]
+    bu = np.random.default_rng(10).normal(size=3 * op.n2)
+    bu[~op.vmask(po.GMG_FLAG)] = 0.0
+    op.project(bu)
+    u, p = np.zeros_like(bu), np.zeros(op.n1)
+    its, hist = s.solve(u, p, bu, np.zeros(op.n1), max_iter=40, tol=1e-6)
+    assert its < 40 and hist[-1] <= 1e-6 * hist[0], (its, hist[-1] / hist[0])
+    ru, rp = po.stokes_residual(op, u, p, bu, np.zeros(op.n1), po.GMG_FLAG)
+    assert abs(np.sqrt(ru @ ru + rp @ rp) / hist[-1] - 1) < 1e-3
