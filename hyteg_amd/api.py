@@ -548,16 +548,18 @@ def stokes_minres_solve(A, x, b, level, max_iter=10, rel_tol=1e-6, abs_tol=0.0):
 
 
 def stokes_fgmres_solve(A, x, b, min_level, max_level, rho, a_pre=3, a_post=3, order=3, a_coarse_iter=10,
-                        a_coarse_tol=1e-6, schur_iter=50, schur_tol=1e-6, max_iter=100, restart=50, tol=1e-7):
+                        a_coarse_tol=1e-6, schur_iter=50, schur_tol=1e-6, max_iter=100, restart=50, tol=1e-7,
+                        velocity_dtype=F64):
     """FGMRES with the block-factorisation preconditioner (velocity block: one GMG V-cycle with
-    Chebyshev smoothing, Schur complement: CG on the 1/eta pressure mass); returns
-    (iterations, approximate residual history)"""
+    Chebyshev smoothing, in FP64 or, velocity_dtype F32 / F16, in FP32 arithmetic on FP32 / FP16
+    vectors; Schur complement: CG on the 1/eta pressure mass); returns (iterations, approximate
+    residual history)"""
     hist = np.zeros(max_iter + 1)
     its = C.c_int()
     A.ctx.check(lib().hg_stokes_fgmres_solve(A.ctx.h, A.h, _ptr(x.handles()), _ptr(b.handles()), min_level, max_level,
                                              a_pre, a_post, order, rho, a_coarse_iter, a_coarse_tol, schur_iter,
-                                             schur_tol, int(A.freeslip), max_iter, restart, tol, _ptr(hist),
-                                             C.byref(its)), "Stokes FGMRES")
+                                             schur_tol, int(A.freeslip), int(velocity_dtype), max_iter, restart, tol,
+                                             _ptr(hist), C.byref(its)), "Stokes FGMRES")
     return its.value, hist[:its.value + 1]
 
 

@@ -673,14 +673,61 @@ int hg_stokes_gmg_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg
    } );
 }
 
+} // extern "C"
+
+namespace {
+// the velocity-block solver of the block-factorisation preconditioner: one GMG
+// V-cycle (Chebyshev on [0.1, 1.2] x radius, the TerraNeo interval factors of
+// terraneo/helpers/TerraNeoParameters.hpp:82-83; MINRES with identity
+// preconditioner on minLevel, in FP64).  With free-slip the transfers project
+// their result: the coarse P A P is singular in the normal directions, and an
+// unprojected coarse right-hand side makes the V-cycle diverge.
+template < class VT, class VOp64, class VOpT >
+std::shared_ptr< Solver< VOpT > > velocityVCycle( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minL,
+                                                 uint_t L, const VOp64& A64, const VOpT& A, int aPre, int aPost,
+                                                 int order, double rho, int coarseIt, double coarseTol,
+                                                 const std::shared_ptr< P2ProjectNormalOperator >& projection,
+                                                 std::vector< std::shared_ptr< void > >& keep )
+{
+   auto cheby = std::make_shared< ChebyshevSmoother< VOpT > >( storage, minL, L );
+   cheby->setupCoefficients( uint_t( order ), rho, 1.2, 0.1 );
+   auto minres = std::make_shared< MinResSolver< VOp64 > >( storage, minL, minL, uint_t( coarseIt ), coarseTol, 0.0,
+                                                            std::make_shared< IdentityPreconditioner< VOp64 > >() );
+   std::shared_ptr< Solver< VOpT > > coarse;
+   if constexpr ( std::is_same< VOpT, VOp64 >::value )
+      coarse = minres;
+   else
+      coarse = std::make_shared< FP64CoarseSolver< VOpT, VOp64 > >( storage, minL, A64, minres );
+   keep.push_back( minres );
+   std::shared_ptr< RestrictionOperator< P2VectorFunction< VT > > >  restriction;
+   std::shared_ptr< ProlongationOperator< P2VectorFunction< VT > > > prolongation;
+   if ( projection )
+   {
+      restriction  = std::make_shared< P2toP2QuadraticVectorRestrictionWithProjectionT< VT > >( projection );
+      prolongation = std::make_shared< P2toP2QuadraticVectorProlongationWithProjectionT< VT > >( projection );
+   }
+   else
+   {
+      restriction  = std::make_shared< P2toP2QuadraticVectorRestrictionT< VT > >();
+      prolongation = std::make_shared< P2toP2QuadraticVectorProlongationT< VT > >();
+   }
+   (void) A;
+   return std::make_shared< GeometricMultigridSolver< VOpT > >( storage, cheby, coarse, restriction, prolongation, minL, L,
+                                                                uint_t( aPre ), uint_t( aPost ) );
+}
+} // namespace
+
+extern "C" {
+
 int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel,
                             int maxLevel, int aPre, int aPost, int chebyshevOrder, double spectralRadius,
                             int aCoarseMaxIter, double aCoarseTol, int schurMaxIter, double schurTol, int freeslip,
-                            int maxIter, int restart, double relTol, double* res_hist, int* iterations )
+                            int velocityDtype, int maxIter, int restart, double relTol, double* res_hist,
+                            int* iterations )
 {
    if ( !ctx || !x || !b || minLevel < 0 || maxLevel < minLevel || aPre < 0 || aPost < 0 || chebyshevOrder < 1 ||
         chebyshevOrder > 16 || !( spectralRadius > 0 ) || aCoarseMaxIter < 0 || schurMaxIter < 0 || maxIter < 1 ||
-        restart < 2 )
+        restart < 2 || velocityDtype < HG_F64 || velocityDtype > HG_F16 )
       return HG_ERR_ARG;
    return guarded( ctx, [&] {
       auto         storage = storageOf( ctx, x[0] );
@@ -689,30 +736,27 @@ int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const
          using Op  = std::decay_t< decltype( A ) >;
          using VOp = typename Op::VelocityOperator_T;
          using SOp = P1ElementwiseKMassOperator;
-         TaylorHood X( ctx, x ), B( ctx, b );
-         // velocity block: GMG with Chebyshev smoothing on [0.1 rho, 1.2 rho] (the TerraNeo
-         // interval factors, terraneo/helpers/TerraNeoParameters.hpp:82-83) and MINRES on minLevel
-         auto cheby = std::make_shared< ChebyshevSmoother< VOp > >( storage, minL, L );
-         cheby->setupCoefficients( uint_t( chebyshevOrder ), spectralRadius, 1.2, 0.1 );
-         auto coarse = std::make_shared< MinResSolver< VOp > >( storage, minL, minL, uint_t( aCoarseMaxIter ), aCoarseTol,
-                                                                0.0, std::make_shared< IdentityPreconditioner< VOp > >() );
-         // with free-slip the transfers project their result: the coarse P A P is singular
-         // in the normal directions, and an unprojected coarse right-hand side makes the
-         // velocity V-cycle diverge
-         std::shared_ptr< RestrictionOperator< P2VectorFunction< double > > >  restriction;
-         std::shared_ptr< ProlongationOperator< P2VectorFunction< double > > > prolongation;
-         if ( projection )
-         {
-            restriction  = std::make_shared< P2toP2QuadraticVectorRestrictionWithProjection >( projection );
-            prolongation = std::make_shared< P2toP2QuadraticVectorProlongationWithProjection >( projection );
-         }
+         TaylorHood                               X( ctx, x ), B( ctx, b );
+         std::vector< std::shared_ptr< void > > keep;
+         // velocity block: FP64, or FP32 arithmetic on FP32 / FP16 vectors inside the FP64 iteration
+         std::shared_ptr< Solver< VOp > > ablock;
+         auto lowPrecision = [&]( auto tag ) {
+            using VT   = decltype( tag );
+            using VOpT = P2ElementwiseBlendingFullViscousOperatorT< VT >;
+            auto Alow  = std::make_shared< VOpT >( storage, op, minL, L, projection );
+            keep.push_back( Alow );
+            auto gmg = velocityVCycle< VT, VOp, VOpT >( storage, minL, L, A.getA(), *Alow, aPre, aPost, chebyshevOrder,
+                                                        spectralRadius, aCoarseMaxIter, aCoarseTol, projection, keep );
+            ablock = std::make_shared< MixedPrecisionSolver< VOp, VOpT > >( storage, minL, L, *Alow, gmg, projection );
+         };
+         if ( velocityDtype == HG_F64 )
+            ablock = velocityVCycle< double, VOp, VOp >( storage, minL, L, A.getA(), A.getA(), aPre, aPost,
+                                                         chebyshevOrder, spectralRadius, aCoarseMaxIter, aCoarseTol,
+                                                         projection, keep );
+         else if ( velocityDtype == HG_F32 )
+            lowPrecision( float() );
          else
-         {
-            restriction  = std::make_shared< P2toP2QuadraticVectorRestriction >();
-            prolongation = std::make_shared< P2toP2QuadraticVectorProlongation >();
-         }
-         auto gmgA = std::make_shared< GeometricMultigridSolver< VOp > >( storage, cheby, coarse, restriction, prolongation,
-                                                                          minL, L, uint_t( aPre ), uint_t( aPost ) );
+            lowPrecision( f16() );
          // Schur block: CG on the 1/eta pressure mass, preconditioned with its inverse lumped diagonal
          SOp  S( ctx, op );
          auto cgS = std::make_shared< CGSolver< SOp > >(
@@ -720,7 +764,7 @@ int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const
              std::make_shared< FunctionMultiplicationPreconditioner< SOp > >( S.inverseLumpedDiagonal( storage, L, L ),
                                                                               All ) );
          // the Krylov vectors carry the whole hierarchy: the velocity V-cycle runs on them
-         auto prec = std::make_shared< BlockFactorisationPreconditioner< Op, VOp, SOp > >( storage, minL, L, S, gmgA, cgS,
+         auto prec = std::make_shared< BlockFactorisationPreconditioner< Op, VOp, SOp > >( storage, minL, L, S, ablock, cgS,
                                                                                            1, projection );
          FGMRESSolver< Op > fgmres( storage, minL, L, uint_t( maxIter ) + 1, relTol, 0.0, prec, uint_t( restart ) );
          fgmres.setIterationHook( [&]( uint_t j, double r ) {

@@ -20,6 +20,7 @@
 // |det DF| = q^2 |beta|, so the powers of q cancel in the viscous block and one
 // power remains in the divergence blocks.
 #include <array>
+#include <type_traits>
 #include <cmath>
 #include <mutex>
 
@@ -57,8 +58,37 @@ const double     kSQW[9]    = { -2.0 / 15.0, 3.0 / 40.0, 3.0 / 40.0,  3.0 / 40.0
 constexpr int kPBox  = kTile + 1; // pressure box: the tile's level-L vertices
 constexpr int kPBox3 = kPBox * kPBox * kPBox;
 
+// FP32 copies of the quadrature tables for the reduced-precision velocity block
+__constant__ float cSQ4f[9][4];
+__constant__ float cSQ4mf[9][4];
+__constant__ float cSQLf[9][4];
+__constant__ float cSQWf[9];
+
+template < class R >
+struct SQ;
+template <>
+struct SQ< double >
+{
+   static __device__ __forceinline__ const double* a4( int q ) { return cSQ4[q]; }
+   static __device__ __forceinline__ const double* b4( int q ) { return cSQ4m[q]; }
+   static __device__ __forceinline__ const double* lam( int q ) { return cSQL[q]; }
+   static __device__ __forceinline__ double        w( int q ) { return cSQW[q]; }
+};
+template <>
+struct SQ< float >
+{
+   static __device__ __forceinline__ const float* a4( int q ) { return cSQ4f[q]; }
+   static __device__ __forceinline__ const float* b4( int q ) { return cSQ4mf[q]; }
+   static __device__ __forceinline__ const float* lam( int q ) { return cSQLf[q]; }
+   static __device__ __forceinline__ float        w( int q ) { return cSQWf[q]; }
+};
+
+__device__ __forceinline__ double rexp( double v ) { return exp( v ); }
+__device__ __forceinline__ float  rexp( float v ) { return expf( v ); }
+
 // barycentric derivatives g_k = sum_j u_j dphi_j / dlam_k of a P2 function
-__device__ __forceinline__ void baryGrad( const double* a4, const double* b4, const double u[10], double g[4] )
+template < class R >
+__device__ __forceinline__ void baryGrad( const R* a4, const R* b4, const R u[10], R g[4] )
 {
    g[0] = b4[0] * u[0] + a4[1] * u[4] + a4[2] * u[5] + a4[3] * u[6];
    g[1] = b4[1] * u[1] + a4[0] * u[4] + a4[2] * u[7] + a4[3] * u[8];
@@ -67,22 +97,23 @@ __device__ __forceinline__ void baryGrad( const double* a4, const double* b4, co
 }
 
 // computational-space gradient sum_k (g_k - g_0) grad lam_k
-__device__ __forceinline__ void compGrad( const double* Dc, const double g[4], double v[3] )
+template < class R >
+__device__ __forceinline__ void compGrad( const R* Dc, const R g[4], R v[3] )
 {
-   const double d1 = g[1] - g[0], d2 = g[2] - g[0], d3 = g[3] - g[0];
+   const R d1 = g[1] - g[0], d2 = g[2] - g[0], d3 = g[3] - g[0];
 #pragma unroll
    for ( int d = 0; d < 3; ++d )
       v[d] = d1 * Dc[d] + d2 * Dc[3 + d] + d3 * Dc[6 + d];
 }
 
 // y_j += grad phi_j . tau for the 10 P2 functions (tau in computational space)
-__device__ __forceinline__ void testGrad( const double* a4, const double* b4, const double* Dc, const double tau[3],
-                                          double y[10] )
+template < class R >
+__device__ __forceinline__ void testGrad( const R* a4, const R* b4, const R* Dc, const R tau[3], R y[10] )
 {
-   const double e1 = Dc[0] * tau[0] + Dc[1] * tau[1] + Dc[2] * tau[2];
-   const double e2 = Dc[3] * tau[0] + Dc[4] * tau[1] + Dc[5] * tau[2];
-   const double e3 = Dc[6] * tau[0] + Dc[7] * tau[1] + Dc[8] * tau[2];
-   const double e0 = -( e1 + e2 + e3 );
+   const R e1 = Dc[0] * tau[0] + Dc[1] * tau[1] + Dc[2] * tau[2];
+   const R e2 = Dc[3] * tau[0] + Dc[4] * tau[1] + Dc[5] * tau[2];
+   const R e3 = Dc[6] * tau[0] + Dc[7] * tau[1] + Dc[8] * tau[2];
+   const R e0 = -( e1 + e2 + e3 );
    y[0] += b4[0] * e0;
    y[1] += b4[1] * e1;
    y[2] += b4[2] * e2;
@@ -97,29 +128,31 @@ __device__ __forceinline__ void testGrad( const double* a4, const double* b4, co
 
 // quadrature point q of the thread's micro-cell: computational point x, the
 // shell scalars (q = 1, w = 0, beta = 1 without blending), 1/beta and |F(x)|
+template < class R >
 struct QPoint
 {
-   double x[3], qq, w[3], beta, ib, r;
+   R x[3], qq, w[3], beta, ib, r;
 };
 
-__device__ __forceinline__ QPoint qpoint( const P2CellGeom& g, const TileThread< double >& th, int q )
+template < class R >
+__device__ __forceinline__ QPoint< R > qpoint( const P2CellGeomT< R >& g, const TileThread< R >& th, int q )
 {
-   QPoint        P;
-   const double* xq = g.XQ[th.t][q];
+   QPoint< R > P;
+   const R*    xq = g.XQ[th.t][q];
 #pragma unroll
    for ( int d = 0; d < 3; ++d )
       P.x[d] = th.p0[d] + xq[d];
    if ( g.kind )
    {
-      const double ls = th.ls0 + xq[3];
+      const R ls = th.ls0 + xq[3];
       shellScalars( g, P.x, ls, P.qq, P.w, P.beta );
       P.ib = fastRcp( P.beta );
       P.r  = g.r0 + fabs( ls ) * g.dr;
    }
    else
    {
-      P.qq = P.beta = P.ib = 1.0;
-      P.w[0] = P.w[1] = P.w[2] = 0.0;
+      P.qq = P.beta = P.ib = R( 1 );
+      P.w[0] = P.w[1] = P.w[2] = R( 0 );
       P.r                      = sqrt( P.x[0] * P.x[0] + P.x[1] * P.x[1] + P.x[2] * P.x[2] );
    }
    return P;
@@ -128,26 +161,30 @@ __device__ __forceinline__ QPoint qpoint( const P2CellGeom& g, const TileThread<
 // y (=|+=) the selected blocks on the cell lattices (halo positions included):
 // MASK bit 1: yu = A u, bit 2: yu += B^T p, bit 4: yp = B u; MASK 8 alone:
 // yp = M_k p, the P1 mass with k = 1/eta (the Schur-complement approximation
-// of the block-factorisation preconditioner), 4-point degree-2 rule
-template < int MASK >
+// of the block-factorisation preconditioner), 4-point degree-2 rule.
+// T is the value type of the functions (double; float or half for the
+// reduced-precision velocity block), the arithmetic runs in Acc<T>.
+template < int MASK, class T >
 __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
-    const P2CellGeom* __restrict__ geom, double c0, double c1, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
-    int64_t sc0, int64_t w0, const double* __restrict__ u0, const double* __restrict__ u1, const double* __restrict__ u2,
-    const double* __restrict__ pin, double* __restrict__ y0, double* __restrict__ y1, double* __restrict__ y2,
-    double* __restrict__ yp )
+    const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, typename Acc< T >::type c0,
+    typename Acc< T >::type c1, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1, int64_t sc0, int64_t w0,
+    const T* __restrict__ u0, const T* __restrict__ u1, const T* __restrict__ u2, const T* __restrict__ pin,
+    T* __restrict__ y0, T* __restrict__ y1, T* __restrict__ y2, T* __restrict__ yp )
 {
+   using R  = typename Acc< T >::type;
+   using QC = SQ< R >;
    constexpr bool DO_A = MASK & 1, DO_BT = MASK & 2, DO_B = MASK & 4, DO_M = MASK & 8;
    constexpr bool NEED_U = DO_A || DO_B, OUT_U = DO_A || DO_BT, NEED_P = DO_BT || DO_M, OUT_P = DO_B || DO_M;
    // LDS boxes: first the tile's input values, then the accumulators
-   __shared__ double su[3][kBox3];
-   __shared__ double sp[kPBox3];
+   __shared__ R su[3][kBox3];
+   __shared__ R sp[kPBox3];
    const int64_t lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
       return;
    const TileCoord tc    = tileOf( lb, mm );
    const int64_t   base1 = tc.c * sc1, base0 = tc.c * sc0;
    const int32_t   top1 = int32_t( 2 * mm ), top0 = int32_t( mm );
-   const double*   uin[3] = { u0, u1, u2 };
+   const T*        uin[3] = { u0, u1, u2 };
    if constexpr ( NEED_U )
       for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
       {
@@ -157,92 +194,92 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
          const int64_t idx = base1 + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
 #pragma unroll
          for ( int c = 0; c < 3; ++c )
-            su[c][p] = in ? uin[c][idx] : 0.0;
+            su[c][p] = in ? R( toAcc( uin[c][idx] ) ) : R( 0 );
       }
    if constexpr ( NEED_P )
       for ( int p = threadIdx.x; p < kPBox3; p += kTileThreads )
       {
          const int32_t lx = p % kPBox, ly = ( p / kPBox ) % kPBox, lz = p / ( kPBox * kPBox );
          const int32_t X = int32_t( tc.x0 ) + lx, Y = int32_t( tc.y0 ) + ly, Z = int32_t( tc.z0 ) + lz;
-         sp[p]           = ( X + Y + Z <= top0 ) ? pin[base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X] : 0.0;
+         sp[p] = ( X + Y + Z <= top0 ) ? R( toAcc( pin[base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X] ) ) : R( 0 );
       }
    __syncthreads();
-   const P2CellGeom&          g  = geom[tc.c];
-   const TileThread< double > th = tileThread( g, tc, mm );
-   int                        pos[10], ppos[4];
+   const P2CellGeomT< R >& g  = geom[tc.c];
+   const TileThread< R >   th = tileThread( g, tc, mm );
+   int                     pos[10], ppos[4];
    elementBoxPositions( th, pos );
 #pragma unroll
    for ( int k = 0; k < 4; ++k )
       ppos[k] = ( th.ax + cVof[th.t][k][0] ) + kPBox * ( ( th.ay + cVof[th.t][k][1] ) + kPBox * ( th.az + cVof[th.t][k][2] ) );
-   double u[3][10], pv[4];
+   R u[3][10], pv[4];
    if constexpr ( NEED_U )
    {
 #pragma unroll
       for ( int c = 0; c < 3; ++c )
 #pragma unroll
          for ( int i = 0; i < 10; ++i )
-            u[c][i] = th.valid ? su[c][pos[i]] : 0.0;
+            u[c][i] = th.valid ? su[c][pos[i]] : R( 0 );
    }
    if constexpr ( NEED_P )
    {
 #pragma unroll
       for ( int k = 0; k < 4; ++k )
-         pv[k] = th.valid ? sp[ppos[k]] : 0.0;
+         pv[k] = th.valid ? sp[ppos[k]] : R( 0 );
    }
    __syncthreads();
    if constexpr ( OUT_U )
       for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
-         su[0][p] = su[1][p] = su[2][p] = 0.0;
+         su[0][p] = su[1][p] = su[2][p] = R( 0 );
    if constexpr ( OUT_P )
       for ( int p = threadIdx.x; p < kPBox3; p += kTileThreads )
-         sp[p] = 0.0;
+         sp[p] = R( 0 );
    __syncthreads();
    if ( th.valid )
    {
-      const double* Dc   = g.Dc[th.t];
-      const double  adet = g.adet[th.t];
-      double        y[3][10], yq[4];
+      const R* Dc   = g.Dc[th.t];
+      const R  adet = g.adet[th.t];
+      R        y[3][10], yq[4];
 #pragma unroll
       for ( int c = 0; c < 3; ++c )
 #pragma unroll
          for ( int i = 0; i < 10; ++i )
-            y[c][i] = 0.0;
+            y[c][i] = R( 0 );
 #pragma unroll
       for ( int k = 0; k < 4; ++k )
-         yq[k] = 0.0;
+         yq[k] = R( 0 );
       if constexpr ( DO_A )
       {
 #pragma unroll 1
          for ( int q = 0; q < 5; ++q )
          {
-            const double* a4 = cSQ4[q];
-            const double* b4 = cSQ4m[q];
-            const QPoint  P  = qpoint( g, th, q );
+            const R*          a4 = QC::a4( q );
+            const R*          b4 = QC::b4( q );
+            const QPoint< R > P  = qpoint( g, th, q );
             // q DF^-T (computational gradient) of each component: rows of q grad u
-            double G[3][3];
+            R G[3][3];
 #pragma unroll
             for ( int c = 0; c < 3; ++c )
             {
-               double gm[4];
+               R gm[4];
                baryGrad( a4, b4, u[c], gm );
                compGrad( Dc, gm, G[c] );
-               const double t = ( P.x[0] * G[c][0] + P.x[1] * G[c][1] + P.x[2] * G[c][2] ) * P.ib;
+               const R t = ( P.x[0] * G[c][0] + P.x[1] * G[c][1] + P.x[2] * G[c][2] ) * P.ib;
 #pragma unroll
                for ( int d = 0; d < 3; ++d )
                   G[c][d] -= P.w[d] * t;
             }
-            const double eta = exp( c0 + c1 * P.r );
-            const double tr  = ( 2.0 / 3.0 ) * eta * ( G[0][0] + G[1][1] + G[2][2] );
-            const double cc  = cSQW[q] * adet * fabs( P.beta );
+            const R eta = rexp( c0 + c1 * P.r );
+            const R tr  = ( R( 2 ) / R( 3 ) ) * eta * ( G[0][0] + G[1][1] + G[2][2] );
+            const R cc  = QC::w( q ) * adet * fabs( P.beta );
 #pragma unroll
             for ( int c = 0; c < 3; ++c )
             {
                // row c of the stress (times q), then q DF^-1 of it
-               double s[3];
+               R s[3];
 #pragma unroll
                for ( int d = 0; d < 3; ++d )
-                  s[d] = eta * ( G[c][d] + G[d][c] ) - ( c == d ? tr : 0.0 );
-               const double t = ( P.w[0] * s[0] + P.w[1] * s[1] + P.w[2] * s[2] ) * P.ib;
+                  s[d] = eta * ( G[c][d] + G[d][c] ) - ( c == d ? tr : R( 0 ) );
+               const R t = ( P.w[0] * s[0] + P.w[1] * s[1] + P.w[2] * s[2] ) * P.ib;
 #pragma unroll
                for ( int d = 0; d < 3; ++d )
                   s[d] = cc * ( s[d] - P.x[d] * t );
@@ -255,11 +292,11 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
 #pragma unroll 1
          for ( int q = 5; q < 9; ++q )
          {
-            const double* lam = cSQL[q];
-            const QPoint  P   = qpoint( g, th, q );
+            const R*          lam = QC::lam( q );
+            const QPoint< R > P   = qpoint( g, th, q );
             // |det D(F J)| = q^2 |beta| adet; k = 1/eta at the blended point
-            const double cc = cSQW[q] * adet * fabs( P.beta ) * P.qq * P.qq * exp( -( c0 + c1 * P.r ) );
-            const double s  = cc * ( lam[0] * pv[0] + lam[1] * pv[1] + lam[2] * pv[2] + lam[3] * pv[3] );
+            const R cc = QC::w( q ) * adet * fabs( P.beta ) * P.qq * P.qq * rexp( -( c0 + c1 * P.r ) );
+            const R s  = cc * ( lam[0] * pv[0] + lam[1] * pv[1] + lam[2] * pv[2] + lam[3] * pv[3] );
 #pragma unroll
             for ( int k = 0; k < 4; ++k )
                yq[k] += lam[k] * s;
@@ -270,39 +307,39 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
 #pragma unroll 1
          for ( int q = 5; q < 9; ++q )
          {
-            const double* a4  = cSQ4[q];
-            const double* b4  = cSQ4m[q];
-            const double* lam = cSQL[q];
-            const QPoint  P   = qpoint( g, th, q );
-            const double  cc  = cSQW[q] * adet * fabs( P.beta ) * P.qq;
+            const R*          a4  = QC::a4( q );
+            const R*          b4  = QC::b4( q );
+            const R*          lam = QC::lam( q );
+            const QPoint< R > P   = qpoint( g, th, q );
+            const R           cc  = QC::w( q ) * adet * fabs( P.beta ) * P.qq;
             if constexpr ( DO_B )
             {
                // q div u = trace of q grad u
-               double dq = 0.0;
+               R dq = R( 0 );
 #pragma unroll
                for ( int c = 0; c < 3; ++c )
                {
-                  double gm[4], v[3];
+                  R gm[4], v[3];
                   baryGrad( a4, b4, u[c], gm );
                   compGrad( Dc, gm, v );
                   dq += v[c] - P.w[c] * ( P.x[0] * v[0] + P.x[1] * v[1] + P.x[2] * v[2] ) * P.ib;
                }
-               const double s = -cc * dq;
+               const R s = -cc * dq;
 #pragma unroll
                for ( int k = 0; k < 4; ++k )
                   yq[k] += lam[k] * s;
             }
             if constexpr ( DO_BT )
             {
-               const double s = -cc * ( lam[0] * pv[0] + lam[1] * pv[1] + lam[2] * pv[2] + lam[3] * pv[3] );
+               const R s = -cc * ( lam[0] * pv[0] + lam[1] * pv[1] + lam[2] * pv[2] + lam[3] * pv[3] );
 #pragma unroll
                for ( int c = 0; c < 3; ++c )
                {
                   // q DF^-1 e_c, scaled
-                  double tau[3];
+                  R tau[3];
 #pragma unroll
                   for ( int d = 0; d < 3; ++d )
-                     tau[d] = s * ( ( c == d ? 1.0 : 0.0 ) - P.x[d] * P.w[c] * P.ib );
+                     tau[d] = s * ( ( c == d ? R( 1 ) : R( 0 ) ) - P.x[d] * P.w[c] * P.ib );
                   testGrad( a4, b4, Dc, tau, y[c] );
                }
             }
@@ -326,7 +363,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
    __syncthreads();
    if constexpr ( OUT_U )
    {
-      double* yout[3] = { y0, y1, y2 };
+      T* yout[3] = { y0, y1, y2 };
       for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
       {
          const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
@@ -373,10 +410,10 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileDiag( const P2CellG
    double        dg[3][10] = {};
    for ( int q = 0; q < 5; ++q )
    {
-      const double* a4 = cSQ4[q];
-      const double* b4 = cSQ4m[q];
-      const QPoint  P  = qpoint( g, th, q );
-      const double  cc = cSQW[q] * adet * fabs( P.beta ) * exp( c0 + c1 * P.r );
+      const double*          a4 = cSQ4[q];
+      const double*          b4 = cSQ4m[q];
+      const QPoint< double > P  = qpoint( g, th, q );
+      const double           cc = cSQW[q] * adet * fabs( P.beta ) * exp( c0 + c1 * P.r );
       // computational gradients of lam_0..3
       double gl[4][3];
       for ( int d = 0; d < 3; ++d )
@@ -495,19 +532,22 @@ __global__ void kInvertFlagged( int64_t ncell, int64_t nb, const uint8_t* __rest
 
 // u -= (u . n) n on the owned boundary DoFs whose type is in flag
 // (P2ProjectNormalOperator::project, p1functionspace/freeslip/VertexDoFProjectNormal.hpp:43-120)
+template < class T >
 __global__ void kProjectNormal( int64_t nb, int64_t ncell, const uint8_t* __restrict__ bndFlag, int flag,
-                                const double* __restrict__ nrm, double* __restrict__ u0, double* __restrict__ u1,
-                                double* __restrict__ u2 )
+                                const double* __restrict__ nrm, T* __restrict__ u0, T* __restrict__ u1,
+                                T* __restrict__ u2 )
 {
+   using R         = typename Acc< T >::type;
    const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
    if ( b >= nb || !( bndFlag[b] & flag ) )
       return;
-   const double* n = nrm + 3 * b;
-   const int64_t p = ncell + b;
-   const double  d = n[0] * u0[p] + n[1] * u1[p] + n[2] * u2[p];
-   u0[p] -= d * n[0];
-   u1[p] -= d * n[1];
-   u2[p] -= d * n[2];
+   const double* n    = nrm + 3 * b;
+   const int64_t p    = ncell + b;
+   const R       v[3] = { R( toAcc( u0[p] ) ), R( toAcc( u1[p] ) ), R( toAcc( u2[p] ) ) };
+   const R       d    = R( n[0] ) * v[0] + R( n[1] ) * v[1] + R( n[2] ) * v[2];
+   u0[p]              = fromAcc< T >( v[0] - d * R( n[0] ) );
+   u1[p]              = fromAcc< T >( v[1] - d * R( n[1] ) );
+   u2[p]              = fromAcc< T >( v[2] - d * R( n[2] ) );
 }
 
 struct StokesOperator : OperatorBase
@@ -516,16 +556,18 @@ struct StokesOperator : OperatorBase
    int                                                 minL = 0, maxL = 0;
    double                                              c0 = 0.0, c1 = 0.0;
    std::map< int, DevBuf< P2CellGeom > >               tgeom;   // by P2 level
+   std::map< int, DevBuf< P2CellGeomT< float > > >    tgeom32; // the same in FP32 (reduced-precision velocity block)
    std::map< int, std::array< DevBuf< double >, 3 > > invDiag; // velocity, lattice L+1
    std::map< int, DevBuf< double > >                   pspgInv, lumpedInv; // lattice L
 };
 
 namespace {
 
-void checkVelocity( const StokesOperator& op, const Function& f, int L )
+void checkVelocity( const StokesOperator& op, const Function& f, int L, int dtype = 0 )
 {
-   if ( f.st != op.st || f.off != 1 || f.lv != 0 || f.dtype != 0 )
-      throw Error( HG_ERR_ARG, "Stokes velocity components must be FP64 P2 functions on the operator's storage" );
+   if ( f.st != op.st || f.off != 1 || f.lv != 0 || f.dtype != dtype )
+      throw Error( HG_ERR_ARG, dtype == 0 ? "Stokes velocity components must be FP64 P2 functions on the operator's storage" :
+                                            "the velocity components of a reduced-precision apply must share one type" );
    if ( L < op.minL || L > op.maxL || L < f.minL || L > f.maxL )
       throw Error( HG_ERR_ARG, "level outside the operator's or function's level range" );
 }
@@ -542,13 +584,14 @@ void checkPressure( const StokesOperator& op, const Function& f, int L )
 // createAllInnerBC): a velocity flag selects every pressure DoF or none
 int pressureFlag( int flag ) { return ( flag & kInner ) ? kAll : 0; }
 
-void zeroTarget( Context* ctx, Storage& st, Lattice& lat, double* d, bool replaceAll )
+template < class T >
+void zeroTarget( Context* ctx, Storage& st, Lattice& lat, T* d, bool replaceAll )
 {
    if ( replaceAll )
-      kFillRange< double ><<< gridFor( lat.nc * lat.sc ), kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, d, 0.0 );
+      kFillRange< T ><<< gridFor( lat.nc * lat.sc ), kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, d, 0.0 );
    else if ( lat.haloPerCell() > 0 )
-      kZeroHalos< double ><<< gridFor( lat.haloPerCell() * lat.nc ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lat ),
-                                                                                              lat.nc, d );
+      kZeroHalos< T ><<< gridFor( lat.haloPerCell() * lat.nc ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lat ),
+                                                                                         lat.nc, d );
    HG_CHECK_HIP( hipGetLastError() );
 }
 
@@ -556,10 +599,57 @@ template < int MASK >
 void launchStokes( Context* ctx, StokesOperator& op, int L, int64_t ntiles, Lattice& l1, Lattice& l0, double* const u[3],
                    const double* p, double* const y[3], double* yp )
 {
-   kStokesTileApply< MASK ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+   kStokesTileApply< MASK, double ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
        op.tgeom.at( L ).p, op.c0, op.c1, int64_t( 1 ) << L, ntiles, l1.sc, l1.w, l0.sc, l0.w, u[0], u[1], u[2], p, y[0],
        y[1], y[2], yp );
    HG_CHECK_HIP( hipGetLastError() );
+}
+
+// the velocity block alone on FP32 or FP16 velocity functions (FP32 arithmetic)
+void stokesViscousLowPrecision( Context* ctx, StokesOperator& op, Function* const src[4], Function* const dst[4], int L,
+                                int flag, int update )
+{
+   const int dtype = src[0]->dtype;
+   for ( int c = 0; c < 3; ++c )
+   {
+      checkVelocity( op, *src[c], L, dtype );
+      checkVelocity( op, *dst[c], L, dtype );
+      for ( int k = 0; k < 3; ++k )
+         if ( src[c] == dst[k] )
+            throw Error( HG_ERR_ARG, "apply needs distinct src and dst functions" );
+   }
+   Storage&      st = *op.st;
+   Lattice&      l1 = st.lattice( ctx, L + 1 );
+   Lattice&      l0 = st.lattice( ctx, L );
+   const int64_t mm = int64_t( 1 ) << L;
+   if ( l1.w > 1024 )
+      throw Error( HG_ERR_UNSUPPORT, "Stokes apply supports levels up to 9 (32-bit lattice offsets)" );
+   const int64_t ntiles = tilesPerCell( mm ) * l1.nc;
+   dispatchType( dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      if constexpr ( std::is_same< T, double >::value )
+         throw Error( HG_ERR_ARG, "FP64 velocity goes through the FP64 apply" );
+      else
+      {
+         T* u[3];
+         T* y[3];
+         for ( int c = 0; c < 3; ++c )
+         {
+            u[c] = (T*) src[c]->ptr( L );
+            pullHalos( ctx, st, l1, u[c], dtype );
+            y[c] = (T*) dst[c]->ptr( L );
+            zeroTarget( ctx, st, l1, y[c], update == HG_REPLACE && ( flag & kInner ) );
+         }
+         ctx->beginTiming();
+         kStokesTileApply< 1, T ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+             op.tgeom32.at( L ).p, float( op.c0 ), float( op.c1 ), mm, ntiles, l1.sc, l1.w, l0.sc, l0.w, u[0], u[1], u[2],
+             nullptr, y[0], y[1], y[2], nullptr );
+         HG_CHECK_HIP( hipGetLastError() );
+         ctx->endTiming();
+         for ( int c = 0; c < 3; ++c )
+            pushAdditive( ctx, st, l1, y[c], dtype, kAll ^ flag, update == HG_REPLACE );
+      }
+   } );
 }
 
 void stokesApply( Context* ctx, StokesOperator& op, Function* const src[4], Function* const dst[4], int L, int flag,
@@ -571,6 +661,11 @@ void stokesApply( Context* ctx, StokesOperator& op, Function* const src[4], Func
    for ( int c = 0; c < 4; ++c )
       if ( ( ( c < 3 ? needU : needP ) && !src[c] ) || ( ( c < 3 ? outU : outP ) && !dst[c] ) )
          throw Error( HG_ERR_ARG, "a function the block mask needs is missing" );
+   if ( mask == 1 && src[0]->dtype != 0 )
+   {
+      stokesViscousLowPrecision( ctx, op, src, dst, L, flag, update );
+      return;
+   }
    for ( int c = 0; c < 3; ++c )
    {
       if ( needU )
@@ -763,6 +858,21 @@ int hg_stokes_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, double
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQ4m ), q4m, sizeof( q4m ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQL ), kSQP, sizeof( kSQP ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQW ), kSQW, sizeof( kSQW ) ) );
+         float f4[9][4], f4m[9][4], fl[9][4], fw[9];
+         for ( int q = 0; q < 9; ++q )
+         {
+            for ( int k = 0; k < 4; ++k )
+            {
+               f4[q][k]  = float( q4[q][k] );
+               f4m[q][k] = float( q4m[q][k] );
+               fl[q][k]  = float( kSQP[q][k] );
+            }
+            fw[q] = float( kSQW[q] );
+         }
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQ4f ), f4, sizeof( f4 ) ) );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQ4mf ), f4m, sizeof( f4m ) ) );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQLf ), fl, sizeof( fl ) ) );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cSQWf ), fw, sizeof( fw ) ) );
       } );
       Storage& st = ctx->storage( s );
       auto     op = std::make_unique< StokesOperator >();
@@ -777,6 +887,7 @@ int hg_stokes_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, double
          std::vector< P2CellGeom > gl = tg;
          tileGeometryLevel( gl, int64_t( 1 ) << L, kSQP, 9 );
          op->tgeom[L].upload( ctx, gl );
+         op->tgeom32[L].upload( ctx, toFloat( gl ) );
          st.lattice( ctx, L );
          st.lattice( ctx, L + 1 );
       }
@@ -850,19 +961,21 @@ int hg_p2_project_normal( hg_ctx* h, const hg_handle* u, int level, int flag )
       for ( int c = 0; c < 3; ++c )
       {
          f[c] = &ctx->function( u[c] );
-         if ( f[c]->off != 1 || f[c]->lv != 0 || f[c]->dtype != 0 || f[c]->st != f[0]->st || level < f[c]->minL ||
-              level > f[c]->maxL )
-            throw Error( HG_ERR_ARG, "normal projection needs three FP64 P2 functions of one storage" );
+         if ( f[c]->off != 1 || f[c]->lv != 0 || f[c]->dtype != f[0]->dtype || f[c]->st != f[0]->st ||
+              level < f[c]->minL || level > f[c]->maxL )
+            throw Error( HG_ERR_ARG, "normal projection needs three P2 functions of one storage and one type" );
       }
       Storage& st  = *f[0]->st;
       Lattice& lat = st.lattice( ctx, level + 1 );
       if ( lat.nb == 0 )
          return;
       const double* nrm = boundaryNormals( ctx, st, lat );
-      kProjectNormal<<< gridFor( lat.nb ), kBlock, 0, ctx->stream >>>( lat.nb, lat.nc * lat.sc, lat.bndFlag.p, flag, nrm,
-                                                                       (double*) f[0]->ptr( level ),
-                                                                       (double*) f[1]->ptr( level ),
-                                                                       (double*) f[2]->ptr( level ) );
+      dispatchType( f[0]->dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         kProjectNormal< T ><<< gridFor( lat.nb ), kBlock, 0, ctx->stream >>>(
+             lat.nb, lat.nc * lat.sc, lat.bndFlag.p, flag, nrm, (T*) f[0]->ptr( level ), (T*) f[1]->ptr( level ),
+             (T*) f[2]->ptr( level ) );
+      } );
       HG_CHECK_HIP( hipGetLastError() );
    } );
 }

@@ -399,11 +399,15 @@ int hg_stokes_gmg_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg
  * mass (mask 8 of hg_stokes_apply) preconditioned with its inverse lumped
  * diagonal.  Iterates until the approximate residual drops by relTol or maxIter
  * iterations ran (restart every `restart`); res_hist[0..iterations] holds the
- * initial and the approximate residual 2-norms over Inner | Neumann | FreeSlip. */
+ * initial and the approximate residual 2-norms over Inner | Neumann | FreeSlip.
+ * velocityDtype HG_F32 / HG_F16 runs the velocity V-cycle in FP32 arithmetic on
+ * FP32 / FP16 vectors (coarsest level in FP64) inside the FP64 iteration: the
+ * mixed-precision variant; HG_F64 runs it in FP64. */
 int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel,
                             int maxLevel, int aPre, int aPost, int chebyshevOrder, double spectralRadius,
                             int aCoarseMaxIter, double aCoarseTol, int schurMaxIter, double schurTol, int freeslip,
-                            int maxIter, int restart, double relTol, double* res_hist, int* iterations );
+                            int velocityDtype, int maxIter, int restart, double relTol, double* res_hist,
+                            int* iterations );
 
 #ifdef __cplusplus
 }

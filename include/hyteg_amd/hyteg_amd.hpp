@@ -1032,8 +1032,8 @@ template < typename VT >
 class P2VectorFunction
 {
  public:
-   using valueType = VT;
-   using ValueType = VT;
+   using valueType = typename ScalarOf< VT >::type; // the arithmetic type (float for f16 storage)
+   using ValueType = valueType;
 
    P2VectorFunction( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
                      uint_t maxLevel, int nvec = 1 )
@@ -1052,27 +1052,27 @@ class P2VectorFunction
    const P2Function< VT >& operator[]( int k ) const { return *c_[k]; }
    hg_ctx*                 ctx() const { return c_[0]->ctx(); }
 
-   void assign( const std::vector< VT >& scalars, const std::vector< std::reference_wrapper< const P2VectorFunction > >& fs,
+   void assign( const std::vector< ValueType >& scalars, const std::vector< std::reference_wrapper< const P2VectorFunction > >& fs,
                 uint_t level, DoFType flag = All ) const
    {
       for ( int k = 0; k < 3; ++k )
          c_[k]->assign( scalars, component( fs, k ), level, flag );
    }
-   void add( const std::vector< VT >& scalars, const std::vector< std::reference_wrapper< const P2VectorFunction > >& fs,
+   void add( const std::vector< ValueType >& scalars, const std::vector< std::reference_wrapper< const P2VectorFunction > >& fs,
              uint_t level, DoFType flag = All ) const
    {
       for ( int k = 0; k < 3; ++k )
          c_[k]->add( scalars, component( fs, k ), level, flag );
    }
-   void interpolate( VT constant, uint_t level, DoFType flag = All ) const
+   void interpolate( ValueType constant, uint_t level, DoFType flag = All ) const
    {
       for ( int k = 0; k < 3; ++k )
          c_[k]->interpolate( constant, level, flag );
    }
-   void setToZero( uint_t level ) const { interpolate( VT( 0 ), level, All ); }
-   VT   dotGlobal( const P2VectorFunction& o, uint_t level, DoFType flag = All ) const
+   void setToZero( uint_t level ) const { interpolate( ValueType( 0 ), level, All ); }
+   ValueType dotGlobal( const P2VectorFunction& o, uint_t level, DoFType flag = All ) const
    {
-      VT s = 0;
+      ValueType s = 0;
       for ( int k = 0; k < 3; ++k )
          s += c_[k]->dotGlobal( *o.c_[k], level, flag );
       return s;
@@ -1082,6 +1082,13 @@ class P2VectorFunction
    {
       for ( int k = 0; k < 3; ++k )
          c_[k]->multElementwise( component( fs, k ), level, flag );
+   }
+   // copyFrom<OtherT>: the values of another precision, cast
+   template < typename OT >
+   void copyFrom( const P2VectorFunction< OT >& o, uint_t level ) const
+   {
+      for ( int k = 0; k < 3; ++k )
+         c_[k]->copyFrom( o[k], level );
    }
 
  private:
@@ -1184,7 +1191,8 @@ class P2ProjectNormalOperator
  public:
    P2ProjectNormalOperator() = default;
    P2ProjectNormalOperator( const std::shared_ptr< PrimitiveStorage >&, uint_t, uint_t ) {}
-   void project( const P2VectorFunction< double >& u, uint_t level, DoFType flag ) const
+   template < typename VT >
+   void project( const P2VectorFunction< VT >& u, uint_t level, DoFType flag ) const
    {
       if ( !flag )
          return;
@@ -1774,46 +1782,52 @@ class P2P1StokesToP2P1StokesProlongationWithFreeSlipProjection : public P2P1Stok
 // P2toP2QuadraticVectorRestriction / Prolongation
 // (gridtransferoperators/P2toP2QuadraticVectorRestriction.hpp, ...Prolongation.hpp):
 // the quadratic transfers of each velocity component
-class P2toP2QuadraticVectorRestriction : public RestrictionOperator< P2VectorFunction< double > >
+// (VT: the value type of the transferred functions)
+template < typename VT = double >
+class P2toP2QuadraticVectorRestrictionT : public RestrictionOperator< P2VectorFunction< VT > >
 {
  public:
-   void restrict( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   void restrict( const P2VectorFunction< VT >& f, uint_t sourceLevel, DoFType flag ) const override
    {
       for ( int k = 0; k < 3; ++k )
          check( f.ctx(), hg_p2_restrict( f.ctx(), f[k].handle(), int( sourceLevel ), flag ), "restrict" );
    }
 };
-class P2toP2QuadraticVectorProlongation : public ProlongationOperator< P2VectorFunction< double > >
+template < typename VT = double >
+class P2toP2QuadraticVectorProlongationT : public ProlongationOperator< P2VectorFunction< VT > >
 {
  public:
-   void prolongate( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   void prolongate( const P2VectorFunction< VT >& f, uint_t sourceLevel, DoFType flag ) const override
    {
       for ( int k = 0; k < 3; ++k )
          check( f.ctx(), hg_p2_prolongate( f.ctx(), f[k].handle(), int( sourceLevel ), flag, HG_REPLACE ), "prolongate" );
    }
-   void prolongateAndAdd( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   void prolongateAndAdd( const P2VectorFunction< VT >& f, uint_t sourceLevel, DoFType flag ) const override
    {
       for ( int k = 0; k < 3; ++k )
          check( f.ctx(), hg_p2_prolongate( f.ctx(), f[k].handle(), int( sourceLevel ), flag, HG_ADD ), "prolongate" );
    }
 };
+using P2toP2QuadraticVectorRestriction  = P2toP2QuadraticVectorRestrictionT< double >;
+using P2toP2QuadraticVectorProlongation = P2toP2QuadraticVectorProlongationT< double >;
 
 // P2toP2QuadraticVector{Restriction,Prolongation}WithProjection with their default
 // configuration (P2toP2QuadraticVectorRestriction.hpp:54-150,
 // P2toP2QuadraticVectorProlongation.hpp:64-230: no pre-projection, post-projection
 // allowed to change the destination): transfer, then project the result on the
 // projectionFlag DoFs of the target level
-class P2toP2QuadraticVectorRestrictionWithProjection : public P2toP2QuadraticVectorRestriction
+template < typename VT = double >
+class P2toP2QuadraticVectorRestrictionWithProjectionT : public P2toP2QuadraticVectorRestrictionT< VT >
 {
  public:
-   explicit P2toP2QuadraticVectorRestrictionWithProjection( std::shared_ptr< P2ProjectNormalOperator > projection,
-                                                            DoFType projectionFlag = FreeslipBoundary )
+   explicit P2toP2QuadraticVectorRestrictionWithProjectionT( std::shared_ptr< P2ProjectNormalOperator > projection,
+                                                             DoFType projectionFlag = FreeslipBoundary )
    : projection_( std::move( projection ) )
    , projectionFlag_( projectionFlag )
    {}
-   void restrict( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   void restrict( const P2VectorFunction< VT >& f, uint_t sourceLevel, DoFType flag ) const override
    {
-      P2toP2QuadraticVectorRestriction::restrict( f, sourceLevel, flag );
+      P2toP2QuadraticVectorRestrictionT< VT >::restrict( f, sourceLevel, flag );
       projection_->project( f, sourceLevel - 1, projectionFlag_ );
    }
 
@@ -1821,28 +1835,155 @@ class P2toP2QuadraticVectorRestrictionWithProjection : public P2toP2QuadraticVec
    std::shared_ptr< P2ProjectNormalOperator > projection_;
    DoFType                                    projectionFlag_;
 };
-class P2toP2QuadraticVectorProlongationWithProjection : public P2toP2QuadraticVectorProlongation
+template < typename VT = double >
+class P2toP2QuadraticVectorProlongationWithProjectionT : public P2toP2QuadraticVectorProlongationT< VT >
 {
  public:
-   explicit P2toP2QuadraticVectorProlongationWithProjection( std::shared_ptr< P2ProjectNormalOperator > projection,
-                                                             DoFType projectionFlag = FreeslipBoundary )
+   explicit P2toP2QuadraticVectorProlongationWithProjectionT( std::shared_ptr< P2ProjectNormalOperator > projection,
+                                                              DoFType projectionFlag = FreeslipBoundary )
    : projection_( std::move( projection ) )
    , projectionFlag_( projectionFlag )
    {}
-   void prolongate( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   void prolongate( const P2VectorFunction< VT >& f, uint_t sourceLevel, DoFType flag ) const override
    {
-      P2toP2QuadraticVectorProlongation::prolongate( f, sourceLevel, flag );
+      P2toP2QuadraticVectorProlongationT< VT >::prolongate( f, sourceLevel, flag );
       projection_->project( f, sourceLevel + 1, projectionFlag_ );
    }
-   void prolongateAndAdd( const P2VectorFunction< double >& f, uint_t sourceLevel, DoFType flag ) const override
+   void prolongateAndAdd( const P2VectorFunction< VT >& f, uint_t sourceLevel, DoFType flag ) const override
    {
-      P2toP2QuadraticVectorProlongation::prolongateAndAdd( f, sourceLevel, flag );
+      P2toP2QuadraticVectorProlongationT< VT >::prolongateAndAdd( f, sourceLevel, flag );
       projection_->project( f, sourceLevel + 1, projectionFlag_ );
    }
 
  private:
    std::shared_ptr< P2ProjectNormalOperator > projection_;
    DoFType                                    projectionFlag_;
+};
+using P2toP2QuadraticVectorRestrictionWithProjection  = P2toP2QuadraticVectorRestrictionWithProjectionT< double >;
+using P2toP2QuadraticVectorProlongationWithProjection = P2toP2QuadraticVectorProlongationWithProjectionT< double >;
+
+// The velocity block of the Stokes operator on FP32 or FP16 functions (FP32
+// arithmetic; the reduced-precision half of a mixed-precision velocity
+// solver), sharing the Stokes operator's device data.  With a projection it is
+// the free-slip P A P of P2P1FullStokesFreeslipWrapper::getA().  The inverse
+// diagonal is the FP64 one, cast.
+template < typename VT >
+class P2ElementwiseBlendingFullViscousOperatorT : public Operator< P2VectorFunction< VT >, P2VectorFunction< VT > >,
+                                                  public OperatorWithInverseDiagonal< P2VectorFunction< VT > >
+{
+   using Scalar = typename ScalarOf< VT >::type;
+
+ public:
+   P2ElementwiseBlendingFullViscousOperatorT( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle stokesOperator,
+                                              uint_t minLevel, uint_t maxLevel,
+                                              std::shared_ptr< P2ProjectNormalOperator > projection = nullptr )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , h_( stokesOperator )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , projection_( std::move( projection ) )
+   {
+      if ( projection_ )
+      {
+         tmp_    = std::make_unique< P2VectorFunction< VT > >( "viscous_low_tmp", storage, minLevel, maxLevel );
+         tmpdst_ = std::make_unique< P2VectorFunction< VT > >( "viscous_low_tmpdst", storage, minLevel, maxLevel );
+      }
+   }
+   void apply( const P2VectorFunction< VT >& src, const P2VectorFunction< VT >& dst, uint_t level, DoFType flag,
+               UpdateType updateType = Replace ) const override
+   {
+      if ( !projection_ )
+      {
+         raw( src, dst, level, flag, updateType );
+         return;
+      }
+      tmp_->assign( { Scalar( 1 ) }, { src }, level, All );
+      projection_->project( *tmp_, level, FreeslipBoundary );
+      if ( updateType == Replace )
+      {
+         raw( *tmp_, dst, level, flag, Replace );
+         projection_->project( dst, level, DoFType( flag & FreeslipBoundary ) );
+         return;
+      }
+      raw( *tmp_, *tmpdst_, level, flag, Replace );
+      projection_->project( *tmpdst_, level, FreeslipBoundary );
+      dst.add( { Scalar( 1 ) }, { *tmpdst_ }, level, flag );
+   }
+   std::shared_ptr< P2VectorFunction< VT > > getInverseDiagonalValues() const override
+   {
+      if ( !invDiag_ )
+         const_cast< P2ElementwiseBlendingFullViscousOperatorT* >( this )->computeInverseDiagonalOperatorValues();
+      return invDiag_;
+   }
+   void computeInverseDiagonalOperatorValues() override
+   {
+      invDiag_ = std::make_shared< P2VectorFunction< VT > >( "inverseDiagonal_low", storage_, minLevel_, maxLevel_ );
+      P2VectorFunction< double > d64( "inverseDiagonal64", storage_, minLevel_, maxLevel_ );
+      for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
+      {
+         const hg_handle d[3] = { d64[0].handle(), d64[1].handle(), d64[2].handle() };
+         check( ctx_, hg_stokes_velocity_inverse_diagonal( ctx_, h_, int( L ), d ), "inverse diagonal" );
+         invDiag_->copyFrom( d64, L );
+      }
+   }
+
+ private:
+   void raw( const P2VectorFunction< VT >& src, const P2VectorFunction< VT >& dst, uint_t level, DoFType flag,
+             UpdateType updateType ) const
+   {
+      const hg_handle s[4] = { src[0].handle(), src[1].handle(), src[2].handle(), 0 };
+      const hg_handle d[4] = { dst[0].handle(), dst[1].handle(), dst[2].handle(), 0 };
+      detail::stokesApply( ctx_, h_, s, d, level, flag, 1, updateType );
+   }
+   std::shared_ptr< PrimitiveStorage >        storage_;
+   hg_ctx*                                    ctx_;
+   hg_handle                                  h_;
+   uint_t                                     minLevel_, maxLevel_;
+   std::shared_ptr< P2ProjectNormalOperator > projection_;
+   std::unique_ptr< P2VectorFunction< VT > >  tmp_, tmpdst_;
+   std::shared_ptr< P2VectorFunction< VT > >  invDiag_;
+};
+
+// A solver of a reduced-precision operator used on FP64 functions: b and the
+// start vector are cast to the solver's type, the result is cast back (the
+// precision boundary of a mixed-precision preconditioner inside an FP64 outer
+// iteration, as the iterative-refinement driver of
+// mixedPrecisionIterativeRefinement.cpp:163-172 casts around its inner GMG).
+// With a projection the result is projected again in FP64: a projection applied
+// in FP16 storage leaves normal components of the rounding's size, which the
+// projected operator does not see and an outer iteration would accumulate.
+template < class OperatorType64, class OperatorTypeLow >
+class MixedPrecisionSolver : public Solver< OperatorType64 >
+{
+ public:
+   using FunctionType64  = typename OperatorType64::srcType;
+   using FunctionTypeLow = typename OperatorTypeLow::srcType;
+   MixedPrecisionSolver( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel,
+                         const OperatorTypeLow& Alow, std::shared_ptr< Solver< OperatorTypeLow > > inner,
+                         std::shared_ptr< P2ProjectNormalOperator > projection = nullptr )
+   : Alow_( Alow )
+   , inner_( std::move( inner ) )
+   , projection_( std::move( projection ) )
+   , x_( "mixed_x", storage, minLevel, maxLevel )
+   , b_( "mixed_b", storage, minLevel, maxLevel )
+   {}
+   void solve( const OperatorType64&, const FunctionType64& x, const FunctionType64& b, uint_t level ) override
+   {
+      b_.copyFrom( b, level );
+      x_.copyFrom( x, level );
+      inner_->solve( Alow_, x_, b_, level );
+      x.copyFrom( x_, level );
+      if constexpr ( std::is_same< FunctionType64, P2VectorFunction< double > >::value )
+         if ( projection_ )
+            projection_->project( x, level, FreeslipBoundary );
+   }
+
+ private:
+   const OperatorTypeLow&                       Alow_;
+   std::shared_ptr< Solver< OperatorTypeLow > > inner_;
+   std::shared_ptr< P2ProjectNormalOperator >   projection_;
+   FunctionTypeLow                              x_, b_;
 };
 
 // IdentityPreconditioner (solvers/preconditioners/IdentityPreconditioner.hpp): x = b

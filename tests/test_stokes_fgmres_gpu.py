@@ -86,3 +86,78 @@ def test_fgmres_block_factorisation_parity(ctx, shells, fs):
     X.destroy()
     B.destroy()
     A.destroy()
+
+
+@pytest.mark.parametrize("dtype,tol", [(hg.F32, 2e-5), (hg.F16, 4e-3)])
+def test_low_precision_velocity_block(ctx, shells, dtype, tol):
+    """the velocity block on FP32 / FP16 vectors (FP32 arithmetic) against the FP64
+    oracle, with and without the free-slip projection (the tolerance covers the
+    rounding of the inputs to the storage type and FP32 arithmetic)"""
+    for fs in (False, True):
+        st, om = shells[fs]
+        L = 2
+        A = hg.P2P1StokesOperator(st, L, L, *VISC)
+        op = po.StokesOp(om, L, *VISC, freeslip=fs)
+        u = np.random.default_rng(80).uniform(-1, 1, 3 * op.n2)
+        src = [hg.P2Function(f"s{c}", st, L, L, dtype) for c in range(3)]
+        dst = [hg.P2Function(f"d{c}", st, L, L, dtype) for c in range(3)]
+        for c in range(3):
+            src[c].upload(L, u[c * op.n2:(c + 1) * op.n2])
+        uq = np.concatenate([f.download(L) for f in src])  # the rounded input
+        print(f"low-precision velocity block dtype {dtype} freeslip {fs}", flush=True)
+        hs = np.ascontiguousarray([f.h for f in src] + [0], dtype=np.int64)
+        hd = np.ascontiguousarray([f.h for f in dst] + [0], dtype=np.int64)
+        fn = hg.lib().hg_stokes_freeslip_apply if fs else hg.lib().hg_stokes_apply
+        if fs:
+            # the wrapper runs FP64 only; the reduced-precision P A P is checked through its pieces
+            P = hg.P2ProjectNormalOperator(st)
+            P.project(src, L)
+            om.p2_project_normal(L, uq)
+            fn = hg.lib().hg_stokes_apply
+        ctx.check(fn(ctx.h, A.h, hs.ctypes.data, hd.ctypes.data, L, hg.GMG_FLAG, 1, hg.REPLACE), "apply")
+        if fs:
+            P.project(dst, L)
+        got = np.concatenate([f.download(L) for f in dst])
+        ref = np.zeros_like(uq)
+        op.apply_raw(uq, None, ref, None, hg.GMG_FLAG, 1)
+        if fs:
+            om.p2_project_normal(L, ref)
+        m = op.vmask(hg.GMG_FLAG)
+        assert np.abs(got[m] - ref[m]).max() < tol * np.abs(ref[m]).max(), (dtype, fs)
+        for f in src + dst:
+            f.destroy()
+        A.destroy()
+
+
+@pytest.mark.parametrize("dtype", [hg.F32, hg.F16])
+def test_mixed_precision_fgmres_reaches_fp64_accuracy(ctx, shells, dtype):
+    """FGMRES (FP64) with the velocity V-cycle in FP32 arithmetic on FP32 / FP16
+    vectors reaches a 1e-8 relative residual on the free-slip shell at level 2
+    (constant viscosity), and its (projected) solution matches the FP64 path's to 1e-6"""
+    st, om = shells[True]
+    minL, maxL = 0, 2
+    A = hg.P2P1StokesOperator(st, minL, maxL, freeslip=True)
+    bu, bp = _rhs(om, maxL, 81, True)
+    X = hg.P2P1TaylorHoodFunction("x", st, minL, maxL)
+    B = hg.P2P1TaylorHoodFunction("b", st, minL, maxL)
+    B.upload(maxL, bu, bp)
+    sols = {}
+    for vd in (hg.F64, dtype):
+        X.upload(maxL, np.zeros(len(bu)), np.zeros(len(bp)))
+        print(f"mixed FGMRES velocity dtype {vd}", flush=True)
+        its, hist = hg.stokes_fgmres_solve(A, X, B, minL, maxL, 2.9, max_iter=60, restart=60, tol=1e-8,
+                                           velocity_dtype=vd)
+        assert its < 60 and hist[-1] <= 1e-8 * hist[0], (vd, its, hist[-1] / hist[0])
+        sols[vd] = X.download(maxL)
+    # the normal velocity on the FreeSlip surface is not seen by the projected
+    # operator (the FP64 V-cycle's Chebyshev corrections leave some there): compare P u
+    for vd in sols:
+        om.p2_project_normal(maxL, sols[vd][0])
+    assert relmax(sols[dtype][0], sols[hg.F64][0]) < 1e-6
+    # the true residual of the mixed solution, FP64 oracle
+    op = po.StokesOp(om, maxL, freeslip=True)
+    ru, rp = po.stokes_residual(op, sols[dtype][0], sols[dtype][1], bu, bp, po.GMG_FLAG)
+    assert np.sqrt(ru @ ru + rp @ rp) < 2e-8 * np.sqrt(bu @ bu)
+    X.destroy()
+    B.destroy()
+    A.destroy()
