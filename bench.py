@@ -287,55 +287,83 @@ def serving_cfg5(n=400, rate=200.0):
                         "P2 cube L3-5 20%, P2 shell L2-3 15%, each solved to 1e-8", "rate_per_s": rate, **res}
 
 
-def stokes_cfg3(hg, ctx, L=5, cycles=8):
-    """P2-P1 Stokes on the shell, eta = exp(ln(1e4)(1-r)/0.45), nodal buoyancy of 12
-    Gaussian blobs on inner velocity DoFs, no-slip on both surfaces; the Uzawa GMG of
-    the free-slip solver template: V(3,3) + 2 per coarser level, omega 0.3, Chebyshev(3)
-    velocity smoothing with the radius from 25 power iterations, 10 MINRES steps on
-    level 0, pressure mean projected; `cycles` V-cycles or 1e-7 relative residual"""
+def stokes_cfg3(hg, ctx, L=5, uzawa_cycles=4, max_iter=150, restart=40, tol=1e-7):
+    """P2-P1 Stokes on the shell at level L, eta = exp(ln(1e4)(1-r)/0.45), nodal
+    buoyancy of 12 Gaussian blobs, free-slip on the inner and no-slip on the outer
+    surface (free-slip operator wrapper, normal projections).  Three solves:
+    (1) the Uzawa GMG of the free-slip solver template, V(3,3) + 2 per coarser level,
+    omega 0.3, Chebyshev(3) velocity smoothing, MINRES(10) on level 0, `uzawa_cycles`
+    cycles (its factors on this contrast); (2) FGMRES with the block-factorisation
+    preconditioner (velocity: one V(3,3) Chebyshev(3) cycle, Schur: 5 CG steps on the
+    1/eta pressure mass) to `tol`, FP64; (3) the same with the velocity V-cycle in FP32
+    arithmetic on FP32 vectors (mixed), whose projected solution is compared with (2)."""
     from hyteg_amd import problems
-    st = hg.PrimitiveStorage.shell(ctx, **problems.SHELL)
-    A = hg.P2P1StokesOperator(st, 0, L, *problems.STOKES_VISCOSITY)
+    st = hg.PrimitiveStorage.shell(ctx, **problems.SHELL, inner_flag=hg.FREESLIP, outer_flag=hg.DIRICHLET)
+    A = hg.P2P1StokesOperator(st, 0, L, *problems.STOKES_VISCOSITY, freeslip=True)
+    P = hg.P2ProjectNormalOperator(st)
     X = hg.P2P1TaylorHoodFunction("x", st, 0, L)
     B = hg.P2P1TaylorHoodFunction("b", st, 0, L)
     n2, n1 = st.p2_ndofs(L), st.ndofs(L)
     t0 = time.perf_counter()
     xyz = st.coords(L + 1)
     f = problems.buoyancy(xyz)
-    inner = st.flags(L + 1) == hg.INNER
+    fl = st.flags(L + 1)
+    live = (fl == hg.INNER) | (fl == hg.FREESLIP)
     del xyz
-    B.upload(L, np.concatenate([np.where(inner, f[:, c], 0.0) for c in range(3)]), np.zeros(n1))
-    del f
+    B.upload(L, np.concatenate([np.where(live, f[:, c], 0.0) for c in range(3)]), np.zeros(n1))
+    del f, fl, live
+    P.project(B, L)
     for c in range(3):
         X.uvw[c].interpolate_random(7 + c, -1.0, 1.0, L, hg.INNER)
     T = hg.P2P1TaylorHoodFunction("t", st, L, L)
     rho = hg.stokes_estimate_radius(A, X, T, L, 25)
     T.destroy()
-    hg.stokes_gmg_solve(A, X, B, 0, L, rho, cycles=1)  # builds the per-level data once
-    X.interpolate(0.0, L)
     ctx.synchronize()
     setup = time.perf_counter() - t0
     log(f"Stokes level {L}: {3 * n2 + n1} DoFs, radius {rho:.4f}, set-up {setup:.2f} s")
     Y = hg.P2P1TaylorHoodFunction("y", st, L, L)
     kms = kernel_ms(ctx, lambda: A.apply(B, Y, L, hg.GMG_FLAG), 3)
     Y.destroy()
+    X.interpolate(0.0, L)
     ctx.synchronize()
     t0 = time.perf_counter()
-    k, hist = hg.stokes_gmg_solve(A, X, B, 0, L, rho, cycles=cycles, tol=1e-7)
+    k, hist = hg.stokes_gmg_solve(A, X, B, 0, L, rho, cycles=uzawa_cycles)
     ctx.synchronize()
-    dt = time.perf_counter() - t0
-    log(f"Stokes GMG level {L}: {k} cycles, {dt:.2f} s, relative residual {hist[-1] / hist[0]:.2e}")
-    for fn in (X, B):
+    t_uz = time.perf_counter() - t0
+    log(f"Stokes Uzawa GMG: {k} cycles {t_uz:.2f} s, factors {np.round(hist[1:] / hist[:-1], 3)}")
+    runs = {}
+    X64 = hg.P2P1TaylorHoodFunction("x64", st, L, L)
+    for name, vd in (("f64", hg.F64), ("mixed_f32", hg.F32)):
+        X.interpolate(0.0, L)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        its, h = hg.stokes_fgmres_solve(A, X, B, 0, L, rho, a_coarse_iter=5, schur_iter=5, restart=restart,
+                                        max_iter=max_iter, tol=tol, velocity_dtype=vd)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        P.project(X, L)  # the normal velocity on the free-slip surface is not part of the solution
+        runs[name] = {"iterations": its, "time_s": dt, "s_per_iteration": dt / max(its, 1),
+                      "relative_residual": float(h[-1] / h[0]), "reached_tol": bool(h[-1] <= tol * h[0])}
+        log(f"Stokes FGMRES ({name}): {its} iterations {dt:.2f} s, relative residual {h[-1] / h[0]:.2e}")
+        if vd == hg.F64:
+            X64.assign([1.0], [X], L)
+        else:
+            X64.assign([1.0, -1.0], [X, X64], L)
+            ref = X.dot(X, L)
+            runs[name]["rel_l2_error_vs_f64"] = float(np.sqrt(X64.dot(X64, L) / ref))
+    for fn in (X, B, X64):
         fn.destroy()
     A.destroy()
     return {"workload": f"P2-P1 Stokes, shell level {L} ({3 * n2} velocity + {n1} pressure DoFs), "
-                        "eta = exp(ln(1e4)(1-r)/0.45), 12-blob buoyancy, no-slip, FP64 Uzawa GMG V(3,3)+2, "
-                        "Chebyshev(3) velocity smoothing, omega 0.3, MINRES(10) on level 0",
-            "dofs": 3 * n2 + n1, "cycles": k, "time_s": dt, "s_per_cycle": dt / max(k, 1), "setup_s": setup,
-            "spectral_radius": rho, "relative_residual": float(hist[-1] / hist[0]),
-            "reached_1e-7": bool(hist[-1] <= 1e-7 * hist[0]),
-            "factors": [float(v) for v in hist[1:] / hist[:-1]],
-            "stokes_apply_kernel_ms": kms, "stokes_apply_dof_per_s": (3 * n2 + n1) / (kms * 1e-3)}
+                        "eta = exp(ln(1e4)(1-r)/0.45), 12-blob buoyancy, free-slip inner / no-slip outer",
+            "dofs": 3 * n2 + n1, "setup_s": setup, "spectral_radius": rho,
+            "stokes_apply_kernel_ms": kms, "stokes_apply_dof_per_s": (3 * n2 + n1) / (kms * 1e-3),
+            "uzawa_gmg_f64": {"solver": "Uzawa GMG V(3,3)+2, omega 0.3, Chebyshev(3), MINRES(10) on level 0",
+                              "cycles": k, "time_s": t_uz, "s_per_cycle": t_uz / max(k, 1),
+                              "factors": [float(v) for v in hist[1:] / hist[:-1]]},
+            "fgmres_block_factorisation": {"solver": f"FGMRES(restart {restart}), block-factorisation: velocity "
+                                                     "V(3,3) Chebyshev(3), Schur CG(5) on the 1/eta mass", "tol": tol,
+                                           **runs}}
 
 
 def main():
@@ -379,7 +407,7 @@ def main():
         if args.workload == "p2_shell" and not args.no_batched:
             extra["batched_p1_cube_L6_x64"] = batched_cfg4(hg, ctx)
         if args.workload == "p2_shell" and not args.no_stokes:
-            extra["stokes_shell_L5_f64"] = stokes_cfg3(hg, ctx)
+            extra["stokes_shell_L5"] = stokes_cfg3(hg, ctx)
         if args.workload == "p2_shell" and not args.no_serving:
             extra["serving_mixed_stream"] = serving_cfg5()
     cpu = None if (args.no_cpu or args.only_apply) else cpu_baseline(args.workload, args.cpu_seconds)

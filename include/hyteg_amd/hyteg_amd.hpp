@@ -2145,6 +2145,11 @@ class FGMRESSolver : public Solver< OperatorType >
 
    void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
    {
+      if ( level != vLevel_ )
+      {
+         V_.clear();
+         vLevel_ = level;
+      }
       r0_.setToZero( level );
       init( A, x, b, level, false );
       iterations_   = maxKrylowDim_ > 0 ? maxKrylowDim_ - 1 : 0; // iterations performed without an early stop
@@ -2158,10 +2163,11 @@ class FGMRESSolver : public Solver< OperatorType >
             continue;
          }
          const size_t k = j % restartLength_; // the Krylov index of this iteration
+         // the preconditioner may work on coarser levels of z; the basis v lives on `level` alone
          while ( Z_.size() < k )
             Z_.push_back( std::make_unique< FunctionType >( "z", storage_, minLevel_, maxLevel_ ) );
          while ( V_.size() <= k )
-            V_.push_back( std::make_unique< FunctionType >( "w", storage_, minLevel_, maxLevel_ ) );
+            V_.push_back( std::make_unique< FunctionType >( "w", storage_, level, level ) );
          Z_[k - 1]->setToZero( level );
          prec_->solve( A, *Z_[k - 1], *V_[k - 1], level );
          A.apply( *Z_[k - 1], *V_[k], level, flag_ );
@@ -2205,7 +2211,7 @@ class FGMRESSolver : public Solver< OperatorType >
       g_.assign( 1, beta_ );
       y_.clear();
       if ( V_.empty() )
-         V_.push_back( std::make_unique< FunctionType >( "v0", storage_, minLevel_, maxLevel_ ) );
+         V_.push_back( std::make_unique< FunctionType >( "v0", storage_, level, level ) );
       V_[0]->assign( { 1.0 / beta_ }, { r0_ }, level, flag_ );
       if ( Z_.empty() )
          Z_.push_back( std::make_unique< FunctionType >( "z0", storage_, minLevel_, maxLevel_ ) );
@@ -2263,7 +2269,7 @@ class FGMRESSolver : public Solver< OperatorType >
    std::vector< std::pair< double, double > >          rot_;
    std::vector< double >                               g_, y_;
    double                                              beta_ = 0.0, initRes_ = 0.0;
-   uint_t                                              iterations_ = 0;
+   uint_t                                              iterations_ = 0, vLevel_ = ~uint_t( 0 );
    Hook                                                hook_;
    DoFType                                             flag_ = Inner | NeumannBoundary | FreeslipBoundary;
 };
