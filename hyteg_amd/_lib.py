@@ -79,6 +79,7 @@ SIGNATURES = {
     "hg_stokes_pressure_diagonal": [_p, _i64, _i, _i, _i64],
     "hg_p2_project_normal": [_p, _p, _i, _i],
     "hg_function_invert": [_p, _i64, _i, _i],
+    "hg_vtk_write": [_p, C.c_char_p, _i64, _i, _i, _p, _p, _i],
     "hg_stokes_fgmres_solve": [_p, _i64, _p, _p, _i, _i, _i, _i, _i, _d, _i, _d, _i, _d, _i, _i, _i, _i, _d, _p, _p],
     "hg_stokes_freeslip_apply": [_p, _i64, _p, _p, _i, _i, _i, _i],
     "hg_stokes_estimate_radius": [_p, _i64, _p, _p, _i, _i, _i, _p],

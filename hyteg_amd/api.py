@@ -575,3 +575,14 @@ def stokes_gmg_solve(A, x, b, min_level, max_level, rho, pre=3, post=3, smooth_i
                                           coarse_max_iter, coarse_tol, int(project), int(A.freeslip), cycles, tol,
                                           _ptr(hist), C.byref(done)), "Stokes GMG")
     return done.value, hist[:done.value + 1]
+
+
+def vtk_write(path, storage, level, functions, names=None, binary=False):
+    """VTKOutput::write: one .vtu file with the P1 or P2 functions on the level's
+    micro-tetrahedra (per-macro-cell points, blended); names default to f0, f1, ..."""
+    names = names or [f"f{i}" for i in range(len(functions))]
+    h = np.ascontiguousarray([f.h for f in functions], dtype=np.int64)
+    enc = [n.encode() for n in names]
+    arr = (C.c_char_p * len(enc))(*enc)
+    storage.ctx.check(lib().hg_vtk_write(storage.ctx.h, str(path).encode(), storage.h, level, len(functions),
+                                         _ptr(h), C.cast(arr, C.c_void_p), int(binary)), "vtk_write")

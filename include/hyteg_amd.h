@@ -409,6 +409,16 @@ int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const
                             int velocityDtype, int maxIter, int restart, double relTol, double* res_hist,
                             int* iterations );
 
+/* ---- VTK output ----------------------------------------------------------------------
+ * VTKOutput::write for P1 or P2 functions (dataexport/VTKOutput/VTKOutput.cpp,
+ * VTKP1Writer.cpp:34-120, VTKP2Writer.cpp:34-160): one XML unstructured-grid file
+ * with every macro-cell's lattice points (mapped by the blending) and the level's
+ * micro-tetrahedra (VTK_TETRA for P1, VTK_QUADRATIC_TETRA for P2), one point-data
+ * array per function named names[i].  The functions are single-vector functions
+ * of one space on the storage; binary != 0 writes VTK's inline base64 format. */
+int hg_vtk_write( hg_ctx* ctx, const char* path, hg_handle storage, int level, int nfun, const hg_handle* functions,
+                  const char* const* names, int binary );
+
 #ifdef __cplusplus
 }
 #endif
