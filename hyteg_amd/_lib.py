@@ -80,6 +80,11 @@ SIGNATURES = {
     "hg_p2_project_normal": [_p, _p, _i, _i],
     "hg_function_invert": [_p, _i64, _i, _i],
     "hg_vtk_write": [_p, C.c_char_p, _i64, _i, _i, _p, _p, _i],
+    "hg_p2_const_create": [_p, _i64, _i, _i, _p],
+    "hg_p2_const_apply": [_p, _i64, _i64, _i64, _i, _i, _i],
+    "hg_p2_const_smooth_sor": [_p, _i64, _i64, _i64, _i, _i, _d],
+    "hg_p2_const_inverse_diagonal": [_p, _i64, _i, _i64],
+    "hg_p2_const_gmg_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _d, _i, _d, _p, _p],
     "hg_stokes_fgmres_solve": [_p, _i64, _p, _p, _i, _i, _i, _i, _i, _d, _i, _d, _i, _d, _i, _i, _i, _i, _d, _p, _p],
     "hg_stokes_freeslip_apply": [_p, _i64, _p, _p, _i, _i, _i, _i],
     "hg_stokes_estimate_radius": [_p, _i64, _p, _p, _i, _i, _i, _p],

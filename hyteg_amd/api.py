@@ -586,3 +586,50 @@ def vtk_write(path, storage, level, functions, names=None, binary=False):
     arr = (C.c_char_p * len(enc))(*enc)
     storage.ctx.check(lib().hg_vtk_write(storage.ctx.h, str(path).encode(), storage.h, level, len(functions),
                                          _ptr(h), C.cast(arr, C.c_void_p), int(binary)), "vtk_write")
+
+
+class P2ConstantLaplaceOperator:
+    """P2ConstantLaplaceOperator: the P2 Laplacian of the affine macro-cells with constant
+    stencils, Gauss-Seidel / SOR smoothable"""
+
+    def __init__(self, storage, min_level, max_level):
+        self.storage = storage
+        self.ctx = storage.ctx
+        h = C.c_int64()
+        self.ctx.check(lib().hg_p2_const_create(self.ctx.h, storage.h, min_level, max_level, C.byref(h)),
+                       "P2ConstantLaplaceOperator")
+        self.h = h.value
+
+    def destroy(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().hg_operator_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def apply(self, src, dst, level, flag=ALL, update=REPLACE):
+        self.ctx.check(lib().hg_p2_const_apply(self.ctx.h, self.h, src.h, dst.h, level, flag, update), "apply")
+
+    def smooth_sor(self, x, b, level, flag=GMG_FLAG, relax=1.0):
+        self.ctx.check(lib().hg_p2_const_smooth_sor(self.ctx.h, self.h, x.h, b.h, level, flag, relax), "smooth_sor")
+
+    def smooth_gs(self, x, b, level, flag=GMG_FLAG):
+        self.smooth_sor(x, b, level, flag, 1.0)
+
+    def inverse_diagonal(self, level, dst):
+        self.ctx.check(lib().hg_p2_const_inverse_diagonal(self.ctx.h, self.h, level, dst.h), "inverse diagonal")
+
+
+def p2_const_gmg_solve(A, x, b, min_level, max_level, pre=2, post=2, coarse_iter=1000, coarse_tol=1e-14,
+                       cycles=10, tol=0.0):
+    """GeometricMultigridSolver<P2ConstantLaplaceOperator> with Gauss-Seidel smoothing and CG on
+    min_level; returns (cycles done, Inner residual history)"""
+    hist = np.zeros(cycles + 1)
+    done = C.c_int()
+    A.ctx.check(lib().hg_p2_const_gmg_solve(A.ctx.h, A.h, x.h, b.h, min_level, max_level, pre, post, coarse_iter,
+                                            coarse_tol, cycles, tol, _ptr(hist), C.byref(done)), "P2 constant GMG")
+    return done.value, hist[:done.value + 1]

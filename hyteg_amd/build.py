@@ -26,7 +26,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 SOURCES = ["topology.cpp", "runtime.hip", "p1.hip", "p2.hip", "p2transfer.hip", "stokes.hip", "solvers_abi.cpp",
-           "serving.cpp", "vtk.hip"]
+           "serving.cpp", "vtk.hip", "p2const.hip"]
 
 
 def _compile(src):

@@ -719,6 +719,31 @@ std::shared_ptr< Solver< VOpT > > velocityVCycle( const std::shared_ptr< Primiti
 
 extern "C" {
 
+int hg_p2_const_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre,
+                           int post, int coarseMaxIter, double coarseTol, int cycles, double tol, double* res_hist,
+                           int* cycles_done )
+{
+   if ( !ctx || minLevel < 0 || maxLevel < minLevel || pre < 0 || post < 0 || coarseMaxIter < 0 || cycles < 0 )
+      return HG_ERR_ARG;
+   return guarded( ctx, [&] {
+      using Op     = P2ConstantLaplaceOperator;
+      auto storage = storageOf( ctx, x );
+      auto A       = Op::borrow( storage, op, uint_t( minLevel ), uint_t( maxLevel ) );
+      auto X       = P2Function< double >::borrow( ctx, x );
+      auto B       = P2Function< double >::borrow( ctx, b );
+      auto coarse  = std::make_shared< CGSolver< Op > >( storage, uint_t( minLevel ), uint_t( minLevel ),
+                                                        uint_t( coarseMaxIter ), coarseTol, 1e-16 );
+      GeometricMultigridSolver< Op > gmg( storage, std::make_shared< GaussSeidelSmoother< Op > >(), coarse,
+                                          std::make_shared< P2toP2QuadraticRestriction<> >(),
+                                          std::make_shared< P2toP2QuadraticProlongation<> >(), uint_t( minLevel ),
+                                          uint_t( maxLevel ), uint_t( pre ), uint_t( post ) );
+      P2Function< double > r( "residual", storage, uint_t( maxLevel ), uint_t( maxLevel ) );
+      const int done = cycle( *A, gmg, *X, *B, r, uint_t( maxLevel ), cycles, tol, res_hist );
+      if ( cycles_done )
+         *cycles_done = done;
+   } );
+}
+
 int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const hg_handle* b, int minLevel,
                             int maxLevel, int aPre, int aPost, int chebyshevOrder, double spectralRadius,
                             int aCoarseMaxIter, double aCoarseTol, int schurMaxIter, double schurTol, int freeslip,

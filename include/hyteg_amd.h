@@ -409,6 +409,28 @@ int hg_stokes_fgmres_solve( hg_ctx* ctx, hg_handle op, const hg_handle* x, const
                             int velocityDtype, int maxIter, int restart, double relTol, double* res_hist,
                             int* iterations );
 
+/* ---- P2 constant-stencil Laplacian ------------------------------------------------------
+ * P2ConstantLaplaceOperator(storage, minLevel, maxLevel)
+ * (constant_stencil_operator/P2ConstantOperator.cpp:100-1350): the P2 Laplacian of the
+ * affine macro-cells (blending is not applied), one element matrix per micro-cell type
+ * and macro-cell. */
+int hg_p2_const_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, hg_handle* out );
+/* Operator::apply on FP64 P2 functions (P2ConstantOperator::apply) */
+int hg_p2_const_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update );
+/* SORSmoothable::smooth_sor (P2ConstantOperator.cpp:1267-1330; relax 1 is smooth_gs):
+ * the owned macro vertex/edge/face DoFs in a greedy multicolour order, then the cell
+ * interiors in 27 colours */
+int hg_p2_const_smooth_sor( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int flag, double relax );
+/* OperatorWithInverseDiagonal values into dst (every DoF) */
+int hg_p2_const_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, hg_handle dst );
+/* GeometricMultigridSolver<P2ConstantLaplaceOperator> with GaussSeidelSmoother, CGSolver
+ * (no preconditioner) on minLevel and quadratic transfers, the set-up of
+ * P2GMG3DConvergenceTest.cpp:60-100; V(pre,post) cycles until the Inner residual drops
+ * by tol or `cycles` ran; res_hist[0..done] */
+int hg_p2_const_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre,
+                           int post, int coarseMaxIter, double coarseTol, int cycles, double tol, double* res_hist,
+                           int* cycles_done );
+
 /* ---- VTK output ----------------------------------------------------------------------
  * VTKOutput::write for P1 or P2 functions (dataexport/VTKOutput/VTKOutput.cpp,
  * VTKP1Writer.cpp:34-120, VTKP2Writer.cpp:34-160): one XML unstructured-grid file

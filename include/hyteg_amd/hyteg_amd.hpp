@@ -591,6 +591,84 @@ class P2toP2QuadraticProlongation : public ProlongationOperator< P2Function< Val
 };
 
 // Solver<Op>
+// P2ConstantLaplaceOperator (constant_stencil_operator/P2ConstantOperator.hpp,
+// P2ConstantOperator.cpp:100-1350): the P2 Laplacian of affine macro-cells with
+// constant stencils, Gauss-Seidel / SOR smoothable, with inverse diagonal
+class P2ConstantLaplaceOperator : public Operator< P2Function< double >, P2Function< double > >,
+                                  public GSSmoothable< P2Function< double > >,
+                                  public SORSmoothable< P2Function< double > >,
+                                  public OperatorWithInverseDiagonal< P2Function< double > >
+{
+ public:
+   P2ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , owned_( true )
+   {
+      check( ctx_, hg_p2_const_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), &h_ ),
+             "P2ConstantLaplaceOperator" );
+   }
+   static std::unique_ptr< P2ConstantLaplaceOperator >
+       borrow( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel, uint_t maxLevel )
+   {
+      return std::unique_ptr< P2ConstantLaplaceOperator >( new P2ConstantLaplaceOperator( storage, h, minLevel, maxLevel ) );
+   }
+   ~P2ConstantLaplaceOperator() override
+   {
+      if ( owned_ && h_ )
+         hg_operator_destroy( ctx_, h_ );
+   }
+   void apply( const P2Function< double >& src, const P2Function< double >& dst, uint_t level, DoFType flag,
+               UpdateType updateType = Replace ) const override
+   {
+      check( ctx_, hg_p2_const_apply( ctx_, h_, src.handle(), dst.handle(), int( level ), flag, updateType ),
+             "P2ConstantLaplaceOperator::apply" );
+   }
+   void smooth_gs( const P2Function< double >& dst, const P2Function< double >& rhs, uint_t level,
+                   DoFType flag ) const override
+   {
+      smooth_sor( dst, rhs, 1.0, level, flag );
+   }
+   void smooth_sor( const P2Function< double >& dst, const P2Function< double >& rhs, double relax, uint_t level,
+                    DoFType flag ) const override
+   {
+      check( ctx_, hg_p2_const_smooth_sor( ctx_, h_, dst.handle(), rhs.handle(), int( level ), flag, relax ),
+             "P2ConstantLaplaceOperator::smooth_sor" );
+   }
+   std::shared_ptr< P2Function< double > > getInverseDiagonalValues() const override
+   {
+      if ( !invDiag_ )
+         const_cast< P2ConstantLaplaceOperator* >( this )->computeInverseDiagonalOperatorValues();
+      return invDiag_;
+   }
+   void computeInverseDiagonalOperatorValues() override
+   {
+      invDiag_ = std::make_shared< P2Function< double > >( "inverseDiagonal", storage_, minLevel_, maxLevel_ );
+      for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
+         check( ctx_, hg_p2_const_inverse_diagonal( ctx_, h_, int( L ), invDiag_->handle() ), "inverse diagonal" );
+   }
+   hg_handle handle() const { return h_; }
+
+ private:
+   P2ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel,
+                              uint_t maxLevel )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , h_( h )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , owned_( false )
+   {}
+   std::shared_ptr< PrimitiveStorage >       storage_;
+   hg_ctx*                                   ctx_;
+   hg_handle                                 h_ = 0;
+   uint_t                                    minLevel_, maxLevel_;
+   bool                                      owned_;
+   std::shared_ptr< P2Function< double > >   invDiag_;
+};
+
 template < class OperatorType >
 class Solver
 {

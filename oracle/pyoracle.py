@@ -916,3 +916,50 @@ class StokesFGMRES:
         u[:] = x[:n2]
         p[:] = x[n2:]
         return its, np.array(hist)
+
+
+# ---- P2 Gauss-Seidel and the P2 GMG of P2GMG3DConvergenceTest ---------------------
+def gs_sweep(A, x, b, m):
+    """one forward Gauss-Seidel sweep in canonical order over the rows of mask m
+    (SORSmoothable::smooth_gs restated on the assembled matrix A):
+    (D + L)_mm x_m = b_m - U_mm x_m - A_m,~m x_~m, the other entries unchanged"""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import spsolve_triangular
+    Am = A[m]
+    M = Am[:, m]
+    rhs = b[m] - sp.triu(M, 1) @ x[m] - Am[:, ~m] @ x[~m]
+    x[m] = spsolve_triangular(sp.tril(M).tocsr(), rhs, lower=True)
+
+
+class P2GSGMG:
+    """GeometricMultigridSolver<P2ConstantLaplaceOperator> with GaussSeidelSmoother, CG
+    (no preconditioner) on the coarsest level and quadratic transfers
+    (P2GMG3DConvergenceTest.cpp:60-100, GeometricMultigridSolver.hpp:207-300)"""
+
+    def __init__(self, mesh, min_level, max_level, pre=2, post=2, coarse_iter=1000, coarse_tol=1e-14):
+        self.m, self.minL, self.maxL, self.pre, self.post = mesh, min_level, max_level, pre, post
+        self.coarse_iter, self.coarse_tol = coarse_iter, coarse_tol
+        self.ops = {L: P2Op(mesh, L) for L in range(min_level, max_level + 1)}
+        self.A = {L: self.ops[L].matrix() for L in self.ops}
+        self.mask = {L: (mesh.flags(L + 1) & GMG_FLAG) != 0 for L in self.ops}
+
+    def residual(self, L, x, b):
+        r = np.zeros_like(x)
+        m = self.mask[L]
+        r[m] = b[m] - (self.A[L] @ x)[m]
+        return r
+
+    def cycle(self, L, x, b):
+        if L == self.minL:
+            self.ops[L].cg(x, b, GMG_FLAG, self.coarse_iter, self.coarse_tol, 1e-16, jacobi=False)
+            return
+        for _ in range(self.pre):
+            gs_sweep(self.A[L], x, b, self.mask[L])
+        r = self.residual(L, x, b)
+        cb = np.zeros(self.ops[L - 1].n)
+        self.m.p2_restrict(L, r, cb, GMG_FLAG)
+        cx = np.zeros_like(cb)
+        self.cycle(L - 1, cx, cb)
+        self.m.p2_prolongate(L - 1, cx, x, GMG_FLAG, True)
+        for _ in range(self.post):
+            gs_sweep(self.A[L], x, b, self.mask[L])
