@@ -95,7 +95,7 @@ def p2_shell(hg, ctx, L, steps, warmup):
                                            f"k = exp(16 sine modes), FP64), spherical shell r 0.55..1, ntan 5, "
                                            f"3 layers (1920 macro-cells), level {L} ({n} DoFs)",
                                "level": L, "dofs": n, "mesh": "meshSphericalShell(5, 3, 0.55, 1.0)"}, \
-        "kP2TileApply<true>", f"p2_apply_shell_L{L}_f64"
+        "kP2TileApplyStored<double>", f"p2_apply_shell_L{L}_f64"
 
 
 def p1_cube(hg, ctx, L, steps, warmup):
@@ -115,6 +115,26 @@ def p1_cube(hg, ctx, L, steps, warmup):
     return n, sec, kms, roof, {"workload": f"P1ConstantLaplaceOperator apply, unit cube 6 macro-tets, level {L} "
                                           f"({n} DoFs), FP64", "level": L, "dofs": n, "mesh": "meshCuboid 1x1x1"}, \
         "kCellApply<double>", f"p1_apply_cube_L{L}_f64"
+
+
+def p2_const_cube(hg, ctx, L=8, steps=10, warmup=2):
+    """P2ConstantLaplaceOperator apply on the unit cube (6 macro-tets), level L, FP64:
+    one 10x10 element product per micro-cell"""
+    st = hg.PrimitiveStorage.cuboid(ctx)
+    A = hg.P2ConstantLaplaceOperator(st, L, L)
+    n = st.p2_ndofs(L)
+    src = hg.P2Function("src", st, L, L)
+    dst = hg.P2Function("dst", st, L, L)
+    src.interpolate_random(1, -1.0, 1.0, L)
+    apply = lambda: A.apply(src, dst, L, hg.ALL)  # noqa: E731
+    sec = timed_applies(ctx, apply, steps, warmup)
+    kms = kernel_ms(ctx, apply, 5)
+    log(f"P2 constant apply cube level {L}: {n} DoFs, {1e3 * sec:.3f} ms/apply, kernel {kms:.3f} ms")
+    for f in (src, dst):
+        f.destroy()
+    A.destroy()
+    return {"dofs": n, "ms_per_apply": 1e3 * sec, "dof_per_s": n / sec, "kernel_ms": kms,
+            "kernel_gbs_algorithmic": 16.0 * n / (kms * 1e-3) / 1e9}
 
 
 def cpu_baseline(workload, seconds):
@@ -381,6 +401,10 @@ def main():
     ap.add_argument("--no-stokes", action="store_true", help="skip the Stokes solve")
     ap.add_argument("--only-apply", action="store_true",
                     help="run the apply measurement only (for counter collection)")
+    ap.add_argument("--only-stokes", action="store_true", help="the apply measurement and the Stokes leg only")
+    ap.add_argument("--only-p2const", action="store_true",
+                    help="the apply measurement and the P2 constant-stencil apply only")
+    ap.add_argument("--stokes-max-iter", type=int, default=150, help="FGMRES iteration cap of the Stokes leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard by "
@@ -396,21 +420,27 @@ def main():
     roofline["frac"] = roofline["achieved"] / roofline["peak"]
     roofline.update({"traffic": load_traffic(tkey), "kernel": kname, "kernel_avg_ms": kms})
     extra = {}
-    if not args.only_apply:
+    if args.only_stokes:
+        extra["stokes_shell_L5"] = stokes_cfg3(hg, ctx, max_iter=args.stokes_max_iter)
+    elif args.only_p2const:
+        extra["p2_const_apply_cube_L8_f64"] = p2_const_cube(hg, ctx)
+    elif not args.only_apply:
         if args.workload == "p2_shell":
             n1, sec1, kms1, roof1, cfg1, _, _ = p1_cube(hg, ctx, 9, 20, 3)
             extra["p1_apply_cube_L9_f64"] = {"dofs": n1, "ms_per_apply": 1e3 * sec1, "dof_per_s": n1 / sec1,
                                              "kernel_ms": kms1, "kernel_gbs": roof1["achieved"]}
+            extra["p2_const_apply_cube_L8_f64"] = p2_const_cube(hg, ctx)
             if not args.no_gmg:
                 extra["gmg_p2_shell_f64"] = gmg_p2_shell(hg, ctx, L)
         extra["gmg_p1_cube_L5_f64"] = gmg_cfg1(hg, ctx)
         if args.workload == "p2_shell" and not args.no_batched:
             extra["batched_p1_cube_L6_x64"] = batched_cfg4(hg, ctx)
         if args.workload == "p2_shell" and not args.no_stokes:
-            extra["stokes_shell_L5"] = stokes_cfg3(hg, ctx)
+            extra["stokes_shell_L5"] = stokes_cfg3(hg, ctx, max_iter=args.stokes_max_iter)
         if args.workload == "p2_shell" and not args.no_serving:
             extra["serving_mixed_stream"] = serving_cfg5()
-    cpu = None if (args.no_cpu or args.only_apply) else cpu_baseline(args.workload, args.cpu_seconds)
+    quick = args.only_apply or args.only_stokes or args.only_p2const
+    cpu = None if (args.no_cpu or quick) else cpu_baseline(args.workload, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": n / sec, "unit": "DoF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": True, "scaling": "weak",

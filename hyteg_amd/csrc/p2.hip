@@ -333,10 +333,10 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileCoefficients( const P2C
 // T: value type of src / dst; K: value type of the coefficient table; the
 // arithmetic runs in Acc<T> (FP64 for double, FP32 for float and half).
 template < class T, class K, bool STORED >
-__global__ __launch_bounds__( kTileThreads ) void kP2TileApply(
-    const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
-    const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
-    const T* __restrict__ src, T* __restrict__ dst )
+__device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom,
+                                                 const P2CellGeom* __restrict__ geom64, const KParams* __restrict__ kp,
+                                                 const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1,
+                                                 int64_t w1, const T* __restrict__ src, T* __restrict__ dst )
 {
    using R  = typename Acc< T >::type;
    using QC = QuadConst< R >;
@@ -476,6 +476,28 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApply(
    }
 }
 
+// stored coefficients: amdgpu_waves_per_eu(6) gives the register budget of 6
+// waves per SIMD (80 VGPRs, a few spilled), the most the 39.6 KB of LDS per
+// workgroup allows; 5 waves at 88 VGPRs measured 47.4 ms per level-6 shell
+// apply, 6 waves 41.7 ms
+template < class T, class K >
+__global__ __launch_bounds__( kTileThreads ) __attribute__( ( amdgpu_waves_per_eu( 6, 8 ) ) ) void kP2TileApplyStored(
+    const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
+    const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
+    const T* __restrict__ src, T* __restrict__ dst )
+{
+   p2TileApplyBody< T, K, true >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst );
+}
+// coefficients evaluated in place (the compiler's own register budget)
+template < class T, class K >
+__global__ __launch_bounds__( kTileThreads ) void kP2TileApplyEval(
+    const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
+    const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
+    const T* __restrict__ src, T* __restrict__ dst )
+{
+   p2TileApplyBody< T, K, false >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst );
+}
+
 __global__ void kP2ElementDiag( const CellGeom* __restrict__ geom, const KParams* __restrict__ kp, int64_t mm,
                                 int64_t nc, int64_t sc1, int64_t w1, double* __restrict__ dst )
 {
@@ -583,10 +605,10 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
       const auto*   geom   = p2Geometry< T >( op, L );
       ctx->beginTiming();
       if ( table )
-         kP2TileApply< T, K, true ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+         kP2TileApplyStored< T, K ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
              geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d );
       else
-         kP2TileApply< T, K, false ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+         kP2TileApplyEval< T, K ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
              geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d );
       HG_CHECK_HIP( hipGetLastError() );
       ctx->endTiming();
