@@ -94,6 +94,11 @@ SIGNATURES = {
     "hg_function_copy_vector": [_p, _i64, _i, _i64, _i, _i],
     "hg_function_dot_batched": [_p, _i64, _i64, _i, _i, _p],
     "hg_function_scale_batched": [_p, _i64, _p, _i, _i],
+    "hg_slots_alloc": [_p, _i, _p],
+    "hg_slots_free": [_p, _i, _i],
+    "hg_slots_read": [_p, _i, _i, _p],
+    "hg_function_dot_slot": [_p, _i64, _i64, _i, _i, _i],
+    "hg_function_lincomb_slots": [_p, _i64, _i, _p, _p, _p, _p, _i, _i, _i],
     "hg_p1_batched_gmg_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _i, _d, _p, _p],
     "hg_p1_batched_ir_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _i, _i, _i, _d, _p, _p],
     "hg_p2_restrict": [_p, _i64, _i, _i],

@@ -324,6 +324,72 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileCoefficients( const P2C
 }
 
 
+// y = A_e u for the micro-cell of th: the element action of
+// p2_div_k_grad_blending_q4 in barycentric form (11-point rule); kq(q) gives k at
+// quadrature point q of the micro-cell
+template < class R, class KQ >
+__device__ __forceinline__ void p2ElementAction( const P2CellGeomT< R >& g, const TileThread< R >& th, const R u[10],
+                                                 R y[10], KQ&& kq )
+{
+   using QC = QuadConst< R >;
+#pragma unroll
+   for ( int i = 0; i < 10; ++i )
+      y[i] = R( 0 );
+   const R* Dc   = g.Dc[th.t];
+   const R  adet = g.adet[th.t];
+#pragma unroll 1
+   for ( int q = 0; q < 11; ++q )
+   {
+      const R* a4 = QC::a4( q );
+      const R* b4 = QC::b4( q );
+      const R  g0 = b4[0] * u[0] + a4[1] * u[4] + a4[2] * u[5] + a4[3] * u[6];
+      const R  g1 = b4[1] * u[1] + a4[0] * u[4] + a4[2] * u[7] + a4[3] * u[8];
+      const R  g2 = b4[2] * u[2] + a4[0] * u[5] + a4[1] * u[7] + a4[3] * u[9];
+      const R  g3 = b4[3] * u[3] + a4[0] * u[6] + a4[1] * u[8] + a4[2] * u[9];
+      const R  d1 = g1 - g0, d2 = g2 - g0, d3 = g3 - g0;
+      R        v[3];
+#pragma unroll
+      for ( int d = 0; d < 3; ++d )
+         v[d] = d1 * Dc[d] + d2 * Dc[3 + d] + d3 * Dc[6 + d];
+      const R* xq   = g.XQ[th.t][q];
+      const R  x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
+      const R  kqv  = kq( q );
+      R        cc   = QC::w()[q] * adet * kqv;
+      if ( g.kind )
+      {
+         R qq, w[3], beta;
+         shellScalars( g, x, th.ls0 + xq[3], qq, w, beta );
+         const R ib = fastRcp( beta );
+         const R t1 = ( x[0] * v[0] + x[1] * v[1] + x[2] * v[2] ) * ib;
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            v[d] -= w[d] * t1;
+         const R t2 = ( w[0] * v[0] + w[1] * v[1] + w[2] * v[2] ) * ib;
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            v[d] -= x[d] * t2;
+         cc *= fabs( beta );
+      }
+#pragma unroll
+      for ( int d = 0; d < 3; ++d )
+         v[d] *= cc;
+      const R e1 = Dc[0] * v[0] + Dc[1] * v[1] + Dc[2] * v[2];
+      const R e2 = Dc[3] * v[0] + Dc[4] * v[1] + Dc[5] * v[2];
+      const R e3 = Dc[6] * v[0] + Dc[7] * v[1] + Dc[8] * v[2];
+      const R e0 = -( e1 + e2 + e3 );
+      y[0] += b4[0] * e0;
+      y[1] += b4[1] * e1;
+      y[2] += b4[2] * e2;
+      y[3] += b4[3] * e3;
+      y[4] += a4[1] * e0 + a4[0] * e1;
+      y[5] += a4[2] * e0 + a4[0] * e2;
+      y[6] += a4[3] * e0 + a4[0] * e3;
+      y[7] += a4[2] * e1 + a4[1] * e2;
+      y[8] += a4[3] * e1 + a4[1] * e3;
+      y[9] += a4[3] * e2 + a4[2] * e3;
+   }
+}
+
 // y = A x on the cell lattices of level L+1 (halo positions included): the
 // element action of p2_div_k_grad_blending_q4 evaluated in barycentric form,
 //   gamma_k = sum_j u_j dphi_j/dlam_k,  g = sum_k gamma_k grad_c lam_k,
@@ -388,30 +454,9 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
    if ( th.valid )
    {
       R y[10];
-#pragma unroll
-      for ( int i = 0; i < 10; ++i )
-         y[i] = R( 0 );
-      const R* Dc   = g.Dc[th.t];
-      const R  adet = g.adet[th.t];
-#pragma unroll 1
-      for ( int q = 0; q < 11; ++q )
-      {
-         const R* a4 = QC::a4( q );
-         const R* b4 = QC::b4( q );
-         const R  g0 = b4[0] * u[0] + a4[1] * u[4] + a4[2] * u[5] + a4[3] * u[6];
-         const R  g1 = b4[1] * u[1] + a4[0] * u[4] + a4[2] * u[7] + a4[3] * u[8];
-         const R  g2 = b4[2] * u[2] + a4[0] * u[5] + a4[1] * u[7] + a4[3] * u[9];
-         const R  g3 = b4[3] * u[3] + a4[0] * u[6] + a4[1] * u[8] + a4[2] * u[9];
-         const R  d1 = g1 - g0, d2 = g2 - g0, d3 = g3 - g0;
-         R        v[3];
-#pragma unroll
-         for ( int d = 0; d < 3; ++d )
-            v[d] = d1 * Dc[d] + d2 * Dc[3 + d] + d3 * Dc[6 + d];
-         const R* xq   = g.XQ[th.t][q];
-         const R  x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
-         R        kqv;
+      p2ElementAction( g, th, u, y, [&]( int q ) -> R {
          if constexpr ( STORED )
-            kqv = sk[q * kTileThreads + threadIdx.x];
+            return sk[q * kTileThreads + threadIdx.x];
          else
          {
             // k from the FP64 geometry of the same point
@@ -419,42 +464,9 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
             const TileThread< double > t64  = tileThread( g64, tc, mm );
             const double*              xq64 = g64.XQ[th.t][q];
             const double x64[3] = { t64.p0[0] + xq64[0], t64.p0[1] + xq64[1], t64.p0[2] + xq64[2] };
-            kqv                 = R( kAt( g64, *kp, x64, t64.ls0 + xq64[3] ) );
+            return R( kAt( g64, *kp, x64, t64.ls0 + xq64[3] ) );
          }
-         R cc = QC::w()[q] * adet * kqv;
-         if ( g.kind )
-         {
-            R qq, w[3], beta;
-            shellScalars( g, x, th.ls0 + xq[3], qq, w, beta );
-            const R ib = fastRcp( beta );
-            const R t1 = ( x[0] * v[0] + x[1] * v[1] + x[2] * v[2] ) * ib;
-#pragma unroll
-            for ( int d = 0; d < 3; ++d )
-               v[d] -= w[d] * t1;
-            const R t2 = ( w[0] * v[0] + w[1] * v[1] + w[2] * v[2] ) * ib;
-#pragma unroll
-            for ( int d = 0; d < 3; ++d )
-               v[d] -= x[d] * t2;
-            cc *= fabs( beta );
-         }
-#pragma unroll
-         for ( int d = 0; d < 3; ++d )
-            v[d] *= cc;
-         const R e1 = Dc[0] * v[0] + Dc[1] * v[1] + Dc[2] * v[2];
-         const R e2 = Dc[3] * v[0] + Dc[4] * v[1] + Dc[5] * v[2];
-         const R e3 = Dc[6] * v[0] + Dc[7] * v[1] + Dc[8] * v[2];
-         const R e0 = -( e1 + e2 + e3 );
-         y[0] += b4[0] * e0;
-         y[1] += b4[1] * e1;
-         y[2] += b4[2] * e2;
-         y[3] += b4[3] * e3;
-         y[4] += a4[1] * e0 + a4[0] * e1;
-         y[5] += a4[2] * e0 + a4[0] * e2;
-         y[6] += a4[3] * e0 + a4[0] * e3;
-         y[7] += a4[2] * e1 + a4[1] * e2;
-         y[8] += a4[3] * e1 + a4[1] * e3;
-         y[9] += a4[3] * e2 + a4[2] * e3;
-      }
+      } );
       int pos[10];
       elementBoxPositions( th, pos );
 #pragma unroll
@@ -541,6 +553,90 @@ static void p2CheckFunction( const P2Operator& op, const Function& f, int L )
       throw Error( -1, "level outside the operator's or function's level range" );
 }
 
+// Levels 0 and 1 (mm <= 2: one tile per macro-cell and at most 8 micro-cells in
+// it, so the tile kernel would keep at most 8 of its 384 threads busy): one
+// wavefront per macro-cell.  Lane l < 24 takes micro-cell type l / 4 at the
+// anchor (0,0,0), (1,0,0), (0,1,0) or (0,0,1) (the only anchors holding
+// micro-cells there); the cell's lattice (at most 35 points) is staged in the
+// tile kernel's LDS box layout, and the element action and the coefficient
+// table are the tile kernel's.
+constexpr int kSmallMaxMM = 2;
+template < class T, class K, bool STORED >
+__global__ __launch_bounds__( 64 ) void kP2SmallApply( const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom,
+                                                       const P2CellGeom* __restrict__ geom64,
+                                                       const KParams* __restrict__ kp, const K* __restrict__ table,
+                                                       int64_t mm, int64_t nc, int64_t sc1, int64_t w1,
+                                                       const T* __restrict__ src, T* __restrict__ dst )
+{
+   using R = typename Acc< T >::type;
+   __shared__ R  sb[kBox3];
+   const int64_t c = blockIdx.x;
+   if ( c >= nc )
+      return;
+   const TileCoord tc{ c, 0, 0, 0 };
+   const int32_t   w    = int32_t( w1 );
+   const int32_t   top  = w - 1;
+   const T*        srcC = src + c * sc1;
+   for ( int p = threadIdx.x; p < w * w * w; p += 64 )
+   {
+      const int32_t X = p % w, Y = ( p / w ) % w, Z = p / ( w * w );
+      if ( X + Y + Z <= top )
+         sb[X + kBox * ( Y + kBox * Z )] = R( toAcc( srcC[tetRowStart32( w, Y, Z ) + X] ) );
+   }
+   __syncthreads();
+   const int               l = int( threadIdx.x );
+   const int               t = l < 24 ? l >> 2 : 0;
+   const int               a = ( l & 3 ) == 0 ? 0 : 1 << ( 2 * ( ( l & 3 ) - 1 ) ); // 0, 1, 4, 16
+   const P2CellGeomT< R >& g  = geom[c];
+   TileThread< R >         th = tileThreadAt( g, tc, mm, t, a );
+   th.valid                   = th.valid && l < 24;
+   R u[10];
+   int pos[10];
+   elementBoxPositions( th, pos );
+#pragma unroll
+   for ( int i = 0; i < 10; ++i )
+      u[i] = th.valid ? sb[pos[i]] : R( 0 );
+   __syncthreads();
+   for ( int p = threadIdx.x; p < w * w * w; p += 64 )
+   {
+      const int32_t X = p % w, Y = ( p / w ) % w, Z = p / ( w * w );
+      if ( X + Y + Z <= top )
+         sb[X + kBox * ( Y + kBox * Z )] = R( 0 );
+   }
+   __syncthreads();
+   if ( th.valid )
+   {
+      R y[10];
+      p2ElementAction( g, th, u, y, [&]( int q ) -> R {
+         if constexpr ( STORED )
+            return R( toAcc( table[( c * 11 + q ) * kTileThreads + 64 * t + a] ) );
+         else
+         {
+            const P2CellGeom&          g64  = geom64[c];
+            const TileThread< double > t64  = tileThreadAt( g64, tc, mm, t, a );
+            const double*              xq64 = g64.XQ[t][q];
+            const double x64[3] = { t64.p0[0] + xq64[0], t64.p0[1] + xq64[1], t64.p0[2] + xq64[2] };
+            return R( kAt( g64, *kp, x64, t64.ls0 + xq64[3] ) );
+         }
+      } );
+#pragma unroll
+      for ( int i = 0; i < 10; ++i )
+         atomicAdd( &sb[pos[i]], y[i] );
+   }
+   __syncthreads();
+   // the wave owns the cell's storage (halo copies included): no other wave adds here
+   T* dstC = dst + c * sc1;
+   for ( int p = threadIdx.x; p < w * w * w; p += 64 )
+   {
+      const int32_t X = p % w, Y = ( p / w ) % w, Z = p / ( w * w );
+      if ( X + Y + Z <= top )
+      {
+         T& o = dstC[tetRowStart32( w, Y, Z ) + X];
+         o    = fromAcc< T >( R( toAcc( o ) ) + sb[X + kBox * ( Y + kBox * Z )] );
+      }
+   }
+}
+
 // k at the quadrature points of level L (built on first use when the operator
 // stores coefficients; nullptr when it evaluates k in the apply kernel)
 // k at the quadrature points of level L in value type K (built on first use
@@ -604,7 +700,16 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
       const K*      table  = p2Coefficients< K >( ctx, op, L, src.dtype );
       const auto*   geom   = p2Geometry< T >( op, L );
       ctx->beginTiming();
-      if ( table )
+      if ( mm <= kSmallMaxMM )
+      {
+         if ( table )
+            kP2SmallApply< T, K, true ><<< unsigned( lat.nc ), 64, 0, ctx->stream >>>(
+                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, lat.nc, lat.sc, lat.w, (const T*) s, (T*) d );
+         else
+            kP2SmallApply< T, K, false ><<< unsigned( lat.nc ), 64, 0, ctx->stream >>>(
+                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, lat.nc, lat.sc, lat.w, (const T*) s, (T*) d );
+      }
+      else if ( table )
          kP2TileApplyStored< T, K ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
              geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d );
       else

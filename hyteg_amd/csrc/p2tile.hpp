@@ -141,15 +141,16 @@ struct TileThread
    R    ls0;        // lam(p0) of the shell map
 };
 
+// micro-cell type t at anchor index a (ax + 4 ay + 16 az) of tile tc
 template < class R >
-__device__ __forceinline__ TileThread< R > tileThread( const P2CellGeomT< R >& g, const TileCoord& tc, int64_t mm )
+__device__ __forceinline__ TileThread< R > tileThreadAt( const P2CellGeomT< R >& g, const TileCoord& tc, int64_t mm,
+                                                         int t, int a )
 {
    TileThread< R > th;
-   th.t        = __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
-   const int a = int( threadIdx.x ) & 63;
-   th.ax       = a & 3;
-   th.ay       = ( a >> 2 ) & 3;
-   th.az       = a >> 4;
+   th.t             = t;
+   th.ax            = a & 3;
+   th.ay            = ( a >> 2 ) & 3;
+   th.az            = a >> 4;
    const int64_t gx = tc.x0 + th.ax, gy = tc.y0 + th.ay, gz = tc.z0 + th.az;
    th.valid         = gx + gy + gz <= mm - 1 - MicroCells::rowShift[th.t];
    const R vx = R( gx + cVof[th.t][0][0] ), vy = R( gy + cVof[th.t][0][1] ), vz = R( gz + cVof[th.t][0][2] );
@@ -158,6 +159,13 @@ __device__ __forceinline__ TileThread< R > tileThread( const P2CellGeomT< R >& g
       th.p0[d] = g.v0[d] + g.hE[3 * d] * vx + g.hE[3 * d + 1] * vy + g.hE[3 * d + 2] * vz;
    th.ls0 = g.nl[0] * th.p0[0] + g.nl[1] * th.p0[1] + g.nl[2] * th.p0[2] - g.lam0;
    return th;
+}
+
+// the tile kernels: one type per wavefront, the anchor from the lane
+template < class R >
+__device__ __forceinline__ TileThread< R > tileThread( const P2CellGeomT< R >& g, const TileCoord& tc, int64_t mm )
+{
+   return tileThreadAt( g, tc, mm, __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 ), int( threadIdx.x ) & 63 );
 }
 
 // LDS box positions of the 10 P2 DoFs of the thread's micro-cell (vertices in

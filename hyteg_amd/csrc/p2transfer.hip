@@ -294,19 +294,26 @@ __global__ void kP2RestrictCell( const int32_t* __restrict__ rows, int64_t nrows
    }
 }
 
+// one wavefront per owned boundary DoF: the lanes split the stencil entries of
+// each incident cell (a macro-vertex DoF of the shell sees up to ~36 cells with
+// 125 entries each) and the wave sums its lanes in a fixed butterfly order
+constexpr int kBndWaves = 4;
 template < class T >
-__global__ void kP2RestrictBnd( BndGeom gc, int64_t nbc, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
-                                const double* __restrict__ clsInvN, int64_t scf, int64_t wf, int64_t mmf,
-                                const T* __restrict__ fine, T* __restrict__ coarse )
+__global__ __launch_bounds__( 64 * kBndWaves ) void kP2RestrictBnd( BndGeom gc, int64_t nbc, int64_t nc,
+                                                                   const uint8_t* __restrict__ bndFlag, int flag,
+                                                                   const double* __restrict__ clsInvN, int64_t scf,
+                                                                   int64_t wf, int64_t mmf, const T* __restrict__ fine,
+                                                                   T* __restrict__ coarse )
 {
    using A         = typename Acc< T >::type;
-   const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int     l = int( threadIdx.x ) & 63;
+   const int64_t b = int64_t( blockIdx.x ) * kBndWaves + ( int( threadIdx.x ) >> 6 );
    if ( b >= nbc || !( bndFlag[b] & flag ) )
-      return;
+      return; // uniform over the wave
    A s = A( 0 );
    forEachIncident( gc, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
       const int q = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
-      for ( int e = 0; e < cRN[q]; ++e )
+      for ( int e = l; e < cRN[q]; e += 64 )
       {
          const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + cROff[q][e][1], fz = 2 * z + cROff[q][e][2];
          if ( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf )
@@ -316,7 +323,11 @@ __global__ void kP2RestrictBnd( BndGeom gc, int64_t nbc, int64_t nc, const uint8
          s += w * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
       }
    } );
-   coarse[nc * gc.sc + b] = fromAcc< T >( s );
+#pragma unroll
+   for ( int o = 32; o > 0; o >>= 1 )
+      s += __shfl_xor( s, o, 64 );
+   if ( l == 0 )
+      coarse[nc * gc.sc + b] = fromAcc< T >( s );
 }
 
 static constexpr int kRowBlockT = 64;
@@ -342,9 +353,9 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
       }
       if ( lc.nb > 0 )
       {
-         kP2RestrictBnd< T ><<< gridFor( lc.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lc ), lc.nb, lc.nc,
-                                                                             lc.bndFlag.p, flag, st.clsInvN.p, lf.sc,
-                                                                             lf.w, lf.mm, (const T*) fp, (T*) cp );
+         kP2RestrictBnd< T ><<< gridFor( lc.nb, kBndWaves ), 64 * kBndWaves, 0, ctx->stream >>>(
+             BndGeom::of( st, lc ), lc.nb, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp,
+             (T*) cp );
          HG_CHECK_HIP( hipGetLastError() );
       }
    } );

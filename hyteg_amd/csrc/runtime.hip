@@ -262,6 +262,48 @@ __global__ void kLinComb( int64_t ncell, int64_t nb, const uint8_t* __restrict__
    dst[p] = fromAcc< T >( v );
 }
 
+// dst (+)= sum_i c_i f_i with c_i = sign_i * (slots[num_i] / slots[den_i]); a
+// negative slot index stands for 1.  The quotient is formed first and then
+// signed, which is exactly the host's alpha = num / den followed by -alpha.
+struct SlotCoef
+{
+   double sign;
+   int    num, den;
+};
+__device__ __forceinline__ double slotCoef( const SlotCoef& c, const double* __restrict__ slots )
+{
+   const double n = c.num >= 0 ? slots[c.num] : 1.0;
+   const double d = c.den >= 0 ? slots[c.den] : 1.0;
+   return c.sign * ( n / d );
+}
+template < class T >
+__global__ void kLinCombSlots( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, int nsrc,
+                               SlotCoef c0, SlotCoef c1, SlotCoef c2, const double* __restrict__ slots,
+                               const T* __restrict__ f0, const T* __restrict__ f1, const T* __restrict__ f2, int add,
+                               int lv, T* __restrict__ dst )
+{
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv;
+   const int64_t n   = ( flag & kInner ? ncell : 0 ) + nb;
+   if ( i >= n )
+      return;
+   const int64_t pt = ( flag & kInner ) ? i : ncell + i;
+   if ( pt >= ncell && !( bndFlag[pt - ncell] & flag ) )
+      return;
+   const int64_t p = ( pt << lv ) + ( tid & ( ( int64_t( 1 ) << lv ) - 1 ) );
+   A v = A( 0 );
+   if ( nsrc > 0 )
+      v += A( slotCoef( c0, slots ) ) * toAcc( f0[p] );
+   if ( nsrc > 1 )
+      v += A( slotCoef( c1, slots ) ) * toAcc( f1[p] );
+   if ( nsrc > 2 )
+      v += A( slotCoef( c2, slots ) ) * toAcc( f2[p] );
+   if ( add )
+      v += toAcc( dst[p] );
+   dst[p] = fromAcc< T >( v );
+}
+
 template < class T >
 __global__ void kFill( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, double value, int lv,
                        T* __restrict__ dst )
@@ -429,18 +471,28 @@ __global__ void kDotFinal( const double* __restrict__ partials, int n, int lv, d
       out[blockIdx.x] = red[0];
 }
 
-void dotDeviceBatched( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv, double* out )
+// launches the reduction of the 2^lv dot products into dev[0 .. 2^lv) on the
+// context's stream; the grid covers the flagged entries (one block at least)
+static void dotLaunch( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv,
+                       double* dev )
 {
-   const int nv = 1 << lv;
+   const int     nv     = 1 << lv;
+   const int64_t n      = ( ( flag & kInner ? lat.nc * lat.sc : 0 ) + lat.nb ) << lv;
+   const int     blocks = int( std::min< int64_t >( kDotBlocks, std::max< int64_t >( 1, ( n + kBlock - 1 ) / kBlock ) ) );
    dispatchType( dtype, [&]( auto tag ) {
       using T = decltype( tag );
-      kDotPartial< T ><<< kDotBlocks, kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, lat.nb, lat.cellMask.p,
-                                                                lat.bndFlag.p, flag, (const T*) a, (const T*) b, lv,
-                                                                ctx->partials.p );
+      kDotPartial< T ><<< blocks, kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, lat.nb, lat.cellMask.p, lat.bndFlag.p,
+                                                             flag, (const T*) a, (const T*) b, lv, ctx->partials.p );
    } );
-   double* res = ctx->partials.p + ( int64_t( kDotBlocks ) << kMaxLv );
-   kDotFinal<<< nv, kBlock, 0, ctx->stream >>>( ctx->partials.p, kDotBlocks, lv, res );
+   kDotFinal<<< nv, kBlock, 0, ctx->stream >>>( ctx->partials.p, blocks, lv, dev );
    HG_CHECK_HIP( hipGetLastError() );
+}
+
+void dotDeviceBatched( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv, double* out )
+{
+   const int nv  = 1 << lv;
+   double*   res = ctx->partials.p + ( int64_t( kDotBlocks ) << kMaxLv );
+   dotLaunch( ctx, lat, a, b, dtype, flag, lv, res );
    HG_CHECK_HIP( hipMemcpyAsync( ctx->hostResult, res, sizeof( double ) * nv, hipMemcpyDeviceToHost, ctx->stream ) );
    HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
    for ( int k = 0; k < nv; ++k )
@@ -557,6 +609,8 @@ int hg_ctx_create( int device, hg_ctx** out )
       HG_CHECK_HIP( hipEventCreate( &ctx->ev0 ) );
       HG_CHECK_HIP( hipEventCreate( &ctx->ev1 ) );
       ctx->partials.alloc( ctx, ( size_t( kDotBlocks ) << kMaxLv ) + ( 1 << kMaxLv ) );
+      ctx->slots.alloc( ctx, kSlots );
+      ctx->slotUsed.assign( kSlots, 0 );
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
    } catch ( const std::exception& e )
    {
@@ -582,6 +636,7 @@ int hg_ctx_destroy( hg_ctx* h )
    ctx->functions.clear();
    ctx->storages.clear();
    ctx->partials.release();
+   ctx->slots.release();
    if ( ctx->hostResult )
       (void) hipHostFree( ctx->hostResult );
    if ( ctx->ev0 )
@@ -975,6 +1030,114 @@ int hg_function_interpolate_random( hg_ctx* h, hg_handle fn, uint64_t seed, doub
          if ( l.nb > 0 )
             kRandomOwned< T ><<< gridFor( l.nb << f.lv ), kBlock, 0, ctx->stream >>>(
                 l.nb, l.nc * l.sc, l.bndFlag.p, flag, seed, lo, hi, f.lv, (T*) f.ptr( level ) );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+int hg_slots_alloc( hg_ctx* h, int n, int* first )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( n < 1 || n > kSlots || !first )
+         throw Error( HG_ERR_ARG, "slots: invalid count" );
+      for ( int f = 0; f + n <= kSlots; ++f )
+      {
+         int k = 0;
+         while ( k < n && !ctx->slotUsed[size_t( f + k )] )
+            ++k;
+         if ( k == n )
+         {
+            for ( k = 0; k < n; ++k )
+               ctx->slotUsed[size_t( f + k )] = 1;
+            *first = f;
+            return;
+         }
+         f += k;
+      }
+      throw Error( HG_ERR_STATE, "slots: all " + std::to_string( kSlots ) + " device scalar slots are in use" );
+   } );
+}
+
+int hg_slots_free( hg_ctx* h, int first, int n )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( first < 0 || n < 1 || first + n > kSlots )
+         throw Error( HG_ERR_ARG, "slots: invalid range" );
+      for ( int k = 0; k < n; ++k )
+         ctx->slotUsed[size_t( first + k )] = 0;
+   } );
+}
+
+static void checkSlots( const Context* ctx, int first, int n )
+{
+   if ( first < 0 || n < 1 || first + n > kSlots )
+      throw Error( HG_ERR_ARG, "slot index out of range" );
+   for ( int k = 0; k < n; ++k )
+      if ( !ctx->slotUsed[size_t( first + k )] )
+         throw Error( HG_ERR_ARG, "slot " + std::to_string( first + k ) + " is not allocated" );
+}
+
+int hg_slots_read( hg_ctx* h, int first, int n, double* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      checkSlots( ctx, first, n );
+      if ( !out || n > ( 1 << kMaxLv ) )
+         throw Error( HG_ERR_ARG, "slots: invalid read" );
+      HG_CHECK_HIP( hipMemcpyAsync( ctx->hostResult, ctx->slots.p + first, sizeof( double ) * size_t( n ),
+                                    hipMemcpyDeviceToHost, ctx->stream ) );
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+      for ( int k = 0; k < n; ++k )
+         out[k] = ctx->hostResult[k];
+   } );
+}
+
+int hg_function_dot_slot( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, int slot )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& fa = ctx->function( a );
+      Function& fb = ctx->function( b );
+      checkLevel( fa, level );
+      checkLevel( fb, level );
+      if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off || fa.lv != fb.lv )
+         throw Error( HG_ERR_ARG, "dot needs functions of one storage, value type and vector count" );
+      checkSlots( ctx, slot, fa.nvec() );
+      dotLaunch( ctx, fa.lat( ctx, level ), fa.ptr( level ), fb.ptr( level ), fa.dtype, flag, fa.lv,
+                 ctx->slots.p + slot );
+   } );
+}
+
+int hg_function_lincomb_slots( hg_ctx* h, hg_handle dst, int nsrc, const double* signs, const int* num, const int* den,
+                               const hg_handle* srcs, int level, int flag, int add )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( nsrc < 0 || nsrc > 3 || ( nsrc > 0 && ( !signs || !num || !den || !srcs ) ) )
+         throw Error( HG_ERR_ARG, "lincomb_slots takes 0..3 source functions" );
+      Function& d = ctx->function( dst );
+      checkLevel( d, level );
+      void*    src[3] = { nullptr, nullptr, nullptr };
+      SlotCoef c[3]   = { { 0.0, -1, -1 }, { 0.0, -1, -1 }, { 0.0, -1, -1 } };
+      for ( int i = 0; i < nsrc; ++i )
+      {
+         Function& s = ctx->function( srcs[i] );
+         checkLevel( s, level );
+         if ( s.st != d.st || s.dtype != d.dtype || s.off != d.off || s.lv != d.lv )
+            throw Error( HG_ERR_ARG, "lincomb_slots needs functions of one storage, value type and vector count" );
+         if ( num[i] >= 0 )
+            checkSlots( ctx, num[i], 1 );
+         if ( den[i] >= 0 )
+            checkSlots( ctx, den[i], 1 );
+         src[i] = s.ptr( level );
+         c[i]   = SlotCoef{ signs[i], num[i], den[i] };
+      }
+      Lattice&      l = d.lat( ctx, level );
+      const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
+      if ( n == 0 )
+         return;
+      dispatchType( d.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         kLinCombSlots< T ><<< gridFor( n << d.lv ), kBlock, 0, ctx->stream >>>(
+             l.nc * l.sc, l.nb, l.bndFlag.p, flag, nsrc, c[0], c[1], c[2], ctx->slots.p, (const T*) src[0],
+             (const T*) src[1], (const T*) src[2], add != 0, d.lv, (T*) d.ptr( level ) );
       } );
       HG_CHECK_HIP( hipGetLastError() );
    } );

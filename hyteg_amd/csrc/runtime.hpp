@@ -173,6 +173,11 @@ struct Context
    DevBuf< double > partials;
    double*          hostResult = nullptr; // pinned
 
+   // device-resident FP64 scalars of Krylov loops (dot results and the step
+   // ratios formed from them), handed out in ranges by hg_slots_alloc
+   DevBuf< double >    slots;
+   std::vector< char > slotUsed;
+
    bool       timing = false;
    hipEvent_t ev0 = nullptr, ev1 = nullptr;
    double     lastMs = 0.0;
@@ -234,6 +239,7 @@ inline void DevBuf< T >::release()
 constexpr int kBlock      = 256;
 constexpr int kDotBlocks  = 1024;
 constexpr int kMaxLv      = 6; // at most 64 interleaved vectors per function
+constexpr int kSlots      = 256; // device scalar slots per context
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2): blockIdx b runs on XCD b % 8.  xcdBlock maps the hardware block id to a

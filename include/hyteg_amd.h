@@ -159,6 +159,24 @@ int hg_function_interpolate_random( hg_ctx* ctx, hg_handle fn, uint64_t seed, do
 int hg_function_dot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
 /* per-vector dotGlobal of interleaved functions: out[nvec] */
 int hg_function_dot_batched( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
+/* Device-resident scalars for Krylov loops (no counterpart in the reference,
+ * whose dotGlobal returns every dot product to the host): each context holds
+ * 256 FP64 slots in device memory.  hg_slots_alloc reserves n consecutive slots
+ * (first index in *first), hg_slots_free returns them, hg_slots_read copies n
+ * slot values to the host (the one call here that waits for the stream).
+ * hg_function_dot_slot writes dotGlobal(a, b, level, flag) of each of the
+ * function's vectors into slot, slot+1, ... without waiting.
+ * hg_function_lincomb_slots is assign (add = 0) or add (add = 1) of
+ * hg_function_assign / hg_function_add with the scalars c_i = signs[i] *
+ * (slot[num[i]] / slot[den[i]]) read on the device (a negative index stands
+ * for 1); CGSolver uses them to keep alpha and beta on the device
+ * (CGSolver.hpp:149-190 computes them on the host). */
+int hg_slots_alloc( hg_ctx* ctx, int n, int* first );
+int hg_slots_free( hg_ctx* ctx, int first, int n );
+int hg_slots_read( hg_ctx* ctx, int first, int n, double* out );
+int hg_function_dot_slot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, int slot );
+int hg_function_lincomb_slots( hg_ctx* ctx, hg_handle dst, int nsrc, const double* signs, const int* num, const int* den,
+                               const hg_handle* srcs, int level, int flag, int add );
 /* vector k of fn scaled by scales[k] on flagged DoFs */
 int hg_function_scale_batched( hg_ctx* ctx, hg_handle fn, const double* scales, int level, int flag );
 /* multElementwise({a, b}, level, flag): dst = a * b (VertexDoFFunction.cpp:1408) */

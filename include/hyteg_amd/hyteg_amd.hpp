@@ -275,6 +275,26 @@ class LatticeFunction
       check( ctx_, hg_function_dot( ctx_, h_, other.h_, int( level ), dofs( flag ), &r ), "dotGlobal" );
       return ValueType( r );
    }
+   // dotGlobal into a device scalar slot of the context (no wait for the result)
+   void dotToSlot( const LatticeFunction& other, uint_t level, DoFType flag, int slot ) const
+   {
+      check( ctx_, hg_function_dot_slot( ctx_, h_, other.h_, int( level ), dofs( flag ), slot ), "dotToSlot" );
+   }
+   // assign / add with scalars formed on the device: signs[i] * slot[num[i]] / slot[den[i]]
+   void linCombSlots( const std::vector< double >& signs, const std::vector< int >& num, const std::vector< int >& den,
+                      const std::vector< std::reference_wrapper< const LatticeFunction > >& functions, uint_t level,
+                      DoFType flag, bool add ) const
+   {
+      if ( signs.size() != functions.size() || num.size() != functions.size() || den.size() != functions.size() )
+         throw std::invalid_argument( "linCombSlots needs one coefficient per function" );
+      std::vector< hg_handle > hs;
+      for ( const auto& f : functions )
+         hs.push_back( f.get().h_ );
+      check( ctx_,
+             hg_function_lincomb_slots( ctx_, h_, int( hs.size() ), signs.data(), num.data(), den.data(), hs.data(),
+                                        int( level ), dofs( flag ), add ? 1 : 0 ),
+             "linCombSlots" );
+   }
    template < typename OtherT >
    void copyFrom( const LatticeFunction< OtherT, Space >& other, uint_t level ) const
    {
@@ -737,12 +757,30 @@ class CGSolver : public Solver< OperatorType >
    , z_( "z", storage, minLevel, maxLevel )
    , ap_( "ap", storage, minLevel, maxLevel )
    , r_( "r", storage, minLevel, maxLevel )
-   {}
+   {
+      if constexpr ( kDeviceScalars )
+      {
+         ctx_ = storage->ctx();
+         check( ctx_, hg_slots_alloc( ctx_, 4, &slot0_ ), "CGSolver slots" );
+      }
+   }
+   ~CGSolver() override
+   {
+      if constexpr ( kDeviceScalars )
+         hg_slots_free( ctx_, slot0_, 4 );
+   }
+   CGSolver( const CGSolver& )            = delete;
+   CGSolver& operator=( const CGSolver& ) = delete;
 
    void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
    {
       if ( maxIter_ == 0 )
          return;
+      if constexpr ( kDeviceScalars )
+      {
+         solveDeviceScalars( A, x, b, level );
+         return;
+      }
       copyBCs( x, p_ );
       copyBCs( x, z_ );
       copyBCs( x, ap_ );
@@ -783,6 +821,59 @@ class CGSolver : public Solver< OperatorType >
    uint_t getIterations() const { return iterations_; }
 
  private:
+   // FP64 lattice functions keep pAp, r.r and r.z in device slots of the context
+   static constexpr bool kDeviceScalars =
+       std::is_same_v< FunctionType, LatticeFunction< double, 1 > > || std::is_same_v< FunctionType, LatticeFunction< double, 2 > >;
+
+   // The iteration above with its scalars on the device: alpha = r.z / pAp and
+   // beta = r.z_new / r.z are formed inside the update kernels from slot values,
+   // and r.r is the one value read back per iteration (for the stopping test), so
+   // the iterates and the stopping step are those of the host-scalar loop.
+   void solveDeviceScalars( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level )
+   {
+      const int rr = slot0_, pap = slot0_ + 1;
+      int       rz[2] = { slot0_ + 2, slot0_ + 3 }, cur = 0;
+      copyBCs( x, p_ );
+      copyBCs( x, z_ );
+      copyBCs( x, ap_ );
+      copyBCs( x, r_ );
+      p_.setToZero( level );
+      z_.setToZero( level );
+      ap_.setToZero( level );
+      r_.setToZero( level );
+      A.apply( x, p_, level, flag_, Replace );
+      r_.assign( { 1.0, -1.0 }, { b, p_ }, level, flag_ );
+      precondition( A, level );
+      p_.assign( { 1.0 }, { z_ }, level, flag_ );
+      r_.dotToSlot( z_, level, flag_, rz[cur] );
+      r_.dotToSlot( r_, level, flag_, rr );
+      double rs = 0.0;
+      check( ctx_, hg_slots_read( ctx_, rr, 1, &rs ), "CGSolver" );
+      const double res_start = std::sqrt( rs );
+      iterations_            = 0;
+      if ( res_start < absTol_ )
+         return;
+      for ( uint_t i = 0; i < maxIter_; ++i )
+      {
+         A.apply( p_, ap_, level, flag_, Replace );
+         p_.dotToSlot( ap_, level, flag_, pap );
+         x.linCombSlots( { 1.0 }, { rz[cur] }, { pap }, { p_ }, level, flag_, true );
+         r_.linCombSlots( { -1.0 }, { rz[cur] }, { pap }, { ap_ }, level, flag_, true );
+         r_.dotToSlot( r_, level, flag_, rr );
+         double rsnew = 0.0;
+         check( ctx_, hg_slots_read( ctx_, rr, 1, &rsnew ), "CGSolver" );
+         const double relRes = std::sqrt( rsnew ) / res_start;
+         iterations_         = i;
+         if ( relRes < relTol_ || std::sqrt( rsnew ) < absTol_ )
+            break;
+         z_.interpolate( 0.0, level, All );
+         precondition( A, level );
+         r_.dotToSlot( z_, level, flag_, rz[1 - cur] );
+         p_.linCombSlots( { 1.0, 1.0 }, { -1, rz[1 - cur] }, { -1, rz[cur] }, { z_, p_ }, level, flag_, false );
+         cur = 1 - cur;
+      }
+   }
+
    // z = M^-1 r (identity when no preconditioner is given)
    void precondition( const OperatorType& A, uint_t level )
    {
@@ -797,6 +888,8 @@ class CGSolver : public Solver< OperatorType >
    FunctionType                              p_, z_, ap_, r_;
    DoFType                                   flag_       = Inner | NeumannBoundary | FreeslipBoundary;
    uint_t                                    iterations_ = 0;
+   hg_ctx*                                   ctx_        = nullptr;
+   int                                       slot0_      = -1;
 };
 
 // Preconditioner z = D^-1 r with the operator's inverse diagonal
