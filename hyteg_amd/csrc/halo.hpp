@@ -111,6 +111,60 @@ __device__ __forceinline__ void forEachIncident( const BndGeom& g, int64_t b, F&
    }
 }
 
+// the same incidences by index: entries [e0, e1) of owned boundary DoF b, and
+// the cell point (c; x, y, z) of entry e (the order of forEachIncident)
+__device__ __forceinline__ void incidentRange( const BndGeom& g, int64_t b, int32_t& e0, int32_t& e1 )
+{
+   if ( b < g.eoff )
+   {
+      e0 = g.vptr[b];
+      e1 = g.vptr[b + 1];
+   }
+   else if ( b < g.foff )
+   {
+      const int64_t eid = ( b - g.eoff ) / g.ein;
+      e0                = g.eptr[eid];
+      e1                = g.eptr[eid + 1];
+   }
+   else
+   {
+      const int64_t fid = ( b - g.foff ) / g.fin;
+      e0                = g.fptr[fid];
+      e1                = g.fptr[fid + 1];
+   }
+}
+__device__ __forceinline__ int64_t incidentAt( const BndGeom& g, int64_t b, int32_t e, int64_t& x, int64_t& y,
+                                               int64_t& z )
+{
+   int64_t t[4] = { 0, 0, 0, 0 };
+   int64_t c;
+   if ( b < g.eoff )
+   {
+      c    = g.vcells[2 * e];
+      t[g.vcells[2 * e + 1]] = g.mm;
+   }
+   else if ( b < g.foff )
+   {
+      const int64_t eid = ( b - g.eoff ) / g.ein;
+      const int64_t s   = ( b - g.eoff ) - eid * g.ein + 1;
+      c                 = g.ecells[2 * e];
+      edgeToCell( g.meta[c], g.ecells[2 * e + 1], g.mm, s, t );
+   }
+   else
+   {
+      const int64_t fid = ( b - g.foff ) / g.fin;
+      const int64_t i   = ( b - g.foff ) - fid * g.fin;
+      int64_t       a, bb;
+      decodeTri( g.mm - 2, i, a, bb );
+      c = g.fcells[2 * e];
+      faceToCell( g.meta[c], g.fcells[2 * e + 1], g.mm, a + 1, bb + 1, t );
+   }
+   x = t[1];
+   y = t[2];
+   z = t[3];
+   return c;
+}
+
 // class of a cell lattice point: -1 interior, 0..3 face, 4..9 edge, 10..13 vertex
 __device__ __forceinline__ int pointClass( int64_t mm, int64_t x, int64_t y, int64_t z )
 {

@@ -14,6 +14,7 @@
 #include <array>
 #include <cmath>
 #include <map>
+#include <thread>
 
 #include "abi_util.hpp"
 #include "halo.hpp"
@@ -215,51 +216,106 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
    o.stencil.upload( ctx, o.hStencil );
    o.clsStencil.upload( ctx, o.hCls );
 
-   // diagonal and coupling graph of the owned boundary DoFs
+   // diagonal and coupling graph of the owned boundary DoFs.  Phase 1 walks the
+   // boundary points of contiguous cell ranges in parallel and records every
+   // contribution in a bucket of its row; phase 2 applies the buckets in
+   // parallel, each in the order of the cell ranges, so every row receives its
+   // additions in the order of a serial walk over the cells (the assembled
+   // values do not depend on the thread count).
    std::vector< double >                 diag( lat.nb, 0.0 );
    std::vector< std::vector< int32_t > > adj( lat.nb );
    o.csr = lat.nb > 0 && lat.nb <= kCsrRowBudget;
    std::vector< std::vector< std::pair< int64_t, double > > > row( o.csr ? lat.nb : 0 );
    const int64_t                                             ownerBase = lat.nc * lat.sc;
-   for ( int c = 0; c < t.nc; ++c )
-      for ( int64_t z = 0; z <= mm; ++z )
-         for ( int64_t y = 0; y + z <= mm; ++y )
-            for ( int64_t x = 0; x + y + z <= mm; ++x )
-            {
-               const int cls = hostPointClass( mm, x, y, z );
-               if ( cls < 0 )
-                  continue;
-               const int64_t r = lat.num.locate( t, c, x, y, z );
-               const double* W = &o.hCls[( 14 * size_t( c ) + cls ) * 15];
-               diag[r] += W[0];
-               for ( int s = 1; s < 15; ++s )
+   struct Contribution
+   {
+      int64_t pos;  // storage position of the neighbour, -1: the diagonal
+      double  w;
+      int32_t r;    // row (owned boundary DoF)
+      int32_t adjq; // boundary neighbour for the colouring graph, -1: none
+   };
+   const int nthreads = int( std::max( 1u, std::min( 16u, std::thread::hardware_concurrency() ) ) );
+   const int nbucket  = nthreads;
+   std::vector< std::vector< std::vector< Contribution > > > rec{
+       size_t( nthreads ), std::vector< std::vector< Contribution > >( static_cast< size_t >( nbucket ) ) };
+   std::vector< char > outside( size_t( nthreads ), 0 );
+   auto bucketOf = [&]( int64_t r ) { return int( ( r * nbucket ) / std::max< int64_t >( lat.nb, 1 ) ); };
+   auto phase1   = [&]( int th ) {
+      const int c0 = int( ( int64_t( t.nc ) * th ) / nthreads ), c1 = int( ( int64_t( t.nc ) * ( th + 1 ) ) / nthreads );
+      auto&     out = rec[size_t( th )];
+      for ( int c = c0; c < c1; ++c )
+         for ( int64_t z = 0; z <= mm; ++z )
+            for ( int64_t y = 0; y + z <= mm; ++y )
+               for ( int64_t x = 0; x + y + z <= mm; ++x )
                {
-                  const int64_t qx = x + Stencil15::off[s][0], qy = y + Stencil15::off[s][1], qz = z + Stencil15::off[s][2];
-                  const int     qc = hostPointClass( mm, qx, qy, qz );
-                  if ( qc == -2 )
+                  const int cls = hostPointClass( mm, x, y, z );
+                  if ( cls < 0 )
+                     continue;
+                  const int64_t r  = lat.num.locate( t, c, x, y, z );
+                  auto&         bk = out[size_t( bucketOf( r ) )];
+                  const double* W  = &o.hCls[( 14 * size_t( c ) + cls ) * 15];
+                  bk.push_back( { -1, W[0], int32_t( r ), -1 } );
+                  for ( int s = 1; s < 15; ++s )
                   {
-                     if ( W[s] != 0.0 )
-                        throw Error( -4, "boundary stencil reaches outside its cell" );
-                     continue;
+                     const int64_t qx = x + Stencil15::off[s][0], qy = y + Stencil15::off[s][1],
+                                   qz = z + Stencil15::off[s][2];
+                     const int qc = hostPointClass( mm, qx, qy, qz );
+                     if ( qc == -2 )
+                     {
+                        if ( W[s] != 0.0 )
+                           outside[size_t( th )] = 1;
+                        continue;
+                     }
+                     if ( W[s] == 0.0 )
+                        continue;
+                     const int64_t qb  = qc < 0 ? -1 : lat.num.locate( t, c, qx, qy, qz );
+                     const int64_t pos = qc < 0 ? int64_t( c ) * lat.sc + tetIndex( lat.w, qx, qy, qz ) : ownerBase + qb;
+                     bk.push_back( { pos, W[s], int32_t( r ), int32_t( qb ) } );
                   }
-                  if ( W[s] == 0.0 )
-                     continue;
-                  if ( o.csr )
-                  {
-                     const int64_t pos = qc < 0 ? int64_t( c ) * lat.sc + tetIndex( lat.w, qx, qy, qz ) :
-                                                  ownerBase + lat.num.locate( t, c, qx, qy, qz );
-                     auto& rr = row[r];
-                     auto  it = std::find_if( rr.begin(), rr.end(), [&]( const auto& e ) { return e.first == pos; } );
-                     if ( it == rr.end() )
-                        rr.emplace_back( pos, W[s] );
-                     else
-                        it->second += W[s];
-                  }
-                  if ( qc < 0 )
-                     continue;
-                  adj[r].push_back( int32_t( lat.num.locate( t, c, qx, qy, qz ) ) );
                }
+   };
+   auto phase2 = [&]( int b ) {
+      for ( int th = 0; th < nthreads; ++th )
+         for ( const Contribution& e : rec[size_t( th )][size_t( b )] )
+         {
+            if ( e.pos < 0 )
+            {
+               diag[size_t( e.r )] += e.w;
+               continue;
             }
+            if ( o.csr )
+            {
+               auto& rr = row[size_t( e.r )];
+               auto  it = std::find_if( rr.begin(), rr.end(), [&]( const auto& v ) { return v.first == e.pos; } );
+               if ( it == rr.end() )
+                  rr.emplace_back( e.pos, e.w );
+               else
+                  it->second += e.w;
+            }
+            if ( e.adjq >= 0 )
+               adj[size_t( e.r )].push_back( e.adjq );
+         }
+   };
+   {
+      std::vector< std::thread > pool;
+      for ( int th = 1; th < nthreads; ++th )
+         pool.emplace_back( phase1, th );
+      phase1( 0 );
+      for ( auto& p : pool )
+         p.join();
+   }
+   for ( char bad : outside )
+      if ( bad )
+         throw Error( -4, "boundary stencil reaches outside its cell" );
+   {
+      std::vector< std::thread > pool;
+      for ( int b = 1; b < nbucket; ++b )
+         pool.emplace_back( phase2, b );
+      phase2( 0 );
+      for ( auto& p : pool )
+         p.join();
+   }
+   rec.clear();
    if ( o.csr )
    {
       std::vector< int64_t > ptr( lat.nb + 1, 0 ), pos;

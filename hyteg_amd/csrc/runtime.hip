@@ -197,9 +197,29 @@ __global__ void kPushAdditive( BndGeom g, const uint8_t* __restrict__ bndFlag, i
       return;
    const int64_t base = nc * g.sc;
    A             s    = zeroOut ? A( 0 ) : toAcc( v[( ( base + b ) << lv ) + vec] );
-   forEachIncident( g, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
-      s += toAcc( v[( ( c * g.sc + tetIndex( g.w, x, y, z ) ) << lv ) + vec] );
-   } );
+   // the halo copies in the order of forEachIncident, eight loads in flight (a
+   // macro-vertex of the shell has up to ~36 incident cells)
+   int32_t e0, e1;
+   incidentRange( g, b, e0, e1 );
+   for ( int32_t e = e0; e < e1; e += 8 )
+   {
+      A buf[8];
+#pragma unroll
+      for ( int j = 0; j < 8; ++j )
+      {
+         buf[j] = A( 0 );
+         if ( e + j < e1 )
+         {
+            int64_t       x, y, z;
+            const int64_t c = incidentAt( g, b, e + j, x, y, z );
+            buf[j]          = toAcc( v[( ( c * g.sc + tetIndex( g.w, x, y, z ) ) << lv ) + vec] );
+         }
+      }
+#pragma unroll
+      for ( int j = 0; j < 8; ++j )
+         if ( e + j < e1 )
+            s += buf[j];
+   }
    v[( ( base + b ) << lv ) + vec] = fromAcc< T >( s );
 }
 
@@ -264,7 +284,9 @@ __global__ void kLinComb( int64_t ncell, int64_t nb, const uint8_t* __restrict__
 
 // dst (+)= sum_i c_i f_i with c_i = sign_i * (slots[num_i] / slots[den_i]); a
 // negative slot index stands for 1.  The quotient is formed first and then
-// signed, which is exactly the host's alpha = num / den followed by -alpha.
+// signed, which is exactly the host's alpha = num / den followed by -alpha.  A
+// zero denominator gives c_i = 0 (a Krylov step on an exactly solved system
+// leaves the iterate alone instead of turning it into NaN).
 struct SlotCoef
 {
    double sign;
@@ -274,7 +296,7 @@ __device__ __forceinline__ double slotCoef( const SlotCoef& c, const double* __r
 {
    const double n = c.num >= 0 ? slots[c.num] : 1.0;
    const double d = c.den >= 0 ? slots[c.den] : 1.0;
-   return c.sign * ( n / d );
+   return d == 0.0 ? 0.0 : c.sign * ( n / d );
 }
 template < class T >
 __global__ void kLinCombSlots( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, int nsrc,
@@ -523,6 +545,40 @@ Function& Context::function( int64_t h )
       throw Error( -3, "unknown function handle" );
    return *it->second;
 }
+void* Context::deviceAlloc( size_t bytes )
+{
+   auto it = pool.find( bytes );
+   if ( it != pool.end() )
+   {
+      void* p = it->second;
+      pool.erase( it );
+      poolBytes -= bytes;
+      return p;
+   }
+   void*          p = nullptr;
+   DeviceGateLock gate;
+   HG_CHECK_HIP( hipMalloc( &p, bytes ) );
+   return p;
+}
+void Context::deviceFree( void* p, size_t bytes )
+{
+   if ( bytes <= kPoolBlock && poolBytes + bytes <= kPoolBytes )
+   {
+      pool.emplace( bytes, p );
+      poolBytes += bytes;
+      return;
+   }
+   DeviceGateLock gate;
+   (void) hipFree( p );
+}
+void Context::drainPool()
+{
+   DeviceGateLock gate;
+   for ( auto& e : pool )
+      (void) hipFree( e.second );
+   pool.clear();
+   poolBytes = 0;
+}
 void Context::beginTiming()
 {
    if ( timing )
@@ -590,6 +646,25 @@ void setContextError( hg_ctx* h, const char* msg )
    if ( h )
       reinterpret_cast< Context* >( h )->err = msg ? msg : "";
 }
+std::mutex& deviceGate()
+{
+   static std::mutex gate;
+   return gate;
+}
+
+// ends the capture on ctx's stream and releases the gate it holds
+static hipError_t closeCapture( Context* ctx, hipGraph_t* g )
+{
+   const hipError_t e = hipStreamEndCapture( ctx->stream, g );
+   if ( ctx->capturing )
+   {
+      ctx->capturing = false;
+      --tlsCaptures;
+      deviceGate().unlock();
+   }
+   return e;
+}
+
 } // namespace hg
 
 extern "C" {
@@ -604,10 +679,14 @@ int hg_ctx_create( int device, hg_ctx** out )
    {
       ctx->device = device;
       HG_CHECK_HIP( hipSetDevice( device ) );
-      HG_CHECK_HIP( hipStreamCreateWithFlags( &ctx->stream, hipStreamNonBlocking ) );
-      HG_CHECK_HIP( hipHostMalloc( (void**) &ctx->hostResult, sizeof( double ) * ( 1 << kMaxLv ), hipHostMallocDefault ) );
-      HG_CHECK_HIP( hipEventCreate( &ctx->ev0 ) );
-      HG_CHECK_HIP( hipEventCreate( &ctx->ev1 ) );
+      {
+         DeviceGateLock gate;
+         HG_CHECK_HIP( hipStreamCreateWithFlags( &ctx->stream, hipStreamNonBlocking ) );
+         HG_CHECK_HIP(
+             hipHostMalloc( (void**) &ctx->hostResult, sizeof( double ) * ( 1 << kMaxLv ), hipHostMallocDefault ) );
+         HG_CHECK_HIP( hipEventCreate( &ctx->ev0 ) );
+         HG_CHECK_HIP( hipEventCreate( &ctx->ev1 ) );
+      }
       ctx->partials.alloc( ctx, ( size_t( kDotBlocks ) << kMaxLv ) + ( 1 << kMaxLv ) );
       ctx->slots.alloc( ctx, kSlots );
       ctx->slotUsed.assign( kSlots, 0 );
@@ -627,24 +706,38 @@ int hg_ctx_destroy( hg_ctx* h )
       return HG_ERR_ARG;
    auto* ctx = reinterpret_cast< Context* >( h );
    (void) hipSetDevice( ctx->device );
+   if ( ctx->capturing )
+   {
+      hipGraph_t g = nullptr;
+      (void) closeCapture( ctx, &g );
+      if ( g )
+         (void) hipGraphDestroy( g );
+   }
    if ( ctx->stream )
       (void) hipStreamSynchronize( ctx->stream );
-   for ( auto& g : ctx->graphs )
-      (void) hipGraphExecDestroy( g.second );
+   {
+      DeviceGateLock gate;
+      for ( auto& g : ctx->graphs )
+         (void) hipGraphExecDestroy( g.second );
+   }
    ctx->graphs.clear();
    ctx->operators.clear();
    ctx->functions.clear();
    ctx->storages.clear();
    ctx->partials.release();
    ctx->slots.release();
-   if ( ctx->hostResult )
-      (void) hipHostFree( ctx->hostResult );
-   if ( ctx->ev0 )
-      (void) hipEventDestroy( ctx->ev0 );
-   if ( ctx->ev1 )
-      (void) hipEventDestroy( ctx->ev1 );
-   if ( ctx->stream )
-      (void) hipStreamDestroy( ctx->stream );
+   ctx->drainPool();
+   {
+      DeviceGateLock gate;
+      if ( ctx->hostResult )
+         (void) hipHostFree( ctx->hostResult );
+      if ( ctx->ev0 )
+         (void) hipEventDestroy( ctx->ev0 );
+      if ( ctx->ev1 )
+         (void) hipEventDestroy( ctx->ev1 );
+      if ( ctx->stream )
+         (void) hipStreamDestroy( ctx->stream );
+   }
    delete ctx;
    return HG_OK;
 }
@@ -659,19 +752,39 @@ const char* hg_last_error( const hg_ctx* h )
 int hg_graph_begin_capture( hg_ctx* h )
 {
    return hg_guard( h, [&]( Context* ctx ) {
-      HG_CHECK_HIP( hipStreamBeginCapture( ctx->stream, hipStreamCaptureModeThreadLocal ) );
+      if ( ctx->capturing )
+         throw Error( HG_ERR_STATE, "a capture is already open on this context" );
+      deviceGate().lock();
+      ctx->capturing = true;
+      ++tlsCaptures;
+      const hipError_t e = hipStreamBeginCapture( ctx->stream, hipStreamCaptureModeThreadLocal );
+      if ( e != hipSuccess )
+      {
+         ctx->capturing = false;
+         --tlsCaptures;
+         deviceGate().unlock();
+         HG_CHECK_HIP( e );
+      }
    } );
 }
 
 int hg_graph_end_capture( hg_ctx* h, hg_handle* out )
 {
    return hg_guard( h, [&]( Context* ctx ) {
+      if ( !ctx->capturing )
+         throw Error( HG_ERR_STATE, "no capture is open on this context" );
+      hipGraph_t       g  = nullptr;
+      const hipError_t ec = closeCapture( ctx, &g );
       if ( !out )
+      {
+         if ( g )
+            (void) hipGraphDestroy( g );
          throw Error( HG_ERR_ARG, "null output" );
-      hipGraph_t g = nullptr;
-      HG_CHECK_HIP( hipStreamEndCapture( ctx->stream, &g ) );
-      hipGraphExec_t exec = nullptr;
-      const hipError_t e  = hipGraphInstantiate( &exec, g, nullptr, nullptr, 0 );
+      }
+      HG_CHECK_HIP( ec );
+      DeviceGateLock   gate;
+      hipGraphExec_t   exec = nullptr;
+      const hipError_t e    = hipGraphInstantiate( &exec, g, nullptr, nullptr, 0 );
       (void) hipGraphDestroy( g );
       HG_CHECK_HIP( e );
       *out               = ctx->nextHandle++;
@@ -696,6 +809,7 @@ int hg_graph_destroy( hg_ctx* h, hg_handle graph )
       if ( it == ctx->graphs.end() )
          throw Error( HG_ERR_HANDLE, "unknown graph handle" );
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+      DeviceGateLock gate;
       HG_CHECK_HIP( hipGraphExecDestroy( it->second ) );
       ctx->graphs.erase( it );
    } );

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -165,6 +166,7 @@ struct Context
    int64_t                                            nextHandle = 1;
    std::map< int64_t, std::unique_ptr< Storage > >      storages;
    std::map< int64_t, hipGraphExec_t >                  graphs; // instantiated captures
+   bool                                                 capturing = false; // holds the device gate
    bool                                                 useGraphs = true;
    std::map< int64_t, std::unique_ptr< Function > >     functions;
    std::map< int64_t, std::unique_ptr< OperatorBase > > operators;
@@ -181,6 +183,20 @@ struct Context
    bool       timing = false;
    hipEvent_t ev0 = nullptr, ev1 = nullptr;
    double     lastMs = 0.0;
+
+   // Freed device blocks are kept for allocations of the same size (the
+   // context's work runs on its one stream, so a reused block's earlier work is
+   // ordered before its new use): hipFree waits for every stream of the device,
+   // which a solve that builds and drops per-request operators would otherwise
+   // pay on every request.  At most kPoolBytes are kept, in blocks of at most
+   // kPoolBlock bytes; the rest is freed.
+   static constexpr size_t            kPoolBytes = size_t( 2 ) << 30;
+   static constexpr size_t            kPoolBlock = size_t( 256 ) << 20;
+   std::multimap< size_t, void* >     pool;
+   size_t                             poolBytes = 0;
+   void*                              deviceAlloc( size_t bytes );
+   void                               deviceFree( void* p, size_t bytes );
+   void                               drainPool();
 
    Storage&  storage( int64_t h );
    Function& function( int64_t h );
@@ -199,6 +215,26 @@ struct Context
    void endTiming();
 };
 
+// ---- device gate ------------------------------------------------------------
+// A stream capture must not overlap with another thread's capture, nor with
+// calls that set up or tear down device state (hipFree waits for every stream
+// of the device): such overlaps invalidate the capture.  A capture holds this
+// gate from its begin to its end; device allocation, deallocation, graph
+// instantiation and destruction, and stream and pinned-memory set-up hold it
+// for the call.  A thread that is capturing skips the gate (it must not
+// allocate then anyway: the HIP call fails instead of dead-locking).
+std::mutex&             deviceGate();
+inline thread_local int tlsCaptures = 0;
+struct DeviceGateLock
+{
+   std::unique_lock< std::mutex > lk;
+   DeviceGateLock()
+   {
+      if ( tlsCaptures == 0 )
+         lk = std::unique_lock< std::mutex >( deviceGate() );
+   }
+};
+
 // ---- DevBuf members (need the complete Context) ------------------------
 template < class T >
 inline void DevBuf< T >::alloc( Context* c, size_t count )
@@ -208,7 +244,7 @@ inline void DevBuf< T >::alloc( Context* c, size_t count )
    n   = count;
    if ( count == 0 )
       return;
-   HG_CHECK_HIP( hipMalloc( (void**) &p, count * sizeof( T ) ) );
+   p = static_cast< T* >( c->deviceAlloc( count * sizeof( T ) ) );
    HG_CHECK_HIP( hipMemsetAsync( p, 0, count * sizeof( T ), c->stream ) );
    c->bytes += int64_t( count * sizeof( T ) );
 }
@@ -227,9 +263,16 @@ inline void DevBuf< T >::release()
 {
    if ( p )
    {
-      (void) hipFree( p );
       if ( ctx )
+      {
+         ctx->deviceFree( p, n * sizeof( T ) );
          ctx->bytes -= int64_t( n * sizeof( T ) );
+      }
+      else
+      {
+         DeviceGateLock gate;
+         (void) hipFree( p );
+      }
    }
    p = nullptr;
    n = 0;

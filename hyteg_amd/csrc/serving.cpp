@@ -15,12 +15,14 @@
 // slot's residual is checked; converged requests leave and waiting ones take
 // free slots at the next cycle boundary.  Vectors never interact, so a
 // request's result does not depend on what else is in the batch.  P2 classes
-// solve one request at a time (Chebyshev-smoothed V(2,2), PCG coarse solve),
+// solve one request at a time (Chebyshev-smoothed V(2,2), fixed-count PCG on the
+// coarse level, the V-cycle replayed from a HIP graph captured per request),
 // with the coefficient k generated from the request's seed.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -106,6 +108,23 @@ int64_t latticeSize( int64_t w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
 // the device mostly idle, so a class runs several workers (own context and
 // stream each) while requests are queued
 constexpr int kP2Workers = 6;
+// Jacobi-PCG iterations of the P2 coarse solve (a fixed count: the V-cycle has
+// no host synchronisation and is replayed from a captured HIP graph)
+#ifndef HG_SERVE_P2_COARSE_ITERATIONS
+#define HG_SERVE_P2_COARSE_ITERATIONS 30
+#endif
+constexpr int kP2CoarseIterations = HG_SERVE_P2_COARSE_ITERATIONS;
+
+// HG_SERVE_NO_GRAPHS=1 runs every V-cycle eagerly instead of replaying the
+// captured graph (for kernel tracers that do not follow graph launches)
+bool useGraphs()
+{
+   static const bool on = [] {
+      const char* e = std::getenv( "HG_SERVE_NO_GRAPHS" );
+      return !( e && e[0] == '1' );
+   }();
+   return on;
+}
 
 // device bytes of a class hierarchy (functions of all levels plus operator
 // tables), generous estimate used for admission
@@ -267,7 +286,7 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
          if ( nActive == 0 )
             continue;
          // one V-cycle on every slot (idle slots hold b = 0, x = 0 and stay there)
-         if ( !warmed )
+         if ( !warmed || !useGraphs() )
          {
             gmg.solve( A, X, B, L );
             warmed = true;
@@ -338,6 +357,15 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
       const uint_t         L = uint_t( level );
       P2Function< double > x( "x", st, 0, L ), b( "b", st, 0, L ), t( "t", st, L, L ), t2( "t2", st, L, L ),
           r( "r", st, L, L );
+      using Op = P2ElementwiseBlendingDivKGradOperator;
+      // the solver objects live as long as the worker; the operator, the smoother's
+      // coefficients and the captured V-cycle are per request
+      auto smoother = std::make_shared< ChebyshevSmoother< Op > >( st, 0, L );
+      auto coarse   = std::make_shared< CGSolver< Op > >( st, 0, 0, kP2CoarseIterations, 0.0, 0.0,
+                                                        std::make_shared< InverseDiagonalPreconditioner< Op > >() );
+      coarse->setFixedIterations( true );
+      GeometricMultigridSolver< Op > gmg( st, smoother, coarse, std::make_shared< P2toP2QuadraticRestriction<> >(),
+                                          std::make_shared< P2toP2QuadraticProlongation<> >(), 0, L, 2, 2 );
       for ( ;; )
       {
          std::vector< Request > in;
@@ -348,35 +376,44 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
          const Request&        rq = in[0];
          std::vector< double > a, om, ph;
          coefficientOf( rq.seed, a, om, ph );
-         P2ElementwiseBlendingDivKGradOperator A( st, 0, L, a, om, ph );
+         Op A( st, 0, L, a, om, ph );
          x.interpolate( 0.0, L, All );
          b.interpolate( 0.0, L, All );
          check( ctx->get(), hg_function_interpolate_random( ctx->get(), b.handle(), rq.seed, -1.0, 1.0, int( L ), Inner ),
                 "rhs" );
          check( ctx->get(), hg_function_interpolate_random( ctx->get(), t.handle(), rq.seed + 1, -1.0, 1.0, int( L ), All ),
                 "start vector" );
-         const double rho      = chebyshev::estimateRadius( A, L, 15, t, t2 );
-         auto         smoother = std::make_shared< ChebyshevSmoother< decltype( A ) > >( st, 0, L );
-         smoother->setupCoefficients( 3, rho );
-         auto coarse = std::make_shared< CGSolver< decltype( A ) > >(
-             st, 0, 0, 10000, 1e-14, 1e-300, std::make_shared< InverseDiagonalPreconditioner< decltype( A ) > >() );
-         GeometricMultigridSolver< decltype( A ) > gmg( st, smoother, coarse,
-                                                       std::make_shared< P2toP2QuadraticRestriction<> >(),
-                                                       std::make_shared< P2toP2QuadraticProlongation<> >(), 0, L, 2, 2 );
+         smoother->setupCoefficients( 3, chebyshev::estimateRadius( A, L, 15, t, t2 ) );
          auto resid = [&] {
             A.apply( x, r, L, Inner, Replace );
             r.assign( { 1.0, -1.0 }, { b, r }, L, Inner );
             return std::sqrt( r.dotGlobal( r, L, Inner ) );
          };
-         const double r0 = resid();
-         double       rk = r0;
-         int          k  = 0;
+         const double r0    = resid();
+         double       rk    = r0;
+         int          k     = 0;
+         hg_handle    graph = 0;
          while ( k < 60 && rk > sv.tol * r0 )
          {
-            gmg.solve( A, x, b, L );
+            // the first cycle runs eagerly (it builds the operator's coefficient
+            // tables and inverse diagonals); the second is captured and replayed
+            if ( k == 0 || !useGraphs() )
+               gmg.solve( A, x, b, L );
+            else
+            {
+               if ( !graph )
+               {
+                  check( ctx->get(), hg_graph_begin_capture( ctx->get() ), "capture" );
+                  gmg.solve( A, x, b, L );
+                  check( ctx->get(), hg_graph_end_capture( ctx->get(), &graph ), "capture" );
+               }
+               check( ctx->get(), hg_graph_launch( ctx->get(), graph ), "graph launch" );
+            }
             rk = resid();
             ++k;
          }
+         if ( graph )
+            check( ctx->get(), hg_graph_destroy( ctx->get(), graph ), "graph" );
          sv.finish( rq, r0 > 0 ? rk / r0 : 0.0, x.dotGlobal( x, L, All ), k );
       }
    } catch ( const std::exception& e )

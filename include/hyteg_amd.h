@@ -169,7 +169,7 @@ int hg_function_dot_batched( hg_ctx* ctx, hg_handle a, hg_handle b, int level, i
  * hg_function_lincomb_slots is assign (add = 0) or add (add = 1) of
  * hg_function_assign / hg_function_add with the scalars c_i = signs[i] *
  * (slot[num[i]] / slot[den[i]]) read on the device (a negative index stands
- * for 1); CGSolver uses them to keep alpha and beta on the device
+ * for 1; a zero denominator gives c_i = 0); CGSolver uses them to keep alpha and beta on the device
  * (CGSolver.hpp:149-190 computes them on the host). */
 int hg_slots_alloc( hg_ctx* ctx, int n, int* first );
 int hg_slots_free( hg_ctx* ctx, int first, int n );

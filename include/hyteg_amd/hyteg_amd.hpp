@@ -819,6 +819,15 @@ class CGSolver : public Solver< OperatorType >
       }
    }
    uint_t getIterations() const { return iterations_; }
+   // exactly maxIter iterations with no residual read back (no host
+   // synchronisation, so a surrounding V-cycle can be captured into a HIP graph);
+   // FP64 lattice functions only
+   void setFixedIterations( bool fixed )
+   {
+      if ( fixed && !kDeviceScalars )
+         throw std::invalid_argument( "fixed-iteration CG needs FP64 lattice functions" );
+      fixed_ = fixed;
+   }
 
  private:
    // FP64 lattice functions keep pAp, r.r and r.z in device slots of the context
@@ -846,25 +855,33 @@ class CGSolver : public Solver< OperatorType >
       precondition( A, level );
       p_.assign( { 1.0 }, { z_ }, level, flag_ );
       r_.dotToSlot( z_, level, flag_, rz[cur] );
-      r_.dotToSlot( r_, level, flag_, rr );
-      double rs = 0.0;
-      check( ctx_, hg_slots_read( ctx_, rr, 1, &rs ), "CGSolver" );
-      const double res_start = std::sqrt( rs );
-      iterations_            = 0;
-      if ( res_start < absTol_ )
-         return;
+      double res_start = 0.0;
+      iterations_      = 0;
+      if ( !fixed_ )
+      {
+         r_.dotToSlot( r_, level, flag_, rr );
+         check( ctx_, hg_slots_read( ctx_, rr, 1, &res_start ), "CGSolver" );
+         res_start = std::sqrt( res_start );
+         if ( res_start < absTol_ )
+            return;
+      }
       for ( uint_t i = 0; i < maxIter_; ++i )
       {
          A.apply( p_, ap_, level, flag_, Replace );
          p_.dotToSlot( ap_, level, flag_, pap );
          x.linCombSlots( { 1.0 }, { rz[cur] }, { pap }, { p_ }, level, flag_, true );
          r_.linCombSlots( { -1.0 }, { rz[cur] }, { pap }, { ap_ }, level, flag_, true );
-         r_.dotToSlot( r_, level, flag_, rr );
-         double rsnew = 0.0;
-         check( ctx_, hg_slots_read( ctx_, rr, 1, &rsnew ), "CGSolver" );
-         const double relRes = std::sqrt( rsnew ) / res_start;
-         iterations_         = i;
-         if ( relRes < relTol_ || std::sqrt( rsnew ) < absTol_ )
+         iterations_ = i;
+         if ( !fixed_ )
+         {
+            r_.dotToSlot( r_, level, flag_, rr );
+            double rsnew = 0.0;
+            check( ctx_, hg_slots_read( ctx_, rr, 1, &rsnew ), "CGSolver" );
+            const double relRes = std::sqrt( rsnew ) / res_start;
+            if ( relRes < relTol_ || std::sqrt( rsnew ) < absTol_ )
+               break;
+         }
+         else if ( i + 1 == maxIter_ )
             break;
          z_.interpolate( 0.0, level, All );
          precondition( A, level );
@@ -890,6 +907,7 @@ class CGSolver : public Solver< OperatorType >
    uint_t                                    iterations_ = 0;
    hg_ctx*                                   ctx_        = nullptr;
    int                                       slot0_      = -1;
+   bool                                      fixed_      = false;
 };
 
 // Preconditioner z = D^-1 r with the operator's inverse diagonal
