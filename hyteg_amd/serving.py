@@ -78,7 +78,19 @@ def main():
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--slots", type=int, default=32)
+    ap.add_argument("--kind", choices=list(KIND_NAMES.values()), default=None,
+                    help="instead of the mixed streams: a burst of --requests requests of this kind at t = 0")
+    ap.add_argument("--level", type=int, default=3, help="level of the --kind burst")
     args = ap.parse_args()
+    if args.kind is not None:
+        kind = {v: k for k, v in KIND_NAMES.items()}[args.kind]
+        n = args.requests
+        kinds = np.full(n, kind, dtype=np.int32)
+        out = serve_stream(kinds, np.full(n, args.level, dtype=np.int32), np.arange(1, n + 1, dtype=np.uint64),
+                           np.zeros(n), tol=args.tol, slots=args.slots)
+        cycles = sorted(set(int(c) for c in out["cycles"]))
+        print(json.dumps({"kind": args.kind, "level": args.level, "cycles": cycles, **summary(out, n)}), flush=True)
+        return
     for rate in [float(r) for r in args.rates.split(",")]:
         stream = make_stream(args.requests, rate, seed=0)
         out = serve_stream(*stream, tol=args.tol, slots=args.slots)
