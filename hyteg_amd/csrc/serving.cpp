@@ -119,11 +119,8 @@ constexpr int kP2CoarseIterations = HG_SERVE_P2_COARSE_ITERATIONS;
 // captured graph (for kernel tracers that do not follow graph launches)
 bool useGraphs()
 {
-   static const bool on = [] {
-      const char* e = std::getenv( "HG_SERVE_NO_GRAPHS" );
-      return !( e && e[0] == '1' );
-   }();
-   return on;
+   const char* e = std::getenv( "HG_SERVE_NO_GRAPHS" );
+   return !( e && e[0] == '1' );
 }
 
 // device bytes of a class hierarchy (functions of all levels plus operator

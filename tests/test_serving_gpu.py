@@ -27,3 +27,24 @@ def test_stream_converges_and_results_do_not_depend_on_the_mix(ctx):
             assert one["checksum"][0] == mix["checksum"][i]
         else:
             assert abs(one["checksum"][0] - mix["checksum"][i]) <= 1e-10 * abs(mix["checksum"][i])
+
+
+def test_graph_replayed_cycles_equal_eager_cycles(ctx, monkeypatch):
+    """P2 requests (shell level 2, cube level 3, six at once on six workers): the
+    V-cycle captured once per request and replayed (14 times, with the residual
+    of the eager code between replays) gives the cycle counts and, up to the
+    order of floating-point atomic sums, the results of eager cycles; so do P1
+    requests (one capture per worker, bit for bit)"""
+    kinds = np.array([2, 2, 2, 3, 3, 3, 0, 1], dtype=np.int32)
+    levels = np.array([2, 2, 2, 3, 3, 3, 4, 3], dtype=np.int32)
+    seeds = np.arange(7, 7 + len(kinds), dtype=np.uint64)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("HG_SERVE_NO_GRAPHS", mode)
+        print(f"burst, graphs {'off' if mode == '1' else 'on'}", flush=True)
+        out[mode] = serving.serve_stream(kinds, levels, seeds, np.zeros(len(kinds)), tol=1e-8, slots=8)
+        assert (out[mode]["rel_residual"] <= 1e-8).all(), out[mode]["rel_residual"]
+    assert np.array_equal(out["0"]["cycles"], out["1"]["cycles"])
+    p1 = kinds <= 1
+    assert np.array_equal(out["0"]["checksum"][p1], out["1"]["checksum"][p1])
+    assert np.all(np.abs(out["0"]["checksum"] - out["1"]["checksum"]) <= 1e-10 * np.abs(out["1"]["checksum"]))
