@@ -94,6 +94,7 @@ SIGNATURES = {
     "hg_function_copy_vector": [_p, _i64, _i, _i64, _i, _i],
     "hg_function_dot_batched": [_p, _i64, _i64, _i, _i, _p],
     "hg_function_scale_batched": [_p, _i64, _p, _i, _i],
+    "hg_function_chebyshev_step": [_p, _i64, _i64, _i64, _i64, _i64, _d, _i, _i],
     "hg_slots_alloc": [_p, _i, _p],
     "hg_slots_free": [_p, _i, _i],
     "hg_slots_read": [_p, _i, _i, _p],

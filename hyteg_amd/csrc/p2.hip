@@ -757,6 +757,47 @@ __global__ void kMult( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bn
 }
 
 
+// One step of ChebyshevSmoother::solve (ChebyshevSmoother.hpp:186-231) in one
+// pass over the DoFs: r = b - s on the first step (s = A x) and r = s after it
+// (s = A t), then t = invDiag * r on the flagged DoFs and 0 on all others, and
+// x += c t on the flagged DoFs.  Each intermediate is rounded to the storage
+// type where the separate assign / setToZero / multElementwise / add kernels
+// store it, with the same operations (no contraction), so the result is theirs
+// bit for bit.
+template < class T >
+__global__ void kChebyshevStep( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, int first,
+                                double c, const T* __restrict__ b, const T* __restrict__ s,
+                                const T* __restrict__ invDiag, int lv, T* __restrict__ t, T* __restrict__ x )
+{
+#pragma clang fp contract( off )
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv;
+   if ( i >= ncell + nb )
+      return;
+   const int64_t p  = tid;
+   const bool    on = i < ncell ? ( flag & kInner ) != 0 : ( bndFlag[i - ncell] & flag ) != 0;
+   if ( !on )
+   {
+      t[p] = fromAcc< T >( A( 0 ) );
+      return;
+   }
+   T r = s[p];
+   if ( first )
+   {
+      A v = A( 0 );
+      v += A( 1.0 ) * toAcc( b[p] );
+      v += A( -1.0 ) * toAcc( r );
+      r = fromAcc< T >( v );
+   }
+   const T tv = fromAcc< T >( toAcc( invDiag[p] ) * toAcc( r ) );
+   t[p]       = tv;
+   A v        = A( 0 );
+   v += A( c ) * toAcc( tv );
+   v += toAcc( x[p] );
+   x[p] = fromAcc< T >( v );
+}
+
 } // namespace hg
 
 using namespace hg;
@@ -1008,6 +1049,35 @@ int hg_function_mult( hg_ctx* h, hg_handle dst, hg_handle a, hg_handle b, int le
                                                                          (const T*) fa.ptr( level ),
                                                                          (const T*) fb.ptr( level ), d.lv,
                                                                          (T*) d.ptr( level ) );
+      } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+int hg_function_chebyshev_step( hg_ctx* h, hg_handle x, hg_handle t, hg_handle s, hg_handle b, hg_handle invDiag,
+                                double c, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      Function& fx = ctx->function( x );
+      Function& ft = ctx->function( t );
+      Function& fs = ctx->function( s );
+      Function& fd = ctx->function( invDiag );
+      Function* fb = b ? &ctx->function( b ) : nullptr;
+      for ( Function* f : { &fx, &ft, &fs, &fd, fb } )
+         if ( f && ( f->st != fx.st || f->dtype != fx.dtype || f->off != fx.off || f->lv != fx.lv || level < f->minL ||
+                     level > f->maxL ) )
+            throw Error( HG_ERR_ARG, "chebyshev_step needs functions of one storage, space, value type and vector count" );
+      if ( &fx == &ft || &ft == &fs || &fx == &fs )
+         throw Error( HG_ERR_ARG, "chebyshev_step needs distinct x, t and s" );
+      Lattice&      l = fx.lat( ctx, level );
+      const int64_t n = l.nc * l.sc + l.nb;
+      if ( n == 0 )
+         return;
+      dispatchType( fx.dtype, [&]( auto tag ) {
+         using T = decltype( tag );
+         kChebyshevStep< T ><<< gridFor( n << fx.lv ), kBlock, 0, ctx->stream >>>(
+             l.nc * l.sc, l.nb, l.bndFlag.p, flag, fb ? 1 : 0, c, fb ? (const T*) fb->ptr( level ) : nullptr,
+             (const T*) fs.ptr( level ), (const T*) fd.ptr( level ), fx.lv, (T*) ft.ptr( level ), (T*) fx.ptr( level ) );
       } );
       HG_CHECK_HIP( hipGetLastError() );
    } );

@@ -179,6 +179,13 @@ int hg_function_lincomb_slots( hg_ctx* ctx, hg_handle dst, int nsrc, const doubl
                                const hg_handle* srcs, int level, int flag, int add );
 /* vector k of fn scaled by scales[k] on flagged DoFs */
 int hg_function_scale_batched( hg_ctx* ctx, hg_handle fn, const double* scales, int level, int flag );
+/* one step of ChebyshevSmoother::solve (ChebyshevSmoother.hpp:186-231) fused
+ * into one pass: with r = b - s (b != 0, the first step, s = A x) or r = s (b = 0,
+ * s = A t), t = invDiag * r on the flagged DoFs and 0 on all others, then
+ * x += c * t on the flagged DoFs; bit-identical to the assign / setToZero /
+ * multElementwise / add sequence it replaces */
+int hg_function_chebyshev_step( hg_ctx* ctx, hg_handle x, hg_handle t, hg_handle s, hg_handle b, hg_handle invDiag,
+                                double c, int level, int flag );
 /* multElementwise({a, b}, level, flag): dst = a * b (VertexDoFFunction.cpp:1408) */
 int hg_function_mult( hg_ctx* ctx, hg_handle dst, hg_handle a, hg_handle b, int level, int flag );
 /* copyFrom<otherT>(other, level): cast copy between storage types (VertexDoFFunction.hpp:151) */

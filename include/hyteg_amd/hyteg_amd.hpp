@@ -275,6 +275,15 @@ class LatticeFunction
       check( ctx_, hg_function_dot( ctx_, h_, other.h_, int( level ), dofs( flag ), &r ), "dotGlobal" );
       return ValueType( r );
    }
+   // one fused Chebyshev step on this function (x): t = invDiag * (b - s) (b given)
+   // or invDiag * s on flagged DoFs, 0 elsewhere; x += c t on flagged DoFs
+   void chebyshevStep( const LatticeFunction& t, const LatticeFunction& s, const LatticeFunction* b,
+                       const LatticeFunction& invDiag, double c, uint_t level, DoFType flag ) const
+   {
+      check( ctx_,
+             hg_function_chebyshev_step( ctx_, h_, t.h_, s.h_, b ? b->h_ : 0, invDiag.h_, c, int( level ), dofs( flag ) ),
+             "chebyshevStep" );
+   }
    // dotGlobal into a device scalar slot of the context (no wait for the result)
    void dotToSlot( const LatticeFunction& other, uint_t level, DoFType flag, int slot ) const
    {
@@ -937,12 +946,21 @@ class InverseDiagonalPreconditioner : public Solver< OperatorType >
 // With a projection (ChebyshevSmoother<Op, P2ProjectNormalOperator>), every
 // preconditioned correction D^-1 r is projected on the projectionFlag DoFs before
 // it is added to x (ChebyshevSmoother.hpp:186-231).
+template < class F >
+struct IsLatticeFunction : std::false_type
+{};
+template < class T, int Space >
+struct IsLatticeFunction< LatticeFunction< T, Space > > : std::true_type
+{};
+
 template < class OperatorType, class ProjectionOperatorType = NoOperator >
 class ChebyshevSmoother : public Solver< OperatorType >
 {
  public:
    using FunctionType = typename OperatorType::srcType;
    using ValueType    = typename FunctionType::valueType;
+   // scalar lattice functions run each step as one fused pass
+   static constexpr bool kFusedSteps = IsLatticeFunction< FunctionType >::value;
 
    ChebyshevSmoother( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
    : minLevel_( minLevel )
@@ -1011,6 +1029,20 @@ class ChebyshevSmoother : public Solver< OperatorType >
       if ( c.empty() )
          throw std::runtime_error( "Chebyshev coefficients are not set up" );
       A.apply( x, tmp2_, level, flag_ );
+      if constexpr ( kFusedSteps )
+      {
+         // without a projection each step is one pass over the DoFs (same values)
+         if ( !projection_ )
+         {
+            x.chebyshevStep( tmp1_, tmp2_, &b, invDiag, double( ValueType( c[0] ) ), level, flag_ );
+            for ( size_t k = 1; k < c.size(); ++k )
+            {
+               A.apply( tmp1_, tmp2_, level, flag_ );
+               x.chebyshevStep( tmp1_, tmp2_, nullptr, invDiag, double( ValueType( c[k] ) ), level, flag_ );
+            }
+            return;
+         }
+      }
       tmp2_.assign( { 1.0, -1.0 }, { b, tmp2_ }, level, flag_ );
       tmp1_.setToZero( level );
       tmp1_.multElementwise( { invDiag, tmp2_ }, level, flag_ );
