@@ -354,3 +354,48 @@ def test_block_factorisation_fgmres_converges(fs_shell):
     assert its < 40 and hist[-1] <= 1e-6 * hist[0], (its, hist[-1] / hist[0])
     ru, rp = po.stokes_residual(op, u, p, bu, np.zeros(op.n1), po.GMG_FLAG)
     assert abs(np.sqrt(ru @ ru + rp @ rp) / hist[-1] - 1) < 1e-3
+
+
+def _minres_brute_force(A, P, b, k):
+    """x_k of preconditioned MINRES from x0 = 0: the minimiser of ||b - A x||_P
+    (the P-weighted norm, P SPD) over the Krylov space K_k(P A, P b)"""
+    V = [P @ b]
+    for _ in range(1, k):
+        V.append(P @ (A @ V[-1]))
+    V, _ = np.linalg.qr(np.array(V).T)
+    L = np.linalg.cholesky(P)  # ||r||_P = ||L^T r||
+    y = np.linalg.lstsq(L.T @ (A @ V), L.T @ b, rcond=None)[0]
+    return V @ y
+
+
+def test_minres_iterates_are_the_krylov_minimum():
+    """MinResSolver's iterate after k steps, on a small symmetric indefinite
+    system, is the brute-force minimiser of the residual over the Krylov space
+    (in the preconditioner's norm for an SPD diagonal preconditioner), for every
+    k; run to convergence it reaches the direct solution"""
+    rng = np.random.default_rng(12)
+    n = 20
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    eig = np.concatenate([rng.uniform(1, 4, 12), -rng.uniform(0.5, 3, 8)])
+    A = Q @ np.diag(eig) @ Q.T
+    b = rng.normal(size=n)
+    m = np.ones(n, bool)
+    for P in (np.eye(n), np.diag(rng.uniform(0.3, 2.0, n))):
+        for k in range(1, 11):
+            x = np.zeros(n)
+            po.minres_solve(lambda v: A @ v, lambda v: P @ v, x, b, m, k, 0.0)
+            ref = _minres_brute_force(A, P, b, k)
+            assert np.abs(x - ref).max() < 1e-9 * np.abs(ref).max(), k
+        x = np.zeros(n)
+        po.minres_solve(lambda v: A @ v, lambda v: P @ v, x, b, m, 200, 1e-14)
+        assert np.abs(x - np.linalg.solve(A, b)).max() < 1e-9
+    # a planted slip in a copy of the oracle (the sign of the second-previous
+    # search direction) fails the pin
+    import inspect
+    src = inspect.getsource(po.minres_solve)
+    assert src.count("- (a3 / a1) * wm[m]") == 1
+    ns = dict(vars(po))
+    exec(src.replace("- (a3 / a1) * wm[m]", "+ (a3 / a1) * wm[m]"), ns)
+    x = np.zeros(n)
+    ns["minres_solve"](lambda v: A @ v, lambda v: v, x, b, m, 6, 0.0)
+    assert np.abs(x - _minres_brute_force(A, np.eye(n), b, 6)).max() > 1e-6
