@@ -399,3 +399,42 @@ def test_minres_iterates_are_the_krylov_minimum():
     x = np.zeros(n)
     ns["minres_solve"](lambda v: A @ v, lambda v: v, x, b, m, 6, 0.0)
     assert np.abs(x - _minres_brute_force(A, np.eye(n), b, 6)).max() > 1e-6
+
+
+def _uzawa_gmg_error(ns, mesh, L):
+    """relative residual after 30 cycles and the errors against a chosen discrete
+    solution (u* zero on Dirichlet DoFs, p* of zero mean; pressure compared up to
+    its constant) of the Uzawa GMG built from the namespace ns"""
+    import warnings
+    g = ns["StokesGMG"](mesh, 0, L)
+    op = g.ops[L]
+    inner = mesh.flags(L + 1) == po.INNER
+    rng = np.random.default_rng(13)
+    us = np.concatenate([np.where(inner, rng.uniform(-1, 1, len(inner)), 0.0) for _ in range(3)])
+    ps = rng.uniform(-1, 1, op.n1)
+    ps -= ps.mean()
+    bu, bp = np.zeros_like(us), np.zeros_like(ps)
+    op.apply(us, ps, bu, bp, po.GMG_FLAG, 7)
+    u, p = np.zeros_like(us), np.zeros_like(ps)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        _, hist = g.solve(u, p, bu, bp, cycles=30, tol=1e-11)
+        return hist[-1] / hist[0], np.abs(u - us).max(), np.abs(p - p.mean() - ps).max()
+
+
+def test_uzawa_gmg_reaches_a_chosen_discrete_solution():
+    """the Uzawa-smoothed Stokes GMG (UzawaSmoother, StokesFSGMGUzawaSolverTemplate
+    layout) on the shell, driven by the right-hand side of a chosen discrete
+    solution, converges to that solution; a copy of the oracle whose pressure
+    update has the wrong sign does not"""
+    import inspect
+    mesh = po.Mesh.shell(ntan=3, nrad=2, rmin=0.5, rmax=1.0)
+    rel, eu, ep = _uzawa_gmg_error(vars(po), mesh, 1)
+    assert rel < 1e-10 and eu < 1e-7 and ep < 1e-5, (rel, eu, ep)
+    ns = dict(vars(po))
+    src = inspect.getsource(po.uzawa_smooth)
+    assert src.count("p += pspg_inv * rp") == 1
+    exec(src.replace("p += pspg_inv * rp", "p -= pspg_inv * rp"), ns)
+    exec(inspect.getsource(po.StokesGMG), ns)
+    rel, eu, ep = _uzawa_gmg_error(ns, mesh, 1)
+    assert not rel < 1e-3, rel
