@@ -107,7 +107,10 @@ int64_t latticeSize( int64_t w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
 // P2 requests are solved one at a time per worker; their small kernels leave
 // the device mostly idle, so a class runs several workers (own context and
 // stream each) while requests are queued
-constexpr int kP2Workers = 6;
+#ifndef HG_SERVE_P2_WORKERS
+#define HG_SERVE_P2_WORKERS 6
+#endif
+constexpr int kP2Workers = HG_SERVE_P2_WORKERS;
 // Jacobi-PCG iterations of the P2 coarse solve (a fixed count: the V-cycle has
 // no host synchronisation and is replayed from a captured HIP graph)
 #ifndef HG_SERVE_P2_COARSE_ITERATIONS
@@ -125,8 +128,25 @@ bool useGraphs()
 
 // device bytes of a class hierarchy (functions of all levels plus operator
 // tables), generous estimate used for admission
+// batch width of a P1 class: at most `slots`, halved while a sweep over the
+// finest level would touch more than kBatchPoints values, so a V-cycle of a
+// large class stays short (a request waits for whole cycles of its batch, and
+// every slot is swept whether it holds a request or not)
+constexpr int64_t kBatchPoints = int64_t( 1 ) << 27;
+int batchSlots( int kind, int level, int slots )
+{
+   const int64_t cells = kind == P1Cube ? 6 : 1920;
+   const int64_t pts   = cells * latticeSize( ( int64_t( 1 ) << level ) + 1 );
+   int           S     = slots;
+   while ( S > 1 && pts * S > kBatchPoints )
+      S /= 2;
+   return S;
+}
+
 int64_t footprint( int kind, int level, int slots )
 {
+   if ( kind == P1Cube || kind == P1Shell )
+      slots = batchSlots( kind, level, slots );
    const int64_t cells = ( kind == P1Cube || kind == P2Cube ) ? 6 : 1920;
    const int     L     = ( kind == P2Shell || kind == P2Cube ) ? level + 1 : level;
    int64_t       pts   = 0;
@@ -230,7 +250,7 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
                 "shell" );
          st = std::make_shared< PrimitiveStorage >( ctx, a );
       }
-      const int    S = sv.slots;
+      const int    S = batchSlots( kind, level, sv.slots );
       const uint_t L = uint_t( level );
       P1ConstantLaplaceOperator A( st, 0, L );
       P1Function< double >      X( "x", st, 0, L, S ), B( "b", st, 0, L, S ), R( "r", st, L, L, S );
