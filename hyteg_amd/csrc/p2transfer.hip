@@ -18,6 +18,7 @@
 // a fine point shared by several macro-cells contributes 1/(number of cells
 // sharing its primitive) from each, and restricted values of coarse DoFs on
 // macro vertices/edges/faces are the sum over the incident cells.
+#include <algorithm>
 #include <mutex>
 #include <array>
 #include <vector>
@@ -233,9 +234,29 @@ __global__ void kP2ProlongCell( const int32_t* __restrict__ rows, int64_t nrows,
    const int64_t y   = rows[2 * r];
    const int64_t z   = rows[2 * r + 1];
    const int64_t out = c * scf + tetRowStart( wf, y, z );
+   const int64_t by = 2 * ( y >> 2 ), bz = 2 * ( z >> 2 );
+   // the stencils of one fine row reach the coarse rows (by + 0..2, bz + 0..2):
+   // their starts once per block instead of a lattice index per entry
+   __shared__ int64_t rs[9];
+   if ( threadIdx.x < 9 )
+   {
+      const int64_t cy = by + threadIdx.x % 3, cz = bz + threadIdx.x / 3;
+      rs[threadIdx.x]  = cy + cz <= wc - 1 ? c * scc + tetRowStart( wc, cy, cz ) : -1;
+   }
+   __syncthreads();
    for ( int64_t x = 1 + threadIdx.x; x + y + z < mmf; x += blockDim.x )
    {
-      A v = p2Interp( coarse, c, scc, wc, x, y, z );
+      const int     cls = int( x & 3 ) + 4 * int( y & 3 ) + 16 * int( z & 3 );
+      const int64_t bx  = 2 * ( x >> 2 );
+      A             v   = A( 0 );
+      for ( int e = 0; e < cPN[cls]; ++e )
+      {
+         const int     oy = cPOff[cls][e][1], oz = cPOff[cls][e][2];
+         const int64_t cx = bx + cPOff[cls][e][0], cy = by + oy, cz = bz + oz;
+         if ( cx < 0 || cy < 0 || cz < 0 || cx + cy + cz > wc - 1 )
+            continue;
+         v += A( cPW[cls][e] ) * toAcc( coarse[rs[oy + 3 * oz] + cx] );
+      }
       if ( add )
          v += toAcc( fine[out + x] );
       fine[out + x] = fromAcc< T >( v );
@@ -279,19 +300,31 @@ __global__ void kP2RestrictCell( const int32_t* __restrict__ rows, int64_t nrows
    const int64_t z   = rows[2 * r + 1];
    const int64_t out = c * scc + tetRowStart( wc, y, z );
    const int64_t mmf = wf - 1;
-   for ( int64_t x = 1 + threadIdx.x; x + y + z < mmc; x += blockDim.x )
+   // the fine rows (2y + -3..3, 2z + -3..3) the stencils of this coarse row reach
+   __shared__ int64_t rs[49];
+   for ( int i = threadIdx.x; i < 49; i += blockDim.x )
    {
-      const int q = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
-      A         v = A( 0 );
-      for ( int e = 0; e < cRN[q]; ++e )
-      {
-         const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + cROff[q][e][1], fz = 2 * z + cROff[q][e][2];
-         if ( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf )
-            continue;
-         v += A( cRW[q][e] ) * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
-      }
-      coarse[out + x] = fromAcc< T >( v );
+      const int64_t fy = 2 * y + i % 7 - 3, fz = 2 * z + i / 7 - 3;
+      rs[i]            = fy >= 0 && fz >= 0 && fy + fz <= mmf ? c * scf + tetRowStart( wf, fy, fz ) : -1;
    }
+   __syncthreads();
+   // odd x, then even x: the stencil class (and length: 125 entries for a
+   // vertex DoF, 19 or 27 otherwise) is uniform over a wavefront
+   for ( int par = 1; par >= 0; --par )
+      for ( int64_t x = 2 - par + 2 * int64_t( threadIdx.x ); x + y + z < mmc; x += 2 * int64_t( blockDim.x ) )
+      {
+         const int q = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
+         A         v = A( 0 );
+         for ( int e = 0; e < cRN[q]; ++e )
+         {
+            const int     oy = cROff[q][e][1], oz = cROff[q][e][2];
+            const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + oy, fz = 2 * z + oz;
+            if ( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf )
+               continue;
+            v += A( cRW[q][e] ) * toAcc( fine[rs[( oy + 3 ) + 7 * ( oz + 3 )] + fx] );
+         }
+         coarse[out + x] = fromAcc< T >( v );
+      }
 }
 
 // one wavefront per owned boundary DoF: the lanes split the stencil entries of
@@ -330,6 +363,32 @@ __global__ __launch_bounds__( 64 * kBndWaves ) void kP2RestrictBnd( BndGeom gc, 
       coarse[nc * gc.sc + b] = fromAcc< T >( s );
 }
 
+// owned face DoFs (two incident cells each): one thread per DoF
+template < class T >
+__global__ void kP2RestrictBndFace( BndGeom gc, int64_t nbc, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
+                                    const double* __restrict__ clsInvN, int64_t scf, int64_t wf, int64_t mmf,
+                                    const T* __restrict__ fine, T* __restrict__ coarse )
+{
+   using A         = typename Acc< T >::type;
+   const int64_t b = gc.foff + int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( b >= nbc || !( bndFlag[b] & flag ) )
+      return;
+   A s = A( 0 );
+   forEachIncident( gc, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int ) {
+      const int q = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
+      for ( int e = 0; e < cRN[q]; ++e )
+      {
+         const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + cROff[q][e][1], fz = 2 * z + cROff[q][e][2];
+         if ( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf )
+            continue;
+         const int cl = pointClass( mmf, fx, fy, fz );
+         const A   w  = A( cRW[q][e] ) * ( cl < 0 ? A( 1 ) : A( clsInvN[14 * c + cl] ) );
+         s += w * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
+      }
+   } );
+   coarse[nc * gc.sc + b] = fromAcc< T >( s );
+}
+
 static constexpr int kRowBlockT = 64;
 
 void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
@@ -353,9 +412,16 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
       }
       if ( lc.nb > 0 )
       {
-         kP2RestrictBnd< T ><<< gridFor( lc.nb, kBndWaves ), 64 * kBndWaves, 0, ctx->stream >>>(
-             BndGeom::of( st, lc ), lc.nb, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp,
-             (T*) cp );
+         // macro-vertex and -edge DoFs (many incident cells): a wavefront each;
+         // face DoFs: a thread each
+         const BndGeom gc = BndGeom::of( st, lc );
+         const int64_t nve = std::min( gc.foff, lc.nb );
+         if ( nve > 0 )
+            kP2RestrictBnd< T ><<< gridFor( nve, kBndWaves ), 64 * kBndWaves, 0, ctx->stream >>>(
+                gc, nve, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp, (T*) cp );
+         if ( lc.nb > nve )
+            kP2RestrictBndFace< T ><<< gridFor( lc.nb - nve ), kBlock, 0, ctx->stream >>>(
+                gc, lc.nb, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp, (T*) cp );
          HG_CHECK_HIP( hipGetLastError() );
       }
    } );
