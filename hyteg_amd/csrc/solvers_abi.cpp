@@ -49,9 +49,59 @@ int guarded( hg_ctx* ctx, F&& f )
    }
 }
 
+// Replays `body` (a V-cycle: kernels only) from a HIP graph captured on its
+// first use when the context allows graphs; runs it directly otherwise.
+class GraphedCycle
+{
+ public:
+   explicit GraphedCycle( hg_ctx* ctx )
+   : ctx_( ctx )
+   {
+      int en = 0;
+      check( ctx_, hg_ctx_graphs_enabled( ctx_, &en ), "graphs" );
+      enabled_ = en != 0;
+   }
+   ~GraphedCycle()
+   {
+      if ( graph_ )
+         (void) hg_graph_destroy( ctx_, graph_ );
+   }
+   bool enabled() const { return enabled_; }
+   template < class F >
+   void run( F&& body )
+   {
+      if ( !enabled_ )
+      {
+         body();
+         return;
+      }
+      if ( !graph_ )
+      {
+         check( ctx_, hg_graph_begin_capture( ctx_ ), "begin capture" );
+         try
+         {
+            body();
+         } catch ( ... )
+         {
+            hg_handle dummy = 0;
+            (void) hg_graph_end_capture( ctx_, &dummy );
+            throw;
+         }
+         check( ctx_, hg_graph_end_capture( ctx_, &graph_ ), "end capture" );
+      }
+      check( ctx_, hg_graph_launch( ctx_, graph_ ), "graph launch" );
+   }
+
+ private:
+   hg_ctx*   ctx_;
+   hg_handle graph_   = 0;
+   bool      enabled_ = true;
+};
+
 // V-cycles of `gmg` until the Inner residual drops by `tol` (0: run all cycles)
 template < class Op, class F, class G >
-int cycle( const Op& A, G& gmg, const F& x, const F& b, const F& r, uint_t level, int cycles, double tol, double* hist )
+int cycle( const Op& A, G& gmg, const F& x, const F& b, const F& r, uint_t level, int cycles, double tol, double* hist,
+           GraphedCycle* graph = nullptr )
 {
    const double r0 = residualNorm( A, x, b, r, level );
    if ( hist )
@@ -59,7 +109,12 @@ int cycle( const Op& A, G& gmg, const F& x, const F& b, const F& r, uint_t level
    int k = 0;
    for ( ; k < cycles; ++k )
    {
-      gmg.solve( A, x, b, level );
+      // with a graph: the first cycle runs eagerly (it builds the per-level data),
+      // the later ones replay the cycle captured at the second
+      if ( graph && k > 0 )
+         graph->run( [&] { gmg.solve( A, x, b, level ); } );
+      else
+         gmg.solve( A, x, b, level );
       const double rk = residualNorm( A, x, b, r, level );
       if ( hist )
          hist[k + 1] = rk;
@@ -134,53 +189,6 @@ int iterativeRefinement( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, in
    }
 }
 
-// Replays `body` (a V-cycle: kernels only) from a HIP graph captured on its
-// first use when the context allows graphs; runs it directly otherwise.
-class GraphedCycle
-{
- public:
-   explicit GraphedCycle( hg_ctx* ctx )
-   : ctx_( ctx )
-   {
-      int en = 0;
-      check( ctx_, hg_ctx_graphs_enabled( ctx_, &en ), "graphs" );
-      enabled_ = en != 0;
-   }
-   ~GraphedCycle()
-   {
-      if ( graph_ )
-         (void) hg_graph_destroy( ctx_, graph_ );
-   }
-   template < class F >
-   void run( F&& body )
-   {
-      if ( !enabled_ )
-      {
-         body();
-         return;
-      }
-      if ( !graph_ )
-      {
-         check( ctx_, hg_graph_begin_capture( ctx_ ), "begin capture" );
-         try
-         {
-            body();
-         } catch ( ... )
-         {
-            hg_handle dummy = 0;
-            (void) hg_graph_end_capture( ctx_, &dummy );
-            throw;
-         }
-         check( ctx_, hg_graph_end_capture( ctx_, &graph_ ), "end capture" );
-      }
-      check( ctx_, hg_graph_launch( ctx_, graph_ ), "graph launch" );
-   }
-
- private:
-   hg_ctx*   ctx_;
-   hg_handle graph_   = 0;
-   bool      enabled_ = true;
-};
 
 // ---- interleaved multi-vector P1 solves -------------------------------------
 // per-vector Inner residual norms of b - A x (r is scratch)
@@ -423,12 +431,28 @@ int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int mi
       auto smoother = std::make_shared< GaussSeidelSmoother< P1ConstantLaplaceOperator > >();
       auto coarse   = std::make_shared< CGSolver< P1ConstantLaplaceOperator > >( storage, uint_t( minLevel ), uint_t( minLevel ),
                                                                                uint_t( coarseMaxIter ), coarseTol, coarseTol );
-      auto R        = std::make_shared< P1toP1LinearRestriction<> >();
-      auto P        = std::make_shared< P1toP1LinearProlongation<> >();
+      GraphedCycle graph( ctx );
+      if ( graph.enabled() )
+      {
+         // a V-cycle without host synchronisation, replayed from a HIP graph: the
+         // coarse CG runs a fixed count, the number of free coarse DoFs (the
+         // bound of CG in exact arithmetic), capped by coarseMaxIter
+         const DoFType        freeDoFs = Inner | NeumannBoundary | FreeslipBoundary;
+         P1Function< double > ones( "free", storage, uint_t( minLevel ), uint_t( minLevel ) );
+         ones.interpolate( 1.0, uint_t( minLevel ), freeDoFs );
+         const double nfree = ones.dotGlobal( ones, uint_t( minLevel ), freeDoFs );
+         coarse             = std::make_shared< CGSolver< P1ConstantLaplaceOperator > >(
+             storage, uint_t( minLevel ), uint_t( minLevel ), uint_t( std::min( double( coarseMaxIter ), nfree ) ), 0.0,
+             0.0 );
+         coarse->setFixedIterations( true );
+      }
+      auto R = std::make_shared< P1toP1LinearRestriction<> >();
+      auto P = std::make_shared< P1toP1LinearProlongation<> >();
       GeometricMultigridSolver< P1ConstantLaplaceOperator > gmg( storage, smoother, coarse, R, P, uint_t( minLevel ),
                                                                 uint_t( maxLevel ), uint_t( pre ), uint_t( post ) );
       P1Function< double > r( "residual", storage, uint_t( maxLevel ), uint_t( maxLevel ) );
-      const int            k = cycle( *A, gmg, *X, *B, r, uint_t( maxLevel ), cycles, tol, res_hist );
+      const int k = cycle( *A, gmg, *X, *B, r, uint_t( maxLevel ), cycles, tol, res_hist,
+                           graph.enabled() ? &graph : nullptr );
       if ( cycles_done )
          *cycles_done = k;
    } );
