@@ -269,7 +269,10 @@ int hg_p1_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int lev
 /* GeometricMultigridSolver(storage, GaussSeidelSmoother, CGSolver(minLevel), P1toP1LinearRestriction,
  * P1toP1LinearProlongation, minLevel, maxLevel, pre, post).solve(A, x, b, maxLevel), called
  * `cycles` times or until ||b - Ax||_Inner <= tol * ||b - Ax0||_Inner (tol > 0).
- * res_hist (cycles+1 entries, may be NULL) receives the residual norms.
+ * res_hist (cycles+1 entries, may be NULL) receives the residual norms.  When
+ * the context allows graphs (hg_ctx_set_graphs, the default) the cycles after the
+ * first are replayed from a HIP graph and the coarse CG runs a fixed count,
+ * min(coarseMaxIter, number of free coarse DoFs), instead of stopping at coarseTol.
  * (solvers/GeometricMultigridSolver.hpp:126-308, solvers/GaussSeidelSmoother.hpp:31-55) */
 int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre, int post,
                      int cycles, double tol, int coarseMaxIter, double coarseTol, double* res_hist, int* cycles_done );
