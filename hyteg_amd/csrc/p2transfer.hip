@@ -315,13 +315,24 @@ __global__ void kP2RestrictCell( const int32_t* __restrict__ rows, int64_t nrows
       {
          const int q = int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 );
          A         v = A( 0 );
-         for ( int e = 0; e < cRN[q]; ++e )
+         // eight gathers in flight, then their sum in entry order
+         for ( int e0 = 0; e0 < cRN[q]; e0 += 8 )
          {
-            const int     oy = cROff[q][e][1], oz = cROff[q][e][2];
-            const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + oy, fz = 2 * z + oz;
-            if ( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf )
-               continue;
-            v += A( cRW[q][e] ) * toAcc( fine[rs[( oy + 3 ) + 7 * ( oz + 3 )] + fx] );
+            A    buf[8];
+            bool ok[8];
+#pragma unroll
+            for ( int j = 0; j < 8; ++j )
+            {
+               const int     e  = e0 + j;
+               const int     oy = cROff[q][e][1], oz = cROff[q][e][2];
+               const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + oy, fz = 2 * z + oz;
+               ok[j]  = e < cRN[q] && !( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf );
+               buf[j] = ok[j] ? toAcc( fine[rs[( oy + 3 ) + 7 * ( oz + 3 )] + fx] ) : A( 0 );
+            }
+#pragma unroll
+            for ( int j = 0; j < 8; ++j )
+               if ( ok[j] )
+                  v += A( cRW[q][e0 + j] ) * buf[j];
          }
          coarse[out + x] = fromAcc< T >( v );
       }
