@@ -113,6 +113,17 @@ SIGNATURES = {
     "hg_p1_prolongate": [_p, _i64, _i, _i, _i],
     "hg_p1_cg_solve": [_p, _i64, _i64, _i64, _i, _i, _d, _d, _p],
     "hg_p1_gmg_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _d, _i, _d, _p, _p],
+    "hg_p1_smooth_sor_backwards": [_p, _i64, _i64, _i64, _d, _i, _i],
+    "hg_p1_sor_colours": [_p, _i64, _i, _p, _p],
+    "hg_p1_inverse_diagonal": [_p, _i64, _i, _i64],
+    "hg_p1_smooth_jac": [_p, _i64, _i64, _i64, _i64, _d, _i, _i],
+    "hg_p1_gemv": [_p, _i64, _d, _i64, _d, _i64, _i, _i],
+    "hg_p2_gemv": [_p, _i64, _d, _i64, _d, _i64, _i, _i],
+    "hg_p2_smooth_jac": [_p, _i64, _i64, _i64, _i64, _d, _i, _i],
+    "hg_p2_const_smooth_sor_backwards": [_p, _i64, _i64, _i64, _i, _i, _d],
+    "hg_p2_const_sor_colours": [_p, _i64, _i, _p, _p],
+    "hg_function_lincomb_slots_gated": [_p, _i64, _i, _p, _p, _p, _p, _i, _i, _i, _i],
+    "hg_slots_cg_stop": [_p, _i, _i, _d, _d, _i, _i],
 }
 
 _LIB = None

@@ -274,6 +274,18 @@ class P2DivKGradOperator:
     def inverse_diagonal(self, level, dst):
         self.ctx.check(lib().hg_p2_inverse_diagonal(self.ctx.h, self.h, level, dst.h), "inverse diagonal")
 
+    def gemv(self, alpha, src, beta, dst, level, flag):
+        """dst = alpha A src + beta dst on DoFs of type flag"""
+        self.ctx.check(lib().hg_p2_gemv(self.ctx.h, self.h, alpha, src.h, beta, dst.h, level, flag), "gemv")
+
+    def apply_scaled(self, alpha, src, dst, level, flag, update=REPLACE):
+        self.gemv(alpha, src, 0.0 if update == REPLACE else 1.0, dst, level, flag)
+
+    def smooth_jac(self, dst, rhs, src, relax, level, flag):
+        """dst = src + relax D^-1 (rhs - A src)"""
+        self.ctx.check(lib().hg_p2_smooth_jac(self.ctx.h, self.h, dst.h, rhs.h, src.h, relax, level, flag),
+                       "smooth_jac")
+
     def store_coefficients(self, enable=True):
         """keep k at the quadrature points (default) or evaluate it in every apply"""
         self.ctx.check(lib().hg_p2_store_coefficients(self.ctx.h, self.h, int(enable)), "store_coefficients")
@@ -296,6 +308,36 @@ class P1ConstantLaplaceOperator:
 
     def smooth_gs(self, x, b, level, flag):
         self.smooth_sor(x, b, 1.0, level, flag)
+
+    def smooth_sor_backwards(self, x, b, relax, level, flag):
+        self.ctx.check(lib().hg_p1_smooth_sor_backwards(self.ctx.h, self.h, x.h, b.h, relax, level, flag),
+                       "smooth_sor_backwards")
+
+    def smooth_gs_backwards(self, x, b, level, flag):
+        self.smooth_sor_backwards(x, b, 1.0, level, flag)
+
+    def sor_colours(self, level):
+        """(colour of every canonical DoF in sweep order, number of boundary colours)"""
+        n = self.storage.ndofs(level)
+        col = np.zeros(n, dtype=np.int32)
+        nb = C.c_int()
+        self.ctx.check(lib().hg_p1_sor_colours(self.ctx.h, self.h, level, _ptr(col), C.byref(nb)), "sor_colours")
+        return col, nb.value
+
+    def inverse_diagonal(self, level, dst):
+        self.ctx.check(lib().hg_p1_inverse_diagonal(self.ctx.h, self.h, level, dst.h), "inverse diagonal")
+
+    def smooth_jac(self, dst, rhs, src, relax, level, flag):
+        """dst = src + relax D^-1 (rhs - A src)"""
+        self.ctx.check(lib().hg_p1_smooth_jac(self.ctx.h, self.h, dst.h, rhs.h, src.h, relax, level, flag),
+                       "smooth_jac")
+
+    def gemv(self, alpha, src, beta, dst, level, flag):
+        """dst = alpha A src + beta dst on DoFs of type flag"""
+        self.ctx.check(lib().hg_p1_gemv(self.ctx.h, self.h, alpha, src.h, beta, dst.h, level, flag), "gemv")
+
+    def apply_scaled(self, alpha, src, dst, level, flag, update=REPLACE):
+        self.gemv(alpha, src, 0.0 if update == REPLACE else 1.0, dst, level, flag)
 
     def cell_stencil(self, level, cell):
         out = np.zeros(15)
@@ -619,6 +661,19 @@ class P2ConstantLaplaceOperator:
 
     def smooth_gs(self, x, b, level, flag=GMG_FLAG):
         self.smooth_sor(x, b, level, flag, 1.0)
+
+    def smooth_sor_backwards(self, x, b, level, flag=GMG_FLAG, relax=1.0):
+        self.ctx.check(lib().hg_p2_const_smooth_sor_backwards(self.ctx.h, self.h, x.h, b.h, level, flag, relax),
+                       "smooth_sor_backwards")
+
+    def sor_colours(self, level):
+        """(colour of every canonical P2 DoF in sweep order, number of boundary colours)"""
+        n = self.storage.p2_ndofs(level)
+        col = np.zeros(n, dtype=np.int32)
+        nb = C.c_int()
+        self.ctx.check(lib().hg_p2_const_sor_colours(self.ctx.h, self.h, level, _ptr(col), C.byref(nb)),
+                       "sor_colours")
+        return col, nb.value
 
     def inverse_diagonal(self, level, dst):
         self.ctx.check(lib().hg_p2_const_inverse_diagonal(self.ctx.h, self.h, level, dst.h), "inverse diagonal")

@@ -155,6 +155,9 @@ struct P1OpLevel
    bool              csr = false;
    DevBuf< int64_t > csrPtr, csrPos;
    DevBuf< double >  csrVal;
+   // inverse diagonal of every storage position (cell lattices, halo copies
+   // included, then the owned DoFs), built on first use
+   DevBuf< double > invDiagFull;
 };
 
 // boundary rows are assembled into CSR up to this many owned boundary DoFs
@@ -383,7 +386,7 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
 template < class T >
 __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
                             int64_t mm, const double* __restrict__ stencil, const T* __restrict__ src,
-                            T* __restrict__ dst, int add, int lv, int rpbLog )
+                            T* __restrict__ dst, int add, int lv, int rpbLog, double alpha )
 {
    using A             = typename Acc< T >::type;
    const int64_t nrc   = nrows * nc;
@@ -416,6 +419,7 @@ __global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int
 #pragma unroll
       for ( int s = 0; s < 15; ++s )
          v += st[s] * toAcc( src[( ( rb[s] + x ) << lv ) + k] );
+      v               = A( alpha ) * v;
       const int64_t o = ( ( out + x ) << lv ) + k;
       if ( add )
          v += toAcc( dst[o] );
@@ -496,7 +500,7 @@ __device__ __forceinline__ typename Acc< T >::type bndStencilSum( const BndGeom&
 template < class T >
 __global__ void kBndApply( BndGeom g, int64_t nb, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
                            const double* __restrict__ cls, const T* __restrict__ src, T* __restrict__ dst, int add,
-                           int lv )
+                           int lv, double alpha )
 {
    using A           = typename Acc< T >::type;
    const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
@@ -507,6 +511,7 @@ __global__ void kBndApply( BndGeom g, int64_t nb, int64_t nc, const uint8_t* __r
    forEachIncident( g, b, [&]( int64_t c, int64_t x, int64_t y, int64_t z, int k ) {
       s += bndStencilSum( g, cls, src, c, x, y, z, k, 0, lv, vec );
    } );
+   s               = A( alpha ) * s;
    const int64_t o = ( ( nc * g.sc + b ) << lv ) + vec;
    if ( add )
       s += toAcc( dst[o] );
@@ -548,7 +553,7 @@ template < class T >
 __global__ void kBndApplyCsr( int64_t nb, int64_t ownerBase, const uint8_t* __restrict__ bndFlag, int flag,
                               const int64_t* __restrict__ ptr, const int64_t* __restrict__ pos,
                               const double* __restrict__ val, const double* __restrict__ invDiag,
-                              const T* __restrict__ src, T* __restrict__ dst, int add, int lv )
+                              const T* __restrict__ src, T* __restrict__ dst, int add, int lv, double alpha )
 {
    using A           = typename Acc< T >::type;
    const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
@@ -559,6 +564,7 @@ __global__ void kBndApplyCsr( int64_t nb, int64_t ownerBase, const uint8_t* __re
    A             s = A( 1.0 / invDiag[b] ) * toAcc( src[o] );
    for ( int64_t j = ptr[b]; j < ptr[b + 1]; ++j )
       s += A( val[j] ) * toAcc( src[( pos[j] << lv ) + vec] );
+   s = A( alpha ) * s;
    if ( add )
       s += toAcc( dst[o] );
    dst[o] = fromAcc< T >( s );
@@ -722,6 +728,19 @@ __global__ void kProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_t*
    fine[o] = fromAcc< T >( v );
 }
 
+// inverse diagonal of every storage position: 1 / stencil centre on the cell
+// lattices, the assembled boundary rows' inverse diagonal on the owned DoFs
+// (halo positions are refreshed from the owners afterwards)
+__global__ void kP1InvDiagFull( int64_t nc, int64_t sc, int64_t nb, const double* __restrict__ stencil,
+                                const double* __restrict__ invBnd, double* __restrict__ out )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( i < nc * sc )
+      out[i] = 1.0 / stencil[15 * ( i / sc )];
+   else if ( i < nc * sc + nb )
+      out[i] = invBnd[i - nc * sc];
+}
+
 // ------------------------------------------------------------------------
 // host drivers
 // ------------------------------------------------------------------------
@@ -756,7 +775,25 @@ static P1OpLevel& levelOf( Context* ctx, P1Operator& op, int L )
    return it->second;
 }
 
-void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L, int flag, int update )
+// dst = alpha A src (update Replace) or dst + alpha A src (update Add) on the
+// DoFs of type flag
+static const double* p1InverseDiagonal( Context* ctx, P1Operator& op, int L )
+{
+   P1OpLevel& o = levelOf( ctx, op, L );
+   if ( o.invDiagFull.p )
+      return o.invDiagFull.p;
+   Lattice&      lat = op.st->lattice( ctx, L );
+   const int64_t n   = lat.nc * lat.sc + lat.nb;
+   o.invDiagFull.alloc( ctx, size_t( n ) );
+   kP1InvDiagFull<<< gridFor( n ), kBlock, 0, ctx->stream >>>( lat.nc, lat.sc, lat.nb, o.stencil.p, o.invDiag.p,
+                                                              o.invDiagFull.p );
+   HG_CHECK_HIP( hipGetLastError() );
+   pullHalos( ctx, *op.st, lat, o.invDiagFull.p, 0 );
+   return o.invDiagFull.p;
+}
+
+void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L, int flag, int update,
+              double alpha = 1.0 )
 {
    if ( src.st != op.st || dst.st != op.st || src.dtype != dst.dtype || src.off != 0 || dst.off != 0 ||
         src.lv != dst.lv )
@@ -775,7 +812,7 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
       {
          kBndApplyCsr< T ><<< gridFor( lat.nb << lv ), kBlock, 0, ctx->stream >>>(
              lat.nb, lat.nc * lat.sc, lat.bndFlag.p, flag, o.csrPtr.p, o.csrPos.p, o.csrVal.p, o.invDiag.p,
-             (const T*) s, (T*) d, update, lv );
+             (const T*) s, (T*) d, update, lv, alpha );
          HG_CHECK_HIP( hipGetLastError() );
       }
       else if ( lat.nb > 0 )
@@ -783,7 +820,7 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
          kBndApply< T ><<< gridFor( lat.nb << lv ), kBlock, 0, ctx->stream >>>( BndGeom::of( *op.st, lat ), lat.nb,
                                                                                lat.nc, lat.bndFlag.p, flag,
                                                                                o.clsStencil.p, (const T*) s, (T*) d,
-                                                                               update, lv );
+                                                                               update, lv, alpha );
          HG_CHECK_HIP( hipGetLastError() );
       }
       if ( ( flag & kInner ) && lat.nrows > 0 )
@@ -792,14 +829,21 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
          const int rpb = rowsPerBlockLog( ( lat.mm - 3 ) << lv );
          kCellApply< T ><<< cellGrid( lat.nrows * lat.nc, rpb ), kRowBlock, 0, ctx->stream >>>(
              lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update, lv,
-             rpb );
+             rpb, alpha );
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
       }
    } );
 }
 
-void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax, int L, int flag )
+// One SOR sweep in multicolour order.  Forward: the owned vertex/edge/face DoFs
+// colour by colour (greedy colours of their coupling graph), then the 8 parity
+// colours k = (x&1) + 2(y&1) + 4(z&1) of the cell interiors (every non-centre
+// offset of the 15-point stencil has an odd component, so points of one parity
+// never couple); backward: the same colours in reverse order
+// (smooth_sor_backwards, P1Operator.hpp:331-336,381-396).
+void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax, int L, int flag,
+            bool backward = false )
 {
    if ( x.st != op.st || b.st != op.st || x.dtype != b.dtype || &x == &b || x.off != 0 || b.off != 0 || x.lv != b.lv )
       throw Error( -1, "smooth_sor needs distinct x/b on the operator's storage with one value type and vector count" );
@@ -812,15 +856,13 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
    pullHalos( ctx, *op.st, lat, xp, x.dtype, lv );
    dispatchType( x.dtype, [&]( auto tag ) {
       using T = decltype( tag );
-      // owned vertex/edge/face DoFs colour by colour (each update also refreshes
-      // its halo copies), then the 8 parity colours of the cell interiors: every
-      // non-centre offset of the 15-point stencil has an odd component
       const BndGeom g = BndGeom::of( *op.st, lat );
-      for ( size_t cc = 0; cc + 1 < o.colorStart.size(); ++cc )
-      {
+      // boundary colour cc; each update of the non-CSR kernel also refreshes the
+      // DoF's halo copies, the CSR kernel updates the owner only
+      auto boundaryColour = [&]( size_t cc ) {
          const int64_t n = o.colorStart[cc + 1] - o.colorStart[cc];
          if ( n == 0 )
-            continue;
+            return;
          if ( o.csr )
             kBndSORCsr< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>(
                 n, o.colorRows.p + o.colorStart[cc], lat.nc * lat.sc, lat.bndFlag.p, flag, o.csrPtr.p, o.csrPos.p,
@@ -831,27 +873,44 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
                                                                            o.invDiag.p, relax, (const T*) bp, (T*) xp,
                                                                            lv );
          HG_CHECK_HIP( hipGetLastError() );
+      };
+      // cell-interior parity colour k = cx + 2 cy + 4 cz
+      auto interiorColour = [&]( int k ) {
+         const int cx = k & 1, cy = ( k >> 1 ) & 1, cz = k >> 2;
+         const int     grp = cy + 2 * cz;
+         const int64_t n   = lat.rowGroup[grp + 1] - lat.rowGroup[grp];
+         if ( n == 0 )
+            return;
+         const int rpb = rowsPerBlockLog( ( ( lat.mm - 2 ) / 2 ) << lv );
+         kCellSOR< T ><<< cellGrid( n * lat.nc, rpb ), kRowBlock, 0, ctx->stream >>>(
+             lat.rows.p + 2 * lat.rowGroup[grp], n, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, cx, relax,
+             (const T*) bp, (T*) xp, lv, rpb );
+         HG_CHECK_HIP( hipGetLastError() );
+      };
+      const size_t nbc      = o.colorStart.empty() ? 0 : o.colorStart.size() - 1;
+      const bool   interior = ( flag & kInner ) && lat.nrows > 0;
+      if ( !backward )
+      {
+         for ( size_t cc = 0; cc < nbc; ++cc )
+            boundaryColour( cc );
+         // the CSR sweep updated the owners only: refresh the cell copies the
+         // interior sweep reads
+         if ( o.csr && nbc > 0 )
+            pullHalos( ctx, *op.st, lat, xp, x.dtype, lv );
+         if ( interior )
+            for ( int k = 0; k < 8; ++k )
+               interiorColour( k );
       }
-      // the CSR sweep updated the owners only: refresh the cell copies the
-      // interior sweep reads
-      if ( o.csr && !o.colorStart.empty() )
-         pullHalos( ctx, *op.st, lat, xp, x.dtype, lv );
-      if ( !( flag & kInner ) || lat.nrows == 0 )
-         return;
-      for ( int cz = 0; cz < 2; ++cz )
-         for ( int cy = 0; cy < 2; ++cy )
-            for ( int cx = 0; cx < 2; ++cx )
-            {
-               const int     grp = cy + 2 * cz;
-               const int64_t n   = lat.rowGroup[grp + 1] - lat.rowGroup[grp];
-               if ( n == 0 )
-                  continue;
-               const int rpb = rowsPerBlockLog( ( ( lat.mm - 2 ) / 2 ) << lv );
-               kCellSOR< T ><<< cellGrid( n * lat.nc, rpb ), kRowBlock, 0, ctx->stream >>>(
-                   lat.rows.p + 2 * lat.rowGroup[grp], n, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, cx, relax,
-                   (const T*) bp, (T*) xp, lv, rpb );
-               HG_CHECK_HIP( hipGetLastError() );
-            }
+      else
+      {
+         if ( interior )
+            for ( int k = 7; k >= 0; --k )
+               interiorColour( k );
+         for ( size_t cc = nbc; cc-- > 0; )
+            boundaryColour( cc );
+         if ( o.csr && nbc > 0 )
+            pullHalos( ctx, *op.st, lat, xp, x.dtype, lv );
+      }
    } );
    ctx->endTiming();
 }
@@ -962,6 +1021,90 @@ int hg_p1_smooth_sor( hg_ctx* h, hg_handle op, hg_handle x, hg_handle b, double 
 {
    return hg_guard( h, [&]( Context* ctx ) {
       p1SOR( ctx, ctx->op< P1Operator >( op ), ctx->function( x ), ctx->function( b ), relax, level, flag );
+   } );
+}
+
+int hg_p1_smooth_sor_backwards( hg_ctx* h, hg_handle op, hg_handle x, hg_handle b, double relax, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      p1SOR( ctx, ctx->op< P1Operator >( op ), ctx->function( x ), ctx->function( b ), relax, level, flag, true );
+   } );
+}
+
+int hg_p1_sor_colours( hg_ctx* h, hg_handle opH, int level, int32_t* colour, int* nBoundaryColours )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P1Operator& op = ctx->op< P1Operator >( opH );
+      if ( !colour || !nBoundaryColours )
+         throw Error( HG_ERR_ARG, "sor_colours: null output" );
+      P1OpLevel&      o   = levelOf( ctx, op, level );
+      Lattice&        lat = op.st->lattice( ctx, level );
+      const Topology& t   = op.st->topo;
+      const int       nbc = o.colorStart.empty() ? 0 : int( o.colorStart.size() ) - 1;
+      std::vector< int32_t > rows;
+      if ( lat.nb > 0 )
+      {
+         rows.resize( size_t( lat.nb ) );
+         HG_CHECK_HIP( hipMemcpy( rows.data(), o.colorRows.p, size_t( lat.nb ) * sizeof( int32_t ), hipMemcpyDeviceToHost ) );
+      }
+      for ( int cc = 0; cc < nbc; ++cc )
+         for ( int64_t k = o.colorStart[cc]; k < o.colorStart[cc + 1]; ++k )
+            colour[rows[size_t( k )]] = cc;
+      const int64_t mm = lat.mm;
+      for ( int c = 0; c < t.nc; ++c )
+         for ( int64_t z = 1; z < mm; ++z )
+            for ( int64_t y = 1; y + z < mm; ++y )
+               for ( int64_t x = 1; x + y + z < mm; ++x )
+                  colour[lat.num.locate( t, c, x, y, z )] = nbc + int32_t( ( x & 1 ) + 2 * ( y & 1 ) + 4 * ( z & 1 ) );
+      *nBoundaryColours = nbc;
+   } );
+}
+
+int hg_p1_inverse_diagonal( hg_ctx* h, hg_handle opH, int level, hg_handle dst )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P1Operator& op = ctx->op< P1Operator >( opH );
+      Function&   d  = ctx->function( dst );
+      if ( d.st != op.st || d.off != 0 || d.dtype != 0 || d.lv != 0 || level < d.minL || level > d.maxL )
+         throw Error( HG_ERR_ARG, "inverse diagonal needs a single-vector FP64 P1 function on the operator's storage" );
+      const double* inv = p1InverseDiagonal( ctx, op, level );
+      Lattice&      lat = op.st->lattice( ctx, level );
+      HG_CHECK_HIP( hipMemcpyAsync( d.ptr( level ), inv, size_t( lat.nc * lat.sc + lat.nb ) * sizeof( double ),
+                                    hipMemcpyDeviceToDevice, ctx->stream ) );
+   } );
+}
+
+int hg_p1_smooth_jac( hg_ctx* h, hg_handle opH, hg_handle dst, hg_handle rhs, hg_handle src, double relax, int level,
+                      int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P1Operator& op = ctx->op< P1Operator >( opH );
+      Function&   d  = ctx->function( dst );
+      Function&   r  = ctx->function( rhs );
+      Function&   s  = ctx->function( src );
+      if ( r.st != d.st || r.dtype != d.dtype || r.off != d.off || r.lv != d.lv || &r == &d || &s == &d )
+         throw Error( HG_ERR_ARG, "smooth_jac needs dst, rhs, src of one storage, value type and vector count, dst distinct" );
+      const double* inv = p1InverseDiagonal( ctx, op, level );
+      p1Apply( ctx, op, s, d, level, flag, HG_REPLACE );
+      jacobiUpdate( ctx, op.st->lattice( ctx, level ), d.dtype, d.lv, flag, relax, r.ptr( level ), s.ptr( level ), inv,
+                    d.ptr( level ) );
+   } );
+}
+
+int hg_p1_gemv( hg_ctx* h, hg_handle opH, double alpha, hg_handle src, double beta, hg_handle dst, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P1Operator& op = ctx->op< P1Operator >( opH );
+      Function&   s  = ctx->function( src );
+      Function&   d  = ctx->function( dst );
+      if ( beta == 0.0 )
+      {
+         p1Apply( ctx, op, s, d, level, flag, HG_REPLACE, alpha );
+         return;
+      }
+      if ( beta != 1.0 )
+         scaleFlagged( ctx, d, level, flag, beta );
+      p1Apply( ctx, op, s, d, level, flag, HG_ADD, alpha );
    } );
 }
 

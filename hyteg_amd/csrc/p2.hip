@@ -402,7 +402,8 @@ template < class T, class K, bool STORED >
 __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom,
                                                  const P2CellGeom* __restrict__ geom64, const KParams* __restrict__ kp,
                                                  const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1,
-                                                 int64_t w1, const T* __restrict__ src, T* __restrict__ dst )
+                                                 int64_t w1, const T* __restrict__ src, T* __restrict__ dst,
+                                                 double alpha )
 {
    using R  = typename Acc< T >::type;
    using QC = QuadConst< R >;
@@ -484,7 +485,7 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
       if ( X + Y + Z <= top )
-         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, sb[p] );
+         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * sb[p] );
    }
 }
 
@@ -496,18 +497,18 @@ template < class T, class K >
 __global__ __launch_bounds__( kTileThreads ) __attribute__( ( amdgpu_waves_per_eu( 6, 8 ) ) ) void kP2TileApplyStored(
     const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
     const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
-    const T* __restrict__ src, T* __restrict__ dst )
+    const T* __restrict__ src, T* __restrict__ dst, double alpha )
 {
-   p2TileApplyBody< T, K, true >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst );
+   p2TileApplyBody< T, K, true >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst, alpha );
 }
 // coefficients evaluated in place (the compiler's own register budget)
 template < class T, class K >
 __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyEval(
     const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
     const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
-    const T* __restrict__ src, T* __restrict__ dst )
+    const T* __restrict__ src, T* __restrict__ dst, double alpha )
 {
-   p2TileApplyBody< T, K, false >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst );
+   p2TileApplyBody< T, K, false >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst, alpha );
 }
 
 __global__ void kP2ElementDiag( const CellGeom* __restrict__ geom, const KParams* __restrict__ kp, int64_t mm,
@@ -566,7 +567,7 @@ __global__ __launch_bounds__( 64 ) void kP2SmallApply( const P2CellGeomT< typena
                                                        const P2CellGeom* __restrict__ geom64,
                                                        const KParams* __restrict__ kp, const K* __restrict__ table,
                                                        int64_t mm, int64_t nc, int64_t sc1, int64_t w1,
-                                                       const T* __restrict__ src, T* __restrict__ dst )
+                                                       const T* __restrict__ src, T* __restrict__ dst, double alpha )
 {
    using R = typename Acc< T >::type;
    __shared__ R  sb[kBox3];
@@ -632,7 +633,7 @@ __global__ __launch_bounds__( 64 ) void kP2SmallApply( const P2CellGeomT< typena
       if ( X + Y + Z <= top )
       {
          T& o = dstC[tetRowStart32( w, Y, Z ) + X];
-         o    = fromAcc< T >( R( toAcc( o ) ) + sb[X + kBox * ( Y + kBox * Z )] );
+         o    = fromAcc< T >( R( toAcc( o ) ) + R( alpha ) * sb[X + kBox * ( Y + kBox * Z )] );
       }
    }
 }
@@ -669,7 +670,11 @@ static const P2CellGeomT< typename Acc< T >::type >* p2Geometry( P2Operator& op,
       return op.tgeomF.at( L ).p;
 }
 
-void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L, int flag, int update )
+// dst = alpha A src (update Replace) or dst + alpha A src (update Add) on the
+// DoFs of type flag: P2ElementwiseOperator::gemv with beta 0 or 1
+// (P2ElementwiseOperator.cpp:110-233)
+void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L, int flag, int update,
+              double alpha = 1.0 )
 {
    p2CheckFunction( op, src, L );
    p2CheckFunction( op, dst, L );
@@ -704,17 +709,17 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
       {
          if ( table )
             kP2SmallApply< T, K, true ><<< unsigned( lat.nc ), 64, 0, ctx->stream >>>(
-                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, lat.nc, lat.sc, lat.w, (const T*) s, (T*) d );
+                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, lat.nc, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
          else
             kP2SmallApply< T, K, false ><<< unsigned( lat.nc ), 64, 0, ctx->stream >>>(
-                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, lat.nc, lat.sc, lat.w, (const T*) s, (T*) d );
+                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, lat.nc, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
       }
       else if ( table )
          kP2TileApplyStored< T, K ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
-             geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d );
+             geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
       else
          kP2TileApplyEval< T, K ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
-             geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d );
+             geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
       HG_CHECK_HIP( hipGetLastError() );
       ctx->endTiming();
    } );
@@ -1000,6 +1005,42 @@ int hg_p2_apply( hg_ctx* h, hg_handle op, hg_handle src, hg_handle dst, int leve
       if ( update != HG_REPLACE && update != HG_ADD )
          throw Error( HG_ERR_ARG, "invalid update type" );
       p2Apply( ctx, ctx->op< P2Operator >( op ), ctx->function( src ), ctx->function( dst ), level, flag, update );
+   } );
+}
+
+int hg_p2_gemv( hg_ctx* h, hg_handle opH, double alpha, hg_handle src, double beta, hg_handle dst, int level, int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P2Operator& op = ctx->op< P2Operator >( opH );
+      Function&   s  = ctx->function( src );
+      Function&   d  = ctx->function( dst );
+      if ( beta == 0.0 )
+      {
+         p2Apply( ctx, op, s, d, level, flag, HG_REPLACE, alpha );
+         return;
+      }
+      p2CheckFunction( op, d, level );
+      if ( beta != 1.0 )
+         scaleFlagged( ctx, d, level, flag, beta );
+      p2Apply( ctx, op, s, d, level, flag, HG_ADD, alpha );
+   } );
+}
+
+int hg_p2_smooth_jac( hg_ctx* h, hg_handle opH, hg_handle dst, hg_handle rhs, hg_handle src, double relax, int level,
+                      int flag )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P2Operator& op = ctx->op< P2Operator >( opH );
+      Function&   d  = ctx->function( dst );
+      Function&   r  = ctx->function( rhs );
+      Function&   s  = ctx->function( src );
+      p2CheckFunction( op, r, level );
+      if ( r.dtype != d.dtype || &r == &d || &s == &d )
+         throw Error( HG_ERR_ARG, "smooth_jac needs dst, rhs, src of one value type, dst distinct" );
+      const double* inv = p2InverseDiagonal( ctx, op, level );
+      p2Apply( ctx, op, s, d, level, flag, HG_REPLACE );
+      jacobiUpdate( ctx, op.st->lattice( ctx, level + 1 ), d.dtype, 0, flag, relax, r.ptr( level ), s.ptr( level ), inv,
+                    d.ptr( level ) );
    } );
 }
 

@@ -531,48 +531,107 @@ int hg_p2_const_apply( hg_ctx* h, hg_handle opH, hg_handle src, hg_handle dst, i
    } );
 }
 
-int hg_p2_const_smooth_sor( hg_ctx* h, hg_handle opH, hg_handle xH, hg_handle bH, int level, int flag, double relax )
+// SOR in multicolour order: forward, the owned macro vertex/edge/face DoFs
+// colour by colour, then the 27 cell-interior colours k = (X%3) + 3 (Y%3) +
+// 9 (Z%3); backward (smooth_sor_backwards, P2ConstantOperator.hpp:95-106) the
+// same colours in reverse order, the cell interiors first.
+static void p2ConstSOR( Context* ctx, P2ConstOperator& op, Function& x, Function& b, int level, int flag, double relax,
+                        bool backward )
 {
-   return hg_guard( h, [&]( Context* ctx ) {
-      P2ConstOperator& op = ctx->op< P2ConstOperator >( opH );
-      Function&        x  = ctx->function( xH );
-      Function&        b  = ctx->function( bH );
-      checkFn( op, x, level );
-      checkFn( op, b, level );
-      if ( &x == &b )
-         throw Error( HG_ERR_ARG, "smooth needs distinct x and b" );
-      Storage&      st  = *op.st;
-      Lattice&      lat = st.lattice( ctx, level + 1 );
-      P2ConstLevel& o   = op.lv.at( level );
-      const int64_t mm  = int64_t( 1 ) << level;
-      double*       xp  = (double*) x.ptr( level );
-      const double* bp  = (const double*) b.ptr( level );
-      pullHalos( ctx, st, lat, xp, 0 );
-      // owned vertex/edge/face DoFs, colour by colour
-      for ( size_t cc = 0; cc + 1 < o.colorStart.size(); ++cc )
-      {
-         const int64_t n = o.colorStart[cc + 1] - o.colorStart[cc];
-         if ( n == 0 )
-            continue;
-         kP2ConstBndSOR<<< gridFor( n ), kBlock, 0, ctx->stream >>>( n, o.colorRows.p + o.colorStart[cc],
-                                                                      lat.nc * lat.sc, lat.bndFlag.p, flag, o.csrPtr.p,
-                                                                      o.csrPos.p, o.csrVal.p, o.invDiag.p, relax, bp,
-                                                                      xp );
-         HG_CHECK_HIP( hipGetLastError() );
-      }
+   checkFn( op, x, level );
+   checkFn( op, b, level );
+   if ( &x == &b )
+      throw Error( HG_ERR_ARG, "smooth needs distinct x and b" );
+   Storage&      st  = *op.st;
+   Lattice&      lat = st.lattice( ctx, level + 1 );
+   P2ConstLevel& o   = op.lv.at( level );
+   const int64_t mm  = int64_t( 1 ) << level;
+   double*       xp  = (double*) x.ptr( level );
+   const double* bp  = (const double*) b.ptr( level );
+   pullHalos( ctx, st, lat, xp, 0 );
+   const size_t nbc = o.colorStart.empty() ? 0 : o.colorStart.size() - 1;
+   auto boundaryColour = [&]( size_t cc ) {
+      const int64_t n = o.colorStart[cc + 1] - o.colorStart[cc];
+      if ( n == 0 )
+         return;
+      kP2ConstBndSOR<<< gridFor( n ), kBlock, 0, ctx->stream >>>( n, o.colorRows.p + o.colorStart[cc], lat.nc * lat.sc,
+                                                                   lat.bndFlag.p, flag, o.csrPtr.p, o.csrPos.p,
+                                                                   o.csrVal.p, o.invDiag.p, relax, bp, xp );
+      HG_CHECK_HIP( hipGetLastError() );
+   };
+   const int32_t nb3 = boxCount( lat.w );
+   const dim3    grid( unsigned( ( int64_t( nb3 ) * nb3 * nb3 + 255 ) / 256 ), unsigned( lat.nc ) );
+   auto interiorColour = [&]( int k ) {
+      kP2ConstCellSOR<<< grid, 256, 0, ctx->stream >>>( o.Ae.p, mm, lat.sc, lat.w, k % 3, ( k / 3 ) % 3, k / 9, nb3,
+                                                         relax, bp, xp );
+      HG_CHECK_HIP( hipGetLastError() );
+   };
+   if ( !backward )
+   {
+      for ( size_t cc = 0; cc < nbc; ++cc )
+         boundaryColour( cc );
       // the cell interiors read the new boundary values through their halos
       pullHalos( ctx, st, lat, xp, 0 );
       if ( flag & kInner )
+         for ( int k = 0; k < 27; ++k )
+            interiorColour( k );
+   }
+   else
+   {
+      if ( flag & kInner )
+         for ( int k = 26; k >= 0; --k )
+            interiorColour( k );
+      for ( size_t cc = nbc; cc-- > 0; )
+         boundaryColour( cc );
+      pullHalos( ctx, st, lat, xp, 0 );
+   }
+}
+
+int hg_p2_const_smooth_sor( hg_ctx* h, hg_handle opH, hg_handle xH, hg_handle bH, int level, int flag, double relax )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      p2ConstSOR( ctx, ctx->op< P2ConstOperator >( opH ), ctx->function( xH ), ctx->function( bH ), level, flag, relax,
+                  false );
+   } );
+}
+
+int hg_p2_const_smooth_sor_backwards( hg_ctx* h, hg_handle opH, hg_handle xH, hg_handle bH, int level, int flag,
+                                      double relax )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      p2ConstSOR( ctx, ctx->op< P2ConstOperator >( opH ), ctx->function( xH ), ctx->function( bH ), level, flag, relax,
+                  true );
+   } );
+}
+
+int hg_p2_const_sor_colours( hg_ctx* h, hg_handle opH, int level, int32_t* colour, int* nBoundaryColours )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P2ConstOperator& op = ctx->op< P2ConstOperator >( opH );
+      if ( !colour || !nBoundaryColours )
+         throw Error( HG_ERR_ARG, "sor_colours: null output" );
+      if ( level < op.minL || level > op.maxL )
+         throw Error( HG_ERR_ARG, "level outside the operator's level range" );
+      P2ConstLevel&   o   = op.lv.at( level );
+      Lattice&        lat = op.st->lattice( ctx, level + 1 );
+      const Topology& t   = op.st->topo;
+      const int       nbc = o.colorStart.empty() ? 0 : int( o.colorStart.size() ) - 1;
+      std::vector< int32_t > rows;
+      if ( lat.nb > 0 )
       {
-         const int32_t nb3 = boxCount( lat.w );
-         const dim3    grid( unsigned( ( int64_t( nb3 ) * nb3 * nb3 + 255 ) / 256 ), unsigned( lat.nc ) );
-         for ( int cz = 0; cz < 3; ++cz )
-            for ( int cy = 0; cy < 3; ++cy )
-               for ( int cx = 0; cx < 3; ++cx )
-                  kP2ConstCellSOR<<< grid, 256, 0, ctx->stream >>>( o.Ae.p, mm, lat.sc, lat.w, cx, cy, cz, nb3, relax,
-                                                                     bp, xp );
-         HG_CHECK_HIP( hipGetLastError() );
+         rows.resize( size_t( lat.nb ) );
+         HG_CHECK_HIP( hipMemcpy( rows.data(), o.colorRows.p, size_t( lat.nb ) * sizeof( int32_t ), hipMemcpyDeviceToHost ) );
       }
+      for ( int cc = 0; cc < nbc; ++cc )
+         for ( int64_t k = o.colorStart[cc]; k < o.colorStart[cc + 1]; ++k )
+            colour[rows[size_t( k )]] = cc;
+      const int64_t mm = lat.mm;
+      for ( int c = 0; c < t.nc; ++c )
+         for ( int64_t z = 1; z < mm; ++z )
+            for ( int64_t y = 1; y + z < mm; ++y )
+               for ( int64_t x = 1; x + y + z < mm; ++x )
+                  colour[lat.num.locate( t, c, x, y, z )] = nbc + int32_t( x % 3 + 3 * ( y % 3 ) + 9 * ( z % 3 ) );
+      *nBoundaryColours = nbc;
    } );
 }
 

@@ -291,9 +291,12 @@ struct SlotCoef
 {
    double sign;
    int    num, den;
+   int    gate; // slot index: the coefficient is 0 while that slot is nonzero (-1: none)
 };
 __device__ __forceinline__ double slotCoef( const SlotCoef& c, const double* __restrict__ slots )
 {
+   if ( c.gate >= 0 && slots[c.gate] != 0.0 )
+      return 0.0;
    const double n = c.num >= 0 ? slots[c.num] : 1.0;
    const double d = c.den >= 0 ? slots[c.den] : 1.0;
    return d == 0.0 ? 0.0 : c.sign * ( n / d );
@@ -324,6 +327,26 @@ __global__ void kLinCombSlots( int64_t ncell, int64_t nb, const uint8_t* __restr
    if ( add )
       v += toAcc( dst[p] );
    dst[p] = fromAcc< T >( v );
+}
+
+// the stopping test of CGSolver (CGSolver.hpp:180-186) on device scalars:
+// init: done = (|r0| < absTol); otherwise, while done == 0,
+// done = (|r| / |r0| < relTol or |r| < absTol), with |r| = sqrt(slot rr)
+__global__ void kSlotsCGStop( double* __restrict__ slots, int rr, int rr0, double relTol, double absTol, int done,
+                              int init )
+{
+   if ( threadIdx.x != 0 || blockIdx.x != 0 )
+      return;
+   const double r0 = sqrt( slots[rr0] );
+   if ( init )
+   {
+      slots[done] = r0 < absTol ? 1.0 : 0.0;
+      return;
+   }
+   if ( slots[done] != 0.0 )
+      return;
+   const double r = sqrt( slots[rr] );
+   slots[done]    = ( r / r0 < relTol || r < absTol ) ? 1.0 : 0.0;
 }
 
 template < class T >
@@ -429,6 +452,45 @@ __global__ void kCopyVector( int64_t n, const T* __restrict__ src, int ls, int k
    const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
    if ( i < n )
       dst[( i << ld ) + kd] = src[( i << ls ) + ks];
+}
+
+template < class T >
+__global__ void kJacobiUpdate( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, double relax,
+                               const T* __restrict__ rhs, const T* __restrict__ src,
+                               const double* __restrict__ invDiag, int lv, T* __restrict__ dst )
+{
+#pragma clang fp contract( off )
+   using A           = typename Acc< T >::type;
+   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   const int64_t i   = tid >> lv;
+   if ( i >= ncell + nb )
+      return;
+   const bool on = i < ncell ? ( flag & kInner ) != 0 : ( bndFlag[i - ncell] & flag ) != 0;
+   if ( !on )
+      return;
+   A v = A( 0 );
+   v += A( 1.0 ) * toAcc( rhs[tid] );
+   v += A( -1.0 ) * toAcc( dst[tid] );
+   const T r = fromAcc< T >( v );
+   const T t = fromAcc< T >( toAcc( fromAcc< T >( A( invDiag[i] ) ) ) * toAcc( r ) );
+   A       u = A( 0 );
+   u += A( 1.0 ) * toAcc( src[tid] );
+   u += A( relax ) * toAcc( t );
+   dst[tid] = fromAcc< T >( u );
+}
+
+void jacobiUpdate( Context* ctx, Lattice& lat, int dtype, int lv, int flag, double relax, const void* rhs,
+                   const void* src, const double* invDiag, void* dst )
+{
+   const int64_t n = lat.nc * lat.sc + lat.nb;
+   if ( n == 0 )
+      return;
+   dispatchType( dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      kJacobiUpdate< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>(
+          lat.nc * lat.sc, lat.nb, lat.bndFlag.p, flag, relax, (const T*) rhs, (const T*) src, invDiag, lv, (T*) dst );
+   } );
+   HG_CHECK_HIP( hipGetLastError() );
 }
 
 void castCopy( Context* ctx, const void* src, int sdtype, void* dst, int ddtype, int64_t n )
@@ -663,6 +725,22 @@ static hipError_t closeCapture( Context* ctx, hipGraph_t* g )
       deviceGate().unlock();
    }
    return e;
+}
+
+void scaleFlagged( Context* ctx, Function& f, int level, int flag, double beta )
+{
+   checkLevel( f, level );
+   Lattice&      l = f.lat( ctx, level );
+   const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
+   if ( n == 0 )
+      return;
+   dispatchType( f.dtype, [&]( auto tag ) {
+      using T = decltype( tag );
+      T* d    = (T*) f.ptr( level );
+      kLinComb< T ><<< gridFor( n << f.lv ), kBlock, 0, ctx->stream >>>( l.nc * l.sc, l.nb, l.bndFlag.p, flag, 1, beta,
+                                                                         0.0, 0.0, d, nullptr, nullptr, 0, f.lv, d );
+   } );
+   HG_CHECK_HIP( hipGetLastError() );
 }
 
 } // namespace hg
@@ -1223,13 +1301,21 @@ int hg_function_dot_slot( hg_ctx* h, hg_handle a, hg_handle b, int level, int fl
 int hg_function_lincomb_slots( hg_ctx* h, hg_handle dst, int nsrc, const double* signs, const int* num, const int* den,
                                const hg_handle* srcs, int level, int flag, int add )
 {
+   return hg_function_lincomb_slots_gated( h, dst, nsrc, signs, num, den, srcs, level, flag, add, -1 );
+}
+
+int hg_function_lincomb_slots_gated( hg_ctx* h, hg_handle dst, int nsrc, const double* signs, const int* num,
+                                     const int* den, const hg_handle* srcs, int level, int flag, int add, int gate )
+{
    return hg_guard( h, [&]( Context* ctx ) {
+      if ( gate >= 0 )
+         checkSlots( ctx, gate, 1 );
       if ( nsrc < 0 || nsrc > 3 || ( nsrc > 0 && ( !signs || !num || !den || !srcs ) ) )
          throw Error( HG_ERR_ARG, "lincomb_slots takes 0..3 source functions" );
       Function& d = ctx->function( dst );
       checkLevel( d, level );
       void*    src[3] = { nullptr, nullptr, nullptr };
-      SlotCoef c[3]   = { { 0.0, -1, -1 }, { 0.0, -1, -1 }, { 0.0, -1, -1 } };
+      SlotCoef c[3]   = { { 0.0, -1, -1, -1 }, { 0.0, -1, -1, -1 }, { 0.0, -1, -1, -1 } };
       for ( int i = 0; i < nsrc; ++i )
       {
          Function& s = ctx->function( srcs[i] );
@@ -1241,7 +1327,7 @@ int hg_function_lincomb_slots( hg_ctx* h, hg_handle dst, int nsrc, const double*
          if ( den[i] >= 0 )
             checkSlots( ctx, den[i], 1 );
          src[i] = s.ptr( level );
-         c[i]   = SlotCoef{ signs[i], num[i], den[i] };
+         c[i]   = SlotCoef{ signs[i], num[i], den[i], gate };
       }
       Lattice&      l = d.lat( ctx, level );
       const int64_t n = ( flag & kInner ? l.nc * l.sc : 0 ) + l.nb;
@@ -1253,6 +1339,17 @@ int hg_function_lincomb_slots( hg_ctx* h, hg_handle dst, int nsrc, const double*
              l.nc * l.sc, l.nb, l.bndFlag.p, flag, nsrc, c[0], c[1], c[2], ctx->slots.p, (const T*) src[0],
              (const T*) src[1], (const T*) src[2], add != 0, d.lv, (T*) d.ptr( level ) );
       } );
+      HG_CHECK_HIP( hipGetLastError() );
+   } );
+}
+
+int hg_slots_cg_stop( hg_ctx* h, int rr, int rr0, double relTol, double absTol, int done, int init )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      checkSlots( ctx, rr, 1 );
+      checkSlots( ctx, rr0, 1 );
+      checkSlots( ctx, done, 1 );
+      kSlotsCGStop<<< 1, 64, 0, ctx->stream >>>( ctx->slots.p, rr, rr0, relTol, absTol, done, init );
       HG_CHECK_HIP( hipGetLastError() );
    } );
 }

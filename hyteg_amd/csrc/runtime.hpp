@@ -323,6 +323,16 @@ void castCopy( Context* ctx, const void* src, int sdtype, void* dst, int ddtype,
 // excludeFlag are left untouched; zeroOut replaces instead of adding.
 void pushAdditive( Context* ctx, Storage& st, Lattice& lat, void* data, int dtype, int excludeFlag, bool zeroOut,
                    int lv = 0 );
+// f = beta f on the DoFs of type flag (assign({beta}, {f}, level, flag))
+void scaleFlagged( Context* ctx, Function& f, int level, int flag, double beta );
+// Jacobi step of smooth_jac (P1Operator.hpp:432-446, P2ElementwiseOperator.cpp:
+// 342-360) after dst = A src: dst = src + relax * invDiag * (rhs - dst) on the
+// DoFs of type flag (cell-lattice positions count as Inner), each intermediate
+// rounded to the storage type as the assign / multElementwise / assign
+// sequence of the reference rounds it.  invDiag: FP64, one value per storage
+// position of one vector (shared by the 2^lv interleaved vectors).
+void jacobiUpdate( Context* ctx, Lattice& lat, int dtype, int lv, int flag, double relax, const void* rhs,
+                   const void* src, const double* invDiag, void* dst );
 // FP64 dot over owned DoFs of matching type
 double dotDevice( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag );
 // per-vector dot products of two functions with 2^lv interleaved vectors

@@ -436,14 +436,16 @@ int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int mi
       {
          // a V-cycle without host synchronisation, replayed from a HIP graph: the
          // coarse CG runs a fixed count, the number of free coarse DoFs (the
-         // bound of CG in exact arithmetic), capped by coarseMaxIter
+         // bound of CG in exact arithmetic) capped by coarseMaxIter, with its
+         // stopping test on the device (the iterations after it holds leave the
+         // iterate unchanged)
          const DoFType        freeDoFs = Inner | NeumannBoundary | FreeslipBoundary;
          P1Function< double > ones( "free", storage, uint_t( minLevel ), uint_t( minLevel ) );
          ones.interpolate( 1.0, uint_t( minLevel ), freeDoFs );
          const double nfree = ones.dotGlobal( ones, uint_t( minLevel ), freeDoFs );
          coarse             = std::make_shared< CGSolver< P1ConstantLaplaceOperator > >(
-             storage, uint_t( minLevel ), uint_t( minLevel ), uint_t( std::min( double( coarseMaxIter ), nfree ) ), 0.0,
-             0.0 );
+             storage, uint_t( minLevel ), uint_t( minLevel ), uint_t( std::min( double( coarseMaxIter ), nfree ) ),
+             coarseTol, coarseTol );
          coarse->setFixedIterations( true );
       }
       auto R = std::make_shared< P1toP1LinearRestriction<> >();

@@ -177,6 +177,16 @@ int hg_slots_read( hg_ctx* ctx, int first, int n, double* out );
 int hg_function_dot_slot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, int slot );
 int hg_function_lincomb_slots( hg_ctx* ctx, hg_handle dst, int nsrc, const double* signs, const int* num, const int* den,
                                const hg_handle* srcs, int level, int flag, int add );
+/* hg_function_lincomb_slots with every c_i = 0 while slot[gate] != 0 (gate < 0: no
+ * gate), and the stopping test of CGSolver::solve (CGSolver.hpp:180-186) evaluated
+ * on the device: init != 0 sets slot[done] = (sqrt(slot[rr0]) < absTol); otherwise,
+ * while slot[done] == 0, slot[done] = (sqrt(slot[rr]) / sqrt(slot[rr0]) < relTol or
+ * sqrt(slot[rr]) < absTol).  A CG loop of fixed length gated by `done` leaves its
+ * iterate where the host-tested loop would stop, without waiting for the device
+ * (no counterpart in the reference; used by captured V-cycles). */
+int hg_function_lincomb_slots_gated( hg_ctx* ctx, hg_handle dst, int nsrc, const double* signs, const int* num,
+                                     const int* den, const hg_handle* srcs, int level, int flag, int add, int gate );
+int hg_slots_cg_stop( hg_ctx* ctx, int rr, int rr0, double relTol, double absTol, int done, int init );
 /* vector k of fn scaled by scales[k] on flagged DoFs */
 int hg_function_scale_batched( hg_ctx* ctx, hg_handle fn, const double* scales, int level, int flag );
 /* one step of ChebyshevSmoother::solve (ChebyshevSmoother.hpp:186-231) fused
@@ -214,6 +224,32 @@ int hg_p1_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, int le
  * in graph colours, then the 8 parity colours of every macro-cell lattice.
  * smooth_gs == relax 1. */
 int hg_p1_smooth_sor( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, double relax, int level, int flag );
+/* SORBackwardsSmoothable::smooth_sor_backwards(dst, rhs, relax, level, flag)
+ * (solvers/Smoothables.hpp:99-110, P1Operator.hpp:331-336,420-427; GSBackwardsSmoothable
+ * smooth_gs_backwards == relax 1): the colours of hg_p1_smooth_sor in reverse order,
+ * the cell interiors first. */
+int hg_p1_smooth_sor_backwards( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, double relax, int level,
+                                int flag );
+/* The sweep order of hg_p1_smooth_sor on `level` (no counterpart in the reference,
+ * which sweeps lexicographically): colour[i] for every canonical DoF i, the owned
+ * vertex/edge/face DoFs in 0..nb-1 (*nBoundaryColours = nb), the cell interiors in
+ * nb + (x&1) + 2(y&1) + 4(z&1) of their cell-lattice coordinates.  The forward
+ * sweep visits the colours in increasing order; no two coupled DoFs share one. */
+int hg_p1_sor_colours( hg_ctx* ctx, hg_handle op, int level, int32_t* colour, int* nBoundaryColours );
+/* OperatorWithInverseDiagonal::getInverseDiagonalValues (solvers/Smoothables.hpp:112-120,
+ * P1Operator.hpp:458-480): 1/a_ii of every DoF of `level` into the single-vector FP64
+ * P1 function dst (computed on first use and kept by the operator) */
+int hg_p1_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, hg_handle dst );
+/* WeightedJacobiSmoothable::smooth_jac(dst, rhs, src, relax, level, flag)
+ * (solvers/Smoothables.hpp:34-57, P1Operator.hpp:428-447):
+ * dst = src + relax * D^-1 (rhs - A src) on the DoFs of type flag */
+int hg_p1_smooth_jac( hg_ctx* ctx, hg_handle op, hg_handle dst, hg_handle rhs, hg_handle src, double relax, int level,
+                      int flag );
+/* Operator::gemv(alpha, src, beta, dst, level, flag) (operators/Operator.hpp:101-118):
+ * dst = alpha A src + beta dst on the DoFs of type flag; applyScaled(alpha, src, dst,
+ * level, flag, updateType) (Operator.hpp:83-99) is beta = 0 (Replace) or 1 (Add) */
+int hg_p1_gemv( hg_ctx* ctx, hg_handle op, double alpha, hg_handle src, double beta, hg_handle dst, int level,
+                int flag );
 /* stencil weights of the cell interior (15, same order as
  * apply_3D_macrocell_vertexdof_to_vertexdof_replace.cpp:36-50) */
 int hg_p1_cell_stencil( hg_ctx* ctx, hg_handle op, int level, int cell, double* out15 );
@@ -228,6 +264,16 @@ int hg_p2_divkgrad_create( hg_ctx* ctx, hg_handle storage, int minLevel, int max
                            const double* omega, const double* phi, hg_handle* out );
 /* Operator::apply = gemv(1, src, 0|1, dst, level, flag) (P2ElementwiseOperator.cpp:110-328) */
 int hg_p2_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update );
+/* P2ElementwiseOperator::gemv(alpha, src, beta, dst, level, flag): dst = alpha A src + beta dst
+ * on the DoFs of type flag (P2ElementwiseOperator.cpp:110-233, Operator.hpp:101-118);
+ * applyScaled(alpha, src, dst, level, flag, updateType) is beta 0 / 1 (.cpp:331-339) */
+int hg_p2_gemv( hg_ctx* ctx, hg_handle op, double alpha, hg_handle src, double beta, hg_handle dst, int level,
+                int flag );
+/* WeightedJacobiSmoothable::smooth_jac(dst, rhs, src, relax, level, flag)
+ * (P2ElementwiseOperator.cpp:342-371): dst = src + relax * D^-1 (rhs - A src) on the
+ * DoFs of type flag */
+int hg_p2_smooth_jac( hg_ctx* ctx, hg_handle op, hg_handle dst, hg_handle rhs, hg_handle src, double relax, int level,
+                      int flag );
 /* P2ElementwiseOperator::computeAndStoreLocalElementMatrices counterpart
  * (elementwiseoperators/P2ElementwiseOperator.hpp:169,375-385): instead of the
  * 10x10 element matrices (100 values per micro-cell) the operator keeps k at the
@@ -272,7 +318,8 @@ int hg_p1_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int lev
  * res_hist (cycles+1 entries, may be NULL) receives the residual norms.  When
  * the context allows graphs (hg_ctx_set_graphs, the default) the cycles after the
  * first are replayed from a HIP graph and the coarse CG runs a fixed count,
- * min(coarseMaxIter, number of free coarse DoFs), instead of stopping at coarseTol.
+ * min(coarseMaxIter, number of free coarse DoFs), with its coarseTol stopping test
+ * evaluated on the device (the iterations after it holds leave the iterate alone).
  * (solvers/GeometricMultigridSolver.hpp:126-308, solvers/GaussSeidelSmoother.hpp:31-55) */
 int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre, int post,
                      int cycles, double tol, int coarseMaxIter, double coarseTol, double* res_hist, int* cycles_done );
@@ -449,6 +496,15 @@ int hg_p2_const_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, 
  * the owned macro vertex/edge/face DoFs in a greedy multicolour order, then the cell
  * interiors in 27 colours */
 int hg_p2_const_smooth_sor( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int flag, double relax );
+/* SORBackwardsSmoothable::smooth_sor_backwards (P2ConstantOperator.hpp:95-106; relax 1 is
+ * smooth_gs_backwards, :67-77): the colours of hg_p2_const_smooth_sor in reverse order */
+int hg_p2_const_smooth_sor_backwards( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int level, int flag,
+                                      double relax );
+/* the sweep order of hg_p2_const_smooth_sor (no counterpart in the reference): colour[i]
+ * for every canonical P2 DoF i, the owned vertex/edge/face DoFs in 0..nb-1
+ * (*nBoundaryColours = nb), the cell interiors in nb + (X%3) + 3(Y%3) + 9(Z%3) of their
+ * level-(L+1) lattice coordinates */
+int hg_p2_const_sor_colours( hg_ctx* ctx, hg_handle op, int level, int32_t* colour, int* nBoundaryColours );
 /* OperatorWithInverseDiagonal values into dst (every DoF) */
 int hg_p2_const_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, hg_handle dst );
 /* GeometricMultigridSolver<P2ConstantLaplaceOperator> with GaussSeidelSmoother, CGSolver

@@ -289,10 +289,11 @@ class LatticeFunction
    {
       check( ctx_, hg_function_dot_slot( ctx_, h_, other.h_, int( level ), dofs( flag ), slot ), "dotToSlot" );
    }
-   // assign / add with scalars formed on the device: signs[i] * slot[num[i]] / slot[den[i]]
+   // assign / add with scalars formed on the device: signs[i] * slot[num[i]] / slot[den[i]],
+   // all zero while slot[gate] is nonzero (gate < 0: always formed)
    void linCombSlots( const std::vector< double >& signs, const std::vector< int >& num, const std::vector< int >& den,
                       const std::vector< std::reference_wrapper< const LatticeFunction > >& functions, uint_t level,
-                      DoFType flag, bool add ) const
+                      DoFType flag, bool add, int gate = -1 ) const
    {
       if ( signs.size() != functions.size() || num.size() != functions.size() || den.size() != functions.size() )
          throw std::invalid_argument( "linCombSlots needs one coefficient per function" );
@@ -300,8 +301,8 @@ class LatticeFunction
       for ( const auto& f : functions )
          hs.push_back( f.get().h_ );
       check( ctx_,
-             hg_function_lincomb_slots( ctx_, h_, int( hs.size() ), signs.data(), num.data(), den.data(), hs.data(),
-                                        int( level ), dofs( flag ), add ? 1 : 0 ),
+             hg_function_lincomb_slots_gated( ctx_, h_, int( hs.size() ), signs.data(), num.data(), den.data(), hs.data(),
+                                              int( level ), dofs( flag ), add ? 1 : 0, gate ),
              "linCombSlots" );
    }
    template < typename OtherT >
@@ -770,13 +771,13 @@ class CGSolver : public Solver< OperatorType >
       if constexpr ( kDeviceScalars )
       {
          ctx_ = storage->ctx();
-         check( ctx_, hg_slots_alloc( ctx_, 4, &slot0_ ), "CGSolver slots" );
+         check( ctx_, hg_slots_alloc( ctx_, kSlotCount, &slot0_ ), "CGSolver slots" );
       }
    }
    ~CGSolver() override
    {
       if constexpr ( kDeviceScalars )
-         hg_slots_free( ctx_, slot0_, 4 );
+         hg_slots_free( ctx_, slot0_, kSlotCount );
    }
    CGSolver( const CGSolver& )            = delete;
    CGSolver& operator=( const CGSolver& ) = delete;
@@ -829,8 +830,10 @@ class CGSolver : public Solver< OperatorType >
    }
    uint_t getIterations() const { return iterations_; }
    // exactly maxIter iterations with no residual read back (no host
-   // synchronisation, so a surrounding V-cycle can be captured into a HIP graph);
-   // FP64 lattice functions only
+   // synchronisation, so a surrounding V-cycle can be captured into a HIP graph):
+   // the stopping test runs on the device and, once it holds, zeroes the step
+   // sizes of the remaining iterations, so the iterate is the one the host-tested
+   // loop returns; FP64 lattice functions only
    void setFixedIterations( bool fixed )
    {
       if ( fixed && !kDeviceScalars )
@@ -842,6 +845,8 @@ class CGSolver : public Solver< OperatorType >
    // FP64 lattice functions keep pAp, r.r and r.z in device slots of the context
    static constexpr bool kDeviceScalars =
        std::is_same_v< FunctionType, LatticeFunction< double, 1 > > || std::is_same_v< FunctionType, LatticeFunction< double, 2 > >;
+   // slots: r.r, pAp, r.z (two, alternating), r0.r0 and the converged flag
+   static constexpr int kSlotCount = 6;
 
    // The iteration above with its scalars on the device: alpha = r.z / pAp and
    // beta = r.z_new / r.z are formed inside the update kernels from slot values,
@@ -849,7 +854,7 @@ class CGSolver : public Solver< OperatorType >
    // the iterates and the stopping step are those of the host-scalar loop.
    void solveDeviceScalars( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level )
    {
-      const int rr = slot0_, pap = slot0_ + 1;
+      const int rr = slot0_, pap = slot0_ + 1, rr0 = slot0_ + 4, done = slot0_ + 5;
       int       rz[2] = { slot0_ + 2, slot0_ + 3 }, cur = 0;
       copyBCs( x, p_ );
       copyBCs( x, z_ );
@@ -866,6 +871,7 @@ class CGSolver : public Solver< OperatorType >
       r_.dotToSlot( z_, level, flag_, rz[cur] );
       double res_start = 0.0;
       iterations_      = 0;
+      const int gate   = fixed_ ? done : -1;
       if ( !fixed_ )
       {
          r_.dotToSlot( r_, level, flag_, rr );
@@ -874,12 +880,17 @@ class CGSolver : public Solver< OperatorType >
          if ( res_start < absTol_ )
             return;
       }
+      else
+      {
+         r_.dotToSlot( r_, level, flag_, rr0 );
+         check( ctx_, hg_slots_cg_stop( ctx_, rr, rr0, relTol_, absTol_, done, 1 ), "CGSolver" );
+      }
       for ( uint_t i = 0; i < maxIter_; ++i )
       {
          A.apply( p_, ap_, level, flag_, Replace );
          p_.dotToSlot( ap_, level, flag_, pap );
-         x.linCombSlots( { 1.0 }, { rz[cur] }, { pap }, { p_ }, level, flag_, true );
-         r_.linCombSlots( { -1.0 }, { rz[cur] }, { pap }, { ap_ }, level, flag_, true );
+         x.linCombSlots( { 1.0 }, { rz[cur] }, { pap }, { p_ }, level, flag_, true, gate );
+         r_.linCombSlots( { -1.0 }, { rz[cur] }, { pap }, { ap_ }, level, flag_, true, gate );
          iterations_ = i;
          if ( !fixed_ )
          {
@@ -890,12 +901,17 @@ class CGSolver : public Solver< OperatorType >
             if ( relRes < relTol_ || std::sqrt( rsnew ) < absTol_ )
                break;
          }
-         else if ( i + 1 == maxIter_ )
-            break;
+         else
+         {
+            if ( i + 1 == maxIter_ )
+               break;
+            r_.dotToSlot( r_, level, flag_, rr );
+            check( ctx_, hg_slots_cg_stop( ctx_, rr, rr0, relTol_, absTol_, done, 0 ), "CGSolver" );
+         }
          z_.interpolate( 0.0, level, All );
          precondition( A, level );
          r_.dotToSlot( z_, level, flag_, rz[1 - cur] );
-         p_.linCombSlots( { 1.0, 1.0 }, { -1, rz[1 - cur] }, { -1, rz[cur] }, { z_, p_ }, level, flag_, false );
+         p_.linCombSlots( { 1.0, 1.0 }, { -1, rz[1 - cur] }, { -1, rz[cur] }, { z_, p_ }, level, flag_, false, gate );
          cur = 1 - cur;
       }
    }

@@ -1100,3 +1100,71 @@ int or_p1_gmg( const or_mesh* m, int minLevel, int maxLevel, double* x, const do
    free( lv );
    return k;
 }
+
+/* ------------------------------------------------------------------------- */
+/* multicolour Gauss-Seidel / SOR on an assembled matrix, and the vertex set   */
+/* of a DoF's owner primitive                                                 */
+/* ------------------------------------------------------------------------- */
+int or_p1_owner_vertices( const or_mesh* m, int level, int64_t id, int* vs )
+{
+   lvl_t l = or_lvl_info( m, level );
+   return owner_vertices( m, &l, id, vs );
+}
+
+/* One SOR sweep in colour order (P1Operator::smooth_sor with the rows visited
+ * colour by colour instead of lexicographically): colours 0..ncol-1 (backward:
+ * ncol-1..0), inside a colour the rows in increasing index, only rows with
+ * mask[p] != 0.  Every row reads the latest values of its neighbours, so this is
+ * the Gauss-Seidel sweep of that row order; when no two coupled rows share a
+ * colour it equals updating each colour's rows simultaneously.
+ *   x_p <- (1 - relax) x_p + relax (b_p - sum_{q != p} a_pq x_q) / a_pp      */
+void or_sor_mc_csr( int64_t n, const int64_t* rowptr, const int64_t* cols, const double* vals, const int32_t* colour,
+                    int ncol, const unsigned char* mask, double* x, const double* b, double relax, int backward )
+{
+   /* rows grouped by colour, each group in increasing row order (counting sort) */
+   int64_t* start = calloc( (size_t) ncol + 1, sizeof( int64_t ) );
+   int64_t* order = malloc( sizeof( int64_t ) * ( n > 0 ? n : 1 ) );
+   for ( int64_t p = 0; p < n; ++p )
+      if ( mask[p] )
+         start[colour[p] + 1]++;
+   for ( int c = 0; c < ncol; ++c )
+      start[c + 1] += start[c];
+   int64_t* fill = malloc( sizeof( int64_t ) * ( (size_t) ncol + 1 ) );
+   memcpy( fill, start, sizeof( int64_t ) * ( (size_t) ncol + 1 ) );
+   for ( int64_t p = 0; p < n; ++p )
+      if ( mask[p] )
+         order[fill[colour[p]]++] = p;
+   for ( int k = 0; k < ncol; ++k )
+   {
+      const int c = backward ? ncol - 1 - k : k;
+      for ( int64_t i = start[c]; i < start[c + 1]; ++i )
+      {
+         const int64_t p    = order[i];
+         double        s    = b[p];
+         double        diag = 0.0;
+         for ( int64_t e = rowptr[p]; e < rowptr[p + 1]; ++e )
+         {
+            if ( cols[e] == p )
+               diag += vals[e];
+            else
+               s -= vals[e] * x[cols[e]];
+         }
+         x[p] = ( 1.0 - relax ) * x[p] + relax * s / diag;
+      }
+   }
+   free( fill );
+   free( order );
+   free( start );
+}
+
+/* or_p1_gmg for nvec right-hand sides B[k*n .. (k+1)*n) into X (same layout),
+ * the solves distributed over the OpenMP threads; cycles_done[k] per vector */
+void or_p1_gmg_many( const or_mesh* m, int minLevel, int maxLevel, int nvec, double* X, const double* B, int pre,
+                     int post, int cycles, double tol, int coarseMaxIter, double coarseTol, int* cycles_done )
+{
+   const int64_t n = or_p1_ndofs( m, maxLevel );
+#pragma omp parallel for schedule( dynamic, 1 )
+   for ( int k = 0; k < nvec; ++k )
+      cycles_done[k] = or_p1_gmg( m, minLevel, maxLevel, X + (size_t) k * n, B + (size_t) k * n, pre, post, cycles, tol,
+                                  NULL, coarseMaxIter, coarseTol );
+}

@@ -81,6 +81,20 @@ int or_p1_cg( const or_p1op* op, double* x, const double* b, int flag, int maxIt
 int or_p1_gmg( const or_mesh* m, int minLevel, int maxLevel, double* x, const double* b, int pre, int post,
                int cycles, double tol, double* res_hist, int coarseMaxIter, double coarseTol );
 
+/* vertex ids of the macro-primitive owning canonical DoF id (1 vertex, 2 edge
+ * vertices, 3 face vertices or 4 cell vertices); returns their number */
+int or_p1_owner_vertices( const or_mesh* m, int level, int64_t id, int* vs );
+/* One SOR sweep over an assembled CSR matrix with the rows visited colour by
+ * colour (colour[p] in 0..ncol-1; backward: colours in reverse), rows with
+ * mask[p] == 0 left alone; every row reads the latest values (the multicolour
+ * orderings of the GPU Gauss-Seidel sweeps). */
+void or_sor_mc_csr( int64_t n, const int64_t* rowptr, const int64_t* cols, const double* vals, const int32_t* colour,
+                    int ncol, const unsigned char* mask, double* x, const double* b, double relax, int backward );
+/* or_p1_gmg for nvec right-hand sides (vector k at B + k*ndofs), OpenMP over the
+ * vectors; cycles_done[k] per vector */
+void or_p1_gmg_many( const or_mesh* m, int minLevel, int maxLevel, int nvec, double* X, const double* B, int pre,
+                     int post, int cycles, double tol, int coarseMaxIter, double coarseTol, int* cycles_done );
+
 /* ---- P2 on the icosahedral shell ----------------------------------------- */
 typedef struct or_p2op or_p2op;
 

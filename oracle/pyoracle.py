@@ -56,6 +56,9 @@ def lib():
             "or_dot": (_d, [_i64, _p, _p, _p, _i]),
             "or_p1_cg": (_i, [_p, _p, _p, _i, _i, _d, _d]),
             "or_p1_gmg": (_i, [_p, _i, _i, _p, _p, _i, _i, _i, _d, _p, _i, _d]),
+            "or_p1_owner_vertices": (_i, [_p, _i, _i64, _p]),
+            "or_sor_mc_csr": (None, [_i64, _p, _p, _p, _p, _i, _p, _p, _p, _d, _i]),
+            "or_p1_gmg_many": (None, [_p, _i, _i, _i, _p, _p, _i, _i, _i, _d, _i, _d, _p]),
             "or_mesh_shell": (_p, [_i, _i, _d, _d]),
             "or_mesh_blend": (None, [_p, _p]),
             "or_shell_map": (None, [_p, _p, _p, _p]),
@@ -190,6 +193,54 @@ class Mesh:
         k = lib().or_p1_gmg(self.h, min_level, max_level, _ptr(x), _ptr(b), pre, post, cycles, tol,
                             _ptr(hist), coarse_maxit, coarse_tol)
         return k, hist[: k + 1]
+
+    def gmg_many(self, min_level, max_level, X, B, pre=3, post=3, cycles=50, tol=0.0, coarse_maxit=10000,
+                 coarse_tol=1e-16):
+        """or_p1_gmg for every row of B (shape [nvec, ndofs]) into X, OpenMP over the rows"""
+        X = np.ascontiguousarray(X)
+        B = np.ascontiguousarray(B)
+        done = np.zeros(B.shape[0], dtype=np.int32)
+        lib().or_p1_gmg_many(self.h, min_level, max_level, B.shape[0], _ptr(X), _ptr(B), pre, post, cycles, tol,
+                             coarse_maxit, coarse_tol, _ptr(done))
+        return X, done
+
+    def owner_vertices(self, level, dof):
+        vs = np.zeros(4, dtype=np.int32)
+        k = lib().or_p1_owner_vertices(self.h, level, int(dof), _ptr(vs))
+        return tuple(int(v) for v in vs[:k])
+
+
+def sor_multicolour(A, colour, mask, x, b, relax=1.0, backward=False):
+    """one SOR sweep over the CSR matrix A, rows visited colour by colour
+    (colour 0 first, or the last colour first when backward), rows with
+    mask == 0 untouched; x is updated in place"""
+    A = A.tocsr()
+    colour = np.ascontiguousarray(colour, dtype=np.int32)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    rp = np.ascontiguousarray(A.indptr, dtype=np.int64)
+    ci = np.ascontiguousarray(A.indices, dtype=np.int64)
+    va = np.ascontiguousarray(A.data, dtype=np.float64)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    assert x.dtype == np.float64 and x.flags.c_contiguous
+    ncol = int(colour.max()) + 1 if len(colour) else 0
+    lib().or_sor_mc_csr(A.shape[0], _ptr(rp), _ptr(ci), _ptr(va), _ptr(colour), ncol, _ptr(m), _ptr(x), _ptr(bb),
+                        relax, int(backward))
+    return x
+
+
+def colour_conflicts(A, colour, mask=None):
+    """pairs (p, q), p != q, coupled by a nonzero of A (in either direction)
+    with colour[p] == colour[q] among the rows with mask != 0: a multicolour
+    sweep that updates one colour's rows simultaneously equals the sequential
+    sweep only when this is empty"""
+    A = A.tocoo()
+    colour = np.asarray(colour)
+    keep = (A.row != A.col) & (A.data != 0) & (colour[A.row] == colour[A.col])
+    if mask is not None:
+        mask = np.asarray(mask).astype(bool)
+        keep &= mask[A.row] & mask[A.col]
+    pairs = {(int(min(p, q)), int(max(p, q))) for p, q in zip(A.row[keep], A.col[keep])}
+    return sorted(pairs)
 
 
 class P1Op:
@@ -934,10 +985,13 @@ def gs_sweep(A, x, b, m):
 class P2GSGMG:
     """GeometricMultigridSolver<P2ConstantLaplaceOperator> with GaussSeidelSmoother, CG
     (no preconditioner) on the coarsest level and quadratic transfers
-    (P2GMG3DConvergenceTest.cpp:60-100, GeometricMultigridSolver.hpp:207-300)"""
+    (P2GMG3DConvergenceTest.cpp:60-100, GeometricMultigridSolver.hpp:207-300); the
+    sweeps run in canonical order, or colour by colour when `colours` maps each
+    level to a colour per DoF"""
 
-    def __init__(self, mesh, min_level, max_level, pre=2, post=2, coarse_iter=1000, coarse_tol=1e-14):
+    def __init__(self, mesh, min_level, max_level, pre=2, post=2, coarse_iter=1000, coarse_tol=1e-14, colours=None):
         self.m, self.minL, self.maxL, self.pre, self.post = mesh, min_level, max_level, pre, post
+        self.colours = colours
         self.coarse_iter, self.coarse_tol = coarse_iter, coarse_tol
         self.ops = {L: P2Op(mesh, L) for L in range(min_level, max_level + 1)}
         self.A = {L: self.ops[L].matrix() for L in self.ops}
@@ -954,7 +1008,7 @@ class P2GSGMG:
             self.ops[L].cg(x, b, GMG_FLAG, self.coarse_iter, self.coarse_tol, 1e-16, jacobi=False)
             return
         for _ in range(self.pre):
-            gs_sweep(self.A[L], x, b, self.mask[L])
+            self.sweep(L, x, b)
         r = self.residual(L, x, b)
         cb = np.zeros(self.ops[L - 1].n)
         self.m.p2_restrict(L, r, cb, GMG_FLAG)
@@ -962,4 +1016,10 @@ class P2GSGMG:
         self.cycle(L - 1, cx, cb)
         self.m.p2_prolongate(L - 1, cx, x, GMG_FLAG, True)
         for _ in range(self.post):
+            self.sweep(L, x, b)
+
+    def sweep(self, L, x, b):
+        if self.colours is None:
             gs_sweep(self.A[L], x, b, self.mask[L])
+        else:
+            sor_multicolour(self.A[L], self.colours[L], self.mask[L], x, b)

@@ -197,3 +197,128 @@ def test_gmg_converges_to_direct_solution():
     A = po.P1Op(m, L).csr()
     xs = sla.spsolve(A[I][:, I].tocsc(), b[I])
     assert relmax(x[I], xs) < 1e-10
+
+
+# ---- Gauss-Seidel orderings -------------------------------------------------------
+
+
+def _closure(m, L, p, q):
+    """q lies in the closure of p's owner primitive (its owner's vertices are a
+    subset of p's owner's vertices)"""
+    return set(m.owner_vertices(L, q)) <= set(m.owner_vertices(L, p))
+
+
+@pytest.mark.parametrize("name,L", [("cube", 3), ("octa", 2)])
+@pytest.mark.parametrize("relax", [1.0, 1.3])
+def test_lexicographic_sor_row_by_row(name, L, relax):
+    """or_p1_sor equals the row-by-row definition on the assembled matrix:
+    x_p <- (1 - w) x_p + w (b_p - sum_{q != p} a_pq x_q) / a_pp in canonical order
+    (macro vertices, edges, faces, then cell interiors), where a macro-vertex/
+    edge/face DoF reads the neighbours owned by other primitives outside its
+    owner's closure from the values at the sweep's start (HyTeG's ghost layers
+    are synchronised once before the sweep, P1Operator.hpp:373-412) and all
+    other neighbours at their latest values"""
+    m = mesh(name)
+    op = po.P1Op(m, L)
+    A = op.csr()
+    n = A.shape[0]
+    nb = m.nbnd(L)
+    fl = m.flags(L)
+    rng = np.random.default_rng(31 + L)
+    x0, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    # the owner-vertex sets locate each DoF: it is a positive combination of them
+    P = m.coords(L)
+    V = m.coords(0)
+    for p in range(0, n, max(1, n // 50)):
+        vs = list(m.owner_vertices(L, p))
+        M = np.vstack([V[vs].T, np.ones(len(vs))])
+        lam, *_ = np.linalg.lstsq(M, np.r_[P[p], 1.0], rcond=None)
+        assert np.abs(M @ lam - np.r_[P[p], 1.0]).max() < 1e-12 and (lam > 1e-12).all()
+    for flag in (po.GMG_FLAG, po.ALL):
+        x = x0.copy()
+        op.sor(x, b, flag, relax)
+        y = x0.copy()
+        start = x0.copy()
+        for p in range(n):
+            if not (fl[p] & flag):
+                continue
+            row = A.getrow(p)
+            s, diag = b[p], 0.0
+            for q, a in zip(row.indices, row.data):
+                if q == p:
+                    diag += a
+                    continue
+                v = start[q] if (p < nb and q < nb and not _closure(m, L, p, q)) else y[q]
+                s -= a * v
+            y[p] = (1 - relax) * y[p] + relax * s / diag
+        assert np.abs(x - y).max() < 1e-13 * max(1.0, np.abs(y).max()), (flag, relax)
+
+
+def _simultaneous_colour_sweep(A, colour, mask, x, b, relax, backward):
+    """every colour's rows updated at once from the values before that colour"""
+    D = A.diagonal()
+    order = sorted(set(colour[mask].tolist()), reverse=backward)
+    for c in order:
+        rows = np.where(mask & (colour == c))[0]
+        r = b[rows] - A[rows] @ x + D[rows] * x[rows]
+        x[rows] = (1 - relax) * x[rows] + relax * r / D[rows]
+    return x
+
+
+def _greedy(A, rows):
+    """greedy colouring of `rows` in their order over the coupling graph of A"""
+    col = {}
+    A = A.tocsr()
+    for p in rows:
+        used = {col[q] for q in A.indices[A.indptr[p]:A.indptr[p + 1]] if q in col and q != p}
+        c = 0
+        while c in used:
+            c += 1
+        col[p] = c
+    return col
+
+
+@pytest.mark.parametrize("name,L", [("cube", 3), ("octa", 3)])
+def test_multicolour_sweep_and_conflict_checker(name, L):
+    """the colour-order sweep of the oracle equals updating each colour's rows
+    simultaneously exactly when the colour-conflict checker finds no coupled pair
+    inside a colour: with the parity colours of the cell interiors and a greedy
+    colouring of the macro vertex/edge/face DoFs both hold, with two colours merged
+    the checker reports the coupled pairs and the sweeps differ"""
+    m = mesh(name)
+    A = po.P1Op(m, L).csr()
+    n = A.shape[0]
+    nb = m.nbnd(L)
+    g = _greedy(A, range(nb))
+    nbc = max(g.values()) + 1
+    colour = np.zeros(n, dtype=np.int32)
+    for p, c in g.items():
+        colour[p] = c
+    # cell-interior parity colours from the lattice coordinates of each cell
+    mm = 2 ** L
+    for c in range(m.counts()[3]):
+        for z in range(1, mm):
+            for y in range(1, mm - z):
+                for xx in range(1, mm - z - y):
+                    colour[m.locate(L, c, xx, y, z)] = nbc + (xx & 1) + 2 * (y & 1) + 4 * (z & 1)
+    assert po.colour_conflicts(A, colour) == []
+    # brute force: the checker's pairs are all coupled same-colour pairs
+    Ad = A.toarray()
+    rng = np.random.default_rng(7)
+    x0, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    mask = (m.flags(L) & po.GMG_FLAG) != 0
+    for relax in (1.0, 1.3):
+        for backward in (False, True):
+            x = po.sor_multicolour(A, colour, mask, x0.copy(), b, relax, backward)
+            y = _simultaneous_colour_sweep(A, colour, mask, x0.copy(), b, relax, backward)
+            assert np.abs(x - y).max() < 1e-13, (relax, backward)
+            assert np.array_equal(x[~mask], x0[~mask])
+    bad = colour.copy()
+    bad[bad == nbc + 1] = nbc  # merge two interior parity colours
+    pairs = po.colour_conflicts(A, bad)
+    brute = sorted({(min(p, q), max(p, q)) for p in range(n) for q in np.nonzero(Ad[p])[0]
+                    if p != q and bad[p] == bad[q]})
+    assert pairs == brute and len(pairs) > 0
+    x = po.sor_multicolour(A, bad, mask, x0.copy(), b, 1.0)
+    y = _simultaneous_colour_sweep(A, bad, mask, x0.copy(), b, 1.0, False)
+    assert np.abs(x - y).max() > 1e-6

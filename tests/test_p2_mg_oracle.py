@@ -170,3 +170,43 @@ def test_p2_vcycle_converges_on_shell():
     factors = hist[1:] / hist[:-1]
     assert hist[-1] <= 1e-8 * hist[0]
     assert factors.max() < 0.35, factors
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 5])
+def test_chebyshev_step_is_its_error_polynomial(order):
+    """one smoother step maps the error e = x - x* to P(D^-1 A) e with
+    P(t) = T_p((theta - t) / delta) / T_p(theta / delta), theta = (hi + lo) / 2,
+    delta = (hi - lo) / 2, on the interval [0.3 rho, 1.2 rho] the smoother is set up
+    with (ChebyshevSmoother.hpp:112-232, 246-330; Adams et al. 2003): the step is
+    checked on the assembled matrix of the shell's level-0 P2 operator through an
+    eigen-decomposition of D^-1/2 A D^-1/2, independently of the coefficients"""
+    m = po.Mesh.shell(**problems.SHELL)
+    k = po.KCoeff(*problems.shell_coefficient(0))
+    op = po.P2Op(m, 0, k)
+    fl = m.flags(1)
+    inv = op.inv_diag()
+    Ai, idx = _restricted_matrix(op, fl, po.GMG_FLAG)
+    d = inv[idx]
+    S = np.sqrt(d)[:, None] * Ai.toarray() * np.sqrt(d)[None, :]
+    lam, V = np.linalg.eigh(S)
+    rho = lam.max()
+    lo, hi = 0.3 * rho, 1.2 * rho
+    theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
+
+    def cheb(p, t):
+        return np.cos(p * np.arccos(t)) if abs(t) <= 1 else np.cosh(p * np.arccosh(abs(t))) * np.sign(t) ** p
+
+    P = np.array([cheb(order, (theta - t) / delta) for t in lam]) / cheb(order, theta / delta)
+    rng = np.random.default_rng(60 + order)
+    xs = np.zeros(op.n)
+    xs[idx] = rng.uniform(-1, 1, len(idx))
+    b = np.zeros(op.n)
+    b[idx] = Ai @ xs[idx]
+    x = np.zeros(op.n)
+    x[idx] = rng.uniform(-1, 1, len(idx))
+    e0 = (x - xs)[idx]
+    op.chebyshev_smooth(inv, po.chebyshev_coefficients(order, lo, hi), x, b)
+    # D^-1 A = D^-1/2 S D^1/2, so P(D^-1 A) e = D^1/2 V P(Lambda) V^T D^-1/2 e
+    e1 = np.sqrt(d) * (V @ (P * (V.T @ (e0 / np.sqrt(d)))))
+    assert np.abs((x - xs)[idx] - e1).max() < 1e-10 * np.abs(e0).max()
+    assert np.array_equal(x[~np.isin(np.arange(op.n), idx)], xs[~np.isin(np.arange(op.n), idx)])
