@@ -286,6 +286,12 @@ class P2DivKGradOperator:
         self.ctx.check(lib().hg_p2_smooth_jac(self.ctx.h, self.h, dst.h, rhs.h, src.h, relax, level, flag),
                        "smooth_jac")
 
+    def low_precision_form(self, reduced=True, from_level=2):
+        """element form of FP32 / FP16 applies: reduced (affine micro-cells, k constant,
+        4-point rule) on levels >= from_level, or the reference form"""
+        self.ctx.check(lib().hg_p2_low_precision_form(self.ctx.h, self.h, int(reduced), from_level),
+                       "low_precision_form")
+
     def store_coefficients(self, enable=True):
         """keep k at the quadrature points (default) or evaluate it in every apply"""
         self.ctx.check(lib().hg_p2_store_coefficients(self.ctx.h, self.h, int(enable)), "store_coefficients")

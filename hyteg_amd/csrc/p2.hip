@@ -148,6 +148,14 @@ struct P2Operator : OperatorBase
    std::map< int, DevBuf< double > > invDiag; // by P2 level, lattice L+1 storage
    // k at the quadrature points (tile-major), by (level, value type of the table)
    std::map< std::pair< int, int >, DevBuf< char > > kq;
+   // reduced element form of the FP32 / FP16 applies (the preconditioner of the
+   // mixed-precision refinement): on levels >= reducedFrom every micro-cell is
+   // integrated as the affine tetrahedron of its four blended vertices with k
+   // constant (its value at the blended centroid), by the 4-point degree-2 rule
+   bool                                   reduced     = true;
+   int                                    reducedFrom = 2;
+   std::map< int, DevBuf< __half > >      kc;       // k per micro-cell (tile-major), by P2 level
+   std::map< int, DevBuf< float > >       accum;    // FP32 accumulator of the FP16 applies, by P2 level
 };
 
 __device__ __forceinline__ double kEval( const KParams& K, const double x[3] )
@@ -323,6 +331,240 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileCoefficients( const P2C
    }
 }
 
+
+// ---- reduced element form (low-precision applies) ---------------------------
+// k at the blended centroid of every micro-cell, stored tile-major like the
+// quadrature-point table: table[tile * 384 + thread] (FP64 evaluation, FP16 value)
+__global__ __launch_bounds__( kTileThreads ) void kP2TileCentroidK( const P2CellGeom* __restrict__ geom,
+                                                                   const KParams* __restrict__ kp, int64_t mm,
+                                                                   int64_t ntiles, __half* __restrict__ table )
+{
+   const int64_t lb = blockIdx.x;
+   if ( lb >= ntiles )
+      return;
+   const TileCoord            tc = tileOf( lb, mm );
+   const P2CellGeom&          g  = geom[tc.c];
+   const TileThread< double > th = tileThread( g, tc, mm );
+   const double*              J  = g.J[th.t];
+   double                     x[3];
+   for ( int d = 0; d < 3; ++d )
+      x[d] = th.p0[d] + 0.25 * ( J[3 * d] + J[3 * d + 1] + J[3 * d + 2] );
+   const double ls = g.nl[0] * x[0] + g.nl[1] * x[1] + g.nl[2] * x[2] - g.lam0;
+   table[lb * kTileThreads + threadIdx.x] = __float2half( th.valid ? float( kAt( g, *kp, x, ls ) ) : 0.0f );
+}
+
+// the 4-point degree-2 rule: one barycentric coordinate kB, the others kA
+constexpr float kQ2A = 0.1381966011250105f, kQ2B = 0.5854101966249685f;
+
+// y += A_e u of the reduced form: constant metric G (scaled by k |det| w) and
+// the 4 points of the degree-2 rule, in the barycentric form of p2ElementAction
+__device__ __forceinline__ void p2ReducedAction( const float G[6], const float u[10], float y[10] )
+{
+#pragma unroll
+   for ( int q = 0; q < 4; ++q )
+   {
+      float l[4];
+#pragma unroll
+      for ( int k = 0; k < 4; ++k )
+         l[k] = k == q ? kQ2B : kQ2A;
+      const float g0 = ( 4.0f * l[0] - 1.0f ) * u[0] + 4.0f * ( l[1] * u[4] + l[2] * u[5] + l[3] * u[6] );
+      const float g1 = ( 4.0f * l[1] - 1.0f ) * u[1] + 4.0f * ( l[0] * u[4] + l[2] * u[7] + l[3] * u[8] );
+      const float g2 = ( 4.0f * l[2] - 1.0f ) * u[2] + 4.0f * ( l[0] * u[5] + l[1] * u[7] + l[3] * u[9] );
+      const float g3 = ( 4.0f * l[3] - 1.0f ) * u[3] + 4.0f * ( l[0] * u[6] + l[1] * u[8] + l[2] * u[9] );
+      const float d1 = g1 - g0, d2 = g2 - g0, d3 = g3 - g0;
+      const float e1 = G[0] * d1 + G[1] * d2 + G[2] * d3;
+      const float e2 = G[1] * d1 + G[3] * d2 + G[4] * d3;
+      const float e3 = G[2] * d1 + G[4] * d2 + G[5] * d3;
+      const float e0 = -( e1 + e2 + e3 );
+      y[0] += ( 4.0f * l[0] - 1.0f ) * e0;
+      y[1] += ( 4.0f * l[1] - 1.0f ) * e1;
+      y[2] += ( 4.0f * l[2] - 1.0f ) * e2;
+      y[3] += ( 4.0f * l[3] - 1.0f ) * e3;
+      y[4] += 4.0f * ( l[1] * e0 + l[0] * e1 );
+      y[5] += 4.0f * ( l[2] * e0 + l[0] * e2 );
+      y[6] += 4.0f * ( l[3] * e0 + l[0] * e3 );
+      y[7] += 4.0f * ( l[2] * e1 + l[1] * e2 );
+      y[8] += 4.0f * ( l[3] * e1 + l[1] * e3 );
+      y[9] += 4.0f * ( l[3] * e2 + l[2] * e3 );
+   }
+}
+
+// Reduced-form apply, FP32 arithmetic on FP32 or FP16 values.  A tile's source
+// box is staged in LDS, the blended positions of its 5x5x5 micro-vertices
+// (relative to its first one, from FP64 evaluations) too; every thread forms its
+// micro-cell's metric from those, integrates, and sums into the LDS box.  FP32
+// destinations receive the box with FP32 atomics.  FP16 destinations keep every
+// sum in FP32: box-interior points belong to this tile alone and are stored once
+// (or added to, rmw), box-surface points (a lattice coordinate divisible by 8,
+// shared with neighbouring tiles) are added with FP32 atomics into `accum`,
+// which kP2FinishAccum rounds into the FP16 destination afterwards.
+template < class T >
+__global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2CellGeom* __restrict__ geom,
+                                                                      const __half* __restrict__ kc, int64_t mm,
+                                                                      int64_t ntiles, int64_t sc1, int64_t w1,
+                                                                      const T* __restrict__ src, T* __restrict__ dst,
+                                                                      float* __restrict__ accum, int rmw, float alpha )
+{
+   __shared__ float sb[kBox3];
+   __shared__ float sv[( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 )][3];
+   const int64_t    lb = xcdBlock( ntiles );
+   if ( lb >= ntiles )
+      return;
+   const TileCoord tc   = tileOf( lb, mm );
+   const int64_t   base = tc.c * sc1;
+   const int32_t   top  = int32_t( 2 * mm );
+   const float     ke   = __half2float( kc[lb * kTileThreads + threadIdx.x] );
+   const T*        srcC = src + base;
+   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+   {
+      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
+      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
+      sb[p] = ( X + Y + Z <= top ) ? float( toAcc( srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] ) ) : 0.0f;
+   }
+   const P2CellGeom& g  = geom[tc.c];
+   constexpr int     nv = ( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 );
+   if ( threadIdx.x < nv )
+   {
+      // blended vertex positions relative to the tile's first vertex (FP64, then rounded)
+      auto blended = [&]( int i, int j, int k, double F[3] ) {
+         const double vx = double( tc.x0 + i ), vy = double( tc.y0 + j ), vz = double( tc.z0 + k );
+         double       x[3];
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            x[d] = g.v0[d] + g.hE[3 * d] * vx + g.hE[3 * d + 1] * vy + g.hE[3 * d + 2] * vz;
+         double q = 1.0;
+         if ( g.kind )
+         {
+            const double ls = g.nl[0] * x[0] + g.nl[1] * x[1] + g.nl[2] * x[2] - g.lam0;
+            q               = ( g.r0 + fabs( ls ) * g.dr ) / sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
+         }
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            F[d] = q * x[d];
+      };
+      const int i = threadIdx.x % ( kTile + 1 ), j = ( threadIdx.x / ( kTile + 1 ) ) % ( kTile + 1 ),
+                k = threadIdx.x / ( ( kTile + 1 ) * ( kTile + 1 ) );
+      double F[3], F0[3];
+      blended( i, j, k, F );
+      blended( 0, 0, 0, F0 );
+#pragma unroll
+      for ( int d = 0; d < 3; ++d )
+         sv[threadIdx.x][d] = float( F[d] - F0[d] );
+   }
+   __syncthreads();
+   const int t  = __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
+   const int a  = int( threadIdx.x ) & 63;
+   const int ax = a & 3, ay = ( a >> 2 ) & 3, az = a >> 4;
+   const bool valid = tc.x0 + ax + tc.y0 + ay + tc.z0 + az <= mm - 1 - MicroCells::rowShift[t];
+   float u[10], G[6];
+   int   pos[10];
+#pragma unroll
+   for ( int k = 0; k < 4; ++k )
+      pos[k] = 2 * ( ax + cVof[t][k][0] ) + kBox * ( 2 * ( ay + cVof[t][k][1] ) + kBox * 2 * ( az + cVof[t][k][2] ) );
+#pragma unroll
+   for ( int e = 0; e < 6; ++e )
+      pos[4 + e] = ( pos[cP2E[e][0]] + pos[cP2E[e][1]] ) / 2;
+#pragma unroll
+   for ( int i = 0; i < 10; ++i )
+      u[i] = valid ? sb[pos[i]] : 0.0f;
+   {
+      // edge vectors c_k = P_k - P_0 of the blended vertices, n_k their cross
+      // products (det J * rows of J^-1), G = (k w / |det J|) n_i . n_j
+      float P[4][3];
+#pragma unroll
+      for ( int k = 0; k < 4; ++k )
+      {
+         const int vi = ( ax + cVof[t][k][0] ) + ( kTile + 1 ) * ( ( ay + cVof[t][k][1] ) + ( kTile + 1 ) * ( az + cVof[t][k][2] ) );
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            P[k][d] = sv[vi][d];
+      }
+      float c[3][3], n[3][3];
+#pragma unroll
+      for ( int k = 0; k < 3; ++k )
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            c[k][d] = P[k + 1][d] - P[0][d];
+#pragma unroll
+      for ( int k = 0; k < 3; ++k )
+      {
+         const float* b1 = c[( k + 1 ) % 3];
+         const float* b2 = c[( k + 2 ) % 3];
+         n[k][0]         = b1[1] * b2[2] - b1[2] * b2[1];
+         n[k][1]         = b1[2] * b2[0] - b1[0] * b2[2];
+         n[k][2]         = b1[0] * b2[1] - b1[1] * b2[0];
+      }
+      const float det = c[0][0] * n[0][0] + c[0][1] * n[0][1] + c[0][2] * n[0][2];
+      const float s   = ( ke * ( 1.0f / 24.0f ) ) * __builtin_amdgcn_rcpf( fabsf( det ) );
+      G[0]            = s * ( n[0][0] * n[0][0] + n[0][1] * n[0][1] + n[0][2] * n[0][2] );
+      G[1]            = s * ( n[0][0] * n[1][0] + n[0][1] * n[1][1] + n[0][2] * n[1][2] );
+      G[2]            = s * ( n[0][0] * n[2][0] + n[0][1] * n[2][1] + n[0][2] * n[2][2] );
+      G[3]            = s * ( n[1][0] * n[1][0] + n[1][1] * n[1][1] + n[1][2] * n[1][2] );
+      G[4]            = s * ( n[1][0] * n[2][0] + n[1][1] * n[2][1] + n[1][2] * n[2][2] );
+      G[5]            = s * ( n[2][0] * n[2][0] + n[2][1] * n[2][1] + n[2][2] * n[2][2] );
+   }
+   __syncthreads();
+   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+      sb[p] = 0.0f;
+   __syncthreads();
+   if ( valid )
+   {
+      float y[10];
+#pragma unroll
+      for ( int i = 0; i < 10; ++i )
+         y[i] = 0.0f;
+      p2ReducedAction( G, u, y );
+#pragma unroll
+      for ( int i = 0; i < 10; ++i )
+         atomicAdd( &sb[pos[i]], y[i] );
+   }
+   __syncthreads();
+   T* dstC = dst + base;
+   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+   {
+      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
+      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
+      if ( X + Y + Z > top )
+         continue;
+      const int64_t o = tetRowStart32( int32_t( w1 ), Y, Z ) + X;
+      const float   v = alpha * sb[p];
+      if constexpr ( std::is_same< T, float >::value )
+         atomicAdd( dstC + o, v );
+      else
+      {
+         const bool surface = lx == 0 || ly == 0 || lz == 0 || lx == kBox - 1 || ly == kBox - 1 || lz == kBox - 1;
+         if ( surface )
+            atomicAdd( accum + base + o, v );
+         else
+            dstC[o] = __float2half( rmw ? __half2float( dstC[o] ) + v : v );
+      }
+   }
+}
+
+// FP16 destinations of kP2TileApplyReduced: the tile-surface points (a lattice
+// coordinate divisible by 8) receive their FP32 sums, which are reset to zero.
+// One block per (lattice slice Z, cell).
+__global__ void kP2FinishAccum( int64_t sc1, int32_t w1, int rmw, float* __restrict__ accum, __half* __restrict__ dst )
+{
+   const int32_t Z  = int32_t( blockIdx.x );
+   const int64_t c  = blockIdx.y;
+   const int32_t wz = w1 - Z;
+   if ( wz <= 0 )
+      return;
+   for ( int32_t Y = 0; Y < wz; ++Y )
+   {
+      const int64_t start = c * sc1 + tetRowStart32( w1, Y, Z );
+      const int32_t len   = wz - Y;
+      const int32_t step  = ( Z % 8 == 0 || Y % 8 == 0 ) ? 1 : 8; // whole row, or the X divisible by 8
+      for ( int32_t X = int32_t( threadIdx.x ) * step; X < len; X += int32_t( blockDim.x ) * step )
+      {
+         const int64_t o = start + X;
+         const float   v = accum[o];
+         accum[o]        = 0.0f;
+         dst[o]          = __float2half( rmw ? __half2float( dst[o] ) + v : v );
+      }
+   }
+}
 
 // y = A_e u for the micro-cell of th: the element action of
 // p2_div_k_grad_blending_q4 in barycentric form (11-point rule); kq(q) gives k at
@@ -670,6 +912,63 @@ static const P2CellGeomT< typename Acc< T >::type >* p2Geometry( P2Operator& op,
       return op.tgeomF.at( L ).p;
 }
 
+// k at the blended centroids of the micro-cells of level L (tile-major, FP16),
+// built on first use
+static const __half* p2CentroidK( Context* ctx, P2Operator& op, int L )
+{
+   auto it = op.kc.find( L );
+   if ( it != op.kc.end() )
+      return it->second.p;
+   const int64_t mm     = int64_t( 1 ) << L;
+   const int64_t ntiles = tilesPerCell( mm ) * int64_t( op.st->topo.nc );
+   auto&         buf    = op.kc[L];
+   buf.alloc( ctx, size_t( ntiles ) * kTileThreads );
+   kP2TileCentroidK<<< gridFor( ntiles, 1 ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, mm, ntiles,
+                                                                               buf.p );
+   HG_CHECK_HIP( hipGetLastError() );
+   return buf.p;
+}
+
+// the reduced-form apply on a tile level (FP32 arithmetic): the cell-lattice
+// part; the additive push to the owners follows in p2Apply
+template < class T >
+static void p2ApplyReduced( Context* ctx, P2Operator& op, Lattice& lat, int L, const T* s, T* d, int flag, int update,
+                            double alpha )
+{
+   const int64_t mm     = int64_t( 1 ) << L;
+   const int64_t ntiles = tilesPerCell( mm ) * lat.nc;
+   // a Replace that includes the cell interiors overwrites every cell-lattice
+   // point; otherwise the halo copies start from zero and the rest is added to
+   const int     rmw    = !( update == HG_REPLACE && ( flag & kInner ) );
+   Storage&      st     = *op.st;
+   if ( rmw && lat.haloPerCell() > 0 )
+      kZeroHalos< T ><<< gridFor( lat.haloPerCell() * lat.nc ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lat ),
+                                                                                         lat.nc, d );
+   else if ( !rmw && std::is_same< T, float >::value )
+      kFillRange< T ><<< gridFor( lat.nc * lat.sc ), kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, d, 0.0 );
+   HG_CHECK_HIP( hipGetLastError() );
+   const __half* kc    = p2CentroidK( ctx, op, L );
+   float*        accum = nullptr;
+   if constexpr ( std::is_same< T, __half >::value )
+   {
+      auto& buf = op.accum[L];
+      if ( !buf.p )
+         buf.alloc( ctx, size_t( lat.nc * lat.sc ) ); // zero-filled once; every finish pass resets what it read
+      accum = buf.p;
+   }
+   ctx->beginTiming();
+   kP2TileApplyReduced< T ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+       op.tgeom.at( L ).p, kc, mm, ntiles, lat.sc, lat.w, s, d, accum, rmw, float( alpha ) );
+   HG_CHECK_HIP( hipGetLastError() );
+   ctx->endTiming();
+   if constexpr ( std::is_same< T, __half >::value )
+   {
+      kP2FinishAccum<<< dim3( unsigned( lat.w ), unsigned( lat.nc ) ), 256, 0, ctx->stream >>>( lat.sc, int32_t( lat.w ),
+                                                                                               rmw, accum, d );
+      HG_CHECK_HIP( hipGetLastError() );
+   }
+}
+
 // dst = alpha A src (update Replace) or dst + alpha A src (update Add) on the
 // DoFs of type flag: P2ElementwiseOperator::gemv with beta 0 or 1
 // (P2ElementwiseOperator.cpp:110-233)
@@ -694,6 +993,14 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
       using T = decltype( tag );
       // FP64 functions use an FP64 table, FP32 an FP32 one, FP16 an FP16 one
       using K = T;
+      if constexpr ( !std::is_same< T, double >::value )
+      {
+         if ( op.reduced && L >= op.reducedFrom && mm > kSmallMaxMM )
+         {
+            p2ApplyReduced< T >( ctx, op, lat, L, (const T*) s, (T*) d, flag, update, alpha );
+            return;
+         }
+      }
       // zero the halos (all) and, for Replace with Inner in flag, the cell interiors
       if ( update == HG_REPLACE && ( flag & kInner ) )
          kFillRange< T ><<< gridFor( lat.nc * lat.sc ), kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, (T*) d, 0.0 );
@@ -1041,6 +1348,17 @@ int hg_p2_smooth_jac( hg_ctx* h, hg_handle opH, hg_handle dst, hg_handle rhs, hg
       p2Apply( ctx, op, s, d, level, flag, HG_REPLACE );
       jacobiUpdate( ctx, op.st->lattice( ctx, level + 1 ), d.dtype, 0, flag, relax, r.ptr( level ), s.ptr( level ), inv,
                     d.ptr( level ) );
+   } );
+}
+
+int hg_p2_low_precision_form( hg_ctx* h, hg_handle op, int reduced, int fromLevel )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( fromLevel < 0 )
+         throw Error( HG_ERR_ARG, "low_precision_form: negative level" );
+      P2Operator& o = ctx->op< P2Operator >( op );
+      o.reduced     = reduced != 0;
+      o.reducedFrom = fromLevel;
    } );
 }
 

@@ -58,10 +58,11 @@ struct Request
 
 struct Results
 {
-   double* latency;
-   double* relres;
-   double* checksum;
-   int*    cycles;
+   double*        latency;
+   double*        relres;
+   double*        checksum;
+   int*           cycles;
+   double* const* solutions; // per request: host buffer for the canonical solution, or null
 };
 
 thread_local std::string tlsError;
@@ -254,7 +255,7 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
       const uint_t L = uint_t( level );
       P1ConstantLaplaceOperator A( st, 0, L );
       P1Function< double >      X( "x", st, 0, L, S ), B( "b", st, 0, L, S ), R( "r", st, L, L, S );
-      P1Function< double >      one( "rhs", st, L, L ), zero( "zero", st, L, L );
+      P1Function< double >      one( "rhs", st, L, L ), zero( "zero", st, L, L ), out( "out", st, L, L );
       auto smoother = std::make_shared< GaussSeidelSmoother< P1ConstantLaplaceOperator > >();
       auto coarse   = std::make_shared< SmootherSweeps< P1ConstantLaplaceOperator > >( smoother, 30 );
       GeometricMultigridSolver< P1ConstantLaplaceOperator > gmg(
@@ -336,6 +337,12 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
             const std::vector< double > xx = X.dotGlobalBatched( X, L, All );
             for ( int k : done )
             {
+               if ( sv.res.solutions && sv.res.solutions[slot[k].idx] )
+               {
+                  out.copyVector( 0, X, k, L );
+                  check( ctx->get(), hg_function_download( ctx->get(), out.handle(), int( L ), sv.res.solutions[slot[k].idx] ),
+                         "solution" );
+               }
                sv.finish( slot[k], r0[k] > 0 ? std::sqrt( rr[k] ) / r0[k] : 0.0, xx[k], cyc[k] );
                B.copyVector( k, zero, 0, L );
                X.copyVector( k, zero, 0, L );
@@ -431,6 +438,9 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
          }
          if ( graph )
             check( ctx->get(), hg_graph_destroy( ctx->get(), graph ), "graph" );
+         if ( sv.res.solutions && sv.res.solutions[rq.idx] )
+            check( ctx->get(), hg_function_download( ctx->get(), x.handle(), int( L ), sv.res.solutions[rq.idx] ),
+                   "solution" );
          sv.finish( rq, r0 > 0 ? rk / r0 : 0.0, x.dotGlobal( x, L, All ), k );
       }
    } catch ( const std::exception& e )
@@ -481,9 +491,30 @@ const char* hg_serve_last_error( void )
    return tlsError.c_str();
 }
 
+int hg_serve_request_coefficient( uint64_t seed, double* a16, double* omega48, double* phi16 )
+{
+   if ( !a16 || !omega48 || !phi16 )
+      return HG_ERR_ARG;
+   std::vector< double > a, om, ph;
+   coefficientOf( seed, a, om, ph );
+   std::copy( a.begin(), a.end(), a16 );
+   std::copy( om.begin(), om.end(), omega48 );
+   std::copy( ph.begin(), ph.end(), phi16 );
+   return HG_OK;
+}
+
 int hg_serve_stream( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
                      const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
                      double* rel_residual, double* checksum, int* cycles, double* wall_s )
+{
+   return hg_serve_stream_solutions( device, n, kind, level, seed, arrival_s, tol, slots, memBudgetGB, latency_s,
+                                     rel_residual, checksum, cycles, wall_s, nullptr );
+}
+
+int hg_serve_stream_solutions( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
+                               const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
+                               double* rel_residual, double* checksum, int* cycles, double* wall_s,
+                               double* const* solutions )
 {
    tlsError.clear();
    if ( n < 0 || !kind || !level || !seed || !arrival_s || !latency_s || !rel_residual || !checksum || !cycles ||
@@ -507,7 +538,7 @@ int hg_serve_stream( int device, int64_t n, const int* kind, const int* level, c
    sv.slots     = slots;
    sv.device    = device;
    sv.memBudget = int64_t( memBudgetGB * 1e9 );
-   sv.res       = Results{ latency_s, rel_residual, checksum, cycles };
+   sv.res       = Results{ latency_s, rel_residual, checksum, cycles, solutions };
    std::vector< int64_t > order( static_cast< size_t >( n ) );
    for ( int64_t i = 0; i < n; ++i )
       order[size_t( i )] = i;

@@ -38,7 +38,33 @@ def make_stream(n, rate, seed=0):
     return kinds, levels, seeds, arrivals
 
 
-def serve_stream(kinds, levels, seeds, arrivals, tol=1e-8, slots=32, mem_budget_gb=200.0, device=0):
+# macro-primitive counts (vertices, edges, faces, cells) of the two meshes
+MESH_COUNTS = {"cube": (8, 19, 18, 6), "shell": (486, 2724, 4160, 1920)}
+
+
+def ndofs(kind, level):
+    """canonical DoFs of a request's solution: P1 DoFs of `level`, or P2 DoFs of
+    `level` (= P1 DoFs of level + 1)"""
+    nv, ne, nf, nc = MESH_COUNTS["cube" if kind in (P1_CUBE, P2_CUBE) else "shell"]
+    m = 2 ** (level + (1 if kind in (P2_SHELL, P2_CUBE) else 0))
+    tri = lambda k: k * (k + 1) // 2 if k > 0 else 0  # noqa: E731
+    tet = lambda k: k * (k + 1) * (k + 2) // 6 if k > 0 else 0  # noqa: E731
+    return nv + ne * (m - 1) + nf * tri(m - 2) + nc * tet(m - 3)
+
+
+def request_coefficient(seed):
+    """(a, omega, phi) of k(x) = exp(sum_m a_m sin(omega_m . x + phi_m)) of a diffusion request"""
+    a, om, ph = np.zeros(16), np.zeros(48), np.zeros(16)
+    p = lambda v: v.ctypes.data_as(C.c_void_p)  # noqa: E731
+    if lib().hg_serve_request_coefficient(C.c_uint64(int(seed)), p(a), p(om), p(ph)) != 0:
+        raise HygError("hg_serve_request_coefficient failed")
+    return a, om.reshape(16, 3), ph
+
+
+def serve_stream(kinds, levels, seeds, arrivals, tol=1e-8, slots=32, mem_budget_gb=200.0, device=0,
+                 solutions=False):
+    """runs the stream; with solutions=True the result also holds every request's
+    solution (canonical FP64 array) under the key solutions"""
     n = len(kinds)
     kinds = np.ascontiguousarray(kinds, dtype=np.int32)
     levels = np.ascontiguousarray(levels, dtype=np.int32)
@@ -48,11 +74,18 @@ def serve_stream(kinds, levels, seeds, arrivals, tol=1e-8, slots=32, mem_budget_
     cyc = np.zeros(n, dtype=np.int32)
     wall = C.c_double()
     p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    st = lib().hg_serve_stream(device, n, p(kinds), p(levels), p(seeds), p(arrivals), tol, slots, mem_budget_gb,
-                               p(lat), p(rel), p(chk), p(cyc), C.byref(wall))
+    sols, ptrs = None, None
+    if solutions:
+        sols = [np.zeros(ndofs(int(k), int(lv))) for k, lv in zip(kinds, levels)]
+        ptrs = (C.c_void_p * n)(*[s.ctypes.data for s in sols])
+    st = lib().hg_serve_stream_solutions(device, n, p(kinds), p(levels), p(seeds), p(arrivals), tol, slots,
+                                         mem_budget_gb, p(lat), p(rel), p(chk), p(cyc), C.byref(wall), ptrs)
     if st != 0:
         raise HygError(f"hg_serve_stream: status {st}: {lib().hg_serve_last_error().decode()}")
-    return {"latency_s": lat, "rel_residual": rel, "checksum": chk, "cycles": cyc, "wall_s": wall.value}
+    out = {"latency_s": lat, "rel_residual": rel, "checksum": chk, "cycles": cyc, "wall_s": wall.value}
+    if solutions:
+        out["solutions"] = sols
+    return out
 
 
 KIND_NAMES = {P1_CUBE: "p1_cube", P1_SHELL: "p1_shell", P2_SHELL: "p2_shell", P2_CUBE: "p2_cube"}

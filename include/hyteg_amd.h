@@ -281,6 +281,16 @@ int hg_p2_smooth_jac( hg_ctx* ctx, hg_handle op, hg_handle dst, hg_handle rhs, h
  * apply of a level.  enable = 1 (default) stores, 0 evaluates k in every apply
  * and frees the tables.  Results are identical either way. */
 int hg_p2_store_coefficients( hg_ctx* ctx, hg_handle op, int enable );
+/* Element form of the applies on FP32 and FP16 functions (the preconditioner of the
+ * mixed-precision refinement; no counterpart in the reference).  reduced = 1 (the
+ * default, from level 2): on levels >= fromLevel every micro-cell is integrated as
+ * the affine tetrahedron of its four blended vertices, with k constant on it (its
+ * value at the blended centroid, stored in FP16) and the 4-point degree-2 rule, in
+ * FP32 arithmetic with FP32 accumulation (FP16 results are rounded once); the
+ * operator differs from the reference's by O(h) (0.2 % of an element matrix on the
+ * shell at level 6).  reduced = 0: the reference form of hg_p2_apply in FP32
+ * arithmetic on every level.  FP64 applies always use the reference form. */
+int hg_p2_low_precision_form( hg_ctx* ctx, hg_handle op, int reduced, int fromLevel );
 /* OperatorWithInverseDiagonal::getInverseDiagonalValues on `level`, written
  * into the P2 function dst (P2ElementwiseOperator.cpp:418-568) */
 int hg_p2_inverse_diagonal( hg_ctx* ctx, hg_handle op, int level, hg_handle dst );
@@ -393,6 +403,16 @@ int hg_p1_batched_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b,
 int hg_serve_stream( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
                      const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
                      double* rel_residual, double* checksum, int* cycles, double* wall_s );
+/* hg_serve_stream that also copies request i's solution (canonical order, FP64, the
+ * ndofs of its kind and level: P1 DoFs of `level` for kinds 0/1, P2 DoFs for 2/3)
+ * into solutions[i] when that pointer is not null (solutions itself may be null) */
+int hg_serve_stream_solutions( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
+                               const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
+                               double* rel_residual, double* checksum, int* cycles, double* wall_s,
+                               double* const* solutions );
+/* the coefficient k(x) = exp(sum_m a_m sin(omega_m . x + phi_m)), m < 16, the stream
+ * generates for a diffusion request of the given seed */
+int hg_serve_request_coefficient( uint64_t seed, double* a16, double* omega48, double* phi16 );
 /* message of the last hg_serve_stream failure on this thread */
 const char* hg_serve_last_error( void );
 

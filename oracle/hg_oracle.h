@@ -112,6 +112,13 @@ void or_quadrature( double* pts44, double* w11 );
 void or_p2_element_matrix( const double* p, const double* blend, int nm, const double* ka, const double* kom,
                            const double* kph, double* A );
 
+/* The reduced element form of the low-precision applies: the affine
+ * tetrahedron of the four blended vertices, k constant (k at the blended
+ * centroid, rounded to binary16), 4-point degree-2 rule */
+void or_p2_reduced_element_matrix( const double* p, const double* blend, int nm, const double* ka, const double* kom,
+                                   const double* kph, double* A );
+/* the operator's element matrices: 0 the reference form (default), 1 the reduced one */
+void or_p2op_set_reduced( or_p2op* op, int on );
 /* P2ElementwiseOperator<p2_div_k_grad_blending_q4> on P2 level `level`; its
  * DoFs are the canonical P1 DoFs of level+1 */
 or_p2op* or_p2op_create( const or_mesh* m, int level, int nm, const double* ka, const double* kom, const double* kph );

@@ -84,6 +84,7 @@ struct or_p2op
    int64_t*       cmap; /* nc * sc1 */
    kfun_t         K;
    int32_t*       flags;
+   int            reduced; /* element matrices of or_p2_reduced_element_matrix */
 };
 
 lvl_t    or_lvl_info( const or_mesh* m, int L );

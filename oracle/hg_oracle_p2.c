@@ -494,6 +494,71 @@ void or_p2_element_matrix( const double* p /*4x3*/, const double* blend /*14 or 
    }
 }
 
+/* binary16 rounding (round to nearest, ties to even) of a positive value in
+ * the normal binary16 range: 11 significant bits */
+static double round_half( double v )
+{
+   int          e;
+   const double m = frexp( v, &e ); /* v = m 2^e, m in [0.5, 1) */
+   return ldexp( nearbyint( ldexp( m, 11 ) ), e - 11 );
+}
+
+/* The reduced element form of the low-precision applies (the preconditioner
+ * inside mixed-precision refinement; no counterpart in the reference): the
+ * micro-cell is the affine tetrahedron of its four blended vertices F(p_k), k
+ * is constant on it, k(F(x_c)) at the blended centroid rounded to binary16,
+ * and the P2 stiffness of that tetrahedron is integrated by the 4-point
+ * degree-2 rule (exact for it). */
+void or_p2_reduced_element_matrix( const double* p, const double* blend, int nm, const double* ka, const double* kom,
+                                   const double* kph, double* A )
+{
+   kfun_t K = { nm, (double*) ka, (double*) kom, (double*) kph };
+   double P[4][3], xc[3] = { 0, 0, 0 }, Fc[3];
+   for ( int k = 0; k < 4; ++k )
+   {
+      double DF[9];
+      if ( blend )
+         or_shell_map( blend, p + 3 * k, P[k], DF );
+      else
+         memcpy( P[k], p + 3 * k, sizeof( P[k] ) );
+      for ( int d = 0; d < 3; ++d )
+         xc[d] += 0.25 * p[3 * k + d];
+   }
+   if ( blend )
+   {
+      double DF[9];
+      or_shell_map( blend, xc, Fc, DF );
+   }
+   else
+      memcpy( Fc, xc, sizeof( Fc ) );
+   /* rounded to binary32, then to binary16: the conversion the GPU table uses */
+   const double kc = round_half( (double) (float) kfun( &K, Fc ) );
+   double       J[9], Ji[9], detJ;
+   for ( int r = 0; r < 3; ++r )
+      for ( int k = 0; k < 3; ++k )
+         J[3 * r + k] = P[k + 1][r] - P[0][r];
+   inv3( J, Ji, &detJ );
+   for ( int i = 0; i < 100; ++i )
+      A[i] = 0.0;
+   const double a = 0.1381966011250105, b = 0.5854101966249685;
+   for ( int q = 0; q < 4; ++q )
+   {
+      double lam[4] = { a, a, a, a };
+      lam[q]        = b;
+      double g[10][3], pg[10][3];
+      p2_ref_grads( lam, g );
+      for ( int i = 0; i < 10; ++i )
+         for ( int d = 0; d < 3; ++d )
+            pg[i][d] = Ji[d] * g[i][0] + Ji[3 + d] * g[i][1] + Ji[6 + d] * g[i][2];
+      const double wq = ( 1.0 / 24.0 ) * kc * fabs( detJ );
+      for ( int i = 0; i < 10; ++i )
+         for ( int j = 0; j < 10; ++j )
+            A[10 * i + j] += wq * ( pg[i][0] * pg[j][0] + pg[i][1] * pg[j][1] + pg[i][2] * pg[j][2] );
+   }
+}
+
+void or_p2op_set_reduced( or_p2op* op, int on ) { op->reduced = on; }
+
 /* ------------------------------------------------------------------------- */
 /* P2 elementwise operator                                                    */
 /* ------------------------------------------------------------------------- */
@@ -579,7 +644,10 @@ static void for_each_element_in( const or_p2op* op, int c0, int c1, elem_cb f, v
                      const int64_t* b = v[P2E[e][1]];
                      ids[4 + e]       = op->cmap[c * op->sc1 + tet_index( op->w1, a[0] + b[0], a[1] + b[1], a[2] + b[2] )];
                   }
-                  or_p2_element_matrix( p, B, op->K.nm, op->K.a, op->K.omega, op->K.phi, A );
+                  if ( op->reduced )
+                     or_p2_reduced_element_matrix( p, B, op->K.nm, op->K.a, op->K.omega, op->K.phi, A );
+                  else
+                     or_p2_element_matrix( p, B, op->K.nm, op->K.a, op->K.omega, op->K.phi, A );
                   f( ctx, ids, A );
                }
       }

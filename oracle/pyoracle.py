@@ -64,6 +64,8 @@ def lib():
             "or_shell_map": (None, [_p, _p, _p, _p]),
             "or_quadrature": (None, [_p, _p]),
             "or_p2_element_matrix": (None, [_p, _p, _i, _p, _p, _p, _p]),
+            "or_p2_reduced_element_matrix": (None, [_p, _p, _i, _p, _p, _p, _p]),
+            "or_p2op_set_reduced": (None, [_p, _i]),
             "or_p2op_create": (_p, [_p, _i, _i, _p, _p, _p]),
             "or_p2op_free": (None, [_p]),
             "or_p2_apply": (None, [_p, _p, _p, _i, _i]),
@@ -317,21 +319,27 @@ def quadrature():
     return pts, w
 
 
-def p2_element_matrix(p, blend=None, k=None):
+def p2_element_matrix(p, blend=None, k=None, reduced=False):
+    """10x10 element matrix of the reference form, or (reduced) of the reduced form
+    of the low-precision applies"""
     p = np.ascontiguousarray(p, dtype=np.float64)
     k = k or KCoeff()
     A = np.zeros((10, 10))
     b = None if blend is None else np.ascontiguousarray(blend, dtype=np.float64)
-    lib().or_p2_element_matrix(_ptr(p), None if b is None else _ptr(b), *k.args(), _ptr(A))
+    f = lib().or_p2_reduced_element_matrix if reduced else lib().or_p2_element_matrix
+    f(_ptr(p), None if b is None else _ptr(b), *k.args(), _ptr(A))
     return A
 
 
 class P2Op:
-    def __init__(self, mesh, level, k=None):
+    def __init__(self, mesh, level, k=None, reduced=False):
+        """reduced: element matrices of the reduced form of the low-precision applies"""
         self.mesh = mesh
         self.level = level
         self.k = k or KCoeff()
         self.h = lib().or_p2op_create(mesh.h, level, *self.k.args())
+        if reduced:
+            lib().or_p2op_set_reduced(self.h, 1)
         self.n = mesh.ndofs(level + 1)
 
     def __del__(self):

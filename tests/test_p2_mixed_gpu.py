@@ -1,6 +1,12 @@
-"""Reduced-precision P2 path (FP32 arithmetic, FP32 / FP16 storage) and the FP64
-iterative refinement around it, checked against the FP64 HIP path (which the
-other GPU tests pin to the oracle) and the oracle's FP64 solution."""
+"""Reduced-precision P2 path (FP32 arithmetic, FP32 / FP16 storage, FP32
+accumulation) and the FP64 iterative refinement around it:
+
+* on tile levels (>= 2) the FP32 / FP16 applies use the reduced element form
+  (affine micro-cells of the blended vertices, k constant per micro-cell, 4-point
+  rule), checked against the oracle's restatement of that form; below, and with
+  the form switched off, the reference form in FP32 arithmetic, checked against
+  the FP64 HIP path (which the other GPU tests pin to the oracle);
+* the refinement reaches the oracle's FP64 solution."""
 import numpy as np
 import pytest
 
@@ -11,7 +17,8 @@ from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
-# relative max-norm bounds of one reduced-precision apply against FP64
+# relative max-norm bounds of one reduced-precision apply against the FP64 result of
+# the same element form (FP16: input and output rounded to binary16)
 APPLY_TOL = {hg.F32: 2e-6, hg.F16: 4e-3}
 
 
@@ -21,30 +28,65 @@ def shell(ctx):
 
 
 @pytest.mark.parametrize("dtype", [hg.F32, hg.F16])
-def test_low_precision_apply_and_transfer(ctx, shell, dtype):
-    L = 2
-    A = hg.P2DivKGradOperator(shell, 1, L, *problems.shell_coefficient(0))
-    n = shell.p2_ndofs(L)
-    x = problems.canonical_uniform(n, 21)
-    ref = {}
-    for dt in (hg.F64, dtype):
-        src = hg.P2Function("src", shell, 1, L, dtype=dt)
-        dst = hg.P2Function("dst", shell, 1, L, dtype=dt)
-        src.upload(L, x)
-        A.apply(src, dst, L, hg.ALL)
-        hg.restrict(src, L, hg.GMG_FLAG)
-        ref[dt] = (dst.download(L), src.download(L - 1))
-    print(f"dtype {dtype}: apply {relmax(ref[dtype][0], ref[hg.F64][0]):.2e}", flush=True)
-    assert relmax(ref[dtype][0], ref[hg.F64][0]) < APPLY_TOL[dtype]
-    assert relmax(ref[dtype][1], ref[hg.F64][1]) < APPLY_TOL[dtype]
-    if dtype == hg.F32:
-        # stored and in-kernel coefficients agree in FP32 as well
-        A.store_coefficients(False)
+@pytest.mark.parametrize("flag,update", [(hg.ALL, hg.REPLACE), (hg.GMG_FLAG, hg.REPLACE), (hg.ALL, hg.ADD),
+                                         (hg.DIRICHLET, hg.REPLACE)])
+def test_reduced_form_apply_matches_the_oracle(ctx, shell, dtype, flag, update):
+    k = problems.shell_coefficient(0)
+    om = po.Mesh.shell(**problems.SHELL)
+    for L in (2, 3):
+        A = hg.P2DivKGradOperator(shell, L, L, *k)
+        n = shell.p2_ndofs(L)
+        x = problems.canonical_uniform(n, 21)
+        y0 = problems.canonical_uniform(n, 22)
         src = hg.P2Function("src", shell, L, L, dtype=dtype)
         dst = hg.P2Function("dst", shell, L, L, dtype=dtype)
         src.upload(L, x)
-        A.apply(src, dst, L, hg.ALL)
-        assert relmax(dst.download(L), ref[dtype][0]) < 1e-5
+        dst.upload(L, y0)
+        xs, ys = src.download(L), dst.download(L)  # as stored (binary16 for FP16)
+        print(f"reduced apply dtype {dtype} level {L} flag {flag} update {update}", flush=True)
+        A.apply(src, dst, L, flag, update)
+        ref = ys.copy()
+        po.P2Op(om, L, po.KCoeff(*k), reduced=True).apply(xs, ref, flag, update == hg.ADD)
+        err = relmax(dst.download(L), ref)
+        print(f"   relative difference to the oracle's reduced form {err:.2e}", flush=True)
+        assert err < APPLY_TOL[dtype]
+        for f in (src, dst):
+            f.destroy()
+        A.destroy()
+
+
+@pytest.mark.parametrize("dtype", [hg.F32, hg.F16])
+def test_reference_form_in_low_precision_and_transfer(ctx, shell, dtype):
+    """level 1 (below the tile levels) and, with the reduced form switched off,
+    level 2: the reference form in FP32 arithmetic agrees with FP64"""
+    k = problems.shell_coefficient(0)
+    for L, switch_off in ((1, False), (2, True)):
+        A = hg.P2DivKGradOperator(shell, 1, L, *k)
+        if switch_off:
+            A.low_precision_form(False)
+        n = shell.p2_ndofs(L)
+        x = problems.canonical_uniform(n, 21)
+        ref = {}
+        for dt in (hg.F64, dtype):
+            src = hg.P2Function("src", shell, 1, L, dtype=dt)
+            dst = hg.P2Function("dst", shell, 1, L, dtype=dt)
+            src.upload(L, x)
+            A.apply(src, dst, L, hg.ALL)
+            if L > 1:
+                hg.restrict(src, L, hg.GMG_FLAG)
+            ref[dt] = (dst.download(L), src.download(L - 1) if L > 1 else None)
+        print(f"dtype {dtype} level {L}: apply {relmax(ref[dtype][0], ref[hg.F64][0]):.2e}", flush=True)
+        assert relmax(ref[dtype][0], ref[hg.F64][0]) < APPLY_TOL[dtype]
+        if L > 1:
+            assert relmax(ref[dtype][1], ref[hg.F64][1]) < APPLY_TOL[dtype]
+        if dtype == hg.F32 and switch_off:
+            # stored and in-kernel coefficients agree in FP32 as well
+            A.store_coefficients(False)
+            src = hg.P2Function("src", shell, L, L, dtype=dtype)
+            dst = hg.P2Function("dst", shell, L, L, dtype=dtype)
+            src.upload(L, x)
+            A.apply(src, dst, L, hg.ALL)
+            assert relmax(dst.download(L), ref[dtype][0]) < 1e-5
 
 
 @pytest.mark.parametrize("dtype", [hg.F32, hg.F16])

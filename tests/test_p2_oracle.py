@@ -185,3 +185,46 @@ def test_operator_symmetric_on_shell(shell):
     assert np.linalg.eigvalsh(A[inner][:, inner].toarray()).min() > 0
     d = op.inv_diag()
     assert np.allclose(d, 1.0 / A.diagonal())
+
+
+# ---- the reduced element form of the low-precision applies ------------------------
+
+
+def test_reduced_form_equals_the_reference_form_on_affine_cells():
+    """affine tetrahedron, k = 1: the 4-point degree-2 rule integrates the P2
+    stiffness exactly, so the reduced form equals the reference form (whose
+    closed form is pinned above); with a constant k the matrix scales by k rounded
+    to binary16"""
+    p = np.array([[0.1, 0.0, 0.2], [0.9, 0.1, 0.0], [0.2, 1.1, 0.1], [0.1, 0.3, 0.8]])
+    A = po.p2_element_matrix(p)
+    R = po.p2_element_matrix(p, reduced=True)
+    assert np.abs(R - A).max() < 1e-13 * np.abs(A).max()
+    k = po.KCoeff([0.0], [[0.0, 0.0, 0.0]], [0.0])  # exp(0) = 1
+    assert np.abs(po.p2_element_matrix(p, k=k, reduced=True) - A).max() < 1e-13 * np.abs(A).max()
+    c = np.exp(0.7 * np.sin(0.4))  # k = exp(0.7 sin(0 . x + 0.4)), constant
+    k = po.KCoeff([0.7], [[0.0, 0.0, 0.0]], [0.4])
+    c16 = float(np.float16(np.float32(c)))
+    assert np.abs(po.p2_element_matrix(p, k=k, reduced=True) - c16 * A).max() < 1e-13 * c * np.abs(A).max()
+
+
+def test_reduced_form_converges_to_the_reference_form_on_the_shell():
+    """on the blended shell the reduced form is symmetric, has zero row sums
+    (constants in its kernel) and differs from the reference form by O(h): the
+    relative difference of a micro-cell's matrix halves from level to level"""
+    m = po.Mesh.shell(**problems.SHELL)
+    B = m.blend()
+    k = po.KCoeff(*problems.shell_coefficient(0))
+    X, cv = m.coords(0), m.cell_vertices()
+    for c in (0, 700, 1919):
+        P = X[cv[c]]
+        diffs = []
+        for L in range(2, 8):
+            h = 2.0 ** -L
+            pp = np.array([P[0], P[0] + h * (P[1] - P[0]), P[0] + h * (P[2] - P[0]), P[0] + h * (P[3] - P[0])])
+            A = po.p2_element_matrix(pp, B[c], k)
+            R = po.p2_element_matrix(pp, B[c], k, reduced=True)
+            assert np.abs(R - R.T).max() <= 1e-15 * np.abs(R).max()
+            assert np.abs(R.sum(axis=1)).max() < 1e-13 * np.abs(R).max()
+            diffs.append(np.abs(R - A).max() / np.abs(A).max())
+        ratios = np.array(diffs[:-1]) / np.array(diffs[1:])
+        assert diffs[-1] < 2e-3 and np.all((ratios > 1.6) & (ratios < 2.5)), (c, diffs)
