@@ -377,97 +377,150 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
 // Block b of a 1D launch of 8*ceil(n/8) blocks -> logical block such that each
 // XCD (blocks are dispatched round-robin over the 8 XCDs) sweeps one contiguous
 // range of rows and keeps the neighbour rows it shares in its own L2.
-// 15-point stencil on cell interiors; one block per (cell, interior row)
+// Cell-interior kernels (apply, SOR): a block of 256 threads holds 4 row
+// workers of 64 lanes, each sweeping one interior row (y, z) of a cell with x
+// (and, for interleaved functions, the vector index k) across the lanes; short
+// rows (small levels) are packed 2^rpbLog per worker.  The 15-point stencil
+// reaches 7 rows -- (y, z) and its neighbours (y, z+1), (y+1, z-1), (y+1, z),
+// (y-1, z), (y-1, z+1), (y, z-1), 2 or 3 consecutive points each -- whose
+// starts are computed once per row in 32-bit cell-local offsets.
 // Functions may interleave 2^lv vectors per lattice point (value k of point p
-// at (p << lv) + k); the cell kernels walk a row as (x, k) pairs with k fastest,
-// so a wavefront reads 2^lv consecutive values per stencil entry.
-// Short rows (small levels) are packed 2^rpbLog per block, each handled by
-// blockDim.x >> rpbLog threads.
-template < class T >
-__global__ void kCellApply( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
-                            int64_t mm, const double* __restrict__ stencil, const T* __restrict__ src,
-                            T* __restrict__ dst, int add, int lv, int rpbLog, double alpha )
+// at (p << lv) + k).
+constexpr int kCellWorkers = 4;
+
+struct StencilRows
 {
-   using A             = typename Acc< T >::type;
-   const int64_t nrc   = nrows * nc;
-   const int64_t lb    = ( xcdBlock( ( nrc + ( int64_t( 1 ) << rpbLog ) - 1 ) >> rpbLog ) << rpbLog ) +
-                      ( threadIdx.x >> ( 6 - rpbLog ) );
-   if ( lb >= nrc )
-      return;
-   const int64_t c    = lb / nrows;
-   const int64_t r    = lb - c * nrows;
-   const int64_t y    = rows[2 * r];
-   const int64_t z    = rows[2 * r + 1];
-   const int64_t base = c * sc;
-   A             st[15];
-   int64_t       rb[15];
-#pragma unroll
-   for ( int s = 0; s < 15; ++s )
+   int32_t r00, r01, r1m, r10, rm0, rm1, r0m;
+   __device__ __forceinline__ StencilRows( int32_t w, int32_t y, int32_t z )
+   : r00( tetRowStart32( w, y, z ) )
+   , r01( tetRowStart32( w, y, z + 1 ) )
+   , r1m( tetRowStart32( w, y + 1, z - 1 ) )
+   , r10( tetRowStart32( w, y + 1, z ) )
+   , rm0( tetRowStart32( w, y - 1, z ) )
+   , rm1( tetRowStart32( w, y - 1, z + 1 ) )
+   , r0m( tetRowStart32( w, y, z - 1 ) )
+   {}
+   // cell-local offset of stencil entry s (the order of cOff15) at x
+   __device__ __forceinline__ int32_t at( int s, int32_t x ) const
    {
-      st[s] = A( stencil[15 * c + s] );
-      rb[s] = base + tetRowStart( w, y + cOff15[s][1], z + cOff15[s][2] ) + cOff15[s][0];
+      switch ( s )
+      {
+      case 0: return r00 + x;
+      case 1: return r00 + x - 1;
+      case 2: return r01 + x - 1;
+      case 3: return r1m + x - 1;
+      case 4: return r10 + x - 1;
+      case 5: return rm0 + x;
+      case 6: return rm1 + x;
+      case 7: return r0m + x;
+      case 8: return r01 + x;
+      case 9: return r1m + x;
+      case 10: return r10 + x;
+      case 11: return rm0 + x + 1;
+      case 12: return rm1 + x + 1;
+      case 13: return r0m + x + 1;
+      default: return r00 + x + 1;
+      }
    }
-   const int64_t out = base + tetRowStart( w, y, z );
-   const int64_t msk = ( int64_t( 1 ) << lv ) - 1;
-   const int     tpr = int( blockDim.x ) >> rpbLog;
-   for ( int64_t t = threadIdx.x & ( tpr - 1 );; t += tpr )
+};
+
+// the row worker of this thread: logical row index (cell-major over the rows
+// list) and lane inside the worker
+struct RowWorker
+{
+   int64_t row;
+   int     lane, lanes;
+};
+__device__ __forceinline__ RowWorker rowWorker( int64_t nrc, int rpbLog )
+{
+   const int64_t per = int64_t( kCellWorkers ) << rpbLog;
+   RowWorker     rw;
+   rw.row   = xcdBlock( ( nrc + per - 1 ) / per ) * per + ( threadIdx.x >> ( 6 - rpbLog ) );
+   rw.lanes = 64 >> rpbLog;
+   rw.lane  = int( threadIdx.x ) & ( rw.lanes - 1 );
+   return rw;
+}
+
+// 15-point stencil on cell interiors (apply_3D_macrocell_vertexdof_to_vertexdof,
+// P1generatedKernels/apply_3D_macrocell_vertexdof_to_vertexdof_replace.cpp:34-79):
+// dst (=|+=) alpha sum_s st_s src[p + o_s], summed in the order of the offsets
+template < class T >
+__global__ __launch_bounds__( 64 * kCellWorkers ) void kCellApply( const int32_t* __restrict__ rows, int64_t nrows,
+                                                                   int64_t nc, int64_t sc, int64_t w, int64_t mm,
+                                                                   const double* __restrict__ stencil,
+                                                                   const T* __restrict__ src, T* __restrict__ dst,
+                                                                   int add, int lv, int rpbLog, double alpha )
+{
+   using A              = typename Acc< T >::type;
+   const int64_t   nrc  = nrows * nc;
+   const RowWorker rw   = rowWorker( nrc, rpbLog );
+   if ( rw.row >= nrc )
+      return;
+   const int64_t     c = rw.row / nrows;
+   const int64_t     r = rw.row - c * nrows;
+   const int32_t     y = rows[2 * r], z = rows[2 * r + 1];
+   const StencilRows sr( int32_t( w ), y, z );
+   const T*          s = src + ( ( c * sc ) << lv );
+   T*                d = dst + ( ( c * sc ) << lv );
+   A                 st[15];
+#pragma unroll
+   for ( int q = 0; q < 15; ++q )
+      st[q] = A( stencil[15 * c + q] );
+   const int32_t msk = ( 1 << lv ) - 1, top = int32_t( mm ) - y - z;
+   for ( int32_t t = rw.lane;; t += rw.lanes )
    {
-      const int64_t x = 1 + ( t >> lv ), k = t & msk;
-      if ( x + y + z >= mm )
+      const int32_t x = 1 + ( t >> lv ), k = t & msk;
+      if ( x >= top )
          break;
       A v = A( 0 );
 #pragma unroll
-      for ( int s = 0; s < 15; ++s )
-         v += st[s] * toAcc( src[( ( rb[s] + x ) << lv ) + k] );
+      for ( int q = 0; q < 15; ++q )
+         v += st[q] * toAcc( s[( sr.at( q, x ) << lv ) + k] );
       v               = A( alpha ) * v;
-      const int64_t o = ( ( out + x ) << lv ) + k;
+      const int32_t o = ( sr.at( 0, x ) << lv ) + k;
       if ( add )
-         v += toAcc( dst[o] );
-      dst[o] = fromAcc< T >( v );
+         v += toAcc( d[o] );
+      d[o] = fromAcc< T >( v );
    }
 }
 
 // SOR on the cell-interior points of one parity colour, in place
 template < class T >
-__global__ void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t sc, int64_t w,
-                          int64_t mm, const double* __restrict__ stencil, int cx, double relax,
-                          const T* __restrict__ b, T* __restrict__ x, int lv, int rpbLog )
+__global__ __launch_bounds__( 64 * kCellWorkers ) void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows,
+                                                                 int64_t nc, int64_t sc, int64_t w, int64_t mm,
+                                                                 const double* __restrict__ stencil, int cx,
+                                                                 double relax, const T* __restrict__ b,
+                                                                 T* __restrict__ x, int lv, int rpbLog )
 {
-   using A           = typename Acc< T >::type;
-   const int64_t nrc = nrows * nc;
-   const int64_t lb  = ( xcdBlock( ( nrc + ( int64_t( 1 ) << rpbLog ) - 1 ) >> rpbLog ) << rpbLog ) +
-                      ( threadIdx.x >> ( 6 - rpbLog ) );
-   if ( lb >= nrc )
+   using A              = typename Acc< T >::type;
+   const int64_t   nrc  = nrows * nc;
+   const RowWorker rw   = rowWorker( nrc, rpbLog );
+   if ( rw.row >= nrc )
       return;
-   const int64_t c    = lb / nrows;
-   const int64_t r    = lb - c * nrows;
-   const int64_t y    = rows[2 * r];
-   const int64_t z    = rows[2 * r + 1];
-   const int64_t base = c * sc;
-   A             st[15];
-   int64_t       rb[15];
+   const int64_t     c = rw.row / nrows;
+   const int64_t     r = rw.row - c * nrows;
+   const int32_t     y = rows[2 * r], z = rows[2 * r + 1];
+   const StencilRows sr( int32_t( w ), y, z );
+   const T*          bc = b + ( ( c * sc ) << lv );
+   T*                xc = x + ( ( c * sc ) << lv );
+   A                 st[15];
 #pragma unroll
-   for ( int s = 0; s < 15; ++s )
-   {
-      st[s] = A( stencil[15 * c + s] );
-      rb[s] = base + tetRowStart( w, y + cOff15[s][1], z + cOff15[s][2] ) + cOff15[s][0];
-   }
+   for ( int q = 0; q < 15; ++q )
+      st[q] = A( stencil[15 * c + q] );
    const A       invc = A( 1 ) / st[0];
-   const int64_t x0   = cx ? 1 : 2;
-   const int64_t msk  = ( int64_t( 1 ) << lv ) - 1;
-   const int     tpr  = int( blockDim.x ) >> rpbLog;
-   for ( int64_t t = threadIdx.x & ( tpr - 1 );; t += tpr )
+   const int32_t x0 = cx ? 1 : 2, msk = ( 1 << lv ) - 1, top = int32_t( mm ) - y - z;
+   for ( int32_t t = rw.lane;; t += rw.lanes )
    {
-      const int64_t xx = x0 + 2 * ( t >> lv ), k = t & msk;
-      if ( xx + y + z >= mm )
+      const int32_t xx = x0 + 2 * ( t >> lv ), k = t & msk;
+      if ( xx >= top )
          break;
-      const int64_t o = ( ( rb[0] + xx ) << lv ) + k;
-      A             v = toAcc( b[o] );
+      const int32_t o = ( sr.at( 0, xx ) << lv ) + k;
+      A             v = toAcc( bc[o] );
 #pragma unroll
-      for ( int s = 1; s < 15; ++s )
-         v -= st[s] * toAcc( x[( ( rb[s] + xx ) << lv ) + k] );
-      const A xo = toAcc( x[o] );
-      x[o]       = fromAcc< T >( ( A( 1 ) - A( relax ) ) * xo + A( relax ) * invc * v );
+      for ( int q = 1; q < 15; ++q )
+         v -= st[q] * toAcc( xc[( sr.at( q, xx ) << lv ) + k] );
+      const A xo = toAcc( xc[o] );
+      xc[o]      = fromAcc< T >( ( A( 1 ) - A( relax ) ) * xo + A( relax ) * invc * v );
    }
 }
 
@@ -744,21 +797,29 @@ __global__ void kP1InvDiagFull( int64_t nc, int64_t sc, int64_t nb, const double
 // ------------------------------------------------------------------------
 // host drivers
 // ------------------------------------------------------------------------
-static constexpr int kRowBlock = 64; // the cell kernels assume 64 (log2 6)
+static constexpr int kRowBlock = 64; // the transfer kernels' row blocks
 
-// log2 of the rows one 64-thread block of a cell kernel takes: enough lanes for
-// `lanes` values per row, at most 64 rows
+// log2 of the rows one 64-lane row worker of a cell kernel takes: enough lanes
+// for `lanes` values per row, at most 64 rows
 static int rowsPerBlockLog( int64_t lanes )
 {
    int r = 0;
-   while ( r < 6 && ( int64_t( kRowBlock ) >> ( r + 1 ) ) >= lanes )
+   while ( r < 6 && ( int64_t( 64 ) >> ( r + 1 ) ) >= lanes )
       ++r;
    return r;
 }
 
 static unsigned cellGrid( int64_t rowsTimesCells, int rpbLog )
 {
-   return xcdGrid( ( rowsTimesCells + ( int64_t( 1 ) << rpbLog ) - 1 ) >> rpbLog );
+   const int64_t per = int64_t( kCellWorkers ) << rpbLog;
+   return xcdGrid( ( rowsTimesCells + per - 1 ) / per );
+}
+
+// the cell kernels index a cell's values (2^lv per lattice point) in 32 bits
+static void checkCellIndexing( const Lattice& lat, int lv )
+{
+   if ( ( lat.sc << lv ) >= ( int64_t( 1 ) << 31 ) )
+      throw Error( HG_ERR_UNSUPPORT, "a macro-cell lattice of this level holds more than 2^31 values" );
 }
 
 static P1OpLevel& levelOf( Context* ctx, P1Operator& op, int L )
@@ -825,9 +886,10 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
       }
       if ( ( flag & kInner ) && lat.nrows > 0 )
       {
+         checkCellIndexing( lat, lv );
          ctx->beginTiming();
          const int rpb = rowsPerBlockLog( ( lat.mm - 3 ) << lv );
-         kCellApply< T ><<< cellGrid( lat.nrows * lat.nc, rpb ), kRowBlock, 0, ctx->stream >>>(
+         kCellApply< T ><<< cellGrid( lat.nrows * lat.nc, rpb ), 64 * kCellWorkers, 0, ctx->stream >>>(
              lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update, lv,
              rpb, alpha );
          HG_CHECK_HIP( hipGetLastError() );
@@ -882,13 +944,15 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
          if ( n == 0 )
             return;
          const int rpb = rowsPerBlockLog( ( ( lat.mm - 2 ) / 2 ) << lv );
-         kCellSOR< T ><<< cellGrid( n * lat.nc, rpb ), kRowBlock, 0, ctx->stream >>>(
+         kCellSOR< T ><<< cellGrid( n * lat.nc, rpb ), 64 * kCellWorkers, 0, ctx->stream >>>(
              lat.rows.p + 2 * lat.rowGroup[grp], n, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, cx, relax,
              (const T*) bp, (T*) xp, lv, rpb );
          HG_CHECK_HIP( hipGetLastError() );
       };
       const size_t nbc      = o.colorStart.empty() ? 0 : o.colorStart.size() - 1;
       const bool   interior = ( flag & kInner ) && lat.nrows > 0;
+      if ( interior )
+         checkCellIndexing( lat, lv );
       if ( !backward )
       {
          for ( size_t cc = 0; cc < nbc; ++cc )

@@ -405,8 +405,8 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
                                                                       const T* __restrict__ src, T* __restrict__ dst,
                                                                       float* __restrict__ accum, int rmw, float alpha )
 {
-   __shared__ float sb[kBox3];
-   __shared__ float sv[( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 )][3];
+   __shared__ float  sb[kBox3];
+   __shared__ double sv[( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 )][3];
    const int64_t    lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
       return;
@@ -425,7 +425,7 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
    constexpr int     nv = ( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 );
    if ( threadIdx.x < nv )
    {
-      // blended vertex positions relative to the tile's first vertex (FP64, then rounded)
+      // blended vertex positions relative to the tile's first vertex (FP64)
       auto blended = [&]( int i, int j, int k, double F[3] ) {
          const double vx = double( tc.x0 + i ), vy = double( tc.y0 + j ), vz = double( tc.z0 + k );
          double       x[3];
@@ -449,7 +449,7 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
       blended( 0, 0, 0, F0 );
 #pragma unroll
       for ( int d = 0; d < 3; ++d )
-         sv[threadIdx.x][d] = float( F[d] - F0[d] );
+         sv[threadIdx.x][d] = F[d] - F0[d];
    }
    __syncthreads();
    const int t  = __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
@@ -469,8 +469,10 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
       u[i] = valid ? sb[pos[i]] : 0.0f;
    {
       // edge vectors c_k = P_k - P_0 of the blended vertices, n_k their cross
-      // products (det J * rows of J^-1), G = (k w / |det J|) n_i . n_j
-      float P[4][3];
+      // products (det J * rows of J^-1), G = (k w / |det J|) n_i . n_j: formed
+      // in FP64 (MI355X issues FP64 and non-packed FP32 at one rate) and
+      // rounded once
+      double P[4][3];
 #pragma unroll
       for ( int k = 0; k < 4; ++k )
       {
@@ -479,7 +481,7 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
          for ( int d = 0; d < 3; ++d )
             P[k][d] = sv[vi][d];
       }
-      float c[3][3], n[3][3];
+      double c[3][3], n[3][3];
 #pragma unroll
       for ( int k = 0; k < 3; ++k )
 #pragma unroll
@@ -488,20 +490,20 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
 #pragma unroll
       for ( int k = 0; k < 3; ++k )
       {
-         const float* b1 = c[( k + 1 ) % 3];
-         const float* b2 = c[( k + 2 ) % 3];
-         n[k][0]         = b1[1] * b2[2] - b1[2] * b2[1];
-         n[k][1]         = b1[2] * b2[0] - b1[0] * b2[2];
-         n[k][2]         = b1[0] * b2[1] - b1[1] * b2[0];
+         const double* b1 = c[( k + 1 ) % 3];
+         const double* b2 = c[( k + 2 ) % 3];
+         n[k][0]          = b1[1] * b2[2] - b1[2] * b2[1];
+         n[k][1]          = b1[2] * b2[0] - b1[0] * b2[2];
+         n[k][2]          = b1[0] * b2[1] - b1[1] * b2[0];
       }
-      const float det = c[0][0] * n[0][0] + c[0][1] * n[0][1] + c[0][2] * n[0][2];
-      const float s   = ( ke * ( 1.0f / 24.0f ) ) * __builtin_amdgcn_rcpf( fabsf( det ) );
-      G[0]            = s * ( n[0][0] * n[0][0] + n[0][1] * n[0][1] + n[0][2] * n[0][2] );
-      G[1]            = s * ( n[0][0] * n[1][0] + n[0][1] * n[1][1] + n[0][2] * n[1][2] );
-      G[2]            = s * ( n[0][0] * n[2][0] + n[0][1] * n[2][1] + n[0][2] * n[2][2] );
-      G[3]            = s * ( n[1][0] * n[1][0] + n[1][1] * n[1][1] + n[1][2] * n[1][2] );
-      G[4]            = s * ( n[1][0] * n[2][0] + n[1][1] * n[2][1] + n[1][2] * n[2][2] );
-      G[5]            = s * ( n[2][0] * n[2][0] + n[2][1] * n[2][1] + n[2][2] * n[2][2] );
+      const double det = c[0][0] * n[0][0] + c[0][1] * n[0][1] + c[0][2] * n[0][2];
+      const double s   = ( double( ke ) * ( 1.0 / 24.0 ) ) / fabs( det );
+      G[0]             = float( s * ( n[0][0] * n[0][0] + n[0][1] * n[0][1] + n[0][2] * n[0][2] ) );
+      G[1]             = float( s * ( n[0][0] * n[1][0] + n[0][1] * n[1][1] + n[0][2] * n[1][2] ) );
+      G[2]             = float( s * ( n[0][0] * n[2][0] + n[0][1] * n[2][1] + n[0][2] * n[2][2] ) );
+      G[3]             = float( s * ( n[1][0] * n[1][0] + n[1][1] * n[1][1] + n[1][2] * n[1][2] ) );
+      G[4]             = float( s * ( n[1][0] * n[2][0] + n[1][1] * n[2][1] + n[1][2] * n[2][2] ) );
+      G[5]             = float( s * ( n[2][0] * n[2][0] + n[2][1] * n[2][1] + n[2][2] * n[2][2] ) );
    }
    __syncthreads();
    for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )

@@ -459,7 +459,7 @@ __global__ void kJacobiUpdate( int64_t ncell, int64_t nb, const uint8_t* __restr
                                const T* __restrict__ rhs, const T* __restrict__ src,
                                const double* __restrict__ invDiag, int lv, T* __restrict__ dst )
 {
-#pragma clang fp contract( off )
+   // the statements (and so the fused multiply-adds) of kLinComb and kMult
    using A           = typename Acc< T >::type;
    const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
    const int64_t i   = tid >> lv;

@@ -18,8 +18,11 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 # relative max-norm bounds of one reduced-precision apply against the FP64 result of
-# the same element form (FP16: input and output rounded to binary16)
+# the same element form (FP16: input and output rounded to binary16); the reduced
+# form's FP32 element action sums terms about ten times the result, one decimal
+# digit more than the reference form's in FP32
 APPLY_TOL = {hg.F32: 2e-6, hg.F16: 4e-3}
+REDUCED_TOL = {hg.F32: 1e-5, hg.F16: 4e-3}
 
 
 @pytest.fixture(scope="module")
@@ -49,7 +52,7 @@ def test_reduced_form_apply_matches_the_oracle(ctx, shell, dtype, flag, update):
         po.P2Op(om, L, po.KCoeff(*k), reduced=True).apply(xs, ref, flag, update == hg.ADD)
         err = relmax(dst.download(L), ref)
         print(f"   relative difference to the oracle's reduced form {err:.2e}", flush=True)
-        assert err < APPLY_TOL[dtype]
+        assert err < REDUCED_TOL[dtype]
         for f in (src, dst):
             f.destroy()
         A.destroy()
