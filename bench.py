@@ -4,13 +4,18 @@
 
 Default step: one application of the P2 variable-coefficient diffusion
 operator div(k grad) (P2ElementwiseOperator with the blended
-p2_div_k_grad_blending_q4 form, FP64, exact reference formulation) on the thick
-spherical shell r in [0.55, 1] (ntan 5, 3 radial layers, 1920 macro-cells) at
-level 6: 673,710,594 DoFs, inputs resident in HBM.  Prints ONE JSON line with
-DoF/s per apply, the roofline of the dominant kernel (HIP-event timed on the
-library's stream), the CPU oracle timed on this host, and secondary results
-(P1 apply on the cube, FP64 and mixed-precision GMG time-to-1e-8, 64 batched
-P1 solves, and a mixed request stream through the serving engine).
+p2_div_k_grad_blending_q4 form, FP64, exact reference formulation, k stored at
+the 11 quadrature points of every micro-cell) on the thick spherical shell
+r in [0.55, 1] (ntan 5, 3 radial layers, 1920 macro-cells) at level 6:
+673,710,594 DoFs, inputs resident in HBM.  Prints ONE JSON line with DoF/s per
+apply, the HBM roofline of the dominant kernel (HIP-event timed on the
+library's stream; algorithmic bytes: src, dst and one coefficient value per
+DoF), the CPU oracle timed on this host, and secondary results: the apply with
+k evaluated in the kernel, the low-precision (reduced-form) apply, the P1
+apply and Gauss-Seidel sweep and the P2 transfers with their HBM fractions,
+FP64 and mixed-precision GMG time-to-1e-8 (cfg2), the cfg1 GMG, 64 batched P1
+solves (cfg4), the Stokes Uzawa GMG (cfg3), the serving stream (cfg5), and
+CPU-oracle baselines of cfg2 (small level), cfg4 and cfg1.
 """
 import argparse
 import json
@@ -67,6 +72,9 @@ def kernel_ms(ctx, apply, reps):
     return float(np.mean(ms))
 
 
+P2_STATE = {}
+
+
 def p2_shell(hg, ctx, L, steps, warmup):
     from hyteg_amd import problems
     st = hg.PrimitiveStorage.shell(ctx, **problems.SHELL)
@@ -81,21 +89,73 @@ def p2_shell(hg, ctx, L, steps, warmup):
     sec = timed_applies(ctx, apply, steps, warmup)
     kms = kernel_ms(ctx, apply, max(3, min(steps, 5)))
     nelem = 1920 * 8 ** L
-    roof = {"bound": "mfma", "achieved": P2_FLOPS_PER_ELEMENT * nelem / (kms * 1e-3) / 1e12,
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "note": "FP64 VALU-bound element integration (no MFMA); MI355X FP64 vector and matrix peaks are "
-                    "both 78.6 TFLOP/s; algorithmic flops = 1940 per micro-cell x "
-                    f"{nelem} micro-cells",
-            "algorithmic_flops_per_launch": P2_FLOPS_PER_ELEMENT * nelem,
-            # algorithmic HBM bytes: src, dst and one coefficient value per P2 DoF, FP64
-            "hbm": {"algorithmic_bytes_per_launch": 3 * 8 * n,
-                    "achieved_gbs": 3 * 8 * n / (kms * 1e-3) / 1e9,
-                    "frac": 3 * 8 * n / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    alg = 3 * 8 * n  # algorithmic HBM bytes: src, dst and one coefficient value per P2 DoF, FP64
+    roof = {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "algorithmic_bytes_per_launch": alg,
+            "note": "algorithmic bytes = 24 per P2 DoF (src, dst, one k value, FP64); the exact "
+                    "p2_div_k_grad_blending_q4 element integration issues ~1.9 kflop FP64 per micro-cell, so "
+                    "the kernel is bound by FP64 VALU issue, not by HBM (compute_fp64 below)",
+            "compute_fp64": {"achieved_tflops": P2_FLOPS_PER_ELEMENT * nelem / (kms * 1e-3) / 1e12,
+                             "peak_tflops": FP64_PEAK_TFLOPS,
+                             "frac": P2_FLOPS_PER_ELEMENT * nelem / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                             "flops_per_launch": P2_FLOPS_PER_ELEMENT * nelem}}
+    P2_STATE.update(st=st, A=A, src=src, dst=dst, L=L, n=n)
     return n, sec, kms, roof, {"workload": f"P2 div(k grad) apply (blended shell map, 11-point quadrature, "
-                                           f"k = exp(16 sine modes), FP64), spherical shell r 0.55..1, ntan 5, "
-                                           f"3 layers (1920 macro-cells), level {L} ({n} DoFs)",
-                               "level": L, "dofs": n, "mesh": "meshSphericalShell(5, 3, 0.55, 1.0)"}, \
+                                           f"k = exp(16 sine modes) stored at the quadrature points -- 11 FP64 "
+                                           f"values per micro-cell, a 53 GB table built once per level -- FP64), "
+                                           f"spherical shell r 0.55..1, ntan 5, 3 layers (1920 macro-cells), "
+                                           f"level {L} ({n} DoFs)",
+                               "level": L, "dofs": n, "mesh": "meshSphericalShell(5, 3, 0.55, 1.0)",
+                               "coefficients": "stored at the quadrature points"}, \
         "kP2TileApplyStored<double>", f"p2_apply_shell_L{L}_f64"
+
+
+def p2_apply_variants(hg, ctx):
+    """the headline operator with k evaluated in the kernel (the reference's default
+    apply) and the reduced-form low-precision apply of the mixed-precision V-cycle"""
+    st, A, L, n = P2_STATE["st"], P2_STATE["A"], P2_STATE["L"], P2_STATE["n"]
+    src, dst = P2_STATE["src"], P2_STATE["dst"]
+    out = {}
+    A.store_coefficients(False)
+    kms = kernel_ms(ctx, lambda: A.apply(src, dst, L, hg.ALL), 3)
+    A.store_coefficients(True)
+    out["k_in_kernel_f64"] = {"kernel": "kP2TileApplyEval<double>", "kernel_ms": kms, "dof_per_s": n / (kms * 1e-3),
+                              "hbm_frac_algorithmic": 16 * n / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    for dt, name, s in ((hg.F16, "reduced_f16", 2), (hg.F32, "reduced_f32", 4)):
+        fs = hg.P2Function("s", st, L, L, dtype=dt)
+        fd = hg.P2Function("d", st, L, L, dtype=dt)
+        fs.interpolate_random(1, -1.0, 1.0, L)
+        apply = lambda: A.apply(fs, fd, L, hg.ALL)  # noqa: E731
+        kms = kernel_ms(ctx, apply, 5)
+        sec = timed_applies(ctx, apply, 5, 1)
+        # src, dst, and one FP16 k value per micro-cell (about 0.75 per DoF)
+        alg = (2 * s + 2 * 0.75) * n
+        out[name] = {"kernel": f"kP2TileApplyReduced<{'__half' if dt == hg.F16 else 'float'}>",
+                     "kernel_ms": kms, "ms_per_apply": 1e3 * sec, "dof_per_s": n / sec,
+                     "hbm_frac_algorithmic": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"P2 reduced apply {name}: kernel {kms:.3f} ms, {1e3 * sec:.3f} ms per apply")
+        fs.destroy()
+        fd.destroy()
+    return out
+
+
+def p2_transfers(hg, ctx):
+    """P2 quadratic restriction and prolongation between shell levels 6 and 5 (FP64),
+    cell kernels HIP-event timed; algorithmic bytes 9 per fine DoF"""
+    st, L = P2_STATE["st"], P2_STATE["L"]
+    f = hg.P2Function("t", st, L - 1, L)
+    f.interpolate_random(3, -1.0, 1.0, L)
+    f.interpolate_random(4, -1.0, 1.0, L - 1)
+    nf = st.p2_ndofs(L)
+    r = kernel_ms(ctx, lambda: hg.restrict(f, L, hg.GMG_FLAG), 3)
+    p = kernel_ms(ctx, lambda: hg.prolongate(f, L - 1, hg.GMG_FLAG, hg.ADD), 3)
+    f.destroy()
+    alg = 9 * nf
+    log(f"P2 transfers level {L}<->{L - 1}: restrict {r:.3f} ms, prolongate {p:.3f} ms")
+    return {"fine_dofs": nf, "restrict_kernel_ms": r, "prolongate_kernel_ms": p,
+            "restrict_hbm_frac": alg / (r * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "prolongate_hbm_frac": alg / (p * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "note": "kP2RestrictCell / kP2ProlongCell; algorithmic bytes 9 per fine DoF (8 fine + 8/8 coarse)"}
 
 
 def p1_cube(hg, ctx, L, steps, warmup):
@@ -112,6 +172,16 @@ def p1_cube(hg, ctx, L, steps, warmup):
     alg = 2 * 8 * 6 * tet(2 ** L - 3)
     roof = {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "algorithmic_bytes_per_launch": alg}
+    # one Gauss-Seidel sweep (multicolour: owned boundary DoFs, then 8 parity colours
+    # of the cell interiors), timed from its first to its last kernel
+    b = hg.P1Function("b", st, L, L)
+    b.interpolate_random(2, -1.0, 1.0, L)
+    gs = kernel_ms(ctx, lambda: A.smooth_gs(src, b, L, hg.GMG_FLAG), 3)
+    roof["gauss_seidel_sweep"] = {"ms": gs, "dofs": n, "algorithmic_bytes": 3 * 8 * n,
+                                  "achieved_gbs": 3 * 8 * n / (gs * 1e-3) / 1e9,
+                                  "frac": 3 * 8 * n / (gs * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"P1 level {L}: apply kernel {kms:.3f} ms, Gauss-Seidel sweep {gs:.3f} ms")
+    b.destroy()
     return n, sec, kms, roof, {"workload": f"P1ConstantLaplaceOperator apply, unit cube 6 macro-tets, level {L} "
                                           f"({n} DoFs), FP64", "level": L, "dofs": n, "mesh": "meshCuboid 1x1x1"}, \
         "kCellApply<double>", f"p1_apply_cube_L{L}_f64"
@@ -227,26 +297,34 @@ def gmg_p2_shell(hg, ctx, L):
            "factors": [float(v) for v in hist[1:] / hist[:-1]],
            "relative_residual": float(hist[-1] / hist[0])}
     t.destroy()
-    # mixed precision: FP32 arithmetic, FP16 storage inside FP64 iterative refinement
-    xm = hg.P2Function("xm", st, 0, L)
-    hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=hg.F16, max_outer=1)  # builds the FP16 tables once
-    xm.interpolate(0.0, L)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    km, hm = hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=hg.F16, inner_cycles=1, max_outer=60, tol=1e-8)
-    ctx.synchronize()
-    dtm = time.perf_counter() - t0
-    # || x_mixed - x_fp64 ||_2 / || x_fp64 ||_2 on the device
-    xm.assign([1.0, -1.0], [xm, x], L, hg.ALL)
-    err = (xm.dot(xm, L, hg.ALL) / x.dot(x, L, hg.ALL)) ** 0.5
-    log(f"P2 mixed IR level {L}: {km} outer steps, {dtm:.2f} s, speed-up {dt / dtm:.2f}, error vs FP64 {err:.2e}")
-    out["mixed_fp32_compute_fp16_storage"] = {
-        "workload": "FP64 iterative refinement, one V(2,2) Chebyshev(3) cycle per step in FP32 arithmetic with "
-                    "FP16 vectors and FP16 quadrature-point coefficients, FP64 PCG on level 0, to 1e-8",
-        "outer_steps": km, "time_s": dtm, "speedup_vs_fp64": dt / dtm,
-        "relative_residual": float(hm[-1] / hm[0]),
-        "rel_l2_difference_to_fp64_solution": err}
-    for f in (x, b, xm):
+    # mixed precision: FP64 iterative refinement around one V-cycle per step in FP32
+    # arithmetic on FP16 (or FP32) vectors; the low-precision applies use the reduced
+    # element form (affine micro-cells, k constant per micro-cell, FP32 accumulation)
+    for dt, key, what in ((hg.F16, "mixed_fp32_compute_fp16_storage", "FP16 vectors"),
+                          (hg.F32, "mixed_fp32_compute_fp32_storage", "FP32 vectors")):
+        xm = hg.P2Function("xm", st, 0, L, dtype=hg.F64)
+        hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=dt, max_outer=1)  # builds the low-precision tables once
+        xm.interpolate(0.0, L)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        km, hm = hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=dt, inner_cycles=1, max_outer=60, tol=1e-8)
+        ctx.synchronize()
+        dtm = time.perf_counter() - t0
+        # || x_mixed - x_fp64 ||_2 / || x_fp64 ||_2 on the device
+        xm.assign([1.0, -1.0], [xm, x], L, hg.ALL)
+        err = (xm.dot(xm, L, hg.ALL) / x.dot(x, L, hg.ALL)) ** 0.5
+        log(f"P2 mixed IR ({what}) level {L}: {km} outer steps, {dtm:.2f} s, speed-up {dt / dtm:.2f}, "
+            f"error vs FP64 {err:.2e}")
+        out[key] = {
+            "workload": "FP64 iterative refinement (FP64 residual, FP64 PCG on level 0), one V(2,2) Chebyshev(3) "
+                        f"cycle per step in FP32 arithmetic with {what}, FP16 coefficients and FP32 accumulation "
+                        "(reduced element form on levels >= 2), to 1e-8",
+            "outer_steps": km, "time_s": dtm, "speedup_vs_fp64": dt / dtm,
+            "relative_residual": float(hm[-1] / hm[0]),
+            "factors": [float(v) for v in hm[1:] / hm[:-1]],
+            "rel_l2_difference_to_fp64_solution": err}
+        xm.destroy()
+    for f in (x, b):
         f.destroy()
     A.destroy()
     return out
@@ -307,16 +385,13 @@ def serving_cfg5(n=400, rate=200.0):
                         "P2 cube L3-5 20%, P2 shell L2-3 15%, each solved to 1e-8", "rate_per_s": rate, **res}
 
 
-def stokes_cfg3(hg, ctx, L=5, uzawa_cycles=4, max_iter=150, restart=40, tol=1e-7):
+def stokes_cfg3(hg, ctx, L=5, cycles=120, tol=1e-7):
     """P2-P1 Stokes on the shell at level L, eta = exp(ln(1e4)(1-r)/0.45), nodal
     buoyancy of 12 Gaussian blobs, free-slip on the inner and no-slip on the outer
-    surface (free-slip operator wrapper, normal projections).  Three solves:
-    (1) the Uzawa GMG of the free-slip solver template, V(3,3) + 2 per coarser level,
-    omega 0.3, Chebyshev(3) velocity smoothing, MINRES(10) on level 0, `uzawa_cycles`
-    cycles (its factors on this contrast); (2) FGMRES with the block-factorisation
-    preconditioner (velocity: one V(3,3) Chebyshev(3) cycle, Schur: 5 CG steps on the
-    1/eta pressure mass) to `tol`, FP64; (3) the same with the velocity V-cycle in FP32
-    arithmetic on FP32 vectors (mixed), whose projected solution is compared with (2)."""
+    surface (free-slip operator wrapper, normal projections): the all-at-once
+    Uzawa-smoothed GMG of the free-slip solver template (V(3,3) + 2 per coarser
+    level, omega 0.3, Chebyshev(3) velocity smoothing, MINRES(10) on level 0),
+    FP64, until the residual drops by `tol`"""
     from hyteg_amd import problems
     st = hg.PrimitiveStorage.shell(ctx, **problems.SHELL, inner_flag=hg.FREESLIP, outer_flag=hg.DIRICHLET)
     A = hg.P2P1StokesOperator(st, 0, L, *problems.STOKES_VISCOSITY, freeslip=True)
@@ -347,31 +422,12 @@ def stokes_cfg3(hg, ctx, L=5, uzawa_cycles=4, max_iter=150, restart=40, tol=1e-7
     X.interpolate(0.0, L)
     ctx.synchronize()
     t0 = time.perf_counter()
-    k, hist = hg.stokes_gmg_solve(A, X, B, 0, L, rho, cycles=uzawa_cycles)
+    k, hist = hg.stokes_gmg_solve(A, X, B, 0, L, rho, cycles=cycles, tol=tol)
     ctx.synchronize()
     t_uz = time.perf_counter() - t0
-    log(f"Stokes Uzawa GMG: {k} cycles {t_uz:.2f} s, factors {np.round(hist[1:] / hist[:-1], 3)}")
-    runs = {}
-    X64 = hg.P2P1TaylorHoodFunction("x64", st, L, L)
-    for name, vd in (("f64", hg.F64), ("mixed_f32", hg.F32)):
-        X.interpolate(0.0, L)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        its, h = hg.stokes_fgmres_solve(A, X, B, 0, L, rho, a_coarse_iter=5, schur_iter=5, restart=restart,
-                                        max_iter=max_iter, tol=tol, velocity_dtype=vd)
-        ctx.synchronize()
-        dt = time.perf_counter() - t0
-        P.project(X, L)  # the normal velocity on the free-slip surface is not part of the solution
-        runs[name] = {"iterations": its, "time_s": dt, "s_per_iteration": dt / max(its, 1),
-                      "relative_residual": float(h[-1] / h[0]), "reached_tol": bool(h[-1] <= tol * h[0])}
-        log(f"Stokes FGMRES ({name}): {its} iterations {dt:.2f} s, relative residual {h[-1] / h[0]:.2e}")
-        if vd == hg.F64:
-            X64.assign([1.0], [X], L)
-        else:
-            X64.assign([1.0, -1.0], [X, X64], L)
-            ref = X.dot(X, L)
-            runs[name]["rel_l2_error_vs_f64"] = float(np.sqrt(X64.dot(X64, L) / ref))
-    for fn in (X, B, X64):
+    rel = float(hist[-1] / hist[0])
+    log(f"Stokes Uzawa GMG: {k} cycles {t_uz:.2f} s, relative residual {rel:.2e}")
+    for fn in (X, B):
         fn.destroy()
     A.destroy()
     return {"workload": f"P2-P1 Stokes, shell level {L} ({3 * n2} velocity + {n1} pressure DoFs), "
@@ -379,11 +435,43 @@ def stokes_cfg3(hg, ctx, L=5, uzawa_cycles=4, max_iter=150, restart=40, tol=1e-7
             "dofs": 3 * n2 + n1, "setup_s": setup, "spectral_radius": rho,
             "stokes_apply_kernel_ms": kms, "stokes_apply_dof_per_s": (3 * n2 + n1) / (kms * 1e-3),
             "uzawa_gmg_f64": {"solver": "Uzawa GMG V(3,3)+2, omega 0.3, Chebyshev(3), MINRES(10) on level 0",
-                              "cycles": k, "time_s": t_uz, "s_per_cycle": t_uz / max(k, 1),
-                              "factors": [float(v) for v in hist[1:] / hist[:-1]]},
-            "fgmres_block_factorisation": {"solver": f"FGMRES(restart {restart}), block-factorisation: velocity "
-                                                     "V(3,3) Chebyshev(3), Schur CG(5) on the 1/eta mass", "tol": tol,
-                                           **runs}}
+                              "tol": tol, "cycles": k, "time_s": t_uz, "s_per_cycle": t_uz / max(k, 1),
+                              "relative_residual": rel, "reached_tol": bool(rel <= tol),
+                              "factors_first": [float(v) for v in (hist[1:] / hist[:-1])[:5]],
+                              "mean_factor": float(rel ** (1.0 / max(k, 1)))}}
+
+
+def cpu_extra_baselines():
+    """oracle (C restatement, all host cores) solves of cfg1, cfg2 (at a level the
+    oracle finishes in bounded time) and cfg4"""
+    from oracle import pyoracle as po
+    from hyteg_amd import problems
+    out = {}
+    # cfg2 at shell level 2 (174,114 DoFs): FP64 GMG V(2,2) Chebyshev(3) to 1e-8
+    m = po.Mesh.shell(**problems.SHELL)
+    k = po.KCoeff(*problems.shell_coefficient(0))
+    L = 2
+    fl = m.flags(L + 1)
+    b = np.where(fl == po.INNER, problems.canonical_uniform(len(fl), 0), 0.0)
+    t0 = time.perf_counter()
+    op = po.P2Op(m, L, k)
+    rho = op.power_iteration(op.inv_diag(), problems.canonical_uniform(len(fl), 1), 20)
+    x = np.zeros(len(fl))
+    kk, hist = m.p2_gmg(0, L, k, x, b, 2, 2, 3, rho, cycles=60, tol=1e-8)
+    out["cfg2_oracle_shell_L2"] = {"dofs": len(fl), "cycles": kk, "time_s": time.perf_counter() - t0,
+                                   "note": "power iteration (20) + FP64 GMG V(2,2) Chebyshev(3) to 1e-8"}
+    # cfg4: 64 cube level-6 systems, FP64 GMG V(3,3) lexicographic GS to 1e-10, OpenMP over the systems
+    mc = po.Mesh.cuboid()
+    L = 6
+    fl = mc.flags(L)
+    B = np.stack([np.where(fl == po.INNER, problems.canonical_uniform(len(fl), s), 0.0) for s in range(64)])
+    t0 = time.perf_counter()
+    X, done = mc.gmg_many(0, L, np.zeros_like(B), B, cycles=60, tol=1e-10)
+    dt = time.perf_counter() - t0
+    out["cfg4_oracle_64_solves"] = {"time_s": dt, "solves_per_s": 64 / dt, "cycles": int(done.max()),
+                                    "note": "oracle GMG, one system per OpenMP thread"}
+    out["cores"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return out
 
 
 def main():
@@ -401,10 +489,12 @@ def main():
     ap.add_argument("--no-stokes", action="store_true", help="skip the Stokes solve")
     ap.add_argument("--only-apply", action="store_true",
                     help="run the apply measurement only (for counter collection)")
-    ap.add_argument("--only-stokes", action="store_true", help="the apply measurement and the Stokes leg only")
-    ap.add_argument("--only-p2const", action="store_true",
-                    help="the apply measurement and the P2 constant-stencil apply only")
-    ap.add_argument("--stokes-max-iter", type=int, default=150, help="FGMRES iteration cap of the Stokes leg")
+    ap.add_argument("--only", default=None,
+                    help="the apply measurement and the named legs only (comma separated: variants, transfers, "
+                         "p1, p2const, gmg, cfg1, cfg4, stokes, serving, cpu)")
+    ap.add_argument("--stokes-cycles", type=int, default=120, help="V-cycle cap of the Stokes leg")
+    ap.add_argument("--serving-requests", type=int, default=2000)
+    ap.add_argument("--serving-rates", default="200,500")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard by "
@@ -419,28 +509,50 @@ def main():
     log(f"{args.steps} applies: {1e3 * sec:.3f} ms/apply, dominant kernel {kms:.3f} ms")
     roofline["frac"] = roofline["achieved"] / roofline["peak"]
     roofline.update({"traffic": load_traffic(tkey), "kernel": kname, "kernel_avg_ms": kms})
+    legs = set() if args.only_apply else (
+        set(args.only.split(",")) if args.only else
+        {"variants", "transfers", "p1", "p2const", "gmg", "cfg1", "cfg4", "stokes", "serving", "cpu"})
+    if args.no_gmg:
+        legs.discard("gmg")
+    if args.no_batched:
+        legs.discard("cfg4")
+    if args.no_serving:
+        legs.discard("serving")
+    if args.no_stokes:
+        legs.discard("stokes")
+    if args.no_cpu:
+        legs.discard("cpu")
     extra = {}
-    if args.only_stokes:
-        extra["stokes_shell_L5"] = stokes_cfg3(hg, ctx, max_iter=args.stokes_max_iter)
-    elif args.only_p2const:
+    p2 = args.workload == "p2_shell"
+    if p2 and "variants" in legs:
+        extra["p2_apply_variants_shell_L6"] = p2_apply_variants(hg, ctx)
+    if p2 and "transfers" in legs:
+        extra["p2_transfers_shell_L6_f64"] = p2_transfers(hg, ctx)
+    if p2 and "p1" in legs:
+        n1, sec1, kms1, roof1, cfg1, _, _ = p1_cube(hg, ctx, 9, 20, 3)
+        extra["p1_apply_cube_L9_f64"] = {"dofs": n1, "ms_per_apply": 1e3 * sec1, "dof_per_s": n1 / sec1,
+                                         "kernel_ms": kms1, "kernel_gbs": roof1["achieved"],
+                                         "hbm_frac": roof1["achieved"] / HBM_PEAK_GBS,
+                                         "traffic": load_traffic("p1_apply_cube_L9_f64"),
+                                         "gauss_seidel_sweep": roof1["gauss_seidel_sweep"]}
+    if p2 and "p2const" in legs:
         extra["p2_const_apply_cube_L8_f64"] = p2_const_cube(hg, ctx)
-    elif not args.only_apply:
-        if args.workload == "p2_shell":
-            n1, sec1, kms1, roof1, cfg1, _, _ = p1_cube(hg, ctx, 9, 20, 3)
-            extra["p1_apply_cube_L9_f64"] = {"dofs": n1, "ms_per_apply": 1e3 * sec1, "dof_per_s": n1 / sec1,
-                                             "kernel_ms": kms1, "kernel_gbs": roof1["achieved"]}
-            extra["p2_const_apply_cube_L8_f64"] = p2_const_cube(hg, ctx)
-            if not args.no_gmg:
-                extra["gmg_p2_shell_f64"] = gmg_p2_shell(hg, ctx, L)
+    if p2 and "gmg" in legs:
+        extra["gmg_p2_shell_f64"] = gmg_p2_shell(hg, ctx, L)
+    if "cfg1" in legs:
         extra["gmg_p1_cube_L5_f64"] = gmg_cfg1(hg, ctx)
-        if args.workload == "p2_shell" and not args.no_batched:
-            extra["batched_p1_cube_L6_x64"] = batched_cfg4(hg, ctx)
-        if args.workload == "p2_shell" and not args.no_stokes:
-            extra["stokes_shell_L5"] = stokes_cfg3(hg, ctx, max_iter=args.stokes_max_iter)
-        if args.workload == "p2_shell" and not args.no_serving:
-            extra["serving_mixed_stream"] = serving_cfg5()
-    quick = args.only_apply or args.only_stokes or args.only_p2const
-    cpu = None if (args.no_cpu or quick) else cpu_baseline(args.workload, args.cpu_seconds)
+    if p2 and "cfg4" in legs:
+        extra["batched_p1_cube_L6_x64"] = batched_cfg4(hg, ctx)
+    if p2 and "stokes" in legs:
+        extra["stokes_shell_L5"] = stokes_cfg3(hg, ctx, cycles=args.stokes_cycles)
+    if p2 and "serving" in legs:
+        extra["serving_mixed_stream"] = [serving_cfg5(args.serving_requests, float(r))
+                                         for r in args.serving_rates.split(",")]
+    cpu = None
+    if "cpu" in legs:
+        cpu = cpu_baseline(args.workload, args.cpu_seconds)
+        if p2:
+            extra["cpu_oracle_solves"] = cpu_extra_baselines()
     out = {
         "metric": METRIC, "value": n / sec, "unit": "DoF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": True, "scaling": "weak",

@@ -41,6 +41,24 @@ class Context:
         self.check(lib().hg_ctx_last_kernel_ms(self.h, C.byref(out)), "last_ms")
         return out.value
 
+    def timing(self, enable=True):
+        """record the solvers' named timing regions (HIP events on the context stream)"""
+        self.check(lib().hg_timing_enable(self.h, int(enable)), "timing")
+
+    def timing_report(self, reset=False):
+        """{region path: (count, total device ms)} of the recorded regions"""
+        need = C.c_int64()
+        self.check(lib().hg_timing_report(self.h, None, 0, C.byref(need)), "timing report")
+        buf = C.create_string_buffer(need.value)
+        self.check(lib().hg_timing_report(self.h, buf, need.value, None), "timing report")
+        if reset:
+            self.check(lib().hg_timing_reset(self.h), "timing reset")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            path, count, ms = line.split("\t")
+            out[path] = (int(count), float(ms))
+        return out
+
     def set_graphs(self, enable=True):
         """let the batched multigrid solvers replay their V-cycle from a captured HIP graph"""
         self.check(lib().hg_ctx_set_graphs(self.h, int(enable)), "set_graphs")

@@ -417,9 +417,11 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
       using T = decltype( tag );
       if ( lc.nrows > 0 )
       {
+         ctx->beginTiming();
          kP2RestrictCell< T ><<< xcdGrid( lc.nrows * lc.nc ), kRowBlockT, 0, ctx->stream >>>(
              lc.rowsNat.p, lc.nrows, lc.nc, lc.sc, lc.w, lc.mm, lf.sc, lf.w, (const T*) fp, (T*) cp );
          HG_CHECK_HIP( hipGetLastError() );
+         ctx->endTiming();
       }
       if ( lc.nb > 0 )
       {
@@ -453,9 +455,11 @@ void p2Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update 
       using T = decltype( tag );
       if ( lf.nrows > 0 )
       {
+         ctx->beginTiming();
          kP2ProlongCell< T ><<< xcdGrid( lf.nrows * lf.nc ), kRowBlockT, 0, ctx->stream >>>(
              lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update );
          HG_CHECK_HIP( hipGetLastError() );
+         ctx->endTiming();
       }
       if ( lf.nb > 0 )
       {

@@ -557,19 +557,68 @@ __global__ void kDotFinal( const double* __restrict__ partials, int n, int lv, d
 
 // launches the reduction of the 2^lv dot products into dev[0 .. 2^lv) on the
 // context's stream; the grid covers the flagged entries (one block at least)
-static void dotLaunch( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv,
-                       double* dev )
+static int dotBlocks( const Lattice& lat, int flag, int lv )
 {
-   const int     nv     = 1 << lv;
-   const int64_t n      = ( ( flag & kInner ? lat.nc * lat.sc : 0 ) + lat.nb ) << lv;
-   const int     blocks = int( std::min< int64_t >( kDotBlocks, std::max< int64_t >( 1, ( n + kBlock - 1 ) / kBlock ) ) );
+   const int64_t n = ( ( flag & kInner ? lat.nc * lat.sc : 0 ) + lat.nb ) << lv;
+   return int( std::min< int64_t >( kDotBlocks, std::max< int64_t >( 1, ( n + kBlock - 1 ) / kBlock ) ) );
+}
+
+// partials: (blocks << lv) doubles of scratch
+static void dotLaunchWith( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv,
+                           double* partials, double* dev )
+{
+   const int nv     = 1 << lv;
+   const int blocks = dotBlocks( lat, flag, lv );
    dispatchType( dtype, [&]( auto tag ) {
       using T = decltype( tag );
       kDotPartial< T ><<< blocks, kBlock, 0, ctx->stream >>>( lat.nc * lat.sc, lat.nb, lat.cellMask.p, lat.bndFlag.p,
-                                                             flag, (const T*) a, (const T*) b, lv, ctx->partials.p );
+                                                             flag, (const T*) a, (const T*) b, lv, partials );
    } );
-   kDotFinal<<< nv, kBlock, 0, ctx->stream >>>( ctx->partials.p, blocks, lv, dev );
+   kDotFinal<<< nv, kBlock, 0, ctx->stream >>>( partials, blocks, lv, dev );
    HG_CHECK_HIP( hipGetLastError() );
+}
+
+static void dotLaunch( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv,
+                       double* dev )
+{
+   dotLaunchWith( ctx, lat, a, b, dtype, flag, lv, ctx->partials.p, dev );
+}
+
+// ---- dot products without allocation ------------------------------------------
+// Partial sums and results of hg_function_dot_batched_noalloc live in this
+// array: one copy per device, held until the process ends.  The name is a
+// host-side stand-in; host code reaches each device's copy through the address
+// recorded by recordScratch at context creation (scratchOf).
+constexpr size_t kScratchDoubles = ( size_t( kDotBlocks ) << kMaxLv ) + ( size_t( 1 ) << kMaxLv );
+static __device__ __align__( 256 ) double scratch[kScratchDoubles];
+
+static std::mutex&               scratchMutex()
+{
+   static std::mutex m;
+   return m;
+}
+static std::map< int, double* >& scratchTable()
+{
+   static std::map< int, double* > t;
+   return t;
+}
+// once per device, with that device current
+static void recordScratch( int device )
+{
+   std::lock_guard< std::mutex > lk( scratchMutex() );
+   if ( scratchTable().count( device ) )
+      return;
+   double* ptr = nullptr;
+   HG_CHECK_HIP( hipGetSymbolAddress( (void**) &ptr, HIP_SYMBOL( scratch ) ) );
+   scratchTable()[device] = ptr;
+}
+static double* scratchOf( int device )
+{
+   std::lock_guard< std::mutex > lk( scratchMutex() );
+   auto                          it = scratchTable().find( device );
+   if ( it == scratchTable().end() )
+      throw Error( HG_ERR_STATE, "no reduction scratch recorded for this device" );
+   return it->second;
 }
 
 void dotDeviceBatched( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv, double* out )
@@ -766,6 +815,7 @@ int hg_ctx_create( int device, hg_ctx** out )
          HG_CHECK_HIP( hipEventCreate( &ctx->ev1 ) );
       }
       ctx->partials.alloc( ctx, ( size_t( kDotBlocks ) << kMaxLv ) + ( 1 << kMaxLv ) );
+      recordScratch( device );
       ctx->slots.alloc( ctx, kSlots );
       ctx->slotUsed.assign( kSlots, 0 );
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
@@ -777,6 +827,8 @@ int hg_ctx_create( int device, hg_ctx** out )
    *out = reinterpret_cast< hg_ctx* >( ctx );
    return HG_OK;
 }
+
+static void treeRelease( Context* ctx );
 
 int hg_ctx_destroy( hg_ctx* h )
 {
@@ -802,6 +854,7 @@ int hg_ctx_destroy( hg_ctx* h )
    ctx->operators.clear();
    ctx->functions.clear();
    ctx->storages.clear();
+   treeRelease( ctx );
    ctx->partials.release();
    ctx->slots.release();
    ctx->drainPool();
@@ -818,6 +871,59 @@ int hg_ctx_destroy( hg_ctx* h )
    }
    delete ctx;
    return HG_OK;
+}
+
+// ---- timing tree ------------------------------------------------------------------
+static hipEvent_t treeEvent( Context* ctx )
+{
+   TimingTree& t = ctx->tree;
+   if ( !t.spare.empty() )
+   {
+      hipEvent_t e = t.spare.back();
+      t.spare.pop_back();
+      return e;
+   }
+   hipEvent_t e = nullptr;
+   HG_CHECK_HIP( hipEventCreate( &e ) );
+   return e;
+}
+
+// reads the elapsed times of the closed regions (waits for the stream)
+static void treeCollect( Context* ctx )
+{
+   TimingTree& t = ctx->tree;
+   if ( t.closed.empty() )
+      return;
+   HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+   for ( auto& c : t.closed )
+   {
+      float ms = 0.f;
+      HG_CHECK_HIP( hipEventElapsedTime( &ms, c.a, c.b ) );
+      auto& s = t.sums[c.path];
+      s.first += 1;
+      s.second += double( ms );
+      t.spare.push_back( c.a );
+      t.spare.push_back( c.b );
+   }
+   t.closed.clear();
+}
+
+static void treeRelease( Context* ctx )
+{
+   TimingTree& t = ctx->tree;
+   for ( auto& o : t.open )
+      t.spare.push_back( o.ev );
+   for ( auto& c : t.closed )
+   {
+      t.spare.push_back( c.a );
+      t.spare.push_back( c.b );
+   }
+   t.open.clear();
+   t.closed.clear();
+   t.stack.clear();
+   for ( hipEvent_t e : t.spare )
+      (void) hipEventDestroy( e );
+   t.spare.clear();
 }
 
 const char* hg_last_error( const hg_ctx* h )
@@ -1383,6 +1489,108 @@ int hg_function_dot_batched( hg_ctx* h, hg_handle a, hg_handle b, int level, int
       if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off || fa.lv != fb.lv )
          throw Error( HG_ERR_ARG, "dot needs functions of one storage, value type and vector count" );
       dotDeviceBatched( ctx, fa.lat( ctx, level ), fa.ptr( level ), fb.ptr( level ), fa.dtype, flag, fa.lv, out );
+   } );
+}
+
+int hg_timing_enable( hg_ctx* h, int enable )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !enable && !ctx->tree.open.empty() )
+         throw Error( HG_ERR_STATE, "timing: regions are still open" );
+      ctx->tree.enabled = enable != 0;
+   } );
+}
+
+int hg_timing_start( hg_ctx* h, const char* name )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      TimingTree& t = ctx->tree;
+      if ( !t.enabled || ctx->capturing )
+         return;
+      if ( !name || !*name || std::strchr( name, '/' ) )
+         throw Error( HG_ERR_ARG, "timing: region names are non-empty and contain no '/'" );
+      std::string path = t.stack.empty() ? std::string( name ) : t.open.back().path + "/" + name;
+      hipEvent_t  e    = treeEvent( ctx );
+      HG_CHECK_HIP( hipEventRecord( e, ctx->stream ) );
+      t.open.push_back( { std::move( path ), e } );
+      t.stack.push_back( name );
+   } );
+}
+
+int hg_timing_stop( hg_ctx* h, const char* name )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      TimingTree& t = ctx->tree;
+      if ( !t.enabled || ctx->capturing )
+         return;
+      if ( !name || t.stack.empty() || t.stack.back() != name )
+         throw Error( HG_ERR_STATE, std::string( "timing: stop of a region that is not the innermost open one: " ) +
+                                        ( name ? name : "(null)" ) );
+      hipEvent_t e = treeEvent( ctx );
+      HG_CHECK_HIP( hipEventRecord( e, ctx->stream ) );
+      t.closed.push_back( { t.open.back().path, t.open.back().ev, e } );
+      t.open.pop_back();
+      t.stack.pop_back();
+      if ( t.closed.size() >= 4096 )
+         treeCollect( ctx );
+   } );
+}
+
+int hg_timing_report( hg_ctx* h, char* buf, int64_t size, int64_t* needed )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      treeCollect( ctx );
+      std::string out;
+      for ( const auto& kv : ctx->tree.sums )
+      {
+         char line[64];
+         std::snprintf( line, sizeof( line ), "\t%lld\t%.6f\n", (long long) kv.second.first, kv.second.second );
+         out += kv.first + line;
+      }
+      if ( needed )
+         *needed = int64_t( out.size() ) + 1;
+      if ( buf && size > 0 )
+      {
+         const size_t n = std::min< size_t >( out.size(), size_t( size - 1 ) );
+         std::memcpy( buf, out.data(), n );
+         buf[n] = 0;
+      }
+   } );
+}
+
+int hg_timing_reset( hg_ctx* h )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      treeCollect( ctx );
+      ctx->tree.sums.clear();
+   } );
+}
+
+int hg_function_dot_batched_noalloc( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, double* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      Function& fa = ctx->function( a );
+      Function& fb = ctx->function( b );
+      checkLevel( fa, level );
+      checkLevel( fb, level );
+      if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off || fa.lv != fb.lv )
+         throw Error( HG_ERR_ARG, "dot needs functions of one storage, value type and vector count" );
+      Lattice&     lat    = fa.lat( ctx, level );
+      const int    nv     = 1 << fa.lv;
+      const size_t needed = ( size_t( dotBlocks( lat, flag, fa.lv ) ) << fa.lv ) + size_t( nv );
+      if ( needed > kScratchDoubles )
+         throw Error( HG_ERR_ARG, "dot_noalloc: " + std::to_string( needed ) + " partial sums exceed the scratch array of " +
+                                      std::to_string( kScratchDoubles ) + " doubles" );
+      double* ptr = scratchOf( ctx->device );
+      double* res = ptr + ( size_t( kDotBlocks ) << kMaxLv );
+      dotLaunchWith( ctx, lat, fa.ptr( level ), fb.ptr( level ), fa.dtype, flag, fa.lv, ptr, res );
+      HG_CHECK_HIP( hipMemcpyAsync( ctx->hostResult, res, sizeof( double ) * size_t( nv ), hipMemcpyDeviceToHost,
+                                    ctx->stream ) );
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+      for ( int k = 0; k < nv; ++k )
+         out[k] = ctx->hostResult[k];
    } );
 }
 

@@ -157,6 +157,30 @@ struct Function
    Lattice& lat( Context* ctx, int L ) { return st->lattice( ctx, L + off ); }
 };
 
+// Nested named timing regions on a context's stream (the role of waLBerla's
+// WcTimingTree in HyTeG's solvers): each region is bracketed by two HIP events;
+// elapsed times are read and summed per path ("A/B/C") when a report is asked
+// for, so recording a region does not wait for the device.
+struct TimingTree
+{
+   bool                       enabled = false;
+   std::vector< std::string > stack;
+   struct Open
+   {
+      std::string path;
+      hipEvent_t  ev;
+   };
+   struct Closed
+   {
+      std::string path;
+      hipEvent_t  a, b;
+   };
+   std::vector< Open >                                   open;
+   std::vector< Closed >                                 closed;
+   std::vector< hipEvent_t >                             spare;
+   std::map< std::string, std::pair< int64_t, double > > sums; // path -> (count, ms)
+};
+
 struct Context
 {
    int                                                device = 0;
@@ -183,6 +207,7 @@ struct Context
    bool       timing = false;
    hipEvent_t ev0 = nullptr, ev1 = nullptr;
    double     lastMs = 0.0;
+   TimingTree tree;
 
    // Freed device blocks are kept for allocations of the same size (the
    // context's work runs on its one stream, so a reused block's earlier work is

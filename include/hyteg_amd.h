@@ -159,6 +159,15 @@ int hg_function_interpolate_random( hg_ctx* ctx, hg_handle fn, uint64_t seed, do
 int hg_function_dot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
 /* per-vector dotGlobal of interleaved functions: out[nvec] */
 int hg_function_dot_batched( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
+/* hg_function_dot_batched without allocating device memory (no counterpart in the
+ * reference; for callers that cannot allocate: code between allocation-free
+ * set-up and teardown, tight serving loops).  Its partial sums go to a
+ * file-scope device array of 65,600 doubles (513 KiB) per GPU, held until the
+ * process ends, instead of the context's reduction buffer; a dot product needs
+ * at most 1024 x nvec + nvec of them, and larger requests are refused.  Every
+ * call on a GPU shares that array: calls that overlap on one GPU (several
+ * contexts or threads) race. */
+int hg_function_dot_batched_noalloc( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
 /* Device-resident scalars for Krylov loops (no counterpart in the reference,
  * whose dotGlobal returns every dot product to the host): each context holds
  * 256 FP64 slots in device memory.  hg_slots_alloc reserves n consecutive slots
@@ -333,6 +342,20 @@ int hg_p1_cg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int lev
  * (solvers/GeometricMultigridSolver.hpp:126-308, solvers/GaussSeidelSmoother.hpp:31-55) */
 int hg_p1_gmg_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b, int minLevel, int maxLevel, int pre, int post,
                      int cycles, double tol, int coarseMaxIter, double coarseTol, double* res_hist, int* cycles_done );
+
+/* ---- timing tree (walberla::WcTimingTree as HyTeG's solvers use it,
+ * GeometricMultigridSolver.hpp:200-306): nested named regions on the context's
+ * stream, each bracketed by two HIP events; while enabled, the C++ solvers open
+ * "Geometric Multigrid Solver", "Level <L>", "Smoother", "Restriction",
+ * "Prolongation" and "Coarse Grid Solver" like the reference.  The report lists
+ * one line per region path ("A/B/C"): path, tab, count, tab, total ms (device
+ * time), newline; *needed receives the report's size including the final zero.
+ * Regions opened during a graph capture are not recorded. */
+int hg_timing_enable( hg_ctx* ctx, int enable );
+int hg_timing_start( hg_ctx* ctx, const char* name );
+int hg_timing_stop( hg_ctx* ctx, const char* name );
+int hg_timing_report( hg_ctx* ctx, char* buf, int64_t size, int64_t* needed );
+int hg_timing_reset( hg_ctx* ctx );
 
 /* ---- device time (ms, HIP events on the context stream) of the dominant
  * kernel of the last entry point, recorded while event timing is enabled ---- */

@@ -1204,32 +1204,60 @@ class GeometricMultigridSolver : public Solver< OperatorType >
    void solve( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level ) override
    {
       invokedLevel_ = level;
+      hg_ctx* ctx   = x.ctx();
+      region( ctx, "Geometric Multigrid Solver", true );
       solveRecursively( A, x, b, level );
+      region( ctx, "Geometric Multigrid Solver", false );
    }
 
  private:
+   // the timing regions of GeometricMultigridSolver.hpp:200-306 (recorded while
+   // the context's timing tree is enabled)
+   static void region( hg_ctx* ctx, const std::string& name, bool start )
+   {
+      check( ctx, start ? hg_timing_start( ctx, name.c_str() ) : hg_timing_stop( ctx, name.c_str() ), "timing" );
+   }
+
    void solveRecursively( const OperatorType& A, const FunctionType& x, const FunctionType& b, uint_t level )
    {
+      hg_ctx*           ctx = x.ctx();
+      const std::string lv  = "Level " + std::to_string( level );
       if ( level == minLevel_ )
       {
+         region( ctx, lv, true );
+         region( ctx, "Coarse Grid Solver", true );
          coarse_->solve( A, x, b, minLevel_ );
+         region( ctx, "Coarse Grid Solver", false );
+         region( ctx, lv, false );
          return;
       }
+      region( ctx, lv, true );
       const uint_t preSteps = pre_ + inc_ * ( invokedLevel_ - level );
+      region( ctx, "Smoother", true );
       for ( uint_t i = 0; i < preSteps; ++i )
          smoother_->solve( A, x, b, level );
+      region( ctx, "Smoother", false );
       A.apply( x, tmp_, level, flag_ );
       tmp_.assign( { 1.0, -1.0 }, { b, tmp_ }, level, flag_ );
+      region( ctx, "Restriction", true );
       restriction_->restrict( tmp_, level, flag_ );
+      region( ctx, "Restriction", false );
       b.assign( { 1.0 }, { tmp_ }, level - 1, flag_ );
       x.interpolate( 0, level - 1 );
+      region( ctx, lv, false );
       solveRecursively( A, x, b, level - 1 );
       if ( cycleType_ == CycleType::WCYCLE )
          solveRecursively( A, x, b, level - 1 );
+      region( ctx, lv, true );
+      region( ctx, "Prolongation", true );
       prolongation_->prolongateAndAdd( x, level - 1, flag_ );
+      region( ctx, "Prolongation", false );
       const uint_t postSteps = post_ + inc_ * ( invokedLevel_ - level );
+      region( ctx, "Smoother", true );
       for ( uint_t i = 0; i < postSteps; ++i )
          smoother_->solve( A, x, b, level );
+      region( ctx, "Smoother", false );
+      region( ctx, lv, false );
    }
 
    uint_t                                                  minLevel_, maxLevel_, pre_, post_, inc_, invokedLevel_ = 0;
@@ -1365,6 +1393,7 @@ class P2P1TaylorHoodFunction
    }
    const P2VectorFunction< VT >& uvw() const { return *uvw_; }
    const P1Function< VT >&       p() const { return *p_; }
+   hg_ctx*                       ctx() const { return uvw_->ctx(); }
 
    void assign( const std::vector< VT >& scalars,
                 const std::vector< std::reference_wrapper< const P2P1TaylorHoodFunction > >& fs, uint_t level,
