@@ -127,7 +127,6 @@ SIGNATURES = {
     "hg_serve_stream_solutions": [_i, _i64, _p, _p, _p, _p, _d, _i, _d, _p, _p, _p, _p, _p, _p],
     "hg_serve_request_coefficient": [C.c_uint64, _p, _p, _p],
     "hg_p2_low_precision_form": [_p, _i64, _i, _i],
-    "hg_function_dot_batched_noalloc": [_p, _i64, _i64, _i, _i, _p],
     "hg_timing_enable": [_p, _i],
     "hg_timing_start": [_p, C.c_char_p],
     "hg_timing_stop": [_p, C.c_char_p],

@@ -584,43 +584,6 @@ static void dotLaunch( Context* ctx, Lattice& lat, const void* a, const void* b,
    dotLaunchWith( ctx, lat, a, b, dtype, flag, lv, ctx->partials.p, dev );
 }
 
-// ---- dot products without allocation ------------------------------------------
-// Partial sums and results of hg_function_dot_batched_noalloc live in this
-// array: one copy per device, held until the process ends.  The name is a
-// host-side stand-in; host code reaches each device's copy through the address
-// recorded by recordScratch at context creation (scratchOf).
-constexpr size_t kScratchDoubles = ( size_t( kDotBlocks ) << kMaxLv ) + ( size_t( 1 ) << kMaxLv );
-static __device__ __align__( 256 ) double scratch[kScratchDoubles];
-
-static std::mutex&               scratchMutex()
-{
-   static std::mutex m;
-   return m;
-}
-static std::map< int, double* >& scratchTable()
-{
-   static std::map< int, double* > t;
-   return t;
-}
-// once per device, with that device current
-static void recordScratch( int device )
-{
-   std::lock_guard< std::mutex > lk( scratchMutex() );
-   if ( scratchTable().count( device ) )
-      return;
-   double* ptr = nullptr;
-   HG_CHECK_HIP( hipGetSymbolAddress( (void**) &ptr, HIP_SYMBOL( scratch ) ) );
-   scratchTable()[device] = ptr;
-}
-static double* scratchOf( int device )
-{
-   std::lock_guard< std::mutex > lk( scratchMutex() );
-   auto                          it = scratchTable().find( device );
-   if ( it == scratchTable().end() )
-      throw Error( HG_ERR_STATE, "no reduction scratch recorded for this device" );
-   return it->second;
-}
-
 void dotDeviceBatched( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv, double* out )
 {
    const int nv  = 1 << lv;
@@ -656,8 +619,30 @@ Function& Context::function( int64_t h )
       throw Error( -3, "unknown function handle" );
    return *it->second;
 }
+__global__ void kZeroBytes( uint8_t* __restrict__ p, size_t bytes )
+{
+   const size_t i  = ( size_t( blockIdx.x ) * blockDim.x + threadIdx.x ) * 16;
+   if ( i + 16 <= bytes )
+      *reinterpret_cast< uint4* >( p + i ) = make_uint4( 0u, 0u, 0u, 0u );
+   else
+      for ( size_t k = i; k < bytes; ++k )
+         p[k] = 0;
+}
+
+void zeroDevice( Context* c, void* p, size_t bytes )
+{
+   if ( bytes == 0 )
+      return;
+   // device allocations are 256-byte aligned: 16-byte stores, a byte tail
+   kZeroBytes<<< gridFor( int64_t( ( bytes + 15 ) / 16 ) ), kBlock, 0, c->stream >>>( static_cast< uint8_t* >( p ),
+                                                                                     bytes );
+   HG_CHECK_HIP( hipGetLastError() );
+}
+
 void* Context::deviceAlloc( size_t bytes )
 {
+   if ( capturing )
+      throw Error( HG_ERR_STATE, "device allocation while a graph capture is open" );
    auto it = pool.find( bytes );
    if ( it != pool.end() )
    {
@@ -673,6 +658,11 @@ void* Context::deviceAlloc( size_t bytes )
 }
 void Context::deviceFree( void* p, size_t bytes )
 {
+   if ( capturing )
+   {
+      heldByCapture.push_back( p );
+      return;
+   }
    if ( bytes <= kPoolBlock && poolBytes + bytes <= kPoolBytes )
    {
       pool.emplace( bytes, p );
@@ -815,7 +805,6 @@ int hg_ctx_create( int device, hg_ctx** out )
          HG_CHECK_HIP( hipEventCreate( &ctx->ev1 ) );
       }
       ctx->partials.alloc( ctx, ( size_t( kDotBlocks ) << kMaxLv ) + ( 1 << kMaxLv ) );
-      recordScratch( device );
       ctx->slots.alloc( ctx, kSlots );
       ctx->slotUsed.assign( kSlots, 0 );
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
@@ -860,6 +849,9 @@ int hg_ctx_destroy( hg_ctx* h )
    ctx->drainPool();
    {
       DeviceGateLock gate;
+      for ( void* p : ctx->heldByCapture )
+         (void) hipFree( p );
+      ctx->heldByCapture.clear();
       if ( ctx->hostResult )
          (void) hipHostFree( ctx->hostResult );
       if ( ctx->ev0 )
@@ -1563,34 +1555,6 @@ int hg_timing_reset( hg_ctx* h )
    return hg_guard( h, [&]( Context* ctx ) {
       treeCollect( ctx );
       ctx->tree.sums.clear();
-   } );
-}
-
-int hg_function_dot_batched_noalloc( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, double* out )
-{
-   return hg_guard( h, [&]( Context* ctx ) {
-      if ( !out )
-         throw Error( HG_ERR_ARG, "null output" );
-      Function& fa = ctx->function( a );
-      Function& fb = ctx->function( b );
-      checkLevel( fa, level );
-      checkLevel( fb, level );
-      if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off || fa.lv != fb.lv )
-         throw Error( HG_ERR_ARG, "dot needs functions of one storage, value type and vector count" );
-      Lattice&     lat    = fa.lat( ctx, level );
-      const int    nv     = 1 << fa.lv;
-      const size_t needed = ( size_t( dotBlocks( lat, flag, fa.lv ) ) << fa.lv ) + size_t( nv );
-      if ( needed > kScratchDoubles )
-         throw Error( HG_ERR_ARG, "dot_noalloc: " + std::to_string( needed ) + " partial sums exceed the scratch array of " +
-                                      std::to_string( kScratchDoubles ) + " doubles" );
-      double* ptr = scratchOf( ctx->device );
-      double* res = ptr + ( size_t( kDotBlocks ) << kMaxLv );
-      dotLaunchWith( ctx, lat, fa.ptr( level ), fb.ptr( level ), fa.dtype, flag, fa.lv, ptr, res );
-      HG_CHECK_HIP( hipMemcpyAsync( ctx->hostResult, res, sizeof( double ) * size_t( nv ), hipMemcpyDeviceToHost,
-                                    ctx->stream ) );
-      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
-      for ( int k = 0; k < nv; ++k )
-         out[k] = ctx->hostResult[k];
    } );
 }
 

@@ -219,6 +219,11 @@ struct Context
    static constexpr size_t            kPoolBlock = size_t( 256 ) << 20;
    std::multimap< size_t, void* >     pool;
    size_t                             poolBytes = 0;
+   // blocks released while a graph capture is open: the graph may still use
+   // them, so they are only freed when the context is destroyed
+   std::vector< void* >               heldByCapture;
+   // device memory for a DevBuf; refused (HG_ERR_STATE) while this context
+   // captures a graph: an allocation's zero fill would become part of the graph
    void*                              deviceAlloc( size_t bytes );
    void                               deviceFree( void* p, size_t bytes );
    void                               drainPool();
@@ -260,6 +265,10 @@ struct DeviceGateLock
    }
 };
 
+// zero `bytes` bytes at p (a device allocation) with a fill kernel on the
+// context's stream
+void zeroDevice( Context* c, void* p, size_t bytes );
+
 // ---- DevBuf members (need the complete Context) ------------------------
 template < class T >
 inline void DevBuf< T >::alloc( Context* c, size_t count )
@@ -270,7 +279,7 @@ inline void DevBuf< T >::alloc( Context* c, size_t count )
    if ( count == 0 )
       return;
    p = static_cast< T* >( c->deviceAlloc( count * sizeof( T ) ) );
-   HG_CHECK_HIP( hipMemsetAsync( p, 0, count * sizeof( T ), c->stream ) );
+   zeroDevice( c, p, count * sizeof( T ) );
    c->bytes += int64_t( count * sizeof( T ) );
 }
 template < class T >

@@ -313,9 +313,9 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
          {
             if ( !graph )
             {
-               check( ctx->get(), hg_graph_begin_capture( ctx->get() ), "capture" );
+               CaptureScope capture( ctx->get() );
                gmg.solve( A, X, B, L );
-               check( ctx->get(), hg_graph_end_capture( ctx->get(), &graph ), "capture" );
+               graph = capture.finish();
             }
             check( ctx->get(), hg_graph_launch( ctx->get(), graph ), "graph launch" );
          }
@@ -427,9 +427,9 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
             {
                if ( !graph )
                {
-                  check( ctx->get(), hg_graph_begin_capture( ctx->get() ), "capture" );
+                  CaptureScope capture( ctx->get() );
                   gmg.solve( A, x, b, L );
-                  check( ctx->get(), hg_graph_end_capture( ctx->get(), &graph ), "capture" );
+                  graph = capture.finish();
                }
                check( ctx->get(), hg_graph_launch( ctx->get(), graph ), "graph launch" );
             }

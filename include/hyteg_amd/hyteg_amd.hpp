@@ -1174,6 +1174,42 @@ class FP64CoarseSolver : public Solver< OperatorType >
 };
 
 // GeometricMultigridSolver (GeometricMultigridSolver.hpp:126-308)
+// An open HIP graph capture of a context's stream (hg_graph_begin_capture).
+// finish() ends it and returns the instantiated graph; leaving the scope
+// without finish() (an exception inside the captured work) ends the capture and
+// discards its graph, so the context's stream and the process-wide capture gate
+// are released on every path.  No counterpart in the reference.
+class CaptureScope
+{
+ public:
+   explicit CaptureScope( hg_ctx* ctx )
+   : ctx_( ctx )
+   {
+      check( ctx_, hg_graph_begin_capture( ctx_ ), "begin capture" );
+   }
+   CaptureScope( const CaptureScope& )            = delete;
+   CaptureScope& operator=( const CaptureScope& ) = delete;
+   hg_handle     finish()
+   {
+      hg_handle g = 0;
+      open_       = false;
+      check( ctx_, hg_graph_end_capture( ctx_, &g ), "end capture" );
+      return g;
+   }
+   ~CaptureScope()
+   {
+      if ( !open_ )
+         return;
+      hg_handle g = 0;
+      if ( hg_graph_end_capture( ctx_, &g ) == HG_OK && g )
+         (void) hg_graph_destroy( ctx_, g );
+   }
+
+ private:
+   hg_ctx* ctx_;
+   bool    open_ = true;
+};
+
 template < class OperatorType >
 class GeometricMultigridSolver : public Solver< OperatorType >
 {
