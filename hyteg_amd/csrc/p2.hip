@@ -32,6 +32,8 @@ __constant__ double cQW[11];
 __constant__ double cQ4[11][4];  // 4 lam_k at quadrature point q
 __constant__ double cQ4m[11][4]; // 4 lam_k - 1
 __constant__ float  cQWf[11], cQ4f[11][4], cQ4mf[11][4]; // the same in FP32
+__constant__ double cQL[11][3];  // lam_1, lam_2, lam_3 at quadrature point q
+__constant__ float  cQLf[11][3];
 
 template < class R >
 struct QuadConst;
@@ -41,6 +43,7 @@ struct QuadConst< double >
    __device__ static const double* w() { return cQW; }
    __device__ static const double* a4( int q ) { return cQ4[q]; }
    __device__ static const double* b4( int q ) { return cQ4m[q]; }
+   __device__ static const double* lam( int q ) { return cQL[q]; }
 };
 template <>
 struct QuadConst< float >
@@ -48,6 +51,7 @@ struct QuadConst< float >
    __device__ static const float* w() { return cQWf; }
    __device__ static const float* a4( int q ) { return cQ4f[q]; }
    __device__ static const float* b4( int q ) { return cQ4mf[q]; }
+   __device__ static const float* lam( int q ) { return cQLf[q]; }
 };
 
 // Xiao-Gimbutas degree-4 rule on the tetrahedron, 11 points; the weights are
@@ -569,42 +573,86 @@ __global__ void kP2FinishAccum( int64_t sc1, int32_t w1, int rmw, float* __restr
 }
 
 // y = A_e u for the micro-cell of th: the element action of
-// p2_div_k_grad_blending_q4 in barycentric form (11-point rule); kq(q) gives k at
-// quadrature point q of the micro-cell
+// p2_div_k_grad_blending_q4 (11-point rule); kq(q) gives k at quadrature point q
+// of the micro-cell.
+//
+// With barycentric coordinates lam_0..3 (lam_0 = 1 - lam_1 - lam_2 - lam_3), the
+// P2 function u = sum_k u_k lam_k (2 lam_k - 1) + sum_(i<j) 4 u_ij lam_i lam_j has
+// the computational-space gradient  g = sum_(k=1..3) d_k grad lam_k  with
+// d_k = du/dlam_k - du/dlam_0, which is affine in (lam_1, lam_2, lam_3):
+//   d = d0 + 4 B lam',  d0_k = 4 u_0k - 3 u_0 - u_k,
+//   B_kk = u_k + u_0 - 2 u_0k,  B_kj = u_kj - u_0j - u_0k + u_0 (k != j),
+// so g(q) = v0 + V lam'_q with v0 = Dc^T d0 and V = 4 Dc^T B, formed once per
+// micro-cell.  The flux m_q = w_q k_q |det DF J| (DF^-1 DF^-T) g(q) enters the
+// result only through Tm = sum_q m_q lam'_q^T and Sm = sum_q m_q; with
+// T = Dc Tm, S = Dc Sm (the sums of lam'_j (grad lam_k . m) and grad lam_k . m),
+// the element result is the transpose of the map u -> d applied to them:
+//   y_0 = 4 sum(T) - 3 sum(S),  y_k = 4 T_kk - S_k,
+//   y_0k = 4 (S_k - sum_j T_kj - sum_j T_jk),  y_kj = 4 (T_kj + T_jk).
+// Per quadrature point that leaves 9 FMA for g and 12 for the sums, besides
+// the blending.  The shell map F(x) = q(x) x has DF = q I + x w^T (w = grad q),
+// inverted in closed form (Sherman-Morrison); det DF = q^2 beta with
+// beta = q + w . x = s (n . x), s = sign(lam) dr / |x|, so the powers of q cancel.
 template < class R, class KQ >
 __device__ __forceinline__ void p2ElementAction( const P2CellGeomT< R >& g, const TileThread< R >& th, const R u[10],
                                                  R y[10], KQ&& kq )
 {
-   using QC = QuadConst< R >;
+   using QC      = QuadConst< R >;
+   const R*  Dc  = g.Dc[th.t];
+   const R   adet = g.adet[th.t];
+   R         v0[3], V[3][3];
+   {
+      // d0 and B (symmetric) from the element values (edges 4..9 in the order
+      // (0,1), (0,2), (0,3), (1,2), (1,3), (2,3))
+      const R d0[3] = { R( 4 ) * u[4] - R( 3 ) * u[0] - u[1], R( 4 ) * u[5] - R( 3 ) * u[0] - u[2],
+                        R( 4 ) * u[6] - R( 3 ) * u[0] - u[3] };
+      const R b00 = u[1] + u[0] - R( 2 ) * u[4], b11 = u[2] + u[0] - R( 2 ) * u[5], b22 = u[3] + u[0] - R( 2 ) * u[6];
+      const R b01 = u[7] - u[5] - u[4] + u[0], b02 = u[8] - u[6] - u[4] + u[0], b12 = u[9] - u[6] - u[5] + u[0];
+      const R B[3][3] = { { b00, b01, b02 }, { b01, b11, b12 }, { b02, b12, b22 } };
 #pragma unroll
-   for ( int i = 0; i < 10; ++i )
-      y[i] = R( 0 );
-   const R* Dc   = g.Dc[th.t];
-   const R  adet = g.adet[th.t];
+      for ( int d = 0; d < 3; ++d )
+      {
+         v0[d] = Dc[d] * d0[0] + Dc[3 + d] * d0[1] + Dc[6 + d] * d0[2];
+#pragma unroll
+         for ( int j = 0; j < 3; ++j )
+            V[d][j] = R( 4 ) * ( Dc[d] * B[0][j] + Dc[3 + d] * B[1][j] + Dc[6 + d] * B[2][j] );
+      }
+   }
+   R Tm[3][3], Sm[3];
+#pragma unroll
+   for ( int d = 0; d < 3; ++d )
+   {
+      Sm[d] = R( 0 );
+#pragma unroll
+      for ( int j = 0; j < 3; ++j )
+         Tm[d][j] = R( 0 );
+   }
 #pragma unroll 1
    for ( int q = 0; q < 11; ++q )
    {
-      const R* a4 = QC::a4( q );
-      const R* b4 = QC::b4( q );
-      const R  g0 = b4[0] * u[0] + a4[1] * u[4] + a4[2] * u[5] + a4[3] * u[6];
-      const R  g1 = b4[1] * u[1] + a4[0] * u[4] + a4[2] * u[7] + a4[3] * u[8];
-      const R  g2 = b4[2] * u[2] + a4[0] * u[5] + a4[1] * u[7] + a4[3] * u[9];
-      const R  g3 = b4[3] * u[3] + a4[0] * u[6] + a4[1] * u[8] + a4[2] * u[9];
-      const R  d1 = g1 - g0, d2 = g2 - g0, d3 = g3 - g0;
+      const R* lq = QC::lam( q );
       R        v[3];
 #pragma unroll
       for ( int d = 0; d < 3; ++d )
-         v[d] = d1 * Dc[d] + d2 * Dc[3 + d] + d3 * Dc[6 + d];
-      const R* xq   = g.XQ[th.t][q];
-      const R  x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
-      const R  kqv  = kq( q );
-      R        cc   = QC::w()[q] * adet * kqv;
+         v[d] = v0[d] + V[d][0] * lq[0] + V[d][1] * lq[1] + V[d][2] * lq[2];
+      const R* xq = g.XQ[th.t][q];
+      R        cc = QC::w()[q] * adet * kq( q );
       if ( g.kind )
       {
-         R qq, w[3], beta;
-         shellScalars( g, x, th.ls0 + xq[3], qq, w, beta );
-         const R ib = fastRcp( beta );
-         const R t1 = ( x[0] * v[0] + x[1] * v[1] + x[2] * v[2] ) * ib;
+         const R x[3] = { th.p0[0] + xq[0], th.p0[1] + xq[1], th.p0[2] + xq[2] };
+         const R ls   = th.ls0 + xq[3];
+         const R r2   = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+         const R irho = fastRsqrt( r2 );
+         const R qq   = ( g.r0 + fabs( ls ) * g.dr ) * irho;
+         const R s    = copysign( g.dr, ls ) * irho;
+         const R qr2  = qq * ( irho * irho );
+         R       w[3];
+#pragma unroll
+         for ( int d = 0; d < 3; ++d )
+            w[d] = s * g.nl[d] - qr2 * x[d];
+         const R beta = s * ( ls + g.lam0 );
+         const R ib   = fastRcp( beta );
+         const R t1   = ( x[0] * v[0] + x[1] * v[1] + x[2] * v[2] ) * ib;
 #pragma unroll
          for ( int d = 0; d < 3; ++d )
             v[d] -= w[d] * t1;
@@ -616,22 +664,35 @@ __device__ __forceinline__ void p2ElementAction( const P2CellGeomT< R >& g, cons
       }
 #pragma unroll
       for ( int d = 0; d < 3; ++d )
-         v[d] *= cc;
-      const R e1 = Dc[0] * v[0] + Dc[1] * v[1] + Dc[2] * v[2];
-      const R e2 = Dc[3] * v[0] + Dc[4] * v[1] + Dc[5] * v[2];
-      const R e3 = Dc[6] * v[0] + Dc[7] * v[1] + Dc[8] * v[2];
-      const R e0 = -( e1 + e2 + e3 );
-      y[0] += b4[0] * e0;
-      y[1] += b4[1] * e1;
-      y[2] += b4[2] * e2;
-      y[3] += b4[3] * e3;
-      y[4] += a4[1] * e0 + a4[0] * e1;
-      y[5] += a4[2] * e0 + a4[0] * e2;
-      y[6] += a4[3] * e0 + a4[0] * e3;
-      y[7] += a4[2] * e1 + a4[1] * e2;
-      y[8] += a4[3] * e1 + a4[1] * e3;
-      y[9] += a4[3] * e2 + a4[2] * e3;
+      {
+         const R m = cc * v[d];
+         Sm[d] += m;
+#pragma unroll
+         for ( int j = 0; j < 3; ++j )
+            Tm[d][j] += lq[j] * m;
+      }
    }
+   R T[3][3], S[3];
+#pragma unroll
+   for ( int k = 0; k < 3; ++k )
+   {
+      S[k] = Dc[3 * k] * Sm[0] + Dc[3 * k + 1] * Sm[1] + Dc[3 * k + 2] * Sm[2];
+#pragma unroll
+      for ( int j = 0; j < 3; ++j )
+         T[k][j] = Dc[3 * k] * Tm[0][j] + Dc[3 * k + 1] * Tm[1][j] + Dc[3 * k + 2] * Tm[2][j];
+   }
+   const R rs[3] = { T[0][0] + T[0][1] + T[0][2], T[1][0] + T[1][1] + T[1][2], T[2][0] + T[2][1] + T[2][2] };
+   const R cs[3] = { T[0][0] + T[1][0] + T[2][0], T[0][1] + T[1][1] + T[2][1], T[0][2] + T[1][2] + T[2][2] };
+   y[0]          = R( 4 ) * ( rs[0] + rs[1] + rs[2] ) - R( 3 ) * ( S[0] + S[1] + S[2] );
+#pragma unroll
+   for ( int k = 0; k < 3; ++k )
+   {
+      y[1 + k] = R( 4 ) * T[k][k] - S[k];
+      y[4 + k] = R( 4 ) * ( S[k] - rs[k] - cs[k] );
+   }
+   y[7] = R( 4 ) * ( T[0][1] + T[1][0] );
+   y[8] = R( 4 ) * ( T[0][2] + T[2][0] );
+   y[9] = R( 4 ) * ( T[1][2] + T[2][1] );
 }
 
 // y = A x on the cell lattices of level L+1 (halo positions included): the
@@ -733,12 +794,11 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
    }
 }
 
-// stored coefficients: amdgpu_waves_per_eu(6) gives the register budget of 6
-// waves per SIMD (80 VGPRs, a few spilled), the most the 39.6 KB of LDS per
-// workgroup allows; 5 waves at 88 VGPRs measured 47.4 ms per level-6 shell
-// apply, 6 waves 41.7 ms
+// stored coefficients, the compiler's own register budget: the FP64 kernel
+// holds 94 VGPRs (5 waves per SIMD) without spills; a budget of 80 (6 waves)
+// spills the micro-cell's vertex and shell coordinate inside the quadrature loop
 template < class T, class K >
-__global__ __launch_bounds__( kTileThreads ) __attribute__( ( amdgpu_waves_per_eu( 6, 8 ) ) ) void kP2TileApplyStored(
+__global__ __launch_bounds__( kTileThreads ) void kP2TileApplyStored(
     const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
     const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
     const T* __restrict__ src, T* __restrict__ dst, double alpha )
@@ -1245,6 +1305,16 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQWf ), wf, sizeof( wf ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4f ), q4f, sizeof( q4f ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQ4mf ), q4mf, sizeof( q4mf ) ) );
+         double ql[11][3];
+         float  qlf[11][3];
+         for ( int q = 0; q < 11; ++q )
+            for ( int k = 0; k < 3; ++k )
+            {
+               ql[q][k]  = kQP[q][k + 1];
+               qlf[q][k] = float( kQP[q][k + 1] );
+            }
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQL ), ql, sizeof( ql ) ) );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQLf ), qlf, sizeof( qlf ) ) );
       } );
       Storage&        st = ctx->storage( s );
       const Topology& t  = st.topo;

@@ -94,20 +94,21 @@ __device__ __forceinline__ TileCoord tileOf( int64_t lb, int64_t mm )
    return tc;
 }
 
-// 1/b and 1/sqrt(b) from the hardware estimates (relative error below 2^-22)
-// refined by two Newton steps (error below 2^-52 after rounding)
+// 1/b and 1/sqrt(b) from the hardware estimates (relative error e below 2^-22)
+// refined by one third-order step each (error O(e^3), below the rounding of the
+// result): r (1 + e + e^2) with e = 1 - b r, and y (1 + e/2 + 3e^2/8) with
+// e = 1 - b y^2
 __device__ __forceinline__ double rcpNewton( double b )
 {
-   double r = __builtin_amdgcn_rcp( b );
-   r        = fma( r, fma( -b, r, 1.0 ), r );
-   return fma( r, fma( -b, r, 1.0 ), r );
+   const double r = __builtin_amdgcn_rcp( b );
+   const double e = fma( -b, r, 1.0 );
+   return fma( r, fma( e, e, e ), r );
 }
 __device__ __forceinline__ double rsqrtNewton( double b )
 {
-   double       y = __builtin_amdgcn_rsq( b );
-   const double h = 0.5 * b;
-   y              = y * fma( -h * y, y, 1.5 );
-   return y * fma( -h * y, y, 1.5 );
+   const double y = __builtin_amdgcn_rsq( b );
+   const double e = fma( -b, y * y, 1.0 );
+   return fma( y, e * fma( e, 0.375, 0.5 ), y );
 }
 // FP64: refined estimates; FP32: the hardware estimates (about 1 ulp)
 __device__ __forceinline__ double fastRcp( double b ) { return rcpNewton( b ); }

@@ -18,11 +18,13 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 # relative max-norm bounds of one reduced-precision apply against the FP64 result of
-# the same element form (FP16: input and output rounded to binary16); the reduced
-# form's FP32 element action sums terms about ten times the result, one decimal
-# digit more than the reference form's in FP32
+# the same element form (FP16: input and output rounded to binary16).  The reduced
+# form contracts the rounded FP32 metric with barycentric differences whose terms
+# cancel to about 1/100 of their size, so its FP32 apply lands near 200 units of
+# FP32 rounding (measured on the MI355X: 1.2e-5 on shell levels 2 and 3); the
+# reference form in FP32 stays within 2e-6
 APPLY_TOL = {hg.F32: 2e-6, hg.F16: 4e-3}
-REDUCED_TOL = {hg.F32: 1e-5, hg.F16: 4e-3}
+REDUCED_TOL = {hg.F32: 4e-5, hg.F16: 4e-3}
 
 
 @pytest.fixture(scope="module")
