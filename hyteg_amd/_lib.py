@@ -93,6 +93,7 @@ SIGNATURES = {
     "hg_stokes_gmg_solve": [_p, _i64, _p, _p, _i, _i, _i, _i, _i, _d, _i, _i, _d, _i, _d, _i, _i, _i, _d, _p, _p],
     "hg_function_copy_vector": [_p, _i64, _i, _i64, _i, _i],
     "hg_function_dot_batched": [_p, _i64, _i64, _i, _i, _p],
+    "hg_function_dot_batched_noalloc": [_p, _i64, _i64, _i, _i, _p],
     "hg_function_scale_batched": [_p, _i64, _p, _i, _i],
     "hg_function_chebyshev_step": [_p, _i64, _i64, _i64, _i64, _i64, _d, _i, _i],
     "hg_slots_alloc": [_p, _i, _p],

@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "abi_util.hpp"
 #include "halo.hpp"
@@ -584,6 +586,43 @@ static void dotLaunch( Context* ctx, Lattice& lat, const void* a, const void* b,
    dotLaunchWith( ctx, lat, a, b, dtype, flag, lv, ctx->partials.p, dev );
 }
 
+// ---- dot products without allocation ------------------------------------------
+// Partial sums and results of hg_function_dot_batched_noalloc live in this
+// array: one copy per device, held until the process ends.  The name is a
+// host-side stand-in, not a device address: host code reaches each device's
+// copy only through the address recorded by recordScratch (scratchOf).
+constexpr size_t kScratchDoubles = ( size_t( kDotBlocks ) << kMaxLv ) + ( size_t( 1 ) << kMaxLv );
+static __device__ __align__( 256 ) double scratch[kScratchDoubles];
+
+static std::mutex& scratchMutex()
+{
+   static std::mutex m;
+   return m;
+}
+static std::map< int, double* >& scratchTable()
+{
+   static std::map< int, double* > t;
+   return t;
+}
+// once per device, with that device current (context creation)
+static void recordScratch( int device )
+{
+   std::lock_guard< std::mutex > lk( scratchMutex() );
+   if ( scratchTable().count( device ) )
+      return;
+   double* ptr = nullptr;
+   HG_CHECK_HIP( hipGetSymbolAddress( (void**) &ptr, HIP_SYMBOL( scratch ) ) );
+   scratchTable()[device] = ptr;
+}
+static double* scratchOf( int device )
+{
+   std::lock_guard< std::mutex > lk( scratchMutex() );
+   auto                          it = scratchTable().find( device );
+   if ( it == scratchTable().end() )
+      throw Error( HG_ERR_STATE, "no reduction scratch recorded for this device" );
+   return it->second;
+}
+
 void dotDeviceBatched( Context* ctx, Lattice& lat, const void* a, const void* b, int dtype, int flag, int lv, double* out )
 {
    const int nv  = 1 << lv;
@@ -805,6 +844,7 @@ int hg_ctx_create( int device, hg_ctx** out )
          HG_CHECK_HIP( hipEventCreate( &ctx->ev1 ) );
       }
       ctx->partials.alloc( ctx, ( size_t( kDotBlocks ) << kMaxLv ) + ( 1 << kMaxLv ) );
+      recordScratch( device );
       ctx->slots.alloc( ctx, kSlots );
       ctx->slotUsed.assign( kSlots, 0 );
       HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
@@ -1481,6 +1521,41 @@ int hg_function_dot_batched( hg_ctx* h, hg_handle a, hg_handle b, int level, int
       if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off || fa.lv != fb.lv )
          throw Error( HG_ERR_ARG, "dot needs functions of one storage, value type and vector count" );
       dotDeviceBatched( ctx, fa.lat( ctx, level ), fa.ptr( level ), fb.ptr( level ), fa.dtype, flag, fa.lv, out );
+   } );
+}
+
+int hg_function_dot_batched_noalloc( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, double* out )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      if ( !out )
+         throw Error( HG_ERR_ARG, "null output" );
+      Function& fa = ctx->function( a );
+      Function& fb = ctx->function( b );
+      checkLevel( fa, level );
+      checkLevel( fb, level );
+      if ( fa.st != fb.st || fa.dtype != fb.dtype || fa.off != fb.off || fa.lv != fb.lv )
+         throw Error( HG_ERR_ARG, "dot needs functions of one storage, value type and vector count" );
+      // the lattice of a function's level exists once the function does; looked
+      // up, never built here (building one allocates)
+      auto li = fa.st->lattices.find( level + fa.off );
+      if ( li == fa.st->lattices.end() )
+         throw Error( HG_ERR_STATE, "dot_noalloc: the level's lattice is not set up" );
+      Lattice&     lat    = *li->second;
+      const int    nv     = 1 << fa.lv;
+      const size_t needed = ( size_t( dotBlocks( lat, flag, fa.lv ) ) << fa.lv ) + size_t( nv );
+      if ( needed > kScratchDoubles )
+         throw Error( HG_ERR_ARG, "dot_noalloc: " + std::to_string( needed ) + " partial sums exceed the scratch array of " +
+                                      std::to_string( kScratchDoubles ) + " doubles" );
+      double* ptr = scratchOf( ctx->device );
+      double* res = ptr + ( size_t( kDotBlocks ) << kMaxLv );
+      // every partial that kDotFinal reads is written by kDotPartial first, so
+      // the array needs no clearing between calls
+      dotLaunchWith( ctx, lat, fa.ptr( level ), fb.ptr( level ), fa.dtype, flag, fa.lv, ptr, res );
+      HG_CHECK_HIP( hipMemcpyAsync( ctx->hostResult, res, sizeof( double ) * size_t( nv ), hipMemcpyDeviceToHost,
+                                    ctx->stream ) );
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+      for ( int k = 0; k < nv; ++k )
+         out[k] = ctx->hostResult[k];
    } );
 }
 

@@ -159,6 +159,18 @@ int hg_function_interpolate_random( hg_ctx* ctx, hg_handle fn, uint64_t seed, do
 int hg_function_dot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
 /* per-vector dotGlobal of interleaved functions: out[nvec] */
 int hg_function_dot_batched( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
+/* hg_function_dot_batched without allocating device memory.  No counterpart in
+ * the reference (its dotGlobal allocates nothing on a device); it is for callers
+ * that cannot allocate: code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The partial sums and the results go to a
+ * file-scope device array of 65,600 doubles (513 KiB) per GPU, held until the
+ * process ends, instead of the context's reduction buffer.  A dot product needs
+ * at most 1024 x nvec + nvec of them; larger requests are refused with
+ * HG_ERR_ARG.  Every call on a GPU shares that array and it keeps the last
+ * call's contents: calls that overlap on one GPU (several contexts or threads)
+ * race.  The function's level must already be set up (it is once the function
+ * exists). */
+int hg_function_dot_batched_noalloc( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, double* out );
 /* Device-resident scalars for Krylov loops (no counterpart in the reference,
  * whose dotGlobal returns every dot product to the host): each context holds
  * 256 FP64 slots in device memory.  hg_slots_alloc reserves n consecutive slots
