@@ -716,10 +716,11 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
    // read its 10 values) the accumulator of the element results
    __shared__ R sb[kBox3];
    __shared__ R sk[STORED ? 11 * kTileThreads : 1];
-   const int64_t lb = xcdBlock( ntiles );
-   if ( lb >= ntiles )
+   const int64_t li = xcdBlock( ntiles );
+   if ( li >= ntiles )
       return;
-   const TileCoord tc   = tileOf( lb, mm );
+   int64_t         lb;
+   const TileCoord tc   = tileOfInterior( li, mm, lb );
    const int64_t   base = tc.c * sc1;
    const int32_t   top  = int32_t( 2 * mm );
    // the coefficient loads are issued first (their latency overlaps the box
@@ -786,6 +787,77 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
    // (the vector-memory counter covers loads and atomics alike)
    T* dstC = dst + base;
    for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+   {
+      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
+      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
+      if ( X + Y + Z <= top )
+         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * sb[p] );
+   }
+}
+
+// The last tile layer of a macro-cell: one wavefront per tile, one lane per
+// micro-cell inside the macro-cell (cDiagPairs), the type varying over the lanes;
+// same element action, box staging and write-back as the tile kernels.
+template < class T, class K, bool STORED >
+__global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom,
+                                                          const P2CellGeom* __restrict__ geom64,
+                                                          const KParams* __restrict__ kp, const K* __restrict__ table,
+                                                          int64_t mm, int64_t ndiag, int64_t sc1, int64_t w1,
+                                                          const T* __restrict__ src, T* __restrict__ dst, double alpha )
+{
+   using R = typename Acc< T >::type;
+   __shared__ R  sb[kBox3];
+   const int64_t ld = xcdBlock( ndiag );
+   if ( ld >= ndiag )
+      return;
+   int64_t         lb;
+   const TileCoord tc   = tileOfDiagonal( ld, mm, lb );
+   const int64_t   base = tc.c * sc1;
+   const int32_t   top  = int32_t( 2 * mm );
+   const T*        srcC = src + base;
+   for ( int p = threadIdx.x; p < kBox3; p += 64 )
+   {
+      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
+      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
+      sb[p]           = ( X + Y + Z <= top ) ? R( toAcc( srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] ) ) : R( 0 );
+   }
+   __syncthreads();
+   const P2CellGeomT< R >& g    = geom[tc.c];
+   const int32_t           pair = cDiagPairs[threadIdx.x];
+   const int               t = pair & 7, a = pair >> 3;
+   const TileThread< R >   th = tileThreadAt( g, tc, mm, t, a );
+   int                     pos[10];
+   elementBoxPositions( th, pos );
+   R u[10];
+#pragma unroll
+   for ( int i = 0; i < 10; ++i )
+      u[i] = th.valid ? sb[pos[i]] : R( 0 );
+   __syncthreads();
+   for ( int p = threadIdx.x; p < kBox3; p += 64 )
+      sb[p] = R( 0 );
+   __syncthreads();
+   if ( th.valid )
+   {
+      R y[10];
+      p2ElementAction( g, th, u, y, [&]( int q ) -> R {
+         if constexpr ( STORED )
+            return R( toAcc( table[( lb * 11 + q ) * kTileThreads + t * 64 + a] ) );
+         else
+         {
+            const P2CellGeom&          g64  = geom64[tc.c];
+            const TileThread< double > t64  = tileThreadAt( g64, tc, mm, t, a );
+            const double*              xq64 = g64.XQ[t][q];
+            const double x64[3] = { t64.p0[0] + xq64[0], t64.p0[1] + xq64[1], t64.p0[2] + xq64[2] };
+            return R( kAt( g64, *kp, x64, t64.ls0 + xq64[3] ) );
+         }
+      } );
+#pragma unroll
+      for ( int i = 0; i < 10; ++i )
+         atomicAdd( &sb[pos[i]], y[i] );
+   }
+   __syncthreads();
+   T* dstC = dst + base;
+   for ( int p = threadIdx.x; p < kBox3; p += 64 )
    {
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
@@ -1071,6 +1143,8 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
                                                                                             lat.nc, (T*) d );
       HG_CHECK_HIP( hipGetLastError() );
       const int64_t ntiles = tilesPerCell( mm ) * lat.nc;
+      const int64_t nint   = tet( tileEdges( mm ) - 1 ) * lat.nc; // interior tiles
+      const int64_t ndiag  = tri( tileEdges( mm ) ) * lat.nc;     // last-layer tiles
       const K*      table  = p2Coefficients< K >( ctx, op, L, src.dtype );
       const auto*   geom   = p2Geometry< T >( op, L );
       ctx->beginTiming();
@@ -1083,12 +1157,24 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
             kP2SmallApply< T, K, false ><<< unsigned( lat.nc ), 64, 0, ctx->stream >>>(
                 geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, lat.nc, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
       }
-      else if ( table )
-         kP2TileApplyStored< T, K ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
-             geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
       else
-         kP2TileApplyEval< T, K ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
-             geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, ntiles, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+      {
+         if ( mm % kTile != 0 )
+            throw Error( HG_ERR_STATE, "P2 tile apply: micro-edge count not a multiple of the tile width" );
+         if ( nint > 0 && table )
+            kP2TileApplyStored< T, K ><<< xcdGrid( nint ), kTileThreads, 0, ctx->stream >>>(
+                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, nint, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+         else if ( nint > 0 )
+            kP2TileApplyEval< T, K ><<< xcdGrid( nint ), kTileThreads, 0, ctx->stream >>>(
+                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, nint, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+         HG_CHECK_HIP( hipGetLastError() );
+         if ( table )
+            kP2TileApplyDiag< T, K, true ><<< xcdGrid( ndiag ), 64, 0, ctx->stream >>>(
+                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ndiag, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+         else
+            kP2TileApplyDiag< T, K, false ><<< xcdGrid( ndiag ), 64, 0, ctx->stream >>>(
+                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, ndiag, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+      }
       HG_CHECK_HIP( hipGetLastError() );
       ctx->endTiming();
    } );
@@ -1315,6 +1401,10 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
             }
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQL ), ql, sizeof( ql ) ) );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cQLf ), qlf, sizeof( qlf ) ) );
+         const std::vector< int32_t > pairs = diagonalPairs();
+         if ( pairs.size() != 64 )
+            throw Error( -1, "last-layer tile pairs: expected 64" );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cDiagPairs ), pairs.data(), sizeof( int32_t ) * 64 ) );
       } );
       Storage&        st = ctx->storage( s );
       const Topology& t  = st.topo;

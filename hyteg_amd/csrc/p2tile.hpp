@@ -83,15 +83,68 @@ struct TileCoord
 
 __device__ __forceinline__ TileCoord tileOf( int64_t lb, int64_t mm )
 {
-   const int64_t tpc = tilesPerCell( mm );
+   const int64_t T = tileEdges( mm );
    TileCoord     tc;
-   tc.c       = lb / tpc;
+   const int64_t tpc = tilesPerCell( mm );
+   tc.c              = lb / tpc;
    int64_t ti, tj, tk;
-   decodeMicro( tileEdges( mm ), lb - tc.c * tpc, ti, tj, tk );
+   decodeMicro( T, lb - tc.c * tpc, ti, tj, tk );
    tc.x0 = kTile * ti;
    tc.y0 = kTile * tj;
    tc.z0 = kTile * tk;
    return tc;
+}
+
+// Tiles whose anchor sum ti + tj + tk is at most T - 2 (T = tileEdges) lie
+// inside the macro-cell up to a few micro-cells; they form the tile tetrahedron
+// of edge T - 1.  The last layer (ti + tj + tk = T - 1) holds 64 micro-cells per
+// tile (of 384 anchor/type pairs) and is handled by a one-wavefront kernel.
+// tileOfInterior decodes interior tile lb and returns its number `full` in the
+// numbering of all tiles (tileOf), which the coefficient tables use.
+__device__ __forceinline__ TileCoord tileOfInterior( int64_t lb, int64_t mm, int64_t& full )
+{
+   const int64_t T = tileEdges( mm );
+   TileCoord     tc;
+   const int64_t tpi = tet( T - 1 );
+   tc.c              = lb / tpi;
+   int64_t ti, tj, tk;
+   decodeMicro( T - 1, lb - tc.c * tpi, ti, tj, tk );
+   tc.x0 = kTile * ti;
+   tc.y0 = kTile * tj;
+   tc.z0 = kTile * tk;
+   full  = tc.c * tet( T ) + ( tet( T ) - tet( T - tk ) ) + triIndex( T - tk, ti, tj );
+   return tc;
+}
+// last-layer tile lb (of tri(T) per cell) and its number in the numbering of all tiles
+__device__ __forceinline__ TileCoord tileOfDiagonal( int64_t lb, int64_t mm, int64_t& full )
+{
+   const int64_t T = tileEdges( mm );
+   TileCoord     tc;
+   const int64_t per = tri( T );
+   tc.c              = lb / per;
+   int64_t ti, tj;
+   decodeTri( T, lb - tc.c * per, ti, tj );
+   const int64_t tk = T - 1 - ti - tj;
+   tc.x0            = kTile * ti;
+   tc.y0            = kTile * tj;
+   tc.z0            = kTile * tk;
+   full             = tc.c * tet( T ) + ( tet( T ) - tet( T - tk ) ) + triIndex( T - tk, ti, tj );
+   return tc;
+}
+// the 64 (type, anchor) pairs of a last-layer tile inside the macro-cell, for
+// lattices whose micro-edge count is a multiple of kTile: anchor sum at most
+// 3 - rowShift(type); entry = type | anchor << 3
+namespace {
+__constant__ int32_t cDiagPairs[64];
+} // namespace
+inline std::vector< int32_t > diagonalPairs()
+{
+   std::vector< int32_t > out;
+   for ( int t = 0; t < 6; ++t )
+      for ( int a = 0; a < 64; ++a )
+         if ( ( a & 3 ) + ( ( a >> 2 ) & 3 ) + ( a >> 4 ) <= kTile - 1 - MicroCells::rowShift[t] )
+            out.push_back( t | ( a << 3 ) );
+   return out;
 }
 
 // 1/b and 1/sqrt(b) from the hardware estimates (relative error e below 2^-22)
