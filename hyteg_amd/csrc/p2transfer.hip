@@ -39,6 +39,22 @@ __constant__ int    cPN[64];
 __constant__ int8_t cROff[8][kRMax][3];
 __constant__ double cRW[8][kRMax];
 __constant__ int    cRN[8];
+// the restriction stencils grouped by fine row: group g of class q covers the
+// fine row (2y + oy, 2z + oz) and entries [first, first + count) of cRGX / cRGW
+// (x offset and weight)
+constexpr int kRGroups = 49;
+struct alignas( 8 ) RowGroup
+{
+   int16_t oy, oz, first, count; // two dwords: scalar loads
+};
+__constant__ RowGroup cRG[8][kRGroups];
+__constant__ int      cRGN[8];
+__constant__ int32_t  cRGX[8][kRMax];
+__constant__ double   cRGW[8][kRMax];
+// the prolongation offsets packed into one word each (ox | oy << 8 | oz << 16,
+// all in 0..2): the kernels read them with scalar loads, which work on whole
+// dwords
+__constant__ uint32_t cPOffP[64][kPMax];
 
 namespace {
 
@@ -122,12 +138,17 @@ Located locate( const int F[3] )
 
 struct Tables
 {
-   int8_t pOff[64][kPMax][3]{};
-   double pW[64][kPMax]{};
-   int    pN[64]{};
-   int8_t rOff[8][kRMax][3]{};
-   double rW[8][kRMax]{};
-   int    rN[8]{};
+   int8_t   pOff[64][kPMax][3]{};
+   double   pW[64][kPMax]{};
+   int      pN[64]{};
+   int8_t   rOff[8][kRMax][3]{};
+   double   rW[8][kRMax]{};
+   int      rN[8]{};
+   RowGroup rg[8][kRGroups]{};
+   int      rgN[8]{};
+   int32_t  rgX[8][kRMax]{};
+   double   rgW[8][kRMax]{};
+   uint32_t pOffP[64][kPMax]{};
 };
 
 Tables buildTables()
@@ -149,6 +170,11 @@ Tables buildTables()
                   for ( int d = 0; d < 3; ++d )
                      T.pOff[cls][n][d] = int8_t( l.node[i][d] - 32 );
                   T.pW[cls][n] = l.phi[i];
+                  if ( T.pOff[cls][n][0] < 0 || T.pOff[cls][n][1] < 0 || T.pOff[cls][n][2] < 0 ||
+                       T.pOff[cls][n][0] > 2 || T.pOff[cls][n][1] > 2 || T.pOff[cls][n][2] > 2 )
+                     throw Error( -1, "P2 prolongation: coarse node offset outside 0..2" );
+                  T.pOffP[cls][n] = uint32_t( T.pOff[cls][n][0] ) | ( uint32_t( T.pOff[cls][n][1] ) << 8 ) |
+                                    ( uint32_t( T.pOff[cls][n][2] ) << 16 );
                }
          }
    // transpose stencils: coarse node C0 = 32 + q (lattice L+1), fine points F near 2 C0
@@ -174,6 +200,36 @@ Tables buildTables()
                   }
             }
    }
+   // the same entries grouped by fine row (first appearance order of the rows,
+   // table order inside a row)
+   for ( int q = 0; q < 8; ++q )
+   {
+      int n = 0;
+      for ( int e = 0; e < T.rN[q]; ++e )
+      {
+         bool seen = false;
+         for ( int g = 0; g < T.rgN[q]; ++g )
+            seen = seen || ( T.rg[q][g].oy == T.rOff[q][e][1] && T.rg[q][g].oz == T.rOff[q][e][2] );
+         if ( seen )
+            continue;
+         if ( T.rgN[q] >= kRGroups )
+            throw Error( -1, "P2 restriction: too many fine rows" );
+         RowGroup& g = T.rg[q][T.rgN[q]++];
+         g.oy        = T.rOff[q][e][1];
+         g.oz        = T.rOff[q][e][2];
+         g.first     = int16_t( n );
+         for ( int f = e; f < T.rN[q]; ++f )
+            if ( T.rOff[q][f][1] == g.oy && T.rOff[q][f][2] == g.oz )
+            {
+               T.rgX[q][n] = T.rOff[q][f][0];
+               T.rgW[q][n] = T.rW[q][f];
+               ++n;
+            }
+         g.count = int16_t( n - g.first );
+         if ( g.count > 7 )
+            throw Error( -1, "P2 restriction: more than 7 entries on one fine row" );
+      }
+   }
    return T;
 }
 
@@ -196,6 +252,11 @@ void uploadTables()
       HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cROff ), T.rOff, sizeof( T.rOff ) ) );
       HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRW ), T.rW, sizeof( T.rW ) ) );
       HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRN ), T.rN, sizeof( T.rN ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRG ), T.rg, sizeof( T.rg ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRGN ), T.rgN, sizeof( T.rgN ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRGX ), T.rgX, sizeof( T.rgX ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cRGW ), T.rgW, sizeof( T.rgW ) ) );
+      HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cPOffP ), T.pOffP, sizeof( T.pOffP ) ) );
    } );
 }
 
@@ -251,9 +312,10 @@ __global__ void kP2ProlongCell( const int32_t* __restrict__ rows, int64_t nrows,
       A             v   = A( 0 );
       for ( int e = 0; e < cPN[cls]; ++e )
       {
-         const int     oy = cPOff[cls][e][1], oz = cPOff[cls][e][2];
-         const int64_t cx = bx + cPOff[cls][e][0], cy = by + oy, cz = bz + oz;
-         if ( cx < 0 || cy < 0 || cz < 0 || cx + cy + cz > wc - 1 )
+         const uint32_t o  = cPOffP[cls][e]; // offsets 0..2: one word per entry
+         const int      oy = int( ( o >> 8 ) & 255u ), oz = int( o >> 16 );
+         const int64_t  cx = bx + int64_t( o & 255u ), cy = by + oy, cz = bz + oz;
+         if ( cx + cy + cz > wc - 1 )
             continue;
          v += A( cPW[cls][e] ) * toAcc( coarse[rs[oy + 3 * oz] + cx] );
       }
@@ -338,6 +400,112 @@ __global__ void kP2RestrictCell( const int32_t* __restrict__ rows, int64_t nrows
       }
 }
 
+// Restriction on cell-interior coarse points, one thread per point from the
+// lattice's class point list: a wavefront holds points of one parity class, so
+// the stencil (row groups, offsets, weights) is uniform over it and every lane
+// has work.  Per fine row the row start comes from the slice starts in LDS; the
+// entries of a row (at most 7) are gathered before they are summed.
+struct ClassWaves
+{
+   int32_t w[9];
+};
+constexpr int kRestrictBlock = 256;
+constexpr int kMaxSlices     = 1025;
+template < class T >
+__global__ __launch_bounds__( kRestrictBlock ) void kP2RestrictCellList( const uint32_t* __restrict__ pts, ClassWaves cw,
+                                                                        int64_t nc, int64_t scc, int32_t wc, int64_t scf,
+                                                                        int32_t wf, const T* __restrict__ fine,
+                                                                        T* __restrict__ coarse )
+{
+   using A = typename Acc< T >::type;
+   __shared__ int32_t sliceStart[kMaxSlices];
+   const int32_t      wavesPerCell = cw.w[8];
+   const int64_t      bpc          = ( wavesPerCell + kRestrictBlock / 64 - 1 ) / ( kRestrictBlock / 64 );
+   const int64_t      lb           = xcdBlock( nc * bpc );
+   if ( lb >= nc * bpc )
+      return;
+   for ( int32_t z = threadIdx.x; z < wf; z += kRestrictBlock )
+      sliceStart[z] = int32_t( tet( wf ) - tet( wf - z ) );
+   __syncthreads();
+   const int64_t c    = lb / bpc;
+   const int32_t wave = int32_t( lb - c * bpc ) * ( kRestrictBlock / 64 ) +
+                        __builtin_amdgcn_readfirstlane( int32_t( threadIdx.x >> 6 ) );
+   if ( wave >= wavesPerCell )
+      return;
+   int q = 0;
+   while ( wave >= cw.w[q + 1] )
+      ++q;
+   const uint32_t p = pts[int64_t( wave ) * 64 + ( threadIdx.x & 63 )];
+   if ( p == 0xFFFFFFFFu )
+      return;
+   const int32_t x = int32_t( p & 1023u ), y = int32_t( ( p >> 10 ) & 1023u ), z = int32_t( p >> 20 );
+   const int32_t mmf = wf - 1;
+   const T*      f   = fine + c * scf;
+   A             v   = A( 0 );
+   for ( int g = 0; g < cRGN[q]; ++g )
+   {
+      const RowGroup rg = cRG[q][g];
+      const int32_t  fy = 2 * y + rg.oy, fz = 2 * z + rg.oz;
+      if ( fy < 0 || fz < 0 || fy + fz > mmf )
+         continue;
+      const int32_t rs  = sliceStart[fz] + fy * ( wf - fz ) - fy * ( fy - 1 ) / 2;
+      const int32_t lim = mmf - fy - fz;
+      A             buf[7];
+#pragma unroll
+      for ( int j = 0; j < 7; ++j )
+      {
+         buf[j] = A( 0 );
+         if ( j < rg.count )
+         {
+            const int32_t fx = 2 * x + cRGX[q][rg.first + j];
+            if ( fx >= 0 && fx <= lim )
+               buf[j] = toAcc( f[rs + fx] );
+         }
+      }
+#pragma unroll
+      for ( int j = 0; j < 7; ++j )
+         if ( j < rg.count )
+            v += A( cRGW[q][rg.first + j] ) * buf[j];
+   }
+   coarse[c * scc + tetRowStart( wc, y, z ) + x] = fromAcc< T >( v );
+}
+
+// the class point list of a lattice (built on first use)
+static void classPointList( Context* ctx, Lattice& l )
+{
+   // (the restriction kernel keeps the slice starts of the fine lattice, width
+   // 2 mm + 1, in LDS; wider lattices keep the row kernel)
+   if ( l.classPoints.p || l.mm < 4 || 2 * l.mm + 1 > kMaxSlices )
+      return;
+   std::vector< uint32_t > pts;
+   int32_t                 waves = 0;
+   for ( int q = 0; q < 8; ++q )
+   {
+      l.classWave[q] = waves;
+      size_t n0      = pts.size();
+      for ( int64_t z = 1; z <= l.mm - 3; ++z )
+      {
+         if ( ( z & 1 ) != ( ( q >> 2 ) & 1 ) )
+            continue;
+         for ( int64_t y = 1; y + z <= l.mm - 2; ++y )
+         {
+            if ( ( y & 1 ) != ( ( q >> 1 ) & 1 ) )
+               continue;
+            for ( int64_t x = 1; x + y + z <= l.mm - 1; ++x )
+               if ( ( x & 1 ) == ( q & 1 ) )
+                  pts.push_back( uint32_t( x ) | ( uint32_t( y ) << 10 ) | ( uint32_t( z ) << 20 ) );
+         }
+      }
+      while ( ( pts.size() - n0 ) % 64 )
+         pts.push_back( 0xFFFFFFFFu );
+      waves += int32_t( ( pts.size() - n0 ) / 64 );
+   }
+   l.classWave[8] = waves;
+   if ( pts.empty() )
+      pts.push_back( 0xFFFFFFFFu );
+   l.classPoints.upload( ctx, pts );
+}
+
 // one wavefront per owned boundary DoF: the lanes split the stencil entries of
 // each incident cell (a macro-vertex DoF of the shell sees up to ~36 cells with
 // 125 entries each) and the wave sums its lanes in a fixed butterfly order
@@ -410,12 +578,25 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
    Storage& st = *f.st;
    Lattice& lf = st.lattice( ctx, fineL + 1 );
    Lattice& lc = st.lattice( ctx, fineL );
+   classPointList( ctx, lc );
    void*    fp = f.ptr( fineL );
    void*    cp = f.ptr( fineL - 1 );
    pullHalos( ctx, st, lf, fp, f.dtype );
    dispatchType( f.dtype, [&]( auto tag ) {
       using T = decltype( tag );
-      if ( lc.nrows > 0 )
+      if ( lc.nrows > 0 && lc.classPoints.p )
+      {
+         ClassWaves cw;
+         for ( int q = 0; q < 9; ++q )
+            cw.w[q] = lc.classWave[q];
+         const int64_t bpc = ( cw.w[8] + kRestrictBlock / 64 - 1 ) / ( kRestrictBlock / 64 );
+         ctx->beginTiming();
+         kP2RestrictCellList< T ><<< xcdGrid( bpc * lc.nc ), kRestrictBlock, 0, ctx->stream >>>(
+             lc.classPoints.p, cw, lc.nc, lc.sc, int32_t( lc.w ), lf.sc, int32_t( lf.w ), (const T*) fp, (T*) cp );
+         HG_CHECK_HIP( hipGetLastError() );
+         ctx->endTiming();
+      }
+      else if ( lc.nrows > 0 )
       {
          ctx->beginTiming();
          kP2RestrictCell< T ><<< xcdGrid( lc.nrows * lc.nc ), kRowBlockT, 0, ctx->stream >>>(
