@@ -127,6 +127,7 @@ SIGNATURES = {
     "hg_slots_cg_stop": [_p, _i, _i, _d, _d, _i, _i],
     "hg_serve_stream_solutions": [_i, _i64, _p, _p, _p, _p, _d, _i, _d, _p, _p, _p, _p, _p, _p],
     "hg_serve_request_coefficient": [C.c_uint64, _p, _p, _p],
+    "hg_serve_request_blobs": [C.c_uint64, _p],
     "hg_p2_low_precision_form": [_p, _i64, _i, _i],
     "hg_timing_enable": [_p, _i],
     "hg_timing_start": [_p, C.c_char_p],

@@ -42,10 +42,11 @@ using Clock = std::chrono::steady_clock;
 
 enum Kind
 {
-   P1Cube  = 0,
-   P1Shell = 1,
-   P2Shell = 2,
-   P2Cube  = 3
+   P1Cube      = 0,
+   P1Shell     = 1,
+   P2Shell     = 2,
+   P2Cube      = 3,
+   StokesShell = 4
 };
 
 struct Request
@@ -105,6 +106,31 @@ void coefficientOf( uint64_t seed, std::vector< double >& a, std::vector< double
 
 int64_t latticeSize( int64_t w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
 
+// centres of the 12 Gaussian temperature blobs of a Stokes request: directions
+// uniform on the sphere, radii uniform in the shell volume r in [0.55, 1]
+void blobCentresOf( uint64_t seed, double c[12][3] )
+{
+   SplitMix g{ seed ^ 0xB10B5EEDull };
+   for ( int b = 0; b < 12; ++b )
+   {
+      double d[3], n = 0.0;
+      do
+      {
+         n = 0.0;
+         for ( int k = 0; k < 3; ++k )
+         {
+            d[k] = g.normal();
+            n += d[k] * d[k];
+         }
+      } while ( n < 1e-20 );
+      n                 = std::sqrt( n );
+      const double r3lo = 0.55 * 0.55 * 0.55, r3hi = 1.0;
+      const double r    = std::cbrt( r3lo + g.uniform() * ( r3hi - r3lo ) );
+      for ( int k = 0; k < 3; ++k )
+         c[b][k] = r * d[k] / n;
+   }
+}
+
 // P2 requests are solved one at a time per worker; their small kernels leave
 // the device mostly idle, so a class runs several workers (own context and
 // stream each) while requests are queued
@@ -118,6 +144,10 @@ constexpr int kP2Workers = HG_SERVE_P2_WORKERS;
 #define HG_SERVE_P2_COARSE_ITERATIONS 30
 #endif
 constexpr int kP2CoarseIterations = HG_SERVE_P2_COARSE_ITERATIONS;
+// Stokes requests are long (tens of Uzawa V-cycles), so a Stokes class also
+// runs several workers while requests are queued
+constexpr int kStokesWorkers = 4;
+constexpr int kStokesMaxCycles = 150;
 
 // HG_SERVE_NO_GRAPHS=1 runs every V-cycle eagerly instead of replaying the
 // captured graph (for kernel tracers that do not follow graph launches)
@@ -155,6 +185,8 @@ int64_t footprint( int kind, int level, int slots )
       pts += cells * latticeSize( ( int64_t( 1 ) << l ) + 1 );
    if ( kind == P1Cube || kind == P1Shell )
       return pts * 8 * int64_t( slots ) * 5 + ( int64_t( 64 ) << 20 );
+   if ( kind == StokesShell )
+      return pts * 8 * 40 + ( int64_t( 64 ) << 20 ); // four Taylor-Hood functions and the solver's temporaries
    // P2: about ten FP64 functions and the quadrature-point coefficients (~8 values per DoF)
    return pts * 8 * 18 + ( int64_t( 64 ) << 20 );
 }
@@ -214,6 +246,20 @@ struct Server
       cv.notify_all();
    }
 
+   // replaces a worker's reservation by the device memory its context holds
+   // (hierarchy, operator tables and the context's cache of freed blocks), once
+   // that is known: admission of further classes then counts actual bytes
+   void settle( hg_ctx* ctx, int64_t& reserved )
+   {
+      int64_t held = 0;
+      if ( hg_ctx_memory( ctx, &held ) != HG_OK )
+         return;
+      std::lock_guard< std::mutex > lk( m );
+      memUsed += held - reserved;
+      reserved = held;
+      cv.notify_all();
+   }
+
    void fail( const std::string& what )
    {
       std::lock_guard< std::mutex > lk( m );
@@ -227,6 +273,7 @@ struct Server
 void p1Worker( Server& sv, int kind, int level, int64_t bytes )
 {
    const std::pair< int, int > cls{ kind, level };
+   int64_t                     reserved = bytes;
    try
    {
       auto ctx = std::make_shared< Context >( sv.device );
@@ -348,6 +395,7 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
                X.copyVector( k, zero, 0, L );
                active[k] = false;
             }
+            sv.settle( ctx->get(), reserved );
          }
       }
       if ( graph )
@@ -356,13 +404,14 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
    {
       sv.fail( std::string( "P1 worker: " ) + e.what() );
    }
-   sv.release( cls, bytes );
+   sv.release( cls, reserved );
 }
 
 // ---- P2 class: one request at a time -----------------------------------------
 void p2Worker( Server& sv, int kind, int level, int64_t bytes )
 {
    const std::pair< int, int > cls{ kind, level };
+   int64_t                     reserved = bytes;
    try
    {
       auto ctx = std::make_shared< Context >( sv.device );
@@ -442,12 +491,125 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
             check( ctx->get(), hg_function_download( ctx->get(), x.handle(), int( L ), sv.res.solutions[rq.idx] ),
                    "solution" );
          sv.finish( rq, r0 > 0 ? rk / r0 : 0.0, x.dotGlobal( x, L, All ), k );
+         sv.settle( ctx->get(), reserved );
       }
    } catch ( const std::exception& e )
    {
       sv.fail( std::string( "P2 worker: " ) + e.what() );
    }
-   sv.release( cls, bytes );
+   sv.release( cls, reserved );
+}
+
+// ---- Stokes class: one request at a time --------------------------------------
+// Isoviscous P2-P1 Taylor-Hood Stokes on the blended shell with no-slip on both
+// surfaces and the nodal buoyancy T(x) x/|x| of 12 Gaussian blobs (sigma 0.05,
+// amplitude 1) whose centres come from the request's seed; solved by the
+// all-at-once Uzawa-smoothed GMG of stokesGMGUzawaFSSolver (V(3,3) + 2 per
+// coarser level, omega 0.3, two Chebyshev(3) velocity sweeps per Uzawa step --
+// the UzawaSmoother default -- and MINRES(10) on level 0), projecting the
+// pressure mean after every cycle.  (With one velocity sweep per step the
+// isoviscous iteration diverges from level 3 on.  With the 1e4 radial viscosity
+// contrast of the benchmark's Stokes case this solver stalls near a relative
+// residual of 1e-3, far above a serving tolerance.  On level 1 the residual floors
+// near 1e-6: the degree-2 divergence rule on the blended cells leaves the
+// constant pressure slightly outside the kernel of B^T, so Stokes requests
+// start at level 2.)
+void stokesWorker( Server& sv, int level, int64_t bytes )
+{
+   const std::pair< int, int > cls{ StokesShell, level };
+   int64_t                     reserved = bytes;
+   try
+   {
+      auto      ctx = std::make_shared< Context >( sv.device );
+      hg_handle h;
+      check( ctx->get(), hg_storage_shell_bc( ctx->get(), 5, 3, 0.55, 1.0, int( DirichletBoundary ),
+                                              int( DirichletBoundary ), &h ),
+             "shell" );
+      auto           st = std::make_shared< PrimitiveStorage >( ctx, h );
+      const uint_t   L  = uint_t( level );
+      P2P1ElementwiseBlendingFullViscousStokesOperator A( st, 0, L, 0.0, 0.0 );
+      P2P1TaylorHoodFunction< double > x( "x", st, 0, L ), b( "b", st, 0, L ), t( "t", st, L, L );
+      auto handles = []( const P2P1TaylorHoodFunction< double >& f, hg_handle out[4] ) {
+         for ( int k = 0; k < 3; ++k )
+            out[k] = f.uvw()[k].handle();
+         out[3] = f.p().handle();
+      };
+      hg_handle xh[4], bh[4], th[4];
+      handles( x, xh );
+      handles( b, bh );
+      handles( t, th );
+      int64_t n2 = 0, n1 = 0;
+      check( ctx->get(), hg_p2_ndofs( ctx->get(), st->handle(), int( L ), &n2 ), "ndofs" );
+      check( ctx->get(), hg_p1_ndofs( ctx->get(), st->handle(), int( L ), &n1 ), "ndofs" );
+      std::vector< double >  xyz( size_t( 3 * n2 ) );
+      std::vector< int32_t > flags( static_cast< size_t >( n2 ) );
+      check( ctx->get(), hg_p2_dof_coords( ctx->get(), st->handle(), int( L ), xyz.data() ), "coords" );
+      check( ctx->get(), hg_p2_dof_flags( ctx->get(), st->handle(), int( L ), flags.data() ), "flags" );
+      // the velocity block's spectral radius (the same operator for every request)
+      for ( int k = 0; k < 3; ++k )
+         check( ctx->get(), hg_function_interpolate_random( ctx->get(), xh[k], 7 + uint64_t( k ), -1.0, 1.0, int( L ),
+                                                             int( Inner ) ),
+                "start vector" );
+      double rho = 0.0;
+      check( ctx->get(), hg_stokes_estimate_radius( ctx->get(), A.handle(), xh, th, int( L ), 25, 0, &rho ), "radius" );
+      std::vector< double > rhs( static_cast< size_t >( n2 ) ), zeroP( static_cast< size_t >( n1 ), 0.0 );
+      std::vector< double > hist( kStokesMaxCycles + 1 );
+      for ( ;; )
+      {
+         std::vector< Request > in;
+         if ( !sv.take( cls, 1, true, in ) )
+            break;
+         if ( in.empty() )
+            continue;
+         const Request& rq = in[0];
+         double         cb[12][3];
+         blobCentresOf( rq.seed, cb );
+         const double sigma = 0.05, cut = 36.0 * sigma * sigma;
+         std::vector< double > T( static_cast< size_t >( n2 ), 0.0 );
+         for ( int64_t i = 0; i < n2; ++i )
+         {
+            if ( flags[size_t( i )] != int32_t( Inner ) )
+               continue;
+            const double* p = &xyz[size_t( 3 * i )];
+            for ( int bb = 0; bb < 12; ++bb )
+            {
+               const double d2 = ( p[0] - cb[bb][0] ) * ( p[0] - cb[bb][0] ) + ( p[1] - cb[bb][1] ) * ( p[1] - cb[bb][1] ) +
+                                 ( p[2] - cb[bb][2] ) * ( p[2] - cb[bb][2] );
+               if ( d2 < cut )
+                  T[size_t( i )] += std::exp( -d2 / ( 2.0 * sigma * sigma ) );
+            }
+         }
+         for ( int k = 0; k < 3; ++k )
+         {
+            for ( int64_t i = 0; i < n2; ++i )
+            {
+               const double* p = &xyz[size_t( 3 * i )];
+               const double  r = std::sqrt( p[0] * p[0] + p[1] * p[1] + p[2] * p[2] );
+               rhs[size_t( i )] = T[size_t( i )] * p[k] / r;
+            }
+            check( ctx->get(), hg_function_upload( ctx->get(), bh[k], int( L ), rhs.data() ), "rhs" );
+         }
+         check( ctx->get(), hg_function_upload( ctx->get(), bh[3], int( L ), zeroP.data() ), "rhs" );
+         x.interpolate( 0.0, L, All );
+         int done = 0;
+         check( ctx->get(), hg_stokes_gmg_solve( ctx->get(), A.handle(), xh, bh, 0, int( L ), 3, 3, 2, 0.3, 2, 3, rho,
+                                                 10, 1e-6, 1, 0, kStokesMaxCycles, sv.tol, hist.data(), &done ),
+                "Stokes GMG" );
+         if ( sv.res.solutions && sv.res.solutions[rq.idx] )
+         {
+            double* out = sv.res.solutions[rq.idx];
+            for ( int k = 0; k < 3; ++k )
+               check( ctx->get(), hg_function_download( ctx->get(), xh[k], int( L ), out + k * n2 ), "solution" );
+            check( ctx->get(), hg_function_download( ctx->get(), xh[3], int( L ), out + 3 * n2 ), "solution" );
+         }
+         sv.finish( rq, hist[0] > 0 ? hist[size_t( done )] / hist[0] : 0.0, x.uvw().dotGlobal( x.uvw(), L, All ), done );
+         sv.settle( ctx->get(), reserved );
+      }
+   } catch ( const std::exception& e )
+   {
+      sv.fail( std::string( "Stokes worker: " ) + e.what() );
+   }
+   sv.release( cls, reserved );
 }
 
 // starts a worker for every class with queued requests and no live worker,
@@ -459,7 +621,9 @@ void startWorkers( Server& sv )
    {
       if ( q.empty() )
          continue;
-      const int want = ( cls.first == P2Shell || cls.first == P2Cube ) ? kP2Workers : 1;
+      const int want = ( cls.first == P2Shell || cls.first == P2Cube ) ? kP2Workers
+                       : cls.first == StokesShell                     ? kStokesWorkers
+                                                                      : 1;
       while ( sv.live[cls] < want && ( sv.live[cls] == 0 || q.size() > size_t( sv.live[cls] ) ) )
       {
          const int64_t bytes = footprint( cls.first, cls.second, sv.slots );
@@ -476,6 +640,8 @@ void startWorkers( Server& sv )
          sv.live[cls] += 1;
          if ( cls.first == P1Cube || cls.first == P1Shell )
             sv.threads.emplace_back( p1Worker, std::ref( sv ), cls.first, cls.second, bytes );
+         else if ( cls.first == StokesShell )
+            sv.threads.emplace_back( stokesWorker, std::ref( sv ), cls.second, bytes );
          else
             sv.threads.emplace_back( p2Worker, std::ref( sv ), cls.first, cls.second, bytes );
       }
@@ -503,6 +669,18 @@ int hg_serve_request_coefficient( uint64_t seed, double* a16, double* omega48, d
    return HG_OK;
 }
 
+int hg_serve_request_blobs( uint64_t seed, double* centres36 )
+{
+   if ( !centres36 )
+      return HG_ERR_ARG;
+   double c[12][3];
+   blobCentresOf( seed, c );
+   for ( int b = 0; b < 12; ++b )
+      for ( int k = 0; k < 3; ++k )
+         centres36[3 * b + k] = c[b][k];
+   return HG_OK;
+}
+
 int hg_serve_stream( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
                      const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
                      double* rel_residual, double* checksum, int* cycles, double* wall_s )
@@ -527,7 +705,8 @@ int hg_serve_stream_solutions( int device, int64_t n, const int* kind, const int
    {
       const bool p1 = kind[i] == P1Cube || kind[i] == P1Shell;
       const bool p2 = kind[i] == P2Shell || kind[i] == P2Cube;
-      if ( ( !p1 && !p2 ) || level[i] < 1 || level[i] > ( p1 ? 8 : 6 ) )
+      const bool sk = kind[i] == StokesShell;
+      if ( ( !p1 && !p2 && !sk ) || level[i] < ( sk ? 2 : 1 ) || level[i] > ( p1 ? 8 : ( p2 ? 6 : 4 ) ) )
       {
          tlsError = "request " + std::to_string( i ) + ": unknown kind or level out of range";
          return HG_ERR_ARG;

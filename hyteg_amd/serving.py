@@ -6,7 +6,8 @@ hg_serve_stream and their latency / throughput summary.
 Request mix (kind, levels, share): P1 Poisson on the unit cube, levels 3-7,
 40 %; P1 Poisson on the shell macro mesh, levels 3-5, 25 %; P2 div(k grad) on
 the cube, levels 3-5, 20 %; P2 div(k grad) on the blended shell, levels 2-3,
-15 %.  Coefficients and right-hand sides come from each request's seed.
+15 %.  Coefficients and right-hand sides come from each request's seed.  Kind 4
+(P2-P1 Stokes on the shell) is served too; --stokes-share mixes it in.
 """
 import argparse
 import ctypes as C
@@ -16,7 +17,7 @@ import numpy as np
 
 from ._lib import HygError, lib
 
-P1_CUBE, P1_SHELL, P2_SHELL, P2_CUBE = 0, 1, 2, 3
+P1_CUBE, P1_SHELL, P2_SHELL, P2_CUBE, STOKES_SHELL = 0, 1, 2, 3, 4
 MIX = [  # (kind, levels, level weights, share)
     (P1_CUBE, (3, 4, 5, 6, 7), (0.25, 0.25, 0.25, 0.15, 0.10), 0.40),
     (P1_SHELL, (3, 4, 5), (0.4, 0.4, 0.2), 0.25),
@@ -25,15 +26,18 @@ MIX = [  # (kind, levels, level weights, share)
 ]
 
 
-def make_stream(n, rate, seed=0):
-    """n requests, exponential inter-arrival times of mean 1/rate (seeded)"""
+def make_stream(n, rate, seed=0, stokes_share=0.0, stokes_levels=(2,)):
+    """n requests, exponential inter-arrival times of mean 1/rate (seeded); with
+    stokes_share > 0 that share of the requests are Stokes solves on the shell"""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     arrivals -= arrivals[0]
-    shares = np.array([m[3] for m in MIX])
-    which = rng.choice(len(MIX), n, p=shares / shares.sum())
-    kinds = np.array([MIX[w][0] for w in which], dtype=np.int32)
-    levels = np.array([rng.choice(MIX[w][1], p=np.array(MIX[w][2]) / sum(MIX[w][2])) for w in which], dtype=np.int32)
+    mix = MIX + ([(STOKES_SHELL, tuple(stokes_levels), (1.0,) * len(stokes_levels), stokes_share)]
+                 if stokes_share > 0 else [])
+    shares = np.array([m[3] for m in mix])
+    which = rng.choice(len(mix), n, p=shares / shares.sum())
+    kinds = np.array([mix[w][0] for w in which], dtype=np.int32)
+    levels = np.array([rng.choice(mix[w][1], p=np.array(mix[w][2]) / sum(mix[w][2])) for w in which], dtype=np.int32)
     seeds = rng.integers(0, 2 ** 62, n, dtype=np.uint64)
     return kinds, levels, seeds, arrivals
 
@@ -42,14 +46,22 @@ def make_stream(n, rate, seed=0):
 MESH_COUNTS = {"cube": (8, 19, 18, 6), "shell": (486, 2724, 4160, 1920)}
 
 
-def ndofs(kind, level):
-    """canonical DoFs of a request's solution: P1 DoFs of `level`, or P2 DoFs of
-    `level` (= P1 DoFs of level + 1)"""
-    nv, ne, nf, nc = MESH_COUNTS["cube" if kind in (P1_CUBE, P2_CUBE) else "shell"]
-    m = 2 ** (level + (1 if kind in (P2_SHELL, P2_CUBE) else 0))
+def _lattice_dofs(mesh, level):
+    nv, ne, nf, nc = MESH_COUNTS[mesh]
+    m = 2 ** level
     tri = lambda k: k * (k + 1) // 2 if k > 0 else 0  # noqa: E731
     tet = lambda k: k * (k + 1) * (k + 2) // 6 if k > 0 else 0  # noqa: E731
     return nv + ne * (m - 1) + nf * tri(m - 2) + nc * tet(m - 3)
+
+
+def ndofs(kind, level):
+    """canonical DoFs of a request's solution: P1 DoFs of `level`, P2 DoFs of
+    `level` (= P1 DoFs of level + 1), or for Stokes three P2 velocity components
+    and the P1 pressure"""
+    mesh = "cube" if kind in (P1_CUBE, P2_CUBE) else "shell"
+    if kind == STOKES_SHELL:
+        return 3 * _lattice_dofs(mesh, level + 1) + _lattice_dofs(mesh, level)
+    return _lattice_dofs(mesh, level + (1 if kind in (P2_SHELL, P2_CUBE) else 0))
 
 
 def request_coefficient(seed):
@@ -59,6 +71,14 @@ def request_coefficient(seed):
     if lib().hg_serve_request_coefficient(C.c_uint64(int(seed)), p(a), p(om), p(ph)) != 0:
         raise HygError("hg_serve_request_coefficient failed")
     return a, om.reshape(16, 3), ph
+
+
+def request_blobs(seed):
+    """(12, 3) centres of the Gaussian temperature blobs of a Stokes request"""
+    c = np.zeros(36)
+    if lib().hg_serve_request_blobs(C.c_uint64(int(seed)), c.ctypes.data_as(C.c_void_p)) != 0:
+        raise HygError("hg_serve_request_blobs failed")
+    return c.reshape(12, 3)
 
 
 def serve_stream(kinds, levels, seeds, arrivals, tol=1e-8, slots=32, mem_budget_gb=200.0, device=0,
@@ -88,7 +108,8 @@ def serve_stream(kinds, levels, seeds, arrivals, tol=1e-8, slots=32, mem_budget_
     return out
 
 
-KIND_NAMES = {P1_CUBE: "p1_cube", P1_SHELL: "p1_shell", P2_SHELL: "p2_shell", P2_CUBE: "p2_cube"}
+KIND_NAMES = {P1_CUBE: "p1_cube", P1_SHELL: "p1_shell", P2_SHELL: "p2_shell", P2_CUBE: "p2_cube",
+              STOKES_SHELL: "stokes_shell"}
 
 
 def summary(out, n, kinds=None):
@@ -114,6 +135,7 @@ def main():
     ap.add_argument("--kind", choices=list(KIND_NAMES.values()), default=None,
                     help="instead of the mixed streams: a burst of --requests requests of this kind at t = 0")
     ap.add_argument("--level", type=int, default=3, help="level of the --kind burst")
+    ap.add_argument("--stokes-share", type=float, default=0.0, help="share of Stokes requests in the mixed streams")
     args = ap.parse_args()
     if args.kind is not None:
         kind = {v: k for k, v in KIND_NAMES.items()}[args.kind]
@@ -125,7 +147,7 @@ def main():
         print(json.dumps({"kind": args.kind, "level": args.level, "cycles": cycles, **summary(out, n)}), flush=True)
         return
     for rate in [float(r) for r in args.rates.split(",")]:
-        stream = make_stream(args.requests, rate, seed=0)
+        stream = make_stream(args.requests, rate, seed=0, stokes_share=args.stokes_share)
         out = serve_stream(*stream, tol=args.tol, slots=args.slots)
         print(json.dumps({"rate_per_s": rate, **summary(out, args.requests, stream[0])}), flush=True)
 

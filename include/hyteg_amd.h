@@ -418,20 +418,25 @@ int hg_p1_batched_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b,
 /* Runs a stream of n solve requests on `device` with continuous batching (no
  * counterpart in the reference).  Request i of kind[i] (0: P1 Poisson on the unit
  * cube, 1: P1 Poisson on the shell macro mesh, 2: P2 div(k grad) on the blended
- * shell, 3: P2 div(k grad) on the cube; k and the RHS generated from seed[i]) at
- * level[i] arrives arrival_s[i] seconds after the call starts and is solved to
- * relative residual `tol`.  P1 requests of one (kind, level) share an interleaved
- * hierarchy of `slots` vectors (a power of two up to 64) and join / leave it at
- * V-cycle boundaries; every class has its own stream; a class hierarchy is built
- * only while the estimated device memory of the live ones stays below
- * memBudgetGB.  Outputs per request: latency (arrival to completion), reached
- * relative residual, checksum x.x of the solution, V-cycles; wall_s = total time. */
+ * shell, 3: P2 div(k grad) on the cube, 4: isoviscous P2-P1 Stokes on the blended
+ * shell with no-slip surfaces and the buoyancy of 12 Gaussian blobs, levels 2-4;
+ * k, blob centres and the RHS generated from seed[i])
+ * at level[i] arrives arrival_s[i] seconds after the call starts and is solved to
+ * relative residual `tol` (Stokes: at most 150 Uzawa V-cycles).  P1 requests of
+ * one (kind, level) share an interleaved hierarchy of `slots` vectors (a power of
+ * two up to 64) and join / leave it at V-cycle boundaries; every class has its
+ * own stream; a class hierarchy is built only while the device memory of the
+ * live ones stays below memBudgetGB (an estimate until a worker has built its
+ * hierarchy, then the bytes its context actually holds).  Outputs per request:
+ * latency (arrival to completion), reached relative residual, checksum x.x of the
+ * (velocity) solution, V-cycles; wall_s = total time. */
 int hg_serve_stream( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
                      const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
                      double* rel_residual, double* checksum, int* cycles, double* wall_s );
 /* hg_serve_stream that also copies request i's solution (canonical order, FP64, the
- * ndofs of its kind and level: P1 DoFs of `level` for kinds 0/1, P2 DoFs for 2/3)
- * into solutions[i] when that pointer is not null (solutions itself may be null) */
+ * ndofs of its kind and level: P1 DoFs of `level` for kinds 0/1, P2 DoFs for 2/3,
+ * for kind 4 the three P2 velocity components followed by the P1 pressure) into
+ * solutions[i] when that pointer is not null (solutions itself may be null) */
 int hg_serve_stream_solutions( int device, int64_t n, const int* kind, const int* level, const uint64_t* seed,
                                const double* arrival_s, double tol, int slots, double memBudgetGB, double* latency_s,
                                double* rel_residual, double* checksum, int* cycles, double* wall_s,
@@ -439,6 +444,9 @@ int hg_serve_stream_solutions( int device, int64_t n, const int* kind, const int
 /* the coefficient k(x) = exp(sum_m a_m sin(omega_m . x + phi_m)), m < 16, the stream
  * generates for a diffusion request of the given seed */
 int hg_serve_request_coefficient( uint64_t seed, double* a16, double* omega48, double* phi16 );
+/* the 12 blob centres (x, y, z each) the stream generates for a Stokes request of
+ * the given seed */
+int hg_serve_request_blobs( uint64_t seed, double* centres36 );
 /* message of the last hg_serve_stream failure on this thread */
 const char* hg_serve_last_error( void );
 
