@@ -300,14 +300,14 @@ def gmg_p2_shell(hg, ctx, L):
     # mixed precision: FP64 iterative refinement around one V-cycle per step in FP32
     # arithmetic on FP16 (or FP32) vectors; the low-precision applies use the reduced
     # element form (affine micro-cells, k constant per micro-cell, FP32 accumulation)
-    for dt, key, what in ((hg.F16, "mixed_fp32_compute_fp16_storage", "FP16 vectors"),
-                          (hg.F32, "mixed_fp32_compute_fp32_storage", "FP32 vectors")):
+    for low, key, what in ((hg.F16, "mixed_fp32_compute_fp16_storage", "FP16 vectors"),
+                           (hg.F32, "mixed_fp32_compute_fp32_storage", "FP32 vectors")):
         xm = hg.P2Function("xm", st, 0, L, dtype=hg.F64)
-        hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=dt, max_outer=1)  # builds the low-precision tables once
+        hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=low, max_outer=1)  # builds the low-precision tables once
         xm.interpolate(0.0, L)
         ctx.synchronize()
         t0 = time.perf_counter()
-        km, hm = hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=dt, inner_cycles=1, max_outer=60, tol=1e-8)
+        km, hm = hg.p2_ir_solve(A, xm, b, 0, L, rho, low_dtype=low, inner_cycles=1, max_outer=60, tol=1e-8)
         ctx.synchronize()
         dtm = time.perf_counter() - t0
         # || x_mixed - x_fp64 ||_2 / || x_fp64 ||_2 on the device
