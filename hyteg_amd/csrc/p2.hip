@@ -396,7 +396,8 @@ __device__ __forceinline__ void p2ReducedAction( const float G[6], const float u
 // Reduced-form apply, FP32 arithmetic on FP32 or FP16 values.  A tile's source
 // box is staged in LDS, the blended positions of its 5x5x5 micro-vertices
 // (relative to its first one, from FP64 evaluations) too; every thread forms its
-// micro-cell's metric from those, integrates, and sums into the LDS box.  FP32
+// micro-cell's metric from those, integrates, and sums into its wavefront's
+// LDS result box (waveBoxAccumulate).  FP32
 // destinations receive the box with FP32 atomics.  FP16 destinations keep every
 // sum in FP32: box-interior points belong to this tile alone and are stored once
 // (or added to, rmw), box-surface points (a lattice coordinate divisible by 8,
@@ -411,6 +412,7 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
 {
    __shared__ float  sb[kBox3];
    __shared__ double sv[( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 )][3];
+   __shared__ float  sy[kWaveBoxes * kBox3];
    const int64_t    lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
       return;
@@ -509,20 +511,14 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
       G[4]             = float( s * ( n[1][0] * n[2][0] + n[1][1] * n[2][1] + n[1][2] * n[2][2] ) );
       G[5]             = float( s * ( n[2][0] * n[2][0] + n[2][1] * n[2][1] + n[2][2] * n[2][2] ) );
    }
-   __syncthreads();
-   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
-      sb[p] = 0.0f;
-   __syncthreads();
-   if ( valid )
    {
       float y[10];
 #pragma unroll
       for ( int i = 0; i < 10; ++i )
          y[i] = 0.0f;
-      p2ReducedAction( G, u, y );
-#pragma unroll
-      for ( int i = 0; i < 10; ++i )
-         atomicAdd( &sb[pos[i]], y[i] );
+      if ( valid )
+         p2ReducedAction( G, u, y );
+      waveBoxAccumulate( sy, valid, pos, y );
    }
    __syncthreads();
    T* dstC = dst + base;
@@ -533,7 +529,7 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
       if ( X + Y + Z > top )
          continue;
       const int64_t o = tetRowStart32( int32_t( w1 ), Y, Z ) + X;
-      const float   v = alpha * sb[p];
+      const float   v = alpha * waveBoxSum( sy, p );
       if constexpr ( std::is_same< T, float >::value )
          atomicAdd( dstC + o, v );
       else
@@ -712,10 +708,17 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
 {
    using R  = typename Acc< T >::type;
    using QC = QuadConst< R >;
-   // one LDS box: first the tile's source values, then (after every thread has
-   // read its 10 values) the accumulator of the element results
-   __shared__ R sb[kBox3];
-   __shared__ R sk[STORED ? 11 * kTileThreads : 1];
+   // Element results are summed with LDS atomics on the source box in FP64, and
+   // in per-wavefront result boxes (waveBoxAccumulate) in FP32, where the LDS
+   // atomics cost about half the kernel time (in FP64 the boxes are not faster).
+   // LDS: the tile's source box and the staged coefficients; with result boxes,
+   // the same space holds them once every wavefront has integrated.
+   constexpr bool kAtomic = std::is_same< R, double >::value;
+   constexpr int  kStage  = kBox3 + ( STORED ? 11 * kTileThreads : 0 );
+   constexpr int  kLds    = ( kAtomic || kStage > kWaveBoxes * kBox3 ) ? kStage : kWaveBoxes * kBox3;
+   __shared__ R   lds[kLds];
+   R* const       sb = lds;
+   R* const       sk = lds + kBox3;
    const int64_t li = xcdBlock( ntiles );
    if ( li >= ntiles )
       return;
@@ -754,13 +757,16 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
       for ( int i = 0; i < 10; ++i )
          u[i] = th.valid ? sb[pos[i]] : R( 0 );
    }
-   __syncthreads();
-   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
-      sb[p] = R( 0 );
-   __syncthreads();
+   if constexpr ( kAtomic )
+   {
+      __syncthreads();
+      for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+         sb[p] = R( 0 );
+      __syncthreads();
+   }
+   R y[10];
    if ( th.valid )
    {
-      R y[10];
       p2ElementAction( g, th, u, y, [&]( int q ) -> R {
          if constexpr ( STORED )
             return sk[q * kTileThreads + threadIdx.x];
@@ -774,11 +780,24 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
             return R( kAt( g64, *kp, x64, t64.ls0 + xq64[3] ) );
          }
       } );
+   }
+   {
       int pos[10];
       elementBoxPositions( th, pos );
+      if constexpr ( kAtomic )
+      {
+         if ( th.valid )
+         {
 #pragma unroll
-      for ( int i = 0; i < 10; ++i )
-         atomicAdd( &sb[pos[i]], y[i] );
+            for ( int i = 0; i < 10; ++i )
+               atomicAdd( &sb[pos[i]], y[i] );
+         }
+      }
+      else
+      {
+         __syncthreads(); // the source box and the coefficients are no longer read
+         waveBoxAccumulate( lds, th.valid, pos, y );
+      }
    }
    __syncthreads();
    // every box point is added with an atomic that returns nothing: box-surface
@@ -791,7 +810,7 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
       if ( X + Y + Z <= top )
-         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * sb[p] );
+         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * ( kAtomic ? sb[p] : waveBoxSum( lds, p ) ) );
    }
 }
 

@@ -66,8 +66,9 @@ __device__ __forceinline__ void decodeMicro( int64_t wt, int64_t j, int64_t& x, 
 // (all 6 micro-cell types: 384 threads, one micro-cell each, one type per
 // wavefront).  The tile's P2 values sit on a (2 kTile + 1)^3 box of the level
 // L+1 lattice; the box is staged through LDS, element results are summed in
-// LDS, and the box is added back once with atomics that return nothing
-// (box-surface points are shared with neighbouring tiles).
+// LDS (one result box per wavefront), and the box is added back once with
+// atomics that return nothing (box-surface points are shared with neighbouring
+// tiles).
 constexpr int kTile        = 4;
 constexpr int kBox         = 2 * kTile + 1;
 constexpr int kBox3        = kBox * kBox * kBox;
@@ -235,6 +236,41 @@ __device__ __forceinline__ void elementBoxPositions( const TileThread< R >& th, 
 #pragma unroll
    for ( int e = 0; e < 6; ++e )
       pos[4 + e] = ( pos[cP2E[e][0]] + pos[cP2E[e][1]] ) / 2;
+}
+
+// ---- per-wavefront accumulation of element results ---------------------------
+// In the 384-thread tile kernels a wavefront holds the micro-cells of one type at
+// 64 distinct anchors, so for one local DoF i its lanes address 64 distinct box
+// points.  Wavefront w sums its lanes' results into box copy w (kWaveBoxes
+// copies of kBox3 values, one per wavefront of the tile), one local DoF after
+// the other, with plain LDS reads and writes: a wavefront's LDS operations
+// complete in issue order, and the accesses are volatile so that the compiler
+// keeps that order.  The write-back adds the copies (waveBoxSum).  LDS atomics
+// on one shared box cost about twice the whole kernel time of this scheme.
+constexpr int kWaveBoxes = kTileThreads / 64;
+
+template < class R >
+__device__ __forceinline__ void waveBoxAccumulate( R* boxes, bool valid, const int pos[10], const R y[10] )
+{
+   using LdsR     = volatile __attribute__( ( address_space( 3 ) ) ) R;
+   const int lane = int( threadIdx.x ) & 63;
+   LdsR*     b    = (LdsR*) ( boxes + kBox3 * __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 ) );
+   for ( int p = lane; p < kBox3; p += 64 )
+      b[p] = R( 0 );
+   if ( valid )
+   {
+#pragma unroll
+      for ( int i = 0; i < 10; ++i )
+         b[pos[i]] = b[pos[i]] + y[i];
+   }
+}
+// box point p summed over the wavefront copies (after a workgroup barrier)
+template < class R >
+__device__ __forceinline__ R waveBoxSum( const R* boxes, int p )
+{
+   static_assert( kWaveBoxes == 6, "waveBoxSum adds six copies" );
+   return ( ( boxes[p] + boxes[kBox3 + p] ) + ( boxes[2 * kBox3 + p] + boxes[3 * kBox3 + p] ) ) +
+          ( boxes[4 * kBox3 + p] + boxes[5 * kBox3 + p] );
 }
 
 // dst += v at a point shared with other tiles
