@@ -545,27 +545,33 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
 
 // FP16 destinations of kP2TileApplyReduced: the tile-surface points (a lattice
 // coordinate divisible by 8) receive their FP32 sums, which are reset to zero.
-// One block per (lattice slice Z, cell).
-__global__ void kP2FinishAccum( int64_t sc1, int32_t w1, int rmw, float* __restrict__ accum, __half* __restrict__ dst )
+// One thread per (surface point of the list, cell).
+__global__ void kP2FinishAccum( const int32_t* __restrict__ pts, int64_t npts, int64_t sc1, int rmw,
+                                float* __restrict__ accum, __half* __restrict__ dst )
 {
-   const int32_t Z  = int32_t( blockIdx.x );
-   const int64_t c  = blockIdx.y;
-   const int32_t wz = w1 - Z;
-   if ( wz <= 0 )
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( i >= npts )
       return;
-   for ( int32_t Y = 0; Y < wz; ++Y )
-   {
-      const int64_t start = c * sc1 + tetRowStart32( w1, Y, Z );
-      const int32_t len   = wz - Y;
-      const int32_t step  = ( Z % 8 == 0 || Y % 8 == 0 ) ? 1 : 8; // whole row, or the X divisible by 8
-      for ( int32_t X = int32_t( threadIdx.x ) * step; X < len; X += int32_t( blockDim.x ) * step )
-      {
-         const int64_t o = start + X;
-         const float   v = accum[o];
-         accum[o]        = 0.0f;
-         dst[o]          = __float2half( rmw ? __half2float( dst[o] ) + v : v );
-      }
-   }
+   const int64_t o = int64_t( blockIdx.y ) * sc1 + pts[i];
+   const float   v = accum[o];
+   accum[o]        = 0.0f;
+   dst[o]          = __float2half( rmw ? __half2float( dst[o] ) + v : v );
+}
+
+// the lattice's list of tile-surface points (built on first use)
+static void surfacePointList( Context* ctx, Lattice& l )
+{
+   if ( l.surfacePoints.p )
+      return;
+   constexpr int32_t       s = 2 * kTile;
+   const int32_t           w = int32_t( l.w );
+   std::vector< int32_t > pts;
+   for ( int32_t Z = 0; Z < w; ++Z )
+      for ( int32_t Y = 0; Y + Z < w; ++Y )
+         for ( int32_t X = 0; X + Y + Z < w; ++X )
+            if ( X % s == 0 || Y % s == 0 || Z % s == 0 )
+               pts.push_back( tetRowStart32( w, Y, Z ) + X );
+   l.surfacePoints.upload( ctx, pts );
 }
 
 // y = A_e u for the micro-cell of th: the element action of
@@ -1108,6 +1114,7 @@ static void p2ApplyReduced( Context* ctx, P2Operator& op, Lattice& lat, int L, c
       if ( !buf.p )
          buf.alloc( ctx, size_t( lat.nc * lat.sc ) ); // zero-filled once; every finish pass resets what it read
       accum = buf.p;
+      surfacePointList( ctx, lat );
    }
    ctx->beginTiming();
    kP2TileApplyReduced< T ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
@@ -1116,8 +1123,9 @@ static void p2ApplyReduced( Context* ctx, P2Operator& op, Lattice& lat, int L, c
    ctx->endTiming();
    if constexpr ( std::is_same< T, __half >::value )
    {
-      kP2FinishAccum<<< dim3( unsigned( lat.w ), unsigned( lat.nc ) ), 256, 0, ctx->stream >>>( lat.sc, int32_t( lat.w ),
-                                                                                               rmw, accum, d );
+      const int64_t npts = int64_t( lat.surfacePoints.n );
+      kP2FinishAccum<<< dim3( unsigned( ( npts + 255 ) / 256 ), unsigned( lat.nc ) ), 256, 0, ctx->stream >>>(
+          lat.surfacePoints.p, npts, lat.sc, rmw, accum, d );
       HG_CHECK_HIP( hipGetLastError() );
    }
 }

@@ -325,6 +325,102 @@ __global__ void kP2ProlongCell( const int32_t* __restrict__ rows, int64_t nrows,
    }
 }
 
+// Prolongation on cell-interior fine points, one workgroup per chunk of the
+// lattice's chunk list (prolongChunks): the points of wavefront r all have
+// stencil class r + 4 (y mod 4) + 16 (z mod 4), so offsets and weights are
+// uniform over the wavefront, and the four wavefronts of a chunk fill the same
+// fine rows.
+constexpr int kProlongChunk = 256;
+template < class T >
+__global__ __launch_bounds__( kProlongChunk ) void kP2ProlongChunks( const uint32_t* __restrict__ chunks,
+                                                                     int64_t nchunks, int64_t nc, int64_t scf,
+                                                                     int32_t wf, int64_t scc, int32_t wc,
+                                                                     const T* __restrict__ coarse,
+                                                                     T* __restrict__ fine, int add )
+{
+   using A          = typename Acc< T >::type;
+   const int64_t lb = xcdBlock( nchunks * nc );
+   if ( lb >= nchunks * nc )
+      return;
+   const int64_t  c   = lb / nchunks;
+   const int64_t  k   = lb - c * nchunks;
+   const uint32_t pt = chunks[k * kProlongChunk + threadIdx.x];
+   if ( pt == 0xFFFFFFFFu )
+      return;
+   // the stencil class of the wavefront's points, from its first active lane
+   const uint32_t p0  = uint32_t( __builtin_amdgcn_readfirstlane( int( pt ) ) );
+   const int      cls = int( p0 & 3u ) + 4 * int( ( p0 >> 10 ) & 3u ) + 16 * int( ( p0 >> 20 ) & 3u );
+   const int32_t  x = int32_t( pt & 1023u ), y = int32_t( ( pt >> 10 ) & 1023u ), z = int32_t( pt >> 20 );
+   const int32_t bx = 2 * ( x >> 2 ), by = 2 * ( y >> 2 ), bz = 2 * ( z >> 2 );
+   const T*      cc = coarse + c * scc;
+   A             v  = A( 0 );
+   for ( int e = 0; e < cPN[cls]; ++e )
+   {
+      const uint32_t o  = cPOffP[cls][e];
+      const int32_t  cx = bx + int32_t( o & 255u ), cy = by + int32_t( ( o >> 8 ) & 255u ), cz = bz + int32_t( o >> 16 );
+      if ( cx + cy + cz > wc - 1 )
+         continue;
+      v += A( cPW[cls][e] ) * toAcc( cc[tetRowStart32( wc, cy, cz ) + cx] );
+   }
+   T* out = fine + c * scf + tetRowStart32( wf, y, z ) + x;
+   if ( add )
+      v += toAcc( *out );
+   *out = fromAcc< T >( v );
+}
+
+// the chunk list of a fine lattice (built on first use; lattices up to 1023
+// micro-edges, the packing's limit)
+static void prolongChunkList( Context* ctx, Lattice& l )
+{
+   if ( l.prolongChunks.p || l.mm < 4 || l.mm > 1023 )
+      return;
+   std::vector< uint32_t > out;
+   for ( int g = 0; g < 16; ++g )
+   {
+      const int64_t yr = g & 3, zr = g >> 2;
+      std::vector< uint32_t > wave[4];
+      auto flush = [&] {
+         if ( wave[0].empty() && wave[1].empty() && wave[2].empty() && wave[3].empty() )
+            return;
+         for ( auto& w : wave )
+         {
+            w.resize( 64, 0xFFFFFFFFu );
+            out.insert( out.end(), w.begin(), w.end() );
+            w.clear();
+         }
+      };
+      for ( int64_t z = 1; z <= l.mm - 3; ++z )
+      {
+         if ( ( z & 3 ) != zr )
+            continue;
+         for ( int64_t y = 1; y + z <= l.mm - 2; ++y )
+         {
+            if ( ( y & 3 ) != yr )
+               continue;
+            size_t add[4] = { 0, 0, 0, 0 };
+            for ( int64_t x = 1; x + y + z <= l.mm - 1; ++x )
+               ++add[x & 3];
+            for ( int r = 0; r < 4; ++r )
+               if ( wave[r].size() + add[r] > 64 )
+               {
+                  flush();
+                  break;
+               }
+            for ( int64_t x = 1; x + y + z <= l.mm - 1; ++x )
+            {
+               if ( wave[x & 3].size() == 64 ) // a row longer than a chunk
+                  flush();
+               wave[x & 3].push_back( uint32_t( x ) | ( uint32_t( y ) << 10 ) | ( uint32_t( z ) << 20 ) );
+            }
+         }
+      }
+      flush();
+   }
+   if ( out.empty() )
+      return;
+   l.prolongChunks.upload( ctx, out );
+}
+
 template < class T >
 __global__ void kP2ProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
                                int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine, int add )
@@ -636,9 +732,18 @@ void p2Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update 
       using T = decltype( tag );
       if ( lf.nrows > 0 )
       {
+         prolongChunkList( ctx, lf );
          ctx->beginTiming();
-         kP2ProlongCell< T ><<< xcdGrid( lf.nrows * lf.nc ), kRowBlockT, 0, ctx->stream >>>(
-             lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update );
+         if ( lf.prolongChunks.p )
+         {
+            const int64_t nchunks = int64_t( lf.prolongChunks.n ) / kProlongChunk;
+            kP2ProlongChunks< T ><<< xcdGrid( nchunks * lf.nc ), kProlongChunk, 0, ctx->stream >>>(
+                lf.prolongChunks.p, nchunks, lf.nc, lf.sc, int32_t( lf.w ), lc.sc, int32_t( lc.w ), (const T*) cp,
+                (T*) fp, update );
+         }
+         else
+            kP2ProlongCell< T ><<< xcdGrid( lf.nrows * lf.nc ), kRowBlockT, 0, ctx->stream >>>(
+                lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update );
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
       }

@@ -109,6 +109,17 @@ struct Lattice
    DevBuf< uint32_t > classPoints;
    int32_t            classWave[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
 
+   // cell-local offsets of the points with a coordinate divisible by 8 (the
+   // points the 9^3-point boxes of the P2 tile kernels share), ordered by
+   // (z, y, x).  Built on first use (FP16 reduced-form apply).
+   DevBuf< int32_t > surfacePoints;
+
+   // cell-interior points (x | y << 10 | z << 20) in chunks of 4 wavefronts: the
+   // rows of a chunk share (y mod 4, z mod 4), wavefront r of a chunk holds
+   // their points with x mod 4 = r (padded with 0xFFFFFFFF).  Built on first use
+   // (P2 prolongation into this lattice).
+   DevBuf< uint32_t > prolongChunks;
+
    int64_t cellPos( int c, int64_t x, int64_t y, int64_t z ) const { return c * sc + tetIndex( w, x, y, z ); }
    int64_t ownerPos( const Topology& t, int c, int64_t x, int64_t y, int64_t z ) const
    {

@@ -52,6 +52,32 @@ def test_p2_transfer_parity(ctx, cube_pair, shell_pair, mesh):
             assert relmax(f.download(Lc), ref) < TOL, (Lc, flag)
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f16"])
+def test_p2_prolongate_parity_large(ctx, cube_pair, dtype):
+    """prolongation into cube P2 levels 5 and 6 (fine lattices of 64 and 128
+    micro-edges: chunks of several rows per workgroup) against the oracle; FP16
+    values against the oracle applied to the same FP16-rounded coarse values"""
+    st, om = cube_pair
+    rng = np.random.default_rng(9)
+    dt = hg.F64 if dtype == "f64" else hg.F16
+    f = hg.P2Function("f", st, 4, 6, dtype=dt)
+    for Lc in (4, 5):
+        print(f"P2 prolongation parity cube {Lc}->{Lc + 1} {dtype}", flush=True)
+        nc, nf = om.ndofs(Lc + 1), om.ndofs(Lc + 2)
+        c, fine0 = rng.uniform(-1, 1, nc), rng.uniform(-1, 1, nf)
+        if dt == hg.F16:
+            c, fine0 = c.astype(np.float16).astype(np.float64), fine0.astype(np.float16).astype(np.float64)
+        for flag, update in ((hg.ALL, hg.REPLACE), (hg.GMG_FLAG, hg.ADD)):
+            f.upload(Lc, c)
+            f.upload(Lc + 1, fine0)
+            hg.prolongate(f, Lc, flag, update)
+            ref = fine0.copy()
+            om.p2_prolongate(Lc, c, ref, flag, add=update == hg.ADD)
+            tol = TOL if dt == hg.F64 else 2e-3  # one FP16 rounding of values below 4 in magnitude
+            assert relmax(f.download(Lc + 1), ref) < tol, (Lc, flag, update, relmax(f.download(Lc + 1), ref))
+    f.destroy()
+
+
 def _problem(st, om, L, seed=0):
     k = problems.shell_coefficient(0)
     A = hg.P2DivKGradOperator(st, 0, L, *k)
