@@ -408,7 +408,8 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
                                                                       const __half* __restrict__ kc, int64_t mm,
                                                                       int64_t ntiles, int64_t sc1, int64_t w1,
                                                                       const T* __restrict__ src, T* __restrict__ dst,
-                                                                      float* __restrict__ accum, int rmw, float alpha )
+                                                                      float* __restrict__ accum, int rmw, float alpha,
+                                                                      const uint64_t* __restrict__ tiles )
 {
    __shared__ float  sb[kBox3];
    __shared__ double sv[( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 )][3];
@@ -416,7 +417,8 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
    const int64_t    lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
       return;
-   const TileCoord tc   = tileOf( lb, mm );
+   int64_t         full;
+   const TileCoord tc   = tiles ? tileFromList( tiles, lb, mm, full ) : tileOf( lb, mm );
    const int64_t   base = tc.c * sc1;
    const int32_t   top  = int32_t( 2 * mm );
    const float     ke   = __half2float( kc[lb * kTileThreads + threadIdx.x] );
@@ -710,7 +712,7 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
                                                  const P2CellGeom* __restrict__ geom64, const KParams* __restrict__ kp,
                                                  const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1,
                                                  int64_t w1, const T* __restrict__ src, T* __restrict__ dst,
-                                                 double alpha )
+                                                 double alpha, const uint64_t* __restrict__ tiles )
 {
    using R  = typename Acc< T >::type;
    using QC = QuadConst< R >;
@@ -729,28 +731,46 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
    if ( li >= ntiles )
       return;
    int64_t         lb;
-   const TileCoord tc   = tileOfInterior( li, mm, lb );
+   const TileCoord tc   = tiles ? tileFromList( tiles, li, mm, lb ) : tileOfInterior( li, mm, lb );
    const int64_t   base = tc.c * sc1;
    const int32_t   top  = int32_t( 2 * mm );
    // the coefficient loads are issued first (their latency overlaps the box
    // load) and parked in LDS, which keeps them out of the registers of the
    // quadrature loop
+   // and the thread's (at most two) source-box values right behind them, so
+   // that the two load latencies overlap
+   static_assert( kBox3 <= 2 * kTileThreads, "two box points per thread" );
+   K kv[STORED ? 11 : 1];
    if constexpr ( STORED )
    {
-      K kv[11];
 #pragma unroll
       for ( int q = 0; q < 11; ++q )
          kv[q] = table[( lb * 11 + q ) * kTileThreads + threadIdx.x];
+   }
+   const T* srcC = src + base;
+   T        bv[2];
+   bool     bin[2];
+#pragma unroll
+   for ( int j = 0; j < 2; ++j )
+   {
+      const int     p  = int( threadIdx.x ) + j * kTileThreads;
+      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
+      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
+      bin[j]          = p < kBox3 && X + Y + Z <= top;
+      bv[j]           = bin[j] ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : T( 0 );
+   }
+   if constexpr ( STORED )
+   {
 #pragma unroll
       for ( int q = 0; q < 11; ++q )
          sk[q * kTileThreads + threadIdx.x] = R( toAcc( kv[q] ) );
    }
-   const T* srcC = src + base;
-   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+#pragma unroll
+   for ( int j = 0; j < 2; ++j )
    {
-      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
-      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      sb[p]           = ( X + Y + Z <= top ) ? R( toAcc( srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] ) ) : R( 0 );
+      const int p = int( threadIdx.x ) + j * kTileThreads;
+      if ( p < kBox3 )
+         sb[p] = bin[j] ? R( toAcc( bv[j] ) ) : R( 0 );
    }
    __syncthreads();
    const P2CellGeomT< R >& g  = geom[tc.c];
@@ -828,7 +848,8 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
                                                           const P2CellGeom* __restrict__ geom64,
                                                           const KParams* __restrict__ kp, const K* __restrict__ table,
                                                           int64_t mm, int64_t ndiag, int64_t sc1, int64_t w1,
-                                                          const T* __restrict__ src, T* __restrict__ dst, double alpha )
+                                                          const T* __restrict__ src, T* __restrict__ dst, double alpha,
+                                                          const uint64_t* __restrict__ tiles )
 {
    using R = typename Acc< T >::type;
    __shared__ R  sb[kBox3];
@@ -836,15 +857,27 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
    if ( ld >= ndiag )
       return;
    int64_t         lb;
-   const TileCoord tc   = tileOfDiagonal( ld, mm, lb );
+   const TileCoord tc   = tiles ? tileFromList( tiles, ld, mm, lb ) : tileOfDiagonal( ld, mm, lb );
    const int64_t   base = tc.c * sc1;
    const int32_t   top  = int32_t( 2 * mm );
    const T*        srcC = src + base;
-   for ( int p = threadIdx.x; p < kBox3; p += 64 )
+   // the lane's 12 box points: all loads issued before the first LDS write
+   constexpr int kPer = ( kBox3 + 63 ) / 64;
+   T             bv[kPer];
+#pragma unroll
+   for ( int j = 0; j < kPer; ++j )
    {
+      const int     p  = int( threadIdx.x ) + 64 * j;
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      sb[p]           = ( X + Y + Z <= top ) ? R( toAcc( srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] ) ) : R( 0 );
+      bv[j]           = ( p < kBox3 && X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : T( 0 );
+   }
+#pragma unroll
+   for ( int j = 0; j < kPer; ++j )
+   {
+      const int p = int( threadIdx.x ) + 64 * j;
+      if ( p < kBox3 )
+         sb[p] = R( toAcc( bv[j] ) );
    }
    __syncthreads();
    const P2CellGeomT< R >& g    = geom[tc.c];
@@ -898,18 +931,18 @@ template < class T, class K >
 __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyStored(
     const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
     const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
-    const T* __restrict__ src, T* __restrict__ dst, double alpha )
+    const T* __restrict__ src, T* __restrict__ dst, double alpha, const uint64_t* __restrict__ tiles )
 {
-   p2TileApplyBody< T, K, true >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst, alpha );
+   p2TileApplyBody< T, K, true >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst, alpha, tiles );
 }
 // coefficients evaluated in place (the compiler's own register budget)
 template < class T, class K >
 __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyEval(
     const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const P2CellGeom* __restrict__ geom64,
     const KParams* __restrict__ kp, const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1,
-    const T* __restrict__ src, T* __restrict__ dst, double alpha )
+    const T* __restrict__ src, T* __restrict__ dst, double alpha, const uint64_t* __restrict__ tiles )
 {
-   p2TileApplyBody< T, K, false >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst, alpha );
+   p2TileApplyBody< T, K, false >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst, alpha, tiles );
 }
 
 __global__ void kP2ElementDiag( const CellGeom* __restrict__ geom, const KParams* __restrict__ kp, int64_t mm,
@@ -1116,9 +1149,10 @@ static void p2ApplyReduced( Context* ctx, P2Operator& op, Lattice& lat, int L, c
       accum = buf.p;
       surfacePointList( ctx, lat );
    }
+   const uint64_t* tiles = tileList( ctx, lat, mm, kTilesAll );
    ctx->beginTiming();
    kP2TileApplyReduced< T ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
-       op.tgeom.at( L ).p, kc, mm, ntiles, lat.sc, lat.w, s, d, accum, rmw, float( alpha ) );
+       op.tgeom.at( L ).p, kc, mm, ntiles, lat.sc, lat.w, s, d, accum, rmw, float( alpha ), tiles );
    HG_CHECK_HIP( hipGetLastError() );
    ctx->endTiming();
    if constexpr ( std::is_same< T, __half >::value )
@@ -1174,6 +1208,8 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
       const int64_t ndiag  = tri( tileEdges( mm ) ) * lat.nc;     // last-layer tiles
       const K*      table  = p2Coefficients< K >( ctx, op, L, src.dtype );
       const auto*   geom   = p2Geometry< T >( op, L );
+      const uint64_t* tint  = mm > kSmallMaxMM ? tileList( ctx, lat, mm, kTilesInterior ) : nullptr;
+      const uint64_t* tlast = mm > kSmallMaxMM ? tileList( ctx, lat, mm, kTilesLast ) : nullptr;
       ctx->beginTiming();
       if ( mm <= kSmallMaxMM )
       {
@@ -1190,17 +1226,21 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
             throw Error( HG_ERR_STATE, "P2 tile apply: micro-edge count not a multiple of the tile width" );
          if ( nint > 0 && table )
             kP2TileApplyStored< T, K ><<< xcdGrid( nint ), kTileThreads, 0, ctx->stream >>>(
-                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, nint, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, nint, lat.sc, lat.w, (const T*) s, (T*) d, alpha,
+                tint );
          else if ( nint > 0 )
             kP2TileApplyEval< T, K ><<< xcdGrid( nint ), kTileThreads, 0, ctx->stream >>>(
-                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, nint, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, nint, lat.sc, lat.w, (const T*) s, (T*) d, alpha,
+                tint );
          HG_CHECK_HIP( hipGetLastError() );
          if ( table )
             kP2TileApplyDiag< T, K, true ><<< xcdGrid( ndiag ), 64, 0, ctx->stream >>>(
-                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ndiag, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+                geom, op.tgeom.at( L ).p, op.kpar.p, table, mm, ndiag, lat.sc, lat.w, (const T*) s, (T*) d, alpha,
+                tlast );
          else
             kP2TileApplyDiag< T, K, false ><<< xcdGrid( ndiag ), 64, 0, ctx->stream >>>(
-                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, ndiag, lat.sc, lat.w, (const T*) s, (T*) d, alpha );
+                geom, op.tgeom.at( L ).p, op.kpar.p, nullptr, mm, ndiag, lat.sc, lat.w, (const T*) s, (T*) d, alpha,
+                tlast );
       }
       HG_CHECK_HIP( hipGetLastError() );
       ctx->endTiming();

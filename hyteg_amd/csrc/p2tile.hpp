@@ -132,6 +132,71 @@ __device__ __forceinline__ TileCoord tileOfDiagonal( int64_t lb, int64_t mm, int
    full             = tc.c * tet( T ) + ( tet( T ) - tet( T - tk ) ) + triIndex( T - tk, ti, tj );
    return tc;
 }
+// ---- tile lists -------------------------------------------------------------
+// Decoding a tile number by search costs a wavefront several hundred scalar
+// instructions (64-bit divisions) before its first load can issue.  The apply
+// kernels read their tile from a per-lattice list instead, one scalar load:
+// entry = c | m << 24 | ti << 40 | tj << 48 | tk << 56 (cell, number of the tile
+// in the cell in the numbering of all tiles, anchor / kTile).  List kTilesAll
+// follows tileOf, kTilesInterior tileOfInterior, kTilesLast tileOfDiagonal.
+// Lists exist while tet(T) < 2^16 and the cell count is below 2^24 (T: tile
+// edge count); otherwise the kernels decode.
+enum TileListKind
+{
+   kTilesAll      = 0,
+   kTilesInterior = 1,
+   kTilesLast     = 2
+};
+inline const uint64_t* tileList( Context* ctx, Lattice& l, int64_t mm, int kind )
+{
+   const int64_t T = tileEdges( mm );
+   if ( tet( T ) >= ( int64_t( 1 ) << 16 ) || l.nc >= ( int64_t( 1 ) << 24 ) || T > 255 )
+      return nullptr;
+   DevBuf< uint64_t >& buf = l.tileLists[kind];
+   if ( !buf.p )
+   {
+      std::vector< uint64_t > e;
+      auto put = [&]( int64_t c, int64_t ti, int64_t tj, int64_t tk ) {
+         const int64_t m = tet( T ) - tet( T - tk ) + triIndex( T - tk, ti, tj );
+         e.push_back( uint64_t( c ) | ( uint64_t( m ) << 24 ) | ( uint64_t( ti ) << 40 ) | ( uint64_t( tj ) << 48 ) |
+                      ( uint64_t( tk ) << 56 ) );
+      };
+      for ( int64_t c = 0; c < l.nc; ++c )
+      {
+         if ( kind == kTilesLast )
+         {
+            for ( int64_t tj = 0; tj < T; ++tj )
+               for ( int64_t ti = 0; ti + tj < T; ++ti )
+                  put( c, ti, tj, T - 1 - ti - tj );
+            continue;
+         }
+         const int64_t E = kind == kTilesAll ? T : T - 1; // tiles with ti + tj + tk < E
+         for ( int64_t tk = 0; tk < E; ++tk )
+            for ( int64_t tj = 0; tj + tk < E; ++tj )
+               for ( int64_t ti = 0; ti + tj + tk < E; ++ti )
+                  put( c, ti, tj, tk );
+      }
+      if ( e.empty() )
+         return nullptr;
+      buf.upload( ctx, e );
+   }
+   return buf.p;
+}
+// tile i of a list (a uniform index: the entry comes with a scalar load through
+// the constant address space); full = its number in the numbering of all tiles
+__device__ __forceinline__ TileCoord tileFromList( const uint64_t* list, int64_t i, int64_t mm, int64_t& full )
+{
+   using ConstU64   = const __attribute__( ( address_space( 4 ) ) ) uint64_t;
+   const uint64_t e = ( (ConstU64*) list )[i];
+   TileCoord      tc;
+   tc.c  = int64_t( e & 0xFFFFFFu );
+   full  = tc.c * tilesPerCell( mm ) + int64_t( ( e >> 24 ) & 0xFFFFu );
+   tc.x0 = kTile * int64_t( ( e >> 40 ) & 255u );
+   tc.y0 = kTile * int64_t( ( e >> 48 ) & 255u );
+   tc.z0 = kTile * int64_t( e >> 56 );
+   return tc;
+}
+
 // the 64 (type, anchor) pairs of a last-layer tile inside the macro-cell, for
 // lattices whose micro-edge count is a multiple of kTile: anchor sum at most
 // 3 - rowShift(type); entry = type | anchor << 3

@@ -120,6 +120,10 @@ struct Lattice
    // (P2 prolongation into this lattice).
    DevBuf< uint32_t > prolongChunks;
 
+   // tile lists of the P2 tile kernels on this lattice (all tiles, interior
+   // tiles, last-layer tiles), built on first use
+   DevBuf< uint64_t > tileLists[3];
+
    int64_t cellPos( int c, int64_t x, int64_t y, int64_t z ) const { return c * sc + tetIndex( w, x, y, z ); }
    int64_t ownerPos( const Topology& t, int c, int64_t x, int64_t y, int64_t z ) const
    {
