@@ -155,13 +155,15 @@ void p2LaplaceElement( const double p[4][3], double A[10][10] )
 __global__ __launch_bounds__( kTileThreads ) void kP2ConstTileApply( const double* __restrict__ Ae, int64_t mm,
                                                                       int64_t ntiles, int64_t sc1, int64_t w1,
                                                                       const double* __restrict__ src,
-                                                                      double* __restrict__ dst )
+                                                                      double* __restrict__ dst,
+                                                                      const uint64_t* __restrict__ tiles )
 {
    __shared__ double sb[kBox3];
    const int64_t     lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
       return;
-   const TileCoord tc   = tileOf( lb, mm );
+   int64_t         full;
+   const TileCoord tc   = tiles ? tileFromList( tiles, lb, mm, full ) : tileOf( lb, mm );
    const int64_t   base = tc.c * sc1;
    const int32_t   top  = int32_t( 2 * mm );
    const double*   srcC = src + base;
@@ -521,10 +523,11 @@ int hg_p2_const_apply( hg_ctx* h, hg_handle opH, hg_handle src, hg_handle dst, i
          kZeroHalos< double ><<< gridFor( lat.haloPerCell() * lat.nc ), kBlock, 0, ctx->stream >>>(
              BndGeom::of( st, lat ), lat.nc, dp );
       HG_CHECK_HIP( hipGetLastError() );
-      const int64_t ntiles = tilesPerCell( mm ) * lat.nc;
+      const int64_t   ntiles = tilesPerCell( mm ) * lat.nc;
+      const uint64_t* tiles  = tileList( ctx, lat, mm, kTilesAll );
       ctx->beginTiming();
       kP2ConstTileApply<<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>( op.lv.at( level ).Ae.p, mm, ntiles,
-                                                                                lat.sc, lat.w, sp, dp );
+                                                                                lat.sc, lat.w, sp, dp, tiles );
       HG_CHECK_HIP( hipGetLastError() );
       ctx->endTiming();
       pushAdditive( ctx, st, lat, dp, 0, kAll ^ flag, update == HG_REPLACE );

