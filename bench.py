@@ -30,9 +30,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# FP64 operations of one micro-cell action in kP2TileApply: 11 quadrature points x
-# (65 FMA + 34 MUL + 8 ADD + 2 reciprocal / reciprocal-sqrt seeds) + 12 set-up = 1940
-P2_FLOPS_PER_ELEMENT = 11 * (2 * 65 + 34 + 8 + 2) + 26
+# FP64 operations of one micro-cell action in kP2TileApplyStored<double>, counted
+# from its instruction stream (FMA = 2): 11 quadrature points x (43 FMA + 21 MUL +
+# 5 ADD + 2 reciprocal / reciprocal-sqrt seeds) + about 230 for the gradient set-up
+# and the result = 1484
+P2_FLOPS_PER_ELEMENT = 11 * (2 * 43 + 21 + 5 + 2) + 230
 METRIC = "DoF/s per apply; time-to-1e-8 residual mixed vs FP64; solves/s + p99 latency"
 
 
@@ -93,8 +95,8 @@ def p2_shell(hg, ctx, L, steps, warmup):
     roof = {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "algorithmic_bytes_per_launch": alg,
             "note": "algorithmic bytes = 24 per P2 DoF (src, dst, one k value, FP64); the exact "
-                    "p2_div_k_grad_blending_q4 element integration issues ~1.9 kflop FP64 per micro-cell, so "
-                    "the kernel is bound by FP64 VALU issue, not by HBM (compute_fp64 below)",
+                    "p2_div_k_grad_blending_q4 element integration issues ~1.5 kflop FP64 per micro-cell, so "
+                    "the kernel is bound by FP64 VALU issue and latency, not by HBM (compute_fp64 below)",
             "compute_fp64": {"achieved_tflops": P2_FLOPS_PER_ELEMENT * nelem / (kms * 1e-3) / 1e12,
                              "peak_tflops": FP64_PEAK_TFLOPS,
                              "frac": P2_FLOPS_PER_ELEMENT * nelem / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
@@ -107,7 +109,8 @@ def p2_shell(hg, ctx, L, steps, warmup):
                                            f"level {L} ({n} DoFs)",
                                "level": L, "dofs": n, "mesh": "meshSphericalShell(5, 3, 0.55, 1.0)",
                                "coefficients": "stored at the quadrature points"}, \
-        "kP2TileApplyStored<double>", f"p2_apply_shell_L{L}_f64"
+        "kP2TileApplyStored<double> + kP2TileApplyDiag<double> (interior and last-layer tiles of one apply)", \
+        f"p2_apply_shell_L{L}_f64"
 
 
 def p2_apply_variants(hg, ctx):
@@ -155,7 +158,7 @@ def p2_transfers(hg, ctx):
     return {"fine_dofs": nf, "restrict_kernel_ms": r, "prolongate_kernel_ms": p,
             "restrict_hbm_frac": alg / (r * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "prolongate_hbm_frac": alg / (p * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "note": "kP2RestrictCell / kP2ProlongCell; algorithmic bytes 9 per fine DoF (8 fine + 8/8 coarse)"}
+            "note": "kP2RestrictCellList / kP2ProlongChunks; algorithmic bytes 9 per fine DoF (8 fine + 8/8 coarse)"}
 
 
 def p1_cube(hg, ctx, L, steps, warmup):
