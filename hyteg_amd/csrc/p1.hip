@@ -472,16 +472,131 @@ __global__ __launch_bounds__( 64 * kCellWorkers ) void kCellApply( const int32_t
       const int32_t x = 1 + ( t >> lv ), k = t & msk;
       if ( x >= top )
          break;
+      // all 15 loads in flight before the sum (same order of the sum)
+      T val[15];
+#pragma unroll
+      for ( int q = 0; q < 15; ++q )
+         val[q] = s[( sr.at( q, x ) << lv ) + k];
+      __builtin_amdgcn_sched_barrier( 0 ); // keeps the scheduler from moving the loads down to their uses
       A v = A( 0 );
 #pragma unroll
       for ( int q = 0; q < 15; ++q )
-         v += st[q] * toAcc( s[( sr.at( q, x ) << lv ) + k] );
+         v += st[q] * toAcc( val[q] );
       v               = A( alpha ) * v;
       const int32_t o = ( sr.at( 0, x ) << lv ) + k;
       if ( add )
          v += toAcc( d[o] );
       d[o] = fromAcc< T >( v );
    }
+}
+
+// The same apply over bricks of 64 x 4 x 4 cell-interior points (x, y rows, z
+// slices) for single-vector functions on lattices of at least 64 micro-edges:
+// a workgroup stages the brick with its one-point halo (66 x 6 x 6 values) in
+// LDS with coalesced row loads, then every lane sums its four points' 15
+// entries from LDS, in the order of kCellApply (same results bit for bit).
+// The row kernel issues 15 global loads per point and is bound by their
+// issue, not by HBM traffic.
+constexpr int kBrX = 64, kBrY = 4, kBrZ = 4;
+constexpr int kBrLX = kBrX + 2, kBrLY = kBrY + 2, kBrL = kBrLX * kBrLY * ( kBrZ + 2 );
+constexpr int kBrThreads = kBrX * kBrZ;
+// LDS offsets of the 15 stencil entries (the order of StencilRows::at)
+__device__ __forceinline__ constexpr int brickOffset( int q )
+{
+   constexpr int dx[15] = { 0, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1 };
+   constexpr int dy[15] = { 0, 0, 0, 1, 1, -1, -1, 0, 0, 1, 1, -1, -1, 0, 0 };
+   constexpr int dz[15] = { 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 0 };
+   return dx[q] + kBrLX * ( dy[q] + kBrLY * dz[q] );
+}
+
+template < class T >
+__global__ __launch_bounds__( kBrThreads ) void kCellApplyBricks( const uint64_t* __restrict__ bricks, int64_t nbricks,
+                                                                  int64_t sc, int32_t w, int32_t mm,
+                                                                  const double* __restrict__ stencil,
+                                                                  const T* __restrict__ src, T* __restrict__ dst,
+                                                                  int add, double alpha )
+{
+   using A = typename Acc< T >::type;
+   __shared__ A  lds[kBrL];
+   const int64_t bi = xcdBlock( nbricks );
+   if ( bi >= nbricks )
+      return;
+   using ConstU64   = const __attribute__( ( address_space( 4 ) ) ) uint64_t;
+   const uint64_t e = ( (ConstU64*) bricks )[bi];
+   const int64_t  c = int64_t( e & 0xFFFFFFu );
+   const int32_t  x0 = kBrX * int32_t( ( e >> 24 ) & 255u ), y0 = kBrY * int32_t( ( e >> 32 ) & 0xFFFFu ),
+                 z0 = kBrZ * int32_t( e >> 48 );
+   const T* s = src + c * sc;
+   T*       d = dst + c * sc;
+   // the brick and its halo: all loads issued before the first LDS write
+   constexpr int kPer = ( kBrL + kBrThreads - 1 ) / kBrThreads;
+   T             v[kPer];
+#pragma unroll
+   for ( int j = 0; j < kPer; ++j )
+   {
+      const int     i  = int( threadIdx.x ) + j * kBrThreads;
+      const int     lx = i % kBrLX, ly = ( i / kBrLX ) % kBrLY, lz = i / ( kBrLX * kBrLY );
+      const int32_t X = x0 - 1 + lx, Y = y0 - 1 + ly, Z = z0 - 1 + lz;
+      v[j] = ( i < kBrL && X >= 0 && Y >= 0 && Z >= 0 && X + Y + Z <= mm ) ? s[tetRowStart32( w, Y, Z ) + X] : T( 0 );
+   }
+   __builtin_amdgcn_sched_barrier( 0 );
+#pragma unroll
+   for ( int j = 0; j < kPer; ++j )
+   {
+      const int i = int( threadIdx.x ) + j * kBrThreads;
+      if ( i < kBrL )
+         lds[i] = A( toAcc( v[j] ) );
+   }
+   A st[15];
+#pragma unroll
+   for ( int q = 0; q < 15; ++q )
+      st[q] = A( stencil[15 * c + q] );
+   __syncthreads();
+   const int     lane = int( threadIdx.x ) & 63, wv = int( threadIdx.x ) >> 6;
+   const int32_t X = x0 + lane, Z = z0 + wv;
+#pragma unroll
+   for ( int yy = 0; yy < kBrY; ++yy )
+   {
+      const int32_t Y = y0 + yy;
+      if ( X < 1 || Y < 1 || Z < 1 || X + Y + Z > mm - 1 )
+         continue;
+      const int li = ( lane + 1 ) + kBrLX * ( ( yy + 1 ) + kBrLY * ( wv + 1 ) );
+      A         r  = A( 0 );
+#pragma unroll
+      for ( int q = 0; q < 15; ++q )
+         r += st[q] * lds[li + brickOffset( q )];
+      r               = A( alpha ) * r;
+      const int32_t o = tetRowStart32( w, Y, Z ) + X;
+      if ( add )
+         r += toAcc( d[o] );
+      d[o] = fromAcc< T >( r );
+   }
+}
+
+// the lattice's brick list (entry c | bx << 24 | by << 32 | bz << 48), built on
+// first use; null where the row kernel runs
+static const uint64_t* p1BrickList( Context* ctx, Lattice& l )
+{
+   if ( l.mm < kBrX || l.mm > 1023 || l.nc >= ( int64_t( 1 ) << 24 ) )
+      return nullptr;
+   if ( !l.p1Bricks.p )
+   {
+      std::vector< uint64_t > e;
+      const int64_t           mm = l.mm;
+      for ( int64_t c = 0; c < l.nc; ++c )
+         for ( int64_t bz = 0; kBrZ * bz <= mm; ++bz )
+            for ( int64_t by = 0; kBrY * by <= mm; ++by )
+               for ( int64_t bx = 0; kBrX * bx <= mm; ++bx )
+                  if ( std::max< int64_t >( kBrX * bx, 1 ) + std::max< int64_t >( kBrY * by, 1 ) +
+                           std::max< int64_t >( kBrZ * bz, 1 ) <=
+                       mm - 1 )
+                     e.push_back( uint64_t( c ) | ( uint64_t( bx ) << 24 ) | ( uint64_t( by ) << 32 ) |
+                                  ( uint64_t( bz ) << 48 ) );
+      if ( e.empty() )
+         return nullptr;
+      l.p1Bricks.upload( ctx, e );
+   }
+   return l.p1Bricks.p;
 }
 
 // SOR on the cell-interior points of one parity colour, in place
@@ -515,11 +630,17 @@ __global__ __launch_bounds__( 64 * kCellWorkers ) void kCellSOR( const int32_t* 
       if ( xx >= top )
          break;
       const int32_t o = ( sr.at( 0, xx ) << lv ) + k;
-      A             v = toAcc( bc[o] );
+      // all loads in flight before the sum (same order of the sum)
+      T val[15];
+#pragma unroll
+      for ( int q = 0; q < 15; ++q )
+         val[q] = xc[( sr.at( q, xx ) << lv ) + k];
+      A v = toAcc( bc[o] );
+      __builtin_amdgcn_sched_barrier( 0 ); // keeps the scheduler from moving the loads down to their uses
 #pragma unroll
       for ( int q = 1; q < 15; ++q )
-         v -= st[q] * toAcc( xc[( sr.at( q, xx ) << lv ) + k] );
-      const A xo = toAcc( xc[o] );
+         v -= st[q] * toAcc( val[q] );
+      const A xo = toAcc( val[0] );
       xc[o]      = fromAcc< T >( ( A( 1 ) - A( relax ) ) * xo + A( relax ) * invc * v );
    }
 }
@@ -887,11 +1008,22 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
       if ( ( flag & kInner ) && lat.nrows > 0 )
       {
          checkCellIndexing( lat, lv );
+         const uint64_t* bricks = lv == 0 ? p1BrickList( ctx, lat ) : nullptr;
          ctx->beginTiming();
-         const int rpb = rowsPerBlockLog( ( lat.mm - 3 ) << lv );
-         kCellApply< T ><<< cellGrid( lat.nrows * lat.nc, rpb ), 64 * kCellWorkers, 0, ctx->stream >>>(
-             lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update, lv,
-             rpb, alpha );
+         if ( bricks )
+         {
+            const int64_t nb = int64_t( lat.p1Bricks.n );
+            kCellApplyBricks< T ><<< xcdGrid( nb ), kBrThreads, 0, ctx->stream >>>(
+                bricks, nb, lat.sc, int32_t( lat.w ), int32_t( lat.mm ), o.stencil.p, (const T*) s, (T*) d, update,
+                alpha );
+         }
+         else
+         {
+            const int rpb = rowsPerBlockLog( ( lat.mm - 3 ) << lv );
+            kCellApply< T ><<< cellGrid( lat.nrows * lat.nc, rpb ), 64 * kCellWorkers, 0, ctx->stream >>>(
+                lat.rowsNat.p, lat.nrows, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, (const T*) s, (T*) d, update,
+                lv, rpb, alpha );
+         }
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
       }

@@ -124,6 +124,9 @@ struct Lattice
    // tiles, last-layer tiles), built on first use
    DevBuf< uint64_t > tileLists[3];
 
+   // bricks of the P1 cell-interior apply on this lattice (built on first use)
+   DevBuf< uint64_t > p1Bricks;
+
    int64_t cellPos( int c, int64_t x, int64_t y, int64_t z ) const { return c * sc + tetIndex( w, x, y, z ); }
    int64_t ownerPos( const Topology& t, int c, int64_t x, int64_t y, int64_t z ) const
    {

@@ -45,7 +45,10 @@ def test_upload_download_roundtrip_and_halo(ctx, name):
         f.destroy()
 
 
-@pytest.mark.parametrize("name,levels", [("cube", [2, 3, 4, 5]), ("octa", [1, 2, 3, 4])])
+# levels 6 and 7 (64 and 128 micro-edges) run the LDS-brick cell kernel, the
+# smaller ones the row kernel
+@pytest.mark.parametrize("name,levels", [("cube", [2, 3, 4, 5]), ("octa", [1, 2, 3, 4]), ("cube", [6, 7]),
+                                         ("octa", [6])])
 def test_apply_parity(ctx, name, levels):
     st, om = meshes(ctx, name)
     A = hg.P1ConstantLaplaceOperator(st, min(levels), max(levels))
