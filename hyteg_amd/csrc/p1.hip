@@ -490,14 +490,16 @@ __global__ __launch_bounds__( 64 * kCellWorkers ) void kCellApply( const int32_t
    }
 }
 
-// The same apply over bricks of 64 x 4 x 4 cell-interior points (x, y rows, z
+// The same apply over bricks of 64 x 8 x 8 cell-interior points (x, y rows, z
 // slices) for single-vector functions on lattices of at least 64 micro-edges:
-// a workgroup stages the brick with its one-point halo (66 x 6 x 6 values) in
-// LDS with coalesced row loads, then every lane sums its four points' 15
-// entries from LDS, in the order of kCellApply (same results bit for bit).
+// a workgroup (8 wavefronts, one per z slice) stages the brick with its
+// one-point halo (66 x 10 x 10 values) in LDS with coalesced row loads, then
+// every lane sums its eight points' 15 entries from LDS, in the order of
+// kCellApply (same results bit for bit).  (Bricks of 64 x 4 x 4 points took
+// 0.63 ms at cube level 9, 64 x 8 x 8 points 0.57 ms.)
 // The row kernel issues 15 global loads per point and is bound by their
 // issue, not by HBM traffic.
-constexpr int kBrX = 64, kBrY = 4, kBrZ = 4;
+constexpr int kBrX = 64, kBrY = 8, kBrZ = 8;
 constexpr int kBrLX = kBrX + 2, kBrLY = kBrY + 2, kBrL = kBrLX * kBrLY * ( kBrZ + 2 );
 constexpr int kBrThreads = kBrX * kBrZ;
 // LDS offsets of the 15 stencil entries (the order of StencilRows::at)
