@@ -167,11 +167,23 @@ __global__ __launch_bounds__( kTileThreads ) void kP2ConstTileApply( const doubl
    const int64_t   base = tc.c * sc1;
    const int32_t   top  = int32_t( 2 * mm );
    const double*   srcC = src + base;
-   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+   // the thread's (at most two) box points, both loads in flight before the LDS writes
+   static_assert( kBox3 <= 2 * kTileThreads, "two box points per thread" );
+   double bv[2];
+#pragma unroll
+   for ( int j = 0; j < 2; ++j )
    {
+      const int     p  = int( threadIdx.x ) + j * kTileThreads;
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      sb[p]           = ( X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : 0.0;
+      bv[j]           = ( p < kBox3 && X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : 0.0;
+   }
+#pragma unroll
+   for ( int j = 0; j < 2; ++j )
+   {
+      const int p = int( threadIdx.x ) + j * kTileThreads;
+      if ( p < kBox3 )
+         sb[p] = bv[j];
    }
    __syncthreads();
    // the thread's type and anchor (the geometry itself is not needed)

@@ -186,24 +186,53 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
    const int64_t   base1 = tc.c * sc1, base0 = tc.c * sc0;
    const int32_t   top1 = int32_t( 2 * mm ), top0 = int32_t( mm );
    const T*        uin[3] = { u0, u1, u2 };
+   // the thread's (at most two) velocity box points and one pressure box point:
+   // all loads issued before the first LDS write, so their latencies overlap
+   static_assert( kBox3 <= 2 * kTileThreads && kPBox3 <= kTileThreads, "box points per thread" );
+   T    uv[2][3], pvl = T( 0 );
+   bool uok[2] = { false, false }, pok = false;
    if constexpr ( NEED_U )
-      for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
+   {
+#pragma unroll
+      for ( int j = 0; j < 2; ++j )
       {
+         const int     p  = int( threadIdx.x ) + j * kTileThreads;
          const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
          const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-         const bool    in  = X + Y + Z <= top1;
+         uok[j]          = p < kBox3 && X + Y + Z <= top1;
          const int64_t idx = base1 + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
 #pragma unroll
          for ( int c = 0; c < 3; ++c )
-            su[c][p] = in ? R( toAcc( uin[c][idx] ) ) : R( 0 );
+            uv[j][c] = uok[j] ? uin[c][idx] : T( 0 );
       }
+   }
    if constexpr ( NEED_P )
-      for ( int p = threadIdx.x; p < kPBox3; p += kTileThreads )
+   {
+      const int     p  = int( threadIdx.x );
+      const int32_t lx = p % kPBox, ly = ( p / kPBox ) % kPBox, lz = p / ( kPBox * kPBox );
+      const int32_t X = int32_t( tc.x0 ) + lx, Y = int32_t( tc.y0 ) + ly, Z = int32_t( tc.z0 ) + lz;
+      pok             = p < kPBox3 && X + Y + Z <= top0;
+      pvl             = pok ? pin[base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X] : T( 0 );
+   }
+   if constexpr ( NEED_U )
+   {
+#pragma unroll
+      for ( int j = 0; j < 2; ++j )
       {
-         const int32_t lx = p % kPBox, ly = ( p / kPBox ) % kPBox, lz = p / ( kPBox * kPBox );
-         const int32_t X = int32_t( tc.x0 ) + lx, Y = int32_t( tc.y0 ) + ly, Z = int32_t( tc.z0 ) + lz;
-         sp[p] = ( X + Y + Z <= top0 ) ? R( toAcc( pin[base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X] ) ) : R( 0 );
+         const int p = int( threadIdx.x ) + j * kTileThreads;
+         if ( p < kBox3 )
+         {
+#pragma unroll
+            for ( int c = 0; c < 3; ++c )
+               su[c][p] = uok[j] ? R( toAcc( uv[j][c] ) ) : R( 0 );
+         }
       }
+   }
+   if constexpr ( NEED_P )
+   {
+      if ( int( threadIdx.x ) < kPBox3 )
+         sp[threadIdx.x] = pok ? R( toAcc( pvl ) ) : R( 0 );
+   }
    __syncthreads();
    const P2CellGeomT< R >& g  = geom[tc.c];
    const TileThread< R >   th = tileThread( g, tc, mm );
