@@ -423,11 +423,24 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
    const int32_t   top  = int32_t( 2 * mm );
    const float     ke   = __half2float( kc[lb * kTileThreads + threadIdx.x] );
    const T*        srcC = src + base;
-   for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
    {
-      const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
-      const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      sb[p] = ( X + Y + Z <= top ) ? float( toAcc( srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] ) ) : 0.0f;
+      // the thread's (at most two) box points, both loads in flight before the LDS writes
+      T bv[2];
+#pragma unroll
+      for ( int j = 0; j < 2; ++j )
+      {
+         const int     p  = int( threadIdx.x ) + j * kTileThreads;
+         const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
+         const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
+         bv[j]           = ( p < kBox3 && X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : T( 0 );
+      }
+#pragma unroll
+      for ( int j = 0; j < 2; ++j )
+      {
+         const int p = int( threadIdx.x ) + j * kTileThreads;
+         if ( p < kBox3 )
+            sb[p] = float( toAcc( bv[j] ) );
+      }
    }
    const P2CellGeom& g  = geom[tc.c];
    constexpr int     nv = ( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 );

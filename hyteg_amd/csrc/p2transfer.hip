@@ -354,14 +354,34 @@ __global__ __launch_bounds__( kProlongChunk ) void kP2ProlongChunks( const uint3
    const int32_t bx = 2 * ( x >> 2 ), by = 2 * ( y >> 2 ), bz = 2 * ( z >> 2 );
    const T*      cc = coarse + c * scc;
    A             v  = A( 0 );
-   for ( int e = 0; e < cPN[cls]; ++e )
+   // the (at most kPMax) coarse values gathered first, then summed in entry
+   // order; coarse row starts from the lane's three slice starts (the entry
+   // offsets are uniform over the wavefront)
+   // tet(m) = m (m + 1) (m + 2) / 6 in 32-bit unsigned arithmetic (m <= 1024 here)
+   auto          tet32 = []( int32_t m ) { return int32_t( uint32_t( m ) * uint32_t( m + 1 ) * uint32_t( m + 2 ) / 6u ); };
+   const int     n     = cPN[cls];
+   const int32_t tw    = tet32( wc );
+   const int32_t s0 = tw - tet32( wc - bz ), s1 = tw - tet32( wc - bz - 1 ), s2 = tw - tet32( wc - bz - 2 );
+   T             cv[kPMax];
+   bool          ok[kPMax];
+   uint32_t      offs[kPMax]; // entries past n are unused (zero offsets)
+#pragma unroll
+   for ( int e = 0; e < kPMax; ++e )
+      offs[e] = cPOffP[cls][e];
+#pragma unroll
+   for ( int e = 0; e < kPMax; ++e )
    {
-      const uint32_t o  = cPOffP[cls][e];
-      const int32_t  cx = bx + int32_t( o & 255u ), cy = by + int32_t( ( o >> 8 ) & 255u ), cz = bz + int32_t( o >> 16 );
-      if ( cx + cy + cz > wc - 1 )
-         continue;
-      v += A( cPW[cls][e] ) * toAcc( cc[tetRowStart32( wc, cy, cz ) + cx] );
+      const uint32_t o  = offs[e];
+      const int32_t  oz = int32_t( o >> 16 );
+      const int32_t  cx = bx + int32_t( o & 255u ), cy = by + int32_t( ( o >> 8 ) & 255u ), cz = bz + oz;
+      const int32_t  row = ( oz == 0 ? s0 : ( oz == 1 ? s1 : s2 ) ) + cy * ( wc - cz ) - cy * ( cy - 1 ) / 2;
+      ok[e]              = e < n && cx + cy + cz <= wc - 1;
+      cv[e]              = ok[e] ? cc[row + cx] : T( 0 );
    }
+#pragma unroll
+   for ( int e = 0; e < kPMax; ++e )
+      if ( ok[e] ) // false past n
+         v += A( cPW[cls][e] ) * toAcc( cv[e] );
    T* out = fine + c * scf + tetRowStart32( wf, y, z ) + x;
    if ( add )
       v += toAcc( *out );
