@@ -13,6 +13,7 @@
 // (M_q^-T grad phi_i) . (sum_j x_j M_q^-T grad phi_j),  M_q = DF(x_q) J, which is
 // the element matrix of the reference applied to the element vector without
 // forming the 10x10 matrix.
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <type_traits>
@@ -855,7 +856,8 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
 
 // The last tile layer of a macro-cell: one wavefront per tile, one lane per
 // micro-cell inside the macro-cell (cDiagPairs), the type varying over the lanes;
-// same element action, box staging and write-back as the tile kernels.
+// same element action as the tile kernels, box staging, accumulation and
+// write-back over the 165 box points inside the macro-cell (cDiagBox).
 template < class T, class K, bool STORED >
 __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom,
                                                           const P2CellGeom* __restrict__ geom64,
@@ -872,26 +874,25 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
    int64_t         lb;
    const TileCoord tc   = tiles ? tileFromList( tiles, ld, mm, lb ) : tileOfDiagonal( ld, mm, lb );
    const int64_t   base = tc.c * sc1;
-   const int32_t   top  = int32_t( 2 * mm );
    const T*        srcC = src + base;
-   // the lane's 12 box points: all loads issued before the first LDS write
-   constexpr int kPer = ( kBox3 + 63 ) / 64;
+   // the lane's (at most three) of the 165 box points inside the macro-cell:
+   // all loads issued before the first LDS write
+   constexpr int kPer = kDiagBoxP / 64;
+   int           bp[kPer];
    T             bv[kPer];
 #pragma unroll
    for ( int j = 0; j < kPer; ++j )
    {
-      const int     p  = int( threadIdx.x ) + 64 * j;
+      bp[j]           = cDiagBox[int( threadIdx.x ) + 64 * j];
+      const int     p  = bp[j] < 0 ? 0 : bp[j];
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      bv[j]           = ( p < kBox3 && X + Y + Z <= top ) ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : T( 0 );
+      bv[j]           = bp[j] >= 0 ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : T( 0 );
    }
 #pragma unroll
    for ( int j = 0; j < kPer; ++j )
-   {
-      const int p = int( threadIdx.x ) + 64 * j;
-      if ( p < kBox3 )
-         sb[p] = R( toAcc( bv[j] ) );
-   }
+      if ( bp[j] >= 0 )
+         sb[bp[j]] = R( toAcc( bv[j] ) );
    __syncthreads();
    const P2CellGeomT< R >& g    = geom[tc.c];
    const int32_t           pair = cDiagPairs[threadIdx.x];
@@ -904,8 +905,10 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
    for ( int i = 0; i < 10; ++i )
       u[i] = th.valid ? sb[pos[i]] : R( 0 );
    __syncthreads();
-   for ( int p = threadIdx.x; p < kBox3; p += 64 )
-      sb[p] = R( 0 );
+#pragma unroll
+   for ( int j = 0; j < kPer; ++j )
+      if ( bp[j] >= 0 )
+         sb[bp[j]] = R( 0 );
    __syncthreads();
    if ( th.valid )
    {
@@ -928,12 +931,15 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
    }
    __syncthreads();
    T* dstC = dst + base;
-   for ( int p = threadIdx.x; p < kBox3; p += 64 )
+#pragma unroll
+   for ( int j = 0; j < kPer; ++j )
    {
+      if ( bp[j] < 0 )
+         continue;
+      const int     p  = bp[j];
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      if ( X + Y + Z <= top )
-         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * sb[p] );
+      sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * sb[p] );
    }
 }
 
@@ -1485,6 +1491,10 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
          if ( pairs.size() != 64 )
             throw Error( -1, "last-layer tile pairs: expected 64" );
          HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cDiagPairs ), pairs.data(), sizeof( int32_t ) * 64 ) );
+         const std::vector< int32_t > boxp = diagonalBoxPoints();
+         if ( std::count_if( boxp.begin(), boxp.end(), []( int32_t v ) { return v >= 0; } ) != kDiagBox )
+            throw Error( -1, "last-layer box points: expected 165" );
+         HG_CHECK_HIP( hipMemcpyToSymbol( HIP_SYMBOL( cDiagBox ), boxp.data(), sizeof( int32_t ) * kDiagBoxP ) );
       } );
       Storage&        st = ctx->storage( s );
       const Topology& t  = st.topo;

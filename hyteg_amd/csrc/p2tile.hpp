@@ -200,8 +200,14 @@ __device__ __forceinline__ TileCoord tileFromList( const uint64_t* list, int64_t
 // the 64 (type, anchor) pairs of a last-layer tile inside the macro-cell, for
 // lattices whose micro-edge count is a multiple of kTile: anchor sum at most
 // 3 - rowShift(type); entry = type | anchor << 3
+// The box points such a tile's micro-cells touch are those inside the
+// macro-cell: lx + ly + lz <= 2 (kTile - 1) + 2 = 8, tet(9) = 165 of the 729
+// (cDiagBox, box indices lx + 9 ly + 81 lz, padded to 192 with -1).
+constexpr int kDiagBox  = 165;
+constexpr int kDiagBoxP = 192;
 namespace {
 __constant__ int32_t cDiagPairs[64];
+__constant__ int32_t cDiagBox[kDiagBoxP];
 } // namespace
 inline std::vector< int32_t > diagonalPairs()
 {
@@ -210,6 +216,17 @@ inline std::vector< int32_t > diagonalPairs()
       for ( int a = 0; a < 64; ++a )
          if ( ( a & 3 ) + ( ( a >> 2 ) & 3 ) + ( a >> 4 ) <= kTile - 1 - MicroCells::rowShift[t] )
             out.push_back( t | ( a << 3 ) );
+   return out;
+}
+inline std::vector< int32_t > diagonalBoxPoints()
+{
+   std::vector< int32_t > out;
+   for ( int lz = 0; lz < kBox; ++lz )
+      for ( int ly = 0; ly < kBox; ++ly )
+         for ( int lx = 0; lx < kBox; ++lx )
+            if ( lx + ly + lz <= 2 * ( kTile - 1 ) + 2 )
+               out.push_back( lx + kBox * ( ly + kBox * lz ) );
+   out.resize( kDiagBoxP, -1 );
    return out;
 }
 
