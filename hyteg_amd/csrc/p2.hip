@@ -730,17 +730,20 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
 {
    using R  = typename Acc< T >::type;
    using QC = QuadConst< R >;
-   // Element results are summed with LDS atomics on the source box in FP64, and
-   // in per-wavefront result boxes (waveBoxAccumulate) in FP32, where the LDS
-   // atomics cost about half the kernel time (in FP64 the boxes are not faster).
-   // LDS: the tile's source box and the staged coefficients; with result boxes,
-   // the same space holds them once every wavefront has integrated.
+   // Element results are summed with LDS atomics into a result box of their own
+   // in FP64 (cleared while the loads are in flight, so a tile needs two
+   // workgroup barriers), and in per-wavefront result boxes (waveBoxAccumulate)
+   // in FP32, where the LDS atomics cost about half the kernel time (in FP64 the
+   // per-wavefront boxes are not faster).  LDS: the tile's source box and the
+   // staged coefficients (+ the FP64 result box; the FP32 result boxes reuse the
+   // staging space once every wavefront has integrated).
    constexpr bool kAtomic = std::is_same< R, double >::value;
    constexpr int  kStage  = kBox3 + ( STORED ? 11 * kTileThreads : 0 );
-   constexpr int  kLds    = ( kAtomic || kStage > kWaveBoxes * kBox3 ) ? kStage : kWaveBoxes * kBox3;
+   constexpr int  kLds    = kAtomic ? kStage + kBox3 : ( kStage > kWaveBoxes * kBox3 ? kStage : kWaveBoxes * kBox3 );
    __shared__ R   lds[kLds];
    R* const       sb = lds;
    R* const       sk = lds + kBox3;
+   R* const       sa = lds + kStage; // FP64: the result box, apart from the source box
    const int64_t li = xcdBlock( ntiles );
    if ( li >= ntiles )
       return;
@@ -773,6 +776,17 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
       bin[j]          = p < kBox3 && X + Y + Z <= top;
       bv[j]           = bin[j] ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : T( 0 );
    }
+   if constexpr ( kAtomic )
+   {
+      // the result box is cleared while the loads are in flight
+#pragma unroll
+      for ( int j = 0; j < 2; ++j )
+      {
+         const int p = int( threadIdx.x ) + j * kTileThreads;
+         if ( p < kBox3 )
+            sa[p] = R( 0 );
+      }
+   }
    if constexpr ( STORED )
    {
 #pragma unroll
@@ -796,13 +810,6 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
 #pragma unroll
       for ( int i = 0; i < 10; ++i )
          u[i] = th.valid ? sb[pos[i]] : R( 0 );
-   }
-   if constexpr ( kAtomic )
-   {
-      __syncthreads();
-      for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
-         sb[p] = R( 0 );
-      __syncthreads();
    }
    R y[10];
    if ( th.valid )
@@ -830,7 +837,7 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
          {
 #pragma unroll
             for ( int i = 0; i < 10; ++i )
-               atomicAdd( &sb[pos[i]], y[i] );
+               atomicAdd( &sa[pos[i]], y[i] );
          }
       }
       else
@@ -850,7 +857,7 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
       if ( X + Y + Z <= top )
-         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * ( kAtomic ? sb[p] : waveBoxSum( lds, p ) ) );
+         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * ( kAtomic ? sa[p] : waveBoxSum( lds, p ) ) );
    }
 }
 
