@@ -175,9 +175,13 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
    using QC = SQ< R >;
    constexpr bool DO_A = MASK & 1, DO_BT = MASK & 2, DO_B = MASK & 4, DO_M = MASK & 8;
    constexpr bool NEED_U = DO_A || DO_B, OUT_U = DO_A || DO_BT, NEED_P = DO_BT || DO_M, OUT_P = DO_B || DO_M;
-   // LDS boxes: first the tile's input values, then the accumulators
+   // LDS boxes of the tile's input values and, apart from them, the result
+   // boxes (cleared while the input loads are in flight: two workgroup barriers
+   // per tile)
    __shared__ R su[3][kBox3];
    __shared__ R sp[kPBox3];
+   __shared__ R au[OUT_U ? 3 : 1][OUT_U ? kBox3 : 1];
+   __shared__ R ap[OUT_P ? kPBox3 : 1];
    const int64_t lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
       return;
@@ -213,6 +217,21 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
       const int32_t X = int32_t( tc.x0 ) + lx, Y = int32_t( tc.y0 ) + ly, Z = int32_t( tc.z0 ) + lz;
       pok             = p < kPBox3 && X + Y + Z <= top0;
       pvl             = pok ? pin[base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X] : T( 0 );
+   }
+   if constexpr ( OUT_U )
+   {
+#pragma unroll
+      for ( int j = 0; j < 2; ++j )
+      {
+         const int p = int( threadIdx.x ) + j * kTileThreads;
+         if ( p < kBox3 )
+            au[0][p] = au[1][p] = au[2][p] = R( 0 );
+      }
+   }
+   if constexpr ( OUT_P )
+   {
+      if ( int( threadIdx.x ) < kPBox3 )
+         ap[threadIdx.x] = R( 0 );
    }
    if constexpr ( NEED_U )
    {
@@ -256,14 +275,6 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
       for ( int k = 0; k < 4; ++k )
          pv[k] = th.valid ? sp[ppos[k]] : R( 0 );
    }
-   __syncthreads();
-   if constexpr ( OUT_U )
-      for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
-         su[0][p] = su[1][p] = su[2][p] = R( 0 );
-   if constexpr ( OUT_P )
-      for ( int p = threadIdx.x; p < kPBox3; p += kTileThreads )
-         sp[p] = R( 0 );
-   __syncthreads();
    if ( th.valid )
    {
       const R* Dc   = g.Dc[th.t];
@@ -381,13 +392,13 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
          for ( int c = 0; c < 3; ++c )
 #pragma unroll
             for ( int i = 0; i < 10; ++i )
-               atomicAdd( &su[c][pos[i]], y[c][i] );
+               atomicAdd( &au[c][pos[i]], y[c][i] );
       }
       if constexpr ( OUT_P )
       {
 #pragma unroll
          for ( int k = 0; k < 4; ++k )
-            atomicAdd( &sp[ppos[k]], yq[k] );
+            atomicAdd( &ap[ppos[k]], yq[k] );
       }
    }
    __syncthreads();
@@ -405,7 +416,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
          const int64_t idx = base1 + tetRowStart32( int32_t( w1 ), Y, Z ) + X;
 #pragma unroll
          for ( int c = 0; c < 3; ++c )
-            sharedAdd( yout[c] + idx, su[c][p] );
+            sharedAdd( yout[c] + idx, au[c][p] );
       }
    }
    if constexpr ( OUT_P )
@@ -415,7 +426,7 @@ __global__ __launch_bounds__( kTileThreads ) void kStokesTileApply(
          const int32_t X = int32_t( tc.x0 ) + lx, Y = int32_t( tc.y0 ) + ly, Z = int32_t( tc.z0 ) + lz;
          if ( X + Y + Z > top0 )
             continue;
-         sharedAdd( yp + base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X, sp[p] );
+         sharedAdd( yp + base0 + tetRowStart32( int32_t( w0 ), Y, Z ) + X, ap[p] );
       }
 }
 
