@@ -160,6 +160,7 @@ struct P2Operator : OperatorBase
    bool                                   reduced     = true;
    int                                    reducedFrom = 2;
    std::map< int, DevBuf< __half > >      kc;       // k per micro-cell (tile-major), by P2 level
+   std::map< int, DevBuf< float > >       gm;       // reduced-form metric per micro-cell, by P2 level
    std::map< int, DevBuf< float > >       accum;    // FP32 accumulator of the FP16 applies, by P2 level
 };
 
@@ -404,16 +405,119 @@ __device__ __forceinline__ void p2ReducedAction( const float G[6], const float u
 // (or added to, rmw), box-surface points (a lattice coordinate divisible by 8,
 // shared with neighbouring tiles) are added with FP32 atomics into `accum`,
 // which kP2FinishAccum rounds into the FP16 destination afterwards.
-template < class T >
+// blended positions of a tile's 5x5x5 micro-vertices relative to its first one
+// (FP64), staged in LDS by the first 125 threads
+__device__ __forceinline__ void stageBlendedVertices( const P2CellGeom& g, const TileCoord& tc, double ( *sv )[3] )
+{
+   constexpr int nv = ( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 );
+   if ( threadIdx.x >= nv )
+      return;
+   auto blended = [&]( int i, int j, int k, double F[3] ) {
+      const double vx = double( tc.x0 + i ), vy = double( tc.y0 + j ), vz = double( tc.z0 + k );
+      double       x[3];
+#pragma unroll
+      for ( int d = 0; d < 3; ++d )
+         x[d] = g.v0[d] + g.hE[3 * d] * vx + g.hE[3 * d + 1] * vy + g.hE[3 * d + 2] * vz;
+      double q = 1.0;
+      if ( g.kind )
+      {
+         const double ls = g.nl[0] * x[0] + g.nl[1] * x[1] + g.nl[2] * x[2] - g.lam0;
+         q               = ( g.r0 + fabs( ls ) * g.dr ) / sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
+      }
+#pragma unroll
+      for ( int d = 0; d < 3; ++d )
+         F[d] = q * x[d];
+   };
+   const int i = threadIdx.x % ( kTile + 1 ), j = ( threadIdx.x / ( kTile + 1 ) ) % ( kTile + 1 ),
+             k = threadIdx.x / ( ( kTile + 1 ) * ( kTile + 1 ) );
+   double F[3], F0[3];
+   blended( i, j, k, F );
+   blended( 0, 0, 0, F0 );
+#pragma unroll
+   for ( int d = 0; d < 3; ++d )
+      sv[threadIdx.x][d] = F[d] - F0[d];
+}
+
+// the reduced-form metric of micro-cell (type t, anchor ax, ay, az) from the
+// staged vertices: edge vectors c_k = P_k - P_0 of the blended vertices, n_k
+// their cross products (det J * rows of J^-1), G = (k w / |det J|) n_i . n_j,
+// formed in FP64 (MI355X issues FP64 and non-packed FP32 at one rate) and
+// rounded once
+__device__ __forceinline__ void reducedMetric( const double ( *sv )[3], int t, int ax, int ay, int az, float ke,
+                                               float G[6] )
+{
+   double P[4][3];
+#pragma unroll
+   for ( int k = 0; k < 4; ++k )
+   {
+      const int vi = ( ax + cVof[t][k][0] ) + ( kTile + 1 ) * ( ( ay + cVof[t][k][1] ) + ( kTile + 1 ) * ( az + cVof[t][k][2] ) );
+#pragma unroll
+      for ( int d = 0; d < 3; ++d )
+         P[k][d] = sv[vi][d];
+   }
+   double c[3][3], n[3][3];
+#pragma unroll
+   for ( int k = 0; k < 3; ++k )
+#pragma unroll
+      for ( int d = 0; d < 3; ++d )
+         c[k][d] = P[k + 1][d] - P[0][d];
+#pragma unroll
+   for ( int k = 0; k < 3; ++k )
+   {
+      const double* b1 = c[( k + 1 ) % 3];
+      const double* b2 = c[( k + 2 ) % 3];
+      n[k][0]          = b1[1] * b2[2] - b1[2] * b2[1];
+      n[k][1]          = b1[2] * b2[0] - b1[0] * b2[2];
+      n[k][2]          = b1[0] * b2[1] - b1[1] * b2[0];
+   }
+   const double det = c[0][0] * n[0][0] + c[0][1] * n[0][1] + c[0][2] * n[0][2];
+   const double s   = ( double( ke ) * ( 1.0 / 24.0 ) ) / fabs( det );
+   G[0]             = float( s * ( n[0][0] * n[0][0] + n[0][1] * n[0][1] + n[0][2] * n[0][2] ) );
+   G[1]             = float( s * ( n[0][0] * n[1][0] + n[0][1] * n[1][1] + n[0][2] * n[1][2] ) );
+   G[2]             = float( s * ( n[0][0] * n[2][0] + n[0][1] * n[2][1] + n[0][2] * n[2][2] ) );
+   G[3]             = float( s * ( n[1][0] * n[1][0] + n[1][1] * n[1][1] + n[1][2] * n[1][2] ) );
+   G[4]             = float( s * ( n[1][0] * n[2][0] + n[1][1] * n[2][1] + n[1][2] * n[2][2] ) );
+   G[5]             = float( s * ( n[2][0] * n[2][0] + n[2][1] * n[2][1] + n[2][2] * n[2][2] ) );
+}
+
+// the reduced-form metric of every micro-cell of a level, stored once
+// (tile-major [tile][6][384], FP32): the reduced applies then read 24 bytes per
+// micro-cell instead of forming the blended metric in every apply
+__global__ __launch_bounds__( kTileThreads ) void kP2TileReducedMetric( const P2CellGeom* __restrict__ geom,
+                                                                        const __half* __restrict__ kc, int64_t mm,
+                                                                        int64_t ntiles,
+                                                                        const uint64_t* __restrict__ tiles,
+                                                                        float* __restrict__ gm )
+{
+   __shared__ double sv[( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 )][3];
+   const int64_t    lb = xcdBlock( ntiles );
+   if ( lb >= ntiles )
+      return;
+   int64_t         full;
+   const TileCoord tc = tiles ? tileFromList( tiles, lb, mm, full ) : tileOf( lb, mm );
+   const float     ke = __half2float( kc[lb * kTileThreads + threadIdx.x] );
+   stageBlendedVertices( geom[tc.c], tc, sv );
+   __syncthreads();
+   const int t = __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
+   const int a = int( threadIdx.x ) & 63;
+   float     G[6];
+   reducedMetric( sv, t, a & 3, ( a >> 2 ) & 3, a >> 4, ke, G );
+#pragma unroll
+   for ( int i = 0; i < 6; ++i )
+      gm[( lb * 6 + i ) * kTileThreads + threadIdx.x] = G[i];
+}
+
+template < class T, bool STORED_G >
 __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2CellGeom* __restrict__ geom,
-                                                                      const __half* __restrict__ kc, int64_t mm,
+                                                                      const __half* __restrict__ kc,
+                                                                      const float* __restrict__ gm, int64_t mm,
                                                                       int64_t ntiles, int64_t sc1, int64_t w1,
                                                                       const T* __restrict__ src, T* __restrict__ dst,
                                                                       float* __restrict__ accum, int rmw, float alpha,
                                                                       const uint64_t* __restrict__ tiles )
 {
    __shared__ float  sb[kBox3];
-   __shared__ double sv[( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 )][3];
+   __shared__ double sv[STORED_G ? 1 : ( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 )][3];
    __shared__ float  sy[kWaveBoxes * kBox3];
    const int64_t    lb = xcdBlock( ntiles );
    if ( lb >= ntiles )
@@ -422,8 +526,17 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
    const TileCoord tc   = tiles ? tileFromList( tiles, lb, mm, full ) : tileOf( lb, mm );
    const int64_t   base = tc.c * sc1;
    const int32_t   top  = int32_t( 2 * mm );
-   const float     ke   = __half2float( kc[lb * kTileThreads + threadIdx.x] );
-   const T*        srcC = src + base;
+   float           G[6];
+   float           ke = 0.0f;
+   if constexpr ( STORED_G )
+   {
+#pragma unroll
+      for ( int i = 0; i < 6; ++i )
+         G[i] = gm[( lb * 6 + i ) * kTileThreads + threadIdx.x];
+   }
+   else
+      ke = __half2float( kc[lb * kTileThreads + threadIdx.x] );
+   const T* srcC = src + base;
    {
       // the thread's (at most two) box points, both loads in flight before the LDS writes
       T bv[2];
@@ -443,42 +556,14 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
             sb[p] = float( toAcc( bv[j] ) );
       }
    }
-   const P2CellGeom& g  = geom[tc.c];
-   constexpr int     nv = ( kTile + 1 ) * ( kTile + 1 ) * ( kTile + 1 );
-   if ( threadIdx.x < nv )
-   {
-      // blended vertex positions relative to the tile's first vertex (FP64)
-      auto blended = [&]( int i, int j, int k, double F[3] ) {
-         const double vx = double( tc.x0 + i ), vy = double( tc.y0 + j ), vz = double( tc.z0 + k );
-         double       x[3];
-#pragma unroll
-         for ( int d = 0; d < 3; ++d )
-            x[d] = g.v0[d] + g.hE[3 * d] * vx + g.hE[3 * d + 1] * vy + g.hE[3 * d + 2] * vz;
-         double q = 1.0;
-         if ( g.kind )
-         {
-            const double ls = g.nl[0] * x[0] + g.nl[1] * x[1] + g.nl[2] * x[2] - g.lam0;
-            q               = ( g.r0 + fabs( ls ) * g.dr ) / sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
-         }
-#pragma unroll
-         for ( int d = 0; d < 3; ++d )
-            F[d] = q * x[d];
-      };
-      const int i = threadIdx.x % ( kTile + 1 ), j = ( threadIdx.x / ( kTile + 1 ) ) % ( kTile + 1 ),
-                k = threadIdx.x / ( ( kTile + 1 ) * ( kTile + 1 ) );
-      double F[3], F0[3];
-      blended( i, j, k, F );
-      blended( 0, 0, 0, F0 );
-#pragma unroll
-      for ( int d = 0; d < 3; ++d )
-         sv[threadIdx.x][d] = F[d] - F0[d];
-   }
+   if constexpr ( !STORED_G )
+      stageBlendedVertices( geom[tc.c], tc, sv );
    __syncthreads();
    const int t  = __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
    const int a  = int( threadIdx.x ) & 63;
    const int ax = a & 3, ay = ( a >> 2 ) & 3, az = a >> 4;
    const bool valid = tc.x0 + ax + tc.y0 + ay + tc.z0 + az <= mm - 1 - MicroCells::rowShift[t];
-   float u[10], G[6];
+   float u[10];
    int   pos[10];
 #pragma unroll
    for ( int k = 0; k < 4; ++k )
@@ -489,44 +574,8 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyReduced( const P2C
 #pragma unroll
    for ( int i = 0; i < 10; ++i )
       u[i] = valid ? sb[pos[i]] : 0.0f;
-   {
-      // edge vectors c_k = P_k - P_0 of the blended vertices, n_k their cross
-      // products (det J * rows of J^-1), G = (k w / |det J|) n_i . n_j: formed
-      // in FP64 (MI355X issues FP64 and non-packed FP32 at one rate) and
-      // rounded once
-      double P[4][3];
-#pragma unroll
-      for ( int k = 0; k < 4; ++k )
-      {
-         const int vi = ( ax + cVof[t][k][0] ) + ( kTile + 1 ) * ( ( ay + cVof[t][k][1] ) + ( kTile + 1 ) * ( az + cVof[t][k][2] ) );
-#pragma unroll
-         for ( int d = 0; d < 3; ++d )
-            P[k][d] = sv[vi][d];
-      }
-      double c[3][3], n[3][3];
-#pragma unroll
-      for ( int k = 0; k < 3; ++k )
-#pragma unroll
-         for ( int d = 0; d < 3; ++d )
-            c[k][d] = P[k + 1][d] - P[0][d];
-#pragma unroll
-      for ( int k = 0; k < 3; ++k )
-      {
-         const double* b1 = c[( k + 1 ) % 3];
-         const double* b2 = c[( k + 2 ) % 3];
-         n[k][0]          = b1[1] * b2[2] - b1[2] * b2[1];
-         n[k][1]          = b1[2] * b2[0] - b1[0] * b2[2];
-         n[k][2]          = b1[0] * b2[1] - b1[1] * b2[0];
-      }
-      const double det = c[0][0] * n[0][0] + c[0][1] * n[0][1] + c[0][2] * n[0][2];
-      const double s   = ( double( ke ) * ( 1.0 / 24.0 ) ) / fabs( det );
-      G[0]             = float( s * ( n[0][0] * n[0][0] + n[0][1] * n[0][1] + n[0][2] * n[0][2] ) );
-      G[1]             = float( s * ( n[0][0] * n[1][0] + n[0][1] * n[1][1] + n[0][2] * n[1][2] ) );
-      G[2]             = float( s * ( n[0][0] * n[2][0] + n[0][1] * n[2][1] + n[0][2] * n[2][2] ) );
-      G[3]             = float( s * ( n[1][0] * n[1][0] + n[1][1] * n[1][1] + n[1][2] * n[1][2] ) );
-      G[4]             = float( s * ( n[1][0] * n[2][0] + n[1][1] * n[2][1] + n[1][2] * n[2][2] ) );
-      G[5]             = float( s * ( n[2][0] * n[2][0] + n[2][1] * n[2][1] + n[2][2] * n[2][2] ) );
-   }
+   if constexpr ( !STORED_G )
+      reducedMetric( sv, t, ax, ay, az, ke, G );
    {
       float y[10];
 #pragma unroll
@@ -1147,6 +1196,29 @@ static const __half* p2CentroidK( Context* ctx, P2Operator& op, int L )
    return buf.p;
 }
 
+// The reduced-form metric table of level L (6 FP32 values per micro-cell slot,
+// 14.4 GB for the level-6 shell), built on first use when it stays within
+// kReducedMetricMaxBytes; otherwise (null) the applies form the metric in the
+// kernel from the blended vertices, with the same values.
+constexpr int64_t kReducedMetricMaxBytes = int64_t( 32 ) << 30;
+static const float* p2ReducedMetric( Context* ctx, P2Operator& op, Lattice& lat, int L, const __half* kc )
+{
+   auto it = op.gm.find( L );
+   if ( it != op.gm.end() )
+      return it->second.p;
+   const int64_t mm     = int64_t( 1 ) << L;
+   const int64_t ntiles = tilesPerCell( mm ) * int64_t( op.st->topo.nc );
+   const int64_t n      = ntiles * 6 * kTileThreads;
+   if ( n * int64_t( sizeof( float ) ) > kReducedMetricMaxBytes )
+      return nullptr;
+   auto& buf = op.gm[L];
+   buf.alloc( ctx, size_t( n ) );
+   kP2TileReducedMetric<<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+       op.tgeom.at( L ).p, kc, mm, ntiles, tileList( ctx, lat, mm, kTilesAll ), buf.p );
+   HG_CHECK_HIP( hipGetLastError() );
+   return buf.p;
+}
+
 // the reduced-form apply on a tile level (FP32 arithmetic): the cell-lattice
 // part; the additive push to the owners follows in p2Apply
 template < class T >
@@ -1176,9 +1248,14 @@ static void p2ApplyReduced( Context* ctx, P2Operator& op, Lattice& lat, int L, c
       surfacePointList( ctx, lat );
    }
    const uint64_t* tiles = tileList( ctx, lat, mm, kTilesAll );
+   const float*    gm    = p2ReducedMetric( ctx, op, lat, L, kc );
    ctx->beginTiming();
-   kP2TileApplyReduced< T ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
-       op.tgeom.at( L ).p, kc, mm, ntiles, lat.sc, lat.w, s, d, accum, rmw, float( alpha ), tiles );
+   if ( gm )
+      kP2TileApplyReduced< T, true ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+          op.tgeom.at( L ).p, kc, gm, mm, ntiles, lat.sc, lat.w, s, d, accum, rmw, float( alpha ), tiles );
+   else
+      kP2TileApplyReduced< T, false ><<< xcdGrid( ntiles ), kTileThreads, 0, ctx->stream >>>(
+          op.tgeom.at( L ).p, kc, nullptr, mm, ntiles, lat.sc, lat.w, s, d, accum, rmw, float( alpha ), tiles );
    HG_CHECK_HIP( hipGetLastError() );
    ctx->endTiming();
    if constexpr ( std::is_same< T, __half >::value )
