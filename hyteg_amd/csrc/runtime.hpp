@@ -313,6 +313,9 @@ inline void DevBuf< T >::upload( Context* c, const std::vector< T >& h )
    alloc( c, h.size() );
    if ( !h.empty() )
    {
+      // a copy from pageable memory and the wait for it must not run while
+      // another thread captures (work lists are built lazily by serving workers)
+      DeviceGateLock gate;
       HG_CHECK_HIP( hipMemcpyAsync( p, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice, c->stream ) );
       HG_CHECK_HIP( hipStreamSynchronize( c->stream ) );
    }

@@ -422,7 +422,8 @@ int hg_p1_batched_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b,
  * shell with no-slip surfaces and the buoyancy of 12 Gaussian blobs, levels 2-4;
  * k, blob centres and the RHS generated from seed[i])
  * at level[i] arrives arrival_s[i] seconds after the call starts and is solved to
- * relative residual `tol` (Stokes: at most 150 Uzawa V-cycles).  P1 requests of
+ * relative residual `tol` (Stokes: at most 150 Uzawa V-cycles; its residual floors
+ * near 1e-7 to 5e-7, so a tol below 1e-6 runs to that cap).  P1 requests of
  * one (kind, level) share an interleaved hierarchy of `slots` vectors (a power of
  * two up to 64) and join / leave it at V-cycle boundaries; every class has its
  * own stream; a class hierarchy is built only while the device memory of the
