@@ -497,7 +497,7 @@ def main():
                          "p1, p2const, gmg, cfg1, cfg4, stokes, serving, cpu)")
     ap.add_argument("--stokes-cycles", type=int, default=120, help="V-cycle cap of the Stokes leg")
     ap.add_argument("--serving-requests", type=int, default=2000)
-    ap.add_argument("--serving-rates", default="200,500")
+    ap.add_argument("--serving-rates", default="10,50,200,500")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard by "
@@ -549,7 +549,10 @@ def main():
     if p2 and "stokes" in legs:
         extra["stokes_shell_L5"] = stokes_cfg3(hg, ctx, cycles=args.stokes_cycles)
     if p2 and "serving" in legs:
-        extra["serving_mixed_stream"] = [serving_cfg5(args.serving_requests, float(r))
+        # below 200/s a stream of 30 s of arrivals (the device keeps up there, so
+        # the latencies are service times, not queueing)
+        extra["serving_mixed_stream"] = [serving_cfg5(min(args.serving_requests, max(100, int(30 * float(r)))),
+                                                      float(r))
                                          for r in args.serving_rates.split(",")]
     cpu = None
     if "cpu" in legs:
