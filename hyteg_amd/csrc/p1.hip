@@ -141,6 +141,77 @@ static int hostPointClass( int64_t mm, int64_t x, int64_t y, int64_t z )
    return -1;
 }
 
+// Sliced ELL rows: slot i (a row, or -1 for padding) lies in slice i / 64 at
+// lane i % 64; the slice's entries are column-major (entry j of the lane at
+// off[slice] + 64 j + lane) and its width is that of its longest row rounded up
+// to kSellUnroll, so the 64 lanes of a wavefront read each entry column with one
+// coalesced load and run the same number of iterations.  Padding entries point
+// at the row's own slot with weight 0.
+constexpr int kSellRows   = 64;
+constexpr int kSellUnroll = 8;
+
+struct SellRows
+{
+   DevBuf< int64_t > off; // nslot / 64 + 1 entry offsets
+   DevBuf< int32_t > row; // owned boundary DoF of each slot, -1: padding
+   DevBuf< int32_t > pos; // storage positions
+   DevBuf< double >  val;
+   int64_t           nslot = 0;
+};
+
+// slots: the row of each slot (a multiple of 64 of them after padding);
+// withDiag puts the row's own slot with weight 1 / inv[r] first
+static void buildSell( Context* ctx, SellRows& out, std::vector< int32_t > slots,
+                       const std::vector< std::vector< std::pair< int64_t, double > > >& rows,
+                       const std::vector< double >& inv, int64_t ownerBase, bool withDiag )
+{
+   slots.resize( ( slots.size() + kSellRows - 1 ) / kSellRows * kSellRows, -1 );
+   const size_t           nsl = slots.size() / kSellRows;
+   std::vector< int64_t > off( nsl + 1, 0 );
+   for ( size_t s = 0; s < nsl; ++s )
+   {
+      size_t w = 1;
+      for ( int l = 0; l < kSellRows; ++l )
+      {
+         const int32_t r = slots[s * kSellRows + l];
+         if ( r >= 0 )
+            w = std::max( w, rows[size_t( r )].size() + ( withDiag ? 1 : 0 ) );
+      }
+      w          = ( w + kSellUnroll - 1 ) / kSellUnroll * kSellUnroll;
+      off[s + 1] = off[s] + int64_t( w ) * kSellRows;
+   }
+   std::vector< int32_t > pos( size_t( off[nsl] ), 0 );
+   std::vector< double >  val( size_t( off[nsl] ), 0.0 );
+   for ( size_t s = 0; s < nsl; ++s )
+   {
+      const int64_t w = ( off[s + 1] - off[s] ) / kSellRows;
+      for ( int l = 0; l < kSellRows; ++l )
+      {
+         const int32_t r = slots[s * kSellRows + l];
+         if ( r < 0 )
+            continue;
+         const int64_t own = ownerBase + r;
+         int64_t       j   = 0;
+         auto          put = [&]( int64_t p, double v ) {
+            pos[size_t( off[s] + j * kSellRows + l )] = int32_t( p );
+            val[size_t( off[s] + j * kSellRows + l )] = v;
+            ++j;
+         };
+         if ( withDiag )
+            put( own, 1.0 / inv[size_t( r )] );
+         for ( const auto& e : rows[size_t( r )] )
+            put( e.first, e.second );
+         while ( j < w )
+            put( own, 0.0 );
+      }
+   }
+   out.nslot = int64_t( slots.size() );
+   out.off.upload( ctx, off );
+   out.row.upload( ctx, slots );
+   out.pos.upload( ctx, pos );
+   out.val.upload( ctx, val );
+}
+
 struct P1OpLevel
 {
    std::vector< double >  hStencil; // nc*15, cell interiors
@@ -149,20 +220,24 @@ struct P1OpLevel
    DevBuf< double >       invDiag;    // per owned vertex/edge/face DoF
    std::vector< int64_t > colorStart; // owned boundary DoFs grouped by colour
    DevBuf< int32_t >      colorRows;
-   // assembled off-diagonal rows of the owned boundary DoFs (built when they fit,
-   // see kCsrRowBudget): storage positions of the neighbours -- the owner slot of
-   // boundary neighbours, the cell-lattice slot of cell-interior ones -- and weights
-   bool              csr = false;
-   DevBuf< int64_t > csrPtr, csrPos;
-   DevBuf< double >  csrVal;
+   // assembled rows of the owned boundary DoFs (built when they fit, see
+   // kCsrRowBudget), as sliced ELL: storage positions of the neighbours -- the
+   // owner slot of boundary neighbours, the cell-lattice slot of cell-interior
+   // ones -- and weights.  applyRows holds every row in DoF order with its
+   // diagonal first; sorRows holds the off-diagonal part in colour order, each
+   // colour starting on a slice (sorColorStart: slot offsets).
+   bool                   csr = false;
+   SellRows               applyRows, sorRows;
+   std::vector< int64_t > sorColorStart;
    // inverse diagonal of every storage position (cell lattices, halo copies
    // included, then the owned DoFs), built on first use
    DevBuf< double > invDiagFull;
 };
 
-// boundary rows are assembled into CSR up to this many owned boundary DoFs
-// (about 40 entries of 16 bytes each: below 2.7 GB); beyond it the kernels sum
-// the per-cell stencil shares in place
+// boundary rows are assembled up to this many owned boundary DoFs (two copies
+// of about 40 entries of 12 bytes each: below 4.1 GB) when every storage
+// position fits 31 bits; beyond it the kernels sum the per-cell stencil shares
+// in place
 static constexpr int64_t kCsrRowBudget = int64_t( 1 ) << 22;
 
 struct P1Operator : OperatorBase
@@ -227,7 +302,7 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
    // values do not depend on the thread count).
    std::vector< double >                 diag( lat.nb, 0.0 );
    std::vector< std::vector< int32_t > > adj( lat.nb );
-   o.csr = lat.nb > 0 && lat.nb <= kCsrRowBudget;
+   o.csr = lat.nb > 0 && lat.nb <= kCsrRowBudget && lat.nc * lat.sc + lat.nb <= int64_t( INT32_MAX );
    std::vector< std::vector< std::pair< int64_t, double > > > row( o.csr ? lat.nb : 0 );
    const int64_t                                             ownerBase = lat.nc * lat.sc;
    struct Contribution
@@ -319,24 +394,8 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
          p.join();
    }
    rec.clear();
-   if ( o.csr )
-   {
-      std::vector< int64_t > ptr( lat.nb + 1, 0 ), pos;
-      std::vector< double >  val;
-      for ( int64_t r = 0; r < lat.nb; ++r )
-      {
-         std::sort( row[r].begin(), row[r].end() );
-         for ( const auto& e : row[r] )
-         {
-            pos.push_back( e.first );
-            val.push_back( e.second );
-         }
-         ptr[r + 1] = int64_t( pos.size() );
-      }
-      o.csrPtr.upload( ctx, ptr );
-      o.csrPos.upload( ctx, pos );
-      o.csrVal.upload( ctx, val );
-   }
+   for ( auto& rr : row )
+      std::sort( rr.begin(), rr.end() );
    std::vector< double > inv( lat.nb );
    for ( int64_t r = 0; r < lat.nb; ++r )
       inv[r] = 1.0 / diag[r];
@@ -369,6 +428,22 @@ static void buildP1Level( Context* ctx, Storage& st, int L, P1OpLevel& o )
    }
    o.colorStart[ncolor] = int64_t( rows.size() );
    o.colorRows.upload( ctx, rows );
+   if ( o.csr )
+   {
+      std::vector< int32_t > natural( size_t( lat.nb ) ), sorOrder;
+      for ( int64_t r = 0; r < lat.nb; ++r )
+         natural[size_t( r )] = int32_t( r );
+      o.sorColorStart.assign( size_t( ncolor ) + 1, 0 );
+      for ( int cc = 0; cc < ncolor; ++cc )
+      {
+         o.sorColorStart[size_t( cc )] = int64_t( sorOrder.size() );
+         sorOrder.insert( sorOrder.end(), byColor[cc].begin(), byColor[cc].end() );
+         sorOrder.resize( ( sorOrder.size() + kSellRows - 1 ) / kSellRows * kSellRows, -1 );
+      }
+      o.sorColorStart[size_t( ncolor )] = int64_t( sorOrder.size() );
+      buildSell( ctx, o.applyRows, natural, row, inv, ownerBase, true );
+      buildSell( ctx, o.sorRows, sorOrder, row, inv, ownerBase, false );
+   }
 }
 
 // ------------------------------------------------------------------------
@@ -601,11 +676,17 @@ static const uint64_t* p1BrickList( Context* ctx, Lattice& l )
    return l.p1Bricks.p;
 }
 
-// SOR on the cell-interior points of one parity colour, in place
+// SOR on the cell-interior points of the two parity colours (cx, cy, cz) and
+// (1 - cx, cy, cz) of one row group, in place: colour cxFirst first, then the
+// other.  The only stencil neighbours of a point that lie in rows of the same
+// group are x - 1 and x + 1 of its own row, so the second colour depends on the
+// first through the row alone; a row's lanes are in one wavefront, and a
+// workgroup-scope fence between the two passes makes the first pass's stores
+// visible to the second (the result is that of two colour sweeps in turn).
 template < class T >
 __global__ __launch_bounds__( 64 * kCellWorkers ) void kCellSOR( const int32_t* __restrict__ rows, int64_t nrows,
                                                                  int64_t nc, int64_t sc, int64_t w, int64_t mm,
-                                                                 const double* __restrict__ stencil, int cx,
+                                                                 const double* __restrict__ stencil, int cxFirst,
                                                                  double relax, const T* __restrict__ b,
                                                                  T* __restrict__ x, int lv, int rpbLog )
 {
@@ -625,25 +706,31 @@ __global__ __launch_bounds__( 64 * kCellWorkers ) void kCellSOR( const int32_t* 
    for ( int q = 0; q < 15; ++q )
       st[q] = A( stencil[15 * c + q] );
    const A       invc = A( 1 ) / st[0];
-   const int32_t x0 = cx ? 1 : 2, msk = ( 1 << lv ) - 1, top = int32_t( mm ) - y - z;
-   for ( int32_t t = rw.lane;; t += rw.lanes )
+   const int32_t msk = ( 1 << lv ) - 1, top = int32_t( mm ) - y - z;
+   for ( int ph = 0; ph < 2; ++ph )
    {
-      const int32_t xx = x0 + 2 * ( t >> lv ), k = t & msk;
-      if ( xx >= top )
-         break;
-      const int32_t o = ( sr.at( 0, xx ) << lv ) + k;
-      // all loads in flight before the sum (same order of the sum)
-      T val[15];
+      const int32_t x0 = ( ph ? 1 - cxFirst : cxFirst ) ? 1 : 2;
+      for ( int32_t t = rw.lane;; t += rw.lanes )
+      {
+         const int32_t xx = x0 + 2 * ( t >> lv ), k = t & msk;
+         if ( xx >= top )
+            break;
+         const int32_t o = ( sr.at( 0, xx ) << lv ) + k;
+         // all loads in flight before the sum (same order of the sum)
+         T val[15];
 #pragma unroll
-      for ( int q = 0; q < 15; ++q )
-         val[q] = xc[( sr.at( q, xx ) << lv ) + k];
-      A v = toAcc( bc[o] );
-      __builtin_amdgcn_sched_barrier( 0 ); // keeps the scheduler from moving the loads down to their uses
+         for ( int q = 0; q < 15; ++q )
+            val[q] = xc[( sr.at( q, xx ) << lv ) + k];
+         A v = toAcc( bc[o] );
+         __builtin_amdgcn_sched_barrier( 0 ); // keeps the scheduler from moving the loads down to their uses
 #pragma unroll
-      for ( int q = 1; q < 15; ++q )
-         v -= st[q] * toAcc( val[q] );
-      const A xo = toAcc( val[0] );
-      xc[o]      = fromAcc< T >( ( A( 1 ) - A( relax ) ) * xo + A( relax ) * invc * v );
+         for ( int q = 1; q < 15; ++q )
+            v -= st[q] * toAcc( val[q] );
+         const A xo = toAcc( val[0] );
+         xc[o]      = fromAcc< T >( ( A( 1 ) - A( relax ) ) * xo + A( relax ) * invc * v );
+      }
+      if ( ph == 0 )
+         __threadfence_block();
    }
 }
 
@@ -722,49 +809,99 @@ __global__ void kBndSOR( BndGeom g, int64_t n, const int32_t* __restrict__ list,
    } );
 }
 
-// apply / SOR on owned boundary DoFs from their assembled rows: the neighbour
-// loads are independent, so a thread issues them together instead of walking
-// the incident cells one stencil entry at a time
+// apply / SOR on owned boundary DoFs from their assembled rows (sliced ELL):
+// a thread takes one slot (and one vector of a multi-vector function), issues
+// the position and weight loads of kSellUnroll entries, then their gathers,
+// then sums them in entry order.  Logical blocks are spread over the XCDs in
+// contiguous ranges, so neighbouring rows share the gathered lines in one L2.
 template < class T >
-__global__ void kBndApplyCsr( int64_t nb, int64_t ownerBase, const uint8_t* __restrict__ bndFlag, int flag,
-                              const int64_t* __restrict__ ptr, const int64_t* __restrict__ pos,
-                              const double* __restrict__ val, const double* __restrict__ invDiag,
-                              const T* __restrict__ src, T* __restrict__ dst, int add, int lv, double alpha )
+__global__ void __launch_bounds__( kBlock ) kBndApplySell( int64_t nblk, int64_t nslot, const int32_t* __restrict__ rowOf,
+                                                          const uint8_t* __restrict__ bndFlag, int flag,
+                                                          const int64_t* __restrict__ off, const int32_t* __restrict__ pos,
+                                                          const double* __restrict__ val, int64_t ownerBase,
+                                                          const T* __restrict__ src, T* __restrict__ dst, int add, int lv,
+                                                          double alpha )
 {
-   using A           = typename Acc< T >::type;
-   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   const int64_t b   = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
-   if ( b >= nb || !( bndFlag[b] & flag ) )
+   using A          = typename Acc< T >::type;
+   const int64_t lb = xcdBlock( nblk );
+   if ( lb >= nblk )
       return;
+   const int64_t tid = lb * kBlock + threadIdx.x;
+   const int64_t i = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
+   if ( i >= nslot )
+      return;
+   const int32_t b = rowOf[i];
+   if ( b < 0 || !( bndFlag[b] & flag ) )
+      return;
+   const int64_t sl = i / kSellRows, e0 = off[sl] + ( i % kSellRows ), w = ( off[sl + 1] - off[sl] ) / kSellRows;
+   A             s  = A( 0 );
+   for ( int64_t j = 0; j < w; j += kSellUnroll )
+   {
+      int32_t p[kSellUnroll];
+      double  v[kSellUnroll];
+      A       x[kSellUnroll];
+#pragma unroll
+      for ( int u = 0; u < kSellUnroll; ++u )
+      {
+         p[u] = pos[e0 + ( j + u ) * kSellRows];
+         v[u] = val[e0 + ( j + u ) * kSellRows];
+      }
+#pragma unroll
+      for ( int u = 0; u < kSellUnroll; ++u )
+         x[u] = toAcc( src[( int64_t( p[u] ) << lv ) + vec] );
+#pragma unroll
+      for ( int u = 0; u < kSellUnroll; ++u )
+         s += A( v[u] ) * x[u];
+   }
+   s               = A( alpha ) * s;
    const int64_t o = ( ( ownerBase + b ) << lv ) + vec;
-   A             s = A( 1.0 / invDiag[b] ) * toAcc( src[o] );
-   for ( int64_t j = ptr[b]; j < ptr[b + 1]; ++j )
-      s += A( val[j] ) * toAcc( src[( pos[j] << lv ) + vec] );
-   s = A( alpha ) * s;
    if ( add )
       s += toAcc( dst[o] );
    dst[o] = fromAcc< T >( s );
 }
 
 template < class T >
-__global__ void kBndSORCsr( int64_t n, const int32_t* __restrict__ list, int64_t ownerBase,
-                            const uint8_t* __restrict__ bndFlag, int flag, const int64_t* __restrict__ ptr,
-                            const int64_t* __restrict__ pos, const double* __restrict__ val,
-                            const double* __restrict__ invDiag, double relax, const T* __restrict__ rhs,
-                            T* __restrict__ x, int lv )
+__global__ void __launch_bounds__( kBlock ) kBndSORSell( int64_t nblk, int64_t slot0, int64_t nslot,
+                                                        const int32_t* __restrict__ rowOf,
+                                                        const uint8_t* __restrict__ bndFlag, int flag,
+                                                        const int64_t* __restrict__ off, const int32_t* __restrict__ pos,
+                                                        const double* __restrict__ val, int64_t ownerBase,
+                                                        const double* __restrict__ invDiag, double relax,
+                                                        const T* __restrict__ rhs, T* __restrict__ x, int lv )
 {
-   using A           = typename Acc< T >::type;
-   const int64_t tid = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
-   const int64_t i   = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
-   if ( i >= n )
+   using A          = typename Acc< T >::type;
+   const int64_t lb = xcdBlock( nblk );
+   if ( lb >= nblk )
       return;
-   const int64_t b = list[i];
-   if ( !( bndFlag[b] & flag ) )
+   const int64_t tid = lb * kBlock + threadIdx.x;
+   const int64_t li = tid >> lv, vec = tid & ( ( int64_t( 1 ) << lv ) - 1 );
+   if ( li >= nslot )
       return;
-   const int64_t o = ( ( ownerBase + b ) << lv ) + vec;
-   A             s = toAcc( rhs[o] );
-   for ( int64_t j = ptr[b]; j < ptr[b + 1]; ++j )
-      s -= A( val[j] ) * toAcc( x[( pos[j] << lv ) + vec] );
+   const int64_t i = slot0 + li;
+   const int32_t b = rowOf[i];
+   if ( b < 0 || !( bndFlag[b] & flag ) )
+      return;
+   const int64_t sl = i / kSellRows, e0 = off[sl] + ( i % kSellRows ), w = ( off[sl + 1] - off[sl] ) / kSellRows;
+   const int64_t o  = ( ( ownerBase + b ) << lv ) + vec;
+   A             s  = toAcc( rhs[o] );
+   for ( int64_t j = 0; j < w; j += kSellUnroll )
+   {
+      int32_t p[kSellUnroll];
+      double  v[kSellUnroll];
+      A       xv[kSellUnroll];
+#pragma unroll
+      for ( int u = 0; u < kSellUnroll; ++u )
+      {
+         p[u] = pos[e0 + ( j + u ) * kSellRows];
+         v[u] = val[e0 + ( j + u ) * kSellRows];
+      }
+#pragma unroll
+      for ( int u = 0; u < kSellUnroll; ++u )
+         xv[u] = toAcc( x[( int64_t( p[u] ) << lv ) + vec] );
+#pragma unroll
+      for ( int u = 0; u < kSellUnroll; ++u )
+         s -= A( v[u] ) * xv[u];
+   }
    x[o] = fromAcc< T >( ( A( 1 ) - A( relax ) ) * toAcc( x[o] ) + A( relax ) * A( invDiag[b] ) * s );
 }
 
@@ -994,8 +1131,10 @@ void p1Apply( Context* ctx, P1Operator& op, Function& src, Function& dst, int L,
       using T = decltype( tag );
       if ( lat.nb > 0 && o.csr )
       {
-         kBndApplyCsr< T ><<< gridFor( lat.nb << lv ), kBlock, 0, ctx->stream >>>(
-             lat.nb, lat.nc * lat.sc, lat.bndFlag.p, flag, o.csrPtr.p, o.csrPos.p, o.csrVal.p, o.invDiag.p,
+         const SellRows& sr   = o.applyRows;
+         const int64_t   nblk = ( ( sr.nslot << lv ) + kBlock - 1 ) / kBlock;
+         kBndApplySell< T ><<< xcdGrid( nblk ), kBlock, 0, ctx->stream >>>(
+             nblk, sr.nslot, sr.row.p, lat.bndFlag.p, flag, sr.off.p, sr.pos.p, sr.val.p, lat.nc * lat.sc,
              (const T*) s, (T*) d, update, lv, alpha );
          HG_CHECK_HIP( hipGetLastError() );
       }
@@ -1060,9 +1199,14 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
          if ( n == 0 )
             return;
          if ( o.csr )
-            kBndSORCsr< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>(
-                n, o.colorRows.p + o.colorStart[cc], lat.nc * lat.sc, lat.bndFlag.p, flag, o.csrPtr.p, o.csrPos.p,
-                o.csrVal.p, o.invDiag.p, relax, (const T*) bp, (T*) xp, lv );
+         {
+            const SellRows& sr    = o.sorRows;
+            const int64_t   slot0 = o.sorColorStart[cc], ns = o.sorColorStart[cc + 1] - slot0;
+            const int64_t   nblk  = ( ( ns << lv ) + kBlock - 1 ) / kBlock;
+            kBndSORSell< T ><<< xcdGrid( nblk ), kBlock, 0, ctx->stream >>>(
+                nblk, slot0, ns, sr.row.p, lat.bndFlag.p, flag, sr.off.p, sr.pos.p, sr.val.p, lat.nc * lat.sc,
+                o.invDiag.p, relax, (const T*) bp, (T*) xp, lv );
+         }
          else
             kBndSOR< T ><<< gridFor( n << lv ), kBlock, 0, ctx->stream >>>( g, n, o.colorRows.p + o.colorStart[cc],
                                                                            lat.nc, lat.bndFlag.p, flag, o.clsStencil.p,
@@ -1070,16 +1214,15 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
                                                                            lv );
          HG_CHECK_HIP( hipGetLastError() );
       };
-      // cell-interior parity colour k = cx + 2 cy + 4 cz
-      auto interiorColour = [&]( int k ) {
-         const int cx = k & 1, cy = ( k >> 1 ) & 1, cz = k >> 2;
-         const int     grp = cy + 2 * cz;
-         const int64_t n   = lat.rowGroup[grp + 1] - lat.rowGroup[grp];
+      // cell-interior parity colours k = cx + 2 cy + 4 cz, two at a time: the
+      // row group grp = cy + 2 cz with cx = cxFirst, then 1 - cxFirst
+      auto interiorGroup = [&]( int grp, int cxFirst ) {
+         const int64_t n = lat.rowGroup[grp + 1] - lat.rowGroup[grp];
          if ( n == 0 )
             return;
          const int rpb = rowsPerBlockLog( ( ( lat.mm - 2 ) / 2 ) << lv );
          kCellSOR< T ><<< cellGrid( n * lat.nc, rpb ), 64 * kCellWorkers, 0, ctx->stream >>>(
-             lat.rows.p + 2 * lat.rowGroup[grp], n, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, cx, relax,
+             lat.rows.p + 2 * lat.rowGroup[grp], n, lat.nc, lat.sc, lat.w, lat.mm, o.stencil.p, cxFirst, relax,
              (const T*) bp, (T*) xp, lv, rpb );
          HG_CHECK_HIP( hipGetLastError() );
       };
@@ -1096,14 +1239,14 @@ void p1SOR( Context* ctx, P1Operator& op, Function& x, Function& b, double relax
          if ( o.csr && nbc > 0 )
             pullHalos( ctx, *op.st, lat, xp, x.dtype, lv );
          if ( interior )
-            for ( int k = 0; k < 8; ++k )
-               interiorColour( k );
+            for ( int grp = 0; grp < 4; ++grp )
+               interiorGroup( grp, 0 );
       }
       else
       {
          if ( interior )
-            for ( int k = 7; k >= 0; --k )
-               interiorColour( k );
+            for ( int grp = 3; grp >= 0; --grp )
+               interiorGroup( grp, 1 );
          for ( size_t cc = nbc; cc-- > 0; )
             boundaryColour( cc );
          if ( o.csr && nbc > 0 )
