@@ -60,6 +60,9 @@ SIGNATURES = {
     "hg_p2_function_create": [_p, _i64, _i, _i, _i, _p],
     "hg_function_mult": [_p, _i64, _i64, _i64, _i, _i],
     "hg_p2_divkgrad_create": [_p, _i64, _i, _i, _i, _p, _p, _p, _p],
+    "hg_p2_divkgrad_create_batched": [_p, _i64, _i, _i, _i, _p],
+    "hg_p2_divkgrad_set_slot": [_p, _i64, _i, _i, _p, _p, _p],
+    "hg_p2_function_create_batched": [_p, _i64, _i, _i, _i, _i, _p],
     "hg_p2_apply": [_p, _i64, _i64, _i64, _i, _i, _i],
     "hg_p2_inverse_diagonal": [_p, _i64, _i, _i64],
     "hg_p2_store_coefficients": [_p, _i64, _i],
@@ -96,9 +99,11 @@ SIGNATURES = {
     "hg_function_dot_batched_noalloc": [_p, _i64, _i64, _i, _i, _p],
     "hg_function_scale_batched": [_p, _i64, _p, _i, _i],
     "hg_function_chebyshev_step": [_p, _i64, _i64, _i64, _i64, _i64, _d, _i, _i],
+    "hg_function_chebyshev_step_slots": [_p, _i64, _i64, _i64, _i64, _i64, _i, _i, _i],
     "hg_slots_alloc": [_p, _i, _p],
     "hg_slots_free": [_p, _i, _i],
     "hg_slots_read": [_p, _i, _i, _p],
+    "hg_slots_write": [_p, _i, _i, _p],
     "hg_function_dot_slot": [_p, _i64, _i64, _i, _i, _i],
     "hg_function_lincomb_slots": [_p, _i64, _i, _p, _p, _p, _p, _i, _i, _i],
     "hg_p1_batched_gmg_solve": [_p, _i64, _i64, _i64, _i, _i, _i, _i, _i, _i, _d, _p, _p],

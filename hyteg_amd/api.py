@@ -152,7 +152,7 @@ class P1Function:
     OFFSET = 0
 
     def __init__(self, name, storage, min_level, max_level, dtype=F64, nvec=1):
-        """nvec > 1 (P1 only): that many independent vectors interleaved per DoF"""
+        """nvec > 1: that many independent vectors interleaved per DoF"""
         self.name = name
         self.storage = storage
         self.ctx = storage.ctx
@@ -160,10 +160,9 @@ class P1Function:
         self.nvec = nvec
         h = C.c_int64()
         if nvec != 1:
-            if self.OFFSET:
-                raise ValueError("P2 functions hold one vector")
-            self.ctx.check(lib().hg_p1_function_create_batched(self.ctx.h, storage.h, min_level, max_level, dtype,
-                                                               nvec, C.byref(h)), f"Function {name}")
+            create = lib().hg_p2_function_create_batched if self.OFFSET else lib().hg_p1_function_create_batched
+            self.ctx.check(create(self.ctx.h, storage.h, min_level, max_level, dtype, nvec, C.byref(h)),
+                           f"Function {name}")
         else:
             create = lib().hg_p2_function_create if self.OFFSET else lib().hg_p1_function_create
             self.ctx.check(create(self.ctx.h, storage.h, min_level, max_level, dtype, C.byref(h)), f"Function {name}")
@@ -367,6 +366,30 @@ class P1ConstantLaplaceOperator:
         out = np.zeros(15)
         self.ctx.check(lib().hg_p1_cell_stencil(self.ctx.h, self.h, level, cell, _ptr(out)), "cell_stencil")
         return out
+
+
+class P2DivKGradOperatorBatched(P2DivKGradOperator):
+    """nslot coefficient slots for P2 functions of nslot interleaved vectors:
+    vector s is multiplied by the div(k_s grad) operator of slot s (k_s = 1 until
+    set_slot gives it its own modes)"""
+
+    def __init__(self, storage, min_level, max_level, nslot):
+        self.storage = storage
+        self.ctx = storage.ctx
+        self.nslot = nslot
+        h = C.c_int64()
+        self.ctx.check(lib().hg_p2_divkgrad_create_batched(self.ctx.h, storage.h, min_level, max_level, nslot,
+                                                           C.byref(h)), "P2DivKGradOperatorBatched")
+        self.h = h.value
+
+    def set_slot(self, slot, a=(), omega=(), phi=()):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        omega = np.ascontiguousarray(omega, dtype=np.float64).reshape(-1, 3)
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        e = np.zeros(1)
+        nm = len(a)
+        pa, po, pp = (_ptr(a), _ptr(omega), _ptr(phi)) if nm else (_ptr(e), _ptr(e), _ptr(e))
+        self.ctx.check(lib().hg_p2_divkgrad_set_slot(self.ctx.h, self.h, slot, nm, pa, po, pp), "set_slot")
 
 
 def restrict(f, source_level, flag):

@@ -162,6 +162,13 @@ struct P2Operator : OperatorBase
    std::map< int, DevBuf< __half > >      kc;       // k per micro-cell (tile-major), by P2 level
    std::map< int, DevBuf< float > >       gm;       // reduced-form metric per micro-cell, by P2 level
    std::map< int, DevBuf< float > >       accum;    // FP32 accumulator of the FP16 applies, by P2 level
+   // batched operators (hg_p2_divkgrad_create_batched): 2^lvSlots coefficient
+   // slots for as many interleaved vectors; kpar holds one KParams per slot, a
+   // level's coefficient table holds the slots' tables one after another
+   // (tableStride values each) and its inverse diagonal is interleaved
+   int                                    nslot = 1, lvSlots = 0;
+   std::map< int, int64_t >               tableStride; // by P2 level
+   std::map< int, DevBuf< double > >      invScratch;  // one slot's diagonal, by P2 level
 };
 
 __device__ __forceinline__ double kEval( const KParams& K, const double x[3] )
@@ -770,13 +777,23 @@ __device__ __forceinline__ void p2ElementAction( const P2CellGeomT< R >& g, cons
 // the element vector.
 // T: value type of src / dst; K: value type of the coefficient table; the
 // arithmetic runs in Acc<T> (FP64 for double, FP32 for float and half).
-template < class T, class K, bool STORED >
+// BATCH: 2^lv interleaved vectors (vector blockIdx.y, its coefficient table at
+// table + blockIdx.y * tstride); otherwise one vector.
+template < class T, class K, bool STORED, bool BATCH = false >
 __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom,
                                                  const P2CellGeom* __restrict__ geom64, const KParams* __restrict__ kp,
                                                  const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1,
                                                  int64_t w1, const T* __restrict__ src, T* __restrict__ dst,
-                                                 double alpha, const uint64_t* __restrict__ tiles )
+                                                 double alpha, const uint64_t* __restrict__ tiles, int lvB = 0,
+                                                 int64_t tstride = 0 )
 {
+   const int lv = BATCH ? lvB : 0;
+   if constexpr ( BATCH )
+   {
+      table += int64_t( blockIdx.y ) * tstride;
+      src += blockIdx.y;
+      dst += blockIdx.y;
+   }
    using R  = typename Acc< T >::type;
    using QC = QuadConst< R >;
    // Element results are summed with LDS atomics into a result box of their own
@@ -813,7 +830,7 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
       for ( int q = 0; q < 11; ++q )
          kv[q] = table[( lb * 11 + q ) * kTileThreads + threadIdx.x];
    }
-   const T* srcC = src + base;
+   const T* srcC = src + ( base << lv );
    T        bv[2];
    bool     bin[2];
 #pragma unroll
@@ -823,7 +840,7 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
       bin[j]          = p < kBox3 && X + Y + Z <= top;
-      bv[j]           = bin[j] ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : T( 0 );
+      bv[j]           = bin[j] ? srcC[int64_t( tetRowStart32( int32_t( w1 ), Y, Z ) + X ) << lv] : T( 0 );
    }
    if constexpr ( kAtomic )
    {
@@ -900,13 +917,14 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
    // points are shared with neighbouring tiles, and a read-modify-write of the
    // interior points would make the wave wait for its outstanding atomics too
    // (the vector-memory counter covers loads and atomics alike)
-   T* dstC = dst + base;
+   T* dstC = dst + ( base << lv );
    for ( int p = threadIdx.x; p < kBox3; p += kTileThreads )
    {
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
       if ( X + Y + Z <= top )
-         sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * ( kAtomic ? sa[p] : waveBoxSum( lds, p ) ) );
+         sharedAdd( dstC + ( int64_t( tetRowStart32( int32_t( w1 ), Y, Z ) + X ) << lv ),
+                    R( alpha ) * ( kAtomic ? sa[p] : waveBoxSum( lds, p ) ) );
    }
 }
 
@@ -920,9 +938,14 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
                                                           const KParams* __restrict__ kp, const K* __restrict__ table,
                                                           int64_t mm, int64_t ndiag, int64_t sc1, int64_t w1,
                                                           const T* __restrict__ src, T* __restrict__ dst, double alpha,
-                                                          const uint64_t* __restrict__ tiles )
+                                                          const uint64_t* __restrict__ tiles, int lv = 0,
+                                                          int64_t tstride = 0 )
 {
    using R = typename Acc< T >::type;
+   // 2^lv interleaved vectors: vector blockIdx.y with its own coefficient table
+   table += int64_t( blockIdx.y ) * tstride;
+   src += blockIdx.y;
+   dst += blockIdx.y;
    __shared__ R  sb[kBox3];
    const int64_t ld = xcdBlock( ndiag );
    if ( ld >= ndiag )
@@ -930,7 +953,7 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
    int64_t         lb;
    const TileCoord tc   = tiles ? tileFromList( tiles, ld, mm, lb ) : tileOfDiagonal( ld, mm, lb );
    const int64_t   base = tc.c * sc1;
-   const T*        srcC = src + base;
+   const T*        srcC = src + ( base << lv );
    // the lane's (at most three) of the 165 box points inside the macro-cell:
    // all loads issued before the first LDS write
    constexpr int kPer = kDiagBoxP / 64;
@@ -943,7 +966,7 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
       const int     p  = bp[j] < 0 ? 0 : bp[j];
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      bv[j]           = bp[j] >= 0 ? srcC[tetRowStart32( int32_t( w1 ), Y, Z ) + X] : T( 0 );
+      bv[j]           = bp[j] >= 0 ? srcC[int64_t( tetRowStart32( int32_t( w1 ), Y, Z ) + X ) << lv] : T( 0 );
    }
 #pragma unroll
    for ( int j = 0; j < kPer; ++j )
@@ -986,7 +1009,7 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
          atomicAdd( &sb[pos[i]], y[i] );
    }
    __syncthreads();
-   T* dstC = dst + base;
+   T* dstC = dst + ( base << lv );
 #pragma unroll
    for ( int j = 0; j < kPer; ++j )
    {
@@ -995,7 +1018,7 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
       const int     p  = bp[j];
       const int32_t lx = p % kBox, ly = ( p / kBox ) % kBox, lz = p / ( kBox * kBox );
       const int32_t X = 2 * int32_t( tc.x0 ) + lx, Y = 2 * int32_t( tc.y0 ) + ly, Z = 2 * int32_t( tc.z0 ) + lz;
-      sharedAdd( dstC + tetRowStart32( int32_t( w1 ), Y, Z ) + X, R( alpha ) * sb[p] );
+      sharedAdd( dstC + ( int64_t( tetRowStart32( int32_t( w1 ), Y, Z ) + X ) << lv ), R( alpha ) * sb[p] );
    }
 }
 
@@ -1009,6 +1032,17 @@ __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyStored(
     const T* __restrict__ src, T* __restrict__ dst, double alpha, const uint64_t* __restrict__ tiles )
 {
    p2TileApplyBody< T, K, true >( geom, geom64, kp, table, mm, ntiles, sc1, w1, src, dst, alpha, tiles );
+}
+// the same over 2^lv interleaved vectors with one coefficient table each
+// (grid.y = vectors)
+template < class T, class K >
+__global__ __launch_bounds__( kTileThreads ) void kP2TileApplyStoredBatched(
+    const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const K* __restrict__ table, int64_t tstride,
+    int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1, const T* __restrict__ src, T* __restrict__ dst, double alpha,
+    const uint64_t* __restrict__ tiles, int lv )
+{
+   p2TileApplyBody< T, K, true, true >( geom, nullptr, nullptr, table, mm, ntiles, sc1, w1, src, dst, alpha, tiles, lv,
+                                        tstride );
 }
 // coefficients evaluated in place (the compiler's own register budget)
 template < class T, class K >
@@ -1057,8 +1091,8 @@ static int64_t microCellsPerCell( int64_t mm )
 
 static void p2CheckFunction( const P2Operator& op, const Function& f, int L )
 {
-   if ( f.st != op.st || f.off != 1 || f.lv != 0 )
-      throw Error( -1, "P2 operator needs (single-vector) P2 functions on its storage" );
+   if ( f.st != op.st || f.off != 1 || f.lv != op.lvSlots )
+      throw Error( -1, "P2 operator needs P2 functions on its storage with one vector per coefficient slot" );
    if ( L < op.minL || L > op.maxL || L < f.minL || L > f.maxL )
       throw Error( -1, "level outside the operator's or function's level range" );
 }
@@ -1076,9 +1110,14 @@ __global__ __launch_bounds__( 64 ) void kP2SmallApply( const P2CellGeomT< typena
                                                        const P2CellGeom* __restrict__ geom64,
                                                        const KParams* __restrict__ kp, const K* __restrict__ table,
                                                        int64_t mm, int64_t nc, int64_t sc1, int64_t w1,
-                                                       const T* __restrict__ src, T* __restrict__ dst, double alpha )
+                                                       const T* __restrict__ src, T* __restrict__ dst, double alpha,
+                                                       int lv = 0, int64_t tstride = 0 )
 {
    using R = typename Acc< T >::type;
+   // 2^lv interleaved vectors: vector blockIdx.y with its own coefficient table
+   table += int64_t( blockIdx.y ) * tstride;
+   src += blockIdx.y;
+   dst += blockIdx.y;
    __shared__ R  sb[kBox3];
    const int64_t c = blockIdx.x;
    if ( c >= nc )
@@ -1086,12 +1125,12 @@ __global__ __launch_bounds__( 64 ) void kP2SmallApply( const P2CellGeomT< typena
    const TileCoord tc{ c, 0, 0, 0 };
    const int32_t   w    = int32_t( w1 );
    const int32_t   top  = w - 1;
-   const T*        srcC = src + c * sc1;
+   const T*        srcC = src + ( ( c * sc1 ) << lv );
    for ( int p = threadIdx.x; p < w * w * w; p += 64 )
    {
       const int32_t X = p % w, Y = ( p / w ) % w, Z = p / ( w * w );
       if ( X + Y + Z <= top )
-         sb[X + kBox * ( Y + kBox * Z )] = R( toAcc( srcC[tetRowStart32( w, Y, Z ) + X] ) );
+         sb[X + kBox * ( Y + kBox * Z )] = R( toAcc( srcC[int64_t( tetRowStart32( w, Y, Z ) + X ) << lv] ) );
    }
    __syncthreads();
    const int               l = int( threadIdx.x );
@@ -1135,13 +1174,13 @@ __global__ __launch_bounds__( 64 ) void kP2SmallApply( const P2CellGeomT< typena
    }
    __syncthreads();
    // the wave owns the cell's storage (halo copies included): no other wave adds here
-   T* dstC = dst + c * sc1;
+   T* dstC = dst + ( ( c * sc1 ) << lv );
    for ( int p = threadIdx.x; p < w * w * w; p += 64 )
    {
       const int32_t X = p % w, Y = ( p / w ) % w, Z = p / ( w * w );
       if ( X + Y + Z <= top )
       {
-         T& o = dstC[tetRowStart32( w, Y, Z ) + X];
+         T& o = dstC[int64_t( tetRowStart32( w, Y, Z ) + X ) << lv];
          o    = fromAcc< T >( R( toAcc( o ) ) + R( alpha ) * sb[X + kBox * ( Y + kBox * Z )] );
       }
    }
@@ -1155,17 +1194,20 @@ __global__ __launch_bounds__( 64 ) void kP2SmallApply( const P2CellGeomT< typena
 template < class K >
 static const K* p2Coefficients( Context* ctx, P2Operator& op, int L, int dtype )
 {
-   if ( !op.storeCoefficients || op.nm == 0 )
+   if ( op.nslot == 1 && ( !op.storeCoefficients || op.nm == 0 ) )
       return nullptr;
    auto it = op.kq.find( { L, dtype } );
    if ( it != op.kq.end() )
       return (const K*) it->second.p;
    const int64_t mm     = int64_t( 1 ) << L;
    const int64_t ntiles = tilesPerCell( mm ) * int64_t( op.st->topo.nc );
+   const int64_t per    = ntiles * 11 * kTileThreads;
    auto&         buf    = op.kq[{ L, dtype }];
-   buf.alloc( ctx, size_t( ntiles ) * 11 * kTileThreads * sizeof( K ) );
-   kP2TileCoefficients< K ><<< gridFor( ntiles, 1 ), kTileThreads, 0, ctx->stream >>>( op.tgeom.at( L ).p, op.kpar.p, mm,
-                                                                                     ntiles, (K*) buf.p );
+   buf.alloc( ctx, size_t( per * op.nslot ) * sizeof( K ) );
+   op.tableStride[L] = per;
+   for ( int sl = 0; sl < op.nslot; ++sl )
+      kP2TileCoefficients< K ><<< gridFor( ntiles, 1 ), kTileThreads, 0, ctx->stream >>>(
+          op.tgeom.at( L ).p, op.kpar.p + sl, mm, ntiles, (K*) buf.p + sl * per );
    HG_CHECK_HIP( hipGetLastError() );
    return (const K*) buf.p;
 }
@@ -1270,6 +1312,53 @@ static void p2ApplyReduced( Context* ctx, P2Operator& op, Lattice& lat, int L, c
 // dst = alpha A src (update Replace) or dst + alpha A src (update Add) on the
 // DoFs of type flag: P2ElementwiseOperator::gemv with beta 0 or 1
 // (P2ElementwiseOperator.cpp:110-233)
+// the apply of a batched operator: FP64 interleaved vectors, vector v with
+// the coefficients of slot v, the same kernels (and per-vector results) as the
+// single-vector apply with the stored table
+static void p2ApplyBatched( Context* ctx, P2Operator& op, Lattice& lat, int L, Function& src, Function& dst, int flag,
+                            int update, double alpha )
+{
+   if ( src.dtype != 0 )
+      throw Error( HG_ERR_UNSUPPORT, "batched P2 apply: FP64 functions only" );
+   if ( update != HG_REPLACE || !( flag & kInner ) )
+      throw Error( HG_ERR_UNSUPPORT, "batched P2 apply: update Replace with Inner in the flag only" );
+   Storage&       st = *op.st;
+   const int      lv = op.lvSlots;
+   const unsigned nv = 1u << lv;
+   double*        s  = (double*) src.ptr( L );
+   double*        d  = (double*) dst.ptr( L );
+   const int64_t  mm = int64_t( 1 ) << L;
+   pullHalos( ctx, st, lat, s, 0, lv );
+   kFillRange< double ><<< gridFor( ( lat.nc * lat.sc ) << lv ), kBlock, 0, ctx->stream >>>( ( lat.nc * lat.sc ) << lv, d,
+                                                                                          0.0 );
+   HG_CHECK_HIP( hipGetLastError() );
+   const double*    table   = p2Coefficients< double >( ctx, op, L, 0 );
+   const int64_t    tstride = op.tableStride.at( L );
+   const auto*      geom    = op.tgeom.at( L ).p;
+   const int64_t    nint    = tet( tileEdges( mm ) - 1 ) * lat.nc;
+   const int64_t    ndiag   = tri( tileEdges( mm ) ) * lat.nc;
+   const uint64_t*  tint    = mm > kSmallMaxMM ? tileList( ctx, lat, mm, kTilesInterior ) : nullptr;
+   const uint64_t*  tlast   = mm > kSmallMaxMM ? tileList( ctx, lat, mm, kTilesLast ) : nullptr;
+   ctx->beginTiming();
+   if ( mm <= kSmallMaxMM )
+      kP2SmallApply< double, double, true ><<< dim3( unsigned( lat.nc ), nv ), 64, 0, ctx->stream >>>(
+          geom, geom, nullptr, table, mm, lat.nc, lat.sc, lat.w, s, d, alpha, lv, tstride );
+   else
+   {
+      if ( mm % kTile != 0 )
+         throw Error( HG_ERR_STATE, "P2 tile apply: micro-edge count not a multiple of the tile width" );
+      if ( nint > 0 )
+         kP2TileApplyStoredBatched< double, double ><<< dim3( xcdGrid( nint ), nv ), kTileThreads, 0, ctx->stream >>>(
+             geom, table, tstride, mm, nint, lat.sc, lat.w, s, d, alpha, tint, lv );
+      HG_CHECK_HIP( hipGetLastError() );
+      kP2TileApplyDiag< double, double, true ><<< dim3( xcdGrid( ndiag ), nv ), 64, 0, ctx->stream >>>(
+          geom, geom, nullptr, table, mm, ndiag, lat.sc, lat.w, s, d, alpha, tlast, lv, tstride );
+   }
+   HG_CHECK_HIP( hipGetLastError() );
+   ctx->endTiming();
+   pushAdditive( ctx, st, lat, d, 0, kAll ^ flag, true, lv );
+}
+
 void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L, int flag, int update,
               double alpha = 1.0 )
 {
@@ -1286,6 +1375,11 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
    const int64_t mm  = int64_t( 1 ) << L;
    if ( lat.w > 1024 )
       throw Error( HG_ERR_UNSUPPORT, "P2 apply supports levels up to 9 (32-bit lattice offsets)" );
+   if ( op.nslot > 1 )
+   {
+      p2ApplyBatched( ctx, op, lat, L, src, dst, flag, update, alpha );
+      return;
+   }
    pullHalos( ctx, st, lat, s, src.dtype );
    dispatchType( src.dtype, [&]( auto tag ) {
       using T = decltype( tag );
@@ -1351,6 +1445,44 @@ void p2Apply( Context* ctx, P2Operator& op, Function& src, Function& dst, int L,
    pushAdditive( ctx, st, lat, d, dst.dtype, kAll ^ flag, update == HG_REPLACE );
 }
 
+// inverse diagonal of coefficient slot sl into out (one vector, lattice L+1)
+static void p2SlotInverseDiagonal( Context* ctx, P2Operator& op, int L, int sl, double* out )
+{
+   Storage&      st  = *op.st;
+   Lattice&      lat = st.lattice( ctx, L + 1 );
+   const int64_t mm  = int64_t( 1 ) << L;
+   const int64_t n   = microCellsPerCell( mm ) * lat.nc;
+   kP2ElementDiag<<< gridFor( n, 128 ), 128, 0, ctx->stream >>>( op.geom.p, op.kpar.p + sl, mm, lat.nc, lat.sc, lat.w,
+                                                                out );
+   HG_CHECK_HIP( hipGetLastError() );
+   pushAdditive( ctx, st, lat, out, 0, 0, true );
+   kInvertNonZero<<< gridFor( lat.nstore ), kBlock, 0, ctx->stream >>>( lat.nstore, out );
+   HG_CHECK_HIP( hipGetLastError() );
+}
+
+// dst[(i << lv) + k] = src[i], i < n
+__global__ void kInsertVector( int64_t n, const double* __restrict__ src, int lv, int k, double* __restrict__ dst )
+{
+   const int64_t i = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   if ( i < n )
+      dst[( i << lv ) + k] = src[i];
+}
+
+// slot sl of a batched operator's inverse diagonal on level L (built before)
+static void p2RefreshSlotDiagonal( Context* ctx, P2Operator& op, int L, int sl )
+{
+   Lattice& lat = op.st->lattice( ctx, L + 1 );
+   auto&    tmp = op.invScratch[L];
+   if ( !tmp.p )
+      tmp.alloc( ctx, lat.nstore );
+   kFillRange< double ><<< gridFor( lat.nstore ), kBlock, 0, ctx->stream >>>( lat.nstore, tmp.p, 0.0 );
+   HG_CHECK_HIP( hipGetLastError() );
+   p2SlotInverseDiagonal( ctx, op, L, sl, tmp.p );
+   kInsertVector<<< gridFor( lat.nstore ), kBlock, 0, ctx->stream >>>( lat.nstore, tmp.p, op.lvSlots, sl,
+                                                                      op.invDiag.at( L ).p );
+   HG_CHECK_HIP( hipGetLastError() );
+}
+
 double* p2InverseDiagonal( Context* ctx, P2Operator& op, int L )
 {
    auto it = op.invDiag.find( L );
@@ -1359,15 +1491,35 @@ double* p2InverseDiagonal( Context* ctx, P2Operator& op, int L )
    Storage& st  = *op.st;
    Lattice& lat = st.lattice( ctx, L + 1 );
    auto&    buf = op.invDiag[L];
-   buf.alloc( ctx, lat.nstore );
-   const int64_t mm = int64_t( 1 ) << L;
-   const int64_t n  = microCellsPerCell( mm ) * lat.nc;
-   kP2ElementDiag<<< gridFor( n, 128 ), 128, 0, ctx->stream >>>( op.geom.p, op.kpar.p, mm, lat.nc, lat.sc, lat.w, buf.p );
-   HG_CHECK_HIP( hipGetLastError() );
-   pushAdditive( ctx, st, lat, buf.p, 0, 0, true );
-   kInvertNonZero<<< gridFor( lat.nstore ), kBlock, 0, ctx->stream >>>( lat.nstore, buf.p );
-   HG_CHECK_HIP( hipGetLastError() );
+   buf.alloc( ctx, size_t( lat.nstore ) << op.lvSlots );
+   if ( op.nslot == 1 )
+      p2SlotInverseDiagonal( ctx, op, L, 0, buf.p );
+   else
+      for ( int sl = 0; sl < op.nslot; ++sl )
+         p2RefreshSlotDiagonal( ctx, op, L, sl );
    return buf.p;
+}
+
+// new coefficients for slot sl of a batched operator: its KParams, and its
+// part of every coefficient table and inverse diagonal built so far
+static void p2SetSlot( Context* ctx, P2Operator& op, int sl, const KParams& kp )
+{
+   {
+      DeviceGateLock gate;
+      HG_CHECK_HIP( hipMemcpyAsync( op.kpar.p + sl, &kp, sizeof( KParams ), hipMemcpyHostToDevice, ctx->stream ) );
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+   }
+   for ( auto& [key, buf] : op.kq )
+   {
+      const int     L      = key.first;
+      const int64_t mm     = int64_t( 1 ) << L;
+      const int64_t ntiles = tilesPerCell( mm ) * int64_t( op.st->topo.nc );
+      kP2TileCoefficients< double ><<< gridFor( ntiles, 1 ), kTileThreads, 0, ctx->stream >>>(
+          op.tgeom.at( L ).p, op.kpar.p + sl, mm, ntiles, (double*) buf.p + sl * op.tableStride.at( L ) );
+      HG_CHECK_HIP( hipGetLastError() );
+   }
+   for ( auto& [L, buf] : op.invDiag )
+      p2RefreshSlotDiagonal( ctx, op, L, sl );
 }
 
 template < class T >
@@ -1394,10 +1546,12 @@ __global__ void kMult( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bn
 // type where the separate assign / setToZero / multElementwise / add kernels
 // store it, with the same operations (no contraction), so the result is theirs
 // bit for bit.
+// cv (optional): the step size of each of the 2^lv vectors, in place of c
 template < class T >
 __global__ void kChebyshevStep( int64_t ncell, int64_t nb, const uint8_t* __restrict__ bndFlag, int flag, int first,
                                 double c, const T* __restrict__ b, const T* __restrict__ s,
-                                const T* __restrict__ invDiag, int lv, T* __restrict__ t, T* __restrict__ x )
+                                const T* __restrict__ invDiag, int lv, T* __restrict__ t, T* __restrict__ x,
+                                const double* __restrict__ cv )
 {
 #pragma clang fp contract( off )
    using A           = typename Acc< T >::type;
@@ -1405,6 +1559,8 @@ __global__ void kChebyshevStep( int64_t ncell, int64_t nb, const uint8_t* __rest
    const int64_t i   = tid >> lv;
    if ( i >= ncell + nb )
       return;
+   if ( cv )
+      c = cv[tid & ( ( int64_t( 1 ) << lv ) - 1 )];
    const int64_t p  = tid;
    const bool    on = i < ncell ? ( flag & kInner ) != 0 : ( bndFlag[i - ncell] & flag ) != 0;
    if ( !on )
@@ -1642,6 +1798,48 @@ int hg_p2_divkgrad_create( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, i
    } );
 }
 
+int hg_p2_divkgrad_create_batched( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int nslot, hg_handle* out )
+{
+   int lv = 0;
+   while ( ( 1 << lv ) < nslot )
+      ++lv;
+   if ( nslot < 2 || ( 1 << lv ) != nslot || lv > kMaxLv || !out )
+      return hg_guard( h, []( Context* ) {
+         throw Error( HG_ERR_ARG, "batched P2 operator: the slot count must be a power of two from 2 to 64" );
+      } );
+   const int rc = hg_p2_divkgrad_create( h, s, minLevel, maxLevel, 0, nullptr, nullptr, nullptr, out );
+   if ( rc != HG_OK )
+      return rc;
+   return hg_guard( h, [&]( Context* ctx ) {
+      P2Operator& op = ctx->op< P2Operator >( *out );
+      op.nslot       = nslot;
+      op.lvSlots     = lv;
+      op.reduced     = false;
+      std::vector< KParams > kp( size_t( nslot ), KParams{} );
+      op.kpar.upload( ctx, kp );
+   } );
+}
+
+int hg_p2_divkgrad_set_slot( hg_ctx* h, hg_handle opH, int slot, int nm, const double* a, const double* omega,
+                             const double* phi )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P2Operator& op = ctx->op< P2Operator >( opH );
+      if ( op.nslot < 2 || slot < 0 || slot >= op.nslot || nm < 0 || nm > 32 || ( nm > 0 && ( !a || !omega || !phi ) ) )
+         throw Error( HG_ERR_ARG, "set_slot needs a batched operator, a slot index in range and up to 32 modes" );
+      KParams kp{};
+      kp.nm = nm;
+      for ( int m = 0; m < nm; ++m )
+      {
+         kp.a[m]  = a[m];
+         kp.ph[m] = phi[m];
+         for ( int d = 0; d < 3; ++d )
+            kp.om[m][d] = omega[3 * m + d];
+      }
+      p2SetSlot( ctx, op, slot, kp );
+   } );
+}
+
 int hg_p2_apply( hg_ctx* h, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update )
 {
    return hg_guard( h, [&]( Context* ctx ) {
@@ -1716,11 +1914,11 @@ int hg_p2_inverse_diagonal( hg_ctx* h, hg_handle op, int level, hg_handle dst )
       p2CheckFunction( o, f, level );
       const double* inv = p2InverseDiagonal( ctx, o, level );
       Lattice&      lat = o.st->lattice( ctx, level + 1 );
+      const int64_t n   = lat.nstore << o.lvSlots;
       if ( f.dtype == 0 )
-         HG_CHECK_HIP( hipMemcpyAsync( f.ptr( level ), inv, lat.nstore * sizeof( double ), hipMemcpyDeviceToDevice,
-                                       ctx->stream ) );
+         HG_CHECK_HIP( hipMemcpyAsync( f.ptr( level ), inv, n * sizeof( double ), hipMemcpyDeviceToDevice, ctx->stream ) );
       else
-         castCopy( ctx, inv, 0, f.ptr( level ), f.dtype, lat.nstore );
+         castCopy( ctx, inv, 0, f.ptr( level ), f.dtype, n );
    } );
 }
 
@@ -1749,8 +1947,8 @@ int hg_function_mult( hg_ctx* h, hg_handle dst, hg_handle a, hg_handle b, int le
    } );
 }
 
-int hg_function_chebyshev_step( hg_ctx* h, hg_handle x, hg_handle t, hg_handle s, hg_handle b, hg_handle invDiag,
-                                double c, int level, int flag )
+static int chebyshevStep( hg_ctx* h, hg_handle x, hg_handle t, hg_handle s, hg_handle b, hg_handle invDiag, double c,
+                          int slot0, int level, int flag )
 {
    return hg_guard( h, [&]( Context* ctx ) {
       Function& fx = ctx->function( x );
@@ -1768,14 +1966,36 @@ int hg_function_chebyshev_step( hg_ctx* h, hg_handle x, hg_handle t, hg_handle s
       const int64_t n = l.nc * l.sc + l.nb;
       if ( n == 0 )
          return;
+      const double* cv = nullptr;
+      if ( slot0 >= 0 )
+      {
+         if ( slot0 + fx.nvec() > kSlots )
+            throw Error( HG_ERR_ARG, "chebyshev_step_slots: slot range outside the context's slots" );
+         cv = ctx->slots.p + slot0;
+      }
       dispatchType( fx.dtype, [&]( auto tag ) {
          using T = decltype( tag );
          kChebyshevStep< T ><<< gridFor( n << fx.lv ), kBlock, 0, ctx->stream >>>(
              l.nc * l.sc, l.nb, l.bndFlag.p, flag, fb ? 1 : 0, c, fb ? (const T*) fb->ptr( level ) : nullptr,
-             (const T*) fs.ptr( level ), (const T*) fd.ptr( level ), fx.lv, (T*) ft.ptr( level ), (T*) fx.ptr( level ) );
+             (const T*) fs.ptr( level ), (const T*) fd.ptr( level ), fx.lv, (T*) ft.ptr( level ), (T*) fx.ptr( level ),
+             cv );
       } );
       HG_CHECK_HIP( hipGetLastError() );
    } );
+}
+
+int hg_function_chebyshev_step( hg_ctx* h, hg_handle x, hg_handle t, hg_handle s, hg_handle b, hg_handle invDiag,
+                                double c, int level, int flag )
+{
+   return chebyshevStep( h, x, t, s, b, invDiag, c, -1, level, flag );
+}
+
+int hg_function_chebyshev_step_slots( hg_ctx* h, hg_handle x, hg_handle t, hg_handle s, hg_handle b, hg_handle invDiag,
+                                      int slot0, int level, int flag )
+{
+   if ( slot0 < 0 )
+      return hg_guard( h, []( Context* ) { throw Error( HG_ERR_ARG, "chebyshev_step_slots: invalid slot" ); } );
+   return chebyshevStep( h, x, t, s, b, invDiag, 0.0, slot0, level, flag );
 }
 
 } // extern "C"

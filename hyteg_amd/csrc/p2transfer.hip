@@ -265,7 +265,7 @@ void uploadTables()
 // value of the coarse quadratic at fine point (x,y,z) of cell c
 template < class T >
 __device__ __forceinline__ typename Acc< T >::type p2Interp( const T* __restrict__ coarse, int64_t c, int64_t scc,
-                                                             int64_t wc, int64_t x, int64_t y, int64_t z )
+                                                             int64_t wc, int64_t x, int64_t y, int64_t z, int lv )
 {
    using A         = typename Acc< T >::type;
    const int cls   = int( x & 3 ) + 4 * int( y & 3 ) + 16 * int( z & 3 );
@@ -276,7 +276,7 @@ __device__ __forceinline__ typename Acc< T >::type p2Interp( const T* __restrict
       const int64_t cx = bx + cPOff[cls][e][0], cy = by + cPOff[cls][e][1], cz = bz + cPOff[cls][e][2];
       if ( cx < 0 || cy < 0 || cz < 0 || cx + cy + cz > wc - 1 )
          continue;
-      v += A( cPW[cls][e] ) * toAcc( coarse[c * scc + tetIndex( wc, cx, cy, cz )] );
+      v += A( cPW[cls][e] ) * toAcc( coarse[( c * scc + tetIndex( wc, cx, cy, cz ) ) << lv] );
    }
    return v;
 }
@@ -284,10 +284,12 @@ __device__ __forceinline__ typename Acc< T >::type p2Interp( const T* __restrict
 template < class T >
 __global__ void kP2ProlongCell( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t scf, int64_t wf,
                                 int64_t mmf, int64_t scc, int64_t wc, const T* __restrict__ coarse,
-                                T* __restrict__ fine, int add )
+                                T* __restrict__ fine, int add, int lv )
 {
    using A          = typename Acc< T >::type;
    const int64_t lb = xcdBlock( nrows * nc );
+   coarse += blockIdx.y;
+   fine += blockIdx.y;
    if ( lb >= nrows * nc )
       return;
    const int64_t c   = lb / nrows;
@@ -317,11 +319,11 @@ __global__ void kP2ProlongCell( const int32_t* __restrict__ rows, int64_t nrows,
          const int64_t  cx = bx + int64_t( o & 255u ), cy = by + oy, cz = bz + oz;
          if ( cx + cy + cz > wc - 1 )
             continue;
-         v += A( cPW[cls][e] ) * toAcc( coarse[rs[oy + 3 * oz] + cx] );
+         v += A( cPW[cls][e] ) * toAcc( coarse[( rs[oy + 3 * oz] + cx ) << lv] );
       }
       if ( add )
-         v += toAcc( fine[out + x] );
-      fine[out + x] = fromAcc< T >( v );
+         v += toAcc( fine[( out + x ) << lv] );
+      fine[( out + x ) << lv] = fromAcc< T >( v );
    }
 }
 
@@ -336,7 +338,7 @@ __global__ __launch_bounds__( kProlongChunk ) void kP2ProlongChunks( const uint3
                                                                      int64_t nchunks, int64_t nc, int64_t scf,
                                                                      int32_t wf, int64_t scc, int32_t wc,
                                                                      const T* __restrict__ coarse,
-                                                                     T* __restrict__ fine, int add )
+                                                                     T* __restrict__ fine, int add, int lv )
 {
    using A          = typename Acc< T >::type;
    const int64_t lb = xcdBlock( nchunks * nc );
@@ -352,7 +354,7 @@ __global__ __launch_bounds__( kProlongChunk ) void kP2ProlongChunks( const uint3
    const int      cls = int( p0 & 3u ) + 4 * int( ( p0 >> 10 ) & 3u ) + 16 * int( ( p0 >> 20 ) & 3u );
    const int32_t  x = int32_t( pt & 1023u ), y = int32_t( ( pt >> 10 ) & 1023u ), z = int32_t( pt >> 20 );
    const int32_t bx = 2 * ( x >> 2 ), by = 2 * ( y >> 2 ), bz = 2 * ( z >> 2 );
-   const T*      cc = coarse + c * scc;
+   const T*      cc = coarse + blockIdx.y + ( ( c * scc ) << lv );
    A             v  = A( 0 );
    // the (at most kPMax) coarse values gathered first, then summed in entry
    // order; coarse row starts from the lane's three slice starts (the entry
@@ -376,13 +378,13 @@ __global__ __launch_bounds__( kProlongChunk ) void kP2ProlongChunks( const uint3
       const int32_t  cx = bx + int32_t( o & 255u ), cy = by + int32_t( ( o >> 8 ) & 255u ), cz = bz + oz;
       const int32_t  row = ( oz == 0 ? s0 : ( oz == 1 ? s1 : s2 ) ) + cy * ( wc - cz ) - cy * ( cy - 1 ) / 2;
       ok[e]              = e < n && cx + cy + cz <= wc - 1;
-      cv[e]              = ok[e] ? cc[row + cx] : T( 0 );
+      cv[e]              = ok[e] ? cc[int64_t( row + cx ) << lv] : T( 0 );
    }
 #pragma unroll
    for ( int e = 0; e < kPMax; ++e )
       if ( ok[e] ) // false past n
          v += A( cPW[cls][e] ) * toAcc( cv[e] );
-   T* out = fine + c * scf + tetRowStart32( wf, y, z ) + x;
+   T* out = fine + blockIdx.y + ( ( c * scf + tetRowStart32( wf, y, z ) + x ) << lv );
    if ( add )
       v += toAcc( *out );
    *out = fromAcc< T >( v );
@@ -443,10 +445,13 @@ static void prolongChunkList( Context* ctx, Lattice& l )
 
 template < class T >
 __global__ void kP2ProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
-                               int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine, int add )
+                               int64_t scc, int64_t wc, const T* __restrict__ coarse, T* __restrict__ fine, int add,
+                               int lv )
 {
    using A         = typename Acc< T >::type;
    const int64_t b = int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
+   coarse += blockIdx.y;
+   fine += blockIdx.y;
    if ( b >= nbf || !( bndFlag[b] & flag ) )
       return;
    bool done = false;
@@ -455,9 +460,9 @@ __global__ void kP2ProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_
       if ( done )
          return;
       done = true;
-      v    = p2Interp( coarse, c, scc, wc, x, y, z );
+      v    = p2Interp( coarse, c, scc, wc, x, y, z, lv );
    } );
-   const int64_t o = nc * gf.sc + b;
+   const int64_t o = ( nc * gf.sc + b ) << lv;
    if ( add )
       v += toAcc( fine[o] );
    fine[o] = fromAcc< T >( v );
@@ -466,10 +471,12 @@ __global__ void kP2ProlongBnd( BndGeom gf, int64_t nbf, int64_t nc, const uint8_
 template < class T >
 __global__ void kP2RestrictCell( const int32_t* __restrict__ rows, int64_t nrows, int64_t nc, int64_t scc, int64_t wc,
                                  int64_t mmc, int64_t scf, int64_t wf, const T* __restrict__ fine,
-                                 T* __restrict__ coarse )
+                                 T* __restrict__ coarse, int lv )
 {
    using A          = typename Acc< T >::type;
    const int64_t lb = xcdBlock( nrows * nc );
+   fine += blockIdx.y;
+   coarse += blockIdx.y;
    if ( lb >= nrows * nc )
       return;
    const int64_t c   = lb / nrows;
@@ -505,14 +512,14 @@ __global__ void kP2RestrictCell( const int32_t* __restrict__ rows, int64_t nrows
                const int     oy = cROff[q][e][1], oz = cROff[q][e][2];
                const int64_t fx = 2 * x + cROff[q][e][0], fy = 2 * y + oy, fz = 2 * z + oz;
                ok[j]  = e < cRN[q] && !( fx < 0 || fy < 0 || fz < 0 || fx + fy + fz > mmf );
-               buf[j] = ok[j] ? toAcc( fine[rs[( oy + 3 ) + 7 * ( oz + 3 )] + fx] ) : A( 0 );
+               buf[j] = ok[j] ? toAcc( fine[( rs[( oy + 3 ) + 7 * ( oz + 3 )] + fx ) << lv] ) : A( 0 );
             }
 #pragma unroll
             for ( int j = 0; j < 8; ++j )
                if ( ok[j] )
                   v += A( cRW[q][e0 + j] ) * buf[j];
          }
-         coarse[out + x] = fromAcc< T >( v );
+         coarse[( out + x ) << lv] = fromAcc< T >( v );
       }
 }
 
@@ -531,7 +538,7 @@ template < class T >
 __global__ __launch_bounds__( kRestrictBlock ) void kP2RestrictCellList( const uint32_t* __restrict__ pts, ClassWaves cw,
                                                                         int64_t nc, int64_t scc, int32_t wc, int64_t scf,
                                                                         int32_t wf, const T* __restrict__ fine,
-                                                                        T* __restrict__ coarse )
+                                                                        T* __restrict__ coarse, int lv )
 {
    using A = typename Acc< T >::type;
    __shared__ int32_t sliceStart[kMaxSlices];
@@ -556,7 +563,7 @@ __global__ __launch_bounds__( kRestrictBlock ) void kP2RestrictCellList( const u
       return;
    const int32_t x = int32_t( p & 1023u ), y = int32_t( ( p >> 10 ) & 1023u ), z = int32_t( p >> 20 );
    const int32_t mmf = wf - 1;
-   const T*      f   = fine + c * scf;
+   const T*      f   = fine + blockIdx.y + ( ( c * scf ) << lv );
    A             v   = A( 0 );
    for ( int g = 0; g < cRGN[q]; ++g )
    {
@@ -575,7 +582,7 @@ __global__ __launch_bounds__( kRestrictBlock ) void kP2RestrictCellList( const u
          {
             const int32_t fx = 2 * x + cRGX[q][rg.first + j];
             if ( fx >= 0 && fx <= lim )
-               buf[j] = toAcc( f[rs + fx] );
+               buf[j] = toAcc( f[int64_t( rs + fx ) << lv] );
          }
       }
 #pragma unroll
@@ -583,7 +590,7 @@ __global__ __launch_bounds__( kRestrictBlock ) void kP2RestrictCellList( const u
          if ( j < rg.count )
             v += A( cRGW[q][rg.first + j] ) * buf[j];
    }
-   coarse[c * scc + tetRowStart( wc, y, z ) + x] = fromAcc< T >( v );
+   coarse[blockIdx.y + ( ( c * scc + tetRowStart( wc, y, z ) + x ) << lv )] = fromAcc< T >( v );
 }
 
 // the class point list of a lattice (built on first use)
@@ -631,9 +638,11 @@ __global__ __launch_bounds__( 64 * kBndWaves ) void kP2RestrictBnd( BndGeom gc, 
                                                                    const uint8_t* __restrict__ bndFlag, int flag,
                                                                    const double* __restrict__ clsInvN, int64_t scf,
                                                                    int64_t wf, int64_t mmf, const T* __restrict__ fine,
-                                                                   T* __restrict__ coarse )
+                                                                   T* __restrict__ coarse, int lv )
 {
    using A         = typename Acc< T >::type;
+   fine += blockIdx.y;
+   coarse += blockIdx.y;
    const int     l = int( threadIdx.x ) & 63;
    const int64_t b = int64_t( blockIdx.x ) * kBndWaves + ( int( threadIdx.x ) >> 6 );
    if ( b >= nbc || !( bndFlag[b] & flag ) )
@@ -648,23 +657,25 @@ __global__ __launch_bounds__( 64 * kBndWaves ) void kP2RestrictBnd( BndGeom gc, 
             continue;
          const int cl = pointClass( mmf, fx, fy, fz );
          const A   w  = A( cRW[q][e] ) * ( cl < 0 ? A( 1 ) : A( clsInvN[14 * c + cl] ) );
-         s += w * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
+         s += w * toAcc( fine[( c * scf + tetIndex( wf, fx, fy, fz ) ) << lv] );
       }
    } );
 #pragma unroll
    for ( int o = 32; o > 0; o >>= 1 )
       s += __shfl_xor( s, o, 64 );
    if ( l == 0 )
-      coarse[nc * gc.sc + b] = fromAcc< T >( s );
+      coarse[( nc * gc.sc + b ) << lv] = fromAcc< T >( s );
 }
 
 // owned face DoFs (two incident cells each): one thread per DoF
 template < class T >
 __global__ void kP2RestrictBndFace( BndGeom gc, int64_t nbc, int64_t nc, const uint8_t* __restrict__ bndFlag, int flag,
                                     const double* __restrict__ clsInvN, int64_t scf, int64_t wf, int64_t mmf,
-                                    const T* __restrict__ fine, T* __restrict__ coarse )
+                                    const T* __restrict__ fine, T* __restrict__ coarse, int lv )
 {
    using A         = typename Acc< T >::type;
+   fine += blockIdx.y;
+   coarse += blockIdx.y;
    const int64_t b = gc.foff + int64_t( blockIdx.x ) * blockDim.x + threadIdx.x;
    if ( b >= nbc || !( bndFlag[b] & flag ) )
       return;
@@ -678,17 +689,17 @@ __global__ void kP2RestrictBndFace( BndGeom gc, int64_t nbc, int64_t nc, const u
             continue;
          const int cl = pointClass( mmf, fx, fy, fz );
          const A   w  = A( cRW[q][e] ) * ( cl < 0 ? A( 1 ) : A( clsInvN[14 * c + cl] ) );
-         s += w * toAcc( fine[c * scf + tetIndex( wf, fx, fy, fz )] );
+         s += w * toAcc( fine[( c * scf + tetIndex( wf, fx, fy, fz ) ) << lv] );
       }
    } );
-   coarse[nc * gc.sc + b] = fromAcc< T >( s );
+   coarse[( nc * gc.sc + b ) << lv] = fromAcc< T >( s );
 }
 
 static constexpr int kRowBlockT = 64;
 
 void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
 {
-   if ( f.off != 1 || f.lv != 0 || fineL - 1 < f.minL || fineL > f.maxL )
+   if ( f.off != 1 || fineL - 1 < f.minL || fineL > f.maxL )
       throw Error( HG_ERR_ARG, "P2 restriction needs a P2 function holding levels fineLevel-1 and fineLevel" );
    uploadTables();
    Storage& st = *f.st;
@@ -697,7 +708,9 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
    classPointList( ctx, lc );
    void*    fp = f.ptr( fineL );
    void*    cp = f.ptr( fineL - 1 );
-   pullHalos( ctx, st, lf, fp, f.dtype );
+   const int      lv = f.lv;
+   const unsigned nv = 1u << lv;
+   pullHalos( ctx, st, lf, fp, f.dtype, lv );
    dispatchType( f.dtype, [&]( auto tag ) {
       using T = decltype( tag );
       if ( lc.nrows > 0 && lc.classPoints.p )
@@ -707,16 +720,16 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
             cw.w[q] = lc.classWave[q];
          const int64_t bpc = ( cw.w[8] + kRestrictBlock / 64 - 1 ) / ( kRestrictBlock / 64 );
          ctx->beginTiming();
-         kP2RestrictCellList< T ><<< xcdGrid( bpc * lc.nc ), kRestrictBlock, 0, ctx->stream >>>(
-             lc.classPoints.p, cw, lc.nc, lc.sc, int32_t( lc.w ), lf.sc, int32_t( lf.w ), (const T*) fp, (T*) cp );
+         kP2RestrictCellList< T ><<< dim3( xcdGrid( bpc * lc.nc ), nv ), kRestrictBlock, 0, ctx->stream >>>(
+             lc.classPoints.p, cw, lc.nc, lc.sc, int32_t( lc.w ), lf.sc, int32_t( lf.w ), (const T*) fp, (T*) cp, lv );
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
       }
       else if ( lc.nrows > 0 )
       {
          ctx->beginTiming();
-         kP2RestrictCell< T ><<< xcdGrid( lc.nrows * lc.nc ), kRowBlockT, 0, ctx->stream >>>(
-             lc.rowsNat.p, lc.nrows, lc.nc, lc.sc, lc.w, lc.mm, lf.sc, lf.w, (const T*) fp, (T*) cp );
+         kP2RestrictCell< T ><<< dim3( xcdGrid( lc.nrows * lc.nc ), nv ), kRowBlockT, 0, ctx->stream >>>(
+             lc.rowsNat.p, lc.nrows, lc.nc, lc.sc, lc.w, lc.mm, lf.sc, lf.w, (const T*) fp, (T*) cp, lv );
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
       }
@@ -727,11 +740,11 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
          const BndGeom gc = BndGeom::of( st, lc );
          const int64_t nve = std::min( gc.foff, lc.nb );
          if ( nve > 0 )
-            kP2RestrictBnd< T ><<< gridFor( nve, kBndWaves ), 64 * kBndWaves, 0, ctx->stream >>>(
-                gc, nve, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp, (T*) cp );
+            kP2RestrictBnd< T ><<< dim3( gridFor( nve, kBndWaves ), nv ), 64 * kBndWaves, 0, ctx->stream >>>(
+                gc, nve, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp, (T*) cp, lv );
          if ( lc.nb > nve )
-            kP2RestrictBndFace< T ><<< gridFor( lc.nb - nve ), kBlock, 0, ctx->stream >>>(
-                gc, lc.nb, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp, (T*) cp );
+            kP2RestrictBndFace< T ><<< dim3( gridFor( lc.nb - nve ), nv ), kBlock, 0, ctx->stream >>>(
+                gc, lc.nb, lc.nc, lc.bndFlag.p, flag, st.clsInvN.p, lf.sc, lf.w, lf.mm, (const T*) fp, (T*) cp, lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
    } );
@@ -739,7 +752,7 @@ void p2Restrict( Context* ctx, Function& f, int fineL, int flag )
 
 void p2Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update )
 {
-   if ( f.off != 1 || f.lv != 0 || coarseL < f.minL || coarseL + 1 > f.maxL )
+   if ( f.off != 1 || coarseL < f.minL || coarseL + 1 > f.maxL )
       throw Error( HG_ERR_ARG, "P2 prolongation needs a P2 function holding levels sourceLevel and sourceLevel+1" );
    uploadTables();
    Storage& st = *f.st;
@@ -747,7 +760,9 @@ void p2Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update 
    Lattice& lc = st.lattice( ctx, coarseL + 1 );
    void*    fp = f.ptr( coarseL + 1 );
    void*    cp = f.ptr( coarseL );
-   pullHalos( ctx, st, lc, cp, f.dtype );
+   const int      lv = f.lv;
+   const unsigned nv = 1u << lv;
+   pullHalos( ctx, st, lc, cp, f.dtype, lv );
    dispatchType( f.dtype, [&]( auto tag ) {
       using T = decltype( tag );
       if ( lf.nrows > 0 )
@@ -757,21 +772,20 @@ void p2Prolongate( Context* ctx, Function& f, int coarseL, int flag, int update 
          if ( lf.prolongChunks.p )
          {
             const int64_t nchunks = int64_t( lf.prolongChunks.n ) / kProlongChunk;
-            kP2ProlongChunks< T ><<< xcdGrid( nchunks * lf.nc ), kProlongChunk, 0, ctx->stream >>>(
+            kP2ProlongChunks< T ><<< dim3( xcdGrid( nchunks * lf.nc ), nv ), kProlongChunk, 0, ctx->stream >>>(
                 lf.prolongChunks.p, nchunks, lf.nc, lf.sc, int32_t( lf.w ), lc.sc, int32_t( lc.w ), (const T*) cp,
-                (T*) fp, update );
+                (T*) fp, update, lv );
          }
          else
-            kP2ProlongCell< T ><<< xcdGrid( lf.nrows * lf.nc ), kRowBlockT, 0, ctx->stream >>>(
-                lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update );
+            kP2ProlongCell< T ><<< dim3( xcdGrid( lf.nrows * lf.nc ), nv ), kRowBlockT, 0, ctx->stream >>>(
+                lf.rowsNat.p, lf.nrows, lf.nc, lf.sc, lf.w, lf.mm, lc.sc, lc.w, (const T*) cp, (T*) fp, update, lv );
          HG_CHECK_HIP( hipGetLastError() );
          ctx->endTiming();
       }
       if ( lf.nb > 0 )
       {
-         kP2ProlongBnd< T ><<< gridFor( lf.nb ), kBlock, 0, ctx->stream >>>( BndGeom::of( st, lf ), lf.nb, lf.nc,
-                                                                            lf.bndFlag.p, flag, lc.sc, lc.w,
-                                                                            (const T*) cp, (T*) fp, update );
+         kP2ProlongBnd< T ><<< dim3( gridFor( lf.nb ), nv ), kBlock, 0, ctx->stream >>>(
+             BndGeom::of( st, lf ), lf.nb, lf.nc, lf.bndFlag.p, flag, lc.sc, lc.w, (const T*) cp, (T*) fp, update, lv );
          HG_CHECK_HIP( hipGetLastError() );
       }
    } );

@@ -1421,6 +1421,22 @@ int hg_slots_read( hg_ctx* h, int first, int n, double* out )
    } );
 }
 
+int hg_slots_write( hg_ctx* h, int first, int n, const double* values )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      checkSlots( ctx, first, n );
+      if ( !values )
+         throw Error( HG_ERR_ARG, "slots: invalid write" );
+      // a blocking copy, which a stream capture would record: refused there
+      if ( tlsCaptures > 0 )
+         throw Error( HG_ERR_STATE, "slots: write during a stream capture" );
+      DeviceGateLock gate;
+      HG_CHECK_HIP( hipMemcpyAsync( ctx->slots.p + first, values, sizeof( double ) * size_t( n ), hipMemcpyHostToDevice,
+                                    ctx->stream ) );
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
+   } );
+}
+
 int hg_function_dot_slot( hg_ctx* h, hg_handle a, hg_handle b, int level, int flag, int slot )
 {
    return hg_guard( h, [&]( Context* ctx ) {
@@ -1661,6 +1677,12 @@ int hg_p1_function_create_batched( hg_ctx* h, hg_handle s, int minLevel, int max
                                    hg_handle* out )
 {
    return functionCreate( h, s, minLevel, maxLevel, dtype, 0, nvec, out );
+}
+
+int hg_p2_function_create_batched( hg_ctx* h, hg_handle s, int minLevel, int maxLevel, int dtype, int nvec,
+                                   hg_handle* out )
+{
+   return functionCreate( h, s, minLevel, maxLevel, dtype, 1, nvec, out );
 }
 
 int hg_function_copy_vector( hg_ctx* h, hg_handle dst, int kdst, hg_handle src, int ksrc, int level )

@@ -15,9 +15,12 @@
 // slot's residual is checked; converged requests leave and waiting ones take
 // free slots at the next cycle boundary.  Vectors never interact, so a
 // request's result does not depend on what else is in the batch.  P2 classes
-// solve one request at a time (Chebyshev-smoothed V(2,2), fixed-count PCG on the
-// coarse level, the V-cycle replayed from a HIP graph captured per request),
-// with the coefficient k generated from the request's seed.
+// batch the same way: every slot of the interleaved hierarchy has its own
+// coefficient k (generated from the request's seed; its quadrature-point tables
+// and inverse diagonals are rebuilt when a request takes the slot), its own
+// Chebyshev coefficients (from its own power iteration) and shares the
+// Chebyshev-smoothed V(2,2) sweeps of the batch, with Chebyshev sweeps as the
+// coarse solve.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -131,19 +134,17 @@ void blobCentresOf( uint64_t seed, double c[12][3] )
    }
 }
 
-// P2 requests are solved one at a time per worker; their small kernels leave
-// the device mostly idle, so a class runs several workers (own context and
-// stream each) while requests are queued
-#ifndef HG_SERVE_P2_WORKERS
-#define HG_SERVE_P2_WORKERS 6
+// P2 batch width (slots of one class's interleaved hierarchy) and the Chebyshev(3)
+// sweeps of its coarse solve (a fixed count: the V-cycle has no host
+// synchronisation and is replayed from a captured HIP graph)
+#ifndef HG_SERVE_P2_SLOTS
+#define HG_SERVE_P2_SLOTS 8
 #endif
-constexpr int kP2Workers = HG_SERVE_P2_WORKERS;
-// Jacobi-PCG iterations of the P2 coarse solve (a fixed count: the V-cycle has
-// no host synchronisation and is replayed from a captured HIP graph)
-#ifndef HG_SERVE_P2_COARSE_ITERATIONS
-#define HG_SERVE_P2_COARSE_ITERATIONS 30
+constexpr int kP2Slots = HG_SERVE_P2_SLOTS;
+#ifndef HG_SERVE_P2_COARSE_SWEEPS
+#define HG_SERVE_P2_COARSE_SWEEPS 10
 #endif
-constexpr int kP2CoarseIterations = HG_SERVE_P2_COARSE_ITERATIONS;
+constexpr int kP2CoarseSweeps = HG_SERVE_P2_COARSE_SWEEPS;
 // Stokes requests are long (tens of Uzawa V-cycles), so a Stokes class also
 // runs several workers while requests are queued
 constexpr int kStokesWorkers = 4;
@@ -187,8 +188,13 @@ int64_t footprint( int kind, int level, int slots )
       return pts * 8 * int64_t( slots ) * 5 + ( int64_t( 64 ) << 20 );
    if ( kind == StokesShell )
       return pts * 8 * 40 + ( int64_t( 64 ) << 20 ); // four Taylor-Hood functions and the solver's temporaries
-   // P2: about ten FP64 functions and the quadrature-point coefficients (~8 values per DoF)
-   return pts * 8 * 18 + ( int64_t( 64 ) << 20 );
+   // P2: about ten FP64 functions of kP2Slots vectors, and per slot the stored
+   // quadrature-point coefficients (11 values per micro-cell, padded to whole
+   // tiles: at most ~4 mm^3 micro-cell places per macro-cell and level)
+   int64_t tables = 0;
+   for ( int l = 0; l <= level; ++l )
+      tables += cells * 4 * ( int64_t( 1 ) << ( 3 * l ) ) * 11 * 8;
+   return ( pts * 8 * 12 + tables ) * kP2Slots + ( int64_t( 64 ) << 20 );
 }
 
 struct Server
@@ -407,7 +413,7 @@ void p1Worker( Server& sv, int kind, int level, int64_t bytes )
    sv.release( cls, reserved );
 }
 
-// ---- P2 class: one request at a time -----------------------------------------
+// ---- P2 class: continuous batching over interleaved slots ------------------
 void p2Worker( Server& sv, int kind, int level, int64_t bytes )
 {
    const std::pair< int, int > cls{ kind, level };
@@ -427,72 +433,128 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
          check( ctx->get(), hg_storage_shell( ctx->get(), 5, 3, 0.55, 1.0, &h ), "shell" );
          st = std::make_shared< PrimitiveStorage >( ctx, h );
       }
-      const uint_t         L = uint_t( level );
-      P2Function< double > x( "x", st, 0, L ), b( "b", st, 0, L ), t( "t", st, L, L ), t2( "t2", st, L, L ),
-          r( "r", st, L, L );
-      using Op = P2ElementwiseBlendingDivKGradOperator;
-      // the solver objects live as long as the worker; the operator, the smoother's
-      // coefficients and the captured V-cycle are per request
-      auto smoother = std::make_shared< ChebyshevSmoother< Op > >( st, 0, L );
-      auto coarse   = std::make_shared< CGSolver< Op > >( st, 0, 0, kP2CoarseIterations, 0.0, 0.0,
-                                                        std::make_shared< InverseDiagonalPreconditioner< Op > >() );
-      coarse->setFixedIterations( true );
+      const int    S = kP2Slots;
+      const uint_t L = uint_t( level );
+      using Op       = P2ElementwiseBlendingDivKGradOperator;
+      Op                   A( st, 0, L, S );
+      P2Function< double > X( "x", st, 0, L, S ), B( "b", st, 0, L, S ), R( "r", st, L, L, S ),
+          T1( "t1", st, L, L, S ), T2( "t2", st, L, L, S );
+      P2Function< double > one( "one", st, L, L ), zero( "zero", st, L, L ), out( "out", st, L, L );
+      auto smoother = std::make_shared< ChebyshevSmoother< Op > >( st, 0, L, S );
+      auto coarse   = std::make_shared< SmootherSweeps< Op > >( smoother, kP2CoarseSweeps );
       GeometricMultigridSolver< Op > gmg( st, smoother, coarse, std::make_shared< P2toP2QuadraticRestriction<> >(),
-                                          std::make_shared< P2toP2QuadraticProlongation<> >(), 0, L, 2, 2 );
+                                          std::make_shared< P2toP2QuadraticProlongation<> >(), 0, L, 2, 2, 0,
+                                          CycleType::VCYCLE, S );
+      // idle slots hold b = 0, x = 0 (which the cycle keeps) and radius 1
+      std::vector< double > radius( S, 1.0 );
+      smoother->setupCoefficientsPerVector( 3, radius, ctx->get() );
+      hg_handle              graph  = 0;
+      bool                   warmed = false;
+      std::vector< Request > slot( S );
+      std::vector< bool >    active( S, false );
+      std::vector< double >  r0( S, 0.0 );
+      std::vector< int >     cyc( S, 0 );
+      const int              maxCycles = 60;
       for ( ;; )
       {
+         int nActive = 0;
+         for ( int k = 0; k < S; ++k )
+            nActive += active[k];
          std::vector< Request > in;
-         if ( !sv.take( cls, 1, true, in ) )
+         if ( !sv.take( cls, size_t( S - nActive ), nActive == 0, in ) && nActive == 0 )
             break;
-         if ( in.empty() )
-            continue;
-         const Request&        rq = in[0];
-         std::vector< double > a, om, ph;
-         coefficientOf( rq.seed, a, om, ph );
-         Op A( st, 0, L, a, om, ph );
-         x.interpolate( 0.0, L, All );
-         b.interpolate( 0.0, L, All );
-         check( ctx->get(), hg_function_interpolate_random( ctx->get(), b.handle(), rq.seed, -1.0, 1.0, int( L ), Inner ),
-                "rhs" );
-         check( ctx->get(), hg_function_interpolate_random( ctx->get(), t.handle(), rq.seed + 1, -1.0, 1.0, int( L ), All ),
-                "start vector" );
-         smoother->setupCoefficients( 3, chebyshev::estimateRadius( A, L, 15, t, t2 ) );
-         auto resid = [&] {
-            A.apply( x, r, L, Inner, Replace );
-            r.assign( { 1.0, -1.0 }, { b, r }, L, Inner );
-            return std::sqrt( r.dotGlobal( r, L, Inner ) );
-         };
-         const double r0    = resid();
-         double       rk    = r0;
-         int          k     = 0;
-         hg_handle    graph = 0;
-         while ( k < 60 && rk > sv.tol * r0 )
+         if ( !in.empty() )
          {
-            // the first cycle runs eagerly (it builds the operator's coefficient
-            // tables and inverse diagonals); the second is captured and replayed
-            if ( k == 0 || !useGraphs() )
-               gmg.solve( A, x, b, L );
-            else
+            // new requests take free slots: their k, right-hand side, zero start and
+            // power-iteration start vector; the other slots' start vectors are zero
+            for ( int k = 0; k < S; ++k )
+               T1.copyVector( k, zero, 0, L );
+            std::vector< int > joined;
+            int                k = 0;
+            for ( const Request& r : in )
             {
-               if ( !graph )
-               {
-                  CaptureScope capture( ctx->get() );
-                  gmg.solve( A, x, b, L );
-                  graph = capture.finish();
-               }
-               check( ctx->get(), hg_graph_launch( ctx->get(), graph ), "graph launch" );
+               while ( active[k] )
+                  ++k;
+               std::vector< double > a, om, ph;
+               coefficientOf( r.seed, a, om, ph );
+               A.setSlotCoefficients( k, a, om, ph );
+               one.interpolate( 0.0, L, All );
+               check( ctx->get(), hg_function_interpolate_random( ctx->get(), one.handle(), r.seed, -1.0, 1.0, int( L ), Inner ),
+                      "rhs" );
+               B.copyVector( k, one, 0, L );
+               X.copyVector( k, zero, 0, L );
+               check( ctx->get(), hg_function_interpolate_random( ctx->get(), one.handle(), r.seed + 1, -1.0, 1.0, int( L ), All ),
+                      "start vector" );
+               T1.copyVector( k, one, 0, L );
+               slot[k]   = r;
+               active[k] = true;
+               cyc[k]    = 0;
+               joined.push_back( k );
             }
-            rk = resid();
-            ++k;
+            const std::vector< double > rad = chebyshev::estimateRadiusPerVector( A, L, 15, T1, T2 );
+            for ( int j : joined )
+               radius[size_t( j )] = rad[size_t( j )];
+            smoother->setupCoefficientsPerVector( 3, radius, ctx->get() );
+            const std::vector< double > bb = B.dotGlobalBatched( B, L, Inner );
+            for ( int j : joined )
+               r0[size_t( j )] = std::sqrt( bb[size_t( j )] );
          }
-         if ( graph )
-            check( ctx->get(), hg_graph_destroy( ctx->get(), graph ), "graph" );
-         if ( sv.res.solutions && sv.res.solutions[rq.idx] )
-            check( ctx->get(), hg_function_download( ctx->get(), x.handle(), int( L ), sv.res.solutions[rq.idx] ),
-                   "solution" );
-         sv.finish( rq, r0 > 0 ? rk / r0 : 0.0, x.dotGlobal( x, L, All ), k );
-         sv.settle( ctx->get(), reserved );
+         nActive = 0;
+         for ( int k = 0; k < S; ++k )
+            nActive += active[k];
+         if ( nActive == 0 )
+            continue;
+         // one V-cycle on every slot; the first runs eagerly (it builds the tables,
+         // lists and inverse diagonals), the next is captured and replayed
+         if ( !warmed || !useGraphs() )
+         {
+            gmg.solve( A, X, B, L );
+            warmed = true;
+         }
+         else
+         {
+            if ( !graph )
+            {
+               CaptureScope capture( ctx->get() );
+               gmg.solve( A, X, B, L );
+               graph = capture.finish();
+            }
+            check( ctx->get(), hg_graph_launch( ctx->get(), graph ), "graph launch" );
+         }
+         A.apply( X, R, L, Inner, Replace );
+         R.assign( { 1.0, -1.0 }, { B, R }, L, Inner );
+         const std::vector< double > rr = R.dotGlobalBatched( R, L, Inner );
+         std::vector< int >          done;
+         for ( int k = 0; k < S; ++k )
+         {
+            if ( !active[k] )
+               continue;
+            ++cyc[k];
+            const double rel = r0[k] > 0 ? std::sqrt( rr[k] ) / r0[k] : 0.0;
+            if ( rel <= sv.tol || cyc[k] >= maxCycles )
+               done.push_back( k );
+         }
+         if ( !done.empty() )
+         {
+            const std::vector< double > xx = X.dotGlobalBatched( X, L, All );
+            for ( int k : done )
+            {
+               if ( sv.res.solutions && sv.res.solutions[slot[k].idx] )
+               {
+                  out.copyVector( 0, X, k, L );
+                  check( ctx->get(), hg_function_download( ctx->get(), out.handle(), int( L ), sv.res.solutions[slot[k].idx] ),
+                         "solution" );
+               }
+               sv.finish( slot[k], r0[k] > 0 ? std::sqrt( rr[k] ) / r0[k] : 0.0, xx[k], cyc[k] );
+               B.copyVector( k, zero, 0, L );
+               X.copyVector( k, zero, 0, L );
+               active[k] = false;
+            }
+            sv.settle( ctx->get(), reserved );
+         }
       }
+      if ( graph )
+         (void) hg_graph_destroy( ctx->get(), graph );
    } catch ( const std::exception& e )
    {
       sv.fail( std::string( "P2 worker: " ) + e.what() );
@@ -621,9 +683,7 @@ void startWorkers( Server& sv )
    {
       if ( q.empty() )
          continue;
-      const int want = ( cls.first == P2Shell || cls.first == P2Cube ) ? kP2Workers
-                       : cls.first == StokesShell                     ? kStokesWorkers
-                                                                      : 1;
+      const int want = cls.first == StokesShell ? kStokesWorkers : 1;
       while ( sv.live[cls] < want && ( sv.live[cls] == 0 || q.size() > size_t( sv.live[cls] ) ) )
       {
          const int64_t bytes = footprint( cls.first, cls.second, sv.slots );

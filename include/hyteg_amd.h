@@ -138,6 +138,12 @@ int hg_function_nvec( hg_ctx* ctx, hg_handle fn, int* nvec );
 int hg_function_copy_vector( hg_ctx* ctx, hg_handle dst, int kdst, hg_handle src, int ksrc, int level );
 /* P2Function<T>(name, storage, minLevel, maxLevel) (p2functionspace/P2Function.hpp:35-74) */
 int hg_p2_function_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, hg_handle* out );
+/* A P2 function holding nvec (a power of two up to 64) independent vectors
+ * interleaved per DoF, as hg_p1_function_create_batched (no counterpart in the
+ * reference): the operand of a batched P2 operator, the P2 transfers and the
+ * BLAS-1, halo and dot routines. */
+int hg_p2_function_create_batched( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int dtype, int nvec,
+                                   hg_handle* out );
 int hg_function_destroy( hg_ctx* ctx, hg_handle fn );
 /* storage handle a function lives on (Function::getStorage, functions/Function.hpp:35-80) */
 int hg_function_storage( hg_ctx* ctx, hg_handle fn, hg_handle* storage );
@@ -186,6 +192,9 @@ int hg_function_dot_batched_noalloc( hg_ctx* ctx, hg_handle a, hg_handle b, int 
 int hg_slots_alloc( hg_ctx* ctx, int n, int* first );
 int hg_slots_free( hg_ctx* ctx, int first, int n );
 int hg_slots_read( hg_ctx* ctx, int first, int n, double* out );
+/* slots first .. first+n-1 = values (waits for the copy; refused while the
+ * calling thread captures a graph) */
+int hg_slots_write( hg_ctx* ctx, int first, int n, const double* values );
 int hg_function_dot_slot( hg_ctx* ctx, hg_handle a, hg_handle b, int level, int flag, int slot );
 int hg_function_lincomb_slots( hg_ctx* ctx, hg_handle dst, int nsrc, const double* signs, const int* num, const int* den,
                                const hg_handle* srcs, int level, int flag, int add );
@@ -208,6 +217,12 @@ int hg_function_scale_batched( hg_ctx* ctx, hg_handle fn, const double* scales, 
  * multElementwise / add sequence it replaces */
 int hg_function_chebyshev_step( hg_ctx* ctx, hg_handle x, hg_handle t, hg_handle s, hg_handle b, hg_handle invDiag,
                                 double c, int level, int flag );
+/* the same step with a step size per interleaved vector: vector k takes
+ * c = slot[slot0 + k], read on the device when the step runs (a captured graph
+ * picks up new values written between its launches; no counterpart in the
+ * reference) */
+int hg_function_chebyshev_step_slots( hg_ctx* ctx, hg_handle x, hg_handle t, hg_handle s, hg_handle b,
+                                      hg_handle invDiag, int slot0, int level, int flag );
 /* multElementwise({a, b}, level, flag): dst = a * b (VertexDoFFunction.cpp:1408) */
 int hg_function_mult( hg_ctx* ctx, hg_handle dst, hg_handle a, hg_handle b, int level, int flag );
 /* copyFrom<otherT>(other, level): cast copy between storage types (VertexDoFFunction.hpp:151) */
@@ -274,6 +289,19 @@ int hg_p1_cell_stencil( hg_ctx* ctx, hg_handle op, int level, int cell, double* 
  * (nm = 0: k = 1), evaluated at the blended quadrature points. */
 int hg_p2_divkgrad_create( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int nm, const double* a,
                            const double* omega, const double* phi, hg_handle* out );
+/* Batched form of hg_p2_divkgrad_create (no counterpart in the reference, which
+ * solves one system per call): nslot (a power of two, 2..64) coefficient slots,
+ * each with its own k (initially k = 1) and its own stored quadrature-point
+ * table, for functions of nslot interleaved vectors (hg_p2_function_create_batched):
+ * vector s is multiplied by the operator of slot s, with the kernels and the
+ * per-vector results of the single-vector operator.  FP64, update Replace with
+ * Inner in the flag; hg_p2_inverse_diagonal gives the slots' diagonals
+ * interleaved.  hg_p2_divkgrad_set_slot replaces the k of one slot (its tables
+ * and inverse diagonals are rebuilt on the operator's stream). */
+int hg_p2_divkgrad_create_batched( hg_ctx* ctx, hg_handle storage, int minLevel, int maxLevel, int nslot,
+                                   hg_handle* out );
+int hg_p2_divkgrad_set_slot( hg_ctx* ctx, hg_handle op, int slot, int nm, const double* a, const double* omega,
+                             const double* phi );
 /* Operator::apply = gemv(1, src, 0|1, dst, level, flag) (P2ElementwiseOperator.cpp:110-328) */
 int hg_p2_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update );
 /* P2ElementwiseOperator::gemv(alpha, src, beta, dst, level, flag): dst = alpha A src + beta dst

@@ -182,7 +182,7 @@ class LatticeFunction
    using storageType = StorageType;
    using ValueType   = valueType;
 
-   // nvec > 1 (P1 only): that many independent vectors interleaved per DoF
+   // nvec > 1: that many independent vectors interleaved per DoF
    LatticeFunction( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
                     uint_t maxLevel, int nvec = 1 )
    : name_( name )
@@ -196,10 +196,11 @@ class LatticeFunction
                                                                        int( maxLevel ), dt, &h_ ) :
                                                 hg_p1_function_create_batched( ctx_, storage->handle(), int( minLevel ),
                                                                                int( maxLevel ), dt, nvec, &h_ ) ) :
-                                  hg_p2_function_create( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), dt, &h_ );
+                                  ( nvec == 1 ? hg_p2_function_create( ctx_, storage->handle(), int( minLevel ),
+                                                                       int( maxLevel ), dt, &h_ ) :
+                                                hg_p2_function_create_batched( ctx_, storage->handle(), int( minLevel ),
+                                                                               int( maxLevel ), dt, nvec, &h_ ) );
       check( ctx_, st, Space == 1 ? "P1Function" : "P2Function" );
-      if ( Space != 1 && nvec != 1 )
-         throw std::invalid_argument( "P2 functions hold one vector" );
    }
    ~LatticeFunction()
    {
@@ -283,6 +284,15 @@ class LatticeFunction
       check( ctx_,
              hg_function_chebyshev_step( ctx_, h_, t.h_, s.h_, b ? b->h_ : 0, invDiag.h_, c, int( level ), dofs( flag ) ),
              "chebyshevStep" );
+   }
+   // the step with the step size of vector k in device slot slot0 + k
+   void chebyshevStepSlots( const LatticeFunction& t, const LatticeFunction& s, const LatticeFunction* b,
+                            const LatticeFunction& invDiag, int slot0, uint_t level, DoFType flag ) const
+   {
+      check( ctx_,
+             hg_function_chebyshev_step_slots( ctx_, h_, t.h_, s.h_, b ? b->h_ : 0, invDiag.h_, slot0, int( level ),
+                                               dofs( flag ) ),
+             "chebyshevStepSlots" );
    }
    // dotGlobal into a device scalar slot of the context (no wait for the result)
    void dotToSlot( const LatticeFunction& other, uint_t level, DoFType flag, int slot ) const
@@ -543,6 +553,35 @@ class P2ElementwiseBlendingDivKGradOperatorT : public Operator< P2Function< VT >
                                     omega.data(), phi.data(), &h_ ),
              "P2ElementwiseBlendingDivKGradOperator" );
    }
+   // batched form (no counterpart in the reference): nslot coefficient slots for
+   // functions of nslot interleaved vectors, every slot k = 1 until
+   // setSlotCoefficients gives it its own
+   P2ElementwiseBlendingDivKGradOperatorT( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel,
+                                           uint_t maxLevel, int nslot )
+   : storage_( storage )
+   , ctx_( storage->ctx() )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   , owned_( true )
+   , nslot_( nslot )
+   {
+      check( ctx_, hg_p2_divkgrad_create_batched( ctx_, storage->handle(), int( minLevel ), int( maxLevel ), nslot, &h_ ),
+             "P2ElementwiseBlendingDivKGradOperator (batched)" );
+   }
+   // k of slot `slot` (same form as the constructor's); the stored tables and the
+   // inverse diagonals of that slot are rebuilt
+   void setSlotCoefficients( int slot, const std::vector< double >& a, const std::vector< double >& omega,
+                             const std::vector< double >& phi )
+   {
+      if ( omega.size() != 3 * a.size() || phi.size() != a.size() )
+         throw std::invalid_argument( "k: omega needs 3 entries per mode, phi one" );
+      check( ctx_, hg_p2_divkgrad_set_slot( ctx_, h_, slot, int( a.size() ), a.data(), omega.data(), phi.data() ),
+             "setSlotCoefficients" );
+      if ( invDiag_ )
+         for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
+            check( ctx_, hg_p2_inverse_diagonal( ctx_, h_, int( L ), invDiag_->handle() ), "inverse diagonal" );
+   }
+   int slots() const { return nslot_; }
    static std::unique_ptr< P2ElementwiseBlendingDivKGradOperatorT >
        borrow( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel, uint_t maxLevel )
    {
@@ -567,7 +606,7 @@ class P2ElementwiseBlendingDivKGradOperatorT : public Operator< P2Function< VT >
    }
    void computeInverseDiagonalOperatorValues() override
    {
-      invDiag_ = std::make_shared< P2Function< VT > >( "inverseDiagonal", storage_, minLevel_, maxLevel_ );
+      invDiag_ = std::make_shared< P2Function< VT > >( "inverseDiagonal", storage_, minLevel_, maxLevel_, nslot_ );
       for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
          check( ctx_, hg_p2_inverse_diagonal( ctx_, h_, int( L ), invDiag_->handle() ), "inverse diagonal" );
    }
@@ -591,6 +630,7 @@ class P2ElementwiseBlendingDivKGradOperatorT : public Operator< P2Function< VT >
    hg_handle                                     h_   = 0;
    uint_t                                        minLevel_, maxLevel_;
    bool                                          owned_;
+   int                                           nslot_ = 1;
    std::shared_ptr< P2Function< VT > >           invDiag_;
 };
 using P2ElementwiseBlendingDivKGradOperator = P2ElementwiseBlendingDivKGradOperatorT< double >;
@@ -979,11 +1019,15 @@ class ChebyshevSmoother : public Solver< OperatorType >
    static constexpr bool kFusedSteps = IsLatticeFunction< FunctionType >::value;
 
    ChebyshevSmoother( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel )
+   : ChebyshevSmoother( storage, minLevel, maxLevel, 1 )
+   {}
+   // temporaries of nvec interleaved vectors (multi-vector functions)
+   ChebyshevSmoother( const std::shared_ptr< PrimitiveStorage >& storage, uint_t minLevel, uint_t maxLevel, int nvec )
    : minLevel_( minLevel )
    , maxLevel_( maxLevel )
    , coefficients_( maxLevel - minLevel + 1 )
-   , tmp1_( "cheby_tmp1", storage, minLevel, maxLevel )
-   , tmp2_( "cheby_tmp2", storage, minLevel, maxLevel )
+   , tmp1_( "cheby_tmp1", storage, minLevel, maxLevel, nvec )
+   , tmp2_( "cheby_tmp2", storage, minLevel, maxLevel, nvec )
    {}
    // (storage, minLevel, maxLevel, lowMemoryMode, projection, projectionFlag); the
    // temporaries are always kept (lowMemoryMode only selects where they live there)
@@ -1009,6 +1053,39 @@ class ChebyshevSmoother : public Solver< OperatorType >
           coefficientsFor( order, lowerFactor * spectralRadius, upperFactor * spectralRadius );
    }
    const std::vector< double >& coefficients( uint_t level ) const { return coefficients_.at( level - minLevel_ ); }
+
+   // interleaved multi-vector functions (no counterpart in the reference): vector
+   // k gets the coefficients of spectralRadius[k] on every level.  They live in
+   // device slots of the context (order x nvec), so a V-cycle captured into a
+   // graph takes new values written between its launches.
+   void setupCoefficientsPerVector( uint_t order, const std::vector< double >& spectralRadius, hg_ctx* ctx,
+                                    double upperFactor = 1.2, double lowerFactor = 0.3 )
+   {
+      const int nv = int( spectralRadius.size() );
+      if ( vecSlot0_ < 0 )
+      {
+         check( ctx, hg_slots_alloc( ctx, int( order ) * nv, &vecSlot0_ ), "Chebyshev slots" );
+         vecCtx_ = ctx;
+         vecN_   = nv;
+         vecOrd_ = order;
+      }
+      if ( nv != vecN_ || order != vecOrd_ )
+         throw std::invalid_argument( "per-vector Chebyshev coefficients: order and vector count are fixed" );
+      std::vector< double > v( size_t( order ) * size_t( nv ) );
+      for ( int k = 0; k < nv; ++k )
+      {
+         const std::vector< double > c = coefficientsFor( order, lowerFactor * spectralRadius[size_t( k )],
+                                                          upperFactor * spectralRadius[size_t( k )] );
+         for ( uint_t i = 0; i < order; ++i )
+            v[size_t( i ) * size_t( nv ) + size_t( k )] = c[i];
+      }
+      check( ctx, hg_slots_write( ctx, vecSlot0_, int( v.size() ), v.data() ), "Chebyshev slots" );
+   }
+   ~ChebyshevSmoother() override
+   {
+      if ( vecSlot0_ >= 0 )
+         hg_slots_free( vecCtx_, vecSlot0_, int( vecOrd_ ) * vecN_ );
+   }
 
    // monomial coefficients of p from the Chebyshev recurrence T_{j+1} = 2 s T_j - T_{j-1}
    // in s = (theta - t) / delta
@@ -1040,8 +1117,24 @@ class ChebyshevSmoother : public Solver< OperatorType >
       const auto* withDiag = dynamic_cast< const OperatorWithInverseDiagonal< FunctionType >* >( &A );
       if ( !withDiag )
          throw std::runtime_error( "The Chebyshev-Smoother requires the OperatorWithInverseDiagonal interface." );
-      const auto&                  invDiag = *withDiag->getInverseDiagonalValues();
-      const std::vector< double >& c       = coefficients_.at( level - minLevel_ );
+      const auto& invDiag = *withDiag->getInverseDiagonalValues();
+      if constexpr ( kFusedSteps )
+      {
+         if ( vecSlot0_ >= 0 )
+         {
+            if ( x.nvec() != vecN_ || projection_ )
+               throw std::runtime_error( "per-vector Chebyshev coefficients: vector count mismatch or projection" );
+            A.apply( x, tmp2_, level, flag_ );
+            x.chebyshevStepSlots( tmp1_, tmp2_, &b, invDiag, vecSlot0_, level, flag_ );
+            for ( uint_t k = 1; k < vecOrd_; ++k )
+            {
+               A.apply( tmp1_, tmp2_, level, flag_ );
+               x.chebyshevStepSlots( tmp1_, tmp2_, nullptr, invDiag, vecSlot0_ + int( k ) * vecN_, level, flag_ );
+            }
+            return;
+         }
+      }
+      const std::vector< double >& c = coefficients_.at( level - minLevel_ );
       if ( c.empty() )
          throw std::runtime_error( "Chebyshev coefficients are not set up" );
       A.apply( x, tmp2_, level, flag_ );
@@ -1084,6 +1177,9 @@ class ChebyshevSmoother : public Solver< OperatorType >
    uint_t                                    minLevel_, maxLevel_;
    std::vector< std::vector< double > >      coefficients_;
    FunctionType                              tmp1_, tmp2_;
+   int                                       vecSlot0_ = -1, vecN_ = 0;
+   uint_t                                    vecOrd_   = 0;
+   hg_ctx*                                   vecCtx_   = nullptr;
    DoFType                                   flag_ = Inner | NeumannBoundary | FreeslipBoundary;
    std::shared_ptr< ProjectionOperatorType > projection_;
    DoFType                                   projectionFlag_ = FreeslipBoundary;
@@ -1138,6 +1234,41 @@ double estimateRadius( const OperatorType& A, uint_t level, uint_t maxIter, cons
       x.assign( { 1.0 / norm }, { tmp }, level, All );
       applyDA( x, tmp );
       radius = double( x.dotGlobal( tmp, level, All ) );
+   }
+   return radius;
+}
+// estimateRadius for each vector of an interleaved multi-vector function (no
+// counterpart in the reference): the same power iteration with per-vector norms
+// and Rayleigh quotients; a zero start vector gives radius 0
+template < class OperatorType >
+std::vector< double > estimateRadiusPerVector( const OperatorType& A, uint_t level, uint_t maxIter,
+                                               const typename OperatorType::srcType& x,
+                                               const typename OperatorType::srcType& tmp )
+{
+   using FunctionType   = typename OperatorType::srcType;
+   const auto* withDiag = dynamic_cast< const OperatorWithInverseDiagonal< FunctionType >* >( &A );
+   if ( !withDiag )
+      throw std::runtime_error( "estimateRadius requires the OperatorWithInverseDiagonal interface" );
+   const auto& invDiag = *withDiag->getInverseDiagonalValues();
+   auto        applyDA = [&]( const FunctionType& src, const FunctionType& dst ) {
+      A.apply( src, dst, level, All );
+      dst.multElementwise( { invDiag, dst }, level, All );
+   };
+   auto inverseNorms = []( std::vector< double > nn ) {
+      for ( double& v : nn )
+         v = v > 0.0 ? 1.0 / std::sqrt( v ) : 1.0;
+      return nn;
+   };
+   x.scaleBatched( inverseNorms( x.dotGlobalBatched( x, level, All ) ), level, All );
+   applyDA( x, tmp );
+   std::vector< double > radius( static_cast< size_t >( x.nvec() ), 0.0 );
+   for ( uint_t it = 1; it <= maxIter; ++it )
+   {
+      const std::vector< double > inv = inverseNorms( tmp.dotGlobalBatched( tmp, level, All ) );
+      x.assign( { 1.0 }, { tmp }, level, All );
+      x.scaleBatched( inv, level, All );
+      applyDA( x, tmp );
+      radius = x.dotGlobalBatched( tmp, level, All );
    }
    return radius;
 }
