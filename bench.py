@@ -187,7 +187,7 @@ def p1_cube(hg, ctx, L, steps, warmup):
     b.destroy()
     return n, sec, kms, roof, {"workload": f"P1ConstantLaplaceOperator apply, unit cube 6 macro-tets, level {L} "
                                           f"({n} DoFs), FP64", "level": L, "dofs": n, "mesh": "meshCuboid 1x1x1"}, \
-        "kCellApply<double>", f"p1_apply_cube_L{L}_f64"
+        "kCellApplyBricks<double>", f"p1_apply_cube_L{L}_f64"
 
 
 def p2_const_cube(hg, ctx, L=8, steps=10, warmup=2):

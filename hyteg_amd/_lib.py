@@ -62,6 +62,7 @@ SIGNATURES = {
     "hg_p2_divkgrad_create": [_p, _i64, _i, _i, _i, _p, _p, _p, _p],
     "hg_p2_divkgrad_create_batched": [_p, _i64, _i, _i, _i, _p],
     "hg_p2_divkgrad_set_slot": [_p, _i64, _i, _i, _p, _p, _p],
+    "hg_p2_divkgrad_slot_active": [_p, _i64, _i, _i],
     "hg_p2_function_create_batched": [_p, _i64, _i, _i, _i, _i, _p],
     "hg_p2_apply": [_p, _i64, _i64, _i64, _i, _i, _i],
     "hg_p2_inverse_diagonal": [_p, _i64, _i, _i64],

@@ -169,6 +169,7 @@ struct P2Operator : OperatorBase
    int                                    nslot = 1, lvSlots = 0;
    std::map< int, int64_t >               tableStride; // by P2 level
    std::map< int, DevBuf< double > >      invScratch;  // one slot's diagonal, by P2 level
+   DevBuf< int32_t >                      slotOn;      // 1: the apply computes the slot, 0: its result is 0
 };
 
 __device__ __forceinline__ double kEval( const KParams& K, const double x[3] )
@@ -785,11 +786,13 @@ __device__ __forceinline__ void p2TileApplyBody( const P2CellGeomT< typename Acc
                                                  const K* __restrict__ table, int64_t mm, int64_t ntiles, int64_t sc1,
                                                  int64_t w1, const T* __restrict__ src, T* __restrict__ dst,
                                                  double alpha, const uint64_t* __restrict__ tiles, int lvB = 0,
-                                                 int64_t tstride = 0 )
+                                                 int64_t tstride = 0, const int32_t* __restrict__ on = nullptr )
 {
    const int lv = BATCH ? lvB : 0;
    if constexpr ( BATCH )
    {
+      if ( on && !on[blockIdx.y] )
+         return; // an inactive slot: its (zeroed) result stays zero
       table += int64_t( blockIdx.y ) * tstride;
       src += blockIdx.y;
       dst += blockIdx.y;
@@ -939,9 +942,11 @@ __global__ __launch_bounds__( 64 ) void kP2TileApplyDiag( const P2CellGeomT< typ
                                                           int64_t mm, int64_t ndiag, int64_t sc1, int64_t w1,
                                                           const T* __restrict__ src, T* __restrict__ dst, double alpha,
                                                           const uint64_t* __restrict__ tiles, int lv = 0,
-                                                          int64_t tstride = 0 )
+                                                          int64_t tstride = 0, const int32_t* __restrict__ on = nullptr )
 {
    using R = typename Acc< T >::type;
+   if ( on && !on[blockIdx.y] )
+      return; // an inactive slot: its (zeroed) result stays zero
    // 2^lv interleaved vectors: vector blockIdx.y with its own coefficient table
    table += int64_t( blockIdx.y ) * tstride;
    src += blockIdx.y;
@@ -1039,10 +1044,10 @@ template < class T, class K >
 __global__ __launch_bounds__( kTileThreads ) void kP2TileApplyStoredBatched(
     const P2CellGeomT< typename Acc< T >::type >* __restrict__ geom, const K* __restrict__ table, int64_t tstride,
     int64_t mm, int64_t ntiles, int64_t sc1, int64_t w1, const T* __restrict__ src, T* __restrict__ dst, double alpha,
-    const uint64_t* __restrict__ tiles, int lv )
+    const uint64_t* __restrict__ tiles, int lv, const int32_t* __restrict__ on )
 {
    p2TileApplyBody< T, K, true, true >( geom, nullptr, nullptr, table, mm, ntiles, sc1, w1, src, dst, alpha, tiles, lv,
-                                        tstride );
+                                        tstride, on );
 }
 // coefficients evaluated in place (the compiler's own register budget)
 template < class T, class K >
@@ -1111,9 +1116,12 @@ __global__ __launch_bounds__( 64 ) void kP2SmallApply( const P2CellGeomT< typena
                                                        const KParams* __restrict__ kp, const K* __restrict__ table,
                                                        int64_t mm, int64_t nc, int64_t sc1, int64_t w1,
                                                        const T* __restrict__ src, T* __restrict__ dst, double alpha,
-                                                       int lv = 0, int64_t tstride = 0 )
+                                                       int lv = 0, int64_t tstride = 0,
+                                                       const int32_t* __restrict__ on = nullptr )
 {
    using R = typename Acc< T >::type;
+   if ( on && !on[blockIdx.y] )
+      return; // an inactive slot: its (zeroed) result stays zero
    // 2^lv interleaved vectors: vector blockIdx.y with its own coefficient table
    table += int64_t( blockIdx.y ) * tstride;
    src += blockIdx.y;
@@ -1342,17 +1350,17 @@ static void p2ApplyBatched( Context* ctx, P2Operator& op, Lattice& lat, int L, F
    ctx->beginTiming();
    if ( mm <= kSmallMaxMM )
       kP2SmallApply< double, double, true ><<< dim3( unsigned( lat.nc ), nv ), 64, 0, ctx->stream >>>(
-          geom, geom, nullptr, table, mm, lat.nc, lat.sc, lat.w, s, d, alpha, lv, tstride );
+          geom, geom, nullptr, table, mm, lat.nc, lat.sc, lat.w, s, d, alpha, lv, tstride, op.slotOn.p );
    else
    {
       if ( mm % kTile != 0 )
          throw Error( HG_ERR_STATE, "P2 tile apply: micro-edge count not a multiple of the tile width" );
       if ( nint > 0 )
          kP2TileApplyStoredBatched< double, double ><<< dim3( xcdGrid( nint ), nv ), kTileThreads, 0, ctx->stream >>>(
-             geom, table, tstride, mm, nint, lat.sc, lat.w, s, d, alpha, tint, lv );
+             geom, table, tstride, mm, nint, lat.sc, lat.w, s, d, alpha, tint, lv, op.slotOn.p );
       HG_CHECK_HIP( hipGetLastError() );
       kP2TileApplyDiag< double, double, true ><<< dim3( xcdGrid( ndiag ), nv ), 64, 0, ctx->stream >>>(
-          geom, geom, nullptr, table, mm, ndiag, lat.sc, lat.w, s, d, alpha, tlast, lv, tstride );
+          geom, geom, nullptr, table, mm, ndiag, lat.sc, lat.w, s, d, alpha, tlast, lv, tstride, op.slotOn.p );
    }
    HG_CHECK_HIP( hipGetLastError() );
    ctx->endTiming();
@@ -1817,6 +1825,22 @@ int hg_p2_divkgrad_create_batched( hg_ctx* h, hg_handle s, int minLevel, int max
       op.reduced     = false;
       std::vector< KParams > kp( size_t( nslot ), KParams{} );
       op.kpar.upload( ctx, kp );
+      op.slotOn.upload( ctx, std::vector< int32_t >( size_t( nslot ), 1 ) );
+   } );
+}
+
+int hg_p2_divkgrad_slot_active( hg_ctx* h, hg_handle opH, int slot, int active )
+{
+   return hg_guard( h, [&]( Context* ctx ) {
+      P2Operator& op = ctx->op< P2Operator >( opH );
+      if ( op.nslot < 2 || slot < 0 || slot >= op.nslot )
+         throw Error( HG_ERR_ARG, "slot_active needs a batched operator and a slot index in range" );
+      if ( tlsCaptures > 0 )
+         throw Error( HG_ERR_STATE, "slot_active during a stream capture" );
+      const int32_t  v = active ? 1 : 0;
+      DeviceGateLock gate;
+      HG_CHECK_HIP( hipMemcpyAsync( op.slotOn.p + slot, &v, sizeof( v ), hipMemcpyHostToDevice, ctx->stream ) );
+      HG_CHECK_HIP( hipStreamSynchronize( ctx->stream ) );
    } );
 }
 

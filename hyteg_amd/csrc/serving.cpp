@@ -445,8 +445,11 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
       GeometricMultigridSolver< Op > gmg( st, smoother, coarse, std::make_shared< P2toP2QuadraticRestriction<> >(),
                                           std::make_shared< P2toP2QuadraticProlongation<> >(), 0, L, 2, 2, 0,
                                           CycleType::VCYCLE, S );
-      // idle slots hold b = 0, x = 0 (which the cycle keeps) and radius 1
+      // idle slots hold b = 0, x = 0 (which the cycle keeps), radius 1, and the
+      // applies skip them
       std::vector< double > radius( S, 1.0 );
+      for ( int k = 0; k < S; ++k )
+         A.setSlotActive( k, false );
       smoother->setupCoefficientsPerVector( 3, radius, ctx->get() );
       hg_handle              graph  = 0;
       bool                   warmed = false;
@@ -478,6 +481,7 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
                std::vector< double > a, om, ph;
                coefficientOf( r.seed, a, om, ph );
                A.setSlotCoefficients( k, a, om, ph );
+               A.setSlotActive( k, true );
                one.interpolate( 0.0, L, All );
                check( ctx->get(), hg_function_interpolate_random( ctx->get(), one.handle(), r.seed, -1.0, 1.0, int( L ), Inner ),
                       "rhs" );
@@ -548,6 +552,7 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
                sv.finish( slot[k], r0[k] > 0 ? std::sqrt( rr[k] ) / r0[k] : 0.0, xx[k], cyc[k] );
                B.copyVector( k, zero, 0, L );
                X.copyVector( k, zero, 0, L );
+               A.setSlotActive( k, false );
                active[k] = false;
             }
             sv.settle( ctx->get(), reserved );

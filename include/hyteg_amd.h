@@ -302,6 +302,11 @@ int hg_p2_divkgrad_create_batched( hg_ctx* ctx, hg_handle storage, int minLevel,
                                    hg_handle* out );
 int hg_p2_divkgrad_set_slot( hg_ctx* ctx, hg_handle op, int slot, int nm, const double* a, const double* omega,
                              const double* phi );
+/* active = 0: applies of the batched operator skip slot `slot` and give it a zero
+ * result (for slots whose vectors are zero, as the idle slots of a serving
+ * batch); active = 1 (every slot's initial state) computes it again.  Read on
+ * the device when the apply runs (a captured graph follows it). */
+int hg_p2_divkgrad_slot_active( hg_ctx* ctx, hg_handle op, int slot, int active );
 /* Operator::apply = gemv(1, src, 0|1, dst, level, flag) (P2ElementwiseOperator.cpp:110-328) */
 int hg_p2_apply( hg_ctx* ctx, hg_handle op, hg_handle src, hg_handle dst, int level, int flag, int update );
 /* P2ElementwiseOperator::gemv(alpha, src, beta, dst, level, flag): dst = alpha A src + beta dst

@@ -581,6 +581,12 @@ class P2ElementwiseBlendingDivKGradOperatorT : public Operator< P2Function< VT >
          for ( uint_t L = minLevel_; L <= maxLevel_; ++L )
             check( ctx_, hg_p2_inverse_diagonal( ctx_, h_, int( L ), invDiag_->handle() ), "inverse diagonal" );
    }
+   // an inactive slot is skipped by the apply (its result is zero): for slots
+   // whose vectors are zero
+   void setSlotActive( int slot, bool active )
+   {
+      check( ctx_, hg_p2_divkgrad_slot_active( ctx_, h_, slot, active ? 1 : 0 ), "setSlotActive" );
+   }
    int slots() const { return nslot_; }
    static std::unique_ptr< P2ElementwiseBlendingDivKGradOperatorT >
        borrow( const std::shared_ptr< PrimitiveStorage >& storage, hg_handle h, uint_t minLevel, uint_t maxLevel )

@@ -124,3 +124,26 @@ def test_batched_inverse_diagonal_transfers_and_chebyshev_step(ctx, shell_pair):
         assert np.array_equal(xb[:, k], x1.download(L)), k
         assert np.array_equal(tb[:, k], t1.download(L)), k
     ctx.check(lib().hg_slots_free(ctx.h, first.value, NV), "slots")
+
+
+def test_inactive_slots_give_zero_and_leave_the_others_unchanged(ctx, shell_pair):
+    st, om = shell_pair
+    L = 3
+    A = P2DivKGradOperatorBatched(st, L, L, NV)
+    for s in range(NV):
+        A.set_slot(s, *_k(400 + s))
+    x = np.random.default_rng(11).uniform(-1, 1, (st.p2_ndofs(L), NV))
+    src = hg.P2Function("s", st, L, L, nvec=NV)
+    dst = hg.P2Function("d", st, L, L, nvec=NV)
+    src.upload(L, x)
+    A.apply(src, dst, L, hg.ALL)
+    full = dst.download(L)
+    ctx.check(lib().hg_p2_divkgrad_slot_active(ctx.h, A.h, 2, 0), "slot_active")
+    A.apply(src, dst, L, hg.ALL)
+    part = dst.download(L)
+    assert not part[:, 2].any()
+    for s in (0, 1, 3):
+        assert relmax(part[:, s], full[:, s]) < 1e-14, s
+    ctx.check(lib().hg_p2_divkgrad_slot_active(ctx.h, A.h, 2, 1), "slot_active")
+    A.apply(src, dst, L, hg.ALL)
+    assert relmax(dst.download(L)[:, 2], _oracle_apply(om, L, _k(402), x[:, 2])) < TOL
