@@ -495,7 +495,16 @@ void p2Worker( Server& sv, int kind, int level, int64_t bytes )
                cyc[k]    = 0;
                joined.push_back( k );
             }
+            // the power iteration computes the new slots only
+            std::vector< int > others;
+            for ( int j = 0; j < S; ++j )
+               if ( active[j] && std::find( joined.begin(), joined.end(), j ) == joined.end() )
+                  others.push_back( j );
+            for ( int j : others )
+               A.setSlotActive( j, false );
             const std::vector< double > rad = chebyshev::estimateRadiusPerVector( A, L, 15, T1, T2 );
+            for ( int j : others )
+               A.setSlotActive( j, true );
             for ( int j : joined )
                radius[size_t( j )] = rad[size_t( j )];
             smoother->setupCoefficientsPerVector( 3, radius, ctx->get() );
