@@ -23,6 +23,13 @@ import os
 import sys
 import time
 
+# hardware queues of the process (read when the HIP runtime loads): the serving
+# leg runs one stream per request class, and with HIP's default of 4 queues
+# classes sharing a queue serialise (a smaller setting is raised to 16); the
+# other legs use one stream
+if not os.environ.get("GPU_MAX_HW_QUEUES", "").isdigit() or int(os.environ["GPU_MAX_HW_QUEUES"]) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))

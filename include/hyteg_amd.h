@@ -458,8 +458,11 @@ int hg_p1_batched_ir_solve( hg_ctx* ctx, hg_handle op, hg_handle x, hg_handle b,
  * relative residual `tol` (Stokes: at most 150 Uzawa V-cycles; its residual floors
  * near 1e-7 to 5e-7, so a tol below 1e-6 runs to that cap).  P1 requests of
  * one (kind, level) share an interleaved hierarchy of `slots` vectors (a power of
- * two up to 64) and join / leave it at V-cycle boundaries; every class has its
- * own stream; a class hierarchy is built only while the device memory of the
+ * two up to 64), P2 requests one of 8 vectors with a coefficient slot each, and
+ * join / leave it at V-cycle boundaries; every class has its own stream (run the
+ * process with GPU_MAX_HW_QUEUES=16 set before HIP initialises, so that the
+ * classes' streams do not share HIP's default 4 hardware queues); a class
+ * hierarchy is built only while the device memory of the
  * live ones stays below memBudgetGB (an estimate until a worker has built its
  * hierarchy, then the bytes its context actually holds).  Outputs per request:
  * latency (arrival to completion), reached relative residual, checksum x.x of the
