@@ -392,7 +392,8 @@ def serving_cfg5(n=400, rate=200.0):
     log(f"serving {n} requests at {rate:g}/s: {res['solves_per_s']:.1f} solves/s, p50 {res['p50_latency_s'] * 1e3:.1f} ms, "
         f"p99 {res['p99_latency_s'] * 1e3:.1f} ms")
     return {"workload": f"{n} requests, Poisson arrivals at {rate:g}/s, mix P1 cube L3-7 40%, P1 shell L3-5 25%, "
-                        "P2 cube L3-5 20%, P2 shell L2-3 15%, each solved to 1e-8", "rate_per_s": rate, **res}
+                        "P2 cube L3-5 20%, P2 shell L2-3 15%, each solved to 1e-8", "rate_per_s": rate,
+            "hip_hardware_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), **res}
 
 
 def stokes_cfg3(hg, ctx, L=5, cycles=120, tol=1e-7):
