@@ -1818,7 +1818,7 @@ int hg_p2_divkgrad_create_batched( hg_ctx* h, hg_handle s, int minLevel, int max
    const int rc = hg_p2_divkgrad_create( h, s, minLevel, maxLevel, 0, nullptr, nullptr, nullptr, out );
    if ( rc != HG_OK )
       return rc;
-   return hg_guard( h, [&]( Context* ctx ) {
+   const int rc2 = hg_guard( h, [&]( Context* ctx ) {
       P2Operator& op = ctx->op< P2Operator >( *out );
       op.nslot       = nslot;
       op.lvSlots     = lv;
@@ -1827,6 +1827,12 @@ int hg_p2_divkgrad_create_batched( hg_ctx* h, hg_handle s, int minLevel, int max
       op.kpar.upload( ctx, kp );
       op.slotOn.upload( ctx, std::vector< int32_t >( size_t( nslot ), 1 ) );
    } );
+   if ( rc2 != HG_OK )
+   {
+      (void) hg_operator_destroy( h, *out );
+      *out = 0;
+   }
+   return rc2;
 }
 
 int hg_p2_divkgrad_slot_active( hg_ctx* h, hg_handle opH, int slot, int active )
