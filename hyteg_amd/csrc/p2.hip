@@ -1519,6 +1519,8 @@ static void p2SetSlot( Context* ctx, P2Operator& op, int sl, const KParams& kp )
    }
    for ( auto& [key, buf] : op.kq )
    {
+      if ( key.second != 0 )
+         continue; // batched operators build FP64 tables only
       const int     L      = key.first;
       const int64_t mm     = int64_t( 1 ) << L;
       const int64_t ntiles = tilesPerCell( mm ) * int64_t( op.st->topo.nc );
