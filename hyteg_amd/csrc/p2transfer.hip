@@ -555,13 +555,12 @@ __global__ __launch_bounds__( kRestrictBlock ) void kP2RestrictCellList( const u
                         __builtin_amdgcn_readfirstlane( int32_t( threadIdx.x >> 6 ) );
    if ( wave >= wavesPerCell )
       return;
-   int q = 0;
-   while ( wave >= cw.w[q + 1] )
-      ++q;
    const uint32_t p = pts[int64_t( wave ) * 64 + ( threadIdx.x & 63 )];
    if ( p == 0xFFFFFFFFu )
       return;
    const int32_t x = int32_t( p & 1023u ), y = int32_t( ( p >> 10 ) & 1023u ), z = int32_t( p >> 20 );
+   // the wavefront's parity class (uniform over its points), from its first active lane
+   const int q = __builtin_amdgcn_readfirstlane( int( x & 1 ) + 2 * int( y & 1 ) + 4 * int( z & 1 ) );
    const int32_t mmf = wf - 1;
    const T*      f   = fine + blockIdx.y + ( ( c * scf ) << lv );
    A             v   = A( 0 );
@@ -600,16 +599,13 @@ static void classPointList( Context* ctx, Lattice& l )
    // 2 mm + 1, in LDS; wider lattices keep the row kernel)
    if ( l.classPoints.p || l.mm < 4 || 2 * l.mm + 1 > kMaxSlices )
       return;
+   // slice by slice (neighbouring waves gather from the same fine slices), in a
+   // slice the points of each of its four parity classes padded to whole waves
    std::vector< uint32_t > pts;
-   int32_t                 waves = 0;
-   for ( int q = 0; q < 8; ++q )
-   {
-      l.classWave[q] = waves;
-      size_t n0      = pts.size();
-      for ( int64_t z = 1; z <= l.mm - 3; ++z )
+   for ( int64_t z = 1; z <= l.mm - 3; ++z )
+      for ( int q = 4 * int( z & 1 ); q < 4 * int( z & 1 ) + 4; ++q )
       {
-         if ( ( z & 1 ) != ( ( q >> 2 ) & 1 ) )
-            continue;
+         const size_t n0 = pts.size();
          for ( int64_t y = 1; y + z <= l.mm - 2; ++y )
          {
             if ( ( y & 1 ) != ( ( q >> 1 ) & 1 ) )
@@ -618,12 +614,12 @@ static void classPointList( Context* ctx, Lattice& l )
                if ( ( x & 1 ) == ( q & 1 ) )
                   pts.push_back( uint32_t( x ) | ( uint32_t( y ) << 10 ) | ( uint32_t( z ) << 20 ) );
          }
+         while ( ( pts.size() - n0 ) % 64 )
+            pts.push_back( 0xFFFFFFFFu );
       }
-      while ( ( pts.size() - n0 ) % 64 )
-         pts.push_back( 0xFFFFFFFFu );
-      waves += int32_t( ( pts.size() - n0 ) / 64 );
-   }
-   l.classWave[8] = waves;
+   for ( int q = 0; q < 8; ++q )
+      l.classWave[q] = 0; // (unused: a wave's class follows from its points)
+   l.classWave[8] = int32_t( pts.size() / 64 );
    if ( pts.empty() )
       pts.push_back( 0xFFFFFFFFu );
    l.classPoints.upload( ctx, pts );

@@ -102,10 +102,11 @@ struct Lattice
    int64_t           rowGroup[5] = { 0, 0, 0, 0, 0 };
    int64_t           nrows       = 0;
 
-   // cell-interior points (x | y << 10 | z << 20) grouped by parity class
-   // q = x%2 + 2 (y%2) + 4 (z%2), each class ordered by (z, y, x) and padded to
-   // whole wavefronts (0xFFFFFFFF); class q holds wavefronts
-   // [classWave[q], classWave[q+1]).  Built on first use (P2 restriction).
+   // cell-interior points (x | y << 10 | z << 20), slice z by slice, and in a
+   // slice grouped by parity class q = x%2 + 2 (y%2) + 4 (z%2), each group
+   // ordered by (y, x) and padded to whole wavefronts (0xFFFFFFFF), so a
+   // wavefront holds one class; classWave[8] = wavefronts per cell.  Built on
+   // first use (P2 restriction).
    DevBuf< uint32_t > classPoints;
    int32_t            classWave[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
 
