@@ -509,7 +509,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard by "
-              "macro-cell, see DESIGN.md)", file=sys.stderr)
+              "macro-cell: each GPU a share of the macro-cells, the interface halos exchanged between them)",
+              file=sys.stderr)
         sys.exit(2)
 
     import hyteg_amd as hg
